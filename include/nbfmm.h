@@ -1,0 +1,166 @@
+/*
+ * nbfmm.h -- C ABI of the MI355X (gfx950) 3D gravitational FMM library.
+ *
+ * The library replaces the force-evaluation path behind the reference's
+ * system-updater API (rezucch1/nbody-FMM):
+ *   class SystemUpdateMethod { virtual void update(const std::vector<Particle>&) = 0; }
+ *     (src/system_updaters/system_update_method.hpp:24-31)
+ *   FMM::update          (src/system_updaters/FMM.cpp:9-12)        -> nbfmm_eval_accel
+ *   NaiveUpdate::update  (src/system_updaters/naive_update.cpp:9-35) -> nbfmm_direct_accel
+ *   SyplecticEuler::integrate (src/quadratures/symplectic_euler.cpp:8-15) -> nbfmm_step_se_dev
+ *   RungeKutta4::integrate    (src/quadratures/runge_kutta_4.cpp:10-93)   -> nbfmm_step_rk4_dev
+ * Particle data crosses the boundary as plain arrays: positions AoS (n x 3
+ * doubles, x y z per particle, the order of Particle::position), masses
+ * (n doubles, Particle::weight) and accelerations AoS (n x 3 doubles,
+ * Particle::acceleration = G * grad, particle.cpp:16-21).
+ *
+ * Conventions
+ *  - Every function returns an nbfmm_status; 0 is success.  The message of the
+ *    last failure on the calling thread is returned by nbfmm_last_error().
+ *  - A context owns its device buffers and one HIP stream; it is not
+ *    thread-safe.  Different contexts may be used concurrently.
+ *  - *_dev functions take device pointers and are ordered on the context's
+ *    stream; they return before the GPU finishes unless stated otherwise.
+ *    Host-pointer functions synchronise before returning.
+ */
+#ifndef NBFMM_H
+#define NBFMM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nbfmm_status {
+  NBFMM_OK = 0,
+  NBFMM_ERR_ARG = 1,         /* invalid argument (null pointer, n < 1, bad parameter) */
+  NBFMM_ERR_DEGENERATE = 2,  /* non-finite coordinates or masses, zero total mass */
+  NBFMM_ERR_HIP = 3,         /* a HIP runtime call failed */
+  NBFMM_ERR_OOM = 4,         /* device allocation failed */
+  NBFMM_ERR_UNSUPPORTED = 5, /* parameter combination not implemented */
+  NBFMM_ERR_NO_DEVICE = 6    /* no usable GPU */
+} nbfmm_status;
+
+typedef enum nbfmm_precision {
+  NBFMM_PLAIN_FP64 = 0, /* every phase in FP64 */
+  NBFMM_MIXED = 1       /* FP32 P2P (compensated) and FP32 MFMA M2L; FP64 tree/integrator */
+} nbfmm_precision;
+
+typedef struct nbfmm_params {
+  uint32_t order;         /* expansion order p (reference L, hardcoded 2 at tree.cpp:94) */
+  uint32_t leaf_capacity; /* depth-loop threshold (reference 20, tree.cpp:32) */
+  double G;               /* gravitational constant (reference 6.6743e-11, system_update_method.hpp:17) */
+  double eps2;            /* softening of the direct sum (reference NaiveUpdate 1e-4, naive_update.cpp:27) */
+  int32_t precision;      /* nbfmm_precision */
+  int32_t reserved0;
+  /* An interaction-list pair (t, s) is evaluated by M2L, or by direct P2P
+   * between the two cells when n_t * n_s < p2p_cut.  0 = always M2L, as the
+   * reference does (node_i.cpp:105-120).  P2P is exact, so this only changes
+   * cost and removes truncation error from the substituted pairs. */
+  int64_t p2p_cut;
+} nbfmm_params;
+
+/* Reference-compatible defaults: p = 2, leaf capacity 20, G = 6.6743e-11,
+ * eps2 = 1e-4 (NaiveUpdate), FP64, p2p_cut = 0. */
+void nbfmm_default_params(nbfmm_params *params);
+
+typedef struct nbfmm_ctx nbfmm_ctx;
+
+/* Creates a context on the current HIP device.  The context keeps its device
+ * buffers across calls and grows them as needed. */
+int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **ctx);
+int nbfmm_destroy(nbfmm_ctx *ctx);
+/* Replaces the parameters used by later calls. */
+int nbfmm_set_params(nbfmm_ctx *ctx, const nbfmm_params *params);
+/* The context's HIP stream (hipStream_t), for callers that order their own work. */
+void *nbfmm_stream(nbfmm_ctx *ctx);
+
+/* FMM::update (FMM.cpp:9-12): accelerations of all n particles.
+ * Host pointers; synchronous.  acc = G * grad, grad = sum_j m_j (x_j-x_i)/|x_j-x_i|^3
+ * (leaf.cpp:119-142; coincident pairs contribute nothing). */
+int nbfmm_eval_accel(nbfmm_ctx *ctx, int64_t n, const double *xyz, const double *m, double *acc);
+/* Same on device pointers, ordered on the context's stream. */
+int nbfmm_eval_accel_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_m,
+                         double *d_acc);
+
+/* NaiveUpdate::update (naive_update.cpp:9-35): grad -= m_j d/((r^2+eps2) r),
+ * acc = G * grad, computed on the GPU in FP64.  Host pointers; synchronous. */
+int nbfmm_direct_accel(nbfmm_ctx *ctx, int64_t n, const double *xyz, const double *m, double *acc);
+/* Direct sum for the targets listed in d_tgt (indices into d_xyz), written to
+ * d_acc (3 per target).  mode 0 = the NaiveUpdate formula above; mode 1 =
+ * grad = sum m_j d/(r^2+eps2)^{3/2} with compensated (Neumaier) accumulation,
+ * the FP64 oracle of the accuracy metric.  Device pointers, stream-ordered. */
+int nbfmm_direct_accel_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_m,
+                           int64_t ntgt, const int64_t *d_tgt, int mode, double *d_acc);
+
+/* SyplecticEuler::integrate (symplectic_euler.cpp:8-15), nsteps times:
+ * acc = FMM(x); v += dt*acc; x += dt*v.  d_acc receives the last acceleration.
+ * Device pointers, stream-ordered. */
+int nbfmm_step_se_dev(nbfmm_ctx *ctx, int64_t n, double *d_xyz, double *d_vel, double *d_acc,
+                      const double *d_m, double dt, int nsteps);
+/* RungeKutta4::integrate (runge_kutta_4.cpp:10-93), nsteps times.
+ * mode 0 reproduces the reference: every stage starts from a copy of the
+ * input particles, so stage velocities are v + h*a_in with a_in the carried
+ * acceleration in d_acc (lines 41-79), and d_acc is left unchanged.
+ * mode 1 is classic RK4 and leaves a(x_n) of the last step in d_acc.
+ * Device pointers, stream-ordered. */
+int nbfmm_step_rk4_dev(nbfmm_ctx *ctx, int64_t n, double *d_xyz, double *d_vel, double *d_acc,
+                       const double *d_m, double dt, int nsteps, int mode);
+
+/* Statistics of the last evaluation: sizes for the roofline and per-phase
+ * GPU times measured with HIP events on the context's stream. */
+typedef struct nbfmm_stats {
+  int32_t depth;            /* leaf depth of the reference tree rule */
+  int32_t depth_iterations; /* iterations of the depth loop (tree.cpp:43-69) */
+  int64_t n_cells;          /* cells over all levels */
+  int64_t n_leaves;
+  int64_t n_il;             /* interaction-list entries over all levels */
+  int64_t n_m2l;            /* entries evaluated by M2L */
+  int64_t n_m2l_targets;    /* cells with at least one M2L source */
+  int64_t n_m2l_sources;    /* cells used as an M2L source */
+  int64_t n_p2p_pairs;      /* particle pairs evaluated directly (near + substituted) */
+  int64_t n_p2m_terms;      /* particle-cell pairs of the P2M passes */
+  int64_t n_l2p_terms;      /* particle-cell pairs of the L2P passes */
+  double ms_tree;           /* bbox, depth loop, sort, levels, neighbour tables */
+  double ms_lists;          /* interaction lists, classification, P2P lists */
+  double ms_p2m;
+  double ms_m2l;            /* including the per-class operator build */
+  double ms_l2p;
+  double ms_p2p;
+  double ms_total;          /* first to last event of the evaluation */
+  double ms_m2l_kernel;     /* the MFMA M2L contraction kernel alone */
+  double ms_p2p_kernel;     /* the P2P kernel alone */
+  double flop_m2l;          /* algorithmic M2L flops: 2 (p+1)^4 per M2L pair */
+  double flop_p2p;          /* algorithmic P2P flops: 20 per particle pair */
+} nbfmm_stats;
+int nbfmm_get_stats(nbfmm_ctx *ctx, nbfmm_stats *stats);
+/* Enables (1) or disables (0) the per-phase events (on by default). */
+int nbfmm_set_timing(nbfmm_ctx *ctx, int enabled);
+/* Keeps (1) the full interaction lists of later evaluations for
+ * nbfmm_tree_get (off by default: only the lists the evaluation needs are built). */
+int nbfmm_keep_lists(nbfmm_ctx *ctx, int enabled);
+
+/* Tree of the last evaluation, for inspection and parity tests (host copies).
+ * nbfmm_tree_sizes returns the depth, the number of cells and list entries.
+ * nbfmm_tree_get fills (any pointer may be NULL):
+ *   level_start[depth+2]  cells of level d are [level_start[d], level_start[d+1])
+ *   key[n_cells]          Morton key of each cell at its level (64-bit)
+ *   pbeg/pend[n_cells]    particle range in sorted order
+ *   parent[n_cells]       parent cell (-1 for the root)
+ *   perm[n]               sorted position -> input index (stable: input order inside a leaf)
+ *   nbr_off[n_cells+1], nbr[n_nbr]  neighbour lists (node_i.cpp:23-99 order)
+ *   il_off[n_cells+1], il[n_il]     interaction lists (node_i.cpp:23-99 order) */
+int nbfmm_tree_sizes(nbfmm_ctx *ctx, int32_t *depth, int64_t *n_cells, int64_t *n_nbr,
+                     int64_t *n_il);
+int nbfmm_tree_get(nbfmm_ctx *ctx, int64_t *level_start, uint64_t *key, int32_t *pbeg,
+                   int32_t *pend, int32_t *parent, int32_t *perm, int32_t *nbr_off, int32_t *nbr,
+                   int32_t *il_off, int32_t *il);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *nbfmm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

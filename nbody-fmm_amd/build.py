@@ -1,0 +1,66 @@
+"""Builds libnbfmm.so (HIP, gfx950) in-tree and the CPU oracle libraries.
+
+    python nbody-fmm_amd/build.py            # library + oracle (+ reference build if present)
+    python nbody-fmm_amd/build.py --lib-only # only libnbfmm.so
+
+hipcc cross-compiles for gfx950 without a GPU.  Objects go to
+nbody-fmm_amd/_build/, the library to nbody-fmm_amd/libnbfmm.so.
+"""
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libnbfmm.so")
+BUILD = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("NBFMM_ARCH", "gfx950")
+
+# -packed-fp32-ops: no v_pk_*_f32 generation (build notes).
+FLAGS = [
+    "-O3", "-std=c++20", f"--offload-arch={ARCH}", "-fPIC", "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", f"-I{os.path.join(ROOT, 'include')}",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build_lib(jobs=8):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    objs = []
+    cmds = []
+    for f in srcs:
+        o = os.path.join(BUILD, f[:-4] + ".o")
+        objs.append(o)
+        cmds.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, f), "-o", o])
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    tmp = OUT + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+def build_oracle():
+    od = os.path.join(ROOT, "oracle")
+    _run(["make", "-B", "-C", od, "liborc.so"])
+    ref_src = "/root/reference/src"
+    if os.path.isdir(ref_src):
+        _run(["make", "-C", od, "-j8", "ref"])
+
+
+if __name__ == "__main__":
+    build_lib()
+    if "--lib-only" not in sys.argv:
+        build_oracle()
+    print("built", OUT)

@@ -1,0 +1,355 @@
+// capi.hip -- the extern "C" boundary of libnbfmm.so (declared in include/nbfmm.h).
+#include "nbfmm_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+using namespace nbfmm;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int status, const std::string &msg) {
+  g_err = msg;
+  return status;
+}
+
+#define API_BEGIN try {
+#define API_END                                        \
+  }                                                    \
+  catch (const Error &e) {                             \
+    return fail(e.status, e.what());                   \
+  }                                                    \
+  catch (const std::exception &e) {                    \
+    return fail(NBFMM_ERR_HIP, e.what());              \
+  }                                                    \
+  return NBFMM_OK;
+
+int check_params(const nbfmm_params *p) {
+  if (!p) return fail(NBFMM_ERR_ARG, "params is null");
+  if (p->order < 1 || p->order > 14) return fail(NBFMM_ERR_ARG, "order must be in [1, 14]");
+  if (p->leaf_capacity < 1) return fail(NBFMM_ERR_ARG, "leaf_capacity must be >= 1");
+  if (!std::isfinite(p->G)) return fail(NBFMM_ERR_ARG, "G must be finite");
+  if (!(p->eps2 >= 0.0) || !std::isfinite(p->eps2)) return fail(NBFMM_ERR_ARG, "eps2 must be finite and >= 0");
+  if (p->p2p_cut < 0) return fail(NBFMM_ERR_ARG, "p2p_cut must be >= 0");
+  if (p->precision != NBFMM_PLAIN_FP64 && p->precision != NBFMM_MIXED)
+    return fail(NBFMM_ERR_ARG, "unknown precision");
+  if (p->precision == NBFMM_MIXED) return fail(NBFMM_ERR_UNSUPPORTED, "mixed precision is not built yet");
+  return NBFMM_OK;
+}
+
+void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, double *d_acc) {
+  c->plan.n = n;
+  c->nev = 0;
+  mark(c, 0);
+  build_tree(c, d_xyz, d_m);
+  mark(c, 1);
+  build_lists(c);
+  mark(c, 2);
+  run_expansions(c); // marks 3, 4, 5
+  run_p2p(c, d_acc);
+  mark(c, 6);
+  Plan &P = c->plan;
+  nbfmm_stats &S = c->stats;
+  std::memset(&S, 0, sizeof(S));
+  S.depth = P.geo.depth;
+  S.depth_iterations = P.iterations;
+  S.n_cells = P.n_cells;
+  S.n_leaves = P.lvl_start[P.geo.depth + 1] - P.lvl_start[P.geo.depth];
+  S.n_il = P.n_il;
+  S.n_m2l = P.n_m2l;
+  S.n_m2l_targets = P.n_tgt;
+  S.n_m2l_sources = P.n_src;
+  S.n_p2p_pairs = P.n_p2p_pairs;
+  S.n_p2m_terms = P.n_p2m_terms;
+  S.n_l2p_terms = P.n_l2p_terms;
+  if (c->timing) {
+    NB_HIP(hipEventSynchronize(c->ev[6]));
+    float t[7] = {0};
+    for (int k = 1; k <= 6; ++k) NB_HIP(hipEventElapsedTime(&t[k], c->ev[k - 1], c->ev[k]));
+    S.ms_tree = t[1];
+    S.ms_lists = t[2];
+    S.ms_p2m = t[3];
+    S.ms_m2l = t[4];
+    S.ms_l2p = t[5];
+    S.ms_p2p = t[6];
+    float tot = 0;
+    NB_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[6]));
+    S.ms_total = tot;
+    float km = 0, kp = 0;
+    if (P.n_tgt > 0) NB_HIP(hipEventElapsedTime(&km, c->ev[7], c->ev[8]));
+    NB_HIP(hipEventElapsedTime(&kp, c->ev[9], c->ev[10]));
+    S.ms_m2l_kernel = km;
+    S.ms_p2p_kernel = kp;
+  }
+  const double p1 = (double)c->prm.order + 1.0;
+  S.flop_m2l = 2.0 * p1 * p1 * p1 * p1 * (double)P.n_m2l;
+  S.flop_p2p = 20.0 * (double)P.n_p2p_pairs;
+}
+
+void check_n(int64_t n) {
+  if (n < 1) throw Error(NBFMM_ERR_ARG, "n must be >= 1");
+  if (n > (int64_t)1 << 30) throw Error(NBFMM_ERR_ARG, "n must be < 2^30");
+}
+
+} // namespace
+
+namespace nbfmm {
+void *dbuf_ensure(DBuf &b, size_t bytes) {
+  if (b.cap >= bytes && b.p) return b.p;
+  if (b.p) NB_HIP(hipFree(b.p));
+  b.p = nullptr;
+  b.cap = 0;
+  size_t want = bytes + bytes / 4 + 256;
+  NB_HIP(hipMalloc(&b.p, want));
+  b.cap = want;
+  return b.p;
+}
+void dbuf_free(DBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+void mark(nbfmm_ctx *c, int slot) {
+  if (c->timing) NB_HIP(hipEventRecord(c->ev[slot], c->stream));
+}
+} // namespace nbfmm
+
+nbfmm_ctx::~nbfmm_ctx() {
+  DBuf *all[] = {&xyz_in, &m_in, &acc_out, &key[0], &key[1], &idx[0], &idx[1], &sort_hist, &scan_tmp, &pos,
+                 &pcell, &grad, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &cnt_il,
+                 &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &acc_sub, &il, &m2l_src, &m2l_cls, &sub_src,
+                 &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &p2p_off, &p2p_cnt, &p2p_src, &p2m_items,
+                 &p2m_off, &tiles, &stab, &tmat, &mexp, &lexp, &consts, &xk, &vk, &ak, &asum, &vsum};
+  for (DBuf *b : all) dbuf_free(*b);
+  if (hred.p) (void)hipHostFree(hred.p);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+void nbfmm_default_params(nbfmm_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->order = 2;
+  p->leaf_capacity = 20;
+  p->G = 6.6743e-11;
+  p->eps2 = 1.0e-4;
+  p->precision = NBFMM_PLAIN_FP64;
+  p->p2p_cut = 0;
+}
+
+const char *nbfmm_last_error(void) { return g_err.c_str(); }
+
+int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
+  if (!out) return fail(NBFMM_ERR_ARG, "ctx is null");
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  nbfmm_ctx *c = nullptr;
+  API_BEGIN
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(NBFMM_ERR_NO_DEVICE, "no HIP device");
+  c = new nbfmm_ctx();
+  c->prm = *params;
+  NB_HIP(hipGetDevice(&c->device));
+  NB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto &e : c->ev) NB_HIP(hipEventCreate(&e));
+  NB_HIP(hipHostMalloc(&c->hred.p, 4096, hipHostMallocDefault));
+  c->hred.cap = 4096;
+  *out = c;
+  c = nullptr;
+  }
+  catch (const Error &e) {
+    delete c;
+    return fail(e.status, e.what());
+  }
+  return NBFMM_OK;
+}
+
+int nbfmm_destroy(nbfmm_ctx *c) {
+  if (!c) return NBFMM_OK;
+  API_BEGIN
+  (void)hipStreamSynchronize(c->stream);
+  delete c;
+  API_END
+}
+
+int nbfmm_set_params(nbfmm_ctx *c, const nbfmm_params *params) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  int rc = check_params(params);
+  if (rc) return rc;
+  c->prm = *params;
+  return NBFMM_OK;
+}
+
+void *nbfmm_stream(nbfmm_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int nbfmm_set_timing(nbfmm_ctx *c, int enabled) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  c->timing = enabled != 0;
+  return NBFMM_OK;
+}
+
+int nbfmm_keep_lists(nbfmm_ctx *c, int enabled) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  c->keep_lists = enabled != 0;
+  return NBFMM_OK;
+}
+
+int nbfmm_eval_accel_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, double *d_acc) {
+  if (!c || !d_xyz || !d_m || !d_acc) return fail(NBFMM_ERR_ARG, "null argument");
+  API_BEGIN
+  check_n(n);
+  eval_dev(c, n, d_xyz, d_m, d_acc);
+  API_END
+}
+
+int nbfmm_eval_accel(nbfmm_ctx *c, int64_t n, const double *xyz, const double *m, double *acc) {
+  if (!c || !xyz || !m || !acc) return fail(NBFMM_ERR_ARG, "null argument");
+  API_BEGIN
+  check_n(n);
+  double *dx = dget<double>(c->xyz_in, (size_t)n * 3), *dm = dget<double>(c->m_in, (size_t)n);
+  double *da = dget<double>(c->acc_out, (size_t)n * 3);
+  NB_HIP(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  NB_HIP(hipMemcpyAsync(dm, m, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  eval_dev(c, n, dx, dm, da);
+  NB_HIP(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+int nbfmm_direct_accel_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,
+                           const int64_t *d_tgt, int mode, double *d_acc) {
+  if (!c || !d_xyz || !d_m || !d_acc) return fail(NBFMM_ERR_ARG, "null argument");
+  if (mode != 0 && mode != 1) return fail(NBFMM_ERR_ARG, "mode must be 0 or 1");
+  API_BEGIN
+  check_n(n);
+  if (ntgt < 1) throw Error(NBFMM_ERR_ARG, "ntgt must be >= 1");
+  run_direct(c, n, d_xyz, d_m, ntgt, d_tgt, mode, d_acc);
+  API_END
+}
+
+int nbfmm_direct_accel(nbfmm_ctx *c, int64_t n, const double *xyz, const double *m, double *acc) {
+  if (!c || !xyz || !m || !acc) return fail(NBFMM_ERR_ARG, "null argument");
+  API_BEGIN
+  check_n(n);
+  double *dx = dget<double>(c->xyz_in, (size_t)n * 3), *dm = dget<double>(c->m_in, (size_t)n);
+  double *da = dget<double>(c->acc_out, (size_t)n * 3);
+  NB_HIP(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  NB_HIP(hipMemcpyAsync(dm, m, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  run_direct(c, n, dx, dm, n, nullptr, 0, da);
+  NB_HIP(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+int nbfmm_step_se_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt,
+                      int nsteps) {
+  if (!c || !x || !v || !a || !m) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
+  API_BEGIN
+  check_n(n);
+  for (int s = 0; s < nsteps; ++s) {
+    eval_dev(c, n, x, m, a);
+    se_update(c, n, x, v, a, dt);
+  }
+  API_END
+}
+
+int nbfmm_step_rk4_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt,
+                       int nsteps, int mode) {
+  if (!c || !x || !v || !a || !m) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
+  if (mode != 0 && mode != 1) return fail(NBFMM_ERR_ARG, "mode must be 0 or 1");
+  API_BEGIN
+  check_n(n);
+  const size_t m3 = (size_t)n * 3;
+  double *xk = dget<double>(c->xk, m3), *vk = dget<double>(c->vk, m3), *ak = dget<double>(c->ak, m3);
+  double *asum = dget<double>(c->asum, m3), *vsum = dget<double>(c->vsum, m3);
+  const double h2 = dt / 2, h6 = dt / 6;
+  for (int st = 0; st < nsteps; ++st) {
+    copy_d(c, vk, v, (int64_t)m3);         // velocity_k = v      (runge_kutta_4.cpp:28-30)
+    eval_dev(c, n, x, m, ak);              // stage 1              (:32)
+    rk4_accum(c, n, asum, vsum, ak, vk, 1.0, 1);
+    if (mode == 1) copy_d(c, a, ak, (int64_t)m3);
+    for (int stage = 2; stage <= 4; ++stage) {
+      const double h = stage == 4 ? dt : h2;
+      const double *avel = mode == 0 ? a : ak;  // reference: carried acceleration of the input (:46-48)
+      rk4_stage(c, n, x, v, vk, avel, h, xk, vk);
+      eval_dev(c, n, xk, m, ak);
+      rk4_accum(c, n, asum, vsum, ak, vk, stage == 4 ? 1.0 : 2.0, 0);
+    }
+    rk4_final(c, n, x, v, asum, vsum, h6);  // (:89-92)
+  }
+  API_END
+}
+
+int nbfmm_get_stats(nbfmm_ctx *c, nbfmm_stats *s) {
+  if (!c || !s) return fail(NBFMM_ERR_ARG, "null argument");
+  *s = c->stats;
+  return NBFMM_OK;
+}
+
+int nbfmm_tree_sizes(nbfmm_ctx *c, int32_t *depth, int64_t *n_cells, int64_t *n_nbr, int64_t *n_il) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  API_BEGIN
+  const Plan &P = c->plan;
+  if (P.n_cells == 0) throw Error(NBFMM_ERR_ARG, "no evaluation has run");
+  if (depth) *depth = P.geo.depth;
+  if (n_cells) *n_cells = P.n_cells;
+  if (n_nbr) {
+    std::vector<int32_t> nb((size_t)P.n_cells * 27);
+    NB_HIP(hipMemcpy(nb.data(), c->nbr27.p, nb.size() * 4, hipMemcpyDeviceToHost));
+    int64_t k = 0;
+    for (size_t i = 0; i < nb.size(); ++i)
+      if (i % 27 != 13 && nb[i] >= 0) ++k;
+    *n_nbr = k;
+  }
+  if (n_il) *n_il = P.n_il;
+  API_END
+}
+
+int nbfmm_tree_get(nbfmm_ctx *c, int64_t *level_start, uint64_t *key, int32_t *pbeg, int32_t *pend,
+                   int32_t *parent, int32_t *perm, int32_t *nbr_off, int32_t *nbr, int32_t *il_off, int32_t *il) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  API_BEGIN
+  const Plan &P = c->plan;
+  if (P.n_cells == 0) throw Error(NBFMM_ERR_ARG, "no evaluation has run");
+  NB_HIP(hipStreamSynchronize(c->stream));
+  const size_t nc = (size_t)P.n_cells;
+  if (level_start)
+    for (int d = 0; d <= P.geo.depth + 1; ++d) level_start[d] = P.lvl_start[d];
+  if (key) NB_HIP(hipMemcpy(key, c->ckey.p, nc * 8, hipMemcpyDeviceToHost));
+  if (pbeg) NB_HIP(hipMemcpy(pbeg, c->cbeg.p, nc * 4, hipMemcpyDeviceToHost));
+  if (pend) NB_HIP(hipMemcpy(pend, c->cend.p, nc * 4, hipMemcpyDeviceToHost));
+  if (parent) NB_HIP(hipMemcpy(parent, c->cparent.p, nc * 4, hipMemcpyDeviceToHost));
+  if (perm) NB_HIP(hipMemcpy(perm, c->idx[0].p, (size_t)P.n * 4, hipMemcpyDeviceToHost));
+  if (nbr_off || nbr) {
+    std::vector<int32_t> nb(nc * 27);
+    NB_HIP(hipMemcpy(nb.data(), c->nbr27.p, nb.size() * 4, hipMemcpyDeviceToHost));
+    int32_t k = 0;
+    for (size_t cc = 0; cc < nc; ++cc) {
+      if (nbr_off) nbr_off[cc] = k;
+      for (int o = 0; o < 27; ++o) {
+        int32_t q = nb[cc * 27 + o];
+        if (o != 13 && q >= 0) {
+          if (nbr) nbr[k] = q;
+          ++k;
+        }
+      }
+    }
+    if (nbr_off) nbr_off[nc] = k;
+  }
+  if (il_off || il) {
+    if (!c->keep_lists || !c->il.p) throw Error(NBFMM_ERR_ARG, "interaction lists were not kept");
+    if (il_off) NB_HIP(hipMemcpy(il_off, c->off_il.p, (nc + 1) * 4, hipMemcpyDeviceToHost));
+    if (il && P.n_il) NB_HIP(hipMemcpy(il, c->il.p, (size_t)P.n_il * 4, hipMemcpyDeviceToHost));
+  }
+  API_END
+}
+
+} // extern "C"

@@ -1,0 +1,637 @@
+// expansions.hip -- solid-harmonic expansions of the gfx950 FMM.
+//
+// Storage.  A cell's multipole M and local L are kept in the real layout of
+// the reference's PowerSet<3> (power_set3D.cpp:49-63): K = (p+1)^2 doubles,
+// index n^2 -> Re X_n^0, n^2+2m-1 -> Re X_n^m, n^2+2m -> Im X_n^m (m >= 1),
+// padded to KP (a multiple of 16).  Coefficients are Schmidt-normalised and
+// scaled by the cell scale rho_l (the largest cell edge at level l):
+//   M_n^m = sum_i q_i conj(R_n^m((x_i - c)/rho))
+//   phi(x) = (1/rho) sum_{n,m} L_n^m R_n^m((x - c)/rho)
+// where R_n^m are the regular harmonics of PowerSet<3> (power_set3D.cpp:10-47).
+// With this scaling the M2L operator of an offset class d (integer cell
+// offset in [-3,3]^3) is the same matrix at every level, so M2L is one dense
+// KP x KP contraction per class, executed with v_mfma_f64_16x16x4f64.
+//
+// Passes.  The upward pass forms M directly from the particles of every cell
+// that is an M2L source (P2M at every level, the particles of a level-l cell
+// are contiguous in Morton order); the downward pass evaluates the local of
+// every M2L target directly at its particles (L2P at every level).  This is
+// algebraically the M2M/L2L chain of the reference (node.cpp:14-43): M2M of
+// exact multipoles is exact and L2L of a degree-p local is exact, so the two
+// differ only by rounding, while the cost follows the (few) cells that take
+// part in M2L instead of every cell of the tree.
+#include "nbfmm_internal.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace nbfmm {
+namespace {
+
+constexpr int kT = 256;
+constexpr int P2M_CHUNK = 1024;
+
+// Coefficient tables (host-computed once per order, uploaded to c->consts).
+struct Consts {
+  double mmf[kMaxOrder + 2];                  // -sqrt((2k-1)/(2k))
+  double s2m1[kMaxOrder + 2];                 // sqrt(2m+1)
+  double ra[kMaxOrder + 2][kMaxOrder + 2];    // (2n-1)/sqrt(n^2-m^2)
+  double rb[kMaxOrder + 2][kMaxOrder + 2];    // sqrt((n-1)^2-m^2)/sqrt(n^2-m^2)
+  double cz[kMaxOrder + 2][kMaxOrder + 2];    // sqrt((n+m)(n-m))
+  double cu[kMaxOrder + 2][kMaxOrder + 2];    // sqrt((n-m+1)(n-m))
+  double cd[kMaxOrder + 2][kMaxOrder + 2];    // sqrt((n+m+1)(n+m))
+};
+
+void fill_consts(Consts &C) {
+  std::memset(&C, 0, sizeof(C));
+  for (int k = 1; k <= kMaxOrder + 1; ++k) C.mmf[k] = -std::sqrt((2.0 * k - 1.0) / (2.0 * k));
+  for (int m = 0; m <= kMaxOrder + 1; ++m) C.s2m1[m] = std::sqrt(2.0 * m + 1.0);
+  for (int n = 0; n <= kMaxOrder + 1; ++n)
+    for (int m = 0; m <= n && m <= kMaxOrder + 1; ++m) {
+      if (n >= 2 && m <= n - 2) {
+        double den = std::sqrt((double)(n * n - m * m));
+        C.ra[n][m] = (2.0 * n - 1.0) / den;
+        C.rb[n][m] = std::sqrt((double)((n - 1) * (n - 1) - m * m)) / den;
+      }
+      C.cz[n][m] = std::sqrt((double)((n + m) * (n - m)));
+      C.cu[n][m] = std::sqrt((double)((n - m + 1) * (n - m)));
+      C.cd[n][m] = std::sqrt((double)((n + m + 1) * (n + m)));
+    }
+}
+
+struct cd2 {
+  double x, y;
+};
+__device__ __forceinline__ cd2 cmul(cd2 a, cd2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+__device__ __forceinline__ void cell_centre(uint64_t key, int lev, const LevelGeo &g, double c[3]) {
+  uint64_t k = key;
+  uint32_t q[3] = {0, 0, 0};
+  for (int b = 0; b < lev; ++b) {
+    q[0] |= (uint32_t)((k >> (3 * b)) & 1u) << b;
+    q[1] |= (uint32_t)((k >> (3 * b + 1)) & 1u) << b;
+    q[2] |= (uint32_t)((k >> (3 * b + 2)) & 1u) << b;
+  }
+  for (int d = 0; d < 3; ++d) c[d] = g.lo[d] + ((double)q[d] + 0.5) * g.h[lev][d];
+}
+
+__device__ __forceinline__ int level_of(int32_t cell, const LevelGeo &g) {
+  int l = 0;
+  while (l < g.depth && cell >= g.lvl_start[l + 1]) ++l;
+  return l;
+}
+
+// ---- M2L operators -----------------------------------------------------------------
+// Yoshida singular harmonics S_N^M (N <= 2p, 0 <= M <= N) of the scaled class
+// offset, then the real KP x KP matrix in MFMA A-fragment order:
+//   frag[(rb*KS + ks)*64 + lane] = T[rb*16 + lane%16][ks*4 + lane/16].
+// L_j^k = (-1)^{j+k}/N_jk sum_{n,m} (M_n^m/N_nm) S_{n+j}^{m-k}(D),  N_nm = sqrt((n+m)!(n-m)!)
+__global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, double rx, double ry, double rz,
+                                                  double *__restrict__ frag) {
+  const int cls = blockIdx.x;
+  const int dx = cls % 7 - 3, dy = (cls / 7) % 7 - 3, dz = cls / 49 - 3;
+  const int P2 = 2 * p;
+  __shared__ double Sre[(2 * kMaxOrder + 1) * (2 * kMaxOrder + 2) / 2 + 8];
+  __shared__ double Sim[(2 * kMaxOrder + 1) * (2 * kMaxOrder + 2) / 2 + 8];
+  __shared__ double lf[2 * kMaxOrder + 2]; // log-free factorials as doubles
+  const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+  if (threadIdx.x <= (unsigned)(2 * kMaxOrder + 1)) {
+    double f = 1;
+    for (int k = 2; k <= (int)threadIdx.x; ++k) f *= k;
+    lf[threadIdx.x] = f;
+  }
+  const double X = dx * rx, Y = dy * ry, Z = dz * rz;
+  const double r2 = X * X + Y * Y + Z * Z;
+  if (!near && (int)threadIdx.x <= P2) {
+    // column M = threadIdx.x of S (Yoshida normalisation)
+    const int M = threadIdx.x;
+    const double ir2 = 1.0 / r2;
+    double sr = 1.0 / sqrt(r2), si = 0;
+    for (int k = 1; k <= M; ++k) { // S_k^k = -(2k-1)(x+iy)/r^2 S_{k-1}^{k-1}
+      double f = -(2.0 * k - 1.0) * ir2;
+      double nr = f * (X * sr - Y * si), ni = f * (X * si + Y * sr);
+      sr = nr;
+      si = ni;
+    }
+    int idx = M * (M + 1) / 2 + M;
+    Sre[idx] = sr;
+    Sim[idx] = si;
+    double pr = 0, pi = 0, cr = sr, ci = si;
+    for (int N = M + 1; N <= P2; ++N) {
+      double a = (2.0 * N - 1.0) * Z * ir2;
+      double b = (double)((N - 1) * (N - 1) - M * M) * ir2;
+      double nr = a * cr - b * pr, ni = a * ci - b * pi;
+      pr = cr;
+      pi = ci;
+      cr = nr;
+      ci = ni;
+      int id = N * (N + 1) / 2 + M;
+      Sre[id] = cr;
+      Sim[id] = ci;
+    }
+  }
+  __syncthreads();
+  const int K = (p + 1) * (p + 1);
+  const int KS = KP / 4;
+  double *out = frag + (size_t)cls * KP * KP;
+  for (int e = threadIdx.x; e < KP * KP; e += kT) {
+    const int lane = e & 63, blk = e >> 6;
+    const int rb = blk / KS, ks = blk % KS;
+    const int row = rb * 16 + (lane & 15), col = ks * 4 + (lane >> 4);
+    double v = 0;
+    if (!near && row < K && col < K) {
+      int j = (int)sqrt((double)row);
+      while (j * j > row) --j;
+      while ((j + 1) * (j + 1) <= row) ++j;
+      int rr = row - j * j; // 0 -> (k=0, re); 2k-1 -> (k, re); 2k -> (k, im)
+      int k = (rr + 1) / 2, oim = (rr > 0 && (rr & 1) == 0);
+      int n = (int)sqrt((double)col);
+      while (n * n > col) --n;
+      while ((n + 1) * (n + 1) <= col) ++n;
+      int cc = col - n * n;
+      int m = (cc + 1) / 2, iim = (cc > 0 && (cc & 1) == 0);
+      // S_{n+j}^{M'} with sign/conjugation for negative M'
+      auto S = [&](int NN, int MM, double &re, double &im) {
+        if (MM >= 0) {
+          re = Sre[NN * (NN + 1) / 2 + MM];
+          im = Sim[NN * (NN + 1) / 2 + MM];
+        } else {
+          re = Sre[NN * (NN + 1) / 2 - MM];
+          im = -Sim[NN * (NN + 1) / 2 - MM];
+          if ((-MM) & 1) {
+            re = -re;
+            im = -im;
+          }
+        }
+      };
+      const double Njk = sqrt(lf[j + k] * lf[j - k]);
+      const double Nnm = sqrt(lf[n + m] * lf[n - m]);
+      const double cjk = (((j + k) & 1) ? -1.0 : 1.0) / Njk / Nnm;
+      double Are, Aim;
+      S(n + j, m - k, Are, Aim);
+      if (m == 0) {
+        v = cjk * (oim ? Aim : Are);
+      } else {
+        double Bre, Bim;
+        S(n + j, m + k, Bre, Bim);
+        Bim = -Bim; // conj
+        if (k & 1) {
+          Bre = -Bre;
+          Bim = -Bim;
+        }
+        // term = a (A+B) + i b (A-B), M = a + i b
+        if (!oim)
+          v = iim ? -cjk * (Aim - Bim) : cjk * (Are + Bre);
+        else
+          v = iim ? cjk * (Are - Bre) : cjk * (Aim + Bim);
+      }
+    }
+    out[e] = v;
+  }
+}
+
+// ---- P2M ---------------------------------------------------------------------------
+// Item = (source slot, particle range).  Lanes: 16 particle slots x 16 orders m.
+__global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restrict__ items, const double4 *__restrict__ pos,
+                                            const int32_t *__restrict__ src_cells, const uint64_t *__restrict__ ckey,
+                                            LevelGeo g, const Consts *__restrict__ C, double *__restrict__ Mexp) {
+  const int4 it = items[blockIdx.x];
+  const int slot = it.x, beg = it.y, end = it.z;
+  const int32_t cell = src_cells[slot];
+  const int lev = level_of(cell, g);
+  double ctr[3];
+  cell_centre(ckey[cell], lev, g, ctr);
+  const double ir = g.inv_rho[lev];
+  const int m = threadIdx.x & 15, ps = threadIdx.x >> 4;
+  double ar[kMaxOrder + 1], ai[kMaxOrder + 1];
+#pragma unroll
+  for (int n = 0; n <= kMaxOrder; ++n) {
+    ar[n] = 0;
+    ai[n] = 0;
+  }
+  if (m <= p) {
+    for (int i = beg + ps; i < end; i += 16) {
+      const double4 q = pos[i];
+      const double x = (q.x - ctr[0]) * ir, y = (q.y - ctr[1]) * ir, z = (q.z - ctr[2]) * ir;
+      const double r2 = x * x + y * y + z * z;
+      double vr = q.w, vi = 0; // q * R_m^m
+      for (int k = 1; k <= m; ++k) {
+        double f = C->mmf[k];
+        double nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
+        vr = nr;
+        vi = ni;
+      }
+      double pr = 0, pi = 0, cr = vr, ci = vi;
+#pragma unroll
+      for (int n = 0; n <= kMaxOrder; ++n) {
+        if (n == m) {
+          ar[n] += cr;
+          ai[n] -= ci;
+        } else if (n == m + 1 && n <= p) {
+          pr = cr;
+          pi = ci;
+          double f = z * C->s2m1[m];
+          cr = f * pr;
+          ci = f * pi;
+          ar[n] += cr;
+          ai[n] -= ci;
+        } else if (n > m + 1 && n <= p) {
+          double a = C->ra[n][m] * z, b = C->rb[n][m] * r2;
+          double nr = a * cr - b * pr, ni = a * ci - b * pi;
+          pr = cr;
+          pi = ci;
+          cr = nr;
+          ci = ni;
+          ar[n] += cr;
+          ai[n] -= ci;
+        }
+      }
+    }
+  }
+  // reduce the 4 particle slots of each wave (lanes m, m+16, m+32, m+48)
+#pragma unroll
+  for (int n = 0; n <= kMaxOrder; ++n) {
+    ar[n] += __shfl_xor(ar[n], 16);
+    ar[n] += __shfl_xor(ar[n], 32);
+    ai[n] += __shfl_xor(ai[n], 16);
+    ai[n] += __shfl_xor(ai[n], 32);
+  }
+  __shared__ double red[4][16][kMaxOrder + 1][2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane < 16) {
+#pragma unroll
+    for (int n = 0; n <= kMaxOrder; ++n) {
+      red[wave][lane][n][0] = ar[n];
+      red[wave][lane][n][1] = ai[n];
+    }
+  }
+  __syncthreads();
+  double *M = Mexp + (size_t)slot * KP;
+  for (int e = threadIdx.x; e < (p + 1) * (p + 1); e += kT) {
+    int n = (int)sqrt((double)e);
+    while (n * n > e) --n;
+    while ((n + 1) * (n + 1) <= e) ++n;
+    int rr = e - n * n, mm = (rr + 1) / 2, im = (rr > 0 && (rr & 1) == 0);
+    double v = 0;
+    for (int w = 0; w < 4; ++w) v += red[w][mm][n][im];
+    atomicAdd(&M[e], v);
+  }
+}
+
+// ---- M2L on MFMA --------------------------------------------------------------------
+// One wave = 16 target cells (columns); for every offset class present in
+// its targets' lists, B = the 16 source multipoles (zero where a target has
+// no source of that class) and acc[rb] += T_class[rb] * B.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int KP>
+__global__ void __launch_bounds__(kT) k_m2l_mfma(int p, int ntiles, const int32_t *__restrict__ tile_tgt, int32_t ntgt,
+                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
+                                                 const int32_t *__restrict__ m2l_src,
+                                                 const uint16_t *__restrict__ m2l_cls,
+                                                 const int32_t *__restrict__ src_slot,
+                                                 const int32_t *__restrict__ tgt_slot, const double *__restrict__ frag,
+                                                 const double *__restrict__ Mexp, double *__restrict__ Lexp) {
+  constexpr int RB = KP / 16, KS = KP / 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * (kT / 64) + wave;
+  if (tile >= ntiles) return;
+  const int col = lane & 15, kg = lane >> 4;
+  const int ti = tile * 16 + col;
+  const int32_t tc = ti < ntgt ? tile_tgt[ti] : -1;
+  int cur = tc >= 0 ? off_m2l[tc] : 0;
+  const int endc = tc >= 0 ? cur + cnt_m2l[tc] : 0;
+  dbl4 acc[RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) acc[r] = dbl4{0, 0, 0, 0};
+  int next = cur < endc ? (int)m2l_cls[cur] : 1 << 20;
+  while (true) {
+    // smallest pending class over the wave
+    int cls = next;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) cls = min(cls, __shfl_xor(cls, o));
+    if (cls >= kNumClasses) break;
+    const bool has = next == cls;
+    const double *Mb = nullptr;
+    if (has) {
+      Mb = Mexp + (size_t)src_slot[m2l_src[cur]] * KP;
+      ++cur;
+      next = cur < endc ? (int)m2l_cls[cur] : 1 << 20;
+    }
+    const double *A = frag + (size_t)cls * KP * KP + lane;
+    for (int ks = 0; ks < KS; ++ks) {
+      const double b = has ? Mb[ks * 4 + kg] : 0.0;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const double a = A[(size_t)(r * KS + ks) * 64];
+        acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[r], 0, 0, 0);
+      }
+    }
+  }
+  if (tc < 0) return;
+  // v_mfma_f64_16x16x4f64 result layout: lane l, register q holds
+  // D[l/16 + 4q][l%16] (checked on gfx950).
+  double *L = Lexp + (size_t)tgt_slot[tc] * KP;
+  const int K = (p + 1) * (p + 1);
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = r * 16 + kg + 4 * q;
+      if (row < K) L[row] = acc[r][q];
+    }
+}
+
+// ---- L2P --------------------------------------------------------------------------------
+// grad phi at every particle from the locals of its ancestors that are M2L
+// targets:  grad = (1/rho^2) sum_{n'<p, m'>=0} w(m') Re[R_n'^m'(t) G(n'+1, m')]
+// with G_z = cz L_n^m', G_x = (cu L_n^{m'-1} - cd L_n^{m'+1})/2,
+// G_y = -(i/2)(cu L_n^{m'-1} + cd L_n^{m'+1}), L_n^{-1} = -conj(L_n^1).
+__device__ __forceinline__ cd2 lget(const double *L, int n, int m, int p) {
+  if (m > n || n > p) return {0, 0};
+  if (m == 0) return {L[n * n], 0};
+  if (m > 0) return {L[n * n + 2 * m - 1], L[n * n + 2 * m]};
+  // m == -1
+  return {-L[n * n + 1], L[n * n + 2]};
+}
+
+__global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const double4 *__restrict__ pos,
+                                            const int32_t *__restrict__ pcell, const int32_t *__restrict__ cparent,
+                                            const uint64_t *__restrict__ ckey, const int32_t *__restrict__ tgt_slot,
+                                            LevelGeo g, const Consts *__restrict__ C, const double *__restrict__ Lexp,
+                                            double *__restrict__ grad) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const double4 q = pos[i];
+  double gx = 0, gy = 0, gz = 0;
+  int32_t cell = pcell[i];
+  for (int lev = g.depth; lev >= 2; --lev) {
+    const int32_t ts = tgt_slot[cell];
+    if (ts >= 0) {
+      double ctr[3];
+      cell_centre(ckey[cell], lev, g, ctr);
+      const double ir = g.inv_rho[lev];
+      const double x = (q.x - ctr[0]) * ir, y = (q.y - ctr[1]) * ir, z = (q.z - ctr[2]) * ir;
+      const double r2 = x * x + y * y + z * z;
+      const double *L = Lexp + (size_t)ts * KP;
+      double hx = 0, hy = 0, hz = 0;
+      double vr = 1, vi = 0; // R_m^m
+      for (int m = 0; m < p; ++m) {
+        if (m > 0) {
+          double f = C->mmf[m];
+          double nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
+          vr = nr;
+          vi = ni;
+        }
+        const double w = m == 0 ? 1.0 : 2.0;
+        double pr = 0, pi = 0, cr = vr, ci = vi;
+        for (int np = m; np < p; ++np) {
+          if (np == m + 1) {
+            pr = cr;
+            pi = ci;
+            double f = z * C->s2m1[m];
+            cr = f * pr;
+            ci = f * pi;
+          } else if (np > m + 1) {
+            double a = C->ra[np][m] * z, b = C->rb[np][m] * r2;
+            double nr = a * cr - b * pr, ni = a * ci - b * pi;
+            pr = cr;
+            pi = ci;
+            cr = nr;
+            ci = ni;
+          }
+          const int nn = np + 1;
+          const cd2 Lm = lget(L, nn, m - 1, p), L0 = lget(L, nn, m, p), Lp = lget(L, nn, m + 1, p);
+          const double cz = C->cz[nn][m], cu = C->cu[nn][m], cdd = C->cd[nn][m];
+          // Re(R * G)
+          hz += w * cz * (cr * L0.x - ci * L0.y);
+          const double ux = cu * Lm.x - cdd * Lp.x, uy = cu * Lm.y - cdd * Lp.y;
+          hx += 0.5 * w * (cr * ux - ci * uy);
+          const double vx = cu * Lm.x + cdd * Lp.x, vy = cu * Lm.y + cdd * Lp.y;
+          hy += 0.5 * w * (cr * vy + ci * vx);
+        }
+      }
+      const double s = ir * ir;
+      gx += s * hx;
+      gy += s * hy;
+      gz += s * hz;
+    }
+    cell = cparent[cell];
+  }
+  grad[3 * i] = gx;
+  grad[3 * i + 1] = gy;
+  grad[3 * i + 2] = gz;
+}
+
+__global__ void k_p2m_count(const int32_t *src_cells, int64_t nsrc, const int32_t *cbeg, const int32_t *cend,
+                            int32_t *cnt) {
+  int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s >= nsrc) return;
+  int32_t c = src_cells[s];
+  cnt[s] = (cend[c] - cbeg[c] + P2M_CHUNK - 1) / P2M_CHUNK;
+}
+__global__ void k_p2m_items(const int32_t *src_cells, int64_t nsrc, const int32_t *cbeg, const int32_t *cend,
+                            const int32_t *off, int4 *items) {
+  int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s >= nsrc) return;
+  int32_t c = src_cells[s];
+  int32_t w = off[s];
+  for (int32_t b = cbeg[c]; b < cend[c]; b += P2M_CHUNK)
+    items[w++] = make_int4((int)s, b, min(b + P2M_CHUNK, cend[c]), 0);
+}
+__global__ void k_terms(const int32_t *cells, int64_t ncell, const int32_t *cbeg, const int32_t *cend,
+                        unsigned long long *out) {
+  __shared__ unsigned long long s[kT];
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  s[threadIdx.x] = i < ncell ? (unsigned long long)(cend[cells[i]] - cbeg[cells[i]]) : 0ull;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
+// Targets ordered by (octant within the parent, cell): targets of one octant
+// share their set of offset classes, so a 16-target tile fills its MFMA
+// columns for every class of a dense region.
+__global__ void k_tile_order(const int32_t *tgt_cells, int64_t ntgt, const uint64_t *ckey, int32_t *cnt8) {
+  int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= ntgt) return;
+  atomicAdd(&cnt8[ckey[tgt_cells[t]] & 7u], 1);
+}
+// position = (padded start of octant o - targets before octant o) + exclusive
+// scan value, i.e. the padded start plus the rank inside the octant (stable).
+__global__ void k_tile_fill(const int32_t *tgt_cells, int64_t ntgt, const uint64_t *ckey, const int32_t *base_adj,
+                            const int32_t *scan_oct, int32_t *tile_tgt) {
+  int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= ntgt) return;
+  int o = (int)(ckey[tgt_cells[t]] & 7u);
+  tile_tgt[base_adj[o] + scan_oct[(int64_t)o * ntgt + t]] = tgt_cells[t];
+}
+__global__ void k_oct_flags(const int32_t *tgt_cells, int64_t ntgt, const uint64_t *ckey, int32_t *flags) {
+  int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= ntgt) return;
+  int o = (int)(ckey[tgt_cells[t]] & 7u);
+  for (int k = 0; k < 8; ++k) flags[(int64_t)k * ntgt + t] = (k == o);
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT > 0 ? (n + kT - 1) / kT : 1); }
+
+template <int KP>
+void launch_m2l(nbfmm_ctx *c, int p, int ntiles, int ncols, const int32_t *tile_tgt) {
+  hipLaunchKernelGGL(k_m2l_mfma<KP>, dim3((ntiles + 3) / 4), dim3(kT), 0, c->stream, p, ntiles, tile_tgt,
+                     (int32_t)ncols, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                     (const int32_t *)c->m2l_src.p, (const uint16_t *)c->m2l_cls.p, (const int32_t *)c->src_slot.p,
+                     (const int32_t *)c->tgt_slot.p, (const double *)c->tmat.p, (const double *)c->mexp.p,
+                     (double *)c->lexp.p);
+  NB_LAUNCH();
+}
+
+} // namespace
+
+int padded_k(int p) { return ((p + 1) * (p + 1) + 15) / 16 * 16; }
+
+void run_expansions(nbfmm_ctx *c) {
+  Plan &P = c->plan;
+  hipStream_t s = c->stream;
+  const int p = (int)c->prm.order;
+  const int KP = padded_k(p);
+  P.kp = KP;
+  const int D = P.geo.depth;
+  LevelGeo g;
+  std::memset(&g, 0, sizeof(g));
+  for (int d = 0; d < 3; ++d) g.lo[d] = P.geo.lo[d];
+  for (int l = 0; l <= D; ++l) {
+    for (int d = 0; d < 3; ++d) g.h[l][d] = P.geo.h(l, d);
+    g.inv_rho[l] = 1.0 / P.geo.rho(l);
+  }
+  for (int l = 0; l <= D + 1; ++l) g.lvl_start[l] = P.lvl_start[l];
+  g.depth = D;
+  // constants
+  Consts *dC = (Consts *)dbuf_ensure(c->consts, sizeof(Consts));
+  {
+    static Consts hC;
+    static bool init = false;
+    if (!init) {
+      fill_consts(hC);
+      init = true;
+    }
+    NB_HIP(hipMemcpyAsync(dC, &hC, sizeof(Consts), hipMemcpyHostToDevice, s));
+  }
+  double *grad = dget<double>(c->grad, (size_t)P.n * 3);
+  if (P.n_tgt == 0) {
+    NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * (size_t)P.n, s));
+    P.n_p2m_terms = P.n_l2p_terms = 0;
+    mark(c, 3);
+    mark(c, 4);
+    mark(c, 5);
+    return;
+  }
+  const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
+  const uint64_t *ckey = (const uint64_t *)c->ckey.p;
+  // ---- P2M of every M2L source cell ----
+  int32_t *pcnt = dget<int32_t>(c->p2m_off, (size_t)P.n_src * 2 + 2 + 8);
+  int32_t *poff = pcnt + P.n_src + 1;
+  hipLaunchKernelGGL(k_p2m_count, dim3(nblk(P.n_src)), dim3(kT), 0, s, (const int32_t *)c->src_cells.p, P.n_src,
+                     cbeg, cend, pcnt);
+  NB_LAUNCH();
+  NB_HIP(hipMemsetAsync(pcnt + P.n_src, 0, sizeof(int32_t), s));
+  scan_excl_i32(pcnt, poff, P.n_src + 1, c->scan_tmp, s);
+  int32_t nitems;
+  NB_HIP(hipMemcpyAsync(&nitems, poff + P.n_src, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  unsigned long long *terms = (unsigned long long *)((double *)c->red.p + 6 * 1024 + 48);
+  NB_HIP(hipMemsetAsync(terms, 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_terms, dim3(nblk(P.n_src)), dim3(kT), 0, s, (const int32_t *)c->src_cells.p, P.n_src, cbeg,
+                     cend, terms);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_terms, dim3(nblk(P.n_tgt)), dim3(kT), 0, s, (const int32_t *)c->tgt_cells.p, P.n_tgt, cbeg,
+                     cend, terms + 1);
+  NB_LAUNCH();
+  unsigned long long th[2];
+  NB_HIP(hipMemcpyAsync(th, terms, sizeof(th), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_p2m_items = nitems;
+  P.n_p2m_terms = (int64_t)th[0];
+  P.n_l2p_terms = (int64_t)th[1];
+  int4 *items = dget<int4>(c->p2m_items, (size_t)nitems + 1);
+  hipLaunchKernelGGL(k_p2m_items, dim3(nblk(P.n_src)), dim3(kT), 0, s, (const int32_t *)c->src_cells.p, P.n_src, cbeg,
+                     cend, poff, items);
+  NB_LAUNCH();
+  double *Mexp = dget<double>(c->mexp, (size_t)P.n_src * KP);
+  NB_HIP(hipMemsetAsync(Mexp, 0, sizeof(double) * (size_t)P.n_src * KP, s));
+  hipLaunchKernelGGL(k_p2m, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
+                     (const int32_t *)c->src_cells.p, ckey, g, dC, Mexp);
+  NB_LAUNCH();
+  mark(c, 3);
+  // ---- M2L ----
+  // offset classes scaled by (hx, hy, hz) / rho (level independent)
+  const double rho = P.geo.rho(D);
+  double *frag = dget<double>(c->tmat, (size_t)kNumClasses * KP * KP);
+  hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, P.geo.h(D, 0) / rho,
+                     P.geo.h(D, 1) / rho, P.geo.h(D, 2) / rho, frag);
+  NB_LAUNCH();
+  // tiles of 16 targets, targets grouped by octant (padded per octant)
+  const int64_t nt = P.n_tgt;
+  const size_t fl = (size_t)std::max<int64_t>(8 * nt, std::max<int64_t>(P.n, P.n_cells)) + 1;
+  int32_t *flags = dget<int32_t>(c->flags, fl);
+  int32_t *fscan = dget<int32_t>(c->fscan, fl);
+  hipLaunchKernelGGL(k_oct_flags, dim3(nblk(nt)), dim3(kT), 0, s, (const int32_t *)c->tgt_cells.p, nt, ckey, flags);
+  NB_LAUNCH();
+  scan_excl_i32(flags, fscan, 8 * nt, c->scan_tmp, s);
+  // fscan[o*nt] = number of targets in octants < o
+  int32_t base8h[8];
+  for (int o = 0; o < 8; ++o)
+    NB_HIP(hipMemcpyAsync(&base8h[o], fscan + (int64_t)o * nt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  int32_t pad8[9];
+  pad8[0] = 0;
+  for (int o = 0; o < 8; ++o) {
+    int32_t cnt = (o < 7 ? base8h[o + 1] : (int32_t)nt) - base8h[o];
+    pad8[o + 1] = pad8[o] + (cnt + 15) / 16 * 16;
+  }
+  const int ncols = pad8[8];
+  const int ntiles = ncols / 16;
+  P.n_m2l_tiles = ntiles;
+  int32_t *tile_tgt = dget<int32_t>(c->tiles, (size_t)ncols + 16);
+  NB_HIP(hipMemsetAsync(tile_tgt, 0xff, sizeof(int32_t) * (size_t)ncols, s));
+  int32_t base_adj[8];
+  for (int o = 0; o < 8; ++o) base_adj[o] = pad8[o] - base8h[o];
+  int32_t *dbase = dget<int32_t>(c->p2m_off, (size_t)P.n_src * 2 + 2 + 8) + P.n_src * 2 + 2;
+  NB_HIP(hipMemcpyAsync(dbase, base_adj, sizeof(base_adj), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_tile_fill, dim3(nblk(nt)), dim3(kT), 0, s, (const int32_t *)c->tgt_cells.p, nt, ckey, dbase,
+                     fscan, tile_tgt);
+  NB_LAUNCH();
+  double *Lexp = dget<double>(c->lexp, (size_t)nt * KP);
+  mark(c, 7);
+  switch (KP) {
+#define NB_M2L_CASE(k)                                  \
+  case k:                                               \
+    launch_m2l<k>(c, p, ntiles, ncols, tile_tgt);       \
+    break;
+    NB_M2L_CASE(16)
+    NB_M2L_CASE(32)
+    NB_M2L_CASE(48)
+    NB_M2L_CASE(64)
+    NB_M2L_CASE(96)
+    NB_M2L_CASE(112)
+    NB_M2L_CASE(128)
+    NB_M2L_CASE(144)
+    NB_M2L_CASE(176)
+    NB_M2L_CASE(208)
+    NB_M2L_CASE(240)
+#undef NB_M2L_CASE
+    default:
+      throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
+  }
+  mark(c, 8);
+  mark(c, 4);
+  // ---- L2P ----
+  hipLaunchKernelGGL(k_l2p, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
+                     (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)c->tgt_slot.p, g,
+                     dC, Lexp, grad);
+  NB_LAUNCH();
+  mark(c, 5);
+}
+
+} // namespace nbfmm

@@ -1,0 +1,140 @@
+// nbfmm_internal.h -- internal declarations of the gfx950 FMM library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nbfmm.h"
+
+namespace nbfmm {
+
+// ---- errors ----------------------------------------------------------------
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+#define NB_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::nbfmm::Error(e_ == hipErrorOutOfMemory ? NBFMM_ERR_OOM : NBFMM_ERR_HIP,   \
+                           std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+#define NB_LAUNCH() NB_HIP(hipGetLastError())
+
+// ---- device buffers grown on demand ------------------------------------------
+struct DBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+void *dbuf_ensure(DBuf &b, size_t bytes);
+void dbuf_free(DBuf &b);
+template <class T>
+inline T *dget(DBuf &b, size_t count) {
+  return static_cast<T *>(dbuf_ensure(b, count * sizeof(T) + 16));
+}
+
+// ---- geometry of one evaluation ----------------------------------------------
+constexpr int kMaxDepth = 16;   // ((64/3)>>3)<<3, tree.cpp:38
+constexpr int kNumClasses = 343; // offsets in [-3,3]^3, class = (dz+3)*49+(dy+3)*7+(dx+3)
+constexpr int kMaxOrder = 16;
+
+struct Geometry {
+  double lo[3], hi[3];
+  int depth = 0;
+  // cell size per dimension at level l is (hi-lo)/2^l; rho_l = max of the three
+  double h(int l, int d) const { return (hi[d] - lo[d]) / (double)(1ull << l); }
+  double rho(int l) const {
+    double r = h(l, 0);
+    if (h(l, 1) > r) r = h(l, 1);
+    if (h(l, 2) > r) r = h(l, 2);
+    return r > 0 ? r : 1.0;
+  }
+};
+
+// Per-level geometry passed to kernels by value.
+struct LevelGeo {
+  double lo[3];
+  double h[kMaxDepth + 1][3];
+  double inv_rho[kMaxDepth + 1];
+  int32_t lvl_start[kMaxDepth + 2];
+  int depth;
+};
+
+// ---- the device-side state of one evaluation -------------------------------------
+struct Plan {
+  int64_t n = 0;
+  Geometry geo;
+  int iterations = 0;
+  std::vector<int32_t> lvl_start; // depth+2, cells concatenated from level 0
+  int64_t n_cells = 0, n_leaves = 0;
+  int64_t n_il = 0, n_m2l = 0, n_sub = 0, n_p2p_src = 0;
+  int64_t n_src = 0, n_tgt = 0, n_p2p_pairs = 0, n_p2m_terms = 0, n_l2p_terms = 0;
+  int64_t n_p2m_items = 0, n_m2l_tiles = 0;
+  int kp = 0; // padded coefficient count (multiple of 16)
+};
+
+} // namespace nbfmm
+
+// The context (definition shared by the host translation units).
+struct nbfmm_ctx {
+  nbfmm_params prm{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  bool timing = true;
+  bool keep_lists = false;
+  nbfmm::Plan plan;
+  nbfmm_stats stats{};
+  // particle arrays
+  nbfmm::DBuf xyz_in, m_in, acc_out;           // staging for host-pointer calls
+  nbfmm::DBuf key[2], idx[2], sort_hist, scan_tmp;
+  nbfmm::DBuf pos;                             // double4 sorted (x,y,z,m)
+  nbfmm::DBuf pcell;                           // leaf cell of each sorted particle
+  nbfmm::DBuf grad;                            // 3 doubles per sorted particle
+  nbfmm::DBuf red;                             // small reductions (bbox, flags, counts)
+  nbfmm::DBuf hred;                            // pinned host mirror of `red`
+  // cells
+  nbfmm::DBuf ckey, cbeg, cend, cparent, child8, nbr27, flags, fscan;
+  nbfmm::DBuf cnt_il, cnt_m2l, cnt_sub, off_il, off_m2l, off_sub, acc_sub;
+  nbfmm::DBuf il, m2l_src, m2l_cls, sub_src;
+  nbfmm::DBuf src_slot, tgt_slot, src_cells, tgt_cells, is_src;
+  nbfmm::DBuf p2p_off, p2p_cnt, p2p_src, p2m_items, p2m_off, tiles;
+  // expansions
+  nbfmm::DBuf stab, tmat, mexp, lexp, consts;
+  // integrator scratch
+  nbfmm::DBuf xk, vk, ak, asum, vsum;
+  // events
+  hipEvent_t ev[16] = {};
+  int nev = 0;
+  ~nbfmm_ctx();
+};
+
+namespace nbfmm {
+// scan.hip
+void scan_excl_i32(const int32_t *in, int32_t *out, int64_t n, DBuf &tmp, hipStream_t s);
+void scan_excl_i64(const int64_t *in, int64_t *out, int64_t n, DBuf &tmp, hipStream_t s);
+// tree.hip
+void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m);
+void build_lists(nbfmm_ctx *c);
+// expansions.hip
+void run_expansions(nbfmm_ctx *c);
+// p2p.hip
+void run_p2p(nbfmm_ctx *c, double *d_acc_out);
+void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,
+                const int64_t *d_tgt, int mode, double *d_acc);
+// integrate.hip
+void se_update(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *a, double dt);
+void rk4_stage(nbfmm_ctx *c, int64_t n, const double *x, const double *v, const double *vk_prev,
+               const double *a_vel, double h, double *xk, double *vk_out);
+void rk4_accum(nbfmm_ctx *c, int64_t n, double *asum, double *vsum, const double *a,
+               const double *vk, double w, int first);
+void rk4_final(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *asum,
+               const double *vsum, double h6);
+void copy_d(nbfmm_ctx *c, double *dst, const double *src, int64_t count);
+// timing helpers (capi.cpp)
+void mark(nbfmm_ctx *c, int slot);
+} // namespace nbfmm

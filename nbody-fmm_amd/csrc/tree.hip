@@ -1,0 +1,726 @@
+// tree.hip -- the reference's uniform-depth sparse octree on the GPU.
+//
+// Follows src/system_updaters/tree_components/tree.cpp:15-90:
+//   bbox (tree.cpp:20-30) -> depth loop (tree.cpp:32-69: quantise, Morton key,
+//   max occupancy, repeat) -> leaves holding their particles in input order
+//   (tree.cpp:71-74) -> ancestors (tree.cpp:76-84) -> neighbour and
+//   interaction lists (node_i.cpp:23-99).
+// Keys are 64-bit (the reference's 32-bit truncation, F6, only matters for
+// depth >= 10); the key layout is Morton::encode's (morton_code.cpp:9-22):
+// bit k of coordinate i -> bit 3k+i.
+//
+// The sort is an LSD radix sort on (key, input index) with 8-bit digits:
+// per pass a block histogram in LDS, a device-wide exclusive scan of the
+// digit-major histogram, and a stable scatter whose in-block ranks come from
+// wavefront ballots (match of equal digits) plus LDS prefix counts.  LSD
+// passes are stable, and the initial order is the input order, so equal keys
+// keep input order exactly as the reference's leaves do.
+#include "nbfmm_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace nbfmm {
+namespace {
+
+constexpr int kT = 256;           // threads per block
+constexpr int kSortItems = 16;    // items per thread per radix tile
+constexpr int kSortTile = kT * kSortItems;
+
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {
+  x &= 0x1fffffull;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+__device__ __forceinline__ uint32_t compact3(uint64_t x) {
+  x &= 0x1249249249249249ull;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
+  x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
+  x = (x ^ (x >> 32)) & 0x1fffffull;
+  return (uint32_t)x;
+}
+
+// ---- bbox + finiteness --------------------------------------------------------
+__global__ void __launch_bounds__(kT) k_bbox(const double *__restrict__ xyz, const double *__restrict__ m,
+                                             int64_t n, double *part, int *bad) {
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = xyz[d];
+    hi[d] = xyz[d];
+  }
+  int b = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    for (int d = 0; d < 3; ++d) {
+      double v = xyz[3 * i + d];
+      if (!isfinite(v)) b = 1;
+      lo[d] = fmin(lo[d], v);
+      hi[d] = fmax(hi[d], v);
+    }
+    if (!isfinite(m[i])) b = 1;
+  }
+  if (b) atomicOr(bad, 1);
+  __shared__ double s[6][kT];
+  for (int d = 0; d < 3; ++d) {
+    s[d][threadIdx.x] = lo[d];
+    s[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int d = 0; d < 3; ++d) {
+        s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + o]);
+        s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + o]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void k_bbox_final(const double *part, int nb, double *out) {
+  int t = threadIdx.x;
+  if (t >= 6) return;
+  double v = part[t];
+  for (int b = 1; b < nb; ++b) v = t < 3 ? fmin(v, part[b * 6 + t]) : fmax(v, part[b * 6 + t]);
+  out[t] = v;
+}
+
+// ---- quantisation + Morton key (tree.cpp:55-61) -------------------------------
+// q = (x - a) / leaf_size (IEEE division), truncated, clamped to 2^depth - 1.
+// A zero extent gives NaN, mapped to 0 (what the reference's x86-64 build does).
+__device__ __forceinline__ uint32_t quantize(double x, double a, double ls, uint32_t nld) {
+  double q = __ddiv_rn(x - a, ls);
+  uint32_t c;
+  if (!(q == q) || q < 0.0)
+    c = 0;
+  else if (q >= 4294967295.0)
+    c = 0xffffffffu;
+  else
+    c = (uint32_t)q;
+  return c >= nld ? nld - 1 : c;
+}
+
+__global__ void __launch_bounds__(kT) k_keys(const double *__restrict__ xyz, int64_t n, double a0, double a1,
+                                             double a2, double l0, double l1, double l2, uint32_t nld,
+                                             uint64_t *__restrict__ key, int32_t *__restrict__ idx) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  uint32_t cx = quantize(xyz[3 * i], a0, l0, nld);
+  uint32_t cy = quantize(xyz[3 * i + 1], a1, l1, nld);
+  uint32_t cz = quantize(xyz[3 * i + 2], a2, l2, nld);
+  key[i] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+  idx[i] = (int32_t)i;
+}
+
+// ---- radix sort ------------------------------------------------------------------
+__global__ void __launch_bounds__(kT) k_rs_hist(const uint64_t *__restrict__ key, int64_t n, int shift,
+                                                int nblocks, int32_t *__restrict__ hist) {
+  __shared__ int32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    int64_t i = base + (int64_t)k * kT + threadIdx.x;
+    if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kT) k_rs_scatter(const uint64_t *__restrict__ kin,
+                                                   const int32_t *__restrict__ vin, int64_t n, int shift,
+                                                   int nblocks, const int32_t *__restrict__ off,
+                                                   uint64_t *__restrict__ kout, int32_t *__restrict__ vout) {
+  __shared__ int32_t run[256];      // running count per digit in this tile
+  __shared__ int32_t wcnt[4][256];  // per-wave digit counts of the current round
+  __shared__ int32_t goff[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  run[threadIdx.x] = 0;
+  goff[threadIdx.x] = off[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    int64_t i = base + (int64_t)k * kT + threadIdx.x;
+    bool valid = i < n;
+    uint64_t kv = valid ? kin[i] : 0ull;
+    int32_t vv = valid ? vin[i] : 0;
+    uint32_t dg = valid ? (uint32_t)((kv >> shift) & 255u) : 0u;
+    // lanes of this wave holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = __ballot((dg >> b) & 1u);
+      peers &= ((dg >> b) & 1u) ? bb : ~bb;
+    }
+    for (int j = threadIdx.x; j < 4 * 256; j += kT) (&wcnt[0][0])[j] = 0;
+    __syncthreads();
+    int rank_w = __popcll(peers & lt);
+    bool leader = valid && (peers & lt) == 0ull;
+    if (leader) wcnt[wave][dg] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int r = run[dg] + rank_w;
+      for (int w = 0; w < wave; ++w) r += wcnt[w][dg];
+      int64_t dst = (int64_t)goff[dg] + r;
+      kout[dst] = kv;
+      vout[dst] = vv;
+    }
+    __syncthreads();
+    run[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] + wcnt[3][threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// ---- runs of equal keys ----------------------------------------------------------
+__global__ void k_run_flags(const uint64_t *__restrict__ key, int64_t n, int32_t *flags) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) flags[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+__global__ void k_run_starts(const int32_t *flags, const int32_t *scan, int64_t n, int32_t *starts) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n && flags[i]) starts[scan[i]] = (int32_t)i;
+}
+__global__ void k_run_max(const int32_t *starts, int64_t nruns, int64_t n, uint32_t *out) {
+  __shared__ uint32_t s[kT];
+  uint32_t best = 0;
+  for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < nruns; r += (int64_t)gridDim.x * kT) {
+    int64_t e = r + 1 < nruns ? starts[r + 1] : n;
+    uint32_t len = (uint32_t)(e - starts[r]);
+    best = len > best ? len : best;
+  }
+  s[threadIdx.x] = best;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, s[0]);
+}
+
+__global__ void k_gather_pos(const double *__restrict__ xyz, const double *__restrict__ m,
+                             const int32_t *__restrict__ idx, int64_t n, double4 *__restrict__ pos) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  int64_t j = idx[i];
+  pos[i] = make_double4(xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2], m[j]);
+}
+
+// ---- levels ------------------------------------------------------------------------
+// For consecutive sorted keys, the coarsest level at which they differ is
+// D - floor(msb/3); every level from there to D starts a new cell at i.
+__global__ void k_level_counts(const uint64_t *__restrict__ key, int64_t n, int D, uint32_t *counts) {
+  __shared__ uint32_t c[kMaxDepth + 1];
+  if (threadIdx.x <= (unsigned)D) c[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) {
+    int d0;
+    if (i == 0) {
+      d0 = 0;
+    } else {
+      uint64_t x = key[i] ^ key[i - 1];
+      d0 = x == 0 ? D + 1 : D - (63 - __clzll((long long)x)) / 3;
+    }
+    for (int d = d0; d <= D; ++d) atomicAdd(&c[d], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x <= (unsigned)D && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
+}
+
+__global__ void k_leaf_fill(const uint64_t *__restrict__ key, int64_t n, const int32_t *__restrict__ flags,
+                            const int32_t *__restrict__ scan, int32_t base, uint64_t *ckey, int32_t *cbeg,
+                            int32_t *cend, int32_t *pcell) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  int32_t leaf = base + scan[i] + flags[i] - 1;
+  pcell[i] = leaf;
+  if (flags[i]) {
+    ckey[leaf] = key[i];
+    cbeg[leaf] = (int32_t)i;
+    if (i > 0) cend[leaf - 1] = (int32_t)i;
+  }
+  if (i == n - 1) cend[leaf] = (int32_t)n;
+}
+
+__global__ void k_parent_flags(const uint64_t *__restrict__ ckey, int32_t cb, int32_t ce, int32_t *flags) {
+  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  flags[c - cb] = (c == cb || (ckey[c] >> 3) != (ckey[c - 1] >> 3)) ? 1 : 0;
+}
+
+__global__ void k_parent_fill(uint64_t *ckey, int32_t *cbeg, int32_t *cend, int32_t *cparent, int32_t cb,
+                              int32_t ce, const int32_t *__restrict__ flags, const int32_t *__restrict__ scan,
+                              int32_t pbase) {
+  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  int32_t k = c - cb;
+  int32_t p = pbase + scan[k] + flags[k] - 1;
+  cparent[c] = p;
+  if (flags[k]) {
+    ckey[p] = ckey[c] >> 3;
+    cbeg[p] = cbeg[c];
+    if (c > cb) cend[p - 1] = cbeg[c];
+  }
+  if (c == ce - 1) cend[p] = cend[c];
+}
+
+__global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent, int32_t cb,
+                         int32_t ce, int32_t *child8) {
+  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  child8[(int64_t)cparent[c] * 8 + (int)(ckey[c] & 7u)] = c;
+}
+
+// 27-neighbourhood (offsets x fastest, self at 13) from the parent's table.
+__global__ void k_nbr27(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent,
+                        const int32_t *__restrict__ child8, int32_t cb, int32_t ce, int lev, int32_t *nbr27) {
+  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  uint64_t k = ckey[c];
+  int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
+  int side = 1 << lev;
+  int32_t P = cparent[c];
+  for (int o = 0; o < 27; ++o) {
+    int qx = x + o % 3 - 1, qy = y + (o / 3) % 3 - 1, qz = z + o / 9 - 1;
+    int32_t r = -1;
+    if (qx >= 0 && qy >= 0 && qz >= 0 && qx < side && qy < side && qz < side) {
+      int px = (qx >> 1) - (x >> 1) + 1, py = (qy >> 1) - (y >> 1) + 1, pz = (qz >> 1) - (z >> 1) + 1;
+      int32_t Pn = nbr27[(int64_t)P * 27 + pz * 9 + py * 3 + px];
+      if (Pn >= 0) r = child8[(int64_t)Pn * 8 + ((qx & 1) | ((qy & 1) << 1) | ((qz & 1) << 2))];
+    }
+    nbr27[(int64_t)c * 27 + o] = r;
+  }
+}
+
+// ---- interaction lists (node_i.cpp:23-99) -----------------------------------------
+// Window offsets d in [-2-(c%2), 3-(c%2)] per axis, x fastest; cells at
+// Chebyshev distance >= 2 form the interaction list.  A listed pair (t, s)
+// is evaluated by M2L, or by direct P2P when n_t * n_s < cut.
+template <class F>
+__device__ __forceinline__ void for_each_il(int32_t c, const uint64_t *ckey, const int32_t *cparent,
+                                            const int32_t *nbr27, const int32_t *child8, int lev, F f) {
+  uint64_t k = ckey[c];
+  int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
+  int side = 1 << lev;
+  int sx = -2 - (x & 1), sy = -2 - (y & 1), sz = -2 - (z & 1);
+  int32_t P = cparent[c];
+  for (int wz = 0; wz < 6; ++wz)
+    for (int wy = 0; wy < 6; ++wy)
+      for (int wx = 0; wx < 6; ++wx) {
+        int dx = sx + wx, dy = sy + wy, dz = sz + wz;
+        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+        int qx = x + dx, qy = y + dy, qz = z + dz;
+        if (qx < 0 || qy < 0 || qz < 0 || qx >= side || qy >= side || qz >= side) continue;
+        int px = (qx >> 1) - (x >> 1) + 1, py = (qy >> 1) - (y >> 1) + 1, pz = (qz >> 1) - (z >> 1) + 1;
+        int32_t Pn = nbr27[(int64_t)P * 27 + pz * 9 + py * 3 + px];
+        if (Pn < 0) continue;
+        int32_t s = child8[(int64_t)Pn * 8 + ((qx & 1) | ((qy & 1) << 1) | ((qz & 1) << 2))];
+        if (s < 0) continue;
+        f(s, (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3));
+      }
+}
+
+__global__ void k_il_count(const uint64_t *ckey, const int32_t *cparent, const int32_t *nbr27,
+                           const int32_t *child8, const int32_t *cbeg, const int32_t *cend, int32_t cb,
+                           int32_t ce, int lev, int64_t cut, int32_t *cnt_il, int32_t *cnt_m2l,
+                           int32_t *cnt_sub) {
+  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  int64_t nt = cend[c] - cbeg[c];
+  int32_t a = 0, m = 0, s = 0;
+  for_each_il(c, ckey, cparent, nbr27, child8, lev, [&](int32_t src, int) {
+    ++a;
+    if (nt * (int64_t)(cend[src] - cbeg[src]) < cut)
+      ++s;
+    else
+      ++m;
+  });
+  cnt_il[c] = a;
+  cnt_m2l[c] = m;
+  cnt_sub[c] = s;
+}
+
+__global__ void k_il_fill(const uint64_t *ckey, const int32_t *cparent, const int32_t *nbr27,
+                          const int32_t *child8, const int32_t *cbeg, const int32_t *cend, int32_t cb,
+                          int32_t ce, int lev, int64_t cut, const int32_t *off_il, const int32_t *off_m2l,
+                          const int32_t *off_sub, int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls,
+                          int32_t *sub_src, int32_t *is_src) {
+  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  int64_t nt = cend[c] - cbeg[c];
+  int32_t a = off_il ? off_il[c] : 0, m = off_m2l[c], s = off_sub[c];
+  for_each_il(c, ckey, cparent, nbr27, child8, lev, [&](int32_t src, int cls) {
+    if (il) il[a++] = src;
+    if (nt * (int64_t)(cend[src] - cbeg[src]) < cut) {
+      sub_src[s++] = src;
+    } else {
+      m2l_src[m] = src;
+      m2l_cls[m] = (uint16_t)cls;
+      ++m;
+      is_src[src] = 1;
+    }
+  });
+}
+
+// acc_sub[c] = cnt_sub[c] + acc_sub[parent(c)]
+__global__ void k_acc_sub(const int32_t *cparent, const int32_t *cnt_sub, int32_t cb, int32_t ce,
+                          int32_t *acc_sub) {
+  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  acc_sub[c] = cnt_sub[c] + (cparent[c] >= 0 ? acc_sub[cparent[c]] : 0);
+}
+
+// P2P source cells of each leaf: itself, its neighbour leaves (node_i.cpp
+// order), then the substituted interaction-list cells of the leaf and of
+// every ancestor (their particles act on all particles below them).
+__global__ void k_p2p_count(const int32_t *nbr27, const int32_t *acc_sub, int32_t cb, int32_t ce,
+                            const int32_t *cbeg, const int32_t *cend, int32_t *cnt, int64_t *pairs) {
+  int32_t l = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  int64_t np = 0;
+  if (l < ce) {
+    int32_t k = 1;
+    int64_t nt = cend[l] - cbeg[l];
+    np = nt * (nt - 1);
+    for (int o = 0; o < 27; ++o) {
+      int32_t q = nbr27[(int64_t)l * 27 + o];
+      if (o != 13 && q >= 0) {
+        ++k;
+        np += nt * (int64_t)(cend[q] - cbeg[q]);
+      }
+    }
+    cnt[l - cb] = k + acc_sub[l];
+  }
+  // pairs of the near field (substituted pairs are counted in k_p2p_fill)
+  __shared__ int64_t s[kT];
+  s[threadIdx.x] = np;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd((unsigned long long *)pairs, (unsigned long long)s[0]);
+}
+
+__global__ void k_p2p_fill(const int32_t *nbr27, const int32_t *cparent, const int32_t *off_sub,
+                           const int32_t *cnt_sub, const int32_t *sub_src, int32_t cb, int32_t ce,
+                           const int32_t *cbeg, const int32_t *cend, const int32_t *p2p_off, int32_t *p2p_src,
+                           int64_t *pairs) {
+  int32_t l = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  int64_t np = 0;
+  if (l < ce) {
+    int32_t w = p2p_off[l - cb];
+    p2p_src[w++] = l;
+    for (int o = 0; o < 27; ++o) {
+      int32_t q = nbr27[(int64_t)l * 27 + o];
+      if (o != 13 && q >= 0) p2p_src[w++] = q;
+    }
+    int64_t nt = cend[l] - cbeg[l];
+    for (int32_t a = l; a >= 0; a = cparent[a]) {
+      for (int32_t j = 0; j < cnt_sub[a]; ++j) {
+        int32_t s = sub_src[off_sub[a] + j];
+        p2p_src[w++] = s;
+        np += nt * (int64_t)(cend[s] - cbeg[s]);
+      }
+    }
+  }
+  __shared__ int64_t s[kT];
+  s[threadIdx.x] = np;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd((unsigned long long *)pairs, (unsigned long long)s[0]);
+}
+
+__global__ void k_flag_pos(const int32_t *v, int64_t n, int32_t *flags) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) flags[i] = v[i] > 0 ? 1 : 0;
+}
+__global__ void k_compact(const int32_t *flags, const int32_t *scan, int64_t n, int32_t *slot, int32_t *list) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i]) {
+    slot[i] = scan[i];
+    list[scan[i]] = (int32_t)i;
+  } else {
+    slot[i] = -1;
+  }
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
+
+// Sorts (key, idx) by the low `bits` bits of key.  Returns the buffer index
+// (0 or 1) holding the result.
+int radix_sort(nbfmm_ctx *c, int64_t n, int bits, int cur) {
+  const int nb = (int)((n + kSortTile - 1) / kSortTile);
+  int32_t *hist = dget<int32_t>(c->sort_hist, (size_t)256 * nb);
+  for (int shift = 0; shift < bits; shift += 8) {
+    uint64_t *kin = (uint64_t *)c->key[cur].p, *kout = (uint64_t *)c->key[cur ^ 1].p;
+    int32_t *vin = (int32_t *)c->idx[cur].p, *vout = (int32_t *)c->idx[cur ^ 1].p;
+    hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kT), 0, c->stream, kin, n, shift, nb, hist);
+    NB_LAUNCH();
+    scan_excl_i32(hist, hist, (int64_t)256 * nb, c->scan_tmp, c->stream);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kT), 0, c->stream, kin, vin, n, shift, nb, hist, kout, vout);
+    NB_LAUNCH();
+    cur ^= 1;
+  }
+  return cur;
+}
+
+int bit_width_u64(uint64_t v) {
+  int w = 0;
+  while (v) {
+    ++w;
+    v >>= 1;
+  }
+  return w;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------
+void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
+  Plan &P = c->plan;
+  const int64_t n = P.n;
+  hipStream_t s = c->stream;
+  // ---- bbox (tree.cpp:20-30) ----
+  int nbb = (int)std::min<int64_t>(1024, (n + kT - 1) / kT);
+  double *red = dget<double>(c->red, 8 * 1024 + 64);
+  double *out6 = red + 6 * 1024;
+  int *bad = (int *)(out6 + 8);
+  NB_HIP(hipMemsetAsync(bad, 0, sizeof(int) * 4, s));
+  hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(kT), 0, s, d_xyz, d_m, n, red, bad);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, red, nbb, out6);
+  NB_LAUNCH();
+  double *h = (double *)dbuf_ensure(c->hred, 0); // pinned host mirror
+  NB_HIP(hipMemcpyAsync(h, out6, sizeof(double) * 8 + sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  if (((int *)(h + 8))[0]) throw Error(NBFMM_ERR_DEGENERATE, "non-finite coordinate or mass");
+  for (int d = 0; d < 3; ++d) {
+    P.geo.lo[d] = h[d];
+    P.geo.hi[d] = h[3 + d];
+  }
+  // ---- depth loop (tree.cpp:32-69) ----
+  const uint32_t thr = c->prm.leaf_capacity;
+  uint32_t maxcnt = (uint32_t)n;
+  int depth = 0;
+  dget<uint64_t>(c->key[0], (size_t)n);
+  dget<uint64_t>(c->key[1], (size_t)n);
+  dget<int32_t>(c->idx[0], (size_t)n);
+  dget<int32_t>(c->idx[1], (size_t)n);
+  int32_t *flags = dget<int32_t>(c->flags, (size_t)n + 1);
+  int32_t *fscan = dget<int32_t>(c->fscan, (size_t)n + 1);
+  uint32_t *dmax = (uint32_t *)(out6 + 12);
+  int cur = 0;
+  P.iterations = 0;
+  double ls[3];
+  do {
+    uint64_t nl = (uint64_t)((maxcnt - 1u) / thr + 1u);
+    depth += bit_width_u64(nl) / 3 + 1;
+    if (depth > kMaxDepth) depth = kMaxDepth;
+    uint64_t nld = 1ull << depth;
+    for (int d = 0; d < 3; ++d) ls[d] = (P.geo.hi[d] - P.geo.lo[d]) / (double)nld;
+    hipLaunchKernelGGL(k_keys, dim3(nblk(n)), dim3(kT), 0, s, d_xyz, n, P.geo.lo[0], P.geo.lo[1], P.geo.lo[2],
+                       ls[0], ls[1], ls[2], (uint32_t)nld, (uint64_t *)c->key[0].p, (int32_t *)c->idx[0].p);
+    NB_LAUNCH();
+    cur = radix_sort(c, n, 3 * depth, 0);
+    const uint64_t *k = (const uint64_t *)c->key[cur].p;
+    hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(kT), 0, s, k, n, flags);
+    NB_LAUNCH();
+    scan_excl_i32(flags, fscan, n, c->scan_tmp, s);
+    // number of runs = fscan[n-1] + flags[n-1]
+    int32_t *starts = (int32_t *)c->idx[cur ^ 1].p; // free while not sorting
+    hipLaunchKernelGGL(k_run_starts, dim3(nblk(n)), dim3(kT), 0, s, flags, fscan, n, starts);
+    NB_LAUNCH();
+    int32_t tail[2];
+    NB_HIP(hipMemcpyAsync(&tail[0], fscan + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NB_HIP(hipMemcpyAsync(&tail[1], flags + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NB_HIP(hipStreamSynchronize(s));
+    int64_t nruns = (int64_t)tail[0] + tail[1];
+    NB_HIP(hipMemsetAsync(dmax, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_run_max, dim3((unsigned)std::min<int64_t>(1024, nblk(nruns))), dim3(kT), 0, s, starts,
+                       nruns, n, dmax);
+    NB_LAUNCH();
+    NB_HIP(hipMemcpyAsync(&maxcnt, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NB_HIP(hipStreamSynchronize(s));
+    P.iterations++;
+    P.n_leaves = nruns;
+  } while (depth < kMaxDepth && maxcnt > thr);
+  P.geo.depth = depth;
+  if (cur != 0) { // keep the sorted keys/indices in buffer 0
+    std::swap(c->key[0], c->key[1]);
+    std::swap(c->idx[0], c->idx[1]);
+  }
+  const uint64_t *key = (const uint64_t *)c->key[0].p;
+  const int32_t *idx = (const int32_t *)c->idx[0].p;
+  double4 *pos = dget<double4>(c->pos, (size_t)n);
+  hipLaunchKernelGGL(k_gather_pos, dim3(nblk(n)), dim3(kT), 0, s, d_xyz, d_m, idx, n, pos);
+  NB_LAUNCH();
+
+  // ---- cells of every level ----
+  const int D = depth;
+  uint32_t *dcounts = (uint32_t *)(out6 + 16);
+  NB_HIP(hipMemsetAsync(dcounts, 0, sizeof(uint32_t) * (kMaxDepth + 1), s));
+  hipLaunchKernelGGL(k_level_counts, dim3(nblk(n)), dim3(kT), 0, s, key, n, D, dcounts);
+  NB_LAUNCH();
+  uint32_t counts[kMaxDepth + 1];
+  NB_HIP(hipMemcpyAsync(counts, dcounts, sizeof(uint32_t) * (D + 1), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.lvl_start.assign(D + 2, 0);
+  for (int d = 0; d <= D; ++d) P.lvl_start[d + 1] = P.lvl_start[d] + (int32_t)counts[d];
+  const int64_t nc = P.lvl_start[D + 1];
+  P.n_cells = nc;
+  uint64_t *ckey = dget<uint64_t>(c->ckey, (size_t)nc);
+  int32_t *cbeg = dget<int32_t>(c->cbeg, (size_t)nc);
+  int32_t *cend = dget<int32_t>(c->cend, (size_t)nc);
+  int32_t *cpar = dget<int32_t>(c->cparent, (size_t)nc);
+  int32_t *pcell = dget<int32_t>(c->pcell, (size_t)n);
+  // leaves
+  hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(kT), 0, s, key, n, flags);
+  NB_LAUNCH();
+  scan_excl_i32(flags, fscan, n, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_leaf_fill, dim3(nblk(n)), dim3(kT), 0, s, key, n, flags, fscan, P.lvl_start[D], ckey, cbeg,
+                     cend, pcell);
+  NB_LAUNCH();
+  // ancestors
+  for (int d = D - 1; d >= 0; --d) {
+    int32_t cb = P.lvl_start[d + 1], ce = P.lvl_start[d + 2];
+    hipLaunchKernelGGL(k_parent_flags, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cb, ce, flags);
+    NB_LAUNCH();
+    scan_excl_i32(flags, fscan, ce - cb, c->scan_tmp, s);
+    hipLaunchKernelGGL(k_parent_fill, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cbeg, cend, cpar, cb, ce, flags,
+                       fscan, P.lvl_start[d]);
+    NB_LAUNCH();
+  }
+  NB_HIP(hipMemsetAsync(cpar, 0xff, sizeof(int32_t), s)); // root parent = -1
+  // neighbour tables
+  int32_t *child8 = dget<int32_t>(c->child8, (size_t)nc * 8);
+  int32_t *nbr27 = dget<int32_t>(c->nbr27, (size_t)nc * 27);
+  NB_HIP(hipMemsetAsync(child8, 0xff, sizeof(int32_t) * (size_t)nc * 8, s));
+  if (nc > 1) {
+    hipLaunchKernelGGL(k_child8, dim3(nblk(nc - 1)), dim3(kT), 0, s, ckey, cpar, 1, (int32_t)nc, child8);
+    NB_LAUNCH();
+  }
+  NB_HIP(hipMemsetAsync(nbr27, 0xff, sizeof(int32_t) * 27, s));
+  NB_HIP(hipMemsetAsync(nbr27 + 13, 0, sizeof(int32_t), s));
+  for (int d = 1; d <= D; ++d) {
+    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
+    hipLaunchKernelGGL(k_nbr27, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d, nbr27);
+    NB_LAUNCH();
+  }
+}
+
+void build_lists(nbfmm_ctx *c) {
+  Plan &P = c->plan;
+  hipStream_t s = c->stream;
+  const int D = P.geo.depth;
+  const int64_t nc = P.n_cells;
+  const uint64_t *ckey = (const uint64_t *)c->ckey.p;
+  const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
+  const int32_t *cpar = (const int32_t *)c->cparent.p;
+  const int32_t *child8 = (const int32_t *)c->child8.p, *nbr27 = (const int32_t *)c->nbr27.p;
+  const int64_t cut = c->prm.p2p_cut;
+  int32_t *cnt_il = dget<int32_t>(c->cnt_il, (size_t)nc + 1);
+  int32_t *cnt_m2l = dget<int32_t>(c->cnt_m2l, (size_t)nc + 1);
+  int32_t *cnt_sub = dget<int32_t>(c->cnt_sub, (size_t)nc + 1);
+  int32_t *off_il = dget<int32_t>(c->off_il, (size_t)nc + 1);
+  int32_t *off_m2l = dget<int32_t>(c->off_m2l, (size_t)nc + 1);
+  int32_t *off_sub = dget<int32_t>(c->off_sub, (size_t)nc + 1);
+  NB_HIP(hipMemsetAsync(cnt_il, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  NB_HIP(hipMemsetAsync(cnt_m2l, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  for (int d = 2; d <= D; ++d) {
+    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
+    hipLaunchKernelGGL(k_il_count, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cpar, nbr27, child8, cbeg, cend, cb, ce,
+                       d, cut, cnt_il, cnt_m2l, cnt_sub);
+    NB_LAUNCH();
+  }
+  scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
+  scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
+  scan_excl_i32(cnt_sub, off_sub, nc + 1, c->scan_tmp, s);
+  int32_t tot[3];
+  NB_HIP(hipMemcpyAsync(&tot[0], off_il + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&tot[1], off_m2l + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&tot[2], off_sub + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_il = tot[0];
+  P.n_m2l = tot[1];
+  P.n_sub = tot[2];
+  int32_t *il = c->keep_lists ? dget<int32_t>(c->il, (size_t)P.n_il + 1) : nullptr;
+  int32_t *m2l_src = dget<int32_t>(c->m2l_src, (size_t)P.n_m2l + 1);
+  uint16_t *m2l_cls = dget<uint16_t>(c->m2l_cls, (size_t)P.n_m2l + 1);
+  int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
+  int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
+  NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  for (int d = 2; d <= D; ++d) {
+    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
+    hipLaunchKernelGGL(k_il_fill, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cpar, nbr27, child8, cbeg, cend, cb, ce,
+                       d, cut, il ? off_il : nullptr, off_m2l, off_sub, il, m2l_src, m2l_cls, sub_src, is_src);
+    NB_LAUNCH();
+  }
+  // source / target compaction
+  int32_t *flags = dget<int32_t>(c->flags, (size_t)std::max<int64_t>(nc, P.n) + 1);
+  int32_t *fscan = dget<int32_t>(c->fscan, (size_t)std::max<int64_t>(nc, P.n) + 1);
+  int32_t *src_slot = dget<int32_t>(c->src_slot, (size_t)nc);
+  int32_t *tgt_slot = dget<int32_t>(c->tgt_slot, (size_t)nc);
+  int32_t *src_cells = dget<int32_t>(c->src_cells, (size_t)nc);
+  int32_t *tgt_cells = dget<int32_t>(c->tgt_cells, (size_t)nc);
+  int32_t nst[4];
+  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, is_src, nc, flags);
+  NB_LAUNCH();
+  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, src_slot, src_cells);
+  NB_LAUNCH();
+  NB_HIP(hipMemcpyAsync(&nst[0], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&nst[1], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_src = (int64_t)nst[0] + nst[1];
+  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, cnt_m2l, nc, flags);
+  NB_LAUNCH();
+  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, tgt_slot, tgt_cells);
+  NB_LAUNCH();
+  NB_HIP(hipMemcpyAsync(&nst[2], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&nst[3], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_tgt = (int64_t)nst[2] + nst[3];
+
+  // ---- P2P source lists of the leaves ----
+  int32_t *acc_sub = dget<int32_t>(c->acc_sub, (size_t)nc);
+  for (int d = 0; d <= D; ++d) {
+    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
+    hipLaunchKernelGGL(k_acc_sub, dim3(nblk(ce - cb)), dim3(kT), 0, s, cpar, cnt_sub, cb, ce, acc_sub);
+    NB_LAUNCH();
+  }
+  const int32_t lb = P.lvl_start[D], le = P.lvl_start[D + 1];
+  const int64_t nl = le - lb;
+  int32_t *p2p_cnt = dget<int32_t>(c->p2p_cnt, (size_t)nl + 1);
+  int32_t *p2p_off = dget<int32_t>(c->p2p_off, (size_t)nl + 1);
+  int64_t *pairs = (int64_t *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
+  NB_HIP(hipMemsetAsync(pairs, 0, sizeof(int64_t) * 2, s));
+  NB_HIP(hipMemsetAsync(p2p_cnt + nl, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_p2p_count, dim3(nblk(nl)), dim3(kT), 0, s, nbr27, acc_sub, lb, le, cbeg, cend, p2p_cnt, pairs);
+  NB_LAUNCH();
+  scan_excl_i32(p2p_cnt, p2p_off, nl + 1, c->scan_tmp, s);
+  int32_t np2p;
+  NB_HIP(hipMemcpyAsync(&np2p, p2p_off + nl, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_p2p_src = np2p;
+  int32_t *p2p_src = dget<int32_t>(c->p2p_src, (size_t)np2p + 1);
+  hipLaunchKernelGGL(k_p2p_fill, dim3(nblk(nl)), dim3(kT), 0, s, nbr27, cpar, off_sub, cnt_sub, sub_src, lb, le, cbeg,
+                     cend, p2p_off, p2p_src, pairs + 1);
+  NB_LAUNCH();
+  int64_t pr[2];
+  NB_HIP(hipMemcpyAsync(pr, pairs, sizeof(int64_t) * 2, hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_p2p_pairs = pr[0] + pr[1];
+}
+
+} // namespace nbfmm

@@ -87,14 +87,17 @@ int ref_naive_update(int64_t n, const double *xyz, const double *m, double *acc)
 }
 
 // Runs `nsteps` integrator steps (kind 0 = SyplecticEuler, 1 = RungeKutta4)
-// with updater 0 = NaiveUpdate, 1 = FMM.  xyz/vel/acc are updated in place;
-// acc returns each particle's carried acceleration after the last step.
+// with updater 0 = NaiveUpdate, 1 = FMM.  With preupdate != 0 the updater
+// runs once before the first step, as tests/test_naive_update.cpp:27 does,
+// so the particles carry a(x0).  xyz/vel/acc are updated in place; acc
+// returns each particle's carried acceleration after the last step.
 int ref_integrate(int64_t n, double *xyz, double *vel, double *acc, const double *m, int kind,
-                  int updater, double dt, int nsteps) {
+                  int updater, double dt, int nsteps, int preupdate) {
   auto P = make_particles(n, xyz, m, vel);
   NaiveUpdate nu;
   FMM fm;
   SystemUpdateMethod *u = updater == 0 ? (SystemUpdateMethod *)&nu : (SystemUpdateMethod *)&fm;
+  if (preupdate) u->update(P);
   SyplecticEuler se;
   RungeKutta4 rk;
   QuadratureMethod *q = kind == 0 ? (QuadratureMethod *)&se : (QuadratureMethod *)&rk;

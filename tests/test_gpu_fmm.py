@@ -1,0 +1,159 @@
+"""GPU parity tests: libnbfmm.so (through its C ABI) against the CPU oracle.
+
+Bars: bit-exact for the tree (keys, leaves, order, lists) and for the
+NaiveUpdate formula; FP64 FMM within 1e-12 relative (L2 over all particles)
+of the oracle's exact FMM on the same tree, lists and expansion order.
+"""
+import numpy as np
+import pytest
+
+import nbfmm
+import orc
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = nbfmm.Context(order=4, leaf_capacity=16, G=1.0, eps2=0.0)
+    yield c
+    c.close()
+
+
+def _check_tree(gt, ot, n):
+    assert gt["depth"] == ot["depth"]
+    assert np.array_equal(gt["level_start"], ot["level_start"])
+    assert np.array_equal(gt["key"], ot["key"])
+    assert np.array_equal(gt["pbeg"], ot["pbeg"])
+    assert np.array_equal(gt["pend"], ot["pend"])
+    assert np.array_equal(gt["parent"], ot["parent"])
+    assert np.array_equal(gt["perm"], ot["perm"])
+    assert np.array_equal(gt["nbr_off"], ot["nbr_off"])
+    assert np.array_equal(gt["nbr"], ot["nbr"])
+    if "il" in gt:
+        assert np.array_equal(gt["il_off"], ot["il_off"])
+        assert np.array_equal(gt["il"], ot["il"])
+
+
+@pytest.mark.parametrize("case", ["uniform2k", "plummer50k", "tiny", "clustered"])
+def test_tree_bitexact(ctx, case):
+    if case == "uniform2k":
+        xyz, m = workloads.uniform_cube(2000, seed=3)
+        thr = 16
+    elif case == "plummer50k":
+        xyz, m = workloads.plummer(50000, seed=5)
+        thr = 64
+    elif case == "tiny":
+        xyz, m = workloads.uniform_cube(7, seed=9)
+        thr = 20
+    else:
+        xyz, m = workloads.plummer_mix(20000, n_spheres=6, seed=2)
+        thr = 20
+    ctx.set_params(leaf_capacity=thr, order=4, p2p_cut=0)
+    print(f"[tree] case={case} n={len(m)} thr={thr}", flush=True)
+    _, gt = ctx.eval_tree(xyz, m, with_il=True)
+    ot = orc.tree(xyz, thr)
+    _check_tree(gt, ot, len(m))
+
+
+def test_direct_naive_bitwise(ctx):
+    xyz, m = workloads.uniform_cube(1500, seed=11)
+    ctx.set_params(G=6.6743e-11, eps2=1e-4)
+    a = ctx.direct_accel(xyz, m)
+    b = orc.direct_naive(xyz, m, G=6.6743e-11, eps2=1e-4)
+    ctx.set_params(G=1.0, eps2=0.0)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("p,cut", [(2, 0), (4, 0), (8, 0), (10, 256), (12, 1024), (6, 1 << 40)])
+def test_fmm_parity_exact(ctx, p, cut):
+    xyz, m = workloads.uniform_cube(3000, seed=p)
+    ctx.set_params(order=p, leaf_capacity=16, p2p_cut=cut, G=1.0, eps2=0.0)
+    print(f"[fmm] p={p} cut={cut}", flush=True)
+    a = ctx.eval_accel(xyz, m)
+    b = orc.fmm_exact(xyz, m, p, 16, G=1.0, center_mode=0, p2p_cut=cut)
+    assert _rel(a, b) < 1e-12
+
+
+def test_fmm_parity_plummer(ctx):
+    xyz, m = workloads.plummer(40000, seed=1)
+    ctx.set_params(order=8, leaf_capacity=64, p2p_cut=512, G=1.0, eps2=0.0)
+    a = ctx.eval_accel(xyz, m)
+    b = orc.fmm_exact(xyz, m, 8, 64, G=1.0, center_mode=0, p2p_cut=512)
+    assert _rel(a, b) < 1e-12
+
+
+def test_all_p2p_equals_direct(ctx):
+    # every interaction-list pair evaluated directly: the FMM is the direct sum
+    xyz, m = workloads.plummer(20000, seed=8)
+    ctx.set_params(order=4, leaf_capacity=32, p2p_cut=1 << 50, G=1.0, eps2=0.0)
+    a = ctx.eval_accel(xyz, m)
+    n = len(m)
+    dx, dm, da = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m), nbfmm.DeviceArray(np.zeros((n, 3)))
+    ctx.direct_accel_dev(n, dx.ptr, dm.ptr, n, None, 1, da.ptr)
+    d = da.download()
+    assert _rel(a, d) < 1e-13
+
+
+def test_fmm_accuracy_vs_direct(ctx):
+    xyz, m = workloads.plummer(100000, seed=2)
+    n = len(m)
+    tgt = workloads.sample_targets(n, 2048)
+    dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
+    dt_ = nbfmm.DeviceArray(tgt)
+    dd = nbfmm.DeviceArray(np.zeros((len(tgt), 3)))
+    ctx.direct_accel_dev(n, dx.ptr, dm.ptr, len(tgt), dt_.ptr, 1, dd.ptr)
+    ref = dd.download()
+    # the GPU direct oracle agrees with the CPU oracle on a few targets
+    few = tgt[:32]
+    cpu = orc.direct_targets(xyz, m, few, G=1.0)
+    assert _rel(ref[:32], cpu) < 1e-14
+    errs = []
+    for p in (4, 8, 12):
+        ctx.set_params(order=p, leaf_capacity=64, p2p_cut=1024)
+        a = ctx.eval_accel(xyz, m)
+        errs.append(_rel(a[tgt], ref))
+    print("errors vs direct", errs)
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 2e-7
+
+
+def test_se_rk4_steps(ctx):
+    xyz, m, vel = workloads.plummer(3000, seed=4, with_velocity=True)
+    n = len(m)
+    acc0 = np.zeros((n, 3))
+    p, thr, cut, dt = 6, 16, 256, 1e-3
+    ctx.set_params(order=p, leaf_capacity=thr, p2p_cut=cut, G=1.0, eps2=0.0)
+    cfg = orc.UpdCfg(orc.UPD_FMM_EXACT, p, thr, 1.0, 0.0, 0, cut)
+    for kind, mode in (("se", 0), ("rk4", 0), ("rk4", 1)):
+        dx, dv, da, dm = (nbfmm.DeviceArray(a) for a in (xyz, vel, acc0, m))
+        if kind == "se":
+            ctx.step_se_dev(n, dx.ptr, dv.ptr, da.ptr, dm.ptr, dt, 2)
+        else:
+            ctx.step_rk4_dev(n, dx.ptr, dv.ptr, da.ptr, dm.ptr, dt, 2, mode)
+        gx, gv = dx.download(), dv.download()
+        ox, ov, oa = xyz, vel, acc0
+        for _ in range(2):
+            ox, ov, oa = orc.step(kind, ox, ov, oa, m, cfg, dt, rk4_mode=mode)
+        assert _rel(gx, ox) < 1e-14, kind
+        assert _rel(gv - vel, ov - vel) < 1e-10, kind
+
+
+def test_bad_inputs(ctx):
+    xyz, m = workloads.uniform_cube(100, seed=1)
+    xyz[5, 1] = np.nan
+    with pytest.raises(nbfmm.NbfmmError) as e:
+        ctx.eval_accel(xyz, m)
+    assert e.value.status == 2
+    with pytest.raises(nbfmm.NbfmmError):
+        ctx.set_params(order=0)
+    ctx.params.order = 4
+    one = ctx.eval_accel(np.zeros((1, 3)), np.ones(1))
+    assert np.all(one == 0)
+    same = ctx.eval_accel(np.zeros((50, 3)), np.ones(50))
+    assert np.all(same == 0)
