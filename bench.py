@@ -109,13 +109,13 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = n * args.steps / wall
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step (events {gpu_ms / args.steps:.2f})")
-    acc = da.download()
+    acc = da.download(ctx.stream)
     # accuracy vs the FP64 direct sum on sampled targets (outside the timed region)
     tgt = workloads.sample_targets(n, args.targets)
     dt_ = nbfmm.DeviceArray(tgt)
     dd = nbfmm.DeviceArray(nbytes=len(tgt) * 24, shape=(len(tgt), 3))
     ctx.direct_accel_dev(n, dx.ptr, dm.ptr, len(tgt), dt_.ptr, 1, dd.ptr)
-    ref = dd.download()
+    ref = dd.download(ctx.stream)
     diff = acc[tgt] - ref
     e_i = np.linalg.norm(diff, axis=1) / np.linalg.norm(ref, axis=1)
     err = {"l2": float(np.linalg.norm(diff) / np.linalg.norm(ref)), "max": float(e_i.max()),

@@ -51,6 +51,26 @@ def build_lib(jobs=8):
     return OUT
 
 
+HOST_LIB = os.path.join(HERE, "libnbfmm_host.so")
+DROPIN_TEST = os.path.join(BUILD, "test_dropin")
+
+
+def build_host():
+    """The C++20 drop-in classes (Tensor, Particle, FMM, NaiveUpdate,
+    SyplecticEuler, RungeKutta4) over the C ABI, and the C++ test program."""
+    os.makedirs(BUILD, exist_ok=True)
+    inc = os.path.join(HERE, "host", "include")
+    srcs = [os.path.join(HERE, "host", "src", f) for f in sorted(os.listdir(os.path.join(HERE, "host", "src")))
+            if f.endswith(".cpp")]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", f"-I{inc}", "-o", HOST_LIB, *srcs,
+          f"-L{HERE}", "-lnbfmm", rpath])
+    _run(["g++", "-std=c++20", "-O2", f"-I{inc}", "-o", DROPIN_TEST,
+          os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp"), f"-L{HERE}", "-lnbfmm_host", "-lnbfmm",
+          f"-Wl,-rpath,{HERE}"])
+    return HOST_LIB
+
+
 def build_oracle():
     od = os.path.join(ROOT, "oracle")
     _run(["make", "-B", "-C", od, "liborc.so"])
@@ -61,6 +81,7 @@ def build_oracle():
 
 if __name__ == "__main__":
     build_lib()
+    build_host()
     if "--lib-only" not in sys.argv:
         build_oracle()
     print("built", OUT)

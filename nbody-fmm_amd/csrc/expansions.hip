@@ -100,7 +100,8 @@ __global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, double rx, doub
     for (int k = 2; k <= (int)threadIdx.x; ++k) f *= k;
     lf[threadIdx.x] = f;
   }
-  const double X = dx * rx, Y = dy * ry, Z = dz * rz;
+  // the class offset d is source - target; M2L needs c_L - c_M = -d
+  const double X = -dx * rx, Y = -dy * ry, Z = -dz * rz;
   const double r2 = X * X + Y * Y + Z * Z;
   if (!near && (int)threadIdx.x <= P2) {
     // column M = threadIdx.x of S (Yoshida normalisation)

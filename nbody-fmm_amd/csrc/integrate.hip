@@ -10,6 +10,10 @@
 //   each component by the scalar: tensor.cpp:226-232.)
 #include "nbfmm_internal.h"
 
+// HIP contracts a*b+c into FMA by default (and __dmul_rn is a plain multiply);
+// the reference's x86-64 build rounds every operation, so contraction is off.
+#pragma clang fp contract(off)
+
 namespace nbfmm {
 namespace {
 constexpr int kT = 256;

@@ -74,7 +74,7 @@ struct Plan {
   int64_t n_cells = 0, n_leaves = 0;
   int64_t n_il = 0, n_m2l = 0, n_sub = 0, n_p2p_src = 0;
   int64_t n_src = 0, n_tgt = 0, n_p2p_pairs = 0, n_p2m_terms = 0, n_l2p_terms = 0;
-  int64_t n_p2m_items = 0, n_m2l_tiles = 0;
+  int64_t n_p2m_items = 0, n_m2l_tiles = 0, n_tasks = 0;
   int kp = 0; // padded coefficient count (multiple of 16)
 };
 
@@ -102,7 +102,8 @@ struct nbfmm_ctx {
   nbfmm::DBuf cnt_il, cnt_m2l, cnt_sub, off_il, off_m2l, off_sub, acc_sub;
   nbfmm::DBuf il, m2l_src, m2l_cls, sub_src;
   nbfmm::DBuf src_slot, tgt_slot, src_cells, tgt_cells, is_src;
-  nbfmm::DBuf p2p_off, p2p_cnt, p2p_src, p2m_items, p2m_off, tiles;
+  nbfmm::DBuf p2p_src, p2m_items, p2m_off, tiles;
+  nbfmm::DBuf task_cell, task_off, task_cnt, dpart;
   // expansions
   nbfmm::DBuf stab, tmat, mexp, lexp, consts;
   // integrator scratch

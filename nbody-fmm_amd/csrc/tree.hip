@@ -82,12 +82,21 @@ __global__ void __launch_bounds__(kT) k_bbox(const double *__restrict__ xyz, con
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void k_bbox_final(const double *part, int nb, double *out) {
-  int t = threadIdx.x;
-  if (t >= 6) return;
-  double v = part[t];
-  for (int b = 1; b < nb; ++b) v = t < 3 ? fmin(v, part[b * 6 + t]) : fmax(v, part[b * 6 + t]);
-  out[t] = v;
+__global__ void __launch_bounds__(kT) k_bbox_final(const double *part, int nb, double *out) {
+  __shared__ double s[6][kT];
+  double v[6];
+  for (int d = 0; d < 6; ++d) v[d] = part[d];
+  for (int b = threadIdx.x; b < nb; b += kT)
+    for (int d = 0; d < 6; ++d) v[d] = d < 3 ? fmin(v[d], part[b * 6 + d]) : fmax(v[d], part[b * 6 + d]);
+  for (int d = 0; d < 6; ++d) s[d][threadIdx.x] = v[d];
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int d = 0; d < 6; ++d)
+        s[d][threadIdx.x] = d < 3 ? fmin(s[d][threadIdx.x], s[d][threadIdx.x + o]) : fmax(s[d][threadIdx.x], s[d][threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 // ---- quantisation + Morton key (tree.cpp:55-61) -------------------------------
@@ -367,35 +376,31 @@ __global__ void k_il_fill(const uint64_t *ckey, const int32_t *cparent, const in
   });
 }
 
-// acc_sub[c] = cnt_sub[c] + acc_sub[parent(c)]
-__global__ void k_acc_sub(const int32_t *cparent, const int32_t *cnt_sub, int32_t cb, int32_t ce,
-                          int32_t *acc_sub) {
-  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
-  acc_sub[c] = cnt_sub[c] + (cparent[c] >= 0 ? acc_sub[cparent[c]] : 0);
-}
-
-// P2P source cells of each leaf: itself, its neighbour leaves (node_i.cpp
-// order), then the substituted interaction-list cells of the leaf and of
-// every ancestor (their particles act on all particles below them).
-__global__ void k_p2p_count(const int32_t *nbr27, const int32_t *acc_sub, int32_t cb, int32_t ce,
-                            const int32_t *cbeg, const int32_t *cend, int32_t *cnt, int64_t *pairs) {
-  int32_t l = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+// P2P tasks: every leaf (sources: itself, its neighbour leaves in node_i.cpp
+// order, its directly-evaluated interaction-list cells) and every coarser
+// cell with directly-evaluated interaction-list cells.
+__global__ void k_task_count(const int32_t *nbr27, const int32_t *cnt_sub, const int32_t *cbeg, const int32_t *cend,
+                             int32_t nc, int32_t lb, int32_t *cnt, int32_t *flags, int64_t *pairs) {
+  int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
   int64_t np = 0;
-  if (l < ce) {
-    int32_t k = 1;
-    int64_t nt = cend[l] - cbeg[l];
-    np = nt * (nt - 1);
-    for (int o = 0; o < 27; ++o) {
-      int32_t q = nbr27[(int64_t)l * 27 + o];
-      if (o != 13 && q >= 0) {
-        ++k;
-        np += nt * (int64_t)(cend[q] - cbeg[q]);
+  if (c < nc) {
+    int32_t k = 0;
+    const int64_t nt = cend[c] - cbeg[c];
+    if (c >= lb) {
+      k = 1;
+      np = nt * (nt - 1);
+      for (int o = 0; o < 27; ++o) {
+        int32_t q = nbr27[(int64_t)c * 27 + o];
+        if (o != 13 && q >= 0) {
+          ++k;
+          np += nt * (int64_t)(cend[q] - cbeg[q]);
+        }
       }
     }
-    cnt[l - cb] = k + acc_sub[l];
+    k += cnt_sub[c];
+    cnt[c] = k;
+    flags[c] = k > 0 ? 1 : 0;
   }
-  // pairs of the near field (substituted pairs are counted in k_p2p_fill)
   __shared__ int64_t s[kT];
   s[threadIdx.x] = np;
   __syncthreads();
@@ -406,26 +411,27 @@ __global__ void k_p2p_count(const int32_t *nbr27, const int32_t *acc_sub, int32_
   if (threadIdx.x == 0) atomicAdd((unsigned long long *)pairs, (unsigned long long)s[0]);
 }
 
-__global__ void k_p2p_fill(const int32_t *nbr27, const int32_t *cparent, const int32_t *off_sub,
-                           const int32_t *cnt_sub, const int32_t *sub_src, int32_t cb, int32_t ce,
-                           const int32_t *cbeg, const int32_t *cend, const int32_t *p2p_off, int32_t *p2p_src,
-                           int64_t *pairs) {
-  int32_t l = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+__global__ void k_task_fill(const int32_t *nbr27, const int32_t *off_sub, const int32_t *cnt_sub, const int32_t *sub_src,
+                            const int32_t *cbeg, const int32_t *cend, int32_t nc, int32_t lb, const int32_t *cnt,
+                            const int32_t *toff, const int32_t *flags, const int32_t *fscan, int32_t *task_cell,
+                            int32_t *src, int64_t *pairs) {
+  int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
   int64_t np = 0;
-  if (l < ce) {
-    int32_t w = p2p_off[l - cb];
-    p2p_src[w++] = l;
-    for (int o = 0; o < 27; ++o) {
-      int32_t q = nbr27[(int64_t)l * 27 + o];
-      if (o != 13 && q >= 0) p2p_src[w++] = q;
-    }
-    int64_t nt = cend[l] - cbeg[l];
-    for (int32_t a = l; a >= 0; a = cparent[a]) {
-      for (int32_t j = 0; j < cnt_sub[a]; ++j) {
-        int32_t s = sub_src[off_sub[a] + j];
-        p2p_src[w++] = s;
-        np += nt * (int64_t)(cend[s] - cbeg[s]);
+  if (c < nc && flags[c]) {
+    task_cell[fscan[c]] = c;
+    int32_t w = toff[c];
+    if (c >= lb) {
+      src[w++] = c;
+      for (int o = 0; o < 27; ++o) {
+        int32_t q = nbr27[(int64_t)c * 27 + o];
+        if (o != 13 && q >= 0) src[w++] = q;
       }
+    }
+    const int64_t nt = cend[c] - cbeg[c];
+    for (int32_t j = 0; j < cnt_sub[c]; ++j) {
+      int32_t s = sub_src[off_sub[c] + j];
+      src[w++] = s;
+      np += nt * (int64_t)(cend[s] - cbeg[s]);
     }
   }
   __shared__ int64_t s[kT];
@@ -497,7 +503,7 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   NB_HIP(hipMemsetAsync(bad, 0, sizeof(int) * 4, s));
   hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(kT), 0, s, d_xyz, d_m, n, red, bad);
   NB_LAUNCH();
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, red, nbb, out6);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kT), 0, s, red, nbb, out6);
   NB_LAUNCH();
   double *h = (double *)dbuf_ensure(c->hred, 0); // pinned host mirror
   NB_HIP(hipMemcpyAsync(h, out6, sizeof(double) * 8 + sizeof(int) * 4, hipMemcpyDeviceToHost, s));
@@ -692,30 +698,29 @@ void build_lists(nbfmm_ctx *c) {
   NB_HIP(hipStreamSynchronize(s));
   P.n_tgt = (int64_t)nst[2] + nst[3];
 
-  // ---- P2P source lists of the leaves ----
-  int32_t *acc_sub = dget<int32_t>(c->acc_sub, (size_t)nc);
-  for (int d = 0; d <= D; ++d) {
-    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_acc_sub, dim3(nblk(ce - cb)), dim3(kT), 0, s, cpar, cnt_sub, cb, ce, acc_sub);
-    NB_LAUNCH();
-  }
-  const int32_t lb = P.lvl_start[D], le = P.lvl_start[D + 1];
-  const int64_t nl = le - lb;
-  int32_t *p2p_cnt = dget<int32_t>(c->p2p_cnt, (size_t)nl + 1);
-  int32_t *p2p_off = dget<int32_t>(c->p2p_off, (size_t)nl + 1);
+  // ---- P2P tasks ----
+  const int32_t lb = P.lvl_start[D];
+  int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)nc + 1);
+  int32_t *toff = dget<int32_t>(c->task_off, (size_t)nc + 1);
   int64_t *pairs = (int64_t *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
   NB_HIP(hipMemsetAsync(pairs, 0, sizeof(int64_t) * 2, s));
-  NB_HIP(hipMemsetAsync(p2p_cnt + nl, 0, sizeof(int32_t), s));
-  hipLaunchKernelGGL(k_p2p_count, dim3(nblk(nl)), dim3(kT), 0, s, nbr27, acc_sub, lb, le, cbeg, cend, p2p_cnt, pairs);
+  NB_HIP(hipMemsetAsync(tcnt + nc, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, nbr27, cnt_sub, cbeg, cend, (int32_t)nc, lb, tcnt,
+                     flags, pairs);
   NB_LAUNCH();
-  scan_excl_i32(p2p_cnt, p2p_off, nl + 1, c->scan_tmp, s);
-  int32_t np2p;
-  NB_HIP(hipMemcpyAsync(&np2p, p2p_off + nl, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  scan_excl_i32(tcnt, toff, nc + 1, c->scan_tmp, s);
+  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
+  int32_t tt[3];
+  NB_HIP(hipMemcpyAsync(&tt[0], toff + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&tt[1], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&tt[2], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
-  P.n_p2p_src = np2p;
-  int32_t *p2p_src = dget<int32_t>(c->p2p_src, (size_t)np2p + 1);
-  hipLaunchKernelGGL(k_p2p_fill, dim3(nblk(nl)), dim3(kT), 0, s, nbr27, cpar, off_sub, cnt_sub, sub_src, lb, le, cbeg,
-                     cend, p2p_off, p2p_src, pairs + 1);
+  P.n_p2p_src = tt[0];
+  P.n_tasks = (int64_t)tt[1] + tt[2];
+  int32_t *task_cell = dget<int32_t>(c->task_cell, (size_t)P.n_tasks + 1);
+  int32_t *p2p_src = dget<int32_t>(c->p2p_src, (size_t)P.n_p2p_src + 1);
+  hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, nbr27, off_sub, cnt_sub, sub_src, cbeg, cend,
+                     (int32_t)nc, lb, tcnt, toff, flags, fscan, task_cell, p2p_src, pairs + 1);
   NB_LAUNCH();
   int64_t pr[2];
   NB_HIP(hipMemcpyAsync(pr, pairs, sizeof(int64_t) * 2, hipMemcpyDeviceToHost, s));

@@ -281,7 +281,13 @@ class DeviceArray:
         if rc != 0:
             raise RuntimeError(f"hipMemcpy H2D failed: {rc}")
 
-    def download(self):
+    def download(self, stream=None):
+        """Copies to host; `stream` (e.g. Context.stream) is synchronised first,
+        since work ordered on a non-blocking stream is not seen by hipMemcpy."""
+        if stream is not None:
+            rc = hip().hipStreamSynchronize(stream)
+            if rc != 0:
+                raise RuntimeError(f"hipStreamSynchronize failed: {rc}")
         out = np.empty(self.shape, self.dtype)
         rc = hip().hipMemcpy(_ptr(out), self.ptr, self.nbytes, 2)
         if rc != 0:
