@@ -1,0 +1,295 @@
+// lists.hip -- interaction lists, their M2L / direct classification and the
+// P2P work list of the gfx950 FMM.
+//
+// Interaction lists follow NodeI::compute_interaction_list (node_i.cpp:23-99):
+// for a cell c the reference scans the 6x6x6 window of its parent's
+// neighbours' children, offsets [-2-(c%2), 3-(c%2)] per axis with x fastest;
+// occupied cells at Chebyshev distance >= 2 form the interaction list, in that
+// order.  The window is the same for the eight children of one parent (child
+// coordinates 2P-2 .. 2P+3 per axis), so one wavefront per parent stages the
+// 216 candidate cells in LDS once, in window order, and classifies them for
+// every child; ballot prefix counts keep the reference order when the entries
+// are written.  A listed pair (t, s) is evaluated by M2L, or directly (P2P)
+// when n_t * n_s < p2p_cut.
+#include "nbfmm_internal.h"
+
+#include <algorithm>
+
+namespace nbfmm {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kW = 64;
+constexpr int kWin = 216;
+// direct-evaluation work per P2P sub-task (particle pairs)
+constexpr int64_t kTaskPairs = 1 << 17;
+
+template <bool FILL>
+__global__ void __launch_bounds__(kW) k_lists(int32_t p0, const int32_t *__restrict__ nbr27,
+                                              const int32_t *__restrict__ child8, const int32_t *__restrict__ cbeg,
+                                              const int32_t *__restrict__ cend, int64_t cut, int32_t leaf_begin,
+                                              int32_t *cnt_il, int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
+                                              const int32_t *off_il, const int32_t *off_m2l, const int32_t *off_sub,
+                                              int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls, int32_t *sub_src,
+                                              int32_t *is_src, unsigned long long *near_pairs) {
+  __shared__ int32_t cand[kWin];
+  __shared__ int32_t csz[kWin];
+  const int32_t P = p0 + (int32_t)blockIdx.x;
+  const int lane = threadIdx.x;
+  for (int w = lane; w < kWin; w += kW) {
+    const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
+    const int32_t Pn = nbr27[(int64_t)P * 27 + (wz >> 1) * 9 + (wy >> 1) * 3 + (wx >> 1)];
+    const int32_t q = Pn >= 0 ? child8[(int64_t)Pn * 8 + ((wx & 1) | ((wy & 1) << 1) | ((wz & 1) << 2))] : -1;
+    cand[w] = q;
+    csz[w] = q >= 0 ? cend[q] - cbeg[q] : 0;
+  }
+  __syncthreads();
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long np_near = 0;
+  for (int o = 0; o < 8; ++o) {
+    const int32_t s = child8[(int64_t)P * 8 + o];
+    if (s < 0) continue;
+    const int bx = o & 1, by = (o >> 1) & 1, bz = o >> 2;
+    const int64_t nt = cend[s] - cbeg[s];
+    const bool leaf = s >= leaf_begin;
+    int32_t a_il = 0, a_m = 0, a_s = 0;
+    if (FILL) {
+      a_il = il ? off_il[s] : 0;
+      a_m = off_m2l[s];
+      a_s = off_sub[s];
+    }
+    int32_t c_il = 0, c_m = 0, c_s = 0;
+    int64_t sp = 0;
+    for (int base = 0; base < kWin; base += kW) {
+      const int w = base + lane;
+      const bool valid = w < kWin;
+      const int32_t q = valid ? cand[w] : -1;
+      const int64_t ns = valid ? csz[w] : 0;
+      const int dx = w % 6 - 2 - bx, dy = (w / 6) % 6 - 2 - by, dz = w / 36 - 2 - bz;
+      const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+      const bool far = q >= 0 && !near;
+      const bool sub = far && nt * ns < cut;
+      const bool m2l = far && !sub;
+      const uint64_t bf = __ballot(far), bs = __ballot(sub), bm = __ballot(m2l);
+      if (FILL) {
+        if (il && far) il[a_il + c_il + __popcll(bf & lt)] = q;
+        if (sub) sub_src[a_s + c_s + __popcll(bs & lt)] = q;
+        if (m2l) {
+          const int32_t k = a_m + c_m + __popcll(bm & lt);
+          m2l_src[k] = q;
+          m2l_cls[k] = (uint16_t)((dz + 3) * 49 + (dy + 3) * 7 + (dx + 3));
+          is_src[q] = 1;
+        }
+      } else {
+        if (sub) sp += nt * ns;
+        if (leaf && near && q >= 0) np_near += (unsigned long long)(q == s ? nt * (nt - 1) : nt * ns);
+      }
+      c_il += __popcll(bf);
+      c_s += __popcll(bs);
+      c_m += __popcll(bm);
+    }
+    if (!FILL) {
+      for (int off = 32; off >= 1; off >>= 1) sp += __shfl_xor(sp, off);
+      if (lane == 0) {
+        cnt_il[s] = c_il;
+        cnt_m2l[s] = c_m;
+        cnt_sub[s] = c_s;
+        sub_pairs[s] = sp;
+      }
+    }
+  }
+  if (!FILL) {
+    for (int off = 32; off >= 1; off >>= 1) np_near += __shfl_xor(np_near, off);
+    if (lane == 0 && np_near) atomicAdd(near_pairs, np_near);
+  }
+}
+
+__global__ void k_flag_pos(const int32_t *v, int64_t n, int32_t *flags) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) flags[i] = v[i] > 0 ? 1 : 0;
+}
+__global__ void k_compact(const int32_t *flags, const int32_t *scan, int64_t n, int32_t *slot, int32_t *list) {
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i]) {
+    slot[i] = scan[i];
+    list[scan[i]] = (int32_t)i;
+  } else {
+    slot[i] = -1;
+  }
+}
+__global__ void k_sum_i64(const int64_t *v, int64_t n, unsigned long long *out) {
+  __shared__ unsigned long long s[kT];
+  unsigned long long a = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) a += (unsigned long long)v[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
+// P2P sub-tasks.  Every leaf (near field: itself and its neighbour leaves)
+// and every cell with directly-evaluated interaction-list cells takes part;
+// its targets are cut into groups of kTaskTargets and its directly-evaluated
+// source list into segments of about kTaskPairs pair interactions.
+__device__ __forceinline__ void task_shape(int32_t c, int32_t leaf_begin, const int32_t *cbeg, const int32_t *cend,
+                                           const int32_t *cnt_sub, const int64_t *sub_pairs, int32_t &ntg,
+                                           int32_t &nseg_sub, int32_t &seglen) {
+  const int32_t nc = cend[c] - cbeg[c];
+  ntg = (nc + kTaskTargets - 1) / kTaskTargets;
+  const int32_t ns = cnt_sub[c];
+  if (ns > 0) {
+    int64_t k = (sub_pairs[c] / ntg + kTaskPairs - 1) / kTaskPairs;
+    if (k < 1) k = 1;
+    if (k > ns) k = ns;
+    seglen = (int32_t)((ns + k - 1) / k);
+    nseg_sub = (ns + seglen - 1) / seglen;
+  } else {
+    nseg_sub = 0;
+    seglen = 0;
+  }
+  (void)leaf_begin;
+}
+
+__global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cbeg, const int32_t *cend,
+                             const int32_t *cnt_sub, const int64_t *sub_pairs, int32_t *cnt) {
+  const int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= nc) return;
+  int32_t ntg, nseg, seglen;
+  task_shape(c, leaf_begin, cbeg, cend, cnt_sub, sub_pairs, ntg, nseg, seglen);
+  cnt[c] = ntg * (nseg + (c >= leaf_begin ? 1 : 0));
+}
+
+// task = {cell, first target offset, first sub entry, end sub entry}; a
+// near-field task has entries (-1, -1).
+__global__ void k_task_fill(int32_t nc, int32_t leaf_begin, const int32_t *cbeg, const int32_t *cend,
+                            const int32_t *cnt_sub, const int32_t *off_sub, const int64_t *sub_pairs,
+                            const int32_t *toff, int4 *tasks) {
+  const int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= nc) return;
+  int32_t ntg, nseg, seglen;
+  task_shape(c, leaf_begin, cbeg, cend, cnt_sub, sub_pairs, ntg, nseg, seglen);
+  int32_t w = toff[c];
+  const int32_t e0 = off_sub[c], e1 = e0 + cnt_sub[c];
+  for (int32_t g = 0; g < ntg; ++g) {
+    if (c >= leaf_begin) tasks[w++] = make_int4(c, g * kTaskTargets, -1, -1);
+    for (int32_t k = 0; k < nseg; ++k) {
+      const int32_t a = e0 + k * seglen;
+      tasks[w++] = make_int4(c, g * kTaskTargets, a, min(a + seglen, e1));
+    }
+  }
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
+
+} // namespace
+
+void build_lists(nbfmm_ctx *c) {
+  Plan &P = c->plan;
+  hipStream_t s = c->stream;
+  const int D = P.geo.depth;
+  const int64_t nc = P.n_cells;
+  const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
+  const int32_t *child8 = (const int32_t *)c->child8.p, *nbr27 = (const int32_t *)c->nbr27.p;
+  const int64_t cut = c->prm.p2p_cut;
+  const int32_t leaf_begin = P.lvl_start[D];
+  int32_t *cnt_il = dget<int32_t>(c->cnt_il, (size_t)nc + 1);
+  int32_t *cnt_m2l = dget<int32_t>(c->cnt_m2l, (size_t)nc + 1);
+  int32_t *cnt_sub = dget<int32_t>(c->cnt_sub, (size_t)nc + 1);
+  int32_t *off_il = dget<int32_t>(c->off_il, (size_t)nc + 1);
+  int32_t *off_m2l = dget<int32_t>(c->off_m2l, (size_t)nc + 1);
+  int32_t *off_sub = dget<int32_t>(c->off_sub, (size_t)nc + 1);
+  int64_t *sub_pairs = dget<int64_t>(c->sub_pairs, (size_t)nc + 1);
+  unsigned long long *pairs = (unsigned long long *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
+  NB_HIP(hipMemsetAsync(pairs, 0, sizeof(unsigned long long) * 2, s));
+  // cells without a parent pass (the root) have empty lists
+  NB_HIP(hipMemsetAsync(cnt_il, 0, sizeof(int32_t) * 2, s));
+  NB_HIP(hipMemsetAsync(cnt_m2l, 0, sizeof(int32_t) * 2, s));
+  NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * 2, s));
+  NB_HIP(hipMemsetAsync(sub_pairs, 0, sizeof(int64_t) * 2, s));
+  NB_HIP(hipMemsetAsync(cnt_il + nc, 0, sizeof(int32_t), s));
+  NB_HIP(hipMemsetAsync(cnt_m2l + nc, 0, sizeof(int32_t), s));
+  NB_HIP(hipMemsetAsync(cnt_sub + nc, 0, sizeof(int32_t), s));
+  const int32_t nparents = leaf_begin; // every non-leaf cell is a parent
+  hipLaunchKernelGGL(k_lists<false>, dim3((unsigned)nparents), dim3(kW), 0, s, 0, nbr27, child8, cbeg, cend, cut,
+                     leaf_begin, cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, pairs);
+  NB_LAUNCH();
+  scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
+  scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
+  scan_excl_i32(cnt_sub, off_sub, nc + 1, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nc))), dim3(kT), 0, s, (const int64_t *)sub_pairs, nc,
+                     pairs + 1);
+  NB_LAUNCH();
+  int32_t tot[3];
+  unsigned long long pr[2];
+  NB_HIP(hipMemcpyAsync(&tot[0], off_il + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&tot[1], off_m2l + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&tot[2], off_sub + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(pr, pairs, sizeof(pr), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_il = tot[0];
+  P.n_m2l = tot[1];
+  P.n_sub = tot[2];
+  P.n_p2p_pairs = (int64_t)(pr[0] + pr[1]);
+  int32_t *il = c->keep_lists ? dget<int32_t>(c->il, (size_t)P.n_il + 1) : nullptr;
+  int32_t *m2l_src = dget<int32_t>(c->m2l_src, (size_t)P.n_m2l + 1);
+  uint16_t *m2l_cls = dget<uint16_t>(c->m2l_cls, (size_t)P.n_m2l + 1);
+  int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
+  int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
+  NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  hipLaunchKernelGGL(k_lists<true>, dim3((unsigned)nparents), dim3(kW), 0, s, 0, nbr27, child8, cbeg, cend, cut,
+                     leaf_begin, nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il,
+                     m2l_src, m2l_cls, sub_src, is_src, nullptr);
+  NB_LAUNCH();
+
+  // ---- M2L source / target compaction ----
+  const size_t fl = (size_t)std::max<int64_t>(nc, P.n) + 1;
+  int32_t *flags = dget<int32_t>(c->flags, fl);
+  int32_t *fscan = dget<int32_t>(c->fscan, fl);
+  int32_t *src_slot = dget<int32_t>(c->src_slot, (size_t)nc);
+  int32_t *tgt_slot = dget<int32_t>(c->tgt_slot, (size_t)nc);
+  int32_t *src_cells = dget<int32_t>(c->src_cells, (size_t)nc);
+  int32_t *tgt_cells = dget<int32_t>(c->tgt_cells, (size_t)nc);
+  int32_t nst[4];
+  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, is_src, nc, flags);
+  NB_LAUNCH();
+  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, src_slot, src_cells);
+  NB_LAUNCH();
+  NB_HIP(hipMemcpyAsync(&nst[0], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&nst[1], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_src = (int64_t)nst[0] + nst[1];
+  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, cnt_m2l, nc, flags);
+  NB_LAUNCH();
+  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, tgt_slot, tgt_cells);
+  NB_LAUNCH();
+  NB_HIP(hipMemcpyAsync(&nst[2], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&nst[3], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_tgt = (int64_t)nst[2] + nst[3];
+
+  // ---- P2P sub-tasks ----
+  int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)nc + 1);
+  int32_t *toff = dget<int32_t>(c->task_off, (size_t)nc + 1);
+  NB_HIP(hipMemsetAsync(tcnt + nc, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cbeg, cend,
+                     (const int32_t *)cnt_sub, (const int64_t *)sub_pairs, tcnt);
+  NB_LAUNCH();
+  scan_excl_i32(tcnt, toff, nc + 1, c->scan_tmp, s);
+  int32_t ntask;
+  NB_HIP(hipMemcpyAsync(&ntask, toff + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.n_tasks = ntask;
+  int4 *tasks = dget<int4>(c->tasks, (size_t)ntask + 1);
+  hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cbeg, cend,
+                     (const int32_t *)cnt_sub, (const int32_t *)off_sub, (const int64_t *)sub_pairs, toff, tasks);
+  NB_LAUNCH();
+}
+
+} // namespace nbfmm

@@ -42,6 +42,7 @@ inline T *dget(DBuf &b, size_t count) {
 constexpr int kMaxDepth = 16;   // ((64/3)>>3)<<3, tree.cpp:38
 constexpr int kNumClasses = 343; // offsets in [-3,3]^3, class = (dz+3)*49+(dy+3)*7+(dx+3)
 constexpr int kMaxOrder = 16;
+constexpr int kTaskTargets = 256; // targets of one P2P sub-task (one wavefront, 4 per lane)
 
 struct Geometry {
   double lo[3], hi[3];
@@ -72,7 +73,7 @@ struct Plan {
   int iterations = 0;
   std::vector<int32_t> lvl_start; // depth+2, cells concatenated from level 0
   int64_t n_cells = 0, n_leaves = 0;
-  int64_t n_il = 0, n_m2l = 0, n_sub = 0, n_p2p_src = 0;
+  int64_t n_il = 0, n_m2l = 0, n_sub = 0;
   int64_t n_src = 0, n_tgt = 0, n_p2p_pairs = 0, n_p2m_terms = 0, n_l2p_terms = 0;
   int64_t n_p2m_items = 0, n_m2l_tiles = 0, n_tasks = 0;
   int kp = 0; // padded coefficient count (multiple of 16)
@@ -102,8 +103,8 @@ struct nbfmm_ctx {
   nbfmm::DBuf cnt_il, cnt_m2l, cnt_sub, off_il, off_m2l, off_sub, acc_sub;
   nbfmm::DBuf il, m2l_src, m2l_cls, sub_src;
   nbfmm::DBuf src_slot, tgt_slot, src_cells, tgt_cells, is_src;
-  nbfmm::DBuf p2p_src, p2m_items, p2m_off, tiles;
-  nbfmm::DBuf task_cell, task_off, task_cnt, dpart;
+  nbfmm::DBuf sub_pairs, p2m_items, p2m_off, tiles;
+  nbfmm::DBuf tasks, task_off, task_cnt, dpart;
   // expansions
   nbfmm::DBuf stab, tmat, mexp, lexp, consts;
   // integrator scratch
@@ -120,6 +121,7 @@ void scan_excl_i32(const int32_t *in, int32_t *out, int64_t n, DBuf &tmp, hipStr
 void scan_excl_i64(const int64_t *in, int64_t *out, int64_t n, DBuf &tmp, hipStream_t s);
 // tree.hip
 void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m);
+// lists.hip
 void build_lists(nbfmm_ctx *c);
 // expansions.hip
 void run_expansions(nbfmm_ctx *c);

@@ -5,17 +5,17 @@
 // own leaf (j != i) and of its neighbour leaves, here also from the
 // interaction-list cells whose pair (t, s) is evaluated directly.
 //
-// Work is organised as cell tasks: every leaf (sources: itself, its
-// neighbour leaves and its directly-evaluated interaction-list cells) and
-// every coarser cell with directly-evaluated interaction-list cells (sources:
-// those cells).  All particles of a task's cell share its sources, so one
-// wavefront stages each source chunk in LDS once and every target lane
-// reads it (broadcast).  Small cells split the 64 lanes as (target, source
-// slot) and combine the slots with xor-shuffles; large cells hold up to four
-// targets per lane.  Tasks at different levels add into the same targets, so
-// the partial sums are added with FP64 atomics to the far-field gradient of
-// the L2P pass; k_finish writes acc = G * grad in input order
-// (particle.cpp:16-21).
+// Work is organised as sub-tasks {cell, first target, source list}: a
+// leaf's near field (its row of the 27-neighbour table, itself included)
+// and segments of a cell's directly-evaluated interaction-list cells, each
+// for at most kTaskTargets targets.  All targets of a task share its
+// sources, so one wavefront stages each source chunk in LDS once and every
+// target lane reads it (broadcast).  Small cells split the 64 lanes as
+// (target, source slot) and combine the slots with xor-shuffles; larger
+// ones hold up to four targets per lane.  Tasks of one cell and of cells at
+// different levels add into the same targets, so the partial sums are added
+// with FP64 atomics to the far-field gradient of the L2P pass; k_finish
+// writes acc = G * grad in input order (particle.cpp:16-21).
 #include "nbfmm_internal.h"
 
 namespace nbfmm {
@@ -23,6 +23,7 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kW = 64; // one wavefront per task
+static_assert(kTaskTargets == 4 * kW, "a P2P task holds at most four targets per lane");
 
 __device__ __forceinline__ void pair_acc(const double4 &xi, const double4 &xj, bool same, double &ax, double &ay,
                                          double &az) {
@@ -37,17 +38,24 @@ __device__ __forceinline__ void pair_acc(const double4 &xi, const double4 &xj, b
   }
 }
 
-__global__ void __launch_bounds__(kW) k_p2p_tasks(const int32_t *__restrict__ task_cell,
-                                                  const int32_t *__restrict__ task_off,
-                                                  const int32_t *__restrict__ task_cnt,
-                                                  const int32_t *__restrict__ task_src,
+__global__ void __launch_bounds__(kW) k_p2p_tasks(const int4 *__restrict__ tasks, const int32_t *__restrict__ nbr27,
+                                                  const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad) {
   __shared__ double4 sh[kW];
   const int lane = threadIdx.x;
-  const int32_t c = task_cell[blockIdx.x];
-  const int32_t b = cbeg[c], nc = cend[c] - b;
-  const int32_t e0 = task_off[c], e1 = e0 + task_cnt[c];
+  const int4 tk = tasks[blockIdx.x];
+  const int32_t c = tk.x;
+  const int32_t b = cbeg[c] + tk.y, nc = min(cend[c] - b, kTaskTargets);
+  const int32_t *list;
+  int32_t ne;
+  if (tk.z < 0) {
+    list = nbr27 + (int64_t)c * 27;
+    ne = 27;
+  } else {
+    list = sub_src + tk.z;
+    ne = tk.w - tk.z;
+  }
   if (nc <= kW) {
     int np2 = 1;
     while (np2 < nc) np2 <<= 1;
@@ -59,14 +67,16 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const int32_t *__restrict__ ta
     double ax = 0, ay = 0, az = 0;
     if (sub >= 8) {
       // few targets: each lane streams its own sources (little to share)
-      for (int32_t e = e0; e < e1; ++e) {
-        const int32_t s = task_src[e];
+      for (int32_t e = 0; e < ne; ++e) {
+        const int32_t s = list[e];
+        if (s < 0) continue;
         const int32_t je = cend[s];
         for (int32_t j = cbeg[s] + si; j < je; j += sub) pair_acc(xi, pos[j], j == i, ax, ay, az);
       }
     } else {
-      for (int32_t e = e0; e < e1; ++e) {
-        const int32_t s = task_src[e];
+      for (int32_t e = 0; e < ne; ++e) {
+        const int32_t s = list[e];
+        if (s < 0) continue;
         const int32_t sb = cbeg[s], se = cend[s];
         for (int32_t base = sb; base < se; base += kW) {
           const int cnt = min(kW, se - base);
@@ -89,40 +99,39 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const int32_t *__restrict__ ta
     }
     return;
   }
-  // large cells: up to 4 targets per lane, every lane reads the same staged source
-  for (int32_t g0 = 0; g0 < nc; g0 += 4 * kW) {
-    double4 xi[4];
-    double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
-    int32_t ii[4];
+  // up to 4 targets per lane, every lane reads the same staged source
+  double4 xi[4];
+  double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
+  int32_t ii[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int32_t t = g0 + lane + q * kW;
-      ii[q] = t < nc ? b + t : -1;
-      xi[q] = pos[t < nc ? b + t : b];
-    }
-    for (int32_t e = e0; e < e1; ++e) {
-      const int32_t s = task_src[e];
-      const int32_t sb = cbeg[s], se = cend[s];
-      for (int32_t base = sb; base < se; base += kW) {
-        const int cnt = min(kW, se - base);
-        __syncthreads();
-        if (lane < cnt) sh[lane] = pos[base + lane];
-        __syncthreads();
-        for (int k = 0; k < cnt; ++k) {
-          const double4 xj = sh[k];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) pair_acc(xi[q], xj, base + k == ii[q], ax[q], ay[q], az[q]);
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (ii[q] >= 0) {
-        atomicAdd(&grad[3 * (int64_t)ii[q]], ax[q]);
-        atomicAdd(&grad[3 * (int64_t)ii[q] + 1], ay[q]);
-        atomicAdd(&grad[3 * (int64_t)ii[q] + 2], az[q]);
-      }
+  for (int q = 0; q < 4; ++q) {
+    const int32_t t = lane + q * kW;
+    ii[q] = t < nc ? b + t : -1;
+    xi[q] = pos[t < nc ? b + t : b];
   }
+  for (int32_t e = 0; e < ne; ++e) {
+    const int32_t s = list[e];
+    if (s < 0) continue;
+    const int32_t sb = cbeg[s], se = cend[s];
+    for (int32_t base = sb; base < se; base += kW) {
+      const int cnt = min(kW, se - base);
+      __syncthreads();
+      if (lane < cnt) sh[lane] = pos[base + lane];
+      __syncthreads();
+      for (int k = 0; k < cnt; ++k) {
+        const double4 xj = sh[k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pair_acc(xi[q], xj, base + k == ii[q], ax[q], ay[q], az[q]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (ii[q] >= 0) {
+      atomicAdd(&grad[3 * (int64_t)ii[q]], ax[q]);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 1], ay[q]);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 2], az[q]);
+    }
 }
 
 __global__ void k_finish(int64_t n, const double *__restrict__ grad, const int32_t *__restrict__ idx, double G,
@@ -244,9 +253,8 @@ void run_p2p(nbfmm_ctx *c, double *d_acc_out) {
   Plan &P = c->plan;
   mark(c, 9);
   if (P.n_tasks > 0) {
-    hipLaunchKernelGGL(k_p2p_tasks, dim3((unsigned)P.n_tasks), dim3(kW), 0, c->stream, (const int32_t *)c->task_cell.p,
-                       (const int32_t *)c->task_off.p, (const int32_t *)c->task_cnt.p, (const int32_t *)c->p2p_src.p,
-                       (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const double4 *)c->pos.p,
+    hipLaunchKernelGGL(k_p2p_tasks, dim3((unsigned)P.n_tasks), dim3(kW), 0, c->stream, (const int4 *)c->tasks.p,
+                       (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const double4 *)c->pos.p,
                        (double *)c->grad.p);
     NB_LAUNCH();
   }

@@ -306,159 +306,6 @@ __global__ void k_nbr27(const uint64_t *__restrict__ ckey, const int32_t *__rest
   }
 }
 
-// ---- interaction lists (node_i.cpp:23-99) -----------------------------------------
-// Window offsets d in [-2-(c%2), 3-(c%2)] per axis, x fastest; cells at
-// Chebyshev distance >= 2 form the interaction list.  A listed pair (t, s)
-// is evaluated by M2L, or by direct P2P when n_t * n_s < cut.
-template <class F>
-__device__ __forceinline__ void for_each_il(int32_t c, const uint64_t *ckey, const int32_t *cparent,
-                                            const int32_t *nbr27, const int32_t *child8, int lev, F f) {
-  uint64_t k = ckey[c];
-  int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
-  int side = 1 << lev;
-  int sx = -2 - (x & 1), sy = -2 - (y & 1), sz = -2 - (z & 1);
-  int32_t P = cparent[c];
-  for (int wz = 0; wz < 6; ++wz)
-    for (int wy = 0; wy < 6; ++wy)
-      for (int wx = 0; wx < 6; ++wx) {
-        int dx = sx + wx, dy = sy + wy, dz = sz + wz;
-        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
-        int qx = x + dx, qy = y + dy, qz = z + dz;
-        if (qx < 0 || qy < 0 || qz < 0 || qx >= side || qy >= side || qz >= side) continue;
-        int px = (qx >> 1) - (x >> 1) + 1, py = (qy >> 1) - (y >> 1) + 1, pz = (qz >> 1) - (z >> 1) + 1;
-        int32_t Pn = nbr27[(int64_t)P * 27 + pz * 9 + py * 3 + px];
-        if (Pn < 0) continue;
-        int32_t s = child8[(int64_t)Pn * 8 + ((qx & 1) | ((qy & 1) << 1) | ((qz & 1) << 2))];
-        if (s < 0) continue;
-        f(s, (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3));
-      }
-}
-
-__global__ void k_il_count(const uint64_t *ckey, const int32_t *cparent, const int32_t *nbr27,
-                           const int32_t *child8, const int32_t *cbeg, const int32_t *cend, int32_t cb,
-                           int32_t ce, int lev, int64_t cut, int32_t *cnt_il, int32_t *cnt_m2l,
-                           int32_t *cnt_sub) {
-  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
-  int64_t nt = cend[c] - cbeg[c];
-  int32_t a = 0, m = 0, s = 0;
-  for_each_il(c, ckey, cparent, nbr27, child8, lev, [&](int32_t src, int) {
-    ++a;
-    if (nt * (int64_t)(cend[src] - cbeg[src]) < cut)
-      ++s;
-    else
-      ++m;
-  });
-  cnt_il[c] = a;
-  cnt_m2l[c] = m;
-  cnt_sub[c] = s;
-}
-
-__global__ void k_il_fill(const uint64_t *ckey, const int32_t *cparent, const int32_t *nbr27,
-                          const int32_t *child8, const int32_t *cbeg, const int32_t *cend, int32_t cb,
-                          int32_t ce, int lev, int64_t cut, const int32_t *off_il, const int32_t *off_m2l,
-                          const int32_t *off_sub, int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls,
-                          int32_t *sub_src, int32_t *is_src) {
-  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
-  int64_t nt = cend[c] - cbeg[c];
-  int32_t a = off_il ? off_il[c] : 0, m = off_m2l[c], s = off_sub[c];
-  for_each_il(c, ckey, cparent, nbr27, child8, lev, [&](int32_t src, int cls) {
-    if (il) il[a++] = src;
-    if (nt * (int64_t)(cend[src] - cbeg[src]) < cut) {
-      sub_src[s++] = src;
-    } else {
-      m2l_src[m] = src;
-      m2l_cls[m] = (uint16_t)cls;
-      ++m;
-      is_src[src] = 1;
-    }
-  });
-}
-
-// P2P tasks: every leaf (sources: itself, its neighbour leaves in node_i.cpp
-// order, its directly-evaluated interaction-list cells) and every coarser
-// cell with directly-evaluated interaction-list cells.
-__global__ void k_task_count(const int32_t *nbr27, const int32_t *cnt_sub, const int32_t *cbeg, const int32_t *cend,
-                             int32_t nc, int32_t lb, int32_t *cnt, int32_t *flags, int64_t *pairs) {
-  int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
-  int64_t np = 0;
-  if (c < nc) {
-    int32_t k = 0;
-    const int64_t nt = cend[c] - cbeg[c];
-    if (c >= lb) {
-      k = 1;
-      np = nt * (nt - 1);
-      for (int o = 0; o < 27; ++o) {
-        int32_t q = nbr27[(int64_t)c * 27 + o];
-        if (o != 13 && q >= 0) {
-          ++k;
-          np += nt * (int64_t)(cend[q] - cbeg[q]);
-        }
-      }
-    }
-    k += cnt_sub[c];
-    cnt[c] = k;
-    flags[c] = k > 0 ? 1 : 0;
-  }
-  __shared__ int64_t s[kT];
-  s[threadIdx.x] = np;
-  __syncthreads();
-  for (int o = kT / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd((unsigned long long *)pairs, (unsigned long long)s[0]);
-}
-
-__global__ void k_task_fill(const int32_t *nbr27, const int32_t *off_sub, const int32_t *cnt_sub, const int32_t *sub_src,
-                            const int32_t *cbeg, const int32_t *cend, int32_t nc, int32_t lb, const int32_t *cnt,
-                            const int32_t *toff, const int32_t *flags, const int32_t *fscan, int32_t *task_cell,
-                            int32_t *src, int64_t *pairs) {
-  int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
-  int64_t np = 0;
-  if (c < nc && flags[c]) {
-    task_cell[fscan[c]] = c;
-    int32_t w = toff[c];
-    if (c >= lb) {
-      src[w++] = c;
-      for (int o = 0; o < 27; ++o) {
-        int32_t q = nbr27[(int64_t)c * 27 + o];
-        if (o != 13 && q >= 0) src[w++] = q;
-      }
-    }
-    const int64_t nt = cend[c] - cbeg[c];
-    for (int32_t j = 0; j < cnt_sub[c]; ++j) {
-      int32_t s = sub_src[off_sub[c] + j];
-      src[w++] = s;
-      np += nt * (int64_t)(cend[s] - cbeg[s]);
-    }
-  }
-  __shared__ int64_t s[kT];
-  s[threadIdx.x] = np;
-  __syncthreads();
-  for (int o = kT / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd((unsigned long long *)pairs, (unsigned long long)s[0]);
-}
-
-__global__ void k_flag_pos(const int32_t *v, int64_t n, int32_t *flags) {
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < n) flags[i] = v[i] > 0 ? 1 : 0;
-}
-__global__ void k_compact(const int32_t *flags, const int32_t *scan, int64_t n, int32_t *slot, int32_t *list) {
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  if (flags[i]) {
-    slot[i] = scan[i];
-    list[scan[i]] = (int32_t)i;
-  } else {
-    slot[i] = -1;
-  }
-}
-
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
 
 // Sorts (key, idx) by the low `bits` bits of key.  Returns the buffer index
@@ -621,111 +468,6 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
     hipLaunchKernelGGL(k_nbr27, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d, nbr27);
     NB_LAUNCH();
   }
-}
-
-void build_lists(nbfmm_ctx *c) {
-  Plan &P = c->plan;
-  hipStream_t s = c->stream;
-  const int D = P.geo.depth;
-  const int64_t nc = P.n_cells;
-  const uint64_t *ckey = (const uint64_t *)c->ckey.p;
-  const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
-  const int32_t *cpar = (const int32_t *)c->cparent.p;
-  const int32_t *child8 = (const int32_t *)c->child8.p, *nbr27 = (const int32_t *)c->nbr27.p;
-  const int64_t cut = c->prm.p2p_cut;
-  int32_t *cnt_il = dget<int32_t>(c->cnt_il, (size_t)nc + 1);
-  int32_t *cnt_m2l = dget<int32_t>(c->cnt_m2l, (size_t)nc + 1);
-  int32_t *cnt_sub = dget<int32_t>(c->cnt_sub, (size_t)nc + 1);
-  int32_t *off_il = dget<int32_t>(c->off_il, (size_t)nc + 1);
-  int32_t *off_m2l = dget<int32_t>(c->off_m2l, (size_t)nc + 1);
-  int32_t *off_sub = dget<int32_t>(c->off_sub, (size_t)nc + 1);
-  NB_HIP(hipMemsetAsync(cnt_il, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  NB_HIP(hipMemsetAsync(cnt_m2l, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  for (int d = 2; d <= D; ++d) {
-    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_il_count, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cpar, nbr27, child8, cbeg, cend, cb, ce,
-                       d, cut, cnt_il, cnt_m2l, cnt_sub);
-    NB_LAUNCH();
-  }
-  scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
-  scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
-  scan_excl_i32(cnt_sub, off_sub, nc + 1, c->scan_tmp, s);
-  int32_t tot[3];
-  NB_HIP(hipMemcpyAsync(&tot[0], off_il + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&tot[1], off_m2l + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&tot[2], off_sub + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.n_il = tot[0];
-  P.n_m2l = tot[1];
-  P.n_sub = tot[2];
-  int32_t *il = c->keep_lists ? dget<int32_t>(c->il, (size_t)P.n_il + 1) : nullptr;
-  int32_t *m2l_src = dget<int32_t>(c->m2l_src, (size_t)P.n_m2l + 1);
-  uint16_t *m2l_cls = dget<uint16_t>(c->m2l_cls, (size_t)P.n_m2l + 1);
-  int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
-  int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
-  NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  for (int d = 2; d <= D; ++d) {
-    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_il_fill, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cpar, nbr27, child8, cbeg, cend, cb, ce,
-                       d, cut, il ? off_il : nullptr, off_m2l, off_sub, il, m2l_src, m2l_cls, sub_src, is_src);
-    NB_LAUNCH();
-  }
-  // source / target compaction
-  int32_t *flags = dget<int32_t>(c->flags, (size_t)std::max<int64_t>(nc, P.n) + 1);
-  int32_t *fscan = dget<int32_t>(c->fscan, (size_t)std::max<int64_t>(nc, P.n) + 1);
-  int32_t *src_slot = dget<int32_t>(c->src_slot, (size_t)nc);
-  int32_t *tgt_slot = dget<int32_t>(c->tgt_slot, (size_t)nc);
-  int32_t *src_cells = dget<int32_t>(c->src_cells, (size_t)nc);
-  int32_t *tgt_cells = dget<int32_t>(c->tgt_cells, (size_t)nc);
-  int32_t nst[4];
-  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, is_src, nc, flags);
-  NB_LAUNCH();
-  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
-  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, src_slot, src_cells);
-  NB_LAUNCH();
-  NB_HIP(hipMemcpyAsync(&nst[0], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&nst[1], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.n_src = (int64_t)nst[0] + nst[1];
-  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, cnt_m2l, nc, flags);
-  NB_LAUNCH();
-  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
-  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, tgt_slot, tgt_cells);
-  NB_LAUNCH();
-  NB_HIP(hipMemcpyAsync(&nst[2], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&nst[3], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.n_tgt = (int64_t)nst[2] + nst[3];
-
-  // ---- P2P tasks ----
-  const int32_t lb = P.lvl_start[D];
-  int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)nc + 1);
-  int32_t *toff = dget<int32_t>(c->task_off, (size_t)nc + 1);
-  int64_t *pairs = (int64_t *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
-  NB_HIP(hipMemsetAsync(pairs, 0, sizeof(int64_t) * 2, s));
-  NB_HIP(hipMemsetAsync(tcnt + nc, 0, sizeof(int32_t), s));
-  hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, nbr27, cnt_sub, cbeg, cend, (int32_t)nc, lb, tcnt,
-                     flags, pairs);
-  NB_LAUNCH();
-  scan_excl_i32(tcnt, toff, nc + 1, c->scan_tmp, s);
-  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
-  int32_t tt[3];
-  NB_HIP(hipMemcpyAsync(&tt[0], toff + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&tt[1], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&tt[2], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.n_p2p_src = tt[0];
-  P.n_tasks = (int64_t)tt[1] + tt[2];
-  int32_t *task_cell = dget<int32_t>(c->task_cell, (size_t)P.n_tasks + 1);
-  int32_t *p2p_src = dget<int32_t>(c->p2p_src, (size_t)P.n_p2p_src + 1);
-  hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, nbr27, off_sub, cnt_sub, sub_src, cbeg, cend,
-                     (int32_t)nc, lb, tcnt, toff, flags, fscan, task_cell, p2p_src, pairs + 1);
-  NB_LAUNCH();
-  int64_t pr[2];
-  NB_HIP(hipMemcpyAsync(pr, pairs, sizeof(int64_t) * 2, hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.n_p2p_pairs = pr[0] + pr[1];
 }
 
 } // namespace nbfmm
