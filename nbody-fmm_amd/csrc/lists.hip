@@ -131,56 +131,126 @@ __global__ void k_sum_i64(const int64_t *v, int64_t n, unsigned long long *out) 
   if (threadIdx.x == 0) atomicAdd(out, s[0]);
 }
 
-// P2P sub-tasks.  Every leaf (near field: itself and its neighbour leaves)
-// and every cell with directly-evaluated interaction-list cells takes part;
-// its targets are cut into groups of kTaskTargets and its directly-evaluated
-// source list into segments of about kTaskPairs pair interactions.
-__device__ __forceinline__ void task_shape(int32_t c, int32_t leaf_begin, const int32_t *cbeg, const int32_t *cend,
-                                           const int32_t *cnt_sub, const int64_t *sub_pairs, int32_t &ntg,
-                                           int32_t &nseg_sub, int32_t &seglen) {
-  const int32_t nc = cend[c] - cbeg[c];
-  ntg = (nc + kTaskTargets - 1) / kTaskTargets;
-  const int32_t ns = cnt_sub[c];
-  if (ns > 0) {
-    int64_t k = (sub_pairs[c] / ntg + kTaskPairs - 1) / kTaskPairs;
-    if (k < 1) k = 1;
-    if (k > ns) k = ns;
-    seglen = (int32_t)((ns + k - 1) / k);
-    nseg_sub = (ns + seglen - 1) / seglen;
-  } else {
-    nseg_sub = 0;
-    seglen = 0;
-  }
-  (void)leaf_begin;
-}
-
-__global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cbeg, const int32_t *cend,
-                             const int32_t *cnt_sub, const int64_t *sub_pairs, int32_t *cnt) {
-  const int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= nc) return;
-  int32_t ntg, nseg, seglen;
-  task_shape(c, leaf_begin, cbeg, cend, cnt_sub, sub_pairs, ntg, nseg, seglen);
-  cnt[c] = ntg * (nseg + (c >= leaf_begin ? 1 : 0));
-}
-
-// task = {cell, first target offset, first sub entry, end sub entry}; a
-// near-field task has entries (-1, -1).
-__global__ void k_task_fill(int32_t nc, int32_t leaf_begin, const int32_t *cbeg, const int32_t *cend,
-                            const int32_t *cnt_sub, const int32_t *off_sub, const int64_t *sub_pairs,
-                            const int32_t *toff, int4 *tasks) {
-  const int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= nc) return;
-  int32_t ntg, nseg, seglen;
-  task_shape(c, leaf_begin, cbeg, cend, cnt_sub, sub_pairs, ntg, nseg, seglen);
-  int32_t w = toff[c];
-  const int32_t e0 = off_sub[c], e1 = e0 + cnt_sub[c];
-  for (int32_t g = 0; g < ntg; ++g) {
-    if (c >= leaf_begin) tasks[w++] = make_int4(c, g * kTaskTargets, -1, -1);
-    for (int32_t k = 0; k < nseg; ++k) {
-      const int32_t a = e0 + k * seglen;
-      tasks[w++] = make_int4(c, g * kTaskTargets, a, min(a + seglen, e1));
+// ---- P2P sub-tasks ------------------------------------------------------------
+// A cell is closed when its own and all its descendants' interaction-list
+// pairs are evaluated directly (no M2L at or below it).  For the particles
+// of a closed cell A the directly evaluated pairs of the leaves below it
+// (near field plus every substituted interaction list down the tree) are
+// exactly the particles of the 27 neighbours of A's parent, so a maximal
+// closed cell (parent not closed) becomes one dense block of work.  Other
+// leaves get their near field (27-neighbour row) and other cells their
+// substituted list.  Targets are cut into groups of kTaskTargets, the
+// concatenated sources into whole 64-particle chunks of about kTaskPairs
+// pair interactions per task.
+__global__ void k_closed(int32_t cb, int32_t ce, int32_t leaf_begin, const int32_t *__restrict__ cnt_m2l,
+                         const int32_t *__restrict__ child8, int32_t *closed) {
+  const int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  bool ok = cnt_m2l[c] == 0;
+  if (ok && c < leaf_begin)
+    for (int o = 0; o < 8; ++o) {
+      const int32_t ch = child8[(int64_t)c * 8 + o];
+      if (ch >= 0 && !closed[ch]) ok = false;
     }
+  closed[c] = ok ? 1 : 0;
+}
+
+__device__ __forceinline__ int64_t row_particles(const int32_t *row, const int32_t *cbeg, const int32_t *cend) {
+  int64_t t = 0;
+  for (int o = 0; o < 27; ++o) {
+    const int32_t q = row[o];
+    if (q >= 0) t += cend[q] - cbeg[q];
   }
+  return t;
+}
+
+// source groups of cell c: up to two (list, nlist, particles)
+struct SrcGroups {
+  int n;
+  int32_t list[2], nlist[2];
+  int64_t np[2];
+};
+
+__device__ __forceinline__ SrcGroups src_groups(int32_t c, int32_t leaf_begin, const int32_t *cparent,
+                                                const int32_t *nbr27, const int32_t *closed, const int32_t *cbeg,
+                                                const int32_t *cend, const int32_t *cnt_sub, const int32_t *off_sub,
+                                                const int64_t *sub_pairs) {
+  SrcGroups g;
+  g.n = 0;
+  if (closed[c]) {
+    const int32_t P = cparent[c];
+    if (!closed[P]) {
+      g.list[0] = -(P + 1);
+      g.nlist[0] = 27;
+      g.np[0] = row_particles(nbr27 + (int64_t)P * 27, cbeg, cend);
+      g.n = 1;
+    }
+    return g;
+  }
+  if (c >= leaf_begin) {
+    g.list[g.n] = -(c + 1);
+    g.nlist[g.n] = 27;
+    g.np[g.n] = row_particles(nbr27 + (int64_t)c * 27, cbeg, cend);
+    ++g.n;
+  }
+  if (cnt_sub[c] > 0) {
+    g.list[g.n] = off_sub[c];
+    g.nlist[g.n] = cnt_sub[c];
+    g.np[g.n] = sub_pairs[c] / (cend[c] - cbeg[c]);
+    ++g.n;
+  }
+  return g;
+}
+
+__device__ __forceinline__ int32_t seg_len(int64_t ng, int64_t np) {
+  int64_t k = (ng * np + kTaskPairs - 1) / kTaskPairs;
+  if (k < 1) k = 1;
+  int64_t sl = (np + k - 1) / k;
+  sl = (sl + 63) & ~(int64_t)63;
+  return (int32_t)sl;
+}
+
+__global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
+                             const int32_t *closed, const int32_t *cbeg, const int32_t *cend, const int32_t *cnt_sub,
+                             const int32_t *off_sub, const int64_t *sub_pairs, int32_t *cnt) {
+  const int32_t c = 1 + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= nc) return;
+  const SrcGroups g = src_groups(c, leaf_begin, cparent, nbr27, closed, cbeg, cend, cnt_sub, off_sub, sub_pairs);
+  const int32_t nt = cend[c] - cbeg[c];
+  const int32_t ntg = (nt + kTaskTargets - 1) / kTaskTargets;
+  int32_t k = 0;
+  for (int q = 0; q < g.n; ++q) {
+    if (g.np[q] <= 0) continue;
+    const int32_t sl = seg_len(min(nt, kTaskTargets), g.np[q]);
+    k += (int32_t)((g.np[q] + sl - 1) / sl);
+  }
+  cnt[c] = k * ntg;
+}
+
+__global__ void k_task_fill(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
+                            const int32_t *closed, const int32_t *cbeg, const int32_t *cend, const int32_t *cnt_sub,
+                            const int32_t *off_sub, const int64_t *sub_pairs, const int32_t *toff, P2PTask *tasks) {
+  const int32_t c = 1 + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= nc) return;
+  const SrcGroups g = src_groups(c, leaf_begin, cparent, nbr27, closed, cbeg, cend, cnt_sub, off_sub, sub_pairs);
+  const int32_t nt = cend[c] - cbeg[c];
+  const int32_t ntg = (nt + kTaskTargets - 1) / kTaskTargets;
+  int32_t w = toff[c];
+  for (int32_t tg = 0; tg < ntg; ++tg)
+    for (int q = 0; q < g.n; ++q) {
+      if (g.np[q] <= 0) continue;
+      const int64_t sl = seg_len(min(nt, kTaskTargets), g.np[q]);
+      for (int64_t f = 0; f < g.np[q]; f += sl) {
+        P2PTask t;
+        t.tcell = c;
+        t.toff = tg * kTaskTargets;
+        t.list = g.list[q];
+        t.nlist = g.nlist[q];
+        t.f0 = (int32_t)f;
+        t.f1 = (int32_t)min(g.np[q], f + sl);
+        tasks[w++] = t;
+      }
+    }
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
@@ -275,20 +345,32 @@ void build_lists(nbfmm_ctx *c) {
   P.n_tgt = (int64_t)nst[2] + nst[3];
 
   // ---- P2P sub-tasks ----
+  int32_t *closed = dget<int32_t>(c->closed, (size_t)nc + 1);
+  NB_HIP(hipMemsetAsync(closed, 0, sizeof(int32_t), s)); // the root is never a task cell
+  for (int d = D; d >= 1; --d) {
+    const int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
+    hipLaunchKernelGGL(k_closed, dim3(nblk(ce - cb)), dim3(kT), 0, s, cb, ce, leaf_begin, (const int32_t *)cnt_m2l,
+                       child8, closed);
+    NB_LAUNCH();
+  }
+  const int32_t *cpar = (const int32_t *)c->cparent.p;
   int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)nc + 1);
   int32_t *toff = dget<int32_t>(c->task_off, (size_t)nc + 1);
+  NB_HIP(hipMemsetAsync(tcnt, 0, sizeof(int32_t), s));
   NB_HIP(hipMemsetAsync(tcnt + nc, 0, sizeof(int32_t), s));
-  hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cbeg, cend,
-                     (const int32_t *)cnt_sub, (const int64_t *)sub_pairs, tcnt);
+  hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
+                     (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
+                     (const int64_t *)sub_pairs, tcnt);
   NB_LAUNCH();
   scan_excl_i32(tcnt, toff, nc + 1, c->scan_tmp, s);
   int32_t ntask;
   NB_HIP(hipMemcpyAsync(&ntask, toff + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   P.n_tasks = ntask;
-  int4 *tasks = dget<int4>(c->tasks, (size_t)ntask + 1);
-  hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cbeg, cend,
-                     (const int32_t *)cnt_sub, (const int32_t *)off_sub, (const int64_t *)sub_pairs, toff, tasks);
+  P2PTask *tasks = dget<P2PTask>(c->tasks, (size_t)ntask + 1);
+  hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
+                     (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
+                     (const int64_t *)sub_pairs, toff, tasks);
   NB_LAUNCH();
 }
 

@@ -43,6 +43,16 @@ constexpr int kMaxDepth = 16;   // ((64/3)>>3)<<3, tree.cpp:38
 constexpr int kNumClasses = 343; // offsets in [-3,3]^3, class = (dz+3)*49+(dy+3)*7+(dx+3)
 constexpr int kMaxOrder = 16;
 constexpr int kTaskTargets = 256; // targets of one P2P sub-task (one wavefront, 4 per lane)
+constexpr int kMaxList = 192;     // longest source-cell list of a P2P task (interaction list <= 189)
+
+// One P2P sub-task: targets [cbeg[tcell] + toff, + kTaskTargets) of cell
+// tcell against the particles [f0, f1) of the concatenated particle ranges
+// of a list of source cells: list >= 0 names sub_src[list, list + nlist),
+// list < 0 row (-list - 1) of the 27-neighbour table (nlist = 27, empty
+// entries -1).
+struct P2PTask {
+  int32_t tcell, toff, list, nlist, f0, f1;
+};
 
 struct Geometry {
   double lo[3], hi[3];
@@ -99,7 +109,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf red;                             // small reductions (bbox, flags, counts)
   nbfmm::DBuf hred;                            // pinned host mirror of `red`
   // cells
-  nbfmm::DBuf ckey, cbeg, cend, cparent, child8, nbr27, flags, fscan;
+  nbfmm::DBuf ckey, cbeg, cend, cparent, child8, nbr27, flags, fscan, closed;
   nbfmm::DBuf cnt_il, cnt_m2l, cnt_sub, off_il, off_m2l, off_sub, acc_sub;
   nbfmm::DBuf il, m2l_src, m2l_cls, sub_src;
   nbfmm::DBuf src_slot, tgt_slot, src_cells, tgt_cells, is_src;

@@ -5,17 +5,19 @@
 // own leaf (j != i) and of its neighbour leaves, here also from the
 // interaction-list cells whose pair (t, s) is evaluated directly.
 //
-// Work is organised as sub-tasks {cell, first target, source list}: a
-// leaf's near field (its row of the 27-neighbour table, itself included)
-// and segments of a cell's directly-evaluated interaction-list cells, each
-// for at most kTaskTargets targets.  All targets of a task share its
-// sources, so one wavefront stages each source chunk in LDS once and every
-// target lane reads it (broadcast).  Small cells split the 64 lanes as
-// (target, source slot) and combine the slots with xor-shuffles; larger
-// ones hold up to four targets per lane.  Tasks of one cell and of cells at
-// different levels add into the same targets, so the partial sums are added
-// with FP64 atomics to the far-field gradient of the L2P pass; k_finish
-// writes acc = G * grad in input order (particle.cpp:16-21).
+// Work is organised as sub-tasks (P2PTask): up to kTaskTargets targets of
+// one cell against a range of the concatenated particles of a list of
+// source cells (a 27-neighbour row, or a cell's directly evaluated
+// interaction-list cells).  One wavefront runs a task: it prefix-sums the
+// list's cell sizes into LDS, then stages the sources 64 at a time (each
+// lane finds its source's cell by binary search) and every target lane reads
+// the staged chunk (broadcast), so lanes stay busy however small the cells.
+// Up to 64 targets split the lanes as (target, source slot) and combine the
+// slots with xor-shuffles; more targets are held four per lane.  Tasks of
+// one cell and of cells at different levels add into the same targets, so
+// the partial sums are added with FP64 atomics to the far-field gradient of
+// the L2P pass; k_finish writes acc = G * grad in input order
+// (particle.cpp:16-21).
 #include "nbfmm_internal.h"
 
 namespace nbfmm {
@@ -38,24 +40,58 @@ __device__ __forceinline__ void pair_acc(const double4 &xi, const double4 &xj, b
   }
 }
 
-__global__ void __launch_bounds__(kW) k_p2p_tasks(const int4 *__restrict__ tasks, const int32_t *__restrict__ nbr27,
+__global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ tasks, const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad) {
   __shared__ double4 sh[kW];
+  __shared__ int32_t shj[kW];
+  __shared__ int32_t lb[kMaxList];
+  __shared__ int32_t lp[kMaxList];
   const int lane = threadIdx.x;
-  const int4 tk = tasks[blockIdx.x];
-  const int32_t c = tk.x;
-  const int32_t b = cbeg[c] + tk.y, nc = min(cend[c] - b, kTaskTargets);
-  const int32_t *list;
-  int32_t ne;
-  if (tk.z < 0) {
-    list = nbr27 + (int64_t)c * 27;
-    ne = 27;
-  } else {
-    list = sub_src + tk.z;
-    ne = tk.w - tk.z;
+  const P2PTask tk = tasks[blockIdx.x];
+  const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+  const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
+  const int ne = tk.nlist;
+  // first concatenated index of every list cell
+  int32_t carry = 0;
+  for (int base = 0; base < ne; base += kW) {
+    const int e = base + lane;
+    const int32_t q = e < ne ? list[e] : -1;
+    const int32_t sb = q >= 0 ? cbeg[q] : 0;
+    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
+    int32_t x = cnt;
+    for (int off = 1; off < kW; off <<= 1) {
+      const int32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (e < ne) {
+      lb[e] = sb;
+      lp[e] = carry + x - cnt;
+    }
+    carry += __shfl(x, kW - 1);
   }
+  const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
+  // stage sources [fb, fb + 64) of the concatenation; returns their count
+  auto stage = [&](int32_t fb) -> int {
+    const int32_t g = fb + lane;
+    __syncthreads();
+    if (g < f1) {
+      int lo = 0, hi = ne; // last list cell with lp <= g (lp[0] = 0 <= g)
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lp[mid] <= g)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const int32_t j = lb[lo] + (g - lp[lo]);
+      sh[lane] = pos[j];
+      shj[lane] = j;
+    }
+    __syncthreads();
+    return min(kW, f1 - fb);
+  };
   if (nc <= kW) {
     int np2 = 1;
     while (np2 < nc) np2 <<= 1;
@@ -65,27 +101,9 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const int4 *__restrict__ tasks
     const int32_t i = b + (valid ? ti : 0);
     const double4 xi = pos[i];
     double ax = 0, ay = 0, az = 0;
-    if (sub >= 8) {
-      // few targets: each lane streams its own sources (little to share)
-      for (int32_t e = 0; e < ne; ++e) {
-        const int32_t s = list[e];
-        if (s < 0) continue;
-        const int32_t je = cend[s];
-        for (int32_t j = cbeg[s] + si; j < je; j += sub) pair_acc(xi, pos[j], j == i, ax, ay, az);
-      }
-    } else {
-      for (int32_t e = 0; e < ne; ++e) {
-        const int32_t s = list[e];
-        if (s < 0) continue;
-        const int32_t sb = cbeg[s], se = cend[s];
-        for (int32_t base = sb; base < se; base += kW) {
-          const int cnt = min(kW, se - base);
-          __syncthreads();
-          if (lane < cnt) sh[lane] = pos[base + lane];
-          __syncthreads();
-          for (int k = si; k < cnt; k += sub) pair_acc(xi, sh[k], base + k == i, ax, ay, az);
-        }
-      }
+    for (int32_t fb = f0; fb < f1; fb += kW) {
+      const int cnt = stage(fb);
+      for (int k = si; k < cnt; k += sub) pair_acc(xi, sh[k], shj[k] == i, ax, ay, az);
     }
     for (int o = sub >> 1; o >= 1; o >>= 1) {
       ax += __shfl_xor(ax, o);
@@ -109,20 +127,13 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const int4 *__restrict__ tasks
     ii[q] = t < nc ? b + t : -1;
     xi[q] = pos[t < nc ? b + t : b];
   }
-  for (int32_t e = 0; e < ne; ++e) {
-    const int32_t s = list[e];
-    if (s < 0) continue;
-    const int32_t sb = cbeg[s], se = cend[s];
-    for (int32_t base = sb; base < se; base += kW) {
-      const int cnt = min(kW, se - base);
-      __syncthreads();
-      if (lane < cnt) sh[lane] = pos[base + lane];
-      __syncthreads();
-      for (int k = 0; k < cnt; ++k) {
-        const double4 xj = sh[k];
+  for (int32_t fb = f0; fb < f1; fb += kW) {
+    const int cnt = stage(fb);
+    for (int k = 0; k < cnt; ++k) {
+      const double4 xj = sh[k];
+      const int32_t j = shj[k];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) pair_acc(xi[q], xj, base + k == ii[q], ax[q], ay[q], az[q]);
-      }
+      for (int q = 0; q < 4; ++q) pair_acc(xi[q], xj, j == ii[q], ax[q], ay[q], az[q]);
     }
   }
 #pragma unroll
@@ -253,7 +264,7 @@ void run_p2p(nbfmm_ctx *c, double *d_acc_out) {
   Plan &P = c->plan;
   mark(c, 9);
   if (P.n_tasks > 0) {
-    hipLaunchKernelGGL(k_p2p_tasks, dim3((unsigned)P.n_tasks), dim3(kW), 0, c->stream, (const int4 *)c->tasks.p,
+    hipLaunchKernelGGL(k_p2p_tasks, dim3((unsigned)P.n_tasks), dim3(kW), 0, c->stream, (const P2PTask *)c->tasks.p,
                        (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const double4 *)c->pos.p,
                        (double *)c->grad.p);
     NB_LAUNCH();
