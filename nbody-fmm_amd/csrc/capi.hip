@@ -119,9 +119,9 @@ void mark(nbfmm_ctx *c, int slot) {
 nbfmm_ctx::~nbfmm_ctx() {
   DBuf *all[] = {&xyz_in, &m_in, &acc_out, &key[0], &key[1], &idx[0], &idx[1], &sort_hist, &scan_tmp, &pos,
                  &pcell, &grad, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
-                 &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &acc_sub, &il, &m2l_src, &m2l_cls, &sub_src,
+                 &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,
-                 &p2m_off, &tiles, &stab, &tmat, &mexp, &lexp, &consts, &xk, &vk, &ak, &asum, &vsum};
+                 &p2m_off, &tiles, &stab, &tmat, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum};
   for (DBuf *b : all) dbuf_free(*b);
   if (hred.p) (void)hipHostFree(hred.p);
   for (auto &e : ev)
@@ -156,6 +156,7 @@ int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
   c = new nbfmm_ctx();
   c->prm = *params;
   NB_HIP(hipGetDevice(&c->device));
+  NB_HIP(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
   NB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) NB_HIP(hipEventCreate(&e));
   NB_HIP(hipHostMalloc(&c->hred.p, 4096, hipHostMallocDefault));

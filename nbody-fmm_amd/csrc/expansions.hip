@@ -192,7 +192,11 @@ __global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, double rx, doub
 }
 
 // ---- P2M ---------------------------------------------------------------------------
-// Item = (source slot, particle range).  Lanes: 16 particle slots x 16 orders m.
+// Item = (source slot, particle range).  Lane = particle; the orders m are
+// dealt round-robin to the 4 waves of the block, so every wave runs one
+// column m at a time with no divergence: q R_m^m by the sectoral recurrence,
+// then R_n^m, n = m+1..p, by the three-term recurrence in n, accumulated per
+// lane; one xor-shuffle reduction per coefficient at the end of the column.
 __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restrict__ items, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ src_cells, const uint64_t *__restrict__ ckey,
                                             LevelGeo g, const Consts *__restrict__ C, double *__restrict__ Mexp) {
@@ -203,144 +207,184 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
   double ctr[3];
   cell_centre(ckey[cell], lev, g, ctr);
   const double ir = g.inv_rho[lev];
-  const int m = threadIdx.x & 15, ps = threadIdx.x >> 4;
-  double ar[kMaxOrder + 1], ai[kMaxOrder + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double *M = Mexp + (size_t)slot * KP;
+  for (int m = wave; m <= p; m += kT / 64) {
+    double ar[kMaxOrder + 1], ai[kMaxOrder + 1];
 #pragma unroll
-  for (int n = 0; n <= kMaxOrder; ++n) {
-    ar[n] = 0;
-    ai[n] = 0;
-  }
-  if (m <= p) {
-    for (int i = beg + ps; i < end; i += 16) {
+    for (int j = 0; j <= kMaxOrder; ++j) {
+      ar[j] = 0;
+      ai[j] = 0;
+    }
+    const int nj = p - m; // j = n - m in [0, nj]
+    for (int i = beg + lane; i < end; i += 64) {
       const double4 q = pos[i];
       const double x = (q.x - ctr[0]) * ir, y = (q.y - ctr[1]) * ir, z = (q.z - ctr[2]) * ir;
       const double r2 = x * x + y * y + z * z;
       double vr = q.w, vi = 0; // q * R_m^m
       for (int k = 1; k <= m; ++k) {
-        double f = C->mmf[k];
-        double nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
+        const double f = C->mmf[k];
+        const double nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
         vr = nr;
         vi = ni;
       }
-      double pr = 0, pi = 0, cr = vr, ci = vi;
+      ar[0] += vr;
+      ai[0] -= vi;
+      double pr = vr, pi = vi;
+      double cr = 0, ci = 0;
+      if (nj >= 1) {
+        const double f = z * C->s2m1[m];
+        cr = f * pr;
+        ci = f * pi;
+        ar[1] += cr;
+        ai[1] -= ci;
+      }
 #pragma unroll
-      for (int n = 0; n <= kMaxOrder; ++n) {
-        if (n == m) {
-          ar[n] += cr;
-          ai[n] -= ci;
-        } else if (n == m + 1 && n <= p) {
-          pr = cr;
-          pi = ci;
-          double f = z * C->s2m1[m];
-          cr = f * pr;
-          ci = f * pi;
-          ar[n] += cr;
-          ai[n] -= ci;
-        } else if (n > m + 1 && n <= p) {
-          double a = C->ra[n][m] * z, b = C->rb[n][m] * r2;
-          double nr = a * cr - b * pr, ni = a * ci - b * pi;
+      for (int j = 2; j <= kMaxOrder; ++j) {
+        if (j <= nj) {
+          const int n = m + j;
+          const double a = C->ra[n][m] * z, b = C->rb[n][m] * r2;
+          const double nr = a * cr - b * pr, ni = a * ci - b * pi;
           pr = cr;
           pi = ci;
           cr = nr;
           ci = ni;
-          ar[n] += cr;
-          ai[n] -= ci;
+          ar[j] += cr;
+          ai[j] -= ci;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j <= kMaxOrder; ++j) {
+      if (j <= nj) {
+        double vr = ar[j], vi = ai[j];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          vr += __shfl_xor(vr, o);
+          vi += __shfl_xor(vi, o);
+        }
+        if (lane == 0) {
+          const int n = m + j;
+          if (m == 0) {
+            atomicAdd(&M[n * n], vr);
+          } else {
+            atomicAdd(&M[n * n + 2 * m - 1], vr);
+            atomicAdd(&M[n * n + 2 * m], vi);
+          }
         }
       }
     }
   }
-  // reduce the 4 particle slots of each wave (lanes m, m+16, m+32, m+48)
-#pragma unroll
-  for (int n = 0; n <= kMaxOrder; ++n) {
-    ar[n] += __shfl_xor(ar[n], 16);
-    ar[n] += __shfl_xor(ar[n], 32);
-    ai[n] += __shfl_xor(ai[n], 16);
-    ai[n] += __shfl_xor(ai[n], 32);
-  }
-  __shared__ double red[4][16][kMaxOrder + 1][2];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane < 16) {
-#pragma unroll
-    for (int n = 0; n <= kMaxOrder; ++n) {
-      red[wave][lane][n][0] = ar[n];
-      red[wave][lane][n][1] = ai[n];
-    }
-  }
-  __syncthreads();
-  double *M = Mexp + (size_t)slot * KP;
-  for (int e = threadIdx.x; e < (p + 1) * (p + 1); e += kT) {
-    int n = (int)sqrt((double)e);
-    while (n * n > e) --n;
-    while ((n + 1) * (n + 1) <= e) ++n;
-    int rr = e - n * n, mm = (rr + 1) / 2, im = (rr > 0 && (rr & 1) == 0);
-    double v = 0;
-    for (int w = 0; w < 4; ++w) v += red[w][mm][n][im];
-    atomicAdd(&M[e], v);
-  }
 }
 
 // ---- M2L on MFMA --------------------------------------------------------------------
-// One wave = 16 target cells (columns); for every offset class present in
-// its targets' lists, B = the 16 source multipoles (zero where a target has
-// no source of that class) and acc[rb] += T_class[rb] * B.
+// Pairs are sorted by offset class.  A workgroup takes one class and a range
+// of its pairs: it stages the class operator (row blocks [r0, r0 + RBW)) in
+// LDS once, then each wave runs 16-pair tiles, B = the 16 source multipoles,
+// acc[r] += T[r] * B with v_mfma_f64_16x16x4f64, every MFMA column filled.
+// The partial local of every pair goes to Y[pair] (pairs numbered in the
+// target-major list order) and k_m2l_sum adds the partials of each target
+// in list order, so the result does not depend on the schedule.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int kM2LThreads = 512;
+constexpr int kClsChunk = 4096;
 
-template <int KP>
-__global__ void __launch_bounds__(kT) k_m2l_mfma(int p, int ntiles, const int32_t *__restrict__ tile_tgt, int32_t ntgt,
-                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
-                                                 const int32_t *__restrict__ m2l_src,
-                                                 const uint16_t *__restrict__ m2l_cls,
-                                                 const int32_t *__restrict__ src_slot,
-                                                 const int32_t *__restrict__ tgt_slot, const double *__restrict__ frag,
-                                                 const double *__restrict__ Mexp, double *__restrict__ Lexp) {
+__global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cls, int64_t n, int32_t *cnt) {
+  __shared__ int32_t h[kNumClasses];
+  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
+  for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumClasses; k += kT)
+    if (h[k]) atomicAdd(&cnt[k], h[k]);
+}
+
+// cursor[k] starts at the first position of class k; the order of the pairs
+// inside a class is free (every pair's result is computed independently).
+__global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, int64_t n, int32_t *cursor,
+                                                    int32_t *__restrict__ out) {
+  __shared__ int32_t h[kNumClasses];
+  __shared__ int32_t base[kNumClasses];
+  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
+  for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumClasses; k += kT) {
+    base[k] = h[k] ? atomicAdd(&cursor[k], h[k]) : 0;
+    h[k] = 0;
+  }
+  __syncthreads();
+  for (int64_t i = b + threadIdx.x; i < e; i += kT) {
+    const int k = cls[i];
+    out[base[k] + atomicAdd(&h[k], 1)] = (int32_t)i;
+  }
+}
+
+template <int KP, int RBW>
+__global__ void __launch_bounds__(kM2LThreads) k_m2l_cls(const int4 *__restrict__ items,
+                                                         const int32_t *__restrict__ cls_pairs,
+                                                         const int32_t *__restrict__ m2l_src,
+                                                         const int32_t *__restrict__ src_slot,
+                                                         const double *__restrict__ frag,
+                                                         const double *__restrict__ Mexp, double *__restrict__ Y) {
   constexpr int RB = KP / 16, KS = KP / 4;
+  extern __shared__ double sA[];
+  const int4 it = items[blockIdx.x];
+  const int r0 = (int)blockIdx.y * RBW;
+  const int nrb = min(RBW, RB - r0);
+  {
+    const double2 *A = (const double2 *)(frag + ((size_t)it.x * RB + r0) * KS * 64);
+    const int n2 = nrb * KS * 32;
+    for (int e = threadIdx.x; e < n2; e += kM2LThreads) ((double2 *)sA)[e] = A[e];
+  }
+  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * (kT / 64) + wave;
-  if (tile >= ntiles) return;
   const int col = lane & 15, kg = lane >> 4;
-  const int ti = tile * 16 + col;
-  const int32_t tc = ti < ntgt ? tile_tgt[ti] : -1;
-  int cur = tc >= 0 ? off_m2l[tc] : 0;
-  const int endc = tc >= 0 ? cur + cnt_m2l[tc] : 0;
-  dbl4 acc[RB];
+  for (int pb = it.y + wave * 16; pb < it.z; pb += (kM2LThreads / 64) * 16) {
+    const int pi = pb + col;
+    const int32_t pid = pi < it.z ? cls_pairs[pi] : -1;
+    const double *Mb = Mexp + (size_t)(pid >= 0 ? src_slot[m2l_src[pid]] : 0) * KP + kg;
+    double bv[KS];
 #pragma unroll
-  for (int r = 0; r < RB; ++r) acc[r] = dbl4{0, 0, 0, 0};
-  int next = cur < endc ? (int)m2l_cls[cur] : 1 << 20;
-  while (true) {
-    // smallest pending class over the wave
-    int cls = next;
+    for (int ks = 0; ks < KS; ++ks) bv[ks] = pid >= 0 ? Mb[ks * 4] : 0.0;
+    dbl4 acc[RBW];
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) cls = min(cls, __shfl_xor(cls, o));
-    if (cls >= kNumClasses) break;
-    const bool has = next == cls;
-    const double *Mb = nullptr;
-    if (has) {
-      Mb = Mexp + (size_t)src_slot[m2l_src[cur]] * KP;
-      ++cur;
-      next = cur < endc ? (int)m2l_cls[cur] : 1 << 20;
-    }
-    const double *A = frag + (size_t)cls * KP * KP + lane;
-    for (int ks = 0; ks < KS; ++ks) {
-      const double b = has ? Mb[ks * 4 + kg] : 0.0;
+    for (int r = 0; r < RBW; ++r) acc[r] = dbl4{0, 0, 0, 0};
 #pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        const double a = A[(size_t)(r * KS + ks) * 64];
-        acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[r], 0, 0, 0);
-      }
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int r = 0; r < RBW; ++r)
+        if (r < nrb) acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[(r * KS + ks) * 64 + lane], bv[ks], acc[r], 0, 0, 0);
+    if (pid >= 0) {
+      // v_mfma_f64_16x16x4f64 result layout: lane l, register q holds
+      // D[l/16 + 4q][l%16] (checked on gfx950).
+      double *y = Y + (size_t)pid * KP + r0 * 16 + kg;
+#pragma unroll
+      for (int r = 0; r < RBW; ++r)
+        if (r < nrb)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) y[r * 16 + 4 * q] = acc[r][q];
     }
   }
-  if (tc < 0) return;
-  // v_mfma_f64_16x16x4f64 result layout: lane l, register q holds
-  // D[l/16 + 4q][l%16] (checked on gfx950).
-  double *L = Lexp + (size_t)tgt_slot[tc] * KP;
-  const int K = (p + 1) * (p + 1);
-#pragma unroll
-  for (int r = 0; r < RB; ++r)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int row = r * 16 + kg + 4 * q;
-      if (row < K) L[row] = acc[r][q];
-    }
+}
+
+// L of every M2L target = sum of its pairs' partials, in list order.
+__global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
+                                                const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
+                                                const double *__restrict__ Y, double *__restrict__ Lexp) {
+  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= ntgt) return;
+  const int32_t c = tgt_cells[t];
+  const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
+  double *L = Lexp + (size_t)t * KP; // target slot t holds cell tgt_cells[t]
+  for (int row = lane; row < KP; row += 64) {
+    double s = 0;
+    for (int32_t q = b; q < e; ++q) s += Y[(size_t)q * KP + row];
+    L[row] = s;
+  }
 }
 
 // ---- L2P --------------------------------------------------------------------------------
@@ -453,40 +497,32 @@ __global__ void k_terms(const int32_t *cells, int64_t ncell, const int32_t *cbeg
   if (threadIdx.x == 0) atomicAdd(out, s[0]);
 }
 
-// Targets ordered by (octant within the parent, cell): targets of one octant
-// share their set of offset classes, so a 16-target tile fills its MFMA
-// columns for every class of a dense region.
-__global__ void k_tile_order(const int32_t *tgt_cells, int64_t ntgt, const uint64_t *ckey, int32_t *cnt8) {
-  int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (t >= ntgt) return;
-  atomicAdd(&cnt8[ckey[tgt_cells[t]] & 7u], 1);
-}
-// position = (padded start of octant o - targets before octant o) + exclusive
-// scan value, i.e. the padded start plus the rank inside the octant (stable).
-__global__ void k_tile_fill(const int32_t *tgt_cells, int64_t ntgt, const uint64_t *ckey, const int32_t *base_adj,
-                            const int32_t *scan_oct, int32_t *tile_tgt) {
-  int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (t >= ntgt) return;
-  int o = (int)(ckey[tgt_cells[t]] & 7u);
-  tile_tgt[base_adj[o] + scan_oct[(int64_t)o * ntgt + t]] = tgt_cells[t];
-}
-__global__ void k_oct_flags(const int32_t *tgt_cells, int64_t ntgt, const uint64_t *ckey, int32_t *flags) {
-  int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (t >= ntgt) return;
-  int o = (int)(ckey[tgt_cells[t]] & 7u);
-  for (int k = 0; k < 8; ++k) flags[(int64_t)k * ntgt + t] = (k == o);
-}
-
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT > 0 ? (n + kT - 1) / kT : 1); }
 
-template <int KP>
-void launch_m2l(nbfmm_ctx *c, int p, int ntiles, int ncols, const int32_t *tile_tgt) {
-  hipLaunchKernelGGL(k_m2l_mfma<KP>, dim3((ntiles + 3) / 4), dim3(kT), 0, c->stream, p, ntiles, tile_tgt,
-                     (int32_t)ncols, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                     (const int32_t *)c->m2l_src.p, (const uint16_t *)c->m2l_cls.p, (const int32_t *)c->src_slot.p,
-                     (const int32_t *)c->tgt_slot.p, (const double *)c->tmat.p, (const double *)c->mexp.p,
-                     (double *)c->lexp.p);
+constexpr int rbw_for(int KP, int lds) { return lds / (128 * KP) < KP / 16 ? lds / (128 * KP) : KP / 16; }
+
+template <int KP, int RBW>
+void launch_m2l_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs) {
+  constexpr int RB = KP / 16;
+  const size_t lds = (size_t)RBW * 16 * KP * sizeof(double);
+  static bool attr_set[64] = {};
+  if (lds > 65536 && !attr_set[c->device & 63]) {
+    NB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_cls<KP, RBW>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set[c->device & 63] = true;
+  }
+  hipLaunchKernelGGL((k_m2l_cls<KP, RBW>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(kM2LThreads), lds, c->stream,
+                     items, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->src_slot.p,
+                     (const double *)c->tmat.p, (const double *)c->mexp.p, (double *)c->ypair.p);
   NB_LAUNCH();
+}
+
+template <int KP>
+void launch_m2l(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs) {
+  if (c->lds_max >= 163840)
+    launch_m2l_rbw<KP, rbw_for(KP, 163840)>(c, nitems, items, cls_pairs);
+  else
+    launch_m2l_rbw<KP, rbw_for(KP, 65536)>(c, nitems, items, cls_pairs);
 }
 
 } // namespace
@@ -572,43 +608,44 @@ void run_expansions(nbfmm_ctx *c) {
   hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, P.geo.h(D, 0) / rho,
                      P.geo.h(D, 1) / rho, P.geo.h(D, 2) / rho, frag);
   NB_LAUNCH();
-  // tiles of 16 targets, targets grouped by octant (padded per octant)
-  const int64_t nt = P.n_tgt;
-  const size_t fl = (size_t)std::max<int64_t>(8 * nt, std::max<int64_t>(P.n, P.n_cells)) + 1;
-  int32_t *flags = dget<int32_t>(c->flags, fl);
-  int32_t *fscan = dget<int32_t>(c->fscan, fl);
-  hipLaunchKernelGGL(k_oct_flags, dim3(nblk(nt)), dim3(kT), 0, s, (const int32_t *)c->tgt_cells.p, nt, ckey, flags);
+  // sort the pairs by class (counts -> host schedule -> scatter)
+  const int64_t nt = P.n_tgt, npair = P.n_m2l;
+  int32_t *ccnt = dget<int32_t>(c->cls_cnt, 2 * kNumClasses + 8);
+  NB_HIP(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * kNumClasses, s));
+  const unsigned nchunk = (unsigned)((npair + kClsChunk - 1) / kClsChunk);
+  hipLaunchKernelGGL(k_cls_hist, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, ccnt);
   NB_LAUNCH();
-  scan_excl_i32(flags, fscan, 8 * nt, c->scan_tmp, s);
-  // fscan[o*nt] = number of targets in octants < o
-  int32_t base8h[8];
-  for (int o = 0; o < 8; ++o)
-    NB_HIP(hipMemcpyAsync(&base8h[o], fscan + (int64_t)o * nt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  int32_t *hcnt = (int32_t *)c->hred.p; // pinned, kNumClasses ints
+  NB_HIP(hipMemcpyAsync(hcnt, ccnt, sizeof(int32_t) * kNumClasses, hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
-  int32_t pad8[9];
-  pad8[0] = 0;
-  for (int o = 0; o < 8; ++o) {
-    int32_t cnt = (o < 7 ? base8h[o + 1] : (int32_t)nt) - base8h[o];
-    pad8[o + 1] = pad8[o] + (cnt + 15) / 16 * 16;
+  int64_t per = npair / 2048;
+  per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
+  std::vector<int32_t> &hoff = c->h_cls_off;
+  std::vector<int4> &hitems = c->h_m2l_items;
+  hoff.assign(kNumClasses + 1, 0);
+  hitems.clear();
+  for (int k = 0; k < kNumClasses; ++k) {
+    hoff[k + 1] = hoff[k] + hcnt[k];
+    for (int64_t b = hoff[k]; b < hoff[k + 1]; b += per)
+      hitems.push_back(make_int4(k, (int)b, (int)std::min<int64_t>(b + per, hoff[k + 1]), 0));
   }
-  const int ncols = pad8[8];
-  const int ntiles = ncols / 16;
-  P.n_m2l_tiles = ntiles;
-  int32_t *tile_tgt = dget<int32_t>(c->tiles, (size_t)ncols + 16);
-  NB_HIP(hipMemsetAsync(tile_tgt, 0xff, sizeof(int32_t) * (size_t)ncols, s));
-  int32_t base_adj[8];
-  for (int o = 0; o < 8; ++o) base_adj[o] = pad8[o] - base8h[o];
-  int32_t *dbase = dget<int32_t>(c->p2m_off, (size_t)P.n_src * 2 + 2 + 8) + P.n_src * 2 + 2;
-  NB_HIP(hipMemcpyAsync(dbase, base_adj, sizeof(base_adj), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_tile_fill, dim3(nblk(nt)), dim3(kT), 0, s, (const int32_t *)c->tgt_cells.p, nt, ckey, dbase,
-                     fscan, tile_tgt);
+  if (hoff[kNumClasses] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
+  NB_HIP(hipMemcpyAsync(ccnt + kNumClasses, hoff.data(), sizeof(int32_t) * kNumClasses, hipMemcpyHostToDevice, s));
+  int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair + 1);
+  hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair,
+                     ccnt + kNumClasses, cls_pairs);
   NB_LAUNCH();
+  const unsigned nm2l = (unsigned)hitems.size();
+  int4 *ditems = dget<int4>(c->m2l_items, (size_t)nm2l + 1);
+  NB_HIP(hipMemcpyAsync(ditems, hitems.data(), sizeof(int4) * nm2l, hipMemcpyHostToDevice, s));
+  P.n_m2l_tiles = nm2l;
+  dget<double>(c->ypair, (size_t)npair * KP);
   double *Lexp = dget<double>(c->lexp, (size_t)nt * KP);
   mark(c, 7);
   switch (KP) {
 #define NB_M2L_CASE(k)                                  \
   case k:                                               \
-    launch_m2l<k>(c, p, ntiles, ncols, tile_tgt);       \
+    launch_m2l<k>(c, nm2l, ditems, cls_pairs);          \
     break;
     NB_M2L_CASE(16)
     NB_M2L_CASE(32)
@@ -626,6 +663,11 @@ void run_expansions(nbfmm_ctx *c) {
       throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
   }
   mark(c, 8);
+  hipLaunchKernelGGL(k_m2l_sum, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
+                     (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                     (const double *)c->ypair.p, Lexp);
+  NB_LAUNCH();
+
   mark(c, 4);
   // ---- L2P ----
   hipLaunchKernelGGL(k_l2p, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,

@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(kW) k_lists(int32_t p0, const int32_t *__restr
                                               int32_t *cnt_il, int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
                                               const int32_t *off_il, const int32_t *off_m2l, const int32_t *off_sub,
                                               int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls, int32_t *sub_src,
-                                              int32_t *is_src, unsigned long long *near_pairs) {
+                                              int32_t *is_src, int64_t *near_pairs) {
   __shared__ int32_t cand[kWin];
   __shared__ int32_t csz[kWin];
   const int32_t P = p0 + (int32_t)blockIdx.x;
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(kW) k_lists(int32_t p0, const int32_t *__restr
   }
   if (!FILL) {
     for (int off = 32; off >= 1; off >>= 1) np_near += __shfl_xor(np_near, off);
-    if (lane == 0 && np_near) atomicAdd(near_pairs, np_near);
+    if (lane == 0) near_pairs[P] = (int64_t)np_near;
   }
 }
 
@@ -284,15 +284,19 @@ void build_lists(nbfmm_ctx *c) {
   NB_HIP(hipMemsetAsync(cnt_m2l + nc, 0, sizeof(int32_t), s));
   NB_HIP(hipMemsetAsync(cnt_sub + nc, 0, sizeof(int32_t), s));
   const int32_t nparents = leaf_begin; // every non-leaf cell is a parent
+  int64_t *near_pairs = dget<int64_t>(c->near_pairs, (size_t)nparents + 1);
   hipLaunchKernelGGL(k_lists<false>, dim3((unsigned)nparents), dim3(kW), 0, s, 0, nbr27, child8, cbeg, cend, cut,
                      leaf_begin, cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, nullptr, nullptr, pairs);
+                     nullptr, nullptr, nullptr, near_pairs);
   NB_LAUNCH();
   scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_sub, off_sub, nc + 1, c->scan_tmp, s);
   hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nc))), dim3(kT), 0, s, (const int64_t *)sub_pairs, nc,
                      pairs + 1);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nparents))), dim3(kT), 0, s,
+                     (const int64_t *)near_pairs, (int64_t)nparents, pairs);
   NB_LAUNCH();
   int32_t tot[3];
   unsigned long long pr[2];

@@ -96,6 +96,7 @@ struct nbfmm_ctx {
   nbfmm_params prm{};
   hipStream_t stream = nullptr;
   int device = 0;
+  int lds_max = 65536; // LDS bytes one workgroup may allocate on this device
   bool timing = true;
   bool keep_lists = false;
   nbfmm::Plan plan;
@@ -110,13 +111,15 @@ struct nbfmm_ctx {
   nbfmm::DBuf hred;                            // pinned host mirror of `red`
   // cells
   nbfmm::DBuf ckey, cbeg, cend, cparent, child8, nbr27, flags, fscan, closed;
-  nbfmm::DBuf cnt_il, cnt_m2l, cnt_sub, off_il, off_m2l, off_sub, acc_sub;
+  nbfmm::DBuf cnt_il, cnt_m2l, cnt_sub, off_il, off_m2l, off_sub, near_pairs;
   nbfmm::DBuf il, m2l_src, m2l_cls, sub_src;
   nbfmm::DBuf src_slot, tgt_slot, src_cells, tgt_cells, is_src;
   nbfmm::DBuf sub_pairs, p2m_items, p2m_off, tiles;
   nbfmm::DBuf tasks, task_off, task_cnt, dpart;
   // expansions
-  nbfmm::DBuf stab, tmat, mexp, lexp, consts;
+  nbfmm::DBuf stab, tmat, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
+  std::vector<int32_t> h_cls_off; // host schedule of the class-sorted M2L
+  std::vector<int4> h_m2l_items;
   // integrator scratch
   nbfmm::DBuf xk, vk, ak, asum, vsum;
   // events
