@@ -142,9 +142,16 @@ class Context:
             pass
 
     def set_params(self, **kw):
+        """Updates the named parameters; on failure the old ones stay in force."""
+        old = {k: getattr(self.params, k) for k in kw}
         for k, v in kw.items():
             setattr(self.params, k, v)
-        _check(lib().nbfmm_set_params(self.h, ctypes.byref(self.params)))
+        try:
+            _check(lib().nbfmm_set_params(self.h, ctypes.byref(self.params)))
+        except NbfmmError:
+            for k, v in old.items():
+                setattr(self.params, k, v)
+            raise
 
     @property
     def stream(self):
