@@ -102,7 +102,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(how the evaluation would shard is described in DESIGN.md).", file=sys.stderr)
+              f"(sharding over several GPUs is designed, not built).", file=sys.stderr)
         return 2
     cfg = CONFIGS[args.config]
     n = cfg["n"]
