@@ -35,7 +35,6 @@ int check_params(const nbfmm_params *p) {
   if (p->p2p_cut < 0) return fail(NBFMM_ERR_ARG, "p2p_cut must be >= 0");
   if (p->precision != NBFMM_PLAIN_FP64 && p->precision != NBFMM_MIXED)
     return fail(NBFMM_ERR_ARG, "unknown precision");
-  if (p->precision == NBFMM_MIXED) return fail(NBFMM_ERR_UNSUPPORTED, "mixed precision is not built yet");
   return NBFMM_OK;
 }
 
@@ -121,7 +120,7 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &pcell, &grad, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,
-                 &p2m_off, &tiles, &stab, &tmat, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum};
+                 &p2m_off, &tiles, &stab, &tmat, &tmat32, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum};
   for (DBuf *b : all) dbuf_free(*b);
   if (hred.p) (void)hipHostFree(hred.p);
   for (auto &e : ev)

@@ -59,10 +59,6 @@ void fill_consts(Consts &C) {
     }
 }
 
-struct cd2 {
-  double x, y;
-};
-__device__ __forceinline__ cd2 cmul(cd2 a, cd2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
 __device__ __forceinline__ void cell_centre(uint64_t key, int lev, const LevelGeo &g, double c[3]) {
   uint64_t k = key;
@@ -197,9 +193,12 @@ __global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, double rx, doub
 // column m at a time with no divergence: q R_m^m by the sectoral recurrence,
 // then R_n^m, n = m+1..p, by the three-term recurrence in n, accumulated per
 // lane; one xor-shuffle reduction per coefficient at the end of the column.
+// T is the coefficient type (double, or float in mixed precision); the
+// offset from the cell centre is formed in FP64 and scaled before rounding.
+template <class T>
 __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restrict__ items, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ src_cells, const uint64_t *__restrict__ ckey,
-                                            LevelGeo g, const Consts *__restrict__ C, double *__restrict__ Mexp) {
+                                            LevelGeo g, const Consts *__restrict__ C, T *__restrict__ Mexp) {
   const int4 it = items[blockIdx.x];
   const int slot = it.x, beg = it.y, end = it.z;
   const int32_t cell = src_cells[slot];
@@ -208,9 +207,9 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
   cell_centre(ckey[cell], lev, g, ctr);
   const double ir = g.inv_rho[lev];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double *M = Mexp + (size_t)slot * KP;
+  T *M = Mexp + (size_t)slot * KP;
   for (int m = wave; m <= p; m += kT / 64) {
-    double ar[kMaxOrder + 1], ai[kMaxOrder + 1];
+    T ar[kMaxOrder + 1], ai[kMaxOrder + 1];
 #pragma unroll
     for (int j = 0; j <= kMaxOrder; ++j) {
       ar[j] = 0;
@@ -219,21 +218,21 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
     const int nj = p - m; // j = n - m in [0, nj]
     for (int i = beg + lane; i < end; i += 64) {
       const double4 q = pos[i];
-      const double x = (q.x - ctr[0]) * ir, y = (q.y - ctr[1]) * ir, z = (q.z - ctr[2]) * ir;
-      const double r2 = x * x + y * y + z * z;
-      double vr = q.w, vi = 0; // q * R_m^m
+      const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
+      const T r2 = x * x + y * y + z * z;
+      T vr = (T)q.w, vi = 0; // q * R_m^m
       for (int k = 1; k <= m; ++k) {
-        const double f = C->mmf[k];
-        const double nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
+        const T f = (T)C->mmf[k];
+        const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
         vr = nr;
         vi = ni;
       }
       ar[0] += vr;
       ai[0] -= vi;
-      double pr = vr, pi = vi;
-      double cr = 0, ci = 0;
+      T pr = vr, pi = vi;
+      T cr = 0, ci = 0;
       if (nj >= 1) {
-        const double f = z * C->s2m1[m];
+        const T f = z * (T)C->s2m1[m];
         cr = f * pr;
         ci = f * pi;
         ar[1] += cr;
@@ -243,8 +242,8 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
       for (int j = 2; j <= kMaxOrder; ++j) {
         if (j <= nj) {
           const int n = m + j;
-          const double a = C->ra[n][m] * z, b = C->rb[n][m] * r2;
-          const double nr = a * cr - b * pr, ni = a * ci - b * pi;
+          const T a = (T)C->ra[n][m] * z, b = (T)C->rb[n][m] * r2;
+          const T nr = a * cr - b * pr, ni = a * ci - b * pi;
           pr = cr;
           pi = ci;
           cr = nr;
@@ -257,7 +256,7 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
 #pragma unroll
     for (int j = 0; j <= kMaxOrder; ++j) {
       if (j <= nj) {
-        double vr = ar[j], vi = ai[j];
+        T vr = ar[j], vi = ai[j];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
           vr += __shfl_xor(vr, o);
@@ -277,122 +276,17 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
   }
 }
 
-// ---- M2L on MFMA --------------------------------------------------------------------
-// Pairs are sorted by offset class.  A workgroup takes one class and a range
-// of its pairs: it stages the class operator (row blocks [r0, r0 + RBW)) in
-// LDS once, then each wave runs 16-pair tiles, B = the 16 source multipoles,
-// acc[r] += T[r] * B with v_mfma_f64_16x16x4f64, every MFMA column filled.
-// The partial local of every pair goes to Y[pair] (pairs numbered in the
-// target-major list order) and k_m2l_sum adds the partials of each target
-// in list order, so the result does not depend on the schedule.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-constexpr int kM2LThreads = 512;
-constexpr int kClsChunk = 4096;
-
-__global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cls, int64_t n, int32_t *cnt) {
-  __shared__ int32_t h[kNumClasses];
-  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
-  __syncthreads();
-  const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
-  for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
-  __syncthreads();
-  for (int k = threadIdx.x; k < kNumClasses; k += kT)
-    if (h[k]) atomicAdd(&cnt[k], h[k]);
-}
-
-// cursor[k] starts at the first position of class k; the order of the pairs
-// inside a class is free (every pair's result is computed independently).
-__global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, int64_t n, int32_t *cursor,
-                                                    int32_t *__restrict__ out) {
-  __shared__ int32_t h[kNumClasses];
-  __shared__ int32_t base[kNumClasses];
-  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
-  __syncthreads();
-  const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
-  for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
-  __syncthreads();
-  for (int k = threadIdx.x; k < kNumClasses; k += kT) {
-    base[k] = h[k] ? atomicAdd(&cursor[k], h[k]) : 0;
-    h[k] = 0;
-  }
-  __syncthreads();
-  for (int64_t i = b + threadIdx.x; i < e; i += kT) {
-    const int k = cls[i];
-    out[base[k] + atomicAdd(&h[k], 1)] = (int32_t)i;
-  }
-}
-
-template <int KP, int RBW>
-__global__ void __launch_bounds__(kM2LThreads) k_m2l_cls(const int4 *__restrict__ items,
-                                                         const int32_t *__restrict__ cls_pairs,
-                                                         const int32_t *__restrict__ m2l_src,
-                                                         const int32_t *__restrict__ src_slot,
-                                                         const double *__restrict__ frag,
-                                                         const double *__restrict__ Mexp, double *__restrict__ Y) {
-  constexpr int RB = KP / 16, KS = KP / 4;
-  extern __shared__ double sA[];
-  const int4 it = items[blockIdx.x];
-  const int r0 = (int)blockIdx.y * RBW;
-  const int nrb = min(RBW, RB - r0);
-  {
-    const double2 *A = (const double2 *)(frag + ((size_t)it.x * RB + r0) * KS * 64);
-    const int n2 = nrb * KS * 32;
-    for (int e = threadIdx.x; e < n2; e += kM2LThreads) ((double2 *)sA)[e] = A[e];
-  }
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int col = lane & 15, kg = lane >> 4;
-  for (int pb = it.y + wave * 16; pb < it.z; pb += (kM2LThreads / 64) * 16) {
-    const int pi = pb + col;
-    const int32_t pid = pi < it.z ? cls_pairs[pi] : -1;
-    const double *Mb = Mexp + (size_t)(pid >= 0 ? src_slot[m2l_src[pid]] : 0) * KP + kg;
-    double bv[KS];
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) bv[ks] = pid >= 0 ? Mb[ks * 4] : 0.0;
-    dbl4 acc[RBW];
-#pragma unroll
-    for (int r = 0; r < RBW; ++r) acc[r] = dbl4{0, 0, 0, 0};
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-      for (int r = 0; r < RBW; ++r)
-        if (r < nrb) acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[(r * KS + ks) * 64 + lane], bv[ks], acc[r], 0, 0, 0);
-    if (pid >= 0) {
-      // v_mfma_f64_16x16x4f64 result layout: lane l, register q holds
-      // D[l/16 + 4q][l%16] (checked on gfx950).
-      double *y = Y + (size_t)pid * KP + r0 * 16 + kg;
-#pragma unroll
-      for (int r = 0; r < RBW; ++r)
-        if (r < nrb)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) y[r * 16 + 4 * q] = acc[r][q];
-    }
-  }
-}
-
-// L of every M2L target = sum of its pairs' partials, in list order.
-__global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
-                                                const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
-                                                const double *__restrict__ Y, double *__restrict__ Lexp) {
-  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (t >= ntgt) return;
-  const int32_t c = tgt_cells[t];
-  const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
-  double *L = Lexp + (size_t)t * KP; // target slot t holds cell tgt_cells[t]
-  for (int row = lane; row < KP; row += 64) {
-    double s = 0;
-    for (int32_t q = b; q < e; ++q) s += Y[(size_t)q * KP + row];
-    L[row] = s;
-  }
-}
-
 // ---- L2P --------------------------------------------------------------------------------
 // grad phi at every particle from the locals of its ancestors that are M2L
 // targets:  grad = (1/rho^2) sum_{n'<p, m'>=0} w(m') Re[R_n'^m'(t) G(n'+1, m')]
 // with G_z = cz L_n^m', G_x = (cu L_n^{m'-1} - cd L_n^{m'+1})/2,
 // G_y = -(i/2)(cu L_n^{m'-1} + cd L_n^{m'+1}), L_n^{-1} = -conj(L_n^1).
-__device__ __forceinline__ cd2 lget(const double *L, int n, int m, int p) {
+template <class T>
+struct cpx {
+  T x, y;
+};
+template <class T>
+__device__ __forceinline__ cpx<T> lget(const T *L, int n, int m, int p) {
   if (m > n || n > p) return {0, 0};
   if (m == 0) return {L[n * n], 0};
   if (m > 0) return {L[n * n + 2 * m - 1], L[n * n + 2 * m]};
@@ -400,10 +294,12 @@ __device__ __forceinline__ cd2 lget(const double *L, int n, int m, int p) {
   return {-L[n * n + 1], L[n * n + 2]};
 }
 
+// T as in k_p2m; the gradient is accumulated over levels in FP64.
+template <class T>
 __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ pcell, const int32_t *__restrict__ cparent,
                                             const uint64_t *__restrict__ ckey, const int32_t *__restrict__ tgt_slot,
-                                            LevelGeo g, const Consts *__restrict__ C, const double *__restrict__ Lexp,
+                                            LevelGeo g, const Consts *__restrict__ C, const T *__restrict__ Lexp,
                                             double *__restrict__ grad) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
@@ -416,50 +312,50 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
       double ctr[3];
       cell_centre(ckey[cell], lev, g, ctr);
       const double ir = g.inv_rho[lev];
-      const double x = (q.x - ctr[0]) * ir, y = (q.y - ctr[1]) * ir, z = (q.z - ctr[2]) * ir;
-      const double r2 = x * x + y * y + z * z;
-      const double *L = Lexp + (size_t)ts * KP;
-      double hx = 0, hy = 0, hz = 0;
-      double vr = 1, vi = 0; // R_m^m
+      const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
+      const T r2 = x * x + y * y + z * z;
+      const T *L = Lexp + (size_t)ts * KP;
+      T hx = 0, hy = 0, hz = 0;
+      T vr = 1, vi = 0; // R_m^m
       for (int m = 0; m < p; ++m) {
         if (m > 0) {
-          double f = C->mmf[m];
-          double nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
+          const T f = (T)C->mmf[m];
+          const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
           vr = nr;
           vi = ni;
         }
-        const double w = m == 0 ? 1.0 : 2.0;
-        double pr = 0, pi = 0, cr = vr, ci = vi;
+        const T w = m == 0 ? (T)1 : (T)2;
+        T pr = 0, pi = 0, cr = vr, ci = vi;
         for (int np = m; np < p; ++np) {
           if (np == m + 1) {
             pr = cr;
             pi = ci;
-            double f = z * C->s2m1[m];
+            const T f = z * (T)C->s2m1[m];
             cr = f * pr;
             ci = f * pi;
           } else if (np > m + 1) {
-            double a = C->ra[np][m] * z, b = C->rb[np][m] * r2;
-            double nr = a * cr - b * pr, ni = a * ci - b * pi;
+            const T a = (T)C->ra[np][m] * z, b = (T)C->rb[np][m] * r2;
+            const T nr = a * cr - b * pr, ni = a * ci - b * pi;
             pr = cr;
             pi = ci;
             cr = nr;
             ci = ni;
           }
           const int nn = np + 1;
-          const cd2 Lm = lget(L, nn, m - 1, p), L0 = lget(L, nn, m, p), Lp = lget(L, nn, m + 1, p);
-          const double cz = C->cz[nn][m], cu = C->cu[nn][m], cdd = C->cd[nn][m];
+          const cpx<T> Lm = lget(L, nn, m - 1, p), L0 = lget(L, nn, m, p), Lp = lget(L, nn, m + 1, p);
+          const T cz = (T)C->cz[nn][m], cu = (T)C->cu[nn][m], cdd = (T)C->cd[nn][m];
           // Re(R * G)
           hz += w * cz * (cr * L0.x - ci * L0.y);
-          const double ux = cu * Lm.x - cdd * Lp.x, uy = cu * Lm.y - cdd * Lp.y;
-          hx += 0.5 * w * (cr * ux - ci * uy);
-          const double vx = cu * Lm.x + cdd * Lp.x, vy = cu * Lm.y + cdd * Lp.y;
-          hy += 0.5 * w * (cr * vy + ci * vx);
+          const T ux = cu * Lm.x - cdd * Lp.x, uy = cu * Lm.y - cdd * Lp.y;
+          hx += (T)0.5 * w * (cr * ux - ci * uy);
+          const T vx = cu * Lm.x + cdd * Lp.x, vy = cu * Lm.y + cdd * Lp.y;
+          hy += (T)0.5 * w * (cr * vy + ci * vx);
         }
       }
       const double s = ir * ir;
-      gx += s * hx;
-      gy += s * hy;
-      gz += s * hz;
+      gx += s * (double)hx;
+      gy += s * (double)hy;
+      gz += s * (double)hz;
     }
     cell = cparent[cell];
   }
@@ -499,32 +395,6 @@ __global__ void k_terms(const int32_t *cells, int64_t ncell, const int32_t *cbeg
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT > 0 ? (n + kT - 1) / kT : 1); }
 
-constexpr int rbw_for(int KP, int lds) { return lds / (128 * KP) < KP / 16 ? lds / (128 * KP) : KP / 16; }
-
-template <int KP, int RBW>
-void launch_m2l_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs) {
-  constexpr int RB = KP / 16;
-  const size_t lds = (size_t)RBW * 16 * KP * sizeof(double);
-  static bool attr_set[64] = {};
-  if (lds > 65536 && !attr_set[c->device & 63]) {
-    NB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_cls<KP, RBW>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set[c->device & 63] = true;
-  }
-  hipLaunchKernelGGL((k_m2l_cls<KP, RBW>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(kM2LThreads), lds, c->stream,
-                     items, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->src_slot.p,
-                     (const double *)c->tmat.p, (const double *)c->mexp.p, (double *)c->ypair.p);
-  NB_LAUNCH();
-}
-
-template <int KP>
-void launch_m2l(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs) {
-  if (c->lds_max >= 163840)
-    launch_m2l_rbw<KP, rbw_for(KP, 163840)>(c, nitems, items, cls_pairs);
-  else
-    launch_m2l_rbw<KP, rbw_for(KP, 65536)>(c, nitems, items, cls_pairs);
-}
-
 } // namespace
 
 int padded_k(int p) { return ((p + 1) * (p + 1) + 15) / 16 * 16; }
@@ -536,6 +406,7 @@ void run_expansions(nbfmm_ctx *c) {
   const int KP = padded_k(p);
   P.kp = KP;
   const int D = P.geo.depth;
+  const bool f32 = c->prm.precision == NBFMM_MIXED;
   LevelGeo g;
   std::memset(&g, 0, sizeof(g));
   for (int d = 0; d < 3; ++d) g.lo[d] = P.geo.lo[d];
@@ -595,10 +466,15 @@ void run_expansions(nbfmm_ctx *c) {
   hipLaunchKernelGGL(k_p2m_items, dim3(nblk(P.n_src)), dim3(kT), 0, s, (const int32_t *)c->src_cells.p, P.n_src, cbeg,
                      cend, poff, items);
   NB_LAUNCH();
-  double *Mexp = dget<double>(c->mexp, (size_t)P.n_src * KP);
-  NB_HIP(hipMemsetAsync(Mexp, 0, sizeof(double) * (size_t)P.n_src * KP, s));
-  hipLaunchKernelGGL(k_p2m, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
-                     (const int32_t *)c->src_cells.p, ckey, g, dC, Mexp);
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  void *Mexp = dbuf_ensure(c->mexp, (size_t)P.n_src * KP * esz + 16);
+  NB_HIP(hipMemsetAsync(Mexp, 0, esz * (size_t)P.n_src * KP, s));
+  if (f32)
+    hipLaunchKernelGGL(k_p2m<float>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
+                       (const int32_t *)c->src_cells.p, ckey, g, dC, (float *)Mexp);
+  else
+    hipLaunchKernelGGL(k_p2m<double>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
+                       (const int32_t *)c->src_cells.p, ckey, g, dC, (double *)Mexp);
   NB_LAUNCH();
   mark(c, 3);
   // ---- M2L ----
@@ -608,71 +484,17 @@ void run_expansions(nbfmm_ctx *c) {
   hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, P.geo.h(D, 0) / rho,
                      P.geo.h(D, 1) / rho, P.geo.h(D, 2) / rho, frag);
   NB_LAUNCH();
-  // sort the pairs by class (counts -> host schedule -> scatter)
-  const int64_t nt = P.n_tgt, npair = P.n_m2l;
-  int32_t *ccnt = dget<int32_t>(c->cls_cnt, 2 * kNumClasses + 8);
-  NB_HIP(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * kNumClasses, s));
-  const unsigned nchunk = (unsigned)((npair + kClsChunk - 1) / kClsChunk);
-  hipLaunchKernelGGL(k_cls_hist, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, ccnt);
-  NB_LAUNCH();
-  int32_t *hcnt = (int32_t *)c->hred.p; // pinned, kNumClasses ints
-  NB_HIP(hipMemcpyAsync(hcnt, ccnt, sizeof(int32_t) * kNumClasses, hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  int64_t per = npair / 2048;
-  per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
-  std::vector<int32_t> &hoff = c->h_cls_off;
-  std::vector<int4> &hitems = c->h_m2l_items;
-  hoff.assign(kNumClasses + 1, 0);
-  hitems.clear();
-  for (int k = 0; k < kNumClasses; ++k) {
-    hoff[k + 1] = hoff[k] + hcnt[k];
-    for (int64_t b = hoff[k]; b < hoff[k + 1]; b += per)
-      hitems.push_back(make_int4(k, (int)b, (int)std::min<int64_t>(b + per, hoff[k + 1]), 0));
-  }
-  if (hoff[kNumClasses] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
-  NB_HIP(hipMemcpyAsync(ccnt + kNumClasses, hoff.data(), sizeof(int32_t) * kNumClasses, hipMemcpyHostToDevice, s));
-  int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair + 1);
-  hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair,
-                     ccnt + kNumClasses, cls_pairs);
-  NB_LAUNCH();
-  const unsigned nm2l = (unsigned)hitems.size();
-  int4 *ditems = dget<int4>(c->m2l_items, (size_t)nm2l + 1);
-  NB_HIP(hipMemcpyAsync(ditems, hitems.data(), sizeof(int4) * nm2l, hipMemcpyHostToDevice, s));
-  P.n_m2l_tiles = nm2l;
-  dget<double>(c->ypair, (size_t)npair * KP);
-  double *Lexp = dget<double>(c->lexp, (size_t)nt * KP);
-  mark(c, 7);
-  switch (KP) {
-#define NB_M2L_CASE(k)                                  \
-  case k:                                               \
-    launch_m2l<k>(c, nm2l, ditems, cls_pairs);          \
-    break;
-    NB_M2L_CASE(16)
-    NB_M2L_CASE(32)
-    NB_M2L_CASE(48)
-    NB_M2L_CASE(64)
-    NB_M2L_CASE(96)
-    NB_M2L_CASE(112)
-    NB_M2L_CASE(128)
-    NB_M2L_CASE(144)
-    NB_M2L_CASE(176)
-    NB_M2L_CASE(208)
-    NB_M2L_CASE(240)
-#undef NB_M2L_CASE
-    default:
-      throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
-  }
-  mark(c, 8);
-  hipLaunchKernelGGL(k_m2l_sum, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
-                     (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                     (const double *)c->ypair.p, Lexp);
-  NB_LAUNCH();
-
+  run_m2l(c, KP, f32);
   mark(c, 4);
   // ---- L2P ----
-  hipLaunchKernelGGL(k_l2p, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
-                     (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)c->tgt_slot.p, g,
-                     dC, Lexp, grad);
+  if (f32)
+    hipLaunchKernelGGL(k_l2p<float>, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
+                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)c->tgt_slot.p,
+                       g, dC, (const float *)c->lexp.p, grad);
+  else
+    hipLaunchKernelGGL(k_l2p<double>, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
+                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)c->tgt_slot.p,
+                       g, dC, (const double *)c->lexp.p, grad);
   NB_LAUNCH();
   mark(c, 5);
 }

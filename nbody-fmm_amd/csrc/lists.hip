@@ -24,34 +24,57 @@ constexpr int kWin = 216;
 // direct-evaluation work per P2P sub-task (particle pairs)
 constexpr int64_t kTaskPairs = 1 << 17;
 
+// One wavefront per parent, four parents per block.  A child c with
+// n_c * (largest particle count among its parent's 27 neighbours) < cut has
+// every interaction-list pair of its subtree evaluated directly (the target
+// of such a pair lies inside c and its source inside one of those
+// neighbours), so c and all its descendants are closed.  When the caller
+// does not keep the lists and c may be merged into a closed-cell task (cell
+// index >= skip_from), its lists are never read: they are skipped and its
+// counts stay 0.  skip_from < 0 classifies every cell.
 template <bool FILL>
-__global__ void __launch_bounds__(kW) k_lists(int32_t p0, const int32_t *__restrict__ nbr27,
+__global__ void __launch_bounds__(kT) k_lists(int32_t nparents, int32_t skip_from, const int32_t *__restrict__ nbr27,
                                               const int32_t *__restrict__ child8, const int32_t *__restrict__ cbeg,
-                                              const int32_t *__restrict__ cend, int64_t cut, int32_t leaf_begin,
-                                              int32_t *cnt_il, int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
+                                              const int32_t *__restrict__ cend, int64_t cut, int32_t *cnt_il,
+                                              int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
                                               const int32_t *off_il, const int32_t *off_m2l, const int32_t *off_sub,
                                               int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls, int32_t *sub_src,
-                                              int32_t *is_src, int64_t *near_pairs) {
-  __shared__ int32_t cand[kWin];
-  __shared__ int32_t csz[kWin];
-  const int32_t P = p0 + (int32_t)blockIdx.x;
-  const int lane = threadIdx.x;
-  for (int w = lane; w < kWin; w += kW) {
-    const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
-    const int32_t Pn = nbr27[(int64_t)P * 27 + (wz >> 1) * 9 + (wy >> 1) * 3 + (wx >> 1)];
-    const int32_t q = Pn >= 0 ? child8[(int64_t)Pn * 8 + ((wx & 1) | ((wy & 1) << 1) | ((wz & 1) << 2))] : -1;
-    cand[w] = q;
-    csz[w] = q >= 0 ? cend[q] - cbeg[q] : 0;
+                                              int32_t *is_src) {
+  __shared__ int32_t cand_s[kT / kW][kWin];
+  __shared__ int32_t csz_s[kT / kW][kWin];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int32_t P = (int32_t)blockIdx.x * (kT / kW) + wave;
+  const bool active = P < nparents;
+  int32_t *cand = cand_s[wave], *csz = csz_s[wave];
+  int64_t maxn = 0;
+  // children to classify (bit o): all when skipping is off
+  uint32_t todo = 0;
+  if (active) {
+    if (skip_from >= 0) {
+      const int32_t q = lane < 27 ? nbr27[(int64_t)P * 27 + lane] : -1;
+      maxn = q >= 0 ? cend[q] - cbeg[q] : 0;
+      for (int off = 32; off >= 1; off >>= 1) maxn = max(maxn, __shfl_xor(maxn, off));
+    }
+    const int32_t ch = lane < 8 ? child8[(int64_t)P * 8 + lane] : -1;
+    const bool need = ch >= 0 && !(skip_from >= 0 && ch >= skip_from && (int64_t)(cend[ch] - cbeg[ch]) * maxn < cut);
+    todo = (uint32_t)__ballot(need);
+    if (todo)
+      for (int w = lane; w < kWin; w += kW) {
+        const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
+        const int32_t Pn = nbr27[(int64_t)P * 27 + (wz >> 1) * 9 + (wy >> 1) * 3 + (wx >> 1)];
+        const int32_t q = Pn >= 0 ? child8[(int64_t)Pn * 8 + ((wx & 1) | ((wy & 1) << 1) | ((wz & 1) << 2))] : -1;
+        cand[w] = q;
+        csz[w] = q >= 0 ? cend[q] - cbeg[q] : 0;
+      }
   }
   __syncthreads();
+  if (!todo) return;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long np_near = 0;
   for (int o = 0; o < 8; ++o) {
+    if (!((todo >> o) & 1u)) continue;
     const int32_t s = child8[(int64_t)P * 8 + o];
-    if (s < 0) continue;
     const int bx = o & 1, by = (o >> 1) & 1, bz = o >> 2;
     const int64_t nt = cend[s] - cbeg[s];
-    const bool leaf = s >= leaf_begin;
     int32_t a_il = 0, a_m = 0, a_s = 0;
     if (FILL) {
       a_il = il ? off_il[s] : 0;
@@ -80,9 +103,8 @@ __global__ void __launch_bounds__(kW) k_lists(int32_t p0, const int32_t *__restr
           m2l_cls[k] = (uint16_t)((dz + 3) * 49 + (dy + 3) * 7 + (dx + 3));
           is_src[q] = 1;
         }
-      } else {
-        if (sub) sp += nt * ns;
-        if (leaf && near && q >= 0) np_near += (unsigned long long)(q == s ? nt * (nt - 1) : nt * ns);
+      } else if (sub) {
+        sp += nt * ns;
       }
       c_il += __popcll(bf);
       c_s += __popcll(bs);
@@ -97,10 +119,6 @@ __global__ void __launch_bounds__(kW) k_lists(int32_t p0, const int32_t *__restr
         sub_pairs[s] = sp;
       }
     }
-  }
-  if (!FILL) {
-    for (int off = 32; off >= 1; off >>= 1) np_near += __shfl_xor(np_near, off);
-    if (lane == 0) near_pairs[P] = (int64_t)np_near;
   }
 }
 
@@ -210,21 +228,26 @@ __device__ __forceinline__ int32_t seg_len(int64_t ng, int64_t np) {
   return (int32_t)sl;
 }
 
+// Also the particle pairs each cell's tasks evaluate (a 27-neighbour row
+// contains the target cell itself, whose particles skip themselves).
 __global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
                              const int32_t *closed, const int32_t *cbeg, const int32_t *cend, const int32_t *cnt_sub,
-                             const int32_t *off_sub, const int64_t *sub_pairs, int32_t *cnt) {
+                             const int32_t *off_sub, const int64_t *sub_pairs, int32_t *cnt, int64_t *pairs) {
   const int32_t c = 1 + (int32_t)(blockIdx.x * kT + threadIdx.x);
   if (c >= nc) return;
   const SrcGroups g = src_groups(c, leaf_begin, cparent, nbr27, closed, cbeg, cend, cnt_sub, off_sub, sub_pairs);
   const int32_t nt = cend[c] - cbeg[c];
   const int32_t ntg = (nt + kTaskTargets - 1) / kTaskTargets;
   int32_t k = 0;
+  int64_t np = 0;
   for (int q = 0; q < g.n; ++q) {
     if (g.np[q] <= 0) continue;
     const int32_t sl = seg_len(min(nt, kTaskTargets), g.np[q]);
     k += (int32_t)((g.np[q] + sl - 1) / sl);
+    np += (int64_t)nt * (g.np[q] - (g.list[q] < 0 ? 1 : 0));
   }
   cnt[c] = k * ntg;
+  pairs[c] = np;
 }
 
 __global__ void k_task_fill(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
@@ -273,51 +296,41 @@ void build_lists(nbfmm_ctx *c) {
   int32_t *off_m2l = dget<int32_t>(c->off_m2l, (size_t)nc + 1);
   int32_t *off_sub = dget<int32_t>(c->off_sub, (size_t)nc + 1);
   int64_t *sub_pairs = dget<int64_t>(c->sub_pairs, (size_t)nc + 1);
-  unsigned long long *pairs = (unsigned long long *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
-  NB_HIP(hipMemsetAsync(pairs, 0, sizeof(unsigned long long) * 2, s));
-  // cells without a parent pass (the root) have empty lists
-  NB_HIP(hipMemsetAsync(cnt_il, 0, sizeof(int32_t) * 2, s));
-  NB_HIP(hipMemsetAsync(cnt_m2l, 0, sizeof(int32_t) * 2, s));
-  NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * 2, s));
-  NB_HIP(hipMemsetAsync(sub_pairs, 0, sizeof(int64_t) * 2, s));
-  NB_HIP(hipMemsetAsync(cnt_il + nc, 0, sizeof(int32_t), s));
-  NB_HIP(hipMemsetAsync(cnt_m2l + nc, 0, sizeof(int32_t), s));
-  NB_HIP(hipMemsetAsync(cnt_sub + nc, 0, sizeof(int32_t), s));
+  // Closed cells (below) merge the near field of their whole subtree into one
+  // P2P task; cells of level >= lc may be merged.
+  const int lc = 1;
+  const int32_t skip_from = c->keep_lists ? -1 : P.lvl_start[lc];
+  // cells that are not classified (the root, skipped children) have empty lists
+  NB_HIP(hipMemsetAsync(cnt_il, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  NB_HIP(hipMemsetAsync(cnt_m2l, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
+  NB_HIP(hipMemsetAsync(sub_pairs, 0, sizeof(int64_t) * (size_t)(nc + 1), s));
   const int32_t nparents = leaf_begin; // every non-leaf cell is a parent
-  int64_t *near_pairs = dget<int64_t>(c->near_pairs, (size_t)nparents + 1);
-  hipLaunchKernelGGL(k_lists<false>, dim3((unsigned)nparents), dim3(kW), 0, s, 0, nbr27, child8, cbeg, cend, cut,
-                     leaf_begin, cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, nullptr, nullptr, near_pairs);
+  const unsigned nlb = (unsigned)((nparents + kT / kW - 1) / (kT / kW));
+  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut,
+                     cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr);
   NB_LAUNCH();
   scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_sub, off_sub, nc + 1, c->scan_tmp, s);
-  hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nc))), dim3(kT), 0, s, (const int64_t *)sub_pairs, nc,
-                     pairs + 1);
-  NB_LAUNCH();
-  hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nparents))), dim3(kT), 0, s,
-                     (const int64_t *)near_pairs, (int64_t)nparents, pairs);
-  NB_LAUNCH();
   int32_t tot[3];
-  unsigned long long pr[2];
   NB_HIP(hipMemcpyAsync(&tot[0], off_il + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipMemcpyAsync(&tot[1], off_m2l + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipMemcpyAsync(&tot[2], off_sub + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(pr, pairs, sizeof(pr), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   P.n_il = tot[0];
   P.n_m2l = tot[1];
   P.n_sub = tot[2];
-  P.n_p2p_pairs = (int64_t)(pr[0] + pr[1]);
   int32_t *il = c->keep_lists ? dget<int32_t>(c->il, (size_t)P.n_il + 1) : nullptr;
   int32_t *m2l_src = dget<int32_t>(c->m2l_src, (size_t)P.n_m2l + 1);
   uint16_t *m2l_cls = dget<uint16_t>(c->m2l_cls, (size_t)P.n_m2l + 1);
   int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
   int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
   NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  hipLaunchKernelGGL(k_lists<true>, dim3((unsigned)nparents), dim3(kW), 0, s, 0, nbr27, child8, cbeg, cend, cut,
-                     leaf_begin, nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il,
-                     m2l_src, m2l_cls, sub_src, is_src, nullptr);
+  hipLaunchKernelGGL(k_lists<true>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut,
+                     nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il, m2l_src,
+                     m2l_cls, sub_src, is_src);
   NB_LAUNCH();
 
   // ---- M2L source / target compaction ----
@@ -350,8 +363,8 @@ void build_lists(nbfmm_ctx *c) {
 
   // ---- P2P sub-tasks ----
   int32_t *closed = dget<int32_t>(c->closed, (size_t)nc + 1);
-  NB_HIP(hipMemsetAsync(closed, 0, sizeof(int32_t), s)); // the root is never a task cell
-  for (int d = D; d >= 1; --d) {
+  NB_HIP(hipMemsetAsync(closed, 0, sizeof(int32_t) * (size_t)P.lvl_start[lc], s)); // never task cells
+  for (int d = D; d >= lc; --d) {
     const int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
     hipLaunchKernelGGL(k_closed, dim3(nblk(ce - cb)), dim3(kT), 0, s, cb, ce, leaf_begin, (const int32_t *)cnt_m2l,
                        child8, closed);
@@ -362,15 +375,25 @@ void build_lists(nbfmm_ctx *c) {
   int32_t *toff = dget<int32_t>(c->task_off, (size_t)nc + 1);
   NB_HIP(hipMemsetAsync(tcnt, 0, sizeof(int32_t), s));
   NB_HIP(hipMemsetAsync(tcnt + nc, 0, sizeof(int32_t), s));
+  int64_t *cell_pairs = dget<int64_t>(c->near_pairs, (size_t)nc + 1);
+  NB_HIP(hipMemsetAsync(cell_pairs, 0, sizeof(int64_t), s));
+  unsigned long long *pairs = (unsigned long long *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
+  NB_HIP(hipMemsetAsync(pairs, 0, sizeof(unsigned long long), s));
   hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
                      (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
-                     (const int64_t *)sub_pairs, tcnt);
+                     (const int64_t *)sub_pairs, tcnt, cell_pairs);
   NB_LAUNCH();
   scan_excl_i32(tcnt, toff, nc + 1, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nc))), dim3(kT), 0, s,
+                     (const int64_t *)cell_pairs, nc, pairs);
+  NB_LAUNCH();
   int32_t ntask;
+  unsigned long long npairs;
   NB_HIP(hipMemcpyAsync(&ntask, toff + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&npairs, pairs, sizeof(npairs), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   P.n_tasks = ntask;
+  P.n_p2p_pairs = (int64_t)npairs;
   P2PTask *tasks = dget<P2PTask>(c->tasks, (size_t)ntask + 1);
   hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
                      (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,

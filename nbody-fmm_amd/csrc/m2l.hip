@@ -1,0 +1,300 @@
+// m2l.hip -- the M2L translation of the gfx950 FMM on the matrix cores.
+//
+// With coefficients scaled by the cell scale rho_l, the M2L operator of an
+// integer offset class d in [-3,3]^3 is one KP x KP matrix at every level
+// (built by k_build_ops in MFMA A-fragment order).  The M2L pairs of all
+// levels are sorted by class; a workgroup takes one class and a range of its
+// pairs, stages the class operator (row blocks [r0, r0 + RBW)) in LDS once,
+// and each wave runs 16-pair tiles: B = the 16 source multipoles,
+// acc[r] += T[r] * B, every MFMA column filled.  The partial local of every
+// pair goes to Y[pair] (pairs numbered in the target-major list order) and
+// k_m2l_sum adds the partials of each target in list order, so the result
+// does not depend on the schedule.
+//
+// Plain precision runs v_mfma_f64_16x16x4f64 on FP64 coefficients; mixed
+// precision v_mfma_f32_16x16x4f32 on FP32 coefficients (the rho scaling keeps
+// them O(1)), with the per-target sum of the partials in FP64.
+#include "nbfmm_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace nbfmm {
+namespace {
+
+constexpr int kT = 256;
+// workgroup size: 8 waves (FP64, 2 per SIMD at ~254 VGPRs) or 16 waves (FP32)
+template <class T>
+constexpr int m2l_threads() {
+  return sizeof(T) == 4 ? 1024 : 512;
+}
+constexpr int kClsChunk = 4096;
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct Mfma;
+// A[i][k] comes from lane i + 16k, B[k][j] from lane j + 16k (both types).
+template <>
+struct Mfma<double> {
+  typedef dbl4 V;
+  static __device__ __forceinline__ V op(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // v_mfma_f64_16x16x4f64: lane l, register q holds D[l/16 + 4q][l%16]
+  static __device__ __forceinline__ int row(int kg, int q) { return kg + 4 * q; }
+};
+template <>
+struct Mfma<float> {
+  typedef flt4 V;
+  static __device__ __forceinline__ V op(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // v_mfma_f32_16x16x4f32: lane l, register q holds D[4(l/16) + q][l%16]
+  static __device__ __forceinline__ int row(int kg, int q) { return 4 * kg + q; }
+};
+
+__global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cls, int64_t n, int32_t *cnt) {
+  __shared__ int32_t h[kNumClasses];
+  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
+  for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumClasses; k += kT)
+    if (h[k]) atomicAdd(&cnt[k], h[k]);
+}
+
+// cursor[k] starts at the first position of class k; the order of the pairs
+// inside a class is free (every pair's result is computed independently).
+__global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, int64_t n, int32_t *cursor,
+                                                    int32_t *__restrict__ out) {
+  __shared__ int32_t h[kNumClasses];
+  __shared__ int32_t base[kNumClasses];
+  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
+  for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumClasses; k += kT) {
+    base[k] = h[k] ? atomicAdd(&cursor[k], h[k]) : 0;
+    h[k] = 0;
+  }
+  __syncthreads();
+  for (int64_t i = b + threadIdx.x; i < e; i += kT) {
+    const int k = cls[i];
+    out[base[k] + atomicAdd(&h[k], 1)] = (int32_t)i;
+  }
+}
+
+__global__ void k_to_f32(const double *__restrict__ in, int64_t n, float *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+template <int KP, int RBW, class T>
+__global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__restrict__ items,
+                                                         const int32_t *__restrict__ cls_pairs,
+                                                         const int32_t *__restrict__ m2l_src,
+                                                         const int32_t *__restrict__ src_slot, const T *__restrict__ frag,
+                                                         const T *__restrict__ Mexp, T *__restrict__ Y) {
+  constexpr int RB = KP / 16, KS = KP / 4;
+  extern __shared__ double4 s_raw[];
+  T *sA = reinterpret_cast<T *>(s_raw);
+  const int4 it = items[blockIdx.x];
+  const int r0 = (int)blockIdx.y * RBW;
+  const int nrb = min(RBW, RB - r0);
+  {
+    const double4 *A = reinterpret_cast<const double4 *>(frag + ((size_t)it.x * RB + r0) * KS * 64);
+    const int n32 = (int)(nrb * KS * 64 * sizeof(T) / 32);
+    for (int e = threadIdx.x; e < n32; e += m2l_threads<T>()) s_raw[e] = A[e];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, kg = lane >> 4;
+  for (int pb = it.y + wave * 16; pb < it.z; pb += (m2l_threads<T>() / 64) * 16) {
+    const int pi = pb + col;
+    const int32_t pid = pi < it.z ? cls_pairs[pi] : -1;
+    const T *Mb = Mexp + (size_t)(pid >= 0 ? src_slot[m2l_src[pid]] : 0) * KP + kg;
+    typename Mfma<T>::V acc[RBW];
+#pragma unroll
+    for (int r = 0; r < RBW; ++r) acc[r] = typename Mfma<T>::V{0, 0, 0, 0};
+    if constexpr (sizeof(T) == 8) {
+      T bv[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) bv[ks] = pid >= 0 ? Mb[ks * 4] : (T)0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int r = 0; r < RBW; ++r)
+          if (r < nrb) acc[r] = Mfma<T>::op(sA[(r * KS + ks) * 64 + lane], bv[ks], acc[r]);
+    } else {
+      // k in chunks of 4 with the next chunk's multipole values loaded ahead
+      // (a fully unrolled k loop exhausts the registers for KP = 96..128)
+      T bc[4], bn[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) bc[u] = pid >= 0 ? Mb[u * 4] : (T)0;
+#pragma unroll 1
+      for (int k0 = 0; k0 < KS; k0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bn[u] = (pid >= 0 && k0 + 4 < KS) ? Mb[(k0 + 4 + u) * 4] : (T)0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int r = 0; r < RBW; ++r)
+            if (RBW == RB || r < nrb) acc[r] = Mfma<T>::op(sA[(r * KS + k0 + u) * 64 + lane], bc[u], acc[r]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bc[u] = bn[u];
+      }
+    }
+    if (pid >= 0) {
+      T *y = Y + (size_t)pid * KP + r0 * 16;
+#pragma unroll
+      for (int r = 0; r < RBW; ++r)
+        if (r < nrb)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) y[r * 16 + Mfma<T>::row(kg, q)] = acc[r][q];
+    }
+  }
+}
+
+// L of every M2L target = sum of its pairs' partials, in list order (FP64).
+template <class T>
+__global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
+                                                const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
+                                                const T *__restrict__ Y, T *__restrict__ Lexp) {
+  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= ntgt) return;
+  const int32_t c = tgt_cells[t];
+  const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
+  T *L = Lexp + (size_t)t * KP; // target slot t holds cell tgt_cells[t]
+  for (int row = lane; row < KP; row += 64) {
+    double s = 0;
+    for (int32_t q = b; q < e; ++q) s += (double)Y[(size_t)q * KP + row];
+    L[row] = (T)s;
+  }
+}
+
+template <class T>
+constexpr int rbw_for(int KP, int lds) {
+  return lds / (16 * KP * (int)sizeof(T)) < KP / 16 ? lds / (16 * KP * (int)sizeof(T)) : KP / 16;
+}
+
+template <int KP, int RBW, class T>
+void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs, const T *frag) {
+  constexpr int RB = KP / 16;
+  const size_t lds = (size_t)RBW * 16 * KP * sizeof(T);
+  static bool attr_set[64] = {};
+  if (lds > 65536 && !attr_set[c->device & 63]) {
+    NB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_cls<KP, RBW, T>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set[c->device & 63] = true;
+  }
+  hipLaunchKernelGGL((k_m2l_cls<KP, RBW, T>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(m2l_threads<T>()), lds, c->stream,
+                     items, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->src_slot.p, frag,
+                     (const T *)c->mexp.p, (T *)c->ypair.p);
+  NB_LAUNCH();
+}
+
+// gfx950 gives a workgroup up to 160 KB of LDS.
+constexpr int kLds = 163840;
+
+template <int KP, class T>
+void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs, const T *frag) {
+  if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
+  launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, cls_pairs, frag);
+}
+
+template <class T>
+void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, const int32_t *cls_pairs, const T *frag) {
+  switch (KP) {
+#define NB_M2L_CASE(k)                                        \
+  case k:                                                     \
+    launch_kp<k, T>(c, nitems, items, cls_pairs, frag);       \
+    break;
+    NB_M2L_CASE(16)
+    NB_M2L_CASE(32)
+    NB_M2L_CASE(48)
+    NB_M2L_CASE(64)
+    NB_M2L_CASE(96)
+    NB_M2L_CASE(112)
+    NB_M2L_CASE(128)
+    NB_M2L_CASE(144)
+    NB_M2L_CASE(176)
+    NB_M2L_CASE(208)
+    NB_M2L_CASE(240)
+#undef NB_M2L_CASE
+    default:
+      throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
+  }
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
+
+} // namespace
+
+// Sorts the pairs by class (counts -> host schedule -> scatter), runs the
+// MFMA contraction (between events 7 and 8) and the per-target sum.
+// Expects the FP64 operators in c->tmat and the multipoles in c->mexp
+// (FP32 when `f32`).
+void run_m2l(nbfmm_ctx *c, int KP, bool f32) {
+  Plan &P = c->plan;
+  hipStream_t s = c->stream;
+  const int64_t nt = P.n_tgt, npair = P.n_m2l;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  int32_t *ccnt = dget<int32_t>(c->cls_cnt, 2 * kNumClasses + 8);
+  NB_HIP(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * kNumClasses, s));
+  const unsigned nchunk = (unsigned)((npair + kClsChunk - 1) / kClsChunk);
+  hipLaunchKernelGGL(k_cls_hist, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, ccnt);
+  NB_LAUNCH();
+  int32_t *hcnt = (int32_t *)c->hred.p; // pinned, >= kNumClasses ints
+  NB_HIP(hipMemcpyAsync(hcnt, ccnt, sizeof(int32_t) * kNumClasses, hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  int64_t per = npair / 2048;
+  per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
+  std::vector<int32_t> &hoff = c->h_cls_off;
+  std::vector<int4> &hitems = c->h_m2l_items;
+  hoff.assign(kNumClasses + 1, 0);
+  hitems.clear();
+  for (int k = 0; k < kNumClasses; ++k) {
+    hoff[k + 1] = hoff[k] + hcnt[k];
+    for (int64_t b = hoff[k]; b < hoff[k + 1]; b += per)
+      hitems.push_back(make_int4(k, (int)b, (int)std::min<int64_t>(b + per, hoff[k + 1]), 0));
+  }
+  if (hoff[kNumClasses] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
+  NB_HIP(hipMemcpyAsync(ccnt + kNumClasses, hoff.data(), sizeof(int32_t) * kNumClasses, hipMemcpyHostToDevice, s));
+  int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair + 1);
+  hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair,
+                     ccnt + kNumClasses, cls_pairs);
+  NB_LAUNCH();
+  const unsigned nitems = (unsigned)hitems.size();
+  int4 *ditems = dget<int4>(c->m2l_items, (size_t)nitems + 1);
+  NB_HIP(hipMemcpyAsync(ditems, hitems.data(), sizeof(int4) * nitems, hipMemcpyHostToDevice, s));
+  P.n_m2l_tiles = nitems;
+  dbuf_ensure(c->ypair, (size_t)npair * KP * esz + 16);
+  void *Lexp = dbuf_ensure(c->lexp, (size_t)nt * KP * esz + 16);
+  const int64_t nfrag = (int64_t)kNumClasses * KP * KP;
+  if (f32) {
+    float *frag32 = dget<float>(c->tmat32, (size_t)nfrag);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk(nfrag)), dim3(kT), 0, s, (const double *)c->tmat.p, nfrag, frag32);
+    NB_LAUNCH();
+    mark(c, 7);
+    launch<float>(c, KP, nitems, ditems, cls_pairs, frag32);
+    mark(c, 8);
+    hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
+                       (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                       (const float *)c->ypair.p, (float *)Lexp);
+  } else {
+    mark(c, 7);
+    launch<double>(c, KP, nitems, ditems, cls_pairs, (const double *)c->tmat.p);
+    mark(c, 8);
+    hipLaunchKernelGGL(k_m2l_sum<double>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
+                       (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                       (const double *)c->ypair.p, (double *)Lexp);
+  }
+  NB_LAUNCH();
+}
+
+} // namespace nbfmm

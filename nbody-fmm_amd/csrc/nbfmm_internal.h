@@ -117,7 +117,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf sub_pairs, p2m_items, p2m_off, tiles;
   nbfmm::DBuf tasks, task_off, task_cnt, dpart;
   // expansions
-  nbfmm::DBuf stab, tmat, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
+  nbfmm::DBuf stab, tmat, tmat32, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
   std::vector<int32_t> h_cls_off; // host schedule of the class-sorted M2L
   std::vector<int4> h_m2l_items;
   // integrator scratch
@@ -138,6 +138,8 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m);
 void build_lists(nbfmm_ctx *c);
 // expansions.hip
 void run_expansions(nbfmm_ctx *c);
+// m2l.hip
+void run_m2l(nbfmm_ctx *c, int KP, bool f32);
 // p2p.hip
 void run_p2p(nbfmm_ctx *c, double *d_acc_out);
 void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,

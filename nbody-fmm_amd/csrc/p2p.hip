@@ -27,11 +27,12 @@ constexpr int kT = 256;
 constexpr int kW = 64; // one wavefront per task
 static_assert(kTaskTargets == 4 * kW, "a P2P task holds at most four targets per lane");
 
-__device__ __forceinline__ void pair_acc(const double4 &xi, const double4 &xj, bool same, double &ax, double &ay,
-                                         double &az) {
+// A particle meets itself (and coincident particles meet each other) with
+// r2 = 0; those pairs contribute nothing.
+__device__ __forceinline__ void pair_acc(const double4 &xi, const double4 &xj, double &ax, double &ay, double &az) {
   const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
-  if (!same && r2 > 0.0) {
+  if (r2 > 0.0) {
     const double inv = rsqrt(r2);
     const double w = xj.w * inv * inv * inv;
     ax -= dx * w;
@@ -45,7 +46,6 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad) {
   __shared__ double4 sh[kW];
-  __shared__ int32_t shj[kW];
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   const int lane = threadIdx.x;
@@ -87,7 +87,6 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
       }
       const int32_t j = lb[lo] + (g - lp[lo]);
       sh[lane] = pos[j];
-      shj[lane] = j;
     }
     __syncthreads();
     return min(kW, f1 - fb);
@@ -103,7 +102,7 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
     double ax = 0, ay = 0, az = 0;
     for (int32_t fb = f0; fb < f1; fb += kW) {
       const int cnt = stage(fb);
-      for (int k = si; k < cnt; k += sub) pair_acc(xi, sh[k], shj[k] == i, ax, ay, az);
+      for (int k = si; k < cnt; k += sub) pair_acc(xi, sh[k], ax, ay, az);
     }
     for (int o = sub >> 1; o >= 1; o >>= 1) {
       ax += __shfl_xor(ax, o);
@@ -131,9 +130,8 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
     const int cnt = stage(fb);
     for (int k = 0; k < cnt; ++k) {
       const double4 xj = sh[k];
-      const int32_t j = shj[k];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) pair_acc(xi[q], xj, j == ii[q], ax[q], ay[q], az[q]);
+      for (int q = 0; q < 4; ++q) pair_acc(xi[q], xj, ax[q], ay[q], az[q]);
     }
   }
 #pragma unroll
@@ -142,6 +140,181 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
       atomicAdd(&grad[3 * (int64_t)ii[q]], ax[q]);
       atomicAdd(&grad[3 * (int64_t)ii[q] + 1], ay[q]);
       atomicAdd(&grad[3 * (int64_t)ii[q] + 2], az[q]);
+    }
+}
+
+// ---- P2P in FP32 (mixed precision) ----------------------------------------------------
+// Same tasks and staging as k_p2p_tasks.  Positions are offsets from an
+// anchor (the mean of the task's targets), formed in FP64 and kept as FP32
+// pairs hi + lo; the separation (xi_hi - xj_hi) + (xi_lo - xj_lo) is then
+// accurate to a few FP32 ulps of the separation itself, however large the
+// offsets are.  The pair term uses v_rsq_f32.  A target's contributions from
+// one 64-source chunk are summed in FP32 (at most 64 terms per lane); the
+// chunk sums are added with Kahan compensation into an FP32 (sum,
+// correction) pair, and the slots and tasks are combined in FP64.
+struct f2pos {
+  float4 hi; // offset (x, y, z) and the mass
+  float4 lo; // remainder of the offset
+};
+
+__device__ __forceinline__ f2pos split(const double4 &p, const double4 &a) {
+  const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
+  const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+  return {make_float4(hx, hy, hz, (float)p.w),
+          make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.0f)};
+}
+
+__device__ __forceinline__ void pair_acc_f(const f2pos &xi, const float4 &jh, const float4 &jl, float &ax, float &ay,
+                                           float &az) {
+  const float dx = (xi.hi.x - jh.x) + (xi.lo.x - jl.x);
+  const float dy = (xi.hi.y - jh.y) + (xi.lo.y - jl.y);
+  const float dz = (xi.hi.z - jh.z) + (xi.lo.z - jl.z);
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 > 0.0f) {
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float w = jh.w * inv * inv * inv;
+    ax -= dx * w;
+    ay -= dy * w;
+    az -= dz * w;
+  }
+}
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+__global__ void __launch_bounds__(kW) k_p2p_tasks_f32(const P2PTask *__restrict__ tasks,
+                                                      const int32_t *__restrict__ nbr27,
+                                                      const int32_t *__restrict__ sub_src,
+                                                      const int32_t *__restrict__ cbeg,
+                                                      const int32_t *__restrict__ cend,
+                                                      const double4 *__restrict__ pos, double *__restrict__ grad) {
+  __shared__ float4 shh[kW];
+  __shared__ float4 shl[kW];
+  __shared__ int32_t lb[kMaxList];
+  __shared__ int32_t lp[kMaxList];
+  const int lane = threadIdx.x;
+  const P2PTask tk = tasks[blockIdx.x];
+  const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+  // anchor: the mean position of the task's targets
+  double4 anchor = make_double4(0, 0, 0, 0);
+  for (int t = lane; t < nc; t += kW) {
+    const double4 q = pos[b + t];
+    anchor.x += q.x;
+    anchor.y += q.y;
+    anchor.z += q.z;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    anchor.x += __shfl_xor(anchor.x, off);
+    anchor.y += __shfl_xor(anchor.y, off);
+    anchor.z += __shfl_xor(anchor.z, off);
+  }
+  anchor.x /= nc;
+  anchor.y /= nc;
+  anchor.z /= nc;
+  const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
+  const int ne = tk.nlist;
+  int32_t carry = 0;
+  for (int base = 0; base < ne; base += kW) {
+    const int e = base + lane;
+    const int32_t q = e < ne ? list[e] : -1;
+    const int32_t sb = q >= 0 ? cbeg[q] : 0;
+    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
+    int32_t x = cnt;
+    for (int off = 1; off < kW; off <<= 1) {
+      const int32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (e < ne) {
+      lb[e] = sb;
+      lp[e] = carry + x - cnt;
+    }
+    carry += __shfl(x, kW - 1);
+  }
+  const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
+  auto stage = [&](int32_t fb) -> int {
+    const int32_t g = fb + lane;
+    __syncthreads();
+    if (g < f1) {
+      int lo = 0, hi = ne;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lp[mid] <= g)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const f2pos v = split(pos[lb[lo] + (g - lp[lo])], anchor);
+      shh[lane] = v.hi;
+      shl[lane] = v.lo;
+    }
+    __syncthreads();
+    return min(kW, f1 - fb);
+  };
+  if (nc <= kW) {
+    int np2 = 1;
+    while (np2 < nc) np2 <<= 1;
+    const int sub = kW / np2;
+    const int ti = lane / sub, si = lane % sub;
+    const bool valid = ti < nc;
+    const int32_t i = b + (valid ? ti : 0);
+    const f2pos xi = split(pos[i], anchor);
+    float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
+    for (int32_t fb = f0; fb < f1; fb += kW) {
+      const int cnt = stage(fb);
+      float ax = 0, ay = 0, az = 0;
+      for (int k = si; k < cnt; k += sub) pair_acc_f(xi, shh[k], shl[k], ax, ay, az);
+      kahan_add(sx, cx, ax);
+      kahan_add(sy, cy, ay);
+      kahan_add(sz, cz, az);
+    }
+    double gx = (double)sx + (double)cx, gy = (double)sy + (double)cy, gz = (double)sz + (double)cz;
+    for (int o = sub >> 1; o >= 1; o >>= 1) {
+      gx += __shfl_xor(gx, o);
+      gy += __shfl_xor(gy, o);
+      gz += __shfl_xor(gz, o);
+    }
+    if (valid && si == 0) {
+      atomicAdd(&grad[3 * (int64_t)i], gx);
+      atomicAdd(&grad[3 * (int64_t)i + 1], gy);
+      atomicAdd(&grad[3 * (int64_t)i + 2], gz);
+    }
+    return;
+  }
+  f2pos xi[4];
+  float sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0};
+  float cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0};
+  int32_t ii[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int32_t t = lane + q * kW;
+    ii[q] = t < nc ? b + t : -1;
+    xi[q] = split(pos[t < nc ? b + t : b], anchor);
+  }
+  for (int32_t fb = f0; fb < f1; fb += kW) {
+    const int cnt = stage(fb);
+    float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
+    for (int k = 0; k < cnt; ++k) {
+      const float4 jh = shh[k], jl = shl[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pair_acc_f(xi[q], jh, jl, ax[q], ay[q], az[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      kahan_add(sx[q], cx[q], ax[q]);
+      kahan_add(sy[q], cy[q], ay[q]);
+      kahan_add(sz[q], cz[q], az[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (ii[q] >= 0) {
+      atomicAdd(&grad[3 * (int64_t)ii[q]], (double)sx[q] + (double)cx[q]);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 1], (double)sy[q] + (double)cy[q]);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 2], (double)sz[q] + (double)cz[q]);
     }
 }
 
@@ -264,7 +437,8 @@ void run_p2p(nbfmm_ctx *c, double *d_acc_out) {
   Plan &P = c->plan;
   mark(c, 9);
   if (P.n_tasks > 0) {
-    hipLaunchKernelGGL(k_p2p_tasks, dim3((unsigned)P.n_tasks), dim3(kW), 0, c->stream, (const P2PTask *)c->tasks.p,
+    hipLaunchKernelGGL(c->prm.precision == NBFMM_MIXED ? k_p2p_tasks_f32 : k_p2p_tasks, dim3((unsigned)P.n_tasks),
+                       dim3(kW), 0, c->stream, (const P2PTask *)c->tasks.p,
                        (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const double4 *)c->pos.p,
                        (double *)c->grad.p);
     NB_LAUNCH();

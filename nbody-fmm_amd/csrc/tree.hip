@@ -285,25 +285,25 @@ __global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__res
   child8[(int64_t)cparent[c] * 8 + (int)(ckey[c] & 7u)] = c;
 }
 
-// 27-neighbourhood (offsets x fastest, self at 13) from the parent's table.
+// 27-neighbourhood (offsets x fastest, self at 13) from the parent's table;
+// one thread per table entry.
 __global__ void k_nbr27(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent,
                         const int32_t *__restrict__ child8, int32_t cb, int32_t ce, int lev, int32_t *nbr27) {
-  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int32_t c = cb + (int32_t)(e / 27);
   if (c >= ce) return;
-  uint64_t k = ckey[c];
-  int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
-  int side = 1 << lev;
-  int32_t P = cparent[c];
-  for (int o = 0; o < 27; ++o) {
-    int qx = x + o % 3 - 1, qy = y + (o / 3) % 3 - 1, qz = z + o / 9 - 1;
-    int32_t r = -1;
-    if (qx >= 0 && qy >= 0 && qz >= 0 && qx < side && qy < side && qz < side) {
-      int px = (qx >> 1) - (x >> 1) + 1, py = (qy >> 1) - (y >> 1) + 1, pz = (qz >> 1) - (z >> 1) + 1;
-      int32_t Pn = nbr27[(int64_t)P * 27 + pz * 9 + py * 3 + px];
-      if (Pn >= 0) r = child8[(int64_t)Pn * 8 + ((qx & 1) | ((qy & 1) << 1) | ((qz & 1) << 2))];
-    }
-    nbr27[(int64_t)c * 27 + o] = r;
+  const int o = (int)(e % 27);
+  const uint64_t k = ckey[c];
+  const int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
+  const int side = 1 << lev;
+  const int qx = x + o % 3 - 1, qy = y + (o / 3) % 3 - 1, qz = z + o / 9 - 1;
+  int32_t r = -1;
+  if (qx >= 0 && qy >= 0 && qz >= 0 && qx < side && qy < side && qz < side) {
+    const int px = (qx >> 1) - (x >> 1) + 1, py = (qy >> 1) - (y >> 1) + 1, pz = (qz >> 1) - (z >> 1) + 1;
+    const int32_t Pn = nbr27[(int64_t)cparent[c] * 27 + pz * 9 + py * 3 + px];
+    if (Pn >= 0) r = child8[(int64_t)Pn * 8 + ((qx & 1) | ((qy & 1) << 1) | ((qz & 1) << 2))];
   }
+  nbr27[(int64_t)c * 27 + o] = r;
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
@@ -465,7 +465,8 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   NB_HIP(hipMemsetAsync(nbr27 + 13, 0, sizeof(int32_t), s));
   for (int d = 1; d <= D; ++d) {
     int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_nbr27, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d, nbr27);
+    hipLaunchKernelGGL(k_nbr27, dim3(nblk((int64_t)(ce - cb) * 27)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d,
+                       nbr27);
     NB_LAUNCH();
   }
 }

@@ -157,3 +157,59 @@ def test_bad_inputs(ctx):
     assert np.all(one == 0)
     same = ctx.eval_accel(np.zeros((50, 3)), np.ones(50))
     assert np.all(same == 0)
+
+
+# ---- mixed precision (FP32 P2P with compensated sums, FP32 MFMA M2L, FP32 P2M/L2P) ----
+# Tolerance against the FP64 oracle on the same tree, lists, p and cut: 2e-6
+# relative L2 (FP32 rounding of ~1e-7 per term over the expansion and pair
+# sums); a wrong index, sign or layout gives errors of order 1.
+@pytest.mark.parametrize("p,cut", [(4, 0), (8, 256), (10, 0), (12, 1024)])
+def test_mixed_parity(ctx, p, cut):
+    xyz, m = workloads.uniform_cube(3000, seed=p + 7)
+    ctx.set_params(order=p, leaf_capacity=16, p2p_cut=cut, G=1.0, eps2=0.0, precision=1)
+    try:
+        a = ctx.eval_accel(xyz, m)
+    finally:
+        ctx.set_params(precision=0)
+    b = orc.fmm_exact(xyz, m, p, 16, G=1.0, center_mode=0, p2p_cut=cut)
+    err = _rel(a, b)
+    print(f"[mixed] p={p} cut={cut} rel L2 vs FP64 oracle {err:.3e}")
+    assert err < 2e-6
+
+
+def test_mixed_all_p2p_vs_direct(ctx):
+    # everything direct: the FP32 compensated P2P against the FP64 direct sum,
+    # within the metric's 1e-6 (FP32 offsets from the task anchor bound the
+    # error of the closest pairs)
+    xyz, m = workloads.plummer(20000, seed=8)
+    n = len(m)
+    ctx.set_params(order=4, leaf_capacity=32, p2p_cut=1 << 50, G=1.0, eps2=0.0, precision=1)
+    try:
+        a = ctx.eval_accel(xyz, m)
+    finally:
+        ctx.set_params(precision=0)
+    dx, dm, da = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m), nbfmm.DeviceArray(np.zeros((n, 3)))
+    ctx.direct_accel_dev(n, dx.ptr, dm.ptr, n, None, 1, da.ptr)
+    d = da.download(ctx.stream)
+    err = _rel(a, d)
+    print(f"[mixed] all-P2P rel L2 vs direct {err:.3e}")
+    assert err < 1e-6
+
+
+def test_mixed_accuracy_vs_direct(ctx):
+    xyz, m = workloads.plummer(100000, seed=2)
+    n = len(m)
+    tgt = workloads.sample_targets(n, 2048)
+    dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
+    dt_ = nbfmm.DeviceArray(tgt)
+    dd = nbfmm.DeviceArray(np.zeros((len(tgt), 3)))
+    ctx.direct_accel_dev(n, dx.ptr, dm.ptr, len(tgt), dt_.ptr, 1, dd.ptr)
+    ref = dd.download(ctx.stream)
+    ctx.set_params(order=12, leaf_capacity=64, p2p_cut=1024, precision=1)
+    try:
+        a = ctx.eval_accel(xyz, m)
+    finally:
+        ctx.set_params(precision=0)
+    err = _rel(a[tgt], ref)
+    print(f"[mixed] p=12 rel L2 vs direct {err:.3e}")
+    assert err < 1e-6
