@@ -34,9 +34,9 @@ import workloads  # noqa: E402
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X TFLOP/s (vector and dense MFMA alike)
 CONFIGS = {
     # N = 4,194,304 Plummer (a = 1, M = 1, equal masses), leaf capacity 64: the accuracy-gated metric's workload
-    "plummer4m": dict(n=1 << 22, leaf_capacity=64, orders=(9, 10, 11, 12), cuts=(512, 1024, 2048, 4096)),
+    "plummer4m": dict(n=1 << 22, leaf_capacity=64, leaves=(32, 64), orders=(10, 11, 12), cuts=(256, 512, 1024, 2048)),
     # N = 1,048,576 Plummer, p = 10, leaf capacity 64 (the plain FP64 configuration)
-    "plummer1m": dict(n=1 << 20, leaf_capacity=64, orders=(9, 10, 11, 12), cuts=(512, 1024, 2048, 4096)),
+    "plummer1m": dict(n=1 << 20, leaf_capacity=64, leaves=(32, 64), orders=(10, 11, 12), cuts=(256, 512, 1024, 2048)),
 }
 PREC_NAMES = {0: "fp64", 1: "mixed"}
 
@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--config", default="plummer4m", choices=sorted(CONFIGS))
     ap.add_argument("--order", type=int, default=None, help="fix p instead of tuning")
     ap.add_argument("--p2p-cut", type=int, default=None, help="fix the P2P cut instead of tuning")
+    ap.add_argument("--leaf", type=int, default=None, help="fix the leaf capacity instead of tuning")
     ap.add_argument("--precision", default="all", choices=["all", "fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--targets", type=int, default=4096)
@@ -108,7 +109,8 @@ def main():
     n = cfg["n"]
     orders = (args.order,) if args.order else cfg["orders"]
     cuts = (args.p2p_cut,) if args.p2p_cut is not None else cfg["cuts"]
-    log(f"[bench] config={args.config} N={n} orders={orders} cuts={cuts}")
+    leaves = (args.leaf,) if args.leaf else cfg["leaves"]
+    log(f"[bench] config={args.config} N={n} leaves={leaves} orders={orders} cuts={cuts}")
     xyz, m = workloads.plummer(n, seed=42)
     ctx = nbfmm.Context(order=orders[0], leaf_capacity=cfg["leaf_capacity"], G=1.0, eps2=0.0, p2p_cut=cuts[0])
     dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
@@ -125,20 +127,22 @@ def main():
         except nbfmm.NbfmmError as e:
             log(f"[bench] precision {PREC_NAMES[prec]} unavailable: {e}")
             continue
-        best, table = tune.tune(ctx, n, dx, dm, da, tgt, ref, orders, cuts, gate=args.gate, precision=prec, log=log)
+        best, table = tune.tune(ctx, n, dx, dm, da, tgt, ref, orders, cuts, leaves=leaves, gate=args.gate,
+                                precision=prec, log=log)
         if best is None:
             best = min(table, key=lambda r: r["err"]["l2"])
             log(f"[bench] no setting meets the gate for {PREC_NAMES[prec]}; timing the most accurate one")
-        ctx.set_params(order=best["order"], p2p_cut=best["p2p_cut"], precision=prec)
+        ctx.set_params(order=best["order"], p2p_cut=best["p2p_cut"], leaf_capacity=best["leaf_capacity"],
+                       precision=prec)
         wall, m2l_ms, p2p_ms, st = timed(ctx, n, dx, dm, da, args.steps, args.warmup)
         err = tune.rel_errors(da.download(ctx.stream)[tgt], ref)
         res = {"value": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps, "order": best["order"],
-               "p2p_cut": best["p2p_cut"], "err": err, "meets_gate": err["l2"] <= args.gate,
+               "p2p_cut": best["p2p_cut"], "leaf_capacity": best["leaf_capacity"], "err": err, "meets_gate": err["l2"] <= args.gate,
                "m2l_kernel_ms": m2l_ms, "p2p_kernel_ms": p2p_ms, "stats": st.as_dict(),
-               "tuning": [{"order": r["order"], "p2p_cut": r["p2p_cut"], "ms": round(r["ms"], 3),
-                           "l2": r["err"]["l2"]} for r in table]}
+               "tuning": [{"leaf_capacity": r["leaf_capacity"], "order": r["order"], "p2p_cut": r["p2p_cut"],
+                           "ms": round(r["ms"], 3), "l2": r["err"]["l2"]} for r in table]}
         results[PREC_NAMES[prec]] = res
-        log(f"[bench] {PREC_NAMES[prec]}: p={best['order']} cut={best['p2p_cut']} "
+        log(f"[bench] {PREC_NAMES[prec]}: leaf={best['leaf_capacity']} p={best['order']} cut={best['p2p_cut']} "
             f"{res['ms_per_step']:.2f} ms/step, L2 err {err['l2']:.3e}")
     if args.calib_copy:
         nb = args.calib_copy << 20
@@ -169,17 +173,18 @@ def main():
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if head_name == "fp64" else "f32/f64", "data": "synthetic",
         "config": {
-            "workload": f"Plummer sphere a=1 M=1 equal masses, N={n}, leaf capacity {cfg['leaf_capacity']}, "
-                        f"{head_name}, p={head['order']}, P2P cut {head['p2p_cut']}",
+            "workload": f"Plummer sphere a=1 M=1 equal masses, N={n}, {head_name}, leaf capacity "
+                        f"{head['leaf_capacity']}, p={head['order']}, P2P cut {head['p2p_cut']}",
             "N": n, "precision": head_name, "order": head["order"], "p2p_cut": head["p2p_cut"],
+            "leaf_capacity": head["leaf_capacity"],
             "error_gate_l2": args.gate, "error_targets": len(tgt),
             "rel_force_err_vs_direct": head["err"], "meets_gate": head["meets_gate"],
             "phases_ms": {k: st[k] for k in ("ms_tree", "ms_lists", "ms_p2m", "ms_m2l", "ms_l2p", "ms_p2p",
                                               "ms_total")},
             "counts": {k: st[k] for k in ("depth", "n_cells", "n_leaves", "n_il", "n_m2l", "n_m2l_targets",
                                            "n_m2l_sources", "n_p2p_pairs")},
-            "modes": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "order", "p2p_cut", "err", "meets_gate",
-                                               "tuning")} for k, v in results.items()},
+            "modes": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "order", "p2p_cut", "leaf_capacity", "err",
+                                               "meets_gate", "tuning")} for k, v in results.items()},
             "mixed_over_plain": (results["mixed"]["value"] / results["fp64"]["value"]
                                  if "mixed" in results and "fp64" in results else None),
         },

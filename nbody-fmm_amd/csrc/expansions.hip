@@ -12,14 +12,16 @@
 // offset in [-3,3]^3) is the same matrix at every level, so M2L is one dense
 // KP x KP contraction per class, executed with v_mfma_f64_16x16x4f64.
 //
-// Passes.  The upward pass forms M directly from the particles of every cell
-// that is an M2L source (P2M at every level, the particles of a level-l cell
-// are contiguous in Morton order); the downward pass evaluates the local of
-// every M2L target directly at its particles (L2P at every level).  This is
-// algebraically the M2M/L2L chain of the reference (node.cpp:14-43): M2M of
-// exact multipoles is exact and L2L of a degree-p local is exact, so the two
-// differ only by rounding, while the cost follows the (few) cells that take
-// part in M2L instead of every cell of the tree.
+// Passes.  Upward: every cell that needs a multipole (an M2L source, or a
+// child of a cell that forms its own by M2M) forms it either by P2M from its
+// particles (at most kTransCut particles, or a leaf; the particles of a cell
+// are contiguous in Morton order) or by M2M from its children (larger cells),
+// the reference's upward pass (node.cpp:14-24) restricted to the cells that
+// need it.  Downward: every cell that holds a local (an M2L target, or a
+// child of a cell that pushes) either pushes it to its children by L2L
+// (larger cells, node.cpp:33-43) or has it evaluated at its particles (L2P).
+// M2M and L2L are MFMA contractions with one operator per child octant
+// (m2l.hip); P2M and L2P run on the vector ALUs.
 #include "nbfmm_internal.h"
 
 #include <cmath>
@@ -187,6 +189,126 @@ __global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, double rx, doub
   }
 }
 
+// ---- M2M and L2L operators ---------------------------------------------------------------
+// Child octant o (bit d set: upper half along axis d) sits at d = child centre
+// - parent centre = (+-h_x, +-h_y, +-h_z) / 4 of the parent cell.  In the
+// rho-scaled Schmidt storage (rho_child = rho_parent / 2) the translations are
+// the same matrices at every level; with Yoshida harmonics Rt of
+// dh = d / rho_parent and N_nm = sqrt((n+m)!(n-m)!):
+//   M2M  M_n^m(parent) = sum_{n'<=n,k} N_nm/N_n'k 2^-n' conj(Rt_{n-n'}^{m-k}(dh)) M_n'^k(child)
+//   L2L  L_j^k(child)  = sum_{n>=j,m}  N_nm/N_jk 2^-(j+1) Rt_{n-j}^{m-k}(dh) L_n^m(parent)
+// (the Yoshida translations M'_n^m = sum conj(R_j^k(c - c')) M_{n-j}^{m-k} and
+// L'_j^k = sum L_n^m R_{n-j}^{m-k}(c' - c) rewritten for the scaled storage).
+// The real KP x KP matrix of each octant is written in MFMA A-fragment order
+// like the M2L operators; negative orders follow X_n^-m = (-1)^m conj(X_n^m)
+// for input and output.
+__device__ __forceinline__ void yget(const double *Rre, const double *Rim, int n, int m, double &re, double &im) {
+  const int a = m < 0 ? -m : m;
+  re = Rre[n * (n + 1) / 2 + a];
+  im = Rim[n * (n + 1) / 2 + a];
+  if (m < 0) {
+    im = -im;
+    if (a & 1) {
+      re = -re;
+      im = -im;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, double hx, double hy, double hz,
+                                                    double *__restrict__ m2m, double *__restrict__ l2l) {
+  const int o = blockIdx.x, op = blockIdx.y;
+  __shared__ double Rre[(kMaxOrder + 1) * (kMaxOrder + 2) / 2];
+  __shared__ double Rim[(kMaxOrder + 1) * (kMaxOrder + 2) / 2];
+  __shared__ double lf[2 * kMaxOrder + 2];
+  if (threadIdx.x <= (unsigned)(2 * kMaxOrder + 1)) {
+    double f = 1;
+    for (int k = 2; k <= (int)threadIdx.x; ++k) f *= k;
+    lf[threadIdx.x] = f;
+  }
+  if (threadIdx.x == 0) {
+    const double x = (o & 1 ? 0.25 : -0.25) * hx, y = (o & 2 ? 0.25 : -0.25) * hy, z = (o & 4 ? 0.25 : -0.25) * hz;
+    const double r2 = x * x + y * y + z * z;
+    double sr = 1, si = 0; // Rt_m^m
+    for (int m = 0; m <= p; ++m) {
+      if (m > 0) {
+        const double f = -0.5 / m; // Rt_m^m = -(x + iy) / (2m) Rt_{m-1}^{m-1}
+        const double nr = f * (x * sr - y * si), ni = f * (x * si + y * sr);
+        sr = nr;
+        si = ni;
+      }
+      Rre[m * (m + 1) / 2 + m] = sr;
+      Rim[m * (m + 1) / 2 + m] = si;
+      double pr = 0, pi = 0, cr = sr, ci = si;
+      for (int n = m + 1; n <= p; ++n) {
+        // (n+m)(n-m) Rt_n^m = (2n-1) z Rt_{n-1}^m - r^2 Rt_{n-2}^m
+        const double inv = 1.0 / ((double)(n + m) * (double)(n - m));
+        const double nr = ((2.0 * n - 1.0) * z * cr - r2 * pr) * inv, ni = ((2.0 * n - 1.0) * z * ci - r2 * pi) * inv;
+        pr = cr;
+        pi = ci;
+        cr = nr;
+        ci = ni;
+        Rre[n * (n + 1) / 2 + m] = cr;
+        Rim[n * (n + 1) / 2 + m] = ci;
+      }
+    }
+  }
+  __syncthreads();
+  const int K = (p + 1) * (p + 1), KS = KP / 4;
+  auto Nf = [&](int n, int m) { return sqrt(lf[n + m] * lf[n - m]); };
+  // complex coefficient of input (b, kb) in output (a, ka), any signs of ka, kb
+  auto coef = [&](int a, int ka, int b, int kb, double &re, double &im) {
+    re = 0;
+    im = 0;
+    if (op == 0) { // M2M: output (n=a, m=ka), input (n'=b, k=kb)
+      if (b > a || abs(ka - kb) > a - b) return;
+      yget(Rre, Rim, a - b, ka - kb, re, im);
+      im = -im;
+      const double f = Nf(a, ka) / Nf(b, kb) * ldexp(1.0, -b);
+      re *= f;
+      im *= f;
+    } else { // L2L: output (j=a, k=ka), input (n=b, m=kb)
+      if (b < a || abs(kb - ka) > b - a) return;
+      yget(Rre, Rim, b - a, kb - ka, re, im);
+      const double f = Nf(b, kb) / Nf(a, ka) * ldexp(1.0, -(a + 1));
+      re *= f;
+      im *= f;
+    }
+  };
+  double *out = (op == 0 ? m2m : l2l) + (size_t)o * KP * KP;
+  for (int e = threadIdx.x; e < KP * KP; e += kT) {
+    const int lane = e & 63, blk = e >> 6;
+    const int rb = blk / KS, ks = blk % KS;
+    const int row = rb * 16 + (lane & 15), col = ks * 4 + (lane >> 4);
+    double v = 0;
+    if (row < K && col < K) {
+      int a = (int)sqrt((double)row);
+      while (a * a > row) --a;
+      while ((a + 1) * (a + 1) <= row) ++a;
+      const int rr = row - a * a, ka = (rr + 1) / 2, oim = rr > 0 && (rr & 1) == 0;
+      int b = (int)sqrt((double)col);
+      while (b * b > col) --b;
+      while ((b + 1) * (b + 1) <= col) ++b;
+      const int cc = col - b * b, kb = (cc + 1) / 2, iim = cc > 0 && (cc & 1) == 0;
+      double pr, pi;
+      coef(a, ka, b, kb, pr, pi);
+      if (kb == 0) {
+        v = oim ? pi : pr; // real input
+      } else {
+        double mr, mi;
+        coef(a, ka, b, -kb, mr, mi);
+        const double s = (kb & 1) ? -1.0 : 1.0;
+        // input a + ib:  out = (K+ + s K-) a + i (K+ - s K-) b
+        if (!iim)
+          v = oim ? pi + s * mi : pr + s * mr;
+        else
+          v = oim ? pr - s * mr : -(pi - s * mi);
+      }
+    }
+    out[e] = v;
+  }
+}
+
 // ---- P2M ---------------------------------------------------------------------------
 // Item = (source slot, particle range).  Lane = particle; the orders m are
 // dealt round-robin to the 4 waves of the block, so every wave runs one
@@ -197,11 +319,13 @@ __global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, double rx, doub
 // offset from the cell centre is formed in FP64 and scaled before rounding.
 template <class T>
 __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restrict__ items, const double4 *__restrict__ pos,
-                                            const int32_t *__restrict__ src_cells, const uint64_t *__restrict__ ckey,
-                                            LevelGeo g, const Consts *__restrict__ C, T *__restrict__ Mexp) {
+                                            const int32_t *__restrict__ cells, const int32_t *__restrict__ mslot,
+                                            const uint64_t *__restrict__ ckey, LevelGeo g, const Consts *__restrict__ C,
+                                            T *__restrict__ Mexp) {
   const int4 it = items[blockIdx.x];
-  const int slot = it.x, beg = it.y, end = it.z;
-  const int32_t cell = src_cells[slot];
+  const int beg = it.y, end = it.z;
+  const int32_t cell = cells[it.x];
+  const int slot = mslot[cell];
   const int lev = level_of(cell, g);
   double ctr[3];
   cell_centre(ckey[cell], lev, g, ctr);
@@ -277,8 +401,9 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
 }
 
 // ---- L2P --------------------------------------------------------------------------------
-// grad phi at every particle from the locals of its ancestors that are M2L
-// targets:  grad = (1/rho^2) sum_{n'<p, m'>=0} w(m') Re[R_n'^m'(t) G(n'+1, m')]
+// grad phi at every particle from the locals of the cells on its path to the
+// root that evaluate their local (eslot >= 0; the others pushed theirs down
+// by L2L):  grad = (1/rho^2) sum_{n'<p, m'>=0} w(m') Re[R_n'^m'(t) G(n'+1, m')]
 // with G_z = cz L_n^m', G_x = (cu L_n^{m'-1} - cd L_n^{m'+1})/2,
 // G_y = -(i/2)(cu L_n^{m'-1} + cd L_n^{m'+1}), L_n^{-1} = -conj(L_n^1).
 template <class T>
@@ -298,7 +423,7 @@ __device__ __forceinline__ cpx<T> lget(const T *L, int n, int m, int p) {
 template <class T>
 __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ pcell, const int32_t *__restrict__ cparent,
-                                            const uint64_t *__restrict__ ckey, const int32_t *__restrict__ tgt_slot,
+                                            const uint64_t *__restrict__ ckey, const int32_t *__restrict__ eslot,
                                             LevelGeo g, const Consts *__restrict__ C, const T *__restrict__ Lexp,
                                             double *__restrict__ grad) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -307,7 +432,7 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
   double gx = 0, gy = 0, gz = 0;
   int32_t cell = pcell[i];
   for (int lev = g.depth; lev >= 2; --lev) {
-    const int32_t ts = tgt_slot[cell];
+    const int32_t ts = eslot[cell];
     if (ts >= 0) {
       double ctr[3];
       cell_centre(ckey[cell], lev, g, ctr);
@@ -393,6 +518,74 @@ __global__ void k_terms(const int32_t *cells, int64_t ncell, const int32_t *cbeg
   if (threadIdx.x == 0) atomicAdd(out, s[0]);
 }
 
+// ---- upward / downward plan ------------------------------------------------------------
+// A cell needs a multipole when it is an M2L source or its parent forms its
+// own by M2M; it forms it by M2M from its children when it holds more than
+// kTransCut particles (and is not a leaf), else by P2M from its particles.
+// A cell holds a local when it is an M2L target or its parent pushes its
+// local down; it pushes (L2L) when it holds more than kTransCut particles
+// (and is not a leaf), else its local is evaluated at its particles (L2P).
+// Modes: 0 none, 1 P2M / L2P, 2 M2M / L2L.
+constexpr int32_t kTransCut = 256;
+
+__global__ void k_updown_mark(int32_t cb, int32_t ce, int32_t leaf_begin, const int32_t *__restrict__ cparent,
+                              const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                              const int32_t *__restrict__ is_src, const int32_t *__restrict__ cnt_m2l,
+                              int32_t *mode_m, int32_t *mode_l) {
+  const int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= ce) return;
+  const bool big = c < leaf_begin && cend[c] - cbeg[c] > kTransCut;
+  const int32_t P = cparent[c];
+  const bool mneed = is_src[c] || (P >= 0 && mode_m[P] == 2);
+  const bool lhas = cnt_m2l[c] > 0 || (P >= 0 && mode_l[P] == 2);
+  mode_m[c] = mneed ? (big ? 2 : 1) : 0;
+  mode_l[c] = lhas ? (big ? 2 : 1) : 0;
+}
+
+// five flag arrays of length n + 1 (flag n = 0): M slot, P2M, M2M, L slot, L2L
+__global__ void k_updown_flags(const int32_t *__restrict__ mode_m, const int32_t *__restrict__ mode_l, int64_t n,
+                               int32_t *flags) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i > n) return;
+  const int mm = i < n ? mode_m[i] : 0, ml = i < n ? mode_l[i] : 0;
+  const int64_t s = n + 1;
+  flags[i] = mm > 0;
+  flags[s + i] = mm == 1;
+  flags[2 * s + i] = mm == 2;
+  flags[3 * s + i] = ml > 0;
+  flags[4 * s + i] = ml == 2;
+}
+
+// slots (M, L, evaluated L) and the P2M, M2M and L2L cell lists
+__global__ void k_updown_lists(const int32_t *__restrict__ mode_l, const int32_t *__restrict__ flags,
+                               const int32_t *__restrict__ scan, int64_t n, int32_t *mslot, int32_t *lslot,
+                               int32_t *eslot, int32_t *p2m_cells, int32_t *m2m_cells, int32_t *l2l_cells) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = n + 1;
+  mslot[i] = flags[i] ? scan[i] : -1;
+  const int32_t ls = flags[3 * s + i] ? scan[3 * s + i] : -1;
+  lslot[i] = ls;
+  eslot[i] = mode_l[i] == 1 ? ls : -1;
+  if (flags[s + i]) p2m_cells[scan[s + i]] = (int32_t)i;
+  if (flags[2 * s + i]) m2m_cells[scan[2 * s + i]] = (int32_t)i;
+  if (flags[4 * s + i]) l2l_cells[scan[4 * s + i]] = (int32_t)i;
+}
+
+// particles of the cells with mode == v
+__global__ void k_terms_mode(const int32_t *mode, int32_t v, int64_t ncell, const int32_t *cbeg, const int32_t *cend,
+                             unsigned long long *out) {
+  __shared__ unsigned long long s[kT];
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  s[threadIdx.x] = i < ncell && mode[i] == v ? (unsigned long long)(cend[i] - cbeg[i]) : 0ull;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT > 0 ? (n + kT - 1) / kT : 1); }
 
 } // namespace
@@ -438,62 +631,117 @@ void run_expansions(nbfmm_ctx *c) {
   }
   const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
   const uint64_t *ckey = (const uint64_t *)c->ckey.p;
-  // ---- P2M of every M2L source cell ----
-  int32_t *pcnt = dget<int32_t>(c->p2m_off, (size_t)P.n_src * 2 + 2 + 8);
-  int32_t *poff = pcnt + P.n_src + 1;
-  hipLaunchKernelGGL(k_p2m_count, dim3(nblk(P.n_src)), dim3(kT), 0, s, (const int32_t *)c->src_cells.p, P.n_src,
-                     cbeg, cend, pcnt);
+  const int64_t nc = P.n_cells;
+  const int32_t leaf_begin = P.lvl_start[D];
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  // ---- operators: M2L classes and the M2M / L2L octants (level independent) ----
+  const double rho = P.geo.rho(D);
+  const double hx = P.geo.h(D, 0) / rho, hy = P.geo.h(D, 1) / rho, hz = P.geo.h(D, 2) / rho;
+  double *frag = dget<double>(c->tmat, (size_t)kNumClasses * KP * KP);
+  hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, hx, hy, hz, frag);
   NB_LAUNCH();
-  NB_HIP(hipMemsetAsync(pcnt + P.n_src, 0, sizeof(int32_t), s));
-  scan_excl_i32(pcnt, poff, P.n_src + 1, c->scan_tmp, s);
-  int32_t nitems;
-  NB_HIP(hipMemcpyAsync(&nitems, poff + P.n_src, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  double *trans = dget<double>(c->trans, (size_t)16 * KP * KP);
+  hipLaunchKernelGGL(k_build_trans, dim3(8, 2), dim3(kT), 0, s, p, KP, hx, hy, hz, trans,
+                     trans + (size_t)8 * KP * KP);
+  NB_LAUNCH();
+  if (f32) trans_to_f32(c, KP);
+  // ---- plan: modes per level (top-down), slots, lists ----
+  int32_t *mode_m = dget<int32_t>(c->mode_m, (size_t)nc + 1);
+  int32_t *mode_l = dget<int32_t>(c->mode_l, (size_t)nc + 1);
+  for (int d = 0; d <= D; ++d) {
+    const int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
+    hipLaunchKernelGGL(k_updown_mark, dim3(nblk(ce - cb)), dim3(kT), 0, s, cb, ce, leaf_begin,
+                       (const int32_t *)c->cparent.p, cbeg, cend, (const int32_t *)c->is_src.p,
+                       (const int32_t *)c->cnt_m2l.p, mode_m, mode_l);
+    NB_LAUNCH();
+  }
+  const int64_t s5 = nc + 1;
+  int32_t *flags = dget<int32_t>(c->flags, (size_t)5 * s5);
+  int32_t *fscan = dget<int32_t>(c->fscan, (size_t)5 * s5);
+  hipLaunchKernelGGL(k_updown_flags, dim3(nblk(nc + 1)), dim3(kT), 0, s, (const int32_t *)mode_m,
+                     (const int32_t *)mode_l, nc, flags);
+  NB_LAUNCH();
+  for (int k = 0; k < 5; ++k) scan_excl_i32(flags + k * s5, fscan + k * s5, s5, c->scan_tmp, s);
+  int32_t *mslot = dget<int32_t>(c->mslot, (size_t)nc + 1);
+  int32_t *lslot = dget<int32_t>(c->lslot, (size_t)nc + 1);
+  int32_t *eslot = dget<int32_t>(c->eslot, (size_t)nc + 1);
+  int32_t *lists = dget<int32_t>(c->up_cells, (size_t)3 * s5);
+  int32_t *p2m_cells = lists, *m2m_cells = lists + s5, *l2l_cells = lists + 2 * s5;
+  hipLaunchKernelGGL(k_updown_lists, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)mode_l, (const int32_t *)flags,
+                     (const int32_t *)fscan, nc, mslot, lslot, eslot, p2m_cells, m2m_cells, l2l_cells);
+  NB_LAUNCH();
+  // totals and the per-level starts of the M2M and L2L lists (cells are level ordered)
+  int32_t *h = (int32_t *)c->hred.p;
+  for (int k = 0; k < 5; ++k)
+    NB_HIP(hipMemcpyAsync(h + k, fscan + k * s5 + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  for (int d = 0; d <= D + 1; ++d) {
+    NB_HIP(hipMemcpyAsync(h + 8 + d, fscan + 2 * s5 + P.lvl_start[d], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NB_HIP(hipMemcpyAsync(h + 32 + d, fscan + 4 * s5 + P.lvl_start[d], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  }
   unsigned long long *terms = (unsigned long long *)((double *)c->red.p + 6 * 1024 + 48);
   NB_HIP(hipMemsetAsync(terms, 0, 2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_terms, dim3(nblk(P.n_src)), dim3(kT), 0, s, (const int32_t *)c->src_cells.p, P.n_src, cbeg,
-                     cend, terms);
+  hipLaunchKernelGGL(k_terms_mode, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)mode_m, 1, nc, cbeg, cend, terms);
   NB_LAUNCH();
-  hipLaunchKernelGGL(k_terms, dim3(nblk(P.n_tgt)), dim3(kT), 0, s, (const int32_t *)c->tgt_cells.p, P.n_tgt, cbeg,
-                     cend, terms + 1);
+  hipLaunchKernelGGL(k_terms_mode, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)mode_l, 1, nc, cbeg, cend,
+                     terms + 1);
   NB_LAUNCH();
+  NB_HIP(hipStreamSynchronize(s));
+  const int32_t nM = h[0], nP2M = h[1], nL = h[3];
+  int32_t bm[kMaxDepth + 2], bl[kMaxDepth + 2];
+  for (int d = 0; d <= D + 1; ++d) {
+    bm[d] = h[8 + d];
+    bl[d] = h[32 + d];
+  }
   unsigned long long th[2];
   NB_HIP(hipMemcpyAsync(th, terms, sizeof(th), hipMemcpyDeviceToHost, s));
+  // ---- P2M (cells that form their multipole from particles) ----
+  int32_t *pcnt = dget<int32_t>(c->p2m_off, (size_t)nP2M * 2 + 2 + 8);
+  int32_t *poff = pcnt + nP2M + 1;
+  hipLaunchKernelGGL(k_p2m_count, dim3(nblk(nP2M)), dim3(kT), 0, s, (const int32_t *)p2m_cells, (int64_t)nP2M, cbeg,
+                     cend, pcnt);
+  NB_LAUNCH();
+  NB_HIP(hipMemsetAsync(pcnt + nP2M, 0, sizeof(int32_t), s));
+  scan_excl_i32(pcnt, poff, nP2M + 1, c->scan_tmp, s);
+  int32_t nitems;
+  NB_HIP(hipMemcpyAsync(&nitems, poff + nP2M, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   P.n_p2m_items = nitems;
   P.n_p2m_terms = (int64_t)th[0];
   P.n_l2p_terms = (int64_t)th[1];
   int4 *items = dget<int4>(c->p2m_items, (size_t)nitems + 1);
-  hipLaunchKernelGGL(k_p2m_items, dim3(nblk(P.n_src)), dim3(kT), 0, s, (const int32_t *)c->src_cells.p, P.n_src, cbeg,
+  hipLaunchKernelGGL(k_p2m_items, dim3(nblk(nP2M)), dim3(kT), 0, s, (const int32_t *)p2m_cells, (int64_t)nP2M, cbeg,
                      cend, poff, items);
   NB_LAUNCH();
-  const size_t esz = f32 ? sizeof(float) : sizeof(double);
-  void *Mexp = dbuf_ensure(c->mexp, (size_t)P.n_src * KP * esz + 16);
-  NB_HIP(hipMemsetAsync(Mexp, 0, esz * (size_t)P.n_src * KP, s));
-  if (f32)
-    hipLaunchKernelGGL(k_p2m<float>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
-                       (const int32_t *)c->src_cells.p, ckey, g, dC, (float *)Mexp);
-  else
-    hipLaunchKernelGGL(k_p2m<double>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
-                       (const int32_t *)c->src_cells.p, ckey, g, dC, (double *)Mexp);
-  NB_LAUNCH();
+  void *Mexp = dbuf_ensure(c->mexp, (size_t)nM * KP * esz + 16);
+  NB_HIP(hipMemsetAsync(Mexp, 0, esz * (size_t)nM * KP, s));
+  if (nitems > 0) {
+    if (f32)
+      hipLaunchKernelGGL(k_p2m<float>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
+                         (const int32_t *)p2m_cells, (const int32_t *)mslot, ckey, g, dC, (float *)Mexp);
+    else
+      hipLaunchKernelGGL(k_p2m<double>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
+                         (const int32_t *)p2m_cells, (const int32_t *)mslot, ckey, g, dC, (double *)Mexp);
+    NB_LAUNCH();
+  }
+  // ---- M2M, finest level first ----
+  for (int d = D - 1; d >= 0; --d)
+    run_trans(c, false, KP, f32, bm[d + 1] - bm[d], m2m_cells + bm[d]);
   mark(c, 3);
-  // ---- M2L ----
-  // offset classes scaled by (hx, hy, hz) / rho (level independent)
-  const double rho = P.geo.rho(D);
-  double *frag = dget<double>(c->tmat, (size_t)kNumClasses * KP * KP);
-  hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, P.geo.h(D, 0) / rho,
-                     P.geo.h(D, 1) / rho, P.geo.h(D, 2) / rho, frag);
-  NB_LAUNCH();
+  // ---- M2L (writes the locals of its targets) ----
+  dbuf_ensure(c->lexp, (size_t)nL * KP * esz + 16);
+  NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));
   run_m2l(c, KP, f32);
+  // ---- L2L, coarsest level first ----
+  for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
   mark(c, 4);
   // ---- L2P ----
   if (f32)
     hipLaunchKernelGGL(k_l2p<float>, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
-                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)c->tgt_slot.p,
+                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)eslot,
                        g, dC, (const float *)c->lexp.p, grad);
   else
     hipLaunchKernelGGL(k_l2p<double>, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
-                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)c->tgt_slot.p,
+                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)eslot,
                        g, dC, (const double *)c->lexp.p, grad);
   NB_LAUNCH();
   mark(c, 5);

@@ -163,13 +163,14 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
 template <class T>
 __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
-                                                const T *__restrict__ Y, T *__restrict__ Lexp) {
+                                                const int32_t *__restrict__ lslot, const T *__restrict__ Y,
+                                                T *__restrict__ Lexp) {
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (t >= ntgt) return;
   const int32_t c = tgt_cells[t];
   const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
-  T *L = Lexp + (size_t)t * KP; // target slot t holds cell tgt_cells[t]
+  T *L = Lexp + (size_t)lslot[c] * KP;
   for (int row = lane; row < KP; row += 64) {
     double s = 0;
     for (int32_t q = b; q < e; ++q) s += (double)Y[(size_t)q * KP + row];
@@ -193,7 +194,7 @@ void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t 
     attr_set[c->device & 63] = true;
   }
   hipLaunchKernelGGL((k_m2l_cls<KP, RBW, T>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(m2l_threads<T>()), lds, c->stream,
-                     items, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->src_slot.p, frag,
+                     items, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, frag,
                      (const T *)c->mexp.p, (T *)c->ypair.p);
   NB_LAUNCH();
 }
@@ -229,6 +230,117 @@ void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, const int3
     default:
       throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
   }
+}
+
+// ---- M2M and L2L on MFMA ---------------------------------------------------------------
+// Parent-stationary: a wave takes 16 parents of one level (the MFMA columns)
+// and runs the 8 octant operators (8 KP x KP matrices, small enough to stay in
+// the caches, read in A-fragment order).  M2M sums the children's multipoles
+// into the parents' (columns of a missing child are zero); L2L adds the
+// translated parent local to every child's local.
+template <int KP, class T>
+__global__ void __launch_bounds__(256) k_m2m(int32_t ncell, const int32_t *__restrict__ cells,
+                                             const int32_t *__restrict__ child8, const int32_t *__restrict__ mslot,
+                                             const T *__restrict__ frag8, T *__restrict__ Mexp) {
+  constexpr int RB = KP / 16, KS = KP / 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, kg = lane >> 4;
+  const int pi = ((int)blockIdx.x * 4 + wave) * 16 + col;
+  const int32_t cell = pi < ncell ? cells[pi] : -1;
+  typename Mfma<T>::V acc[RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) acc[r] = typename Mfma<T>::V{0, 0, 0, 0};
+  for (int o = 0; o < 8; ++o) {
+    const int32_t ch = cell >= 0 ? child8[(int64_t)cell * 8 + o] : -1;
+    if (!__any(ch >= 0)) continue;
+    const T *Mb = Mexp + (size_t)(ch >= 0 ? mslot[ch] : 0) * KP + kg;
+    const T *A = frag8 + (size_t)o * KP * KP + lane;
+#pragma unroll 1
+    for (int k0 = 0; k0 < KS; k0 += 4) {
+      T b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b[u] = ch >= 0 ? Mb[(k0 + u) * 4] : (T)0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r] = Mfma<T>::op(A[(size_t)(r * KS + k0 + u) * 64], b[u], acc[r]);
+    }
+  }
+  if (cell < 0) return;
+  T *M = Mexp + (size_t)mslot[cell] * KP;
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) M[r * 16 + Mfma<T>::row(kg, q)] = acc[r][q];
+}
+
+template <int KP, class T>
+__global__ void __launch_bounds__(256) k_l2l(int32_t ncell, const int32_t *__restrict__ cells,
+                                             const int32_t *__restrict__ child8, const int32_t *__restrict__ lslot,
+                                             const T *__restrict__ frag8, T *__restrict__ Lexp) {
+  constexpr int RB = KP / 16, KS = KP / 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, kg = lane >> 4;
+  const int pi = ((int)blockIdx.x * 4 + wave) * 16 + col;
+  const int32_t cell = pi < ncell ? cells[pi] : -1;
+  const T *Lb = Lexp + (size_t)(cell >= 0 ? lslot[cell] : 0) * KP + kg;
+  for (int o = 0; o < 8; ++o) {
+    const int32_t ch = cell >= 0 ? child8[(int64_t)cell * 8 + o] : -1;
+    if (!__any(ch >= 0)) continue;
+    typename Mfma<T>::V acc[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) acc[r] = typename Mfma<T>::V{0, 0, 0, 0};
+    const T *A = frag8 + (size_t)o * KP * KP + lane;
+#pragma unroll 1
+    for (int k0 = 0; k0 < KS; k0 += 4) {
+      T b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b[u] = cell >= 0 ? Lb[(k0 + u) * 4] : (T)0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r] = Mfma<T>::op(A[(size_t)(r * KS + k0 + u) * 64], b[u], acc[r]);
+    }
+    if (ch >= 0) {
+      T *L = Lexp + (size_t)lslot[ch] * KP;
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) L[r * 16 + Mfma<T>::row(kg, q)] += acc[r][q];
+    }
+  }
+}
+
+template <class T>
+void launch_trans(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *cells, const int32_t *slot,
+                  const T *frag8, T *X) {
+  if (ncell <= 0) return;
+  const unsigned nb = (unsigned)((ncell + 63) / 64);
+  const int32_t *child8 = (const int32_t *)c->child8.p;
+  switch (KP) {
+#define NB_TR_CASE(k)                                                                                      \
+  case k:                                                                                                  \
+    if (l2l)                                                                                               \
+      hipLaunchKernelGGL((k_l2l<k, T>), dim3(nb), dim3(256), 0, c->stream, ncell, cells, child8, slot, frag8, X); \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_m2m<k, T>), dim3(nb), dim3(256), 0, c->stream, ncell, cells, child8, slot, frag8, X); \
+    break;
+    NB_TR_CASE(16)
+    NB_TR_CASE(32)
+    NB_TR_CASE(48)
+    NB_TR_CASE(64)
+    NB_TR_CASE(96)
+    NB_TR_CASE(112)
+    NB_TR_CASE(128)
+    NB_TR_CASE(144)
+    NB_TR_CASE(176)
+    NB_TR_CASE(208)
+    NB_TR_CASE(240)
+#undef NB_TR_CASE
+    default:
+      throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
+  }
+  NB_LAUNCH();
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
@@ -274,7 +386,7 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32) {
   NB_HIP(hipMemcpyAsync(ditems, hitems.data(), sizeof(int4) * nitems, hipMemcpyHostToDevice, s));
   P.n_m2l_tiles = nitems;
   dbuf_ensure(c->ypair, (size_t)npair * KP * esz + 16);
-  void *Lexp = dbuf_ensure(c->lexp, (size_t)nt * KP * esz + 16);
+  void *Lexp = c->lexp.p; // sized and cleared by the caller (one slot per cell that holds a local)
   const int64_t nfrag = (int64_t)kNumClasses * KP * KP;
   if (f32) {
     float *frag32 = dget<float>(c->tmat32, (size_t)nfrag);
@@ -285,15 +397,32 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32) {
     mark(c, 8);
     hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                        (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                       (const float *)c->ypair.p, (float *)Lexp);
+                       (const int32_t *)c->lslot.p, (const float *)c->ypair.p, (float *)Lexp);
   } else {
     mark(c, 7);
     launch<double>(c, KP, nitems, ditems, cls_pairs, (const double *)c->tmat.p);
     mark(c, 8);
     hipLaunchKernelGGL(k_m2l_sum<double>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                        (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                       (const double *)c->ypair.p, (double *)Lexp);
+                       (const int32_t *)c->lslot.p, (const double *)c->ypair.p, (double *)Lexp);
   }
+  NB_LAUNCH();
+}
+
+void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells) {
+  const int32_t *slot = (const int32_t *)(l2l ? c->lslot.p : c->mslot.p);
+  void *X = l2l ? c->lexp.p : c->mexp.p;
+  const size_t off = l2l ? (size_t)8 * KP * KP : 0;
+  if (f32)
+    launch_trans<float>(c, l2l, KP, ncell, cells, slot, (const float *)c->trans32.p + off, (float *)X);
+  else
+    launch_trans<double>(c, l2l, KP, ncell, cells, slot, (const double *)c->trans.p + off, (double *)X);
+}
+
+void trans_to_f32(nbfmm_ctx *c, int KP) {
+  const int64_t n = (int64_t)16 * KP * KP;
+  float *t32 = dget<float>(c->trans32, (size_t)n);
+  hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(kT), 0, c->stream, (const double *)c->trans.p, n, t32);
   NB_LAUNCH();
 }
 

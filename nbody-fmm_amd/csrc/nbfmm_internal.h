@@ -117,7 +117,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf sub_pairs, p2m_items, p2m_off, tiles;
   nbfmm::DBuf tasks, task_off, task_cnt, dpart;
   // expansions
-  nbfmm::DBuf stab, tmat, tmat32, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
+  nbfmm::DBuf stab, tmat, tmat32, trans, trans32, mode_m, mode_l, mslot, lslot, eslot, up_cells, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
   std::vector<int32_t> h_cls_off; // host schedule of the class-sorted M2L
   std::vector<int4> h_m2l_items;
   // integrator scratch
@@ -140,6 +140,9 @@ void build_lists(nbfmm_ctx *c);
 void run_expansions(nbfmm_ctx *c);
 // m2l.hip
 void run_m2l(nbfmm_ctx *c, int KP, bool f32);
+// M2M (l2l false) or L2L (true) for `ncell` parents listed in `cells` (device)
+void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells);
+void trans_to_f32(nbfmm_ctx *c, int KP);
 // p2p.hip
 void run_p2p(nbfmm_ctx *c, double *d_acc_out);
 void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,

@@ -28,17 +28,24 @@ constexpr int kW = 64; // one wavefront per task
 static_assert(kTaskTargets == 4 * kW, "a P2P task holds at most four targets per lane");
 
 // A particle meets itself (and coincident particles meet each other) with
-// r2 = 0; those pairs contribute nothing.
+// r2 = 0; those pairs contribute nothing (a select, no branch).
+// 1/sqrt(r2) for positive finite r2: v_rsq_f64 estimate y refined by the
+// third-order step y + y e (1/2 + 3e/8), e = 1 - r2 y^2 (the Taylor series of
+// (1 - e)^(-1/2)), whose error is O(e^3).
+__device__ __forceinline__ double rsqrt_pos(double r2) {
+  const double y = __builtin_amdgcn_rsq(r2);
+  const double e = fma(-r2 * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 __device__ __forceinline__ void pair_acc(const double4 &xi, const double4 &xj, double &ax, double &ay, double &az) {
   const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
-  if (r2 > 0.0) {
-    const double inv = rsqrt(r2);
-    const double w = xj.w * inv * inv * inv;
-    ax -= dx * w;
-    ay -= dy * w;
-    az -= dz * w;
-  }
+  const double inv = r2 > 0.0 ? rsqrt_pos(r2) : 0.0;
+  const double w = xj.w * inv * inv * inv;
+  ax -= dx * w;
+  ay -= dy * w;
+  az -= dz * w;
 }
 
 __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ tasks, const int32_t *__restrict__ nbr27,
@@ -102,6 +109,7 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
     double ax = 0, ay = 0, az = 0;
     for (int32_t fb = f0; fb < f1; fb += kW) {
       const int cnt = stage(fb);
+#pragma unroll 4
       for (int k = si; k < cnt; k += sub) pair_acc(xi, sh[k], ax, ay, az);
     }
     for (int o = sub >> 1; o >= 1; o >>= 1) {
@@ -170,13 +178,11 @@ __device__ __forceinline__ void pair_acc_f(const f2pos &xi, const float4 &jh, co
   const float dy = (xi.hi.y - jh.y) + (xi.lo.y - jl.y);
   const float dz = (xi.hi.z - jh.z) + (xi.lo.z - jl.z);
   const float r2 = dx * dx + dy * dy + dz * dz;
-  if (r2 > 0.0f) {
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float w = jh.w * inv * inv * inv;
-    ax -= dx * w;
-    ay -= dy * w;
-    az -= dz * w;
-  }
+  const float inv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+  const float w = jh.w * inv * inv * inv;
+  ax -= dx * w;
+  ay -= dy * w;
+  az -= dz * w;
 }
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
@@ -266,6 +272,7 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks_f32(const P2PTask *__restrict_
     for (int32_t fb = f0; fb < f1; fb += kW) {
       const int cnt = stage(fb);
       float ax = 0, ay = 0, az = 0;
+#pragma unroll 4
       for (int k = si; k < cnt; k += sub) pair_acc_f(xi, shh[k], shl[k], ax, ay, az);
       kahan_add(sx, cx, ax);
       kahan_add(sy, cy, ay);

@@ -1,6 +1,6 @@
-"""Accuracy-gated choice of the expansion order and the P2P cut.
+"""Accuracy-gated choice of the expansion order, the P2P cut and the leaf capacity.
 
-For a particle set resident on the device, every (order, p2p_cut) candidate
+For a particle set resident on the device, every (order, p2p_cut, leaf capacity) candidate
 is evaluated (one warm-up evaluation, then one timed with the library's
 events), its relative force error is measured against the FP64 direct sum
 (compensated, eps2 = 0) on sampled targets, and the fastest candidate whose
@@ -33,22 +33,27 @@ def rel_errors(acc_t, ref):
             "p99": float(np.percentile(e_i, 99))}
 
 
-def tune(ctx, n, d_xyz, d_m, d_acc, tgt, ref, orders, cuts, gate=1e-6, precision=0, log=None):
-    """Returns (best, table): best = dict(order, p2p_cut, ms, err) or None."""
+def tune(ctx, n, d_xyz, d_m, d_acc, tgt, ref, orders, cuts, leaves=(None,), gate=1e-6, precision=0, log=None):
+    """Returns (best, table): best = dict(order, p2p_cut, leaf_capacity, ms, err) or None."""
     table = []
     best = None
-    for p in orders:
-        for cut in cuts:
-            ctx.set_params(order=p, p2p_cut=cut, precision=precision)
-            ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
-            ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
-            ms = ctx.stats().ms_total
-            acc = d_acc.download(ctx.stream)
-            err = rel_errors(acc[tgt], ref)
-            row = {"order": p, "p2p_cut": cut, "ms": ms, "err": err}
-            table.append(row)
-            if log:
-                log(f"[tune] precision={precision} p={p} cut={cut}: {ms:.2f} ms, L2 err {err['l2']:.3e}")
-            if err["l2"] <= gate and (best is None or ms < best["ms"]):
-                best = row
+    for leaf in leaves:
+        if leaf is not None:
+            ctx.set_params(leaf_capacity=leaf)
+        for p in orders:
+            for cut in cuts:
+                ctx.set_params(order=p, p2p_cut=cut, precision=precision)
+                ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
+                ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
+                ms = ctx.stats().ms_total
+                acc = d_acc.download(ctx.stream)
+                err = rel_errors(acc[tgt], ref)
+                row = {"order": p, "p2p_cut": cut, "leaf_capacity": int(ctx.params.leaf_capacity), "ms": ms,
+                       "err": err}
+                table.append(row)
+                if log:
+                    log(f"[tune] precision={precision} leaf={row['leaf_capacity']} p={p} cut={cut}: {ms:.2f} ms, "
+                        f"L2 err {err['l2']:.3e}")
+                if err["l2"] <= gate and (best is None or ms < best["ms"]):
+                    best = row
     return best, table
