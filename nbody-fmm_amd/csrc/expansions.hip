@@ -24,6 +24,7 @@
 // (m2l.hip); P2M and L2P run on the vector ALUs.
 #include "nbfmm_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -576,8 +577,10 @@ __global__ void k_updown_lists(const int32_t *__restrict__ mode_l, const int32_t
 __global__ void k_terms_mode(const int32_t *mode, int32_t v, int64_t ncell, const int32_t *cbeg, const int32_t *cend,
                              unsigned long long *out) {
   __shared__ unsigned long long s[kT];
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  s[threadIdx.x] = i < ncell && mode[i] == v ? (unsigned long long)(cend[i] - cbeg[i]) : 0ull;
+  unsigned long long a = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < ncell; i += (int64_t)gridDim.x * kT)
+    if (mode[i] == v) a += (unsigned long long)(cend[i] - cbeg[i]);
+  s[threadIdx.x] = a;
   __syncthreads();
   for (int o = kT / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
@@ -680,10 +683,11 @@ void run_expansions(nbfmm_ctx *c) {
   }
   unsigned long long *terms = (unsigned long long *)((double *)c->red.p + 6 * 1024 + 48);
   NB_HIP(hipMemsetAsync(terms, 0, 2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_terms_mode, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)mode_m, 1, nc, cbeg, cend, terms);
+  hipLaunchKernelGGL(k_terms_mode, dim3(std::min<unsigned>(1024, nblk(nc))), dim3(kT), 0, s, (const int32_t *)mode_m, 1, nc,
+                     cbeg, cend, terms);
   NB_LAUNCH();
-  hipLaunchKernelGGL(k_terms_mode, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)mode_l, 1, nc, cbeg, cend,
-                     terms + 1);
+  hipLaunchKernelGGL(k_terms_mode, dim3(std::min<unsigned>(1024, nblk(nc))), dim3(kT), 0, s, (const int32_t *)mode_l, 1, nc,
+                     cbeg, cend, terms + 1);
   NB_LAUNCH();
   NB_HIP(hipStreamSynchronize(s));
   const int32_t nM = h[0], nP2M = h[1], nL = h[3];

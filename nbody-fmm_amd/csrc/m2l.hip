@@ -171,11 +171,22 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
   const int32_t c = tgt_cells[t];
   const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
   T *L = Lexp + (size_t)lslot[c] * KP;
-  for (int row = lane; row < KP; row += 64) {
-    double s = 0;
-    for (int32_t q = b; q < e; ++q) s += (double)Y[(size_t)q * KP + row];
-    L[row] = (T)s;
-  }
+  // rows lane, lane + 64, ... (KP <= 256), pairs four at a time
+  double s[4] = {0, 0, 0, 0};
+  int32_t q = b;
+  for (; q + 4 <= e; q += 4)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)(q + u) * KP + lane + 64 * r];
+  for (; q < e; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)q * KP + lane + 64 * r];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    if (lane + 64 * r < KP) L[lane + 64 * r] = (T)s[r];
 }
 
 template <class T>

@@ -10,10 +10,9 @@
 // source cells (a 27-neighbour row, or a cell's directly evaluated
 // interaction-list cells).  One wavefront runs a task: it prefix-sums the
 // list's cell sizes into LDS, then stages the sources 64 at a time (each
-// lane finds its source's cell by binary search) and every target lane reads
-// the staged chunk (broadcast), so lanes stay busy however small the cells.
-// Up to 64 targets split the lanes as (target, source slot) and combine the
-// slots with xor-shuffles; more targets are held four per lane.  Tasks of
+// lane finds its source's cell by binary search), so lanes stay busy however
+// small the cells.  Every lane holds four targets and a source slot; the
+// slots are combined with xor-shuffles at the end.  Tasks of
 // one cell and of cells at different levels add into the same targets, so
 // the partial sums are added with FP64 atomics to the far-field gradient of
 // the L2P pass; k_finish writes acc = G * grad in input order
@@ -38,26 +37,96 @@ __device__ __forceinline__ double rsqrt_pos(double r2) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
-__device__ __forceinline__ void pair_acc(const double4 &xi, const double4 &xj, double &ax, double &ay, double &az) {
-  const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  const double inv = r2 > 0.0 ? rsqrt_pos(r2) : 0.0;
-  const double w = xj.w * inv * inv * inv;
-  ax -= dx * w;
-  ay -= dy * w;
-  az -= dz * w;
-}
+// FP64: positions as stored, FP64 sums.
+struct PairF64 {
+  typedef double4 S;
+  typedef double A;
+  static constexpr bool kAnchor = false;
+  static __device__ __forceinline__ S make(const double4 &p, const double4 &) { return p; }
+  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az) {
+    const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double inv = r2 > 0.0 ? rsqrt_pos(r2) : 0.0;
+    const double w = xj.w * inv * inv * inv;
+    ax -= dx * w;
+    ay -= dy * w;
+    az -= dz * w;
+  }
+  static __device__ __forceinline__ void add(A &s, A &, A v) { s += v; }
+};
 
+// FP32 (mixed precision): offsets from an anchor (the mean of the task's
+// targets) formed in FP64 and kept as FP32 pairs hi + lo; the separation
+// (xi_hi - xj_hi) + (xi_lo - xj_lo) is accurate to a few FP32 ulps of the
+// separation itself however large the offsets are.  v_rsq_f32.  A chunk's
+// contributions (at most 64 sources) are summed in FP32 and the chunk sums
+// added with Kahan compensation into an FP32 (sum, correction) pair.
+struct f2pos {
+  float4 hi; // offset (x, y, z) and the mass
+  float4 lo; // remainder of the offset
+};
+struct PairF32 {
+  typedef f2pos S;
+  typedef float A;
+  static constexpr bool kAnchor = true;
+  static __device__ __forceinline__ S make(const double4 &p, const double4 &a) {
+    const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
+    const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+    return {make_float4(hx, hy, hz, (float)p.w),
+            make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.0f)};
+  }
+  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az) {
+    const float dx = (xi.hi.x - xj.hi.x) + (xi.lo.x - xj.lo.x);
+    const float dy = (xi.hi.y - xj.hi.y) + (xi.lo.y - xj.lo.y);
+    const float dz = (xi.hi.z - xj.hi.z) + (xi.lo.z - xj.lo.z);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float inv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    const float w = xj.hi.w * inv * inv * inv;
+    ax -= dx * w;
+    ay -= dy * w;
+    az -= dz * w;
+  }
+  static __device__ __forceinline__ void add(A &s, A &c, A v) { // Kahan
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+// Lanes: tl target lanes (power of two, 4 targets each: t = ti + tl q) times
+// 64 / tl source slots; a staged chunk of 64 sources is read once per 4 pairs
+// and the slots are combined with xor-shuffles at the end.
+template <class Tr>
 __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ tasks, const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad) {
-  __shared__ double4 sh[kW];
+  typedef typename Tr::S S;
+  typedef typename Tr::A A;
+  __shared__ S sh[kW];
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   const int lane = threadIdx.x;
   const P2PTask tk = tasks[blockIdx.x];
   const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+  double4 anchor = make_double4(0, 0, 0, 0);
+  if (Tr::kAnchor) {
+    for (int t = lane; t < nc; t += kW) {
+      const double4 q = pos[b + t];
+      anchor.x += q.x;
+      anchor.y += q.y;
+      anchor.z += q.z;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      anchor.x += __shfl_xor(anchor.x, off);
+      anchor.y += __shfl_xor(anchor.y, off);
+      anchor.z += __shfl_xor(anchor.z, off);
+    }
+    anchor.x /= nc;
+    anchor.y /= nc;
+    anchor.z /= nc;
+  }
   const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
   const int ne = tk.nlist;
   // first concatenated index of every list cell
@@ -79,8 +148,21 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
     carry += __shfl(x, kW - 1);
   }
   const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
-  // stage sources [fb, fb + 64) of the concatenation; returns their count
-  auto stage = [&](int32_t fb) -> int {
+  int tl = 1;
+  while (tl * 4 < nc) tl <<= 1;
+  const int sub = kW / tl, ti = lane & (tl - 1), si = lane / tl;
+  S xi[4];
+  int32_t ii[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int32_t t = ti + tl * q;
+    ii[q] = t < nc ? b + t : -1;
+    xi[q] = Tr::make(pos[t < nc ? b + t : b], anchor);
+  }
+  A sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0};
+  A cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0};
+  for (int32_t fb = f0; fb < f1; fb += kW) {
+    // stage sources [fb, fb + 64) of the concatenation
     const int32_t g = fb + lane;
     __syncthreads();
     if (g < f1) {
@@ -92,237 +174,38 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
         else
           hi = mid;
       }
-      const int32_t j = lb[lo] + (g - lp[lo]);
-      sh[lane] = pos[j];
+      sh[lane] = Tr::make(pos[lb[lo] + (g - lp[lo])], anchor);
     }
     __syncthreads();
-    return min(kW, f1 - fb);
-  };
-  if (nc <= kW) {
-    int np2 = 1;
-    while (np2 < nc) np2 <<= 1;
-    const int sub = kW / np2;
-    const int ti = lane / sub, si = lane % sub;
-    const bool valid = ti < nc;
-    const int32_t i = b + (valid ? ti : 0);
-    const double4 xi = pos[i];
-    double ax = 0, ay = 0, az = 0;
-    for (int32_t fb = f0; fb < f1; fb += kW) {
-      const int cnt = stage(fb);
-#pragma unroll 4
-      for (int k = si; k < cnt; k += sub) pair_acc(xi, sh[k], ax, ay, az);
-    }
-    for (int o = sub >> 1; o >= 1; o >>= 1) {
-      ax += __shfl_xor(ax, o);
-      ay += __shfl_xor(ay, o);
-      az += __shfl_xor(az, o);
-    }
-    if (valid && si == 0) {
-      atomicAdd(&grad[3 * (int64_t)i], ax);
-      atomicAdd(&grad[3 * (int64_t)i + 1], ay);
-      atomicAdd(&grad[3 * (int64_t)i + 2], az);
-    }
-    return;
-  }
-  // up to 4 targets per lane, every lane reads the same staged source
-  double4 xi[4];
-  double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
-  int32_t ii[4];
+    const int cnt = min(kW, f1 - fb);
+    A ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
+    for (int k = si; k < cnt; k += sub) {
+      const S xj = sh[k];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int32_t t = lane + q * kW;
-    ii[q] = t < nc ? b + t : -1;
-    xi[q] = pos[t < nc ? b + t : b];
-  }
-  for (int32_t fb = f0; fb < f1; fb += kW) {
-    const int cnt = stage(fb);
-    for (int k = 0; k < cnt; ++k) {
-      const double4 xj = sh[k];
+      for (int q = 0; q < 4; ++q) Tr::pair(xi[q], xj, ax[q], ay[q], az[q]);
+    }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) pair_acc(xi[q], xj, ax[q], ay[q], az[q]);
+    for (int q = 0; q < 4; ++q) {
+      Tr::add(sx[q], cx[q], ax[q]);
+      Tr::add(sy[q], cy[q], ay[q]);
+      Tr::add(sz[q], cz[q], az[q]);
     }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-    if (ii[q] >= 0) {
-      atomicAdd(&grad[3 * (int64_t)ii[q]], ax[q]);
-      atomicAdd(&grad[3 * (int64_t)ii[q] + 1], ay[q]);
-      atomicAdd(&grad[3 * (int64_t)ii[q] + 2], az[q]);
-    }
-}
-
-// ---- P2P in FP32 (mixed precision) ----------------------------------------------------
-// Same tasks and staging as k_p2p_tasks.  Positions are offsets from an
-// anchor (the mean of the task's targets), formed in FP64 and kept as FP32
-// pairs hi + lo; the separation (xi_hi - xj_hi) + (xi_lo - xj_lo) is then
-// accurate to a few FP32 ulps of the separation itself, however large the
-// offsets are.  The pair term uses v_rsq_f32.  A target's contributions from
-// one 64-source chunk are summed in FP32 (at most 64 terms per lane); the
-// chunk sums are added with Kahan compensation into an FP32 (sum,
-// correction) pair, and the slots and tasks are combined in FP64.
-struct f2pos {
-  float4 hi; // offset (x, y, z) and the mass
-  float4 lo; // remainder of the offset
-};
-
-__device__ __forceinline__ f2pos split(const double4 &p, const double4 &a) {
-  const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
-  const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
-  return {make_float4(hx, hy, hz, (float)p.w),
-          make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.0f)};
-}
-
-__device__ __forceinline__ void pair_acc_f(const f2pos &xi, const float4 &jh, const float4 &jl, float &ax, float &ay,
-                                           float &az) {
-  const float dx = (xi.hi.x - jh.x) + (xi.lo.x - jl.x);
-  const float dy = (xi.hi.y - jh.y) + (xi.lo.y - jl.y);
-  const float dz = (xi.hi.z - jh.z) + (xi.lo.z - jl.z);
-  const float r2 = dx * dx + dy * dy + dz * dz;
-  const float inv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-  const float w = jh.w * inv * inv * inv;
-  ax -= dx * w;
-  ay -= dy * w;
-  az -= dz * w;
-}
-
-__device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
-  const float y = v - c;
-  const float t = s + y;
-  c = (t - s) - y;
-  s = t;
-}
-
-__global__ void __launch_bounds__(kW) k_p2p_tasks_f32(const P2PTask *__restrict__ tasks,
-                                                      const int32_t *__restrict__ nbr27,
-                                                      const int32_t *__restrict__ sub_src,
-                                                      const int32_t *__restrict__ cbeg,
-                                                      const int32_t *__restrict__ cend,
-                                                      const double4 *__restrict__ pos, double *__restrict__ grad) {
-  __shared__ float4 shh[kW];
-  __shared__ float4 shl[kW];
-  __shared__ int32_t lb[kMaxList];
-  __shared__ int32_t lp[kMaxList];
-  const int lane = threadIdx.x;
-  const P2PTask tk = tasks[blockIdx.x];
-  const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
-  // anchor: the mean position of the task's targets
-  double4 anchor = make_double4(0, 0, 0, 0);
-  for (int t = lane; t < nc; t += kW) {
-    const double4 q = pos[b + t];
-    anchor.x += q.x;
-    anchor.y += q.y;
-    anchor.z += q.z;
-  }
-  for (int off = 32; off >= 1; off >>= 1) {
-    anchor.x += __shfl_xor(anchor.x, off);
-    anchor.y += __shfl_xor(anchor.y, off);
-    anchor.z += __shfl_xor(anchor.z, off);
-  }
-  anchor.x /= nc;
-  anchor.y /= nc;
-  anchor.z /= nc;
-  const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
-  const int ne = tk.nlist;
-  int32_t carry = 0;
-  for (int base = 0; base < ne; base += kW) {
-    const int e = base + lane;
-    const int32_t q = e < ne ? list[e] : -1;
-    const int32_t sb = q >= 0 ? cbeg[q] : 0;
-    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
-    int32_t x = cnt;
-    for (int off = 1; off < kW; off <<= 1) {
-      const int32_t y = __shfl_up(x, off);
-      if (lane >= off) x += y;
-    }
-    if (e < ne) {
-      lb[e] = sb;
-      lp[e] = carry + x - cnt;
-    }
-    carry += __shfl(x, kW - 1);
-  }
-  const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
-  auto stage = [&](int32_t fb) -> int {
-    const int32_t g = fb + lane;
-    __syncthreads();
-    if (g < f1) {
-      int lo = 0, hi = ne;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (lp[mid] <= g)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      const f2pos v = split(pos[lb[lo] + (g - lp[lo])], anchor);
-      shh[lane] = v.hi;
-      shl[lane] = v.lo;
-    }
-    __syncthreads();
-    return min(kW, f1 - fb);
-  };
-  if (nc <= kW) {
-    int np2 = 1;
-    while (np2 < nc) np2 <<= 1;
-    const int sub = kW / np2;
-    const int ti = lane / sub, si = lane % sub;
-    const bool valid = ti < nc;
-    const int32_t i = b + (valid ? ti : 0);
-    const f2pos xi = split(pos[i], anchor);
-    float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
-    for (int32_t fb = f0; fb < f1; fb += kW) {
-      const int cnt = stage(fb);
-      float ax = 0, ay = 0, az = 0;
-#pragma unroll 4
-      for (int k = si; k < cnt; k += sub) pair_acc_f(xi, shh[k], shl[k], ax, ay, az);
-      kahan_add(sx, cx, ax);
-      kahan_add(sy, cy, ay);
-      kahan_add(sz, cz, az);
-    }
-    double gx = (double)sx + (double)cx, gy = (double)sy + (double)cy, gz = (double)sz + (double)cz;
-    for (int o = sub >> 1; o >= 1; o >>= 1) {
+  for (int q = 0; q < 4; ++q) {
+    double gx = (double)sx[q] + (double)cx[q], gy = (double)sy[q] + (double)cy[q],
+           gz = (double)sz[q] + (double)cz[q];
+    for (int o = tl; o < kW; o <<= 1) {
       gx += __shfl_xor(gx, o);
       gy += __shfl_xor(gy, o);
       gz += __shfl_xor(gz, o);
     }
-    if (valid && si == 0) {
-      atomicAdd(&grad[3 * (int64_t)i], gx);
-      atomicAdd(&grad[3 * (int64_t)i + 1], gy);
-      atomicAdd(&grad[3 * (int64_t)i + 2], gz);
-    }
-    return;
-  }
-  f2pos xi[4];
-  float sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0};
-  float cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0};
-  int32_t ii[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int32_t t = lane + q * kW;
-    ii[q] = t < nc ? b + t : -1;
-    xi[q] = split(pos[t < nc ? b + t : b], anchor);
-  }
-  for (int32_t fb = f0; fb < f1; fb += kW) {
-    const int cnt = stage(fb);
-    float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
-    for (int k = 0; k < cnt; ++k) {
-      const float4 jh = shh[k], jl = shl[k];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) pair_acc_f(xi[q], jh, jl, ax[q], ay[q], az[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      kahan_add(sx[q], cx[q], ax[q]);
-      kahan_add(sy[q], cy[q], ay[q]);
-      kahan_add(sz[q], cz[q], az[q]);
+    if (si == 0 && ii[q] >= 0) {
+      atomicAdd(&grad[3 * (int64_t)ii[q]], gx);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 1], gy);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 2], gz);
     }
   }
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    if (ii[q] >= 0) {
-      atomicAdd(&grad[3 * (int64_t)ii[q]], (double)sx[q] + (double)cx[q]);
-      atomicAdd(&grad[3 * (int64_t)ii[q] + 1], (double)sy[q] + (double)cy[q]);
-      atomicAdd(&grad[3 * (int64_t)ii[q] + 2], (double)sz[q] + (double)cz[q]);
-    }
 }
 
 __global__ void k_finish(int64_t n, const double *__restrict__ grad, const int32_t *__restrict__ idx, double G,
@@ -444,7 +327,8 @@ void run_p2p(nbfmm_ctx *c, double *d_acc_out) {
   Plan &P = c->plan;
   mark(c, 9);
   if (P.n_tasks > 0) {
-    hipLaunchKernelGGL(c->prm.precision == NBFMM_MIXED ? k_p2p_tasks_f32 : k_p2p_tasks, dim3((unsigned)P.n_tasks),
+    hipLaunchKernelGGL(c->prm.precision == NBFMM_MIXED ? k_p2p_tasks<PairF32> : k_p2p_tasks<PairF64>,
+                       dim3((unsigned)P.n_tasks),
                        dim3(kW), 0, c->stream, (const P2PTask *)c->tasks.p,
                        (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const double4 *)c->pos.p,
                        (double *)c->grad.p);
