@@ -46,8 +46,13 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
   mark(c, 1);
   build_lists(c);
   mark(c, 2);
-  run_expansions(c); // marks 3, 4, 5
-  run_p2p(c, d_acc);
+  try {
+    run_expansions(c); // marks 3, 4, 5 (7, 8 inside); starts P2P on the second stream (marks 9, 10)
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream2); // P2P still reads the context's buffers
+    throw;
+  }
+  run_finish(c, d_acc);
   mark(c, 6);
   Plan &P = c->plan;
   nbfmm_stats &S = c->stats;
@@ -113,11 +118,16 @@ void dbuf_free(DBuf &b) {
 void mark(nbfmm_ctx *c, int slot) {
   if (c->timing) NB_HIP(hipEventRecord(c->ev[slot], c->stream));
 }
+void mark2(nbfmm_ctx *c, int slot) {
+  if (c->timing) NB_HIP(hipEventRecord(c->ev[slot], c->stream2));
+}
 } // namespace nbfmm
 
 nbfmm_ctx::~nbfmm_ctx() {
+  if (stream2) (void)hipStreamSynchronize(stream2);
+  if (stream) (void)hipStreamSynchronize(stream);
   DBuf *all[] = {&xyz_in, &m_in, &acc_out, &key[0], &key[1], &idx[0], &idx[1], &sort_hist, &scan_tmp, &pos,
-                 &pcell, &grad, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
+                 &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,
                  &p2m_off, &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum};
@@ -125,6 +135,9 @@ nbfmm_ctx::~nbfmm_ctx() {
   if (hred.p) (void)hipHostFree(hred.p);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -156,7 +169,11 @@ int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
   c->prm = *params;
   NB_HIP(hipGetDevice(&c->device));
   NB_HIP(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+  NB_HIP(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device));
   NB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  NB_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  NB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  NB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   for (auto &e : c->ev) NB_HIP(hipEventCreate(&e));
   NB_HIP(hipHostMalloc(&c->hred.p, 4096, hipHostMallocDefault));
   c->hred.cap = 4096;

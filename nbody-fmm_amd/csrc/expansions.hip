@@ -627,6 +627,7 @@ void run_expansions(nbfmm_ctx *c) {
   if (P.n_tgt == 0) {
     NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * (size_t)P.n, s));
     P.n_p2m_terms = P.n_l2p_terms = 0;
+    launch_p2p(c);
     mark(c, 3);
     mark(c, 4);
     mark(c, 5);
@@ -731,6 +732,8 @@ void run_expansions(nbfmm_ctx *c) {
   for (int d = D - 1; d >= 0; --d)
     run_trans(c, false, KP, f32, bm[d + 1] - bm[d], m2m_cells + bm[d]);
   mark(c, 3);
+  // P2P runs on the second stream from here on, beside M2L, L2L and L2P
+  launch_p2p(c);
   // ---- M2L (writes the locals of its targets) ----
   dbuf_ensure(c->lexp, (size_t)nL * KP * esz + 16);
   NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));

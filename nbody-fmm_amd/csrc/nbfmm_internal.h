@@ -95,8 +95,11 @@ struct Plan {
 struct nbfmm_ctx {
   nbfmm_params prm{};
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;            // P2P, concurrent with the expansion passes
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int device = 0;
   int lds_max = 65536; // LDS bytes one workgroup may allocate on this device
+  int n_cu = 256;      // compute units of this device
   bool timing = true;
   bool keep_lists = false;
   nbfmm::Plan plan;
@@ -106,7 +109,8 @@ struct nbfmm_ctx {
   nbfmm::DBuf key[2], idx[2], sort_hist, scan_tmp;
   nbfmm::DBuf pos;                             // double4 sorted (x,y,z,m)
   nbfmm::DBuf pcell;                           // leaf cell of each sorted particle
-  nbfmm::DBuf grad;                            // 3 doubles per sorted particle
+  nbfmm::DBuf grad;                            // 3 doubles per sorted particle (far field)
+  nbfmm::DBuf grad_near;                       // 3 doubles per sorted particle (P2P)
   nbfmm::DBuf red;                             // small reductions (bbox, flags, counts)
   nbfmm::DBuf hred;                            // pinned host mirror of `red`
   // cells
@@ -144,7 +148,10 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32);
 void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells);
 void trans_to_f32(nbfmm_ctx *c, int KP);
 // p2p.hip
-void run_p2p(nbfmm_ctx *c, double *d_acc_out);
+// P2P on the second stream, forked from the main stream after the lists
+void launch_p2p(nbfmm_ctx *c);
+// joins the P2P stream and writes acc = G * (far + near) in input order
+void run_finish(nbfmm_ctx *c, double *d_acc_out);
 void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,
                 const int64_t *d_tgt, int mode, double *d_acc);
 // integrate.hip
@@ -158,4 +165,5 @@ void rk4_final(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *asum
 void copy_d(nbfmm_ctx *c, double *dst, const double *src, int64_t count);
 // timing helpers (capi.cpp)
 void mark(nbfmm_ctx *c, int slot);
+void mark2(nbfmm_ctx *c, int slot); // on the P2P stream
 } // namespace nbfmm

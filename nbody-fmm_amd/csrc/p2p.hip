@@ -14,10 +14,13 @@
 // small the cells.  Every lane holds four targets and a source slot; the
 // slots are combined with xor-shuffles at the end.  Tasks of
 // one cell and of cells at different levels add into the same targets, so
-// the partial sums are added with FP64 atomics to the far-field gradient of
-// the L2P pass; k_finish writes acc = G * grad in input order
-// (particle.cpp:16-21).
+// the partial sums are added with FP64 atomics into the near-field gradient.
+// P2P runs on its own stream, concurrently with the expansion passes (vector
+// ALU work beside MFMA work); k_finish writes acc = G * (far + near) in input
+// order (particle.cpp:16-21).
 #include "nbfmm_internal.h"
+
+#include <algorithm>
 
 namespace nbfmm {
 namespace {
@@ -97,8 +100,11 @@ struct PairF32 {
 // Lanes: tl target lanes (power of two, 4 targets each: t = ti + tl q) times
 // 64 / tl source slots; a staged chunk of 64 sources is read once per 4 pairs
 // and the slots are combined with xor-shuffles at the end.
+// One wavefront per task (tasks blockIdx.x, blockIdx.x + gridDim.x, ... when
+// the grid is smaller than the task list).
 template <class Tr>
-__global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ tasks, const int32_t *__restrict__ nbr27,
+__global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
+                                                  const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad) {
@@ -108,7 +114,9 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   const int lane = threadIdx.x;
-  const P2PTask tk = tasks[blockIdx.x];
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+  __syncthreads(); // the previous task's reads of lb, lp, sh are done
+  const P2PTask tk = tasks[task];
   const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
   double4 anchor = make_double4(0, 0, 0, 0);
   if (Tr::kAnchor) {
@@ -206,16 +214,17 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(const P2PTask *__restrict__ ta
       atomicAdd(&grad[3 * (int64_t)ii[q] + 2], gz);
     }
   }
+  }
 }
 
-__global__ void k_finish(int64_t n, const double *__restrict__ grad, const int32_t *__restrict__ idx, double G,
-                         double *__restrict__ acc_out) {
+__global__ void k_finish(int64_t n, const double *__restrict__ far, const double *__restrict__ near,
+                         const int32_t *__restrict__ idx, double G, double *__restrict__ acc_out) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const int64_t out = idx[i];
-  acc_out[3 * out] = G * grad[3 * i];
-  acc_out[3 * out + 1] = G * grad[3 * i + 1];
-  acc_out[3 * out + 2] = G * grad[3 * i + 2];
+  acc_out[3 * out] = G * (far[3 * i] + near[3 * i]);
+  acc_out[3 * out + 1] = G * (far[3 * i + 1] + near[3 * i + 1]);
+  acc_out[3 * out + 2] = G * (far[3 * i + 2] + near[3 * i + 2]);
 }
 
 // ---- direct sums ----------------------------------------------------------------------
@@ -323,20 +332,32 @@ __global__ void k_direct_combine(int64_t ntgt, int nchunk, const double *__restr
 
 } // namespace
 
-void run_p2p(nbfmm_ctx *c, double *d_acc_out) {
+void launch_p2p(nbfmm_ctx *c) {
   Plan &P = c->plan;
-  mark(c, 9);
+  hipStream_t s2 = c->stream2;
+  NB_HIP(hipEventRecord(c->ev_fork, c->stream));
+  NB_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
+  double *gn = dget<double>(c->grad_near, (size_t)P.n * 3);
+  NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * 3 * (size_t)P.n, s2));
+  mark2(c, 9);
   if (P.n_tasks > 0) {
+    const int64_t grid = std::min<int64_t>(P.n_tasks, (int64_t)1 << 31);
     hipLaunchKernelGGL(c->prm.precision == NBFMM_MIXED ? k_p2p_tasks<PairF32> : k_p2p_tasks<PairF64>,
-                       dim3((unsigned)P.n_tasks),
-                       dim3(kW), 0, c->stream, (const P2PTask *)c->tasks.p,
-                       (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const double4 *)c->pos.p,
-                       (double *)c->grad.p);
+                       dim3((unsigned)grid), dim3(kW), 0, s2, (int64_t)P.n_tasks, (const P2PTask *)c->tasks.p,
+                       (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
+                       (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn);
     NB_LAUNCH();
   }
-  mark(c, 10);
+  mark2(c, 10);
+  NB_HIP(hipEventRecord(c->ev_join, s2));
+}
+
+void run_finish(nbfmm_ctx *c, double *d_acc_out) {
+  Plan &P = c->plan;
+  NB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   hipLaunchKernelGGL(k_finish, dim3((unsigned)((P.n + kT - 1) / kT)), dim3(kT), 0, c->stream, P.n,
-                     (const double *)c->grad.p, (const int32_t *)c->idx[0].p, c->prm.G, d_acc_out);
+                     (const double *)c->grad.p, (const double *)c->grad_near.p, (const int32_t *)c->idx[0].p, c->prm.G,
+                     d_acc_out);
   NB_LAUNCH();
 }
 
