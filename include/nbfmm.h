@@ -108,6 +108,34 @@ int nbfmm_step_se_dev(nbfmm_ctx *ctx, int64_t n, double *d_xyz, double *d_vel, d
 int nbfmm_step_rk4_dev(nbfmm_ctx *ctx, int64_t n, double *d_xyz, double *d_vel, double *d_acc,
                        const double *d_m, double dt, int nsteps, int mode);
 
+/* Accelerations and potentials phi_i = -G sum_{j != i} m_j / |x_i - x_j| of
+ * all n particles (the FMM of nbfmm_eval_accel_dev with the potential formed
+ * beside the gradient: P2P sums m_j / r_ij, L2P evaluates sum L R).  The
+ * reference computes no potential; this serves the energy monitor.  Device
+ * pointers, stream-ordered. */
+int nbfmm_eval_accel_pot_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_m, double *d_acc,
+                             double *d_phi);
+
+/* Sums of a particle state in FP64 with a fixed summation order, written to
+ * the host array out[10]: out[0] kinetic energy sum m v^2 / 2, out[1]
+ * potential energy sum m phi / 2 (0 when d_phi is NULL; d_phi from
+ * nbfmm_eval_accel_pot_dev at the same positions), out[2] total energy,
+ * out[3..5] momentum sum m v, out[6..8] angular momentum sum m x cross v about
+ * the origin, out[9] total mass.  Device pointers; synchronous. */
+int nbfmm_diagnostics_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_vel, const double *d_m,
+                          const double *d_phi, double *out);
+
+/* Multi-step driver: advances nsteps with integrator 0 = SyplecticEuler,
+ * 1 = RungeKutta4 with the reference's semantics, 2 = classic RK4 (as
+ * nbfmm_step_se_dev / nbfmm_step_rk4_dev), and, when h_diag is not NULL,
+ * records the 10 diagnostics of nbfmm_diagnostics_dev before the first step
+ * and after every `every` steps into h_diag (host memory, room for
+ * 10 * (1 + nsteps / every) doubles).  The monitor evaluates into its own
+ * buffers, so d_acc carries the integrator's acceleration as without it.
+ * Device pointers; synchronous. */
+int nbfmm_run_dev(nbfmm_ctx *ctx, int64_t n, double *d_xyz, double *d_vel, double *d_acc, const double *d_m,
+                  double dt, int nsteps, int integrator, int every, double *h_diag);
+
 /* Statistics of the last evaluation: sizes for the roofline and per-phase
  * GPU times measured with HIP events on the context's stream. */
 typedef struct nbfmm_stats {

@@ -38,8 +38,10 @@ int check_params(const nbfmm_params *p) {
   return NBFMM_OK;
 }
 
-void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, double *d_acc) {
+void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, double *d_acc,
+              double *d_phi = nullptr) {
   c->plan.n = n;
+  c->phi_out = d_phi;
   c->nev = 0;
   mark(c, 0);
   build_tree(c, d_xyz, d_m);
@@ -50,9 +52,11 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
     run_expansions(c); // marks 3, 4, 5 (7, 8 inside); starts P2P on the second stream (marks 9, 10)
   } catch (...) {
     (void)hipStreamSynchronize(c->stream2); // P2P still reads the context's buffers
+    c->phi_out = nullptr;
     throw;
   }
   run_finish(c, d_acc);
+  c->phi_out = nullptr;
   mark(c, 6);
   Plan &P = c->plan;
   nbfmm_stats &S = c->stats;
@@ -90,6 +94,33 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
   const double p1 = (double)c->prm.order + 1.0;
   S.flop_m2l = 2.0 * p1 * p1 * p1 * p1 * (double)P.n_m2l;
   S.flop_p2p = 20.0 * (double)P.n_p2p_pairs;
+}
+
+// SyplecticEuler::integrate (symplectic_euler.cpp:8-15), one step
+void step_se(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt) {
+  eval_dev(c, n, x, m, a);
+  se_update(c, n, x, v, a, dt);
+}
+
+// RungeKutta4::integrate (runge_kutta_4.cpp:10-93), one step; mode 0 the
+// reference's semantics, 1 classic
+void step_rk4(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt, int mode) {
+  const size_t m3 = (size_t)n * 3;
+  double *xk = dget<double>(c->xk, m3), *vk = dget<double>(c->vk, m3), *ak = dget<double>(c->ak, m3);
+  double *asum = dget<double>(c->asum, m3), *vsum = dget<double>(c->vsum, m3);
+  const double h2 = dt / 2, h6 = dt / 6;
+  copy_d(c, vk, v, (int64_t)m3);         // velocity_k = v      (runge_kutta_4.cpp:28-30)
+  eval_dev(c, n, x, m, ak);              // stage 1              (:32)
+  rk4_accum(c, n, asum, vsum, ak, vk, 1.0, 1);
+  if (mode == 1) copy_d(c, a, ak, (int64_t)m3);
+  for (int stage = 2; stage <= 4; ++stage) {
+    const double h = stage == 4 ? dt : h2;
+    const double *avel = mode == 0 ? a : ak;  // reference: carried acceleration of the input (:46-48)
+    rk4_stage(c, n, x, v, vk, avel, h, xk, vk);
+    eval_dev(c, n, xk, m, ak);
+    rk4_accum(c, n, asum, vsum, ak, vk, stage == 4 ? 1.0 : 2.0, 0);
+  }
+  rk4_final(c, n, x, v, asum, vsum, h6);  // (:89-92)
 }
 
 void check_n(int64_t n) {
@@ -130,7 +161,7 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,
-                 &p2m_off, &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum};
+                 &p2m_off, &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi};
   for (DBuf *b : all) dbuf_free(*b);
   if (hred.p) (void)hipHostFree(hred.p);
   for (auto &e : ev)
@@ -270,10 +301,7 @@ int nbfmm_step_se_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, 
   if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
   API_BEGIN
   check_n(n);
-  for (int s = 0; s < nsteps; ++s) {
-    eval_dev(c, n, x, m, a);
-    se_update(c, n, x, v, a, dt);
-  }
+  for (int s = 0; s < nsteps; ++s) step_se(c, n, x, v, a, m, dt);
   API_END
 }
 
@@ -284,24 +312,54 @@ int nbfmm_step_rk4_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a,
   if (mode != 0 && mode != 1) return fail(NBFMM_ERR_ARG, "mode must be 0 or 1");
   API_BEGIN
   check_n(n);
-  const size_t m3 = (size_t)n * 3;
-  double *xk = dget<double>(c->xk, m3), *vk = dget<double>(c->vk, m3), *ak = dget<double>(c->ak, m3);
-  double *asum = dget<double>(c->asum, m3), *vsum = dget<double>(c->vsum, m3);
-  const double h2 = dt / 2, h6 = dt / 6;
-  for (int st = 0; st < nsteps; ++st) {
-    copy_d(c, vk, v, (int64_t)m3);         // velocity_k = v      (runge_kutta_4.cpp:28-30)
-    eval_dev(c, n, x, m, ak);              // stage 1              (:32)
-    rk4_accum(c, n, asum, vsum, ak, vk, 1.0, 1);
-    if (mode == 1) copy_d(c, a, ak, (int64_t)m3);
-    for (int stage = 2; stage <= 4; ++stage) {
-      const double h = stage == 4 ? dt : h2;
-      const double *avel = mode == 0 ? a : ak;  // reference: carried acceleration of the input (:46-48)
-      rk4_stage(c, n, x, v, vk, avel, h, xk, vk);
-      eval_dev(c, n, xk, m, ak);
-      rk4_accum(c, n, asum, vsum, ak, vk, stage == 4 ? 1.0 : 2.0, 0);
-    }
-    rk4_final(c, n, x, v, asum, vsum, h6);  // (:89-92)
+  for (int st = 0; st < nsteps; ++st) step_rk4(c, n, x, v, a, m, dt, mode);
+  API_END
+}
+
+int nbfmm_eval_accel_pot_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, double *d_acc,
+                             double *d_phi) {
+  if (!c || !d_xyz || !d_m || !d_acc || !d_phi) return fail(NBFMM_ERR_ARG, "null argument");
+  API_BEGIN
+  check_n(n);
+  eval_dev(c, n, d_xyz, d_m, d_acc, d_phi);
+  API_END
+}
+
+int nbfmm_diagnostics_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_vel, const double *d_m,
+                          const double *d_phi, double *out) {
+  if (!c || !d_xyz || !d_vel || !d_m || !out) return fail(NBFMM_ERR_ARG, "null argument");
+  API_BEGIN
+  check_n(n);
+  diagnostics(c, n, d_xyz, d_vel, d_m, d_phi, out);
+  API_END
+}
+
+int nbfmm_run_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt, int nsteps,
+                  int integrator, int every, double *h_diag) {
+  if (!c || !x || !v || !a || !m) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
+  if (integrator < 0 || integrator > 2) return fail(NBFMM_ERR_ARG, "integrator must be 0, 1 or 2");
+  if (h_diag && every < 1) return fail(NBFMM_ERR_ARG, "every must be >= 1");
+  API_BEGIN
+  check_n(n);
+  // the monitor evaluates into its own buffers, so the state the integrator
+  // carries (the acceleration in d_acc) is not touched
+  double *da = h_diag ? dget<double>(c->diag_acc, (size_t)n * 3) : nullptr;
+  double *dp = h_diag ? dget<double>(c->diag_phi, (size_t)n) : nullptr;
+  int rec = 0;
+  auto monitor = [&]() {
+    eval_dev(c, n, x, m, da, dp);
+    diagnostics(c, n, x, v, m, dp, h_diag + 10 * rec++);
+  };
+  if (h_diag) monitor();
+  for (int s = 1; s <= nsteps; ++s) {
+    if (integrator == 0)
+      step_se(c, n, x, v, a, m, dt);
+    else
+      step_rk4(c, n, x, v, a, m, dt, integrator == 1 ? 0 : 1);
+    if (h_diag && s % every == 0) monitor();
   }
+  NB_HIP(hipStreamSynchronize(c->stream));
   API_END
 }
 

@@ -420,17 +420,18 @@ __device__ __forceinline__ cpx<T> lget(const T *L, int n, int m, int p) {
   return {-L[n * n + 1], L[n * n + 2]};
 }
 
-// T as in k_p2m; the gradient is accumulated over levels in FP64.
-template <class T>
+// T as in k_p2m; the gradient is accumulated over levels in FP64.  POT also
+// evaluates the potential sum (1/rho) sum_{n<=p,m>=0} w(m) Re[L_n^m R_n^m] into pot.
+template <class T, bool POT>
 __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ pcell, const int32_t *__restrict__ cparent,
                                             const uint64_t *__restrict__ ckey, const int32_t *__restrict__ eslot,
                                             LevelGeo g, const Consts *__restrict__ C, const T *__restrict__ Lexp,
-                                            double *__restrict__ grad) {
+                                            double *__restrict__ grad, double *__restrict__ pot) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double4 q = pos[i];
-  double gx = 0, gy = 0, gz = 0;
+  double gx = 0, gy = 0, gz = 0, gp = 0;
   int32_t cell = pcell[i];
   for (int lev = g.depth; lev >= 2; --lev) {
     const int32_t ts = eslot[cell];
@@ -441,9 +442,9 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
       const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
       const T r2 = x * x + y * y + z * z;
       const T *L = Lexp + (size_t)ts * KP;
-      T hx = 0, hy = 0, hz = 0;
+      T hx = 0, hy = 0, hz = 0, hv = 0;
       T vr = 1, vi = 0; // R_m^m
-      for (int m = 0; m < p; ++m) {
+      for (int m = 0; m < (POT ? p + 1 : p); ++m) {
         if (m > 0) {
           const T f = (T)C->mmf[m];
           const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
@@ -452,7 +453,7 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
         }
         const T w = m == 0 ? (T)1 : (T)2;
         T pr = 0, pi = 0, cr = vr, ci = vi;
-        for (int np = m; np < p; ++np) {
+        for (int np = m; np < (POT ? p + 1 : p); ++np) {
           if (np == m + 1) {
             pr = cr;
             pi = ci;
@@ -466,6 +467,11 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
             pi = ci;
             cr = nr;
             ci = ni;
+          }
+          if (POT) {
+            const cpx<T> Lv = lget(L, np, m, p);
+            hv += w * (cr * Lv.x - ci * Lv.y);
+            if (np == p) continue;
           }
           const int nn = np + 1;
           const cpx<T> Lm = lget(L, nn, m - 1, p), L0 = lget(L, nn, m, p), Lp = lget(L, nn, m + 1, p);
@@ -482,12 +488,14 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
       gx += s * (double)hx;
       gy += s * (double)hy;
       gz += s * (double)hz;
+      if (POT) gp += ir * (double)hv;
     }
     cell = cparent[cell];
   }
   grad[3 * i] = gx;
   grad[3 * i + 1] = gy;
   grad[3 * i + 2] = gz;
+  if (POT) pot[i] = gp;
 }
 
 __global__ void k_p2m_count(const int32_t *src_cells, int64_t nsrc, const int32_t *cbeg, const int32_t *cend,
@@ -623,9 +631,12 @@ void run_expansions(nbfmm_ctx *c) {
     }
     NB_HIP(hipMemcpyAsync(dC, &hC, sizeof(Consts), hipMemcpyHostToDevice, s));
   }
-  double *grad = dget<double>(c->grad, (size_t)P.n * 3);
+  // far-field gradient (3 per particle) and, with potentials, far-field potential sums
+  const bool pot = c->phi_out != nullptr;
+  double *grad = dget<double>(c->grad, (size_t)P.n * (pot ? 4 : 3));
+  double *pfar = pot ? grad + 3 * (size_t)P.n : nullptr;
   if (P.n_tgt == 0) {
-    NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * (size_t)P.n, s));
+    NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s));
     P.n_p2m_terms = P.n_l2p_terms = 0;
     launch_p2p(c);
     mark(c, 3);
@@ -743,13 +754,13 @@ void run_expansions(nbfmm_ctx *c) {
   mark(c, 4);
   // ---- L2P ----
   if (f32)
-    hipLaunchKernelGGL(k_l2p<float>, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
-                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)eslot,
-                       g, dC, (const float *)c->lexp.p, grad);
+    hipLaunchKernelGGL((pot ? k_l2p<float, true> : k_l2p<float, false>), dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n,
+                       (const double4 *)c->pos.p, (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey,
+                       (const int32_t *)eslot, g, dC, (const float *)c->lexp.p, grad, pfar);
   else
-    hipLaunchKernelGGL(k_l2p<double>, dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n, (const double4 *)c->pos.p,
-                       (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey, (const int32_t *)eslot,
-                       g, dC, (const double *)c->lexp.p, grad);
+    hipLaunchKernelGGL((pot ? k_l2p<double, true> : k_l2p<double, false>), dim3(nblk(P.n)), dim3(kT), 0, s, p, KP,
+                       P.n, (const double4 *)c->pos.p, (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p,
+                       ckey, (const int32_t *)eslot, g, dC, (const double *)c->lexp.p, grad, pfar);
   NB_LAUNCH();
   mark(c, 5);
 }

@@ -60,6 +60,48 @@ __global__ void k_rk4_final(int64_t m, double *x, double *v, const double *asum,
 }
 
 inline unsigned nb(int64_t m) { return (unsigned)((m + kT - 1) / kT); }
+// ---- diagnostics --------------------------------------------------------------------------
+// Sums of a particle state in FP64, in a fixed order (strided per thread, tree
+// per block, blocks in order): 0 kinetic energy 1/2 m v^2, 1 potential energy
+// 1/2 m phi, 3..5 momentum m v, 6..8 angular momentum m x cross v, 9 mass.
+constexpr int kDiagBlocks = 256;
+__global__ void __launch_bounds__(kT) k_diag_part(int64_t n, const double *__restrict__ x, const double *__restrict__ v,
+                                                  const double *__restrict__ m, const double *__restrict__ phi,
+                                                  double *__restrict__ part) {
+  __shared__ double sh[10][kT];
+  double q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double mi = m[i];
+    const double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
+    const double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
+    q[0] += 0.5 * mi * (vx * vx + vy * vy + vz * vz);
+    if (phi) q[1] += 0.5 * mi * phi[i];
+    q[3] += mi * vx;
+    q[4] += mi * vy;
+    q[5] += mi * vz;
+    q[6] += mi * (py * vz - pz * vy);
+    q[7] += mi * (pz * vx - px * vz);
+    q[8] += mi * (px * vy - py * vx);
+    q[9] += mi;
+  }
+  for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] = q[k];
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 10) part[(size_t)blockIdx.x * 10 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_diag_final(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= 10) return;
+  double s = 0;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)b * 10 + k];
+  out[k] = s;
+}
+
 } // namespace
 
 void se_update(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *a, double dt) {
@@ -82,6 +124,19 @@ void rk4_final(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *asum
 }
 void copy_d(nbfmm_ctx *c, double *dst, const double *src, int64_t count) {
   NB_HIP(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)count, hipMemcpyDeviceToDevice, c->stream));
+}
+
+void diagnostics(nbfmm_ctx *c, int64_t n, const double *x, const double *v, const double *m, const double *phi,
+                 double *h_out) {
+  double *part = dget<double>(c->dpart, (size_t)kDiagBlocks * 10 + 16);
+  double *d_out = part + (size_t)kDiagBlocks * 10;
+  hipLaunchKernelGGL(k_diag_part, dim3(kDiagBlocks), dim3(kT), 0, c->stream, n, x, v, m, phi, part);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c->stream, kDiagBlocks, (const double *)part, d_out);
+  NB_LAUNCH();
+  NB_HIP(hipMemcpyAsync(h_out, d_out, sizeof(double) * 10, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  h_out[2] = h_out[0] + h_out[1];
 }
 
 } // namespace nbfmm

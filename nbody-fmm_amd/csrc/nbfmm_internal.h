@@ -102,6 +102,7 @@ struct nbfmm_ctx {
   int n_cu = 256;      // compute units of this device
   bool timing = true;
   bool keep_lists = false;
+  double *phi_out = nullptr; // potential output of the current evaluation (device, input order) or null
   nbfmm::Plan plan;
   nbfmm_stats stats{};
   // particle arrays
@@ -125,7 +126,7 @@ struct nbfmm_ctx {
   std::vector<int32_t> h_cls_off; // host schedule of the class-sorted M2L
   std::vector<int4> h_m2l_items;
   // integrator scratch
-  nbfmm::DBuf xk, vk, ak, asum, vsum;
+  nbfmm::DBuf xk, vk, ak, asum, vsum, diag_acc, diag_phi;
   // events
   hipEvent_t ev[16] = {};
   int nev = 0;
@@ -163,6 +164,9 @@ void rk4_accum(nbfmm_ctx *c, int64_t n, double *asum, double *vsum, const double
 void rk4_final(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *asum,
                const double *vsum, double h6);
 void copy_d(nbfmm_ctx *c, double *dst, const double *src, int64_t count);
+// 10 FP64 sums of a state (host out): kinetic, potential, total energy, momentum, angular momentum, mass
+void diagnostics(nbfmm_ctx *c, int64_t n, const double *x, const double *v, const double *m, const double *phi,
+                 double *h_out);
 // timing helpers (capi.cpp)
 void mark(nbfmm_ctx *c, int slot);
 void mark2(nbfmm_ctx *c, int slot); // on the P2P stream

@@ -46,14 +46,17 @@ struct PairF64 {
   typedef double A;
   static constexpr bool kAnchor = false;
   static __device__ __forceinline__ S make(const double4 &p, const double4 &) { return p; }
-  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az) {
+  template <bool POT>
+  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
     const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double inv = r2 > 0.0 ? rsqrt_pos(r2) : 0.0;
-    const double w = xj.w * inv * inv * inv;
+    const double mi = xj.w * inv;
+    const double w = mi * inv * inv;
     ax -= dx * w;
     ay -= dy * w;
     az -= dz * w;
+    if (POT) ap += mi;
   }
   static __device__ __forceinline__ void add(A &s, A &, A v) { s += v; }
 };
@@ -78,16 +81,19 @@ struct PairF32 {
     return {make_float4(hx, hy, hz, (float)p.w),
             make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.0f)};
   }
-  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az) {
+  template <bool POT>
+  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
     const float dx = (xi.hi.x - xj.hi.x) + (xi.lo.x - xj.lo.x);
     const float dy = (xi.hi.y - xj.hi.y) + (xi.lo.y - xj.lo.y);
     const float dz = (xi.hi.z - xj.hi.z) + (xi.lo.z - xj.lo.z);
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float inv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-    const float w = xj.hi.w * inv * inv * inv;
+    const float mi = xj.hi.w * inv;
+    const float w = mi * inv * inv;
     ax -= dx * w;
     ay -= dy * w;
     az -= dz * w;
+    if (POT) ap += mi;
   }
   static __device__ __forceinline__ void add(A &s, A &c, A v) { // Kahan
     const float y = v - c;
@@ -101,13 +107,14 @@ struct PairF32 {
 // 64 / tl source slots; a staged chunk of 64 sources is read once per 4 pairs
 // and the slots are combined with xor-shuffles at the end.
 // One wavefront per task (tasks blockIdx.x, blockIdx.x + gridDim.x, ... when
-// the grid is smaller than the task list).
-template <class Tr>
+// the grid is smaller than the task list).  POT also sums m_j / r_ij into pot.
+template <class Tr, bool POT>
 __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
-                                                  const double4 *__restrict__ pos, double *__restrict__ grad) {
+                                                  const double4 *__restrict__ pos, double *__restrict__ grad,
+                                                  double *__restrict__ pot) {
   typedef typename Tr::S S;
   typedef typename Tr::A A;
   __shared__ S sh[kW];
@@ -167,8 +174,8 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *
     ii[q] = t < nc ? b + t : -1;
     xi[q] = Tr::make(pos[t < nc ? b + t : b], anchor);
   }
-  A sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0};
-  A cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0};
+  A sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0}, sp[4] = {0, 0, 0, 0};
+  A cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
   for (int32_t fb = f0; fb < f1; fb += kW) {
     // stage sources [fb, fb + 64) of the concatenation
     const int32_t g = fb + lane;
@@ -186,17 +193,18 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *
     }
     __syncthreads();
     const int cnt = min(kW, f1 - fb);
-    A ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
+    A ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0}, ap[4] = {0, 0, 0, 0};
     for (int k = si; k < cnt; k += sub) {
       const S xj = sh[k];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Tr::pair(xi[q], xj, ax[q], ay[q], az[q]);
+      for (int q = 0; q < 4; ++q) Tr::template pair<POT>(xi[q], xj, ax[q], ay[q], az[q], ap[q]);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       Tr::add(sx[q], cx[q], ax[q]);
       Tr::add(sy[q], cy[q], ay[q]);
       Tr::add(sz[q], cz[q], az[q]);
+      if (POT) Tr::add(sp[q], cp[q], ap[q]);
     }
   }
 #pragma unroll
@@ -208,23 +216,33 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *
       gy += __shfl_xor(gy, o);
       gz += __shfl_xor(gz, o);
     }
+    double gp = 0;
+    if (POT) {
+      gp = (double)sp[q] + (double)cp[q];
+      for (int o = tl; o < kW; o <<= 1) gp += __shfl_xor(gp, o);
+    }
     if (si == 0 && ii[q] >= 0) {
       atomicAdd(&grad[3 * (int64_t)ii[q]], gx);
       atomicAdd(&grad[3 * (int64_t)ii[q] + 1], gy);
       atomicAdd(&grad[3 * (int64_t)ii[q] + 2], gz);
+      if (POT) atomicAdd(&pot[ii[q]], gp);
     }
   }
   }
 }
 
+// acc = G * grad; with potentials, phi = -G * sum_j m_j / r_ij
 __global__ void k_finish(int64_t n, const double *__restrict__ far, const double *__restrict__ near,
-                         const int32_t *__restrict__ idx, double G, double *__restrict__ acc_out) {
+                         const double *__restrict__ pfar, const double *__restrict__ pnear,
+                         const int32_t *__restrict__ idx, double G, double *__restrict__ acc_out,
+                         double *__restrict__ phi_out) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const int64_t out = idx[i];
   acc_out[3 * out] = G * (far[3 * i] + near[3 * i]);
   acc_out[3 * out + 1] = G * (far[3 * i + 1] + near[3 * i + 1]);
   acc_out[3 * out + 2] = G * (far[3 * i + 2] + near[3 * i + 2]);
+  if (phi_out) phi_out[out] = -G * (pfar[i] + pnear[i]);
 }
 
 // ---- direct sums ----------------------------------------------------------------------
@@ -337,15 +355,20 @@ void launch_p2p(nbfmm_ctx *c) {
   hipStream_t s2 = c->stream2;
   NB_HIP(hipEventRecord(c->ev_fork, c->stream));
   NB_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
-  double *gn = dget<double>(c->grad_near, (size_t)P.n * 3);
-  NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * 3 * (size_t)P.n, s2));
+  // near-field gradient (3 per particle) and, with potentials, near-field sums of m / r
+  const bool pot = c->phi_out != nullptr;
+  double *gn = dget<double>(c->grad_near, (size_t)P.n * (pot ? 4 : 3));
+  double *pn = pot ? gn + 3 * (size_t)P.n : nullptr;
+  NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
   mark2(c, 9);
   if (P.n_tasks > 0) {
-    const int64_t grid = std::min<int64_t>(P.n_tasks, (int64_t)1 << 31);
-    hipLaunchKernelGGL(c->prm.precision == NBFMM_MIXED ? k_p2p_tasks<PairF32> : k_p2p_tasks<PairF64>,
-                       dim3((unsigned)grid), dim3(kW), 0, s2, (int64_t)P.n_tasks, (const P2PTask *)c->tasks.p,
+    const int64_t grid = std::min<int64_t>(P.n_tasks, ((int64_t)1 << 31) - 1);
+    const bool f32 = c->prm.precision == NBFMM_MIXED;
+    auto kern = f32 ? (pot ? k_p2p_tasks<PairF32, true> : k_p2p_tasks<PairF32, false>)
+                    : (pot ? k_p2p_tasks<PairF64, true> : k_p2p_tasks<PairF64, false>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, (int64_t)P.n_tasks, (const P2PTask *)c->tasks.p,
                        (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
-                       (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn);
+                       (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn, pn);
     NB_LAUNCH();
   }
   mark2(c, 10);
@@ -355,9 +378,12 @@ void launch_p2p(nbfmm_ctx *c) {
 void run_finish(nbfmm_ctx *c, double *d_acc_out) {
   Plan &P = c->plan;
   NB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  const bool pot = c->phi_out != nullptr;
+  const double *gn = (const double *)c->grad_near.p;
   hipLaunchKernelGGL(k_finish, dim3((unsigned)((P.n + kT - 1) / kT)), dim3(kT), 0, c->stream, P.n,
-                     (const double *)c->grad.p, (const double *)c->grad_near.p, (const int32_t *)c->idx[0].p, c->prm.G,
-                     d_acc_out);
+                     (const double *)c->grad.p, gn, pot ? (const double *)c->grad.p + 3 * (size_t)P.n : nullptr,
+                     pot ? gn + 3 * (size_t)P.n : nullptr, (const int32_t *)c->idx[0].p, c->prm.G, d_acc_out,
+                     c->phi_out);
   NB_LAUNCH();
 }
 

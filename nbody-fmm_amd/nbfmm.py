@@ -65,7 +65,10 @@ EXPORTS = [
     "nbfmm_eval_accel", "nbfmm_eval_accel_dev", "nbfmm_direct_accel", "nbfmm_direct_accel_dev",
     "nbfmm_step_se_dev", "nbfmm_step_rk4_dev", "nbfmm_get_stats", "nbfmm_set_timing",
     "nbfmm_keep_lists", "nbfmm_tree_sizes", "nbfmm_tree_get", "nbfmm_last_error",
+    "nbfmm_eval_accel_pot_dev", "nbfmm_diagnostics_dev", "nbfmm_run_dev",
 ]
+
+DIAG_NAMES = ["kinetic", "potential", "total", "px", "py", "pz", "lx", "ly", "lz", "mass"]
 
 _lib = None
 
@@ -94,6 +97,9 @@ def lib():
         L.nbfmm_keep_lists.argtypes = [vp, ctypes.c_int]
         L.nbfmm_tree_sizes.argtypes = [vp, vp, vp, vp, vp]
         L.nbfmm_tree_get.argtypes = [vp] + [vp] * 10
+        L.nbfmm_eval_accel_pot_dev.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.nbfmm_diagnostics_dev.argtypes = [vp, i64, vp, vp, vp, vp, vp]
+        L.nbfmm_run_dev.argtypes = [vp, i64, vp, vp, vp, vp, dbl, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         _lib = L
     return _lib
 
@@ -182,6 +188,23 @@ class Context:
 
     def step_rk4_dev(self, n, dx, dv, da, dm, dt, nsteps, mode):
         _check(lib().nbfmm_step_rk4_dev(self.h, n, dx, dv, da, dm, dt, nsteps, mode))
+
+    def eval_accel_pot_dev(self, n, d_xyz, d_m, d_acc, d_phi):
+        _check(lib().nbfmm_eval_accel_pot_dev(self.h, n, d_xyz, d_m, d_acc, d_phi))
+
+    def diagnostics_dev(self, n, d_xyz, d_vel, d_m, d_phi=None):
+        """Dict of the 10 sums (DIAG_NAMES) of a device-resident state."""
+        out = np.zeros(10)
+        _check(lib().nbfmm_diagnostics_dev(self.h, n, d_xyz, d_vel, d_m, d_phi, _ptr(out)))
+        return dict(zip(DIAG_NAMES, out.tolist()))
+
+    def run_dev(self, n, dx, dv, da, dm, dt, nsteps, integrator=0, every=1, monitor=True):
+        """Advances nsteps on the device; returns the diagnostics records
+        (array of shape (1 + nsteps // every, 10), columns DIAG_NAMES) or None."""
+        rec = np.zeros((1 + nsteps // every, 10)) if monitor else None
+        _check(lib().nbfmm_run_dev(self.h, n, dx, dv, da, dm, dt, nsteps, integrator, every,
+                                   _ptr(rec) if monitor else None))
+        return rec
 
     def stats(self):
         s = Stats()
