@@ -33,6 +33,7 @@ int check_params(const nbfmm_params *p) {
   if (!std::isfinite(p->G)) return fail(NBFMM_ERR_ARG, "G must be finite");
   if (!(p->eps2 >= 0.0) || !std::isfinite(p->eps2)) return fail(NBFMM_ERR_ARG, "eps2 must be finite and >= 0");
   if (p->p2p_cut < 0) return fail(NBFMM_ERR_ARG, "p2p_cut must be >= 0");
+  if (p->m2l_pass_pairs < 0) return fail(NBFMM_ERR_ARG, "m2l_pass_pairs must be >= 0");
   if (p->precision != NBFMM_PLAIN_FP64 && p->precision != NBFMM_MIXED)
     return fail(NBFMM_ERR_ARG, "unknown precision");
   return NBFMM_OK;

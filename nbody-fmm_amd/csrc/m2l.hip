@@ -9,7 +9,9 @@
 // acc[r] += T[r] * B, every MFMA column filled.  The partial local of every
 // pair goes to Y[pair] (pairs numbered in the target-major list order) and
 // k_m2l_sum adds the partials of each target in list order, so the result
-// does not depend on the schedule.
+// does not depend on the schedule.  When the partials of all pairs would
+// exceed the buffer, the class-sorted pairs are taken in passes and every
+// pass adds its partials to the locals.
 //
 // Plain precision runs v_mfma_f64_16x16x4f64 on FP64 coefficients; mixed
 // precision v_mfma_f32_16x16x4f32 on FP32 coefficients (the rho scaling keeps
@@ -69,7 +71,7 @@ __global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cl
 // cursor[k] starts at the first position of class k; the order of the pairs
 // inside a class is free (every pair's result is computed independently).
 __global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, int64_t n, int32_t *cursor,
-                                                    int32_t *__restrict__ out) {
+                                                    int32_t *__restrict__ out, int32_t *__restrict__ inv) {
   __shared__ int32_t h[kNumClasses];
   __shared__ int32_t base[kNumClasses];
   for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
@@ -84,7 +86,9 @@ __global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__
   __syncthreads();
   for (int64_t i = b + threadIdx.x; i < e; i += kT) {
     const int k = cls[i];
-    out[base[k] + atomicAdd(&h[k], 1)] = (int32_t)i;
+    const int32_t pos = base[k] + atomicAdd(&h[k], 1);
+    out[pos] = (int32_t)i;
+    inv[i] = pos;
   }
 }
 
@@ -94,7 +98,7 @@ __global__ void k_to_f32(const double *__restrict__ in, int64_t n, float *__rest
 }
 
 template <int KP, int RBW, class T>
-__global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__restrict__ items,
+__global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__restrict__ items, int32_t ybase,
                                                          const int32_t *__restrict__ cls_pairs,
                                                          const int32_t *__restrict__ m2l_src,
                                                          const int32_t *__restrict__ src_slot, const T *__restrict__ frag,
@@ -149,7 +153,7 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
       }
     }
     if (pid >= 0) {
-      T *y = Y + (size_t)pid * KP + r0 * 16;
+      T *y = Y + (size_t)(pi - ybase) * KP + r0 * 16;
 #pragma unroll
       for (int r = 0; r < RBW; ++r)
         if (r < nrb)
@@ -159,11 +163,14 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
   }
 }
 
-// L of every M2L target = sum of its pairs' partials, in list order (FP64).
+// L of every M2L target += sum of the partials of its pairs in the pass
+// [cb, ce) of class-sorted positions (pos[q] of list entry q), in list order,
+// in FP64.
 template <class T>
 __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
-                                                const int32_t *__restrict__ lslot, const T *__restrict__ Y,
+                                                const int32_t *__restrict__ lslot, const int32_t *__restrict__ pos,
+                                                int32_t cb, int32_t ce, const T *__restrict__ Y,
                                                 T *__restrict__ Lexp) {
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -174,19 +181,27 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
   // rows lane, lane + 64, ... (KP <= 256), pairs four at a time
   double s[4] = {0, 0, 0, 0};
   int32_t q = b;
-  for (; q + 4 <= e; q += 4)
+  for (; q + 4 <= e; q += 4) {
+    int32_t pr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pr[u] = pos[q + u];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
+      if (pr[u] >= cb && pr[u] < ce)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)(pr[u] - cb) * KP + lane + 64 * r];
+  }
+  for (; q < e; ++q) {
+    const int32_t pr = pos[q];
+    if (pr >= cb && pr < ce)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)(q + u) * KP + lane + 64 * r];
-  for (; q < e; ++q)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)q * KP + lane + 64 * r];
+        if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)(pr - cb) * KP + lane + 64 * r];
+  }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    if (lane + 64 * r < KP) L[lane + 64 * r] = (T)s[r];
+    if (lane + 64 * r < KP) L[lane + 64 * r] = (T)((double)L[lane + 64 * r] + s[r]);
 }
 
 template <class T>
@@ -195,7 +210,8 @@ constexpr int rbw_for(int KP, int lds) {
 }
 
 template <int KP, int RBW, class T>
-void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs, const T *frag) {
+void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+                const T *frag) {
   constexpr int RB = KP / 16;
   const size_t lds = (size_t)RBW * 16 * KP * sizeof(T);
   static bool attr_set[64] = {};
@@ -205,7 +221,7 @@ void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t 
     attr_set[c->device & 63] = true;
   }
   hipLaunchKernelGGL((k_m2l_cls<KP, RBW, T>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(m2l_threads<T>()), lds, c->stream,
-                     items, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, frag,
+                     items, ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, frag,
                      (const T *)c->mexp.p, (T *)c->ypair.p);
   NB_LAUNCH();
 }
@@ -214,17 +230,19 @@ void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t 
 constexpr int kLds = 163840;
 
 template <int KP, class T>
-void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, const int32_t *cls_pairs, const T *frag) {
+void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+               const T *frag) {
   if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
-  launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, cls_pairs, frag);
+  launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag);
 }
 
 template <class T>
-void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, const int32_t *cls_pairs, const T *frag) {
+void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+            const T *frag) {
   switch (KP) {
 #define NB_M2L_CASE(k)                                        \
   case k:                                                     \
-    launch_kp<k, T>(c, nitems, items, cls_pairs, frag);       \
+    launch_kp<k, T>(c, nitems, items, ybase, cls_pairs, frag); \
     break;
     NB_M2L_CASE(16)
     NB_M2L_CASE(32)
@@ -375,49 +393,71 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32) {
   int32_t *hcnt = (int32_t *)c->hred.p; // pinned, >= kNumClasses ints
   NB_HIP(hipMemcpyAsync(hcnt, ccnt, sizeof(int32_t) * kNumClasses, hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> &hoff = c->h_cls_off;
+  hoff.assign(kNumClasses + 1, 0);
+  for (int k = 0; k < kNumClasses; ++k) hoff[k + 1] = hoff[k] + hcnt[k];
+  if (hoff[kNumClasses] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
+  // passes: consecutive ranges of the class-sorted pairs whose partials fit
+  // the buffer (at most kPassBytes, or the caller's pairs-per-pass limit)
+  constexpr int64_t kPassBytes = (int64_t)4 << 30;
+  int64_t pass = c->prm.m2l_pass_pairs > 0 ? c->prm.m2l_pass_pairs : kPassBytes / (KP * (int64_t)esz);
+  pass = std::max<int64_t>(1, std::min<int64_t>(pass, npair));
   int64_t per = npair / 2048;
   per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
-  std::vector<int32_t> &hoff = c->h_cls_off;
   std::vector<int4> &hitems = c->h_m2l_items;
-  hoff.assign(kNumClasses + 1, 0);
+  std::vector<int32_t> pass_item; // first item of every pass, then the item count
   hitems.clear();
-  for (int k = 0; k < kNumClasses; ++k) {
-    hoff[k + 1] = hoff[k] + hcnt[k];
-    for (int64_t b = hoff[k]; b < hoff[k + 1]; b += per)
-      hitems.push_back(make_int4(k, (int)b, (int)std::min<int64_t>(b + per, hoff[k + 1]), 0));
+  for (int64_t pb = 0; pb < npair; pb += pass) {
+    const int64_t pe = std::min<int64_t>(npair, pb + pass);
+    pass_item.push_back((int32_t)hitems.size());
+    for (int k = 0; k < kNumClasses; ++k) {
+      const int64_t kb = std::max<int64_t>(hoff[k], pb), ke = std::min<int64_t>(hoff[k + 1], pe);
+      for (int64_t b = kb; b < ke; b += per) hitems.push_back(make_int4(k, (int)b, (int)std::min<int64_t>(b + per, ke), 0));
+    }
   }
-  if (hoff[kNumClasses] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
+  pass_item.push_back((int32_t)hitems.size());
   NB_HIP(hipMemcpyAsync(ccnt + kNumClasses, hoff.data(), sizeof(int32_t) * kNumClasses, hipMemcpyHostToDevice, s));
-  int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair + 1);
+  int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair * 2 + 2);
+  int32_t *pos = cls_pairs + npair + 1; // class-sorted position of every list entry
   hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair,
-                     ccnt + kNumClasses, cls_pairs);
+                     ccnt + kNumClasses, cls_pairs, pos);
   NB_LAUNCH();
   const unsigned nitems = (unsigned)hitems.size();
   int4 *ditems = dget<int4>(c->m2l_items, (size_t)nitems + 1);
   NB_HIP(hipMemcpyAsync(ditems, hitems.data(), sizeof(int4) * nitems, hipMemcpyHostToDevice, s));
   P.n_m2l_tiles = nitems;
-  dbuf_ensure(c->ypair, (size_t)npair * KP * esz + 16);
+  dbuf_ensure(c->ypair, (size_t)pass * KP * esz + 16);
   void *Lexp = c->lexp.p; // sized and cleared by the caller (one slot per cell that holds a local)
   const int64_t nfrag = (int64_t)kNumClasses * KP * KP;
+  const void *frag = c->tmat.p;
   if (f32) {
     float *frag32 = dget<float>(c->tmat32, (size_t)nfrag);
     hipLaunchKernelGGL(k_to_f32, dim3(nblk(nfrag)), dim3(kT), 0, s, (const double *)c->tmat.p, nfrag, frag32);
     NB_LAUNCH();
-    mark(c, 7);
-    launch<float>(c, KP, nitems, ditems, cls_pairs, frag32);
-    mark(c, 8);
-    hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
-                       (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                       (const int32_t *)c->lslot.p, (const float *)c->ypair.p, (float *)Lexp);
-  } else {
-    mark(c, 7);
-    launch<double>(c, KP, nitems, ditems, cls_pairs, (const double *)c->tmat.p);
-    mark(c, 8);
-    hipLaunchKernelGGL(k_m2l_sum<double>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
-                       (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                       (const int32_t *)c->lslot.p, (const double *)c->ypair.p, (double *)Lexp);
+    frag = frag32;
   }
-  NB_LAUNCH();
+  mark(c, 7);
+  for (size_t ps = 0; ps + 1 < pass_item.size(); ++ps) {
+    const int32_t pb = (int32_t)(ps * pass), pe = (int32_t)std::min<int64_t>(npair, (int64_t)pb + pass);
+    const unsigned ni = (unsigned)(pass_item[ps + 1] - pass_item[ps]);
+    const int4 *it = ditems + pass_item[ps];
+    if (f32)
+      launch<float>(c, KP, ni, it, pb, cls_pairs, (const float *)frag);
+    else
+      launch<double>(c, KP, ni, it, pb, cls_pairs, (const double *)frag);
+    if (ps == 0) mark(c, 8);
+    if (f32)
+      hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
+                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
+                         (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
+                         (const float *)c->ypair.p, (float *)Lexp);
+    else
+      hipLaunchKernelGGL(k_m2l_sum<double>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
+                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
+                         (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
+                         (const double *)c->ypair.p, (double *)Lexp);
+    NB_LAUNCH();
+  }
 }
 
 void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells) {

@@ -25,7 +25,7 @@ class Params(ctypes.Structure):
         ("G", ctypes.c_double),
         ("eps2", ctypes.c_double),
         ("precision", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("m2l_pass_pairs", ctypes.c_int32),
         ("p2p_cut", ctypes.c_int64),
     ]
 
