@@ -53,7 +53,7 @@ typedef struct nbfmm_params {
   double G;               /* gravitational constant (reference 6.6743e-11, system_update_method.hpp:17) */
   double eps2;            /* softening of the direct sum (reference NaiveUpdate 1e-4, naive_update.cpp:27) */
   int32_t precision;      /* nbfmm_precision */
-  /* M2L pairs per pass of the translation (0: as many as 4 GiB of per-pair
+  /* M2L pairs per pass of the translation (0: as many as 16 GiB of per-pair
    * partials hold).  Larger interaction sets are done in several passes. */
   int32_t m2l_pass_pairs;
   /* An interaction-list pair (t, s) is evaluated by M2L, or by direct P2P

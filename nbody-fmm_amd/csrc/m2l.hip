@@ -398,8 +398,8 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32) {
   for (int k = 0; k < kNumClasses; ++k) hoff[k + 1] = hoff[k] + hcnt[k];
   if (hoff[kNumClasses] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
   // passes: consecutive ranges of the class-sorted pairs whose partials fit
-  // the buffer (at most kPassBytes, or the caller's pairs-per-pass limit)
-  constexpr int64_t kPassBytes = (int64_t)4 << 30;
+  // the buffer (at most kPassBytes of the 288 GB, or the caller's pairs-per-pass limit)
+  constexpr int64_t kPassBytes = (int64_t)16 << 30;
   int64_t pass = c->prm.m2l_pass_pairs > 0 ? c->prm.m2l_pass_pairs : kPassBytes / (KP * (int64_t)esz);
   pass = std::max<int64_t>(1, std::min<int64_t>(pass, npair));
   int64_t per = npair / 2048;
