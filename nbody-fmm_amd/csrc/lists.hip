@@ -136,6 +136,18 @@ __global__ void k_compact(const int32_t *flags, const int32_t *scan, int64_t n, 
     slot[i] = -1;
   }
 }
+__global__ void k_sum_i32(const int32_t *v, int64_t n, unsigned long long *out) {
+  __shared__ unsigned long long s[kT];
+  unsigned long long a = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) a += (unsigned)v[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
 __global__ void k_sum_i64(const int64_t *v, int64_t n, unsigned long long *out) {
   __shared__ unsigned long long s[kT];
   unsigned long long a = 0;
@@ -314,14 +326,25 @@ void build_lists(nbfmm_ctx *c) {
   scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_sub, off_sub, nc + 1, c->scan_tmp, s);
-  int32_t tot[3];
-  NB_HIP(hipMemcpyAsync(&tot[0], off_il + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&tot[1], off_m2l + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&tot[2], off_sub + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  // list lengths in 64 bits: the offsets above are 32-bit
+  unsigned long long *tot = (unsigned long long *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 52;
+  NB_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 3, s));
+  const unsigned nsb = std::min<unsigned>(512, nblk(nc));
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_il, nc, tot);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_m2l, nc, tot + 1);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_sub, nc, tot + 2);
+  NB_LAUNCH();
+  unsigned long long htot[3];
+  NB_HIP(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
-  P.n_il = tot[0];
-  P.n_m2l = tot[1];
-  P.n_sub = tot[2];
+  const unsigned long long lim = 0x7fffffffull - 1;
+  if (htot[1] > lim || htot[2] > lim || (c->keep_lists && htot[0] > lim))
+    throw Error(NBFMM_ERR_UNSUPPORTED, "interaction lists exceed 2^31 entries (raise leaf_capacity or p2p_cut)");
+  P.n_il = (int64_t)htot[0];
+  P.n_m2l = (int64_t)htot[1];
+  P.n_sub = (int64_t)htot[2];
   int32_t *il = c->keep_lists ? dget<int32_t>(c->il, (size_t)P.n_il + 1) : nullptr;
   int32_t *m2l_src = dget<int32_t>(c->m2l_src, (size_t)P.n_m2l + 1);
   uint16_t *m2l_cls = dget<uint16_t>(c->m2l_cls, (size_t)P.n_m2l + 1);
@@ -387,12 +410,15 @@ void build_lists(nbfmm_ctx *c) {
   hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nc))), dim3(kT), 0, s,
                      (const int64_t *)cell_pairs, nc, pairs);
   NB_LAUNCH();
-  int32_t ntask;
-  unsigned long long npairs;
-  NB_HIP(hipMemcpyAsync(&ntask, toff + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)tcnt, nc, tot);
+  NB_LAUNCH();
+  unsigned long long ntask, npairs;
+  NB_HIP(hipMemcpyAsync(&ntask, tot, sizeof(ntask), hipMemcpyDeviceToHost, s));
   NB_HIP(hipMemcpyAsync(&npairs, pairs, sizeof(npairs), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
-  P.n_tasks = ntask;
+  if (ntask > lim) throw Error(NBFMM_ERR_UNSUPPORTED, "P2P work list exceeds 2^31 tasks");
+  P.n_tasks = (int64_t)ntask;
   P.n_p2p_pairs = (int64_t)npairs;
   P2PTask *tasks = dget<P2PTask>(c->tasks, (size_t)ntask + 1);
   hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
