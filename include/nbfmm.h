@@ -47,6 +47,11 @@ typedef enum nbfmm_precision {
   NBFMM_MIXED = 1       /* FP32 P2P (compensated) and FP32 MFMA M2L; FP64 tree/integrator */
 } nbfmm_precision;
 
+typedef enum nbfmm_order_select {
+  NBFMM_ORDER_FIXED = 0,  /* every cell uses `order` (the reference's single L) */
+  NBFMM_ORDER_LEARNED = 1 /* evaluation cells take the order the context's model predicts */
+} nbfmm_order_select;
+
 typedef struct nbfmm_params {
   uint32_t order;         /* expansion order p (reference L, hardcoded 2 at tree.cpp:94) */
   uint32_t leaf_capacity; /* depth-loop threshold (reference 20, tree.cpp:32) */
@@ -61,10 +66,20 @@ typedef struct nbfmm_params {
    * reference does (node_i.cpp:105-120).  P2P is exact, so this only changes
    * cost and removes truncation error from the substituted pairs. */
   int64_t p2p_cut;
+  /* nbfmm_order_select.  With NBFMM_ORDER_LEARNED, `order` is the highest
+   * order (multipoles, L2L and the cells that push their local to their
+   * children use it) and every cell that evaluates its local at its particles
+   * gets the lowest order in [order_min, order] whose cumulative probability
+   * under the model reaches order_conf.  No counterpart in the reference
+   * (one L for all cells, tree.cpp:94). */
+  int32_t order_select;
+  int32_t order_min;  /* >= 4 */
+  double order_conf;  /* in (0, 1] */
 } nbfmm_params;
 
 /* Reference-compatible defaults: p = 2, leaf capacity 20, G = 6.6743e-11,
- * eps2 = 1e-4 (NaiveUpdate), FP64, p2p_cut = 0. */
+ * eps2 = 1e-4 (NaiveUpdate), FP64, p2p_cut = 0, fixed order (order_min 4,
+ * order_conf 0.9 for a later learned selection). */
 void nbfmm_default_params(nbfmm_params *params);
 
 typedef struct nbfmm_ctx nbfmm_ctx;
@@ -138,6 +153,61 @@ int nbfmm_diagnostics_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const 
 int nbfmm_run_dev(nbfmm_ctx *ctx, int64_t n, double *d_xyz, double *d_vel, double *d_acc, const double *d_m,
                   double dt, int nsteps, int integrator, int every, double *h_diag);
 
+/* Learned order selection (no counterpart in the reference).
+ *
+ * The model is an MLP 16 -> 64 -> 64 -> 11 (ReLU) over features of a cell,
+ * whose 11 outputs are the orders 4..14; FP32 weights, used as BF16 in the
+ * forward passes, FP32 accumulation.  Weight vector (NBFMM_MLP_WEIGHTS floats):
+ * W1[16][64], b1[64], W2[64][64], b2[64], W3[64][11], b3[11] (the
+ * NBFMM_MLP_PARAMS trainable values), then the input mean[16] and
+ * deviation[16] (features are used as (f - mean) / deviation). */
+#define NBFMM_MLP_FEATURES 16
+#define NBFMM_MLP_ORDERS 11
+#define NBFMM_MLP_ORDER_LO 4
+#define NBFMM_MLP_PARAMS 5963
+#define NBFMM_MLP_WEIGHTS 5995
+/* He-uniform initial weights from `seed`; the normalisation is taken from
+ * the training set by the first nbfmm_order_train call. */
+int nbfmm_order_model_init(nbfmm_ctx *ctx, uint64_t seed);
+int nbfmm_order_model_set(nbfmm_ctx *ctx, const float *w, int64_t nw);
+int nbfmm_order_model_get(nbfmm_ctx *ctx, float *w, int64_t nw);
+/* Training data from one particle set (device pointers): the FMM with the
+ * context's parameters is run at every order 4..14 (fixed selection, same
+ * tree); one particle in every `stride` (a seeded pick in each block of
+ * `stride`) is compared with the FP64 direct sum (compensated, no softening),
+ * and every cell that evaluates its local expansion gets the label "lowest
+ * order whose mean squared force error over its sampled particles is within
+ * tol^2 times the mean squared acceleration of all sampled particles" (the
+ * highest when none is; cells within it keep the global relative L2 error
+ * within tol).  The labelled cells' features are appended to
+ * the context's training set; *added receives their number.  Synchronous. */
+int nbfmm_order_dataset_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_m, int64_t stride,
+                            uint64_t seed, double tol, int64_t *added);
+/* Size of the training set; copies of its first `cap` rows (features
+ * [cap][16], labels [cap] as order - 4) when the pointers are not NULL. */
+int nbfmm_order_dataset_get(nbfmm_ctx *ctx, float *feat, int8_t *label, int64_t cap, int64_t *n);
+/* Appends rows given on the host; clears the set when n == 0 and both pointers are NULL. */
+int nbfmm_order_dataset_put(nbfmm_ctx *ctx, int64_t n, const float *feat, const int8_t *label);
+/* Minibatch Adam (lr, beta 0.9 / 0.999) on the mean cross entropy over the
+ * training set, shuffled by `seed`; loss[e] (host, may be NULL) receives the
+ * mean loss of epoch e.  Runs on the GPU; synchronous. */
+int nbfmm_order_train(nbfmm_ctx *ctx, int32_t epochs, int32_t batch, double lr, uint64_t seed, float *loss);
+/* The GPU forward and backward pass over nb samples given on the host
+ * (features [nb][16], labels as order - 4) with the current weights:
+ * grad[NBFMM_MLP_PARAMS] of the mean cross entropy and *loss.  Synchronous. */
+int nbfmm_order_grad(nbfmm_ctx *ctx, int32_t nb, const float *feat, const int8_t *label, float *grad, float *loss);
+/* Predicted probabilities prob[nb][11] of nb feature rows (host).  Synchronous. */
+int nbfmm_order_predict(nbfmm_ctx *ctx, int64_t nb, const float *feat, float *prob);
+/* nbfmm_eval_accel_dev followed by the sampled check: the relative L2 error
+ * of the accelerations against the FP64 direct sum (compensated, no
+ * softening) on about ntgt sampled targets.  When it exceeds `gate`, the
+ * evaluation is repeated with a fixed order `fallback_order` and checked
+ * again; *err receives the error of the accelerations left in d_acc and
+ * *fell_back whether the fallback ran.  Synchronous. */
+int nbfmm_eval_accel_checked_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_m, double *d_acc,
+                                 int64_t ntgt, uint64_t seed, double gate, int32_t fallback_order, double *err,
+                                 int32_t *fell_back);
+
 /* Statistics of the last evaluation: sizes for the roofline and per-phase
  * GPU times measured with HIP events on the context's stream. */
 typedef struct nbfmm_stats {
@@ -161,8 +231,11 @@ typedef struct nbfmm_stats {
   double ms_total;          /* first to last event of the evaluation */
   double ms_m2l_kernel;     /* the MFMA M2L contraction kernel alone */
   double ms_p2p_kernel;     /* the P2P kernel alone */
-  double flop_m2l;          /* algorithmic M2L flops: 2 (p+1)^4 per M2L pair */
+  double flop_m2l;          /* algorithmic M2L flops: 2 (q+1)^4 per M2L pair of order q */
   double flop_p2p;          /* algorithmic P2P flops: 20 per particle pair */
+  double mean_order;        /* mean order over the M2L pairs */
+  int32_t fallback;         /* nbfmm_eval_accel_checked_dev fell back to the fixed order */
+  int32_t reserved0;
 } nbfmm_stats;
 int nbfmm_get_stats(nbfmm_ctx *ctx, nbfmm_stats *stats);
 /* Enables (1) or disables (0) the per-phase events (on by default). */

@@ -36,6 +36,14 @@ int check_params(const nbfmm_params *p) {
   if (p->m2l_pass_pairs < 0) return fail(NBFMM_ERR_ARG, "m2l_pass_pairs must be >= 0");
   if (p->precision != NBFMM_PLAIN_FP64 && p->precision != NBFMM_MIXED)
     return fail(NBFMM_ERR_ARG, "unknown precision");
+  if (p->order_select != NBFMM_ORDER_FIXED && p->order_select != NBFMM_ORDER_LEARNED)
+    return fail(NBFMM_ERR_ARG, "unknown order_select");
+  if (p->order_select == NBFMM_ORDER_LEARNED) {
+    if (p->order_min < kOrderLo || p->order_min > (int32_t)p->order)
+      return fail(NBFMM_ERR_ARG, "order_min must be in [4, order]");
+    if (p->order < (uint32_t)kOrderLo) return fail(NBFMM_ERR_ARG, "learned order selection needs order >= 4");
+    if (!(p->order_conf > 0.0 && p->order_conf <= 1.0)) return fail(NBFMM_ERR_ARG, "order_conf must be in (0, 1]");
+  }
   return NBFMM_OK;
 }
 
@@ -92,9 +100,22 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
     S.ms_m2l_kernel = km;
     S.ms_p2p_kernel = kp;
   }
-  const double p1 = (double)c->prm.order + 1.0;
-  S.flop_m2l = 2.0 * p1 * p1 * p1 * p1 * (double)P.n_m2l;
+  S.flop_m2l = P.flop_m2l;
   S.flop_p2p = 20.0 * (double)P.n_p2p_pairs;
+  S.mean_order = P.mean_order;
+}
+
+// the evaluation with the parameters `q` in place of the context's
+void eval_with(nbfmm_ctx *c, const nbfmm_params &q, int64_t n, const double *x, const double *m, double *a) {
+  const nbfmm_params save = c->prm;
+  c->prm = q;
+  try {
+    eval_dev(c, n, x, m, a);
+  } catch (...) {
+    c->prm = save;
+    throw;
+  }
+  c->prm = save;
 }
 
 // SyplecticEuler::integrate (symplectic_euler.cpp:8-15), one step
@@ -162,7 +183,9 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,
-                 &p2m_off, &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi};
+                 &p2m_off, &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
+                 &ord, &msum, &feat, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
+                 &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc};
   for (DBuf *b : all) dbuf_free(*b);
   if (hred.p) (void)hipHostFree(hred.p);
   for (auto &e : ev)
@@ -184,6 +207,9 @@ void nbfmm_default_params(nbfmm_params *p) {
   p->eps2 = 1.0e-4;
   p->precision = NBFMM_PLAIN_FP64;
   p->p2p_cut = 0;
+  p->order_select = NBFMM_ORDER_FIXED;
+  p->order_min = 4;
+  p->order_conf = 0.9;
 }
 
 const char *nbfmm_last_error(void) { return g_err.c_str(); }
@@ -207,8 +233,8 @@ int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
   NB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   NB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   for (auto &e : c->ev) NB_HIP(hipEventCreate(&e));
-  NB_HIP(hipHostMalloc(&c->hred.p, 4096, hipHostMallocDefault));
-  c->hred.cap = 4096;
+  NB_HIP(hipHostMalloc(&c->hred.p, 65536, hipHostMallocDefault));
+  c->hred.cap = 65536;
   *out = c;
   c = nullptr;
   }
@@ -361,6 +387,127 @@ int nbfmm_run_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, cons
     if (h_diag && s % every == 0) monitor();
   }
   NB_HIP(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+int nbfmm_order_model_init(nbfmm_ctx *c, uint64_t seed) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  API_BEGIN
+  mlp_init(c, seed);
+  API_END
+}
+
+int nbfmm_order_model_set(nbfmm_ctx *c, const float *w, int64_t nw) {
+  if (!c || !w) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nw != kMlpWeights) return fail(NBFMM_ERR_ARG, "the weight vector has NBFMM_MLP_WEIGHTS entries");
+  API_BEGIN
+  for (int64_t i = 0; i < nw; ++i)
+    if (!std::isfinite(w[i])) throw Error(NBFMM_ERR_ARG, "non-finite weight");
+  for (int i = 0; i < kMlpIn; ++i)
+    if (!(w[kMlpParams + kMlpIn + i] > 0.0f)) throw Error(NBFMM_ERR_ARG, "feature deviations must be > 0");
+  mlp_set(c, w);
+  API_END
+}
+
+int nbfmm_order_model_get(nbfmm_ctx *c, float *w, int64_t nw) {
+  if (!c || !w) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nw < kMlpWeights) return fail(NBFMM_ERR_ARG, "the weight vector has NBFMM_MLP_WEIGHTS entries");
+  API_BEGIN
+  mlp_get(c, w);
+  API_END
+}
+
+int nbfmm_order_dataset_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t stride,
+                            uint64_t seed, double tol, int64_t *added) {
+  if (!c || !d_xyz || !d_m) return fail(NBFMM_ERR_ARG, "null argument");
+  if (stride < 1) return fail(NBFMM_ERR_ARG, "stride must be >= 1");
+  if (!(tol > 0.0)) return fail(NBFMM_ERR_ARG, "tol must be > 0");
+  API_BEGIN
+  check_n(n);
+  double *acc = dget<double>(c->chk_acc, (size_t)n * 3);
+  label_begin(c, n, stride, seed, d_xyz, d_m);
+  nbfmm_params q = c->prm;
+  q.order_select = NBFMM_ORDER_FIXED;
+  for (int qi = 0; qi < kNumOrders; ++qi) {
+    q.order = (uint32_t)(kOrderLo + qi);
+    eval_with(c, q, n, d_xyz, d_m, acc);
+    label_add(c, qi, acc);
+  }
+  const int64_t a = label_finish(c, tol);
+  if (added) *added = a;
+  API_END
+}
+
+int nbfmm_order_dataset_get(nbfmm_ctx *c, float *feat, int8_t *label, int64_t cap, int64_t *n) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  API_BEGIN
+  if (n) *n = c->ds_n;
+  if ((feat || label) && cap > 0) dataset_get(c, feat, label, cap);
+  API_END
+}
+
+int nbfmm_order_dataset_put(nbfmm_ctx *c, int64_t n, const float *feat, const int8_t *label) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  API_BEGIN
+  if (n == 0 && !feat && !label) {
+    c->ds_n = 0;
+  } else {
+    if (n < 1 || !feat || !label) throw Error(NBFMM_ERR_ARG, "n >= 1 rows with features and labels");
+    for (int64_t i = 0; i < n; ++i)
+      if (label[i] < 0 || label[i] >= kNumOrders) throw Error(NBFMM_ERR_ARG, "labels are order - 4 in [0, 10]");
+    dataset_put(c, n, feat, label);
+  }
+  API_END
+}
+
+int nbfmm_order_train(nbfmm_ctx *c, int32_t epochs, int32_t batch, double lr, uint64_t seed, float *loss) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  if (epochs < 0 || batch < 1 || !(lr > 0.0)) return fail(NBFMM_ERR_ARG, "epochs >= 0, batch >= 1, lr > 0");
+  API_BEGIN
+  mlp_train(c, epochs, batch, lr, seed, loss);
+  API_END
+}
+
+int nbfmm_order_grad(nbfmm_ctx *c, int32_t nb, const float *feat, const int8_t *label, float *grad, float *loss) {
+  if (!c || !feat || !label || !grad || !loss) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nb < 1) return fail(NBFMM_ERR_ARG, "nb must be >= 1");
+  API_BEGIN
+  for (int i = 0; i < nb; ++i)
+    if (label[i] < 0 || label[i] >= kNumOrders) throw Error(NBFMM_ERR_ARG, "labels are order - 4 in [0, 10]");
+  mlp_grad(c, nb, feat, label, grad, loss);
+  API_END
+}
+
+int nbfmm_order_predict(nbfmm_ctx *c, int64_t nb, const float *feat, float *prob) {
+  if (!c || !feat || !prob) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nb < 1) return fail(NBFMM_ERR_ARG, "nb must be >= 1");
+  API_BEGIN
+  mlp_predict(c, nb, feat, prob);
+  API_END
+}
+
+int nbfmm_eval_accel_checked_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, double *d_acc,
+                                 int64_t ntgt, uint64_t seed, double gate, int32_t fallback_order, double *err,
+                                 int32_t *fell_back) {
+  if (!c || !d_xyz || !d_m || !d_acc) return fail(NBFMM_ERR_ARG, "null argument");
+  if (ntgt < 1 || !(gate > 0.0)) return fail(NBFMM_ERR_ARG, "ntgt >= 1 and gate > 0");
+  if (fallback_order < 1 || fallback_order > 14) return fail(NBFMM_ERR_ARG, "fallback_order must be in [1, 14]");
+  API_BEGIN
+  check_n(n);
+  eval_dev(c, n, d_xyz, d_m, d_acc);
+  double e = sampled_error(c, n, d_xyz, d_m, d_acc, ntgt, seed);
+  int fb = 0;
+  if (!(e <= gate)) {
+    nbfmm_params q = c->prm;
+    q.order_select = NBFMM_ORDER_FIXED;
+    q.order = (uint32_t)fallback_order;
+    eval_with(c, q, n, d_xyz, d_m, d_acc);
+    e = sampled_error(c, n, d_xyz, d_m, d_acc, ntgt, seed);
+    fb = 1;
+  }
+  c->stats.fallback = fb;
+  if (err) *err = e;
+  if (fell_back) *fell_back = fb;
   API_END
 }
 

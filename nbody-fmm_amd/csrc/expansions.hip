@@ -63,23 +63,6 @@ void fill_consts(Consts &C) {
 }
 
 
-__device__ __forceinline__ void cell_centre(uint64_t key, int lev, const LevelGeo &g, double c[3]) {
-  uint64_t k = key;
-  uint32_t q[3] = {0, 0, 0};
-  for (int b = 0; b < lev; ++b) {
-    q[0] |= (uint32_t)((k >> (3 * b)) & 1u) << b;
-    q[1] |= (uint32_t)((k >> (3 * b + 1)) & 1u) << b;
-    q[2] |= (uint32_t)((k >> (3 * b + 2)) & 1u) << b;
-  }
-  for (int d = 0; d < 3; ++d) c[d] = g.lo[d] + ((double)q[d] + 0.5) * g.h[lev][d];
-}
-
-__device__ __forceinline__ int level_of(int32_t cell, const LevelGeo &g) {
-  int l = 0;
-  while (l < g.depth && cell >= g.lvl_start[l + 1]) ++l;
-  return l;
-}
-
 // ---- M2L operators -----------------------------------------------------------------
 // Yoshida singular harmonics S_N^M (N <= 2p, 0 <= M <= N) of the scaled class
 // offset, then the real KP x KP matrix in MFMA A-fragment order:
@@ -427,7 +410,8 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
                                             const int32_t *__restrict__ pcell, const int32_t *__restrict__ cparent,
                                             const uint64_t *__restrict__ ckey, const int32_t *__restrict__ eslot,
                                             LevelGeo g, const Consts *__restrict__ C, const T *__restrict__ Lexp,
-                                            double *__restrict__ grad, double *__restrict__ pot) {
+                                            double *__restrict__ grad, double *__restrict__ pot,
+                                            const int8_t *__restrict__ ord) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double4 q = pos[i];
@@ -436,6 +420,7 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
   for (int lev = g.depth; lev >= 2; --lev) {
     const int32_t ts = eslot[cell];
     if (ts >= 0) {
+      const int pc = ord ? (int)ord[cell] : p; // order of this cell's local
       double ctr[3];
       cell_centre(ckey[cell], lev, g, ctr);
       const double ir = g.inv_rho[lev];
@@ -444,7 +429,7 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
       const T *L = Lexp + (size_t)ts * KP;
       T hx = 0, hy = 0, hz = 0, hv = 0;
       T vr = 1, vi = 0; // R_m^m
-      for (int m = 0; m < (POT ? p + 1 : p); ++m) {
+      for (int m = 0; m < (POT ? pc + 1 : pc); ++m) {
         if (m > 0) {
           const T f = (T)C->mmf[m];
           const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
@@ -453,7 +438,7 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
         }
         const T w = m == 0 ? (T)1 : (T)2;
         T pr = 0, pi = 0, cr = vr, ci = vi;
-        for (int np = m; np < (POT ? p + 1 : p); ++np) {
+        for (int np = m; np < (POT ? pc + 1 : pc); ++np) {
           if (np == m + 1) {
             pr = cr;
             pi = ci;
@@ -469,12 +454,12 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
             ci = ni;
           }
           if (POT) {
-            const cpx<T> Lv = lget(L, np, m, p);
+            const cpx<T> Lv = lget(L, np, m, pc);
             hv += w * (cr * Lv.x - ci * Lv.y);
-            if (np == p) continue;
+            if (np == pc) continue;
           }
           const int nn = np + 1;
-          const cpx<T> Lm = lget(L, nn, m - 1, p), L0 = lget(L, nn, m, p), Lp = lget(L, nn, m + 1, p);
+          const cpx<T> Lm = lget(L, nn, m - 1, pc), L0 = lget(L, nn, m, pc), Lp = lget(L, nn, m + 1, pc);
           const T cz = (T)C->cz[nn][m], cu = (T)C->cu[nn][m], cdd = (T)C->cd[nn][m];
           // Re(R * G)
           hz += w * cz * (cr * L0.x - ci * L0.y);
@@ -603,14 +588,8 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT > 0 ? (n +
 
 int padded_k(int p) { return ((p + 1) * (p + 1) + 15) / 16 * 16; }
 
-void run_expansions(nbfmm_ctx *c) {
-  Plan &P = c->plan;
-  hipStream_t s = c->stream;
-  const int p = (int)c->prm.order;
-  const int KP = padded_k(p);
-  P.kp = KP;
+LevelGeo level_geo(const Plan &P) {
   const int D = P.geo.depth;
-  const bool f32 = c->prm.precision == NBFMM_MIXED;
   LevelGeo g;
   std::memset(&g, 0, sizeof(g));
   for (int d = 0; d < 3; ++d) g.lo[d] = P.geo.lo[d];
@@ -620,6 +599,21 @@ void run_expansions(nbfmm_ctx *c) {
   }
   for (int l = 0; l <= D + 1; ++l) g.lvl_start[l] = P.lvl_start[l];
   g.depth = D;
+  return g;
+}
+
+void run_expansions(nbfmm_ctx *c) {
+  Plan &P = c->plan;
+  hipStream_t s = c->stream;
+  int p = (int)c->prm.order; // with learned orders: the highest order in use, set below
+  int KP = padded_k(p);
+  P.kp = KP;
+  const int D = P.geo.depth;
+  const bool f32 = c->prm.precision == NBFMM_MIXED;
+  const bool learned = c->prm.order_select == NBFMM_ORDER_LEARNED;
+  const LevelGeo g = level_geo(P);
+  P.flop_m2l = 0;
+  P.mean_order = p;
   // constants
   Consts *dC = (Consts *)dbuf_ensure(c->consts, sizeof(Consts));
   {
@@ -636,6 +630,7 @@ void run_expansions(nbfmm_ctx *c) {
   double *grad = dget<double>(c->grad, (size_t)P.n * (pot ? 4 : 3));
   double *pfar = pot ? grad + 3 * (size_t)P.n : nullptr;
   if (P.n_tgt == 0) {
+    c->n_local = 0;
     NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s));
     P.n_p2m_terms = P.n_l2p_terms = 0;
     launch_p2p(c);
@@ -649,17 +644,6 @@ void run_expansions(nbfmm_ctx *c) {
   const int64_t nc = P.n_cells;
   const int32_t leaf_begin = P.lvl_start[D];
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
-  // ---- operators: M2L classes and the M2M / L2L octants (level independent) ----
-  const double rho = P.geo.rho(D);
-  const double hx = P.geo.h(D, 0) / rho, hy = P.geo.h(D, 1) / rho, hz = P.geo.h(D, 2) / rho;
-  double *frag = dget<double>(c->tmat, (size_t)kNumClasses * KP * KP);
-  hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, hx, hy, hz, frag);
-  NB_LAUNCH();
-  double *trans = dget<double>(c->trans, (size_t)16 * KP * KP);
-  hipLaunchKernelGGL(k_build_trans, dim3(8, 2), dim3(kT), 0, s, p, KP, hx, hy, hz, trans,
-                     trans + (size_t)8 * KP * KP);
-  NB_LAUNCH();
-  if (f32) trans_to_f32(c, KP);
   // ---- plan: modes per level (top-down), slots, lists ----
   int32_t *mode_m = dget<int32_t>(c->mode_m, (size_t)nc + 1);
   int32_t *mode_l = dget<int32_t>(c->mode_l, (size_t)nc + 1);
@@ -703,6 +687,29 @@ void run_expansions(nbfmm_ctx *c) {
   NB_LAUNCH();
   NB_HIP(hipStreamSynchronize(s));
   const int32_t nM = h[0], nP2M = h[1], nL = h[3];
+  c->n_local = nL;
+  // orders per cell (learned selection): the expansions are formed to the
+  // highest order in use
+  const int8_t *ord = nullptr;
+  if (learned) {
+    ord = order_select(c, g);
+    p = c->ord_max;
+    KP = padded_k(p);
+    P.kp = KP;
+    P.mean_order = p;
+  }
+  // ---- operators: M2L classes and the M2M / L2L octants (level independent) ----
+  const double rho = P.geo.rho(D);
+  const double hx = P.geo.h(D, 0) / rho, hy = P.geo.h(D, 1) / rho, hz = P.geo.h(D, 2) / rho;
+  double *frag = dget<double>(c->tmat, (size_t)kNumClasses * KP * KP);
+  hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, hx, hy, hz, frag);
+  NB_LAUNCH();
+  double *trans = dget<double>(c->trans, (size_t)16 * KP * KP);
+  hipLaunchKernelGGL(k_build_trans, dim3(8, 2), dim3(kT), 0, s, p, KP, hx, hy, hz, trans,
+                     trans + (size_t)8 * KP * KP);
+  NB_LAUNCH();
+  if (f32) trans_to_f32(c, KP);
+
   int32_t bm[kMaxDepth + 2], bl[kMaxDepth + 2];
   for (int d = 0; d <= D + 1; ++d) {
     bm[d] = h[8 + d];
@@ -748,7 +755,7 @@ void run_expansions(nbfmm_ctx *c) {
   // ---- M2L (writes the locals of its targets) ----
   dbuf_ensure(c->lexp, (size_t)nL * KP * esz + 16);
   NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));
-  run_m2l(c, KP, f32);
+  run_m2l(c, KP, f32, ord);
   // ---- L2L, coarsest level first ----
   for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
   mark(c, 4);
@@ -756,11 +763,11 @@ void run_expansions(nbfmm_ctx *c) {
   if (f32)
     hipLaunchKernelGGL((pot ? k_l2p<float, true> : k_l2p<float, false>), dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n,
                        (const double4 *)c->pos.p, (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey,
-                       (const int32_t *)eslot, g, dC, (const float *)c->lexp.p, grad, pfar);
+                       (const int32_t *)eslot, g, dC, (const float *)c->lexp.p, grad, pfar, ord);
   else
     hipLaunchKernelGGL((pot ? k_l2p<double, true> : k_l2p<double, false>), dim3(nblk(P.n)), dim3(kT), 0, s, p, KP,
                        P.n, (const double4 *)c->pos.p, (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p,
-                       ckey, (const int32_t *)eslot, g, dC, (const double *)c->lexp.p, grad, pfar);
+                       ckey, (const int32_t *)eslot, g, dC, (const double *)c->lexp.p, grad, pfar, ord);
   NB_LAUNCH();
   mark(c, 5);
 }

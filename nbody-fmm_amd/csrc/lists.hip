@@ -10,12 +10,36 @@
 // 216 candidate cells in LDS once, in window order, and classifies them for
 // every child; ballot prefix counts keep the reference order when the entries
 // are written.  A listed pair (t, s) is evaluated by M2L, or directly (P2P)
-// when n_t * n_s < p2p_cut.
+// when n_t * n_s < p2p_cut or when its offset class is not admissible for
+// the cell shape (see class_mask).
 #include "nbfmm_internal.h"
 
 #include <algorithm>
+#include <cmath>
 
 namespace nbfmm {
+
+// The reference's cells are boxes of the bounding box's aspect ratio
+// (tree.cpp:20-30, not cubed).  The M2L of an offset class d converges at the
+// rate (R_s + R_t) / |d|, R the half diagonal of a cell: below 0.87 for cubes
+// (the nearest classes, |d| = 2 h), but at or above 1 for the short axis of a
+// box whose longest side exceeds sqrt(2) times its shortest, where the series
+// no longer converges for particles near the cell corners.  Pairs of classes
+// with a rate above kMacRatio are evaluated directly, like the pairs below
+// p2p_cut (exact; for bounding boxes within 1.15:1 no class is affected).
+ClsMask class_mask(const Geometry &geo) {
+  ClsMask m{};
+  const double hx = geo.h(0, 0), hy = geo.h(0, 1), hz = geo.h(0, 2);
+  const double diag2 = hx * hx + hy * hy + hz * hz;
+  for (int k = 0; k < kNumClasses; ++k) {
+    const int dx = k % 7 - 3, dy = (k / 7) % 7 - 3, dz = k / 49 - 3;
+    if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) < 2) continue;
+    const double d2 = dx * dx * hx * hx + dy * dy * hy * hy + dz * dz * hz * hz;
+    if (diag2 > kMacRatio * kMacRatio * d2) m.w[k >> 5] |= 1u << (k & 31);
+  }
+  return m;
+}
+
 namespace {
 
 constexpr int kT = 256;
@@ -35,7 +59,8 @@ constexpr int64_t kTaskPairs = 1 << 17;
 template <bool FILL>
 __global__ void __launch_bounds__(kT) k_lists(int32_t nparents, int32_t skip_from, const int32_t *__restrict__ nbr27,
                                               const int32_t *__restrict__ child8, const int32_t *__restrict__ cbeg,
-                                              const int32_t *__restrict__ cend, int64_t cut, int32_t *cnt_il,
+                                              const int32_t *__restrict__ cend, int64_t cut, ClsMask bad,
+                                              int32_t *cnt_il,
                                               int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
                                               const int32_t *off_il, const int32_t *off_m2l, const int32_t *off_sub,
                                               int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls, int32_t *sub_src,
@@ -91,7 +116,8 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t nparents, int32_t skip_fro
       const int dx = w % 6 - 2 - bx, dy = (w / 6) % 6 - 2 - by, dz = w / 36 - 2 - bz;
       const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
       const bool far = q >= 0 && !near;
-      const bool sub = far && nt * ns < cut;
+      const int cls = (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3);
+      const bool sub = far && (nt * ns < cut || ((bad.w[cls >> 5] >> (cls & 31)) & 1u));
       const bool m2l = far && !sub;
       const uint64_t bf = __ballot(far), bs = __ballot(sub), bm = __ballot(m2l);
       if (FILL) {
@@ -100,7 +126,7 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t nparents, int32_t skip_fro
         if (m2l) {
           const int32_t k = a_m + c_m + __popcll(bm & lt);
           m2l_src[k] = q;
-          m2l_cls[k] = (uint16_t)((dz + 3) * 49 + (dy + 3) * 7 + (dx + 3));
+          m2l_cls[k] = (uint16_t)cls;
           is_src[q] = 1;
         }
       } else if (sub) {
@@ -318,8 +344,9 @@ void build_lists(nbfmm_ctx *c) {
   NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
   NB_HIP(hipMemsetAsync(sub_pairs, 0, sizeof(int64_t) * (size_t)(nc + 1), s));
   const int32_t nparents = leaf_begin; // every non-leaf cell is a parent
+  const ClsMask bad = class_mask(P.geo);
   const unsigned nlb = (unsigned)((nparents + kT / kW - 1) / (kT / kW));
-  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut,
+  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
                      cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                      nullptr, nullptr);
   NB_LAUNCH();
@@ -351,7 +378,7 @@ void build_lists(nbfmm_ctx *c) {
   int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
   int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
   NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  hipLaunchKernelGGL(k_lists<true>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut,
+  hipLaunchKernelGGL(k_lists<true>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
                      nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il, m2l_src,
                      m2l_cls, sub_src, is_src);
   NB_LAUNCH();

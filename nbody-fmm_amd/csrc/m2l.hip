@@ -16,6 +16,13 @@
 // Plain precision runs v_mfma_f64_16x16x4f64 on FP64 coefficients; mixed
 // precision v_mfma_f32_16x16x4f32 on FP32 coefficients (the rho scaling keeps
 // them O(1)), with the per-target sum of the partials in FP64.
+//
+// With an order per target cell (learned order selection), the pairs are
+// sorted by (order, class): a pair whose target has order q uses the leading
+// (q+1)^2 x (q+1)^2 block of the class operator (coefficients of degree <= q
+// come first in the storage order), the tile width is the padded block
+// KPT = pad16((q+1)^2), source coefficients past (q+1)^2 are masked and the
+// sum takes the first (q+1)^2 rows.
 #include "nbfmm_internal.h"
 
 #include <algorithm>
@@ -57,29 +64,34 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int kg, int q) { return 4 * kg + q; }
 };
 
-__global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cls, int64_t n, int32_t *cnt) {
-  __shared__ int32_t h[kNumClasses];
-  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
+// bins: the class, plus kNumClasses * (order - kOrderLo) with per-target orders
+constexpr int kMaxBins = kNumClasses * kNumOrders;
+
+__global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cls, int64_t n, int nbins,
+                                                 int32_t *cnt) {
+  __shared__ int32_t h[kMaxBins];
+  for (int k = threadIdx.x; k < nbins; k += kT) h[k] = 0;
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
   for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
   __syncthreads();
-  for (int k = threadIdx.x; k < kNumClasses; k += kT)
+  for (int k = threadIdx.x; k < nbins; k += kT)
     if (h[k]) atomicAdd(&cnt[k], h[k]);
 }
 
-// cursor[k] starts at the first position of class k; the order of the pairs
-// inside a class is free (every pair's result is computed independently).
-__global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, int64_t n, int32_t *cursor,
-                                                    int32_t *__restrict__ out, int32_t *__restrict__ inv) {
-  __shared__ int32_t h[kNumClasses];
-  __shared__ int32_t base[kNumClasses];
-  for (int k = threadIdx.x; k < kNumClasses; k += kT) h[k] = 0;
+// cursor[k] starts at the first position of bin k; the order of the pairs
+// inside a bin is free (every pair's result is computed independently).
+__global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, int64_t n, int nbins,
+                                                    int32_t *cursor, int32_t *__restrict__ out,
+                                                    int32_t *__restrict__ inv) {
+  __shared__ int32_t h[kMaxBins];
+  __shared__ int32_t base[kMaxBins];
+  for (int k = threadIdx.x; k < nbins; k += kT) h[k] = 0;
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * kClsChunk, e = min(n, b + kClsChunk);
   for (int64_t i = b + threadIdx.x; i < e; i += kT) atomicAdd(&h[cls[i]], 1);
   __syncthreads();
-  for (int k = threadIdx.x; k < kNumClasses; k += kT) {
+  for (int k = threadIdx.x; k < nbins; k += kT) {
     base[k] = h[k] ? atomicAdd(&cursor[k], h[k]) : 0;
     h[k] = 0;
   }
@@ -97,22 +109,44 @@ __global__ void k_to_f32(const double *__restrict__ in, int64_t n, float *__rest
   if (i < n) out[i] = (float)in[i];
 }
 
+// bin of every M2L pair with per-target orders: class + kNumClasses * (order - kOrderLo)
+__global__ void k_order_bins(int64_t ntgt, const int32_t *__restrict__ tgt_cells, const int32_t *__restrict__ off_m2l,
+                             const int32_t *__restrict__ cnt_m2l, const int8_t *__restrict__ ord,
+                             uint16_t *__restrict__ cls) {
+  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (t >= ntgt) return;
+  const int32_t c = tgt_cells[t];
+  const int add = kNumClasses * ((int)ord[c] - kOrderLo);
+  for (int32_t q = off_m2l[c] + (threadIdx.x & 63), e = off_m2l[c] + cnt_m2l[c]; q < e; q += 64)
+    cls[q] = (uint16_t)(cls[q] + add);
+}
+
+// KP: tile width (padded block of the pair's order); KPS: storage width of
+// the operators, multipoles and partials; nvalid: source coefficients used
+// ((q+1)^2 for order q; KPS when every pair has the storage order).
 template <int KP, int RBW, class T>
 __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__restrict__ items, int32_t ybase,
                                                          const int32_t *__restrict__ cls_pairs,
                                                          const int32_t *__restrict__ m2l_src,
                                                          const int32_t *__restrict__ src_slot, const T *__restrict__ frag,
-                                                         const T *__restrict__ Mexp, T *__restrict__ Y) {
+                                                         const T *__restrict__ Mexp, T *__restrict__ Y, int KPS,
+                                                         int nvalid) {
   constexpr int RB = KP / 16, KS = KP / 4;
   extern __shared__ double4 s_raw[];
   T *sA = reinterpret_cast<T *>(s_raw);
   const int4 it = items[blockIdx.x];
+  const int cls = it.x % kNumClasses;
   const int r0 = (int)blockIdx.y * RBW;
   const int nrb = min(RBW, RB - r0);
   {
-    const double4 *A = reinterpret_cast<const double4 *>(frag + ((size_t)it.x * RB + r0) * KS * 64);
-    const int n32 = (int)(nrb * KS * 64 * sizeof(T) / 32);
-    for (int e = threadIdx.x; e < n32; e += m2l_threads<T>()) s_raw[e] = A[e];
+    // row blocks r0.. of the class operator, k steps [0, KS) of the KPS / 4 stored
+    constexpr int V = (int)(KS * 64 * sizeof(T) / 32); // 32-byte vectors per row block
+    const size_t rowv = (size_t)(KPS / 4) * 64 * sizeof(T) / 32;
+    const double4 *A = reinterpret_cast<const double4 *>(frag + (size_t)cls * KPS * KPS) + (size_t)r0 * rowv;
+    for (int e = threadIdx.x; e < nrb * V; e += m2l_threads<T>()) {
+      const int r = e / V, v = e - r * V;
+      s_raw[e] = A[(size_t)r * rowv + v];
+    }
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -120,14 +154,14 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
   for (int pb = it.y + wave * 16; pb < it.z; pb += (m2l_threads<T>() / 64) * 16) {
     const int pi = pb + col;
     const int32_t pid = pi < it.z ? cls_pairs[pi] : -1;
-    const T *Mb = Mexp + (size_t)(pid >= 0 ? src_slot[m2l_src[pid]] : 0) * KP + kg;
+    const T *Mb = Mexp + (size_t)(pid >= 0 ? src_slot[m2l_src[pid]] : 0) * KPS + kg;
     typename Mfma<T>::V acc[RBW];
 #pragma unroll
     for (int r = 0; r < RBW; ++r) acc[r] = typename Mfma<T>::V{0, 0, 0, 0};
     if constexpr (sizeof(T) == 8) {
       T bv[KS];
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) bv[ks] = pid >= 0 ? Mb[ks * 4] : (T)0;
+      for (int ks = 0; ks < KS; ++ks) bv[ks] = (pid >= 0 && ks * 4 + kg < nvalid) ? Mb[ks * 4] : (T)0;
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
@@ -138,11 +172,12 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
       // (a fully unrolled k loop exhausts the registers for KP = 96..128)
       T bc[4], bn[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) bc[u] = pid >= 0 ? Mb[u * 4] : (T)0;
+      for (int u = 0; u < 4; ++u) bc[u] = (pid >= 0 && u * 4 + kg < nvalid) ? Mb[u * 4] : (T)0;
 #pragma unroll 1
       for (int k0 = 0; k0 < KS; k0 += 4) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) bn[u] = (pid >= 0 && k0 + 4 < KS) ? Mb[(k0 + 4 + u) * 4] : (T)0;
+        for (int u = 0; u < 4; ++u)
+          bn[u] = (pid >= 0 && k0 + 4 < KS && (k0 + 4 + u) * 4 + kg < nvalid) ? Mb[(k0 + 4 + u) * 4] : (T)0;
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -153,7 +188,7 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
       }
     }
     if (pid >= 0) {
-      T *y = Y + (size_t)(pi - ybase) * KP + r0 * 16;
+      T *y = Y + (size_t)(pi - ybase) * KPS + r0 * 16;
 #pragma unroll
       for (int r = 0; r < RBW; ++r)
         if (r < nrb)
@@ -165,18 +200,20 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
 
 // L of every M2L target += sum of the partials of its pairs in the pass
 // [cb, ce) of class-sorted positions (pos[q] of list entry q), in list order,
-// in FP64.
+// in FP64.  With per-cell orders (ord != null) only the (q+1)^2 coefficients
+// of the target's order q are added.
 template <class T>
 __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
                                                 const int32_t *__restrict__ lslot, const int32_t *__restrict__ pos,
                                                 int32_t cb, int32_t ce, const T *__restrict__ Y,
-                                                T *__restrict__ Lexp) {
+                                                T *__restrict__ Lexp, const int8_t *__restrict__ ord) {
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (t >= ntgt) return;
   const int32_t c = tgt_cells[t];
   const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
+  const int nv = ord ? ((int)ord[c] + 1) * ((int)ord[c] + 1) : KP; // rows added
   T *L = Lexp + (size_t)lslot[c] * KP;
   // rows lane, lane + 64, ... (KP <= 256), pairs four at a time
   double s[4] = {0, 0, 0, 0};
@@ -190,18 +227,18 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
       if (pr[u] >= cb && pr[u] < ce)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)(pr[u] - cb) * KP + lane + 64 * r];
+          if (lane + 64 * r < nv) s[r] += (double)Y[(size_t)(pr[u] - cb) * KP + lane + 64 * r];
   }
   for (; q < e; ++q) {
     const int32_t pr = pos[q];
     if (pr >= cb && pr < ce)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (lane + 64 * r < KP) s[r] += (double)Y[(size_t)(pr - cb) * KP + lane + 64 * r];
+        if (lane + 64 * r < nv) s[r] += (double)Y[(size_t)(pr - cb) * KP + lane + 64 * r];
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    if (lane + 64 * r < KP) L[lane + 64 * r] = (T)((double)L[lane + 64 * r] + s[r]);
+    if (lane + 64 * r < nv) L[lane + 64 * r] = (T)((double)L[lane + 64 * r] + s[r]);
 }
 
 template <class T>
@@ -211,7 +248,7 @@ constexpr int rbw_for(int KP, int lds) {
 
 template <int KP, int RBW, class T>
 void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
-                const T *frag) {
+                const T *frag, int KPS, int nvalid) {
   constexpr int RB = KP / 16;
   const size_t lds = (size_t)RBW * 16 * KP * sizeof(T);
   static bool attr_set[64] = {};
@@ -222,7 +259,7 @@ void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase,
   }
   hipLaunchKernelGGL((k_m2l_cls<KP, RBW, T>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(m2l_threads<T>()), lds, c->stream,
                      items, ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, frag,
-                     (const T *)c->mexp.p, (T *)c->ypair.p);
+                     (const T *)c->mexp.p, (T *)c->ypair.p, KPS, nvalid);
   NB_LAUNCH();
 }
 
@@ -231,18 +268,18 @@ constexpr int kLds = 163840;
 
 template <int KP, class T>
 void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
-               const T *frag) {
+               const T *frag, int KPS, int nvalid) {
   if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
-  launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag);
+  launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
 }
 
 template <class T>
 void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
-            const T *frag) {
+            const T *frag, int KPS, int nvalid) {
   switch (KP) {
-#define NB_M2L_CASE(k)                                        \
-  case k:                                                     \
-    launch_kp<k, T>(c, nitems, items, ybase, cls_pairs, frag); \
+#define NB_M2L_CASE(k)                                                      \
+  case k:                                                                   \
+    launch_kp<k, T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid); \
     break;
     NB_M2L_CASE(16)
     NB_M2L_CASE(32)
@@ -376,51 +413,81 @@ inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT 
 
 } // namespace
 
-// Sorts the pairs by class (counts -> host schedule -> scatter), runs the
+// Sorts the pairs by bin (counts -> host schedule -> scatter), runs the
 // MFMA contraction (between events 7 and 8) and the per-target sum.
 // Expects the FP64 operators in c->tmat and the multipoles in c->mexp
-// (FP32 when `f32`).
-void run_m2l(nbfmm_ctx *c, int KP, bool f32) {
+// (FP32 when `f32`), both of width KP.  `ord` (device, per cell) gives the
+// order of every target, or is null when all have the order of KP.
+void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
   Plan &P = c->plan;
   hipStream_t s = c->stream;
   const int64_t nt = P.n_tgt, npair = P.n_m2l;
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
-  int32_t *ccnt = dget<int32_t>(c->cls_cnt, 2 * kNumClasses + 8);
-  NB_HIP(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * kNumClasses, s));
+  const int nbins = ord ? kMaxBins : kNumClasses;
+  if (ord) {
+    hipLaunchKernelGGL(k_order_bins, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, nt,
+                       (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                       ord, (uint16_t *)c->m2l_cls.p);
+    NB_LAUNCH();
+  }
+  int32_t *ccnt = dget<int32_t>(c->cls_cnt, 2 * kMaxBins + 8);
+  NB_HIP(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * nbins, s));
   const unsigned nchunk = (unsigned)((npair + kClsChunk - 1) / kClsChunk);
-  hipLaunchKernelGGL(k_cls_hist, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, ccnt);
+  hipLaunchKernelGGL(k_cls_hist, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, nbins, ccnt);
   NB_LAUNCH();
-  int32_t *hcnt = (int32_t *)c->hred.p; // pinned, >= kNumClasses ints
-  NB_HIP(hipMemcpyAsync(hcnt, ccnt, sizeof(int32_t) * kNumClasses, hipMemcpyDeviceToHost, s));
+  int32_t *hcnt = (int32_t *)c->hred.p; // pinned, >= kMaxBins ints
+  NB_HIP(hipMemcpyAsync(hcnt, ccnt, sizeof(int32_t) * nbins, hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   std::vector<int32_t> &hoff = c->h_cls_off;
-  hoff.assign(kNumClasses + 1, 0);
-  for (int k = 0; k < kNumClasses; ++k) hoff[k + 1] = hoff[k] + hcnt[k];
-  if (hoff[kNumClasses] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
-  // passes: consecutive ranges of the class-sorted pairs whose partials fit
-  // the buffer (at most kPassBytes of the 288 GB, or the caller's pairs-per-pass limit)
+  hoff.assign(nbins + 1, 0);
+  for (int k = 0; k < nbins; ++k) hoff[k + 1] = hoff[k] + hcnt[k];
+  if (hoff[nbins] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
+  // algorithmic flops: 2 n^2 per pair, n = (q+1)^2 coefficients of the pair's order q
+  {
+    double fl = 0;
+    const int p = (int)c->prm.order;
+    for (int k = 0; k < nbins; ++k) {
+      const int q = ord ? kOrderLo + k / kNumClasses : p;
+      const double nq = (double)(q + 1) * (q + 1);
+      fl += 2.0 * nq * nq * hcnt[k];
+    }
+    P.flop_m2l = fl;
+    double qs = 0;
+    for (int k = 0; k < nbins; ++k) qs += (double)(ord ? kOrderLo + k / kNumClasses : p) * hcnt[k];
+    P.mean_order = npair > 0 ? qs / (double)npair : (double)p;
+  }
+  // passes: consecutive ranges of the sorted pairs whose partials fit the
+  // buffer (at most kPassBytes of the 288 GB, or the caller's pairs-per-pass limit)
   constexpr int64_t kPassBytes = (int64_t)16 << 30;
   int64_t pass = c->prm.m2l_pass_pairs > 0 ? c->prm.m2l_pass_pairs : kPassBytes / (KP * (int64_t)esz);
   pass = std::max<int64_t>(1, std::min<int64_t>(pass, npair));
   int64_t per = npair / 2048;
   per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
   std::vector<int4> &hitems = c->h_m2l_items;
-  std::vector<int32_t> pass_item; // first item of every pass, then the item count
+  // launches: (pass, first item, item count, order index) -- items of one pass
+  // and one order form one launch
+  std::vector<int4> runs;
   hitems.clear();
   for (int64_t pb = 0; pb < npair; pb += pass) {
     const int64_t pe = std::min<int64_t>(npair, pb + pass);
-    pass_item.push_back((int32_t)hitems.size());
-    for (int k = 0; k < kNumClasses; ++k) {
+    for (int k = 0; k < nbins; ++k) {
       const int64_t kb = std::max<int64_t>(hoff[k], pb), ke = std::min<int64_t>(hoff[k + 1], pe);
-      for (int64_t b = kb; b < ke; b += per) hitems.push_back(make_int4(k, (int)b, (int)std::min<int64_t>(b + per, ke), 0));
+      if (kb >= ke) continue;
+      const int oi = k / kNumClasses;
+      const int ps = (int)(pb / pass);
+      if (runs.empty() || runs.back().x != ps || runs.back().w != oi)
+        runs.push_back(make_int4(ps, (int)hitems.size(), 0, oi));
+      for (int64_t b = kb; b < ke; b += per) {
+        hitems.push_back(make_int4(k, (int)b, (int)std::min<int64_t>(b + per, ke), 0));
+        ++runs.back().z;
+      }
     }
   }
-  pass_item.push_back((int32_t)hitems.size());
-  NB_HIP(hipMemcpyAsync(ccnt + kNumClasses, hoff.data(), sizeof(int32_t) * kNumClasses, hipMemcpyHostToDevice, s));
+  NB_HIP(hipMemcpyAsync(ccnt + kMaxBins, hoff.data(), sizeof(int32_t) * nbins, hipMemcpyHostToDevice, s));
   int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair * 2 + 2);
-  int32_t *pos = cls_pairs + npair + 1; // class-sorted position of every list entry
-  hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair,
-                     ccnt + kNumClasses, cls_pairs, pos);
+  int32_t *pos = cls_pairs + npair + 1; // sorted position of every list entry
+  hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, nbins,
+                     ccnt + kMaxBins, cls_pairs, pos);
   NB_LAUNCH();
   const unsigned nitems = (unsigned)hitems.size();
   int4 *ditems = dget<int4>(c->m2l_items, (size_t)nitems + 1);
@@ -437,25 +504,30 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32) {
     frag = frag32;
   }
   mark(c, 7);
-  for (size_t ps = 0; ps + 1 < pass_item.size(); ++ps) {
-    const int32_t pb = (int32_t)(ps * pass), pe = (int32_t)std::min<int64_t>(npair, (int64_t)pb + pass);
-    const unsigned ni = (unsigned)(pass_item[ps + 1] - pass_item[ps]);
-    const int4 *it = ditems + pass_item[ps];
-    if (f32)
-      launch<float>(c, KP, ni, it, pb, cls_pairs, (const float *)frag);
-    else
-      launch<double>(c, KP, ni, it, pb, cls_pairs, (const double *)frag);
+  const int npass = (int)((npair + pass - 1) / pass);
+  size_t r = 0;
+  for (int ps = 0; ps < npass; ++ps) {
+    const int32_t pb = (int32_t)((int64_t)ps * pass), pe = (int32_t)std::min<int64_t>(npair, (int64_t)pb + pass);
+    for (; r < runs.size() && runs[r].x == ps; ++r) {
+      const int q = ord ? kOrderLo + runs[r].w : (int)c->prm.order;
+      const int KPT = ord ? padded_k(q) : KP, nvalid = ord ? (q + 1) * (q + 1) : KP;
+      const int4 *it = ditems + runs[r].y;
+      if (f32)
+        launch<float>(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const float *)frag, KP, nvalid);
+      else
+        launch<double>(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const double *)frag, KP, nvalid);
+    }
     if (ps == 0) mark(c, 8);
     if (f32)
       hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                          (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
                          (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
-                         (const float *)c->ypair.p, (float *)Lexp);
+                         (const float *)c->ypair.p, (float *)Lexp, ord);
     else
       hipLaunchKernelGGL(k_m2l_sum<double>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                          (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
                          (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
-                         (const double *)c->ypair.p, (double *)Lexp);
+                         (const double *)c->ypair.p, (double *)Lexp, ord);
     NB_LAUNCH();
   }
 }

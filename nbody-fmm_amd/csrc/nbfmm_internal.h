@@ -44,6 +44,14 @@ constexpr int kNumClasses = 343; // offsets in [-3,3]^3, class = (dz+3)*49+(dy+3
 constexpr int kMaxOrder = 16;
 constexpr int kTaskTargets = 256; // targets of one P2P sub-task (one wavefront, 4 per lane)
 constexpr int kMaxList = 192;     // longest source-cell list of a P2P task (interaction list <= 189)
+// orders a learned selection may give a cell
+constexpr int kOrderLo = 4, kOrderHi = 14, kNumOrders = kOrderHi - kOrderLo + 1;
+// the order-selection MLP: kMlpIn -> kMlpHidden -> kMlpHidden -> kNumOrders,
+// kMlpParams trainable FP32 values then the input mean and deviation
+constexpr int kMlpIn = 16, kMlpHidden = 64;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
+                           kMlpHidden * kNumOrders + kNumOrders;
+constexpr int kMlpWeights = kMlpParams + 2 * kMlpIn;
 
 // One P2P sub-task: targets [cbeg[tcell] + toff, + kTaskTargets) of cell
 // tcell against the particles [f0, f1) of the concatenated particle ranges
@@ -76,6 +84,32 @@ struct LevelGeo {
   int depth;
 };
 
+// geometric centre of the cell with Morton key `key` at level lev
+__device__ __forceinline__ void cell_centre(uint64_t key, int lev, const LevelGeo &g, double c[3]) {
+  uint64_t k = key;
+  uint32_t q[3] = {0, 0, 0};
+  for (int b = 0; b < lev; ++b) {
+    q[0] |= (uint32_t)((k >> (3 * b)) & 1u) << b;
+    q[1] |= (uint32_t)((k >> (3 * b + 1)) & 1u) << b;
+    q[2] |= (uint32_t)((k >> (3 * b + 2)) & 1u) << b;
+  }
+  for (int d = 0; d < 3; ++d) c[d] = g.lo[d] + ((double)q[d] + 0.5) * g.h[lev][d];
+}
+
+__device__ __forceinline__ int level_of(int32_t cell, const LevelGeo &g) {
+  int l = 0;
+  while (l < g.depth && cell >= g.lvl_start[l + 1]) ++l;
+  return l;
+}
+
+// Offset classes whose M2L does not converge fast enough for the cell shape
+// (bit k: class k); their pairs are evaluated directly (lists.hip).
+struct ClsMask {
+  uint32_t w[(kNumClasses + 31) / 32];
+};
+constexpr double kMacRatio = 0.9;
+ClsMask class_mask(const Geometry &geo);
+
 // ---- the device-side state of one evaluation -------------------------------------
 struct Plan {
   int64_t n = 0;
@@ -87,6 +121,8 @@ struct Plan {
   int64_t n_src = 0, n_tgt = 0, n_p2p_pairs = 0, n_p2m_terms = 0, n_l2p_terms = 0;
   int64_t n_p2m_items = 0, n_m2l_tiles = 0, n_tasks = 0;
   int kp = 0; // padded coefficient count (multiple of 16)
+  double flop_m2l = 0; // algorithmic M2L flops of the evaluation
+  double mean_order = 0; // mean order over the M2L pairs (the fixed order without selection)
 };
 
 } // namespace nbfmm
@@ -127,6 +163,17 @@ struct nbfmm_ctx {
   std::vector<int4> h_m2l_items;
   // integrator scratch
   nbfmm::DBuf xk, vk, ak, asum, vsum, diag_acc, diag_phi;
+  // learned order selection: per-cell orders and features, the model, its Adam
+  // state and training scratch, the training set and the label runs
+  nbfmm::DBuf ord, msum, feat, mlp_w, mlp_wb, mlp_m, mlp_v, mlp_act, mlp_perm;
+  nbfmm::DBuf ds_feat, ds_label, lab_tgt, lab_dir, lab_sum, lab_flag, lab_lab, chk_acc;
+  int64_t n_local = 0; // cells with a local expansion in the last evaluation
+  int ord_max = 0;     // highest order selected in the last evaluation
+  bool mlp_ok = false, mlp_fresh = false;
+  int64_t mlp_step = 0, ds_n = 0;
+  int64_t lab_ns = 0, lab_stride = 1, lab_nl = -1;
+  uint64_t lab_seed = 0;
+  int fallback = 0; // the last checked evaluation fell back to the fixed order
   // events
   hipEvent_t ev[16] = {};
   int nev = 0;
@@ -137,14 +184,33 @@ namespace nbfmm {
 // scan.hip
 void scan_excl_i32(const int32_t *in, int32_t *out, int64_t n, DBuf &tmp, hipStream_t s);
 void scan_excl_i64(const int64_t *in, int64_t *out, int64_t n, DBuf &tmp, hipStream_t s);
+void scan_excl_f64(const double *in, double *out, int64_t n, DBuf &tmp, hipStream_t s);
 // tree.hip
 void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m);
 // lists.hip
 void build_lists(nbfmm_ctx *c);
 // expansions.hip
 void run_expansions(nbfmm_ctx *c);
+int padded_k(int p); // pad16((p+1)^2)
+LevelGeo level_geo(const Plan &P);
+// order.hip
+void order_features(nbfmm_ctx *c, const LevelGeo &g);
+const int8_t *order_select(nbfmm_ctx *c, const LevelGeo &g);
+void mlp_set(nbfmm_ctx *c, const float *w);
+void mlp_get(nbfmm_ctx *c, float *w);
+void mlp_init(nbfmm_ctx *c, uint64_t seed);
+void mlp_predict(nbfmm_ctx *c, int64_t nb, const float *feat_h, float *prob_h);
+void mlp_grad(nbfmm_ctx *c, int nb, const float *feat_h, const int8_t *lab_h, float *grad_h, float *loss_h);
+void mlp_train(nbfmm_ctx *c, int epochs, int batch, double lr, uint64_t seed, float *loss_h);
+void label_begin(nbfmm_ctx *c, int64_t n, int64_t stride, uint64_t seed, const double *d_xyz, const double *d_m);
+void label_add(nbfmm_ctx *c, int qi, const double *d_acc);
+int64_t label_finish(nbfmm_ctx *c, double tol);
+double sampled_error(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, const double *d_acc,
+                     int64_t ntgt, uint64_t seed);
+void dataset_get(nbfmm_ctx *c, float *feat_h, int8_t *lab_h, int64_t cap);
+void dataset_put(nbfmm_ctx *c, int64_t n, const float *feat_h, const int8_t *lab_h);
 // m2l.hip
-void run_m2l(nbfmm_ctx *c, int KP, bool f32);
+void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord);
 // M2M (l2l false) or L2L (true) for `ncell` parents listed in `cells` (device)
 void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells);
 void trans_to_f32(nbfmm_ctx *c, int KP);

@@ -97,5 +97,8 @@ void scan_excl_i32(const int32_t *in, int32_t *out, int64_t n, DBuf &tmp, hipStr
 void scan_excl_i64(const int64_t *in, int64_t *out, int64_t n, DBuf &tmp, hipStream_t s) {
   scan_excl<int64_t>(in, out, n, tmp, s);
 }
+void scan_excl_f64(const double *in, double *out, int64_t n, DBuf &tmp, hipStream_t s) {
+  scan_excl<double>(in, out, n, tmp, s);
+}
 
 } // namespace nbfmm

@@ -16,6 +16,13 @@ LIB_PATH = os.path.join(HERE, "libnbfmm.so")
 
 NBFMM_PLAIN_FP64 = 0
 NBFMM_MIXED = 1
+NBFMM_ORDER_FIXED = 0
+NBFMM_ORDER_LEARNED = 1
+MLP_FEATURES = 16
+MLP_ORDERS = 11
+MLP_ORDER_LO = 4
+MLP_PARAMS = 5963
+MLP_WEIGHTS = 5995
 
 
 class Params(ctypes.Structure):
@@ -27,6 +34,9 @@ class Params(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("m2l_pass_pairs", ctypes.c_int32),
         ("p2p_cut", ctypes.c_int64),
+        ("order_select", ctypes.c_int32),
+        ("order_min", ctypes.c_int32),
+        ("order_conf", ctypes.c_double),
     ]
 
 
@@ -54,6 +64,9 @@ class Stats(ctypes.Structure):
         ("ms_p2p_kernel", ctypes.c_double),
         ("flop_m2l", ctypes.c_double),
         ("flop_p2p", ctypes.c_double),
+        ("mean_order", ctypes.c_double),
+        ("fallback", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -66,6 +79,9 @@ EXPORTS = [
     "nbfmm_step_se_dev", "nbfmm_step_rk4_dev", "nbfmm_get_stats", "nbfmm_set_timing",
     "nbfmm_keep_lists", "nbfmm_tree_sizes", "nbfmm_tree_get", "nbfmm_last_error",
     "nbfmm_eval_accel_pot_dev", "nbfmm_diagnostics_dev", "nbfmm_run_dev",
+    "nbfmm_order_model_init", "nbfmm_order_model_set", "nbfmm_order_model_get", "nbfmm_order_dataset_dev",
+    "nbfmm_order_dataset_get", "nbfmm_order_dataset_put", "nbfmm_order_train", "nbfmm_order_grad",
+    "nbfmm_order_predict", "nbfmm_eval_accel_checked_dev",
 ]
 
 DIAG_NAMES = ["kinetic", "potential", "total", "px", "py", "pz", "lx", "ly", "lz", "mass"]
@@ -100,6 +116,17 @@ def lib():
         L.nbfmm_eval_accel_pot_dev.argtypes = [vp, i64, vp, vp, vp, vp]
         L.nbfmm_diagnostics_dev.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.nbfmm_run_dev.argtypes = [vp, i64, vp, vp, vp, vp, dbl, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        u64, i32 = ctypes.c_uint64, ctypes.c_int32
+        L.nbfmm_order_model_init.argtypes = [vp, u64]
+        L.nbfmm_order_model_set.argtypes = [vp, vp, i64]
+        L.nbfmm_order_model_get.argtypes = [vp, vp, i64]
+        L.nbfmm_order_dataset_dev.argtypes = [vp, i64, vp, vp, i64, u64, dbl, vp]
+        L.nbfmm_order_dataset_get.argtypes = [vp, vp, vp, i64, vp]
+        L.nbfmm_order_dataset_put.argtypes = [vp, i64, vp, vp]
+        L.nbfmm_order_train.argtypes = [vp, i32, i32, dbl, u64, vp]
+        L.nbfmm_order_grad.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.nbfmm_order_predict.argtypes = [vp, i64, vp, vp]
+        L.nbfmm_eval_accel_checked_dev.argtypes = [vp, i64, vp, vp, vp, i64, u64, dbl, i32, vp, vp]
         _lib = L
     return _lib
 
@@ -205,6 +232,71 @@ class Context:
         _check(lib().nbfmm_run_dev(self.h, n, dx, dv, da, dm, dt, nsteps, integrator, every,
                                    _ptr(rec) if monitor else None))
         return rec
+
+    # ---- learned order selection ----
+    def order_model_init(self, seed=0):
+        _check(lib().nbfmm_order_model_init(self.h, seed))
+
+    def order_model_set(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().nbfmm_order_model_set(self.h, _ptr(w), len(w)))
+
+    def order_model_get(self):
+        w = np.zeros(MLP_WEIGHTS, np.float32)
+        _check(lib().nbfmm_order_model_get(self.h, _ptr(w), len(w)))
+        return w
+
+    def order_dataset_dev(self, n, d_xyz, d_m, stride, seed=0, tol=1e-6):
+        """Labels the evaluation cells of one particle set; returns the rows added."""
+        added = ctypes.c_int64()
+        _check(lib().nbfmm_order_dataset_dev(self.h, n, d_xyz, d_m, stride, seed, tol, ctypes.byref(added)))
+        return added.value
+
+    def order_dataset(self):
+        """(features [n, 16] float32, labels [n] int8 as order - 4) of the training set."""
+        n = ctypes.c_int64()
+        _check(lib().nbfmm_order_dataset_get(self.h, None, None, 0, ctypes.byref(n)))
+        f = np.zeros((n.value, MLP_FEATURES), np.float32)
+        y = np.zeros(n.value, np.int8)
+        if n.value:
+            _check(lib().nbfmm_order_dataset_get(self.h, _ptr(f), _ptr(y), n.value, None))
+        return f, y
+
+    def order_dataset_put(self, feat=None, label=None):
+        """Appends rows; with no arguments clears the training set."""
+        if feat is None:
+            _check(lib().nbfmm_order_dataset_put(self.h, 0, None, None))
+            return
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        label = np.ascontiguousarray(label, dtype=np.int8)
+        _check(lib().nbfmm_order_dataset_put(self.h, len(label), _ptr(feat), _ptr(label)))
+
+    def order_train(self, epochs, batch=1024, lr=3e-3, seed=0):
+        loss = np.zeros(max(epochs, 1), np.float32)
+        _check(lib().nbfmm_order_train(self.h, epochs, batch, lr, seed, _ptr(loss)))
+        return loss[:epochs]
+
+    def order_grad(self, feat, label):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        label = np.ascontiguousarray(label, dtype=np.int8)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().nbfmm_order_grad(self.h, len(label), _ptr(feat), _ptr(label), _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def order_predict(self, feat):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        prob = np.zeros((len(feat), MLP_ORDERS), np.float32)
+        _check(lib().nbfmm_order_predict(self.h, len(feat), _ptr(feat), _ptr(prob)))
+        return prob
+
+    def eval_accel_checked_dev(self, n, d_xyz, d_m, d_acc, ntgt=4096, seed=0, gate=1e-6, fallback_order=14):
+        """Evaluation plus sampled direct-sum check; returns (error, fell_back)."""
+        err = ctypes.c_double()
+        fb = ctypes.c_int32()
+        _check(lib().nbfmm_eval_accel_checked_dev(self.h, n, d_xyz, d_m, d_acc, ntgt, seed, gate, fallback_order,
+                                                  ctypes.byref(err), ctypes.byref(fb)))
+        return err.value, bool(fb.value)
 
     def stats(self):
         s = Stats()

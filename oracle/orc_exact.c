@@ -272,11 +272,22 @@ int orc_fmm_exact(int64_t n, const double *xyz, const double *mass, uint32_t pu,
       for (size_t q = 0; q < cs; ++q) Mc[cs * par + q] += tmp[q];
     }
   /* interaction lists: M2L, or direct P2P between the two cells when
-   * n_t * n_s < p2p_cut */
+   * n_t * n_s < p2p_cut or when the offset is not admissible for the cell
+   * shape: (R_s + R_t) / |d| > 0.9 with R the half diagonal of a cell (the
+   * library's rule for boxes of the bounding box's aspect ratio) */
+  const double ex = t->hi[0] - t->lo[0], ey = t->hi[1] - t->lo[1], ez = t->hi[2] - t->lo[2];
+  const double diag2 = ex * ex + ey * ey + ez * ez;
   for (int64_t c = 0; c < nc; ++c)
     for (int64_t q = t->il_off[c]; q < t->il_off[c + 1]; ++q) {
       int64_t s = t->il[q];
-      if ((t->pend[c] - t->pbeg[c]) * (t->pend[s] - t->pbeg[s]) < p2p_cut) {
+      uint32_t qc[3], qs[3];
+      orc_morton_decode3(t->key[c], &qc[0], &qc[1], &qc[2]);
+      orc_morton_decode3(t->key[s], &qs[0], &qs[1], &qs[2]);
+      const double ox = (double)qs[0] - (double)qc[0], oy = (double)qs[1] - (double)qc[1],
+                   oz = (double)qs[2] - (double)qc[2];
+      const double d2 = ox * ox * ex * ex + oy * oy * ey * ey + oz * oz * ez * ez;
+      const int bad = diag2 > 0.9 * 0.9 * d2;
+      if (bad || (t->pend[c] - t->pbeg[c]) * (t->pend[s] - t->pbeg[s]) < p2p_cut) {
         p2p_cells(t, xyz, mass, c, s, grad);
         continue;
       }

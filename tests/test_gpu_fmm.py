@@ -80,6 +80,34 @@ def test_fmm_parity_exact(ctx, p, cut):
     assert _rel(a, b) < 1e-12
 
 
+@pytest.mark.parametrize("cut", [0, 256])
+def test_fmm_parity_elongated(ctx, cut):
+    # a 2.5 : 1.25 : 1 bounding box: the nearest classes along the short axes
+    # are not admissible for M2L and go to P2P (same rule in the oracle)
+    rng = np.random.default_rng(17)
+    xyz = rng.random((6000, 3)) * np.array([2.5, 1.25, 1.0])
+    m = rng.uniform(0.5, 1.5, 6000)
+    ctx.set_params(order=8, leaf_capacity=16, p2p_cut=cut, G=1.0, eps2=0.0)
+    a = ctx.eval_accel(xyz, m)
+    b = orc.fmm_exact(xyz, m, 8, 16, G=1.0, center_mode=0, p2p_cut=cut)
+    assert _rel(a, b) < 1e-12
+
+
+def test_elongated_box_converges(ctx):
+    # with the inadmissible classes evaluated directly the error keeps falling with p
+    rng = np.random.default_rng(18)
+    xyz = rng.random((8000, 3)) * np.array([1.6, 1.0, 1.0])
+    m = rng.uniform(0.5, 1.5, 8000)
+    ctx.set_params(G=1.0, eps2=0.0)
+    d = ctx.direct_accel(xyz, m)
+    err = {}
+    for p in (4, 8, 12):
+        ctx.set_params(order=p, leaf_capacity=16, p2p_cut=0)
+        err[p] = _rel(ctx.eval_accel(xyz, m), d)
+    print(f"[elongated] errors {err}")
+    assert err[12] < 0.1 * err[8] < 0.01 * err[4]
+
+
 def test_fmm_parity_plummer(ctx):
     xyz, m = workloads.plummer(40000, seed=1)
     ctx.set_params(order=8, leaf_capacity=64, p2p_cut=512, G=1.0, eps2=0.0)

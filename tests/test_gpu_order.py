@@ -1,0 +1,204 @@
+"""GPU tests of the learned order selection.
+
+- The GPU forward, backward and weight-gradient kernels against the CPU
+  oracle (oracle/mlp.py, itself pinned by finite differences).
+- Per-cell orders in the FMM: a model that picks one order for every cell
+  reproduces the fixed-order evaluation at that order; a model that splits
+  the cells between two orders gives M2L flops and errors between those of
+  the two fixed orders.
+- Training on the GPU lowers the loss on a learnable synthetic set.
+- End to end on a clustered mix of Plummer spheres: labels from the FMM
+  against sampled direct sums, training, then the checked evaluation meets
+  the error gate with fewer M2L flops than the lowest fixed order that
+  meets it.  The sampled direct-sum check is the judge of the error; when it
+  fails the evaluation falls back to the fixed order.
+"""
+import numpy as np
+import pytest
+
+import mlp
+import nbfmm
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = nbfmm.Context(order=10, leaf_capacity=16, G=1.0, eps2=0.0)
+    yield c
+    c.close()
+
+
+def _const_model(k):
+    """Zero weights, b3 favouring output k (order 4 + k) with probability ~1."""
+    w = np.zeros(nbfmm.MLP_WEIGHTS, np.float32)
+    w[mlp.O_SD:] = 1.0
+    w[mlp.O_B3 + k] = 40.0
+    return w
+
+
+def _split_model():
+    """Order 6 for cells with more than 100 M2L sources (feature 4), else 10."""
+    w = np.zeros(nbfmm.MLP_WEIGHTS, np.float32)
+    w[mlp.O_SD:] = 1.0
+    w[mlp.O_W1 + 4 * 64 + 0] = 1.0           # a1[0] = count
+    w[mlp.O_W2 + 0 * 64 + 0] = 1.0           # a2[0] = count
+    w[mlp.O_W3 + 0 * 11 + (6 - 4)] = 0.5     # z(order 6) = count / 2 - 50.25
+    w[mlp.O_B3:mlp.O_MU] = -1000.0
+    w[mlp.O_B3 + (6 - 4)] = -50.25
+    w[mlp.O_B3 + (10 - 4)] = 0.0             # z(order 10) = 0
+    return w
+
+
+def test_mlp_grad_and_probs_vs_oracle(ctx):
+    ctx.order_model_init(seed=3)
+    w = ctx.order_model_get()
+    rng = np.random.default_rng(8)
+    # move the normalisation and biases off their initial values
+    w[mlp.O_MU:mlp.O_SD] = rng.normal(0, 0.5, 16)
+    w[mlp.O_SD:] = rng.uniform(0.5, 2.0, 16)
+    w[mlp.O_B1:mlp.O_W2] = rng.normal(0, 0.2, 64)
+    w[mlp.O_B3:mlp.O_MU] = rng.normal(0, 0.5, 11)
+    ctx.order_model_set(w)
+    assert np.array_equal(ctx.order_model_get(), w)
+    for nb in (1, 300, 1000):
+        feat = rng.normal(0, 2.0, (nb, 16)).astype(np.float32)
+        lab = rng.integers(0, 11, nb).astype(np.int8)
+        g, loss = ctx.order_grad(feat, lab)
+        L_ref, g_ref = mlp.loss_grad(w, feat, lab, bf16=True)
+        err = float(np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref))
+        print(f"[mlp] nb={nb} loss {loss:.6f} (oracle {L_ref:.6f}), grad rel err {err:.2e}")
+        assert abs(loss - L_ref) <= 1e-5 * max(1.0, abs(L_ref))
+        assert err < 1e-4
+        # per-block check, so a wrong block cannot hide behind a large one
+        for a, b in ((mlp.O_W1, mlp.O_B1), (mlp.O_B1, mlp.O_W2), (mlp.O_W2, mlp.O_B2), (mlp.O_B2, mlp.O_W3),
+                     (mlp.O_W3, mlp.O_B3), (mlp.O_B3, mlp.O_MU)):
+            nr = np.linalg.norm(g_ref[a:b])
+            assert np.linalg.norm(g[a:b] - g_ref[a:b]) <= 1e-4 * nr + 1e-9, (a, b)
+        p = ctx.order_predict(feat)
+        assert np.allclose(p, mlp.probs(w, feat), atol=2e-5)
+
+
+def test_training_lowers_loss(ctx):
+    # labels a fixed function of two features: learnable
+    rng = np.random.default_rng(4)
+    feat = rng.normal(0, 1, (6000, 16)).astype(np.float32)
+    lab = np.clip(np.round(2.5 * feat[:, 0] + 1.5 * feat[:, 3] + 5), 0, 10).astype(np.int8)
+    ctx.order_dataset_put()
+    ctx.order_dataset_put(feat, lab)
+    f2, l2 = ctx.order_dataset()
+    assert np.array_equal(f2, feat) and np.array_equal(l2, lab)
+    ctx.order_model_init(seed=1)
+    loss = ctx.order_train(epochs=30, batch=256, lr=3e-3, seed=2)
+    print(f"[train] loss {loss[0]:.3f} -> {loss[-1]:.3f}")
+    assert loss[-1] < 0.5 * loss[0]
+    acc = (ctx.order_predict(feat).argmax(1) == lab).mean()
+    print(f"[train] accuracy {acc:.3f}")
+    assert acc > 0.8
+    # the trained weights still match the oracle's forward pass
+    w = ctx.order_model_get()
+    assert np.allclose(ctx.order_predict(feat[:500]), mlp.probs(w, feat[:500]), atol=2e-5)
+    ctx.order_dataset_put()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_per_cell_orders(ctx, precision):
+    xyz, m = workloads.plummer(20000, seed=31)
+    n = len(m)
+    dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
+    da = nbfmm.DeviceArray(np.zeros((n, 3)))
+    tgt = workloads.sample_targets(n, 512, seed=1)
+    d_t = nbfmm.DeviceArray(tgt)
+    d_r = nbfmm.DeviceArray(np.zeros((len(tgt), 3)))
+    ctx.direct_accel_dev(n, dx.ptr, dm.ptr, len(tgt), d_t.ptr, 1, d_r.ptr)
+    ref = d_r.download(ctx.stream)
+    base = dict(leaf_capacity=16, p2p_cut=256, precision=precision)
+    try:
+        res = {}
+        for p in (6, 10):
+            ctx.set_params(order=p, order_select=0, **base)
+            ctx.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+            res[p] = (da.download(ctx.stream), ctx.stats())
+        # every evaluation cell at the full order: the fixed evaluation
+        ctx.order_model_set(_const_model(10 - 4))
+        ctx.set_params(order=10, order_select=1, order_min=4, order_conf=0.9, **base)
+        ctx.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+        a_full, st = da.download(ctx.stream), ctx.stats()
+        assert _rel(a_full, res[10][0]) < 1e-13
+        assert st.flop_m2l == pytest.approx(res[10][1].flop_m2l, rel=1e-12) and st.mean_order == 10
+        # every evaluation cell at order 6: the cells that push their local take
+        # the highest order of their children, 6, and so does the storage
+        ctx.order_model_set(_const_model(6 - 4))
+        ctx.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+        assert _rel(da.download(ctx.stream), res[6][0]) < 1e-13
+        assert ctx.stats().flop_m2l == pytest.approx(res[6][1].flop_m2l, rel=1e-12)
+        # order 6 for cells with more than 100 M2L sources, 10 for the others
+        ctx.order_model_set(_split_model())
+        ctx.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+        a_mix, st = da.download(ctx.stream), ctx.stats()
+        e = {k: _rel(v[0][tgt], ref) for k, v in res.items()}
+        e_mix = _rel(a_mix[tgt], ref)
+        print(f"[orders] precision={precision} err fixed 6 {e[6]:.2e}, fixed 10 {e[10]:.2e}, split {e_mix:.2e}; "
+              f"mean order {st.mean_order:.2f}; M2L flops {st.flop_m2l:.3e} "
+              f"(fixed 6 {res[6][1].flop_m2l:.3e}, fixed 10 {res[10][1].flop_m2l:.3e})")
+        assert 6 < st.mean_order < 10
+        assert res[6][1].flop_m2l < st.flop_m2l < res[10][1].flop_m2l
+        assert 0.9 * e[10] <= e_mix <= 1.5 * e[6]
+        # order_min bounds the pick from below
+        ctx.set_params(order_min=8)
+        ctx.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+        assert 8 <= ctx.stats().mean_order < 10
+    finally:
+        ctx.set_params(order_select=0, order_min=4, precision=0)
+
+
+def test_learned_selection_end_to_end():
+    """Clustered mix (the learned-selection workload at reduced N): labels,
+    training, checked evaluation; the gain is M2L work at the same gate."""
+    gate, tol = 1e-6, 5e-7
+    c = nbfmm.Context(order=14, leaf_capacity=32, G=1.0, eps2=0.0, p2p_cut=2048, precision=0)
+    try:
+        c.order_model_init(seed=7)
+        rows = 0
+        for seed in (101, 102):
+            xyz, m = workloads.plummer_mix(60000, n_spheres=8, seed=seed)
+            dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
+            rows += c.order_dataset_dev(len(m), dx.ptr, dm.ptr, stride=4, seed=seed, tol=tol)
+        f, y = c.order_dataset()
+        assert len(y) == rows > 1000
+        hist = np.bincount(y, minlength=11)
+        print(f"[learned] {rows} labelled cells; labels (order 4..14): {hist.tolist()}")
+        loss = c.order_train(epochs=60, batch=512, lr=2e-3, seed=3)
+        print(f"[learned] loss {loss[0]:.3f} -> {loss[-1]:.3f}")
+        assert loss[-1] < loss[0]
+        # a system the model has not seen
+        xyz, m = workloads.plummer_mix(60000, n_spheres=8, seed=103)
+        n = len(m)
+        dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
+        da = nbfmm.DeviceArray(np.zeros((n, 3)))
+        fixed = {}
+        for p in range(6, 15):
+            c.set_params(order=p, order_select=0)
+            err, fb = c.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=4096, seed=9, gate=gate,
+                                               fallback_order=14)
+            fixed[p] = (err, c.stats().flop_m2l)
+            print(f"[learned] fixed order {p}: err {err:.2e}, fell back {fb}")
+            if err <= gate and not fb:
+                break
+        p_fix = min(p for p, (e, _) in fixed.items() if e <= gate)
+        c.set_params(order=14, order_select=1, order_min=4, order_conf=0.9)
+        err, fb = c.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=4096, seed=9, gate=gate,
+                                           fallback_order=14)
+        st = c.stats()
+        print(f"[learned] fixed order {p_fix}: err {fixed[p_fix][0]:.2e}, M2L {fixed[p_fix][1]:.3e} flop; "
+              f"learned: err {err:.2e}, fell back {fb}, mean order {st.mean_order:.2f}, M2L {st.flop_m2l:.3e} flop")
+        assert err <= gate  # the check is the judge (a fallback evaluation must meet it too)
+        if not fb:
+            assert st.flop_m2l < fixed[p_fix][1]
+    finally:
+        c.close()

@@ -225,6 +225,17 @@ def test_exact_fmm_converges_to_direct():
     assert _rel(orc.fmm_exact(xyz, m, 6, 16, G=1.0, p2p_cut=256), d) < errs[1]
 
 
+def test_exact_fmm_converges_on_elongated_box():
+    # a 1.6 : 1 : 1 box: the nearest classes along the short axes converge at
+    # a rate above 1 and are evaluated directly; the error keeps falling with p
+    rng = np.random.default_rng(18)
+    xyz = rng.random((2000, 3)) * np.array([1.6, 1.0, 1.0])
+    m = rng.uniform(0.5, 1.5, 2000)
+    d = orc.direct_targets(xyz, m, np.arange(2000), G=1.0)
+    errs = [_rel(orc.fmm_exact(xyz, m, p, 16, G=1.0), d) for p in (4, 8, 12)]
+    assert errs[2] < 0.1 * errs[1] < 0.01 * errs[0]
+
+
 def test_direct_naive_two_body():
     xyz = np.array([[-10.0, 0, 0], [10.0, 0, 0]])
     m = np.array([1e9, 1e9])
