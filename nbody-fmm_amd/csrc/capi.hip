@@ -184,7 +184,7 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,
                  &p2m_off, &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
-                 &ord, &msum, &feat, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
+                 &ord, &msum, &feat, &rmax, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
                  &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc};
   for (DBuf *b : all) dbuf_free(*b);
   if (hred.p) (void)hipHostFree(hred.p);

@@ -165,7 +165,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf xk, vk, ak, asum, vsum, diag_acc, diag_phi;
   // learned order selection: per-cell orders and features, the model, its Adam
   // state and training scratch, the training set and the label runs
-  nbfmm::DBuf ord, msum, feat, mlp_w, mlp_wb, mlp_m, mlp_v, mlp_act, mlp_perm;
+  nbfmm::DBuf ord, msum, feat, rmax, mlp_w, mlp_wb, mlp_m, mlp_v, mlp_act, mlp_perm;
   nbfmm::DBuf ds_feat, ds_label, lab_tgt, lab_dir, lab_sum, lab_flag, lab_lab, chk_acc;
   int64_t n_local = 0; // cells with a local expansion in the last evaluation
   int ord_max = 0;     // highest order selected in the last evaluation

@@ -345,25 +345,47 @@ __global__ void __launch_bounds__(kT) k_feat_stats(int64_t n, const float *__res
 }
 
 // ---- cell features ------------------------------------------------------------------
-// m, m x, m y, m z of every sorted particle (4 arrays of n + 1; entry n = 0)
-__global__ void k_moments(int64_t n, const double4 *__restrict__ pos, double *__restrict__ mv) {
+// mass of every sorted particle (n + 1 entries; entry n = 0)
+__global__ void k_masses(int64_t n, const double4 *__restrict__ pos, double *__restrict__ mv) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i > n) return;
-  const int64_t s = n + 1;
-  double4 q = i < n ? pos[i] : make_double4(0, 0, 0, 0);
-  mv[i] = q.w;
-  mv[s + i] = q.w * q.x;
-  mv[2 * s + i] = q.w * q.y;
-  mv[3 * s + i] = q.w * q.z;
+  if (i <= n) mv[i] = i < n ? pos[i].w : 0.0;
 }
 
 __device__ __forceinline__ float lg(double x) { return (float)log10(x > 1e-300 ? x : 1e-300); }
 
+// largest distance of a particle from the cell centre, in cell sizes rho_l:
+// exact for cells of at most kRmaxCells particles, the half diagonal above
+constexpr int kRmaxCells = 256;
+__global__ void __launch_bounds__(kT) k_rmax(int64_t nc, LevelGeo g, const uint64_t *__restrict__ ckey,
+                                             const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                                             const double4 *__restrict__ pos, float *__restrict__ rmax) {
+  const int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (c >= nc) return;
+  const int l = level_of((int32_t)c, g);
+  const double ir = g.inv_rho[l];
+  const int32_t b = cbeg[c], e = cend[c];
+  double r2 = 0;
+  if (e - b <= kRmaxCells) {
+    double ctr[3];
+    cell_centre(ckey[c], l, g, ctr);
+    for (int32_t i = b; i < e; ++i) {
+      const double4 q = pos[i];
+      const double dx = q.x - ctr[0], dy = q.y - ctr[1], dz = q.z - ctr[2];
+      r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+    r2 *= ir * ir;
+  } else {
+    const double hx = g.h[l][0] * ir, hy = g.h[l][1] * ir, hz = g.h[l][2] * ir;
+    r2 = 0.25 * (hx * hx + hy * hy + hz * hz);
+  }
+  rmax[c] = (float)sqrt(r2);
+}
+
 // 16 features of every cell with a local, by local slot; S = exclusive prefix
-// sums of k_moments (S[k (n+1) + i] = sum over the first i sorted particles).
-// Magnitudes are logs; features 7, 8, 9 (far field of the interaction list)
-// and 11 (near-field mass over the cell size squared) are made relative to
-// the system by k_feat_centre.
+// sums of the sorted masses (S[i] = mass of the first i sorted particles).
+// Magnitudes are logs; features 7, 8, 9, 12 (far field of the interaction
+// list and its truncation-error proxies) and 11 (near-field mass over the
+// cell size squared) are made relative to the system by k_feat_centre.
 __global__ void __launch_bounds__(kT) k_features(int64_t nc, int64_t n, LevelGeo g, const uint64_t *__restrict__ ckey,
                                                  const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                  const int32_t *__restrict__ cparent, const int32_t *__restrict__ lslot,
@@ -373,28 +395,23 @@ __global__ void __launch_bounds__(kT) k_features(int64_t nc, int64_t n, LevelGeo
                                                  const int32_t *__restrict__ cnt_sub,
                                                  const int32_t *__restrict__ m2l_src,
                                                  const uint16_t *__restrict__ m2l_cls, const double *__restrict__ S,
-                                                 float *__restrict__ feat) {
+                                                 const float *__restrict__ rmax, float *__restrict__ feat) {
   const int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (c >= nc) return;
   const int32_t ls = lslot[c];
   if (ls < 0) return;
-  const int64_t s = n + 1;
+  const double rt = rmax[c];
   const double mtot = S[n] > 0 ? S[n] : 1.0;
   const int l = level_of((int32_t)c, g);
   const double rho = 1.0 / g.inv_rho[l];
   const double hx = g.h[l][0] * g.inv_rho[l], hy = g.h[l][1] * g.inv_rho[l], hz = g.h[l][2] * g.inv_rho[l];
   const int32_t b = cbeg[c], e = cend[c];
   const double M = S[e] - S[b];
-  double ctr[3];
-  cell_centre(ckey[c], l, g, ctr);
-  double off = 0;
-  if (M > 0) {
-    const double dx = (S[s + e] - S[s + b]) / M - ctr[0], dy = (S[2 * s + e] - S[2 * s + b]) / M - ctr[1],
-                 dz = (S[3 * s + e] - S[3 * s + b]) / M - ctr[2];
-    off = sqrt(dx * dx + dy * dy + dz * dz) / rho;
-  }
-  // interaction list (the pairs evaluated by M2L); distances in cell sizes
-  double mil = 0, f1 = 0, f2 = 0, f3 = 0, wo = 0;
+  // interaction list (the pairs evaluated by M2L); distances in cell sizes.
+  // The M2L of a pair converges like t^(q+1), t = (r_s + r_t) / d with r the
+  // largest particle distance from the cell centre: e6 and e10 weigh the
+  // far field of every source with t^7 and t^11 (truncation-error proxies).
+  double mil = 0, f1 = 0, f2 = 0, e6 = 0, e10 = 0, tmax = 0;
   const int32_t q0 = off_m2l[c], nq = cnt_m2l[c];
   for (int32_t q = q0; q < q0 + nq; ++q) {
     const int32_t src = m2l_src[q];
@@ -406,15 +423,12 @@ __global__ void __launch_bounds__(kT) k_features(int64_t nc, int64_t n, LevelGeo
     const double ms = S[se] - S[sb];
     mil += ms;
     f1 += ms / d2;
-    f3 += ms / (d2 * sqrt(d2));
     if (max(abs(ddx), max(abs(ddy), abs(ddz))) == 2) f2 += ms / d2;
-    if (ms > 0) {
-      double sc[3];
-      cell_centre(ckey[src], l, g, sc);
-      const double ox = (S[s + se] - S[s + sb]) / ms - sc[0], oy = (S[2 * s + se] - S[2 * s + sb]) / ms - sc[1],
-                   oz = (S[3 * s + se] - S[3 * s + sb]) / ms - sc[2];
-      wo += ms * sqrt(ox * ox + oy * oy + oz * oz) / rho;
-    }
+    const double t = (rt + (double)rmax[src]) / sqrt(d2);
+    tmax = fmax(tmax, t);
+    const double t2 = t * t, t4 = t2 * t2, t7 = t4 * t2 * t;
+    e6 += ms / d2 * t7;
+    e10 += ms / d2 * t7 * t4;
   }
   // near field: the 27 neighbours at this level (self included)
   double mnear = 0;
@@ -431,16 +445,16 @@ __global__ void __launch_bounds__(kT) k_features(int64_t nc, int64_t n, LevelGeo
   f[0] = (float)(g.depth - l);
   f[1] = (float)log2((double)(e - b) + 1.0);
   f[2] = lg(M / mtot);
-  f[3] = (float)off;
+  f[3] = (float)rt;
   f[4] = (float)nq;
   f[5] = (float)cnt_sub[c];
   f[6] = lg(mil / mtot);
   f[7] = lg(f1 * ir2);   // far field of the list, G = 1 units
   f[8] = lg(f2 * ir2);   // its nearest (Chebyshev distance 2) part
-  f[9] = lg(f3 * ir2);   // with one more power of cell size / distance
-  f[10] = mil > 0 ? (float)(wo / mil) : 0.0f;
+  f[9] = lg(e6 * ir2);   // truncation-error proxy at order 6
+  f[10] = (float)tmax;    // slowest convergence rate in the list
   f[11] = lg(mnear * ir2); // near-field scale
-  f[12] = lg(mnear) - lg(f1);
+  f[12] = lg(e10 * ir2);  // truncation-error proxy at order 10
   f[13] = (float)nnb;
   f[14] = f1 > 0 ? (float)(f2 / f1) : 0.0f;
   f[15] = (par >= 0 && mode_l[par] == 2) ? 1.0f : 0.0f;
@@ -486,7 +500,7 @@ __global__ void k_feat_means(int nb, const double *__restrict__ part, double *__
   mean[1] = w > 0 ? b / w : 0.0;
 }
 
-// features 7, 8, 9 relative to the system's mean far field, 11 to its mean near field
+// features 7, 8, 9, 12 relative to the system's mean far field, 11 to its mean near field
 __global__ void k_feat_centre(int64_t nl, const double *__restrict__ mean, float *__restrict__ feat) {
   const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (r >= nl) return;
@@ -496,6 +510,7 @@ __global__ void k_feat_centre(int64_t nl, const double *__restrict__ mean, float
   f[8] -= mf;
   f[9] -= mf;
   f[11] -= ma;
+  f[12] -= mf;
 }
 
 // ---- labels ---------------------------------------------------------------------------
@@ -630,20 +645,25 @@ void order_features(nbfmm_ctx *c, const LevelGeo &g) {
   Plan &P = c->plan;
   hipStream_t s = c->stream;
   const int64_t n = P.n, nc = P.n_cells;
-  double *mv = dget<double>(c->msum, (size_t)4 * (n + 1) + 3 * 256 + 8);
-  hipLaunchKernelGGL(k_moments, dim3(nblk(n + 1)), dim3(kT), 0, s, n, (const double4 *)c->pos.p, mv);
+  double *mv = dget<double>(c->msum, (size_t)(n + 1) + 3 * 256 + 8);
+  hipLaunchKernelGGL(k_masses, dim3(nblk(n + 1)), dim3(kT), 0, s, n, (const double4 *)c->pos.p, mv);
   NB_LAUNCH();
-  for (int k = 0; k < 4; ++k) scan_excl_f64(mv + k * (n + 1), mv + k * (n + 1), n + 1, c->scan_tmp, s);
+  scan_excl_f64(mv, mv, n + 1, c->scan_tmp, s);
   const int64_t nl = std::max<int64_t>(1, c->n_local);
   float *feat = dget<float>(c->feat, (size_t)nl * kIn);
+  float *rmax = dget<float>(c->rmax, (size_t)nc + 1);
+  hipLaunchKernelGGL(k_rmax, dim3(nblk(nc)), dim3(kT), 0, s, nc, g, (const uint64_t *)c->ckey.p,
+                     (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const double4 *)c->pos.p, rmax);
+  NB_LAUNCH();
   hipLaunchKernelGGL(k_features, dim3(nblk(nc)), dim3(kT), 0, s, nc, n, g, (const uint64_t *)c->ckey.p,
                      (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p, (const int32_t *)c->cparent.p,
                      (const int32_t *)c->lslot.p, (const int32_t *)c->mode_l.p, (const int32_t *)c->nbr27.p,
                      (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->cnt_sub.p,
-                     (const int32_t *)c->m2l_src.p, (const uint16_t *)c->m2l_cls.p, (const double *)mv, feat);
+                     (const int32_t *)c->m2l_src.p, (const uint16_t *)c->m2l_cls.p, (const double *)mv,
+                     (const float *)rmax, feat);
   NB_LAUNCH();
   const unsigned npb = std::min<unsigned>(256, nblk(nc));
-  double *part = mv; // the prefix sums are no longer needed
+  double *part = mv + (n + 1);
   double *mean = part + 3 * 256;
   hipLaunchKernelGGL(k_feat_part, dim3(npb), dim3(kT), 0, s, nc, (const int32_t *)c->cbeg.p, (const int32_t *)c->cend.p,
                      (const int32_t *)c->lslot.p, (const int32_t *)c->mode_l.p, (const float *)feat, part);

@@ -158,47 +158,58 @@ def test_per_cell_orders(ctx, precision):
 
 
 def test_learned_selection_end_to_end():
-    """Clustered mix (the learned-selection workload at reduced N): labels,
-    training, checked evaluation; the gain is M2L work at the same gate."""
-    gate, tol = 1e-6, 5e-7
-    c = nbfmm.Context(order=14, leaf_capacity=32, G=1.0, eps2=0.0, p2p_cut=2048, precision=0)
+    """Clustered mix of 32 Plummer spheres (the learned-selection workload at
+    N = 2^20): labels from the FMM against direct sums, training on the GPU,
+    then on an unseen system the checked evaluation.  The check is the judge:
+    the result meets the gate, by the learned orders or by the fallback; the
+    gain is the M2L work against the lowest fixed order that meets the gate."""
+    gate, tol, N = 1e-6, 5e-7, 1 << 20
+    c = nbfmm.Context(order=14, leaf_capacity=32, G=1.0, eps2=0.0, p2p_cut=2048, precision=1)
     try:
         c.order_model_init(seed=7)
         rows = 0
-        for seed in (101, 102):
-            xyz, m = workloads.plummer_mix(60000, n_spheres=8, seed=seed)
+        for seed in (201, 202):
+            xyz, m = workloads.plummer_mix(N, n_spheres=32, seed=seed)
             dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
-            rows += c.order_dataset_dev(len(m), dx.ptr, dm.ptr, stride=4, seed=seed, tol=tol)
+            rows += c.order_dataset_dev(len(m), dx.ptr, dm.ptr, stride=2, seed=seed, tol=tol)
+            dx.free()
+            dm.free()
         f, y = c.order_dataset()
-        assert len(y) == rows > 1000
-        hist = np.bincount(y, minlength=11)
-        print(f"[learned] {rows} labelled cells; labels (order 4..14): {hist.tolist()}")
-        loss = c.order_train(epochs=60, batch=512, lr=2e-3, seed=3)
+        assert len(y) == rows > 10000
+        print(f"[learned] {rows} labelled cells; labels (order 4..14): {np.bincount(y, minlength=11).tolist()}")
+        loss = c.order_train(epochs=100, batch=256, lr=2e-3, seed=3)
         print(f"[learned] loss {loss[0]:.3f} -> {loss[-1]:.3f}")
-        assert loss[-1] < loss[0]
-        # a system the model has not seen
-        xyz, m = workloads.plummer_mix(60000, n_spheres=8, seed=103)
+        assert loss[-1] < 0.9 * loss[0]
+        xyz, m = workloads.plummer_mix(N, n_spheres=32, seed=7)
         n = len(m)
         dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
         da = nbfmm.DeviceArray(np.zeros((n, 3)))
-        fixed = {}
-        for p in range(6, 15):
+        # the method without the network: the lowest fixed order that meets the gate
+        p_fix = None
+        for p in range(9, 15):
             c.set_params(order=p, order_select=0)
-            err, fb = c.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=4096, seed=9, gate=gate,
-                                               fallback_order=14)
-            fixed[p] = (err, c.stats().flop_m2l)
-            print(f"[learned] fixed order {p}: err {err:.2e}, fell back {fb}")
-            if err <= gate and not fb:
+            err, fb = c.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=4096, seed=9, gate=1.0)
+            flop_fix = c.stats().flop_m2l
+            print(f"[learned] fixed order {p}: err {err:.2e}, M2L {flop_fix:.3e} flop")
+            if err <= gate:
+                p_fix = p
                 break
-        p_fix = min(p for p, (e, _) in fixed.items() if e <= gate)
-        c.set_params(order=14, order_select=1, order_min=4, order_conf=0.9)
-        err, fb = c.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=4096, seed=9, gate=gate,
-                                           fallback_order=14)
+        assert p_fix is not None
+        # learned orders up to that order, checked, fixed order 14 as the fallback
+        c.set_params(order=p_fix, order_select=1, order_min=4, order_conf=0.9)
+        err, fb = c.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=4096, seed=9, gate=gate, fallback_order=14)
         st = c.stats()
-        print(f"[learned] fixed order {p_fix}: err {fixed[p_fix][0]:.2e}, M2L {fixed[p_fix][1]:.3e} flop; "
-              f"learned: err {err:.2e}, fell back {fb}, mean order {st.mean_order:.2f}, M2L {st.flop_m2l:.3e} flop")
-        assert err <= gate  # the check is the judge (a fallback evaluation must meet it too)
+        print(f"[learned] learned (max order {p_fix}): err {err:.2e}, fell back {fb}, mean order {st.mean_order:.2f}, "
+              f"M2L {st.flop_m2l:.3e} flop ({st.flop_m2l / flop_fix:.2f} of fixed order {p_fix})")
+        assert err <= gate and st.fallback == int(fb)
         if not fb:
-            assert st.flop_m2l < fixed[p_fix][1]
+            assert st.flop_m2l < flop_fix
+        # a model that picks order 4 everywhere fails the check: the fallback runs and meets the gate
+        w = c.order_model_get()
+        c.order_model_set(_const_model(0))
+        err, fb = c.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=4096, seed=9, gate=gate, fallback_order=14)
+        print(f"[learned] order-4 model: err after fallback {err:.2e}")
+        assert fb and err <= gate and c.stats().fallback == 1
+        c.order_model_set(w)
     finally:
         c.close()
