@@ -53,6 +53,7 @@ def build_lib(jobs=8):
 
 HOST_LIB = os.path.join(HERE, "libnbfmm_host.so")
 DROPIN_TEST = os.path.join(BUILD, "test_dropin")
+FRAMES_TEST = os.path.join(BUILD, "test_frames")
 
 
 def build_host():
@@ -65,9 +66,9 @@ def build_host():
     rpath = "-Wl,-rpath,$ORIGIN"
     _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", f"-I{inc}", "-o", HOST_LIB, *srcs,
           f"-L{HERE}", "-lnbfmm", rpath])
-    _run(["g++", "-std=c++20", "-O2", f"-I{inc}", "-o", DROPIN_TEST,
-          os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp"), f"-L{HERE}", "-lnbfmm_host", "-lnbfmm",
-          f"-Wl,-rpath,{HERE}"])
+    for exe, src in ((DROPIN_TEST, "test_dropin.cpp"), (FRAMES_TEST, "test_frames.cpp")):
+        _run(["g++", "-std=c++20", "-O2", f"-I{inc}", "-o", exe, os.path.join(ROOT, "tests", "cpp", src),
+              f"-L{HERE}", "-lnbfmm_host", "-lnbfmm", f"-Wl,-rpath,{HERE}"])
     return HOST_LIB
 
 

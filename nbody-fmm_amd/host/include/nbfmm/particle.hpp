@@ -6,6 +6,7 @@
 
 class QuadratureMethod;
 class GpuUpdater;
+struct FrameRestore;
 
 class Particle {
  public:
@@ -20,6 +21,7 @@ class Particle {
 
   friend class QuadratureMethod;
   friend class GpuUpdater;
+  friend struct FrameRestore; // restores the acceleration of a saved frame
 
  protected:
   double weight = 0.0;

@@ -3,12 +3,14 @@
 //   TwoParticlesGetCloserUnderGravity   tests/test_naive_update.cpp:13-55
 //   CompareAccelerations3D              tests/test_fmm_vs_naive_3d.cpp:17-208
 //   RK4 reference semantics / classic   src/quadratures/runge_kutta_4.cpp:10-93
+//   frames written every step (src/main.cpp:78-93) and a run resumed from one
 // Prints one line per check and exits nonzero on the first failure.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
 
+#include "nbfmm/frame_export.hpp"
 #include "nbfmm/quadratures.hpp"
 #include "nbfmm/updaters.hpp"
 
@@ -119,11 +121,53 @@ static void rk4_modes() {
   CHECK(dref == 0.0, "reference RK4 positions");
 }
 
-int main() {
+// main.cpp's loop (update, integrate, export) for 6 steps, then the run
+// resumed from the frame after step 3 must reach the same state after 3 more
+static void export_and_resume(const char *dir) {
+  std::vector<Particle> P;
+  srand(11);
+  for (int i = 0; i < 500; ++i) {
+    double x = rand() / (double)RAND_MAX, y = rand() / (double)RAND_MAX, z = rand() / (double)RAND_MAX;
+    P.push_back(Particle(0.5 + rand() / (double)RAND_MAX, {x, y, z}, {0.0, 0.0, 0.0}));
+  }
+  nbfmm_params p;
+  nbfmm_default_params(&p);
+  p.order = 8;
+  p.leaf_capacity = 16;
+  p.G = 1.0;
+  p.eps2 = 0.0;
+  FMM fmm(p);
+  RungeKutta4 rk4;
+  const double dt = 1e-3;
+  HDF5Exporter ex(dir, "nbody_fmm");
+  fmm.update(P);
+  for (int step = 0; step < 6; ++step) {
+    ex.write_frame(P, step, step * dt);
+    rk4.integrate(&fmm, P, dt);
+  }
+  ex.write_frame(P, 6, 6 * dt);
+  auto f = nbfmm_io::read_frame(ex.index_path(), 3);
+  CHECK(f.step == 3 && f.time == 3 * dt, "frame 3");
+  std::vector<Particle> R = nbfmm_io::particles_from_frame(f);
+  for (int step = 3; step < 6; ++step) rk4.integrate(&fmm, R, dt);
+  double dev = 0, scale = 0;
+  for (size_t i = 0; i < P.size(); ++i) {
+    dev = std::fmax(dev, (R[i].get_position() - P[i].get_position()).norm());
+    scale = std::fmax(scale, P[i].get_position().norm());
+  }
+  std::printf("export_and_resume: %zu frames, resumed-run position deviation %.3e\n",
+              nbfmm_io::read_index(ex.index_path()).size(), dev);
+  CHECK(nbfmm_io::read_index(ex.index_path()).size() == 7, "7 frames indexed");
+  // the near-field sums run in any order on the GPU: equal to rounding
+  CHECK(dev <= 1e-13 * scale, "resumed run");
+}
+
+int main(int argc, char **argv) {
   try {
     two_particles();
     fmm_vs_naive();
     rk4_modes();
+    export_and_resume(argc > 1 ? argv[1] : "dropin_frames");
   } catch (const std::exception &e) {
     std::printf("FAIL: exception %s\n", e.what());
     return 2;
