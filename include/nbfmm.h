@@ -208,6 +208,23 @@ int nbfmm_eval_accel_checked_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz,
                                  int64_t ntgt, uint64_t seed, double gate, int32_t fallback_order, double *err,
                                  int32_t *fell_back);
 
+/* Batched systems (no counterpart in the reference, which advances one
+ * system).  nsys independent systems stored one after the other in the
+ * arrays: system k is particles [offsets[k], offsets[k+1]) (offsets: host
+ * array of nsys + 1, offsets[0] = 0); no system acts on another.  All systems
+ * are evaluated together, every phase of the FMM in one launch for all: each
+ * is scaled by a power of two into its own cell of a grid of isolated cells
+ * and the interaction lists stop at those cells.  Every system needs more
+ * particles than leaf_capacity.  Device pointers, stream-ordered. */
+int nbfmm_batch_eval_accel_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets, const double *d_xyz,
+                               const double *d_m, double *d_acc);
+/* nbfmm_step_se_dev / nbfmm_step_rk4_dev over a batch (each system advanced
+ * as by itself; the update of every stage is one launch for all). */
+int nbfmm_batch_step_se_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets, double *d_xyz, double *d_vel,
+                            double *d_acc, const double *d_m, double dt, int nsteps);
+int nbfmm_batch_step_rk4_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets, double *d_xyz, double *d_vel,
+                             double *d_acc, const double *d_m, double dt, int nsteps, int mode);
+
 /* Statistics of the last evaluation: sizes for the roofline and per-phase
  * GPU times measured with HIP events on the context's stream. */
 typedef struct nbfmm_stats {

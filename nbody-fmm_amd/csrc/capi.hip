@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 
 using namespace nbfmm;
@@ -118,31 +119,56 @@ void eval_with(nbfmm_ctx *c, const nbfmm_params &q, int64_t n, const double *x, 
   c->prm = save;
 }
 
+// accelerations a(x) of the particle set being integrated
+using Eval = std::function<void(const double *x, double *a)>;
+
 // SyplecticEuler::integrate (symplectic_euler.cpp:8-15), one step
-void step_se(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt) {
-  eval_dev(c, n, x, m, a);
+void step_se(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const Eval &ev, double dt) {
+  ev(x, a);
   se_update(c, n, x, v, a, dt);
+}
+void step_se(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt) {
+  step_se(c, n, x, v, a, [&](const double *xx, double *aa) { eval_dev(c, n, xx, m, aa); }, dt);
 }
 
 // RungeKutta4::integrate (runge_kutta_4.cpp:10-93), one step; mode 0 the
 // reference's semantics, 1 classic
-void step_rk4(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt, int mode) {
+void step_rk4(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const Eval &ev, double dt, int mode) {
   const size_t m3 = (size_t)n * 3;
   double *xk = dget<double>(c->xk, m3), *vk = dget<double>(c->vk, m3), *ak = dget<double>(c->ak, m3);
   double *asum = dget<double>(c->asum, m3), *vsum = dget<double>(c->vsum, m3);
   const double h2 = dt / 2, h6 = dt / 6;
   copy_d(c, vk, v, (int64_t)m3);         // velocity_k = v      (runge_kutta_4.cpp:28-30)
-  eval_dev(c, n, x, m, ak);              // stage 1              (:32)
+  ev(x, ak);                             // stage 1              (:32)
   rk4_accum(c, n, asum, vsum, ak, vk, 1.0, 1);
   if (mode == 1) copy_d(c, a, ak, (int64_t)m3);
   for (int stage = 2; stage <= 4; ++stage) {
     const double h = stage == 4 ? dt : h2;
     const double *avel = mode == 0 ? a : ak;  // reference: carried acceleration of the input (:46-48)
     rk4_stage(c, n, x, v, vk, avel, h, xk, vk);
-    eval_dev(c, n, xk, m, ak);
+    ev(xk, ak);
     rk4_accum(c, n, asum, vsum, ak, vk, stage == 4 ? 1.0 : 2.0, 0);
   }
   rk4_final(c, n, x, v, asum, vsum, h6);  // (:89-92)
+}
+void step_rk4(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const double *m, double dt, int mode) {
+  step_rk4(c, n, x, v, a, [&](const double *xx, double *aa) { eval_dev(c, n, xx, m, aa); }, dt, mode);
+}
+
+// the batched evaluation of nsys systems (offsets on the host)
+Eval batch_evaluator(nbfmm_ctx *c, int nsys, const int64_t *off, const double *m) {
+  return [=](const double *x, double *a) {
+    batch_eval(c, nsys, off, x, m, a,
+               [c](int64_t nb, const double *xb, const double *mb, double *ab) { eval_dev(c, nb, xb, mb, ab); });
+  };
+}
+
+void check_batch(int nsys, const int64_t *off) {
+  if (nsys < 1 || !off) throw Error(NBFMM_ERR_ARG, "nsys >= 1 and offsets are required");
+  if (off[0] != 0) throw Error(NBFMM_ERR_ARG, "offsets[0] must be 0");
+  for (int k = 0; k < nsys; ++k)
+    if (off[k + 1] <= off[k]) throw Error(NBFMM_ERR_ARG, "every system needs at least one particle");
+  if (off[nsys] > ((int64_t)1 << 30) - 2) throw Error(NBFMM_ERR_ARG, "too many particles");
 }
 
 void check_n(int64_t n) {
@@ -185,7 +211,8 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,
                  &p2m_off, &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
                  &ord, &msum, &feat, &rmax, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
-                 &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc};
+                 &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc, &bat_off, &bat_bb, &bat_st, &bat_x, &bat_m,
+                 &bat_a};
   for (DBuf *b : all) dbuf_free(*b);
   if (hred.p) (void)hipHostFree(hred.p);
   for (auto &e : ev)
@@ -508,6 +535,38 @@ int nbfmm_eval_accel_checked_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, c
   c->stats.fallback = fb;
   if (err) *err = e;
   if (fell_back) *fell_back = fb;
+  API_END
+}
+
+int nbfmm_batch_eval_accel_dev(nbfmm_ctx *c, int32_t nsys, const int64_t *offsets, const double *d_xyz,
+                               const double *d_m, double *d_acc) {
+  if (!c || !d_xyz || !d_m || !d_acc) return fail(NBFMM_ERR_ARG, "null argument");
+  API_BEGIN
+  check_batch(nsys, offsets);
+  batch_evaluator(c, nsys, offsets, d_m)(d_xyz, d_acc);
+  API_END
+}
+
+int nbfmm_batch_step_se_dev(nbfmm_ctx *c, int32_t nsys, const int64_t *offsets, double *x, double *v, double *a,
+                            const double *m, double dt, int nsteps) {
+  if (!c || !x || !v || !a || !m) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
+  API_BEGIN
+  check_batch(nsys, offsets);
+  const Eval ev = batch_evaluator(c, nsys, offsets, m);
+  for (int st = 0; st < nsteps; ++st) step_se(c, offsets[nsys], x, v, a, ev, dt);
+  API_END
+}
+
+int nbfmm_batch_step_rk4_dev(nbfmm_ctx *c, int32_t nsys, const int64_t *offsets, double *x, double *v, double *a,
+                             const double *m, double dt, int nsteps, int mode) {
+  if (!c || !x || !v || !a || !m) return fail(NBFMM_ERR_ARG, "null argument");
+  if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
+  if (mode != 0 && mode != 1) return fail(NBFMM_ERR_ARG, "mode must be 0 or 1");
+  API_BEGIN
+  check_batch(nsys, offsets);
+  const Eval ev = batch_evaluator(c, nsys, offsets, m);
+  for (int st = 0; st < nsteps; ++st) step_rk4(c, offsets[nsys], x, v, a, ev, dt, mode);
   API_END
 }
 

@@ -57,7 +57,8 @@ constexpr int64_t kTaskPairs = 1 << 17;
 // index >= skip_from), its lists are never read: they are skipped and its
 // counts stay 0.  skip_from < 0 classifies every cell.
 template <bool FILL>
-__global__ void __launch_bounds__(kT) k_lists(int32_t nparents, int32_t skip_from, const int32_t *__restrict__ nbr27,
+__global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, int32_t skip_from,
+                                              const int32_t *__restrict__ nbr27,
                                               const int32_t *__restrict__ child8, const int32_t *__restrict__ cbeg,
                                               const int32_t *__restrict__ cend, int64_t cut, ClsMask bad,
                                               int32_t *cnt_il,
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t nparents, int32_t skip_fro
   __shared__ int32_t cand_s[kT / kW][kWin];
   __shared__ int32_t csz_s[kT / kW][kWin];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int32_t P = (int32_t)blockIdx.x * (kT / kW) + wave;
+  const int32_t P = pbeg + (int32_t)blockIdx.x * (kT / kW) + wave;
   const bool active = P < nparents;
   int32_t *cand = cand_s[wave], *csz = csz_s[wave];
   int64_t maxn = 0;
@@ -335,8 +336,12 @@ void build_lists(nbfmm_ctx *c) {
   int32_t *off_sub = dget<int32_t>(c->off_sub, (size_t)nc + 1);
   int64_t *sub_pairs = dget<int64_t>(c->sub_pairs, (size_t)nc + 1);
   // Closed cells (below) merge the near field of their whole subtree into one
-  // P2P task; cells of level >= lc may be merged.
-  const int lc = 1;
+  // P2P task; cells of level >= lc may be merged.  Cells of levels below lc
+  // have no lists (lc = 1 for one system; below the system cells for a batch).
+  const int lc = std::max(1, c->list_level);
+  if (lc > D)
+    throw Error(NBFMM_ERR_UNSUPPORTED, "batched systems need a tree deeper than the system cells (more particles per "
+                                       "system than leaf_capacity)");
   const int32_t skip_from = c->keep_lists ? -1 : P.lvl_start[lc];
   // cells that are not classified (the root, skipped children) have empty lists
   NB_HIP(hipMemsetAsync(cnt_il, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
@@ -344,9 +349,10 @@ void build_lists(nbfmm_ctx *c) {
   NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
   NB_HIP(hipMemsetAsync(sub_pairs, 0, sizeof(int64_t) * (size_t)(nc + 1), s));
   const int32_t nparents = leaf_begin; // every non-leaf cell is a parent
+  const int32_t pbeg = P.lvl_start[std::min(lc - 1, D)]; // parents of the cells of level lc on
   const ClsMask bad = class_mask(P.geo);
-  const unsigned nlb = (unsigned)((nparents + kT / kW - 1) / (kT / kW));
-  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
+  const unsigned nlb = (unsigned)std::max<int32_t>(1, (nparents - pbeg + kT / kW - 1) / (kT / kW));
+  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
                      cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                      nullptr, nullptr);
   NB_LAUNCH();
@@ -378,7 +384,7 @@ void build_lists(nbfmm_ctx *c) {
   int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
   int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
   NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  hipLaunchKernelGGL(k_lists<true>, dim3(nlb), dim3(kT), 0, s, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
+  hipLaunchKernelGGL(k_lists<true>, dim3(nlb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
                      nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il, m2l_src,
                      m2l_cls, sub_src, is_src);
   NB_LAUNCH();

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -174,6 +175,9 @@ struct nbfmm_ctx {
   int64_t lab_ns = 0, lab_stride = 1, lab_nl = -1;
   uint64_t lab_seed = 0;
   int fallback = 0; // the last checked evaluation fell back to the fixed order
+  // batched systems: offsets, boxes, placement, placed copies; first level with lists
+  nbfmm::DBuf bat_off, bat_bb, bat_st, bat_x, bat_m, bat_a;
+  int list_level = 1;
   // events
   hipEvent_t ev[16] = {};
   int nev = 0;
@@ -221,6 +225,10 @@ void launch_p2p(nbfmm_ctx *c);
 void run_finish(nbfmm_ctx *c, double *d_acc_out);
 void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,
                 const int64_t *d_tgt, int mode, double *d_acc);
+// batch.hip: the systems [off[k], off[k+1]) of one concatenated set evaluated
+// as one forest; eval(n, x, m, a) runs the evaluation of the placed copy
+void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *h_off, const double *d_xyz, const double *d_m,
+                double *d_acc, const std::function<void(int64_t, const double *, const double *, double *)> &eval);
 // integrate.hip
 void se_update(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *a, double dt);
 void rk4_stage(nbfmm_ctx *c, int64_t n, const double *x, const double *v, const double *vk_prev,

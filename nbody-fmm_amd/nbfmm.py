@@ -82,6 +82,7 @@ EXPORTS = [
     "nbfmm_order_model_init", "nbfmm_order_model_set", "nbfmm_order_model_get", "nbfmm_order_dataset_dev",
     "nbfmm_order_dataset_get", "nbfmm_order_dataset_put", "nbfmm_order_train", "nbfmm_order_grad",
     "nbfmm_order_predict", "nbfmm_eval_accel_checked_dev",
+    "nbfmm_batch_eval_accel_dev", "nbfmm_batch_step_se_dev", "nbfmm_batch_step_rk4_dev",
 ]
 
 DIAG_NAMES = ["kinetic", "potential", "total", "px", "py", "pz", "lx", "ly", "lz", "mass"]
@@ -127,6 +128,9 @@ def lib():
         L.nbfmm_order_grad.argtypes = [vp, i32, vp, vp, vp, vp]
         L.nbfmm_order_predict.argtypes = [vp, i64, vp, vp]
         L.nbfmm_eval_accel_checked_dev.argtypes = [vp, i64, vp, vp, vp, i64, u64, dbl, i32, vp, vp]
+        L.nbfmm_batch_eval_accel_dev.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.nbfmm_batch_step_se_dev.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, ctypes.c_int]
+        L.nbfmm_batch_step_rk4_dev.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -297,6 +301,20 @@ class Context:
         _check(lib().nbfmm_eval_accel_checked_dev(self.h, n, d_xyz, d_m, d_acc, ntgt, seed, gate, fallback_order,
                                                   ctypes.byref(err), ctypes.byref(fb)))
         return err.value, bool(fb.value)
+
+    # ---- batched systems ----
+    def batch_eval_accel_dev(self, offsets, d_xyz, d_m, d_acc):
+        """offsets: int64 array of nsys + 1 (system k = particles [offsets[k], offsets[k+1]))."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        _check(lib().nbfmm_batch_eval_accel_dev(self.h, len(off) - 1, _ptr(off), d_xyz, d_m, d_acc))
+
+    def batch_step_se_dev(self, offsets, dx, dv, da, dm, dt, nsteps):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        _check(lib().nbfmm_batch_step_se_dev(self.h, len(off) - 1, _ptr(off), dx, dv, da, dm, dt, nsteps))
+
+    def batch_step_rk4_dev(self, offsets, dx, dv, da, dm, dt, nsteps, mode):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        _check(lib().nbfmm_batch_step_rk4_dev(self.h, len(off) - 1, _ptr(off), dx, dv, da, dm, dt, nsteps, mode))
 
     def stats(self):
         s = Stats()
