@@ -633,6 +633,7 @@ void run_expansions(nbfmm_ctx *c) {
     c->n_local = 0;
     NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s));
     P.n_p2m_terms = P.n_l2p_terms = 0;
+    fork_p2p(c);
     launch_p2p(c);
     mark(c, 3);
     mark(c, 4);
@@ -750,12 +751,19 @@ void run_expansions(nbfmm_ctx *c) {
   for (int d = D - 1; d >= 0; --d)
     run_trans(c, false, KP, f32, bm[d + 1] - bm[d], m2m_cells + bm[d]);
   mark(c, 3);
-  // P2P runs on the second stream from here on, beside M2L, L2L and L2P
-  launch_p2p(c);
   // ---- M2L (writes the locals of its targets) ----
+  // P2P starts on the second stream after the first pass's M2L contraction
+  // (run_m2l), so that the contraction, which holds a CU's whole LDS per
+  // workgroup, is not starved by it, and the memory-bound sums, L2L and L2P
+  // run beside P2P instead of after it
+  c->p2p_started = false;
   dbuf_ensure(c->lexp, (size_t)nL * KP * esz + 16);
   NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));
   run_m2l(c, KP, f32, ord);
+  if (!c->p2p_started) {
+    fork_p2p(c);
+    launch_p2p(c);
+  }
   // ---- L2L, coarsest level first ----
   for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
   mark(c, 4);

@@ -517,7 +517,12 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
       else
         launch<double>(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const double *)frag, KP, nvalid);
     }
-    if (ps == 0) mark(c, 8);
+    if (ps == 0) {
+      mark(c, 8);
+      fork_p2p(c); // P2P beside the rest of the chain
+      launch_p2p(c);
+      c->p2p_started = true;
+    }
     if (f32)
       hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                          (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,

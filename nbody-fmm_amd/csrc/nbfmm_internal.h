@@ -178,6 +178,7 @@ struct nbfmm_ctx {
   // batched systems: offsets, boxes, placement, placed copies; first level with lists
   nbfmm::DBuf bat_off, bat_bb, bat_st, bat_x, bat_m, bat_a;
   int list_level = 1;
+  bool p2p_started = false; // P2P of the current evaluation is enqueued
   // events
   hipEvent_t ev[16] = {};
   int nev = 0;
@@ -219,7 +220,9 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord);
 void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells);
 void trans_to_f32(nbfmm_ctx *c, int KP);
 // p2p.hip
-// P2P on the second stream, forked from the main stream after the lists
+// P2P on the second stream: fork_p2p marks the point of the main stream it
+// waits for (its inputs are complete there), launch_p2p enqueues it later
+void fork_p2p(nbfmm_ctx *c);
 void launch_p2p(nbfmm_ctx *c);
 // joins the P2P stream and writes acc = G * (far + near) in input order
 void run_finish(nbfmm_ctx *c, double *d_acc_out);

@@ -350,10 +350,11 @@ __global__ void k_direct_combine(int64_t ntgt, int nchunk, const double *__restr
 
 } // namespace
 
+void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
+
 void launch_p2p(nbfmm_ctx *c) {
   Plan &P = c->plan;
   hipStream_t s2 = c->stream2;
-  NB_HIP(hipEventRecord(c->ev_fork, c->stream));
   NB_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
   // near-field gradient (3 per particle) and, with potentials, near-field sums of m / r
   const bool pot = c->phi_out != nullptr;
