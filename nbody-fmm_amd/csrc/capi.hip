@@ -97,7 +97,10 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
     S.ms_total = tot;
     float km = 0, kp = 0;
     if (P.n_tgt > 0) NB_HIP(hipEventElapsedTime(&km, c->ev[7], c->ev[8]));
+    float kp2 = 0;
     NB_HIP(hipEventElapsedTime(&kp, c->ev[9], c->ev[10]));
+    NB_HIP(hipEventElapsedTime(&kp2, c->ev[11], c->ev[12]));
+    kp += kp2;
     S.ms_m2l_kernel = km;
     S.ms_p2p_kernel = kp;
   }
@@ -218,6 +221,7 @@ nbfmm_ctx::~nbfmm_ctx() {
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_fork2) (void)hipEventDestroy(ev_fork2);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
@@ -258,6 +262,7 @@ int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
   NB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   NB_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   NB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  NB_HIP(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming));
   NB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   for (auto &e : c->ev) NB_HIP(hipEventCreate(&e));
   NB_HIP(hipHostMalloc(&c->hred.p, 65536, hipHostMallocDefault));

@@ -634,12 +634,17 @@ void run_expansions(nbfmm_ctx *c) {
     NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s));
     P.n_p2m_terms = P.n_l2p_terms = 0;
     fork_p2p(c);
-    launch_p2p(c);
+    launch_p2p(c, 0);
+    launch_p2p(c, 1);
     mark(c, 3);
     mark(c, 4);
     mark(c, 5);
     return;
   }
+  // the first share of P2P runs from here on (its tasks are complete), beside
+  // the planning, P2M, M2M and their host reads
+  fork_p2p(c);
+  launch_p2p(c, 0);
   const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
   const uint64_t *ckey = (const uint64_t *)c->ckey.p;
   const int64_t nc = P.n_cells;
@@ -752,7 +757,7 @@ void run_expansions(nbfmm_ctx *c) {
     run_trans(c, false, KP, f32, bm[d + 1] - bm[d], m2m_cells + bm[d]);
   mark(c, 3);
   // ---- M2L (writes the locals of its targets) ----
-  // P2P starts on the second stream after the first pass's M2L contraction
+  // The rest of P2P starts after the first pass's M2L contraction
   // (run_m2l), so that the contraction, which holds a CU's whole LDS per
   // workgroup, is not starved by it, and the memory-bound sums, L2L and L2P
   // run beside P2P instead of after it
@@ -760,10 +765,7 @@ void run_expansions(nbfmm_ctx *c) {
   dbuf_ensure(c->lexp, (size_t)nL * KP * esz + 16);
   NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));
   run_m2l(c, KP, f32, ord);
-  if (!c->p2p_started) {
-    fork_p2p(c);
-    launch_p2p(c);
-  }
+  if (!c->p2p_started) launch_p2p(c, 1);
   // ---- L2L, coarsest level first ----
   for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
   mark(c, 4);

@@ -519,8 +519,7 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
     }
     if (ps == 0) {
       mark(c, 8);
-      fork_p2p(c); // P2P beside the rest of the chain
-      launch_p2p(c);
+      launch_p2p(c, 1); // the rest of P2P beside the rest of the chain
       c->p2p_started = true;
     }
     if (f32)

@@ -133,7 +133,7 @@ struct nbfmm_ctx {
   nbfmm_params prm{};
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;            // P2P, concurrent with the expansion passes
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_join = nullptr;
   int device = 0;
   int lds_max = 65536; // LDS bytes one workgroup may allocate on this device
   int n_cu = 256;      // compute units of this device
@@ -221,9 +221,12 @@ void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const in
 void trans_to_f32(nbfmm_ctx *c, int KP);
 // p2p.hip
 // P2P on the second stream: fork_p2p marks the point of the main stream it
-// waits for (its inputs are complete there), launch_p2p enqueues it later
+// waits for (its inputs are complete there).  launch_p2p(c, 0) enqueues the
+// first share of the tasks, to run beside the up-pass and its host reads;
+// launch_p2p(c, 1) the rest, after the first M2L contraction, to run beside
+// the sums, L2L and L2P.
 void fork_p2p(nbfmm_ctx *c);
-void launch_p2p(nbfmm_ctx *c);
+void launch_p2p(nbfmm_ctx *c, int part);
 // joins the P2P stream and writes acc = G * (far + near) in input order
 void run_finish(nbfmm_ctx *c, double *d_acc_out);
 void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,

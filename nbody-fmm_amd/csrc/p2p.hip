@@ -352,28 +352,42 @@ __global__ void k_direct_combine(int64_t ntgt, int nchunk, const double *__restr
 
 void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
 
-void launch_p2p(nbfmm_ctx *c) {
+// Share of the P2P tasks run in part 0.
+#ifndef NBFMM_P2P_EARLY
+#define NBFMM_P2P_EARLY 0.2
+#endif
+
+void launch_p2p(nbfmm_ctx *c, int part) {
   Plan &P = c->plan;
   hipStream_t s2 = c->stream2;
-  NB_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
-  // near-field gradient (3 per particle) and, with potentials, near-field sums of m / r
   const bool pot = c->phi_out != nullptr;
   double *gn = dget<double>(c->grad_near, (size_t)P.n * (pot ? 4 : 3));
   double *pn = pot ? gn + 3 * (size_t)P.n : nullptr;
-  NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
-  mark2(c, 9);
-  if (P.n_tasks > 0) {
-    const int64_t grid = std::min<int64_t>(P.n_tasks, ((int64_t)1 << 31) - 1);
+  const int64_t split = (int64_t)((double)P.n_tasks * NBFMM_P2P_EARLY);
+  const int64_t t0 = part == 0 ? 0 : split, t1 = part == 0 ? split : P.n_tasks;
+  if (part == 1) {
+    // after the point of the main stream where part 1 is enqueued (the first M2L contraction)
+    NB_HIP(hipEventRecord(c->ev_fork2, c->stream));
+    NB_HIP(hipStreamWaitEvent(s2, c->ev_fork2, 0));
+  }
+  if (part == 0) {
+    NB_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
+    // near-field gradient (3 per particle) and, with potentials, near-field sums of m / r
+    NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
+  }
+  mark2(c, part == 0 ? 9 : 11);
+  if (t1 > t0) {
+    const int64_t grid = std::min<int64_t>(t1 - t0, ((int64_t)1 << 31) - 1);
     const bool f32 = c->prm.precision == NBFMM_MIXED;
     auto kern = f32 ? (pot ? k_p2p_tasks<PairF32, true> : k_p2p_tasks<PairF32, false>)
                     : (pot ? k_p2p_tasks<PairF64, true> : k_p2p_tasks<PairF64, false>);
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, (int64_t)P.n_tasks, (const P2PTask *)c->tasks.p,
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, t1 - t0, (const P2PTask *)c->tasks.p + t0,
                        (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
                        (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn, pn);
     NB_LAUNCH();
   }
-  mark2(c, 10);
-  NB_HIP(hipEventRecord(c->ev_join, s2));
+  mark2(c, part == 0 ? 10 : 12);
+  if (part == 1) NB_HIP(hipEventRecord(c->ev_join, s2));
 }
 
 void run_finish(nbfmm_ctx *c, double *d_acc_out) {
