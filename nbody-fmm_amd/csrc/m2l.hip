@@ -14,8 +14,10 @@
 // pass adds its partials to the locals.
 //
 // Plain precision runs v_mfma_f64_16x16x4f64 on FP64 coefficients; mixed
-// precision v_mfma_f32_16x16x4f32 on FP32 coefficients (the rho scaling keeps
-// them O(1)), with the per-target sum of the partials in FP64.
+// precision takes FP32 coefficients (the rho scaling keeps them O(1)) and
+// forms the FP32-accurate contraction from three-term bf16 splits on
+// v_mfma_f32_16x16x32_bf16 (k_m2l_bf3); the per-target sum of the partials is
+// in FP64 in both.
 //
 // With an order per target cell (learned order selection), the pairs are
 // sorted by (order, class): a pair whose target has order q uses the leading
@@ -38,6 +40,7 @@ constexpr int m2l_threads() {
   return sizeof(T) == 4 ? 1024 : 512;
 }
 constexpr int kClsChunk = 4096;
+
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -107,6 +110,139 @@ __global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__
 __global__ void k_to_f32(const double *__restrict__ in, int64_t n, float *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i < n) out[i] = (float)in[i];
+}
+
+// ---- mixed-precision M2L as three-term bfloat16 products -------------------------------
+// An FP32 value x is split x = x1 + x2 + x3 with x1 = bf16(x), x2 = bf16(x - x1),
+// x3 = bf16(x - x1 - x2) (8 significant bits each: x1 + x2 + x3 carries 24
+// bits, FP32's).  A product ab is formed as the six terms ai bj with i + j <= 4;
+// the dropped terms are below 2^-24 |ab|.  Products of bf16 are exact in FP32
+// and v_mfma_f32_16x16x32_bf16 accumulates in FP32, so the contraction keeps
+// FP32 accuracy at six bf16 MFMAs per 16x16x32 block instead of eight FP32
+// v_mfma_f32_16x16x4f32 (each about 2.7x the time of a bf16 one on gfx950).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int pad32(int k) { return (k + 31) / 32 * 32; }
+
+// Operator fragments for v_mfma_f32_16x16x32_bf16: lane l holds
+// A[l % 16][8 (l / 16) + j], j < 8, of a 16 x 32 block.  Layout
+// [class][row block r < KP/16][k step < pad32(KP)/32][term < 3][lane] of 8
+// bf16 (16 bytes), from the FP64 operators in FP32 MFMA order
+// (frag[((class RB + r) KP/4 + k/4) 64 + i + 16 (k % 4)]); columns >= KP are 0.
+__global__ void k_to_bf3(int KP, const double *__restrict__ frag, uint4 *__restrict__ out) {
+  const int RB = KP / 16, KS4 = KP / 4, KSB = pad32(KP) / 32;
+  const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; // (class, r, ks, lane)
+  const int64_t nlane = (int64_t)kNumClasses * RB * KSB * 64;
+  if (e >= nlane) return;
+  const int l = (int)(e % 64);
+  const int64_t g = e / 64;
+  const int ks = (int)(g % KSB);
+  const int64_t cr = g / KSB; // class * RB + r
+  const int i = l % 16;
+  bf16x8 t[3];
+  for (int j = 0; j < 8; ++j) {
+    const int k = ks * 32 + 8 * (l / 16) + j;
+    const double x = k < KP ? frag[((size_t)cr * KS4 + k / 4) * 64 + i + 16 * (k % 4)] : 0.0;
+    const __bf16 a1 = (__bf16)(float)x;
+    const double r1 = x - (double)(float)a1;
+    const __bf16 a2 = (__bf16)(float)r1;
+    const double r2 = r1 - (double)(float)a2;
+    t[0][j] = a1;
+    t[1][j] = a2;
+    t[2][j] = (__bf16)(float)r2;
+  }
+  for (int q = 0; q < 3; ++q) out[(g * 3 + q) * 64 + l] = *reinterpret_cast<const uint4 *>(&t[q]);
+}
+
+__device__ __forceinline__ void split3(const float x[8], bf16x8 &b1, bf16x8 &b2, bf16x8 &b3) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const __bf16 h = (__bf16)x[j];
+    const float r1 = x[j] - (float)h; // exact: h is x rounded to 8 bits
+    const __bf16 m = (__bf16)r1;
+    b1[j] = h;
+    b2[j] = m;
+    b3[j] = (__bf16)(r1 - (float)m);
+  }
+}
+
+// k_m2l_cls for FP32 coefficients with the three-term bf16 products; same
+// items, partials and tile/column mapping (D row 4 (l / 16) + q, column l % 16).
+template <int KP, int RBW>
+__global__ void __launch_bounds__(1024) k_m2l_bf3(const int4 *__restrict__ items, int32_t ybase,
+                                                  const int32_t *__restrict__ cls_pairs,
+                                                  const int32_t *__restrict__ m2l_src,
+                                                  const int32_t *__restrict__ src_slot, const uint4 *__restrict__ fragb,
+                                                  const float *__restrict__ Mexp, float *__restrict__ Y, int KPS,
+                                                  int nvalid) {
+  constexpr int RB = KP / 16, KSB = pad32(KP) / 32;
+  constexpr int V = KSB * 3 * 64; // 16-byte vectors per staged row block
+  extern __shared__ uint4 sA[];
+  const int4 it = items[blockIdx.x];
+  const int cls = it.x % kNumClasses;
+  const int r0 = (int)blockIdx.y * RBW;
+  const int nrb = min(RBW, RB - r0);
+  {
+    const int RBS = KPS / 16, KSBS = pad32(KPS) / 32; // layout of the stored (order-KPS) operators
+    const uint4 *A = fragb + (size_t)cls * RBS * KSBS * 192;
+    for (int e = threadIdx.x; e < nrb * V; e += 1024) {
+      const int r = e / V, v = e - r * V;
+      const int ks = v / 192, tl = v - ks * 192;
+      sA[e] = A[((size_t)(r0 + r) * KSBS + ks) * 192 + tl];
+    }
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, kg = lane >> 4;
+  for (int pb = it.y + wave * 16; pb < it.z; pb += 16 * 16) {
+    const int pi = pb + col;
+    const int32_t pid = pi < it.z ? cls_pairs[pi] : -1;
+    const float *Mb = Mexp + (size_t)(pid >= 0 ? src_slot[m2l_src[pid]] : 0) * KPS + 8 * kg;
+    flt4 acc[RBW];
+#pragma unroll
+    for (int r = 0; r < RBW; ++r) acc[r] = flt4{0, 0, 0, 0};
+#pragma unroll 1
+    for (int ks = 0; ks < KSB; ++ks) {
+      const int k0 = ks * 32 + 8 * kg;
+      float x[8];
+      if (pid >= 0 && k0 < KPS) {
+        const float4 u = *reinterpret_cast<const float4 *>(Mb + ks * 32);
+        const float4 w = *reinterpret_cast<const float4 *>(Mb + ks * 32 + 4);
+        x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = w.x, x[5] = w.y, x[6] = w.z, x[7] = w.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (k0 + j >= nvalid) x[j] = 0.0f;
+      bf16x8 b1, b2, b3;
+      split3(x, b1, b2, b3);
+#pragma unroll
+      for (int r = 0; r < RBW; ++r)
+        if (RBW == RB || r < nrb) {
+          const uint4 *a = sA + ((r * KSB + ks) * 3) * 64 + lane;
+          const uint4 u1 = a[0], u2 = a[64], u3 = a[128];
+          const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(&u1), a2 = *reinterpret_cast<const bf16x8 *>(&u2),
+                       a3 = *reinterpret_cast<const bf16x8 *>(&u3);
+          flt4 c = acc[r];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
+          acc[r] = c;
+        }
+    }
+    if (pid >= 0) {
+      float *y = Y + (size_t)(pi - ybase) * KPS + r0 * 16;
+#pragma unroll
+      for (int r = 0; r < RBW; ++r)
+        if (r < nrb)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) y[r * 16 + 4 * kg + q] = acc[r][q];
+    }
+  }
 }
 
 // bin of every M2L pair with per-target orders: class + kNumClasses * (order - kOrderLo)
@@ -271,6 +407,52 @@ void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, 
                const T *frag, int KPS, int nvalid) {
   if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
   launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
+}
+
+constexpr int rbw_bf3(int KP) {
+  return kLds / (pad32(KP) / 32 * 3072) < KP / 16 ? kLds / (pad32(KP) / 32 * 3072) : KP / 16;
+}
+
+template <int KP>
+void launch_bf3_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+                   const uint4 *fragb, int KPS, int nvalid) {
+  constexpr int RB = KP / 16, RBW = rbw_bf3(KP);
+  if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
+  const size_t lds = (size_t)RBW * (pad32(KP) / 32) * 3072;
+  static bool attr_set[64] = {};
+  if (lds > 65536 && !attr_set[c->device & 63]) {
+    NB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf3<KP, RBW>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set[c->device & 63] = true;
+  }
+  hipLaunchKernelGGL((k_m2l_bf3<KP, RBW>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(1024), lds, c->stream, items,
+                     ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, fragb,
+                     (const float *)c->mexp.p, (float *)c->ypair.p, KPS, nvalid);
+  NB_LAUNCH();
+}
+
+void launch_bf3(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+                const uint4 *fragb, int KPS, int nvalid) {
+  switch (KP) {
+#define NB_BF3_CASE(k)                                                                \
+  case k:                                                                             \
+    launch_bf3_kp<k>(c, nitems, items, ybase, cls_pairs, fragb, KPS, nvalid);         \
+    break;
+    NB_BF3_CASE(16)
+    NB_BF3_CASE(32)
+    NB_BF3_CASE(48)
+    NB_BF3_CASE(64)
+    NB_BF3_CASE(96)
+    NB_BF3_CASE(112)
+    NB_BF3_CASE(128)
+    NB_BF3_CASE(144)
+    NB_BF3_CASE(176)
+    NB_BF3_CASE(208)
+    NB_BF3_CASE(240)
+#undef NB_BF3_CASE
+    default:
+      throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
+  }
 }
 
 template <class T>
@@ -495,13 +677,14 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
   P.n_m2l_tiles = nitems;
   dbuf_ensure(c->ypair, (size_t)pass * KP * esz + 16);
   void *Lexp = c->lexp.p; // sized and cleared by the caller (one slot per cell that holds a local)
-  const int64_t nfrag = (int64_t)kNumClasses * KP * KP;
   const void *frag = c->tmat.p;
   if (f32) {
-    float *frag32 = dget<float>(c->tmat32, (size_t)nfrag);
-    hipLaunchKernelGGL(k_to_f32, dim3(nblk(nfrag)), dim3(kT), 0, s, (const double *)c->tmat.p, nfrag, frag32);
+    // three-term bf16 fragments of the operators
+    const int64_t nl = (int64_t)kNumClasses * (KP / 16) * (pad32(KP) / 32) * 64;
+    uint4 *fb = dget<uint4>(c->tmat32, (size_t)nl * 3);
+    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, KP, (const double *)c->tmat.p, fb);
     NB_LAUNCH();
-    frag = frag32;
+    frag = fb;
   }
   mark(c, 7);
   const int npass = (int)((npair + pass - 1) / pass);
@@ -513,7 +696,7 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
       const int KPT = ord ? padded_k(q) : KP, nvalid = ord ? (q + 1) * (q + 1) : KP;
       const int4 *it = ditems + runs[r].y;
       if (f32)
-        launch<float>(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const float *)frag, KP, nvalid);
+        launch_bf3(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const uint4 *)frag, KP, nvalid);
       else
         launch<double>(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const double *)frag, KP, nvalid);
     }
