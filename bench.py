@@ -45,6 +45,9 @@ CONFIGS = {
     "plummer1m": dict(n=1 << 20, leaf_capacity=64, leaves=(32, 64), orders=(10, 11, 12), cuts=(256, 512, 1024, 2048)),
     # 256 independent Plummer spheres, sizes log-uniform in [1e3, 6.4e4] (~3.9M in all), a uniform in [0.5, 2]
     "batched256": dict(n=None, leaf_capacity=32, leaves=(32,), orders=(12, 13, 14), cuts=(2048, 4096, 8192)),
+    # N = 8,388,608: 32 Plummer spheres (centres uniform in [0,100)^3, a log-uniform in [0.1, 2], sphere mass
+    # proportional to a, equal particle counts); learned order selection against the fixed orders
+    "learned8m": dict(n=1 << 23, leaf_capacity=32, leaves=(32,), orders=(10, 11, 12, 13, 14), cuts=(1024,)),
 }
 PREC_NAMES = {0: "fp64", 1: "mixed"}
 
@@ -117,6 +120,8 @@ def main():
     cfg = CONFIGS[args.config]
     if args.config == "batched256":
         return bench_batched(args, cfg)
+    if args.config == "learned8m":
+        return bench_learned(args, cfg)
     n = cfg["n"]
     orders = (args.order,) if args.order else cfg["orders"]
     cuts = (args.p2p_cut,) if args.p2p_cut is not None else cfg["cuts"]
@@ -303,6 +308,86 @@ def bench_batched(args, cfg):
                              for k, v in results.items()},
                    "mixed_over_plain": (results["mixed"]["value"] / results["fp64"]["value"]
                                         if "mixed" in results and "fp64" in results else None)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def bench_learned(args, cfg):
+    """The learned-selection workload: the lowest fixed order meeting the gate
+    (checked on sampled targets) against the learned per-cell orders, whose
+    model is trained in the run on two other mixes of 2^20 particles; fixed
+    order 14 is the fallback of the checked evaluation.  Mixed precision, cubic
+    cells."""
+    n = cfg["n"]
+    cut = args.p2p_cut if args.p2p_cut is not None else cfg["cuts"][0]
+    xyz, m = workloads.plummer_mix(n, n_spheres=32, seed=7)
+    # the mix's bounding box is not a cube (about 1.3 : 1 : 1): cubic cells
+    # keep every interaction class admissible for M2L
+    ctx = nbfmm.Context(order=14, leaf_capacity=cfg["leaf_capacity"], G=1.0, eps2=0.0, p2p_cut=cut, precision=1,
+                        cubic_box=1)
+    t0 = time.time()
+    ctx.order_model_init(seed=7)
+    rows = 0
+    for seed in (201, 202):
+        xt, mt = workloads.plummer_mix(1 << 20, n_spheres=32, seed=seed)
+        tx, tm = nbfmm.DeviceArray(xt), nbfmm.DeviceArray(mt)
+        rows += ctx.order_dataset_dev(len(mt), tx.ptr, tm.ptr, stride=2, seed=seed, tol=0.5 * args.gate)
+        tx.free()
+        tm.free()
+    loss = ctx.order_train(epochs=100, batch=256, lr=2e-3, seed=3)
+    t_train = time.time() - t0
+    log(f"[bench] learned8m: {rows} labelled cells, loss {loss[0]:.3f} -> {loss[-1]:.3f}, {t_train:.1f} s")
+    dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
+    da = nbfmm.DeviceArray(nbytes=n * 24, shape=(n, 3))
+
+    def run(order, select):
+        ctx.set_params(order=order, order_select=select, order_min=4, order_conf=0.9)
+        err, fb = ctx.eval_accel_checked_dev(n, dx.ptr, dm.ptr, da.ptr, ntgt=args.targets, seed=9, gate=args.gate,
+                                             fallback_order=14)
+        for _ in range(args.warmup):
+            ctx.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+        nbfmm.hip().hipStreamSynchronize(ctx.stream)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+        nbfmm.hip().hipStreamSynchronize(ctx.stream)
+        wall = time.perf_counter() - t
+        st = ctx.stats()
+        return {"order": order, "learned": bool(select), "value": n * args.steps / wall,
+                "ms_per_step": wall * 1e3 / args.steps, "checked_err_l2": err, "fell_back": fb,
+                "meets_gate": err <= args.gate and not fb, "mean_order": st.mean_order, "flop_m2l": st.flop_m2l}
+
+    fixed = None
+    table = []
+    for p in cfg["orders"]:
+        r = run(p, 0)
+        table.append(r)
+        log(f"[bench] learned8m fixed p={p}: {r['ms_per_step']:.2f} ms, err {r['checked_err_l2']:.3e}")
+        if r["meets_gate"]:
+            fixed = r
+            break
+    learned = []
+    for pmax in sorted({fixed["order"] if fixed else 14, 14}):
+        r = run(pmax, 1)
+        learned.append(r)
+        log(f"[bench] learned8m learned max order {pmax}: {r['ms_per_step']:.2f} ms, err {r['checked_err_l2']:.3e}, "
+            f"fell back {r['fell_back']}, mean order {r['mean_order']:.2f}, M2L {r['flop_m2l']:.3e} flop")
+    ok = [r for r in learned if r["meets_gate"]]
+    best_l = min(ok, key=lambda r: r["ms_per_step"]) if ok else None
+    cands = [r for r in [fixed, best_l] if r is not None]
+    head = min(cands, key=lambda r: r["ms_per_step"]) if cands else table[-1]
+    out = {
+        "metric": "particle-force evals/s at rel. force err <=1e-6 vs FP64 direct; mixed vs plain",
+        "value": head["value"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32/f64", "data": "synthetic",
+        "config": {"workload": f"32 Plummer spheres (centres in [0,100)^3, a log-uniform 0.1..2), N={n}, mixed, "
+                               f"cut {cut}, cubic box; learned per-cell orders vs the lowest fixed order meeting the gate",
+                   "N": n, "error_gate_l2": args.gate, "error_targets": args.targets,
+                   "fixed_scan": table, "learned": learned, "training_rows": rows, "training_s": round(t_train, 1),
+                   "loss_first_last": [float(loss[0]), float(loss[-1])]},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))

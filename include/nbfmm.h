@@ -75,6 +75,14 @@ typedef struct nbfmm_params {
   int32_t order_select;
   int32_t order_min;  /* >= 4 */
   double order_conf;  /* in (0, 1] */
+  /* 0: the reference's bounding box (tree.cpp:20-30, one extent per axis:
+   * cells have the box's aspect ratio).  1: the bounding cube centred on it
+   * (cubic cells).  With the reference's box, the nearest interaction classes
+   * along the short axes of a box longer than sqrt(2) : 1 converge slowly or
+   * not at all; their pairs are evaluated directly up to n_t * n_s < 2^20
+   * and by M2L above that, so such inputs reach high accuracy only with
+   * cubic_box = 1. */
+  int32_t cubic_box;
 } nbfmm_params;
 
 /* Reference-compatible defaults: p = 2, leaf capacity 20, G = 6.6743e-11,

@@ -37,6 +37,7 @@ int check_params(const nbfmm_params *p) {
   if (p->m2l_pass_pairs < 0) return fail(NBFMM_ERR_ARG, "m2l_pass_pairs must be >= 0");
   if (p->precision != NBFMM_PLAIN_FP64 && p->precision != NBFMM_MIXED)
     return fail(NBFMM_ERR_ARG, "unknown precision");
+  if (p->cubic_box != 0 && p->cubic_box != 1) return fail(NBFMM_ERR_ARG, "cubic_box must be 0 or 1");
   if (p->order_select != NBFMM_ORDER_FIXED && p->order_select != NBFMM_ORDER_LEARNED)
     return fail(NBFMM_ERR_ARG, "unknown order_select");
   if (p->order_select == NBFMM_ORDER_LEARNED) {

@@ -26,7 +26,9 @@ namespace nbfmm {
 // box whose longest side exceeds sqrt(2) times its shortest, where the series
 // no longer converges for particles near the cell corners.  Pairs of classes
 // with a rate above kMacRatio are evaluated directly, like the pairs below
-// p2p_cut (exact; for bounding boxes within 1.15:1 no class is affected).
+// p2p_cut (exact; for bounding boxes within 1.15:1 no class is affected),
+// as long as n_t * n_s < kBadDirect: larger pairs stay M2L, since evaluating
+// them directly grows as N^2 (the cubic_box option avoids such classes).
 ClsMask class_mask(const Geometry &geo) {
   ClsMask m{};
   const double hx = geo.h(0, 0), hy = geo.h(0, 1), hz = geo.h(0, 2);
@@ -118,7 +120,7 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, in
       const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
       const bool far = q >= 0 && !near;
       const int cls = (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3);
-      const bool sub = far && (nt * ns < cut || ((bad.w[cls >> 5] >> (cls & 31)) & 1u));
+      const bool sub = far && (nt * ns < cut || (((bad.w[cls >> 5] >> (cls & 31)) & 1u) && nt * ns < kBadDirect));
       const bool m2l = far && !sub;
       const uint64_t bf = __ballot(far), bs = __ballot(sub), bm = __ballot(m2l);
       if (FILL) {

@@ -109,6 +109,7 @@ struct ClsMask {
   uint32_t w[(kNumClasses + 31) / 32];
 };
 constexpr double kMacRatio = 0.9;
+constexpr int64_t kBadDirect = (int64_t)1 << 20; // largest n_t n_s evaluated directly for such classes
 ClsMask class_mask(const Geometry &geo);
 
 // ---- the device-side state of one evaluation -------------------------------------

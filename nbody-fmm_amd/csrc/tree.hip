@@ -360,6 +360,19 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
     P.geo.lo[d] = h[d];
     P.geo.hi[d] = h[3 + d];
   }
+  if (c->prm.cubic_box) {
+    // the bounding cube centred on the box (cubic cells), never smaller than the box
+    double ext = 0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, P.geo.hi[d] - P.geo.lo[d]);
+    for (int d = 0; d < 3; ++d) {
+      const double cc = 0.5 * (P.geo.lo[d] + P.geo.hi[d]);
+      double lo = cc - 0.5 * ext, hi = cc + 0.5 * ext;
+      if (lo > P.geo.lo[d]) lo = P.geo.lo[d];
+      if (hi < P.geo.hi[d]) hi = P.geo.hi[d];
+      P.geo.lo[d] = lo;
+      P.geo.hi[d] = hi;
+    }
+  }
   // ---- depth loop (tree.cpp:32-69) ----
   const uint32_t thr = c->prm.leaf_capacity;
   uint32_t maxcnt = (uint32_t)n;

@@ -37,6 +37,7 @@ class Params(ctypes.Structure):
         ("order_select", ctypes.c_int32),
         ("order_min", ctypes.c_int32),
         ("order_conf", ctypes.c_double),
+        ("cubic_box", ctypes.c_int32),
     ]
 
 

@@ -51,6 +51,7 @@ typedef struct orc_tree orc_tree;
  * thr = leaf threshold (reference hardcodes 20, tree.cpp:32).
  * Returns NULL on bad input (n < 1, non-finite coordinates). */
 orc_tree *orc_tree_build(int64_t n, const double *xyz, uint32_t thr);
+orc_tree *orc_tree_build_box(int64_t n, const double *xyz, uint32_t thr, int cubic);
 void orc_tree_free(orc_tree *t);
 int orc_tree_depth(const orc_tree *t);
 int64_t orc_tree_ncells(const orc_tree *t);
@@ -118,6 +119,9 @@ void orc_exact_m2p(uint32_t p, const double *M, const double t[3], double *phi, 
  * acc = G * grad.  Returns 0 on success. */
 int orc_fmm_exact(int64_t n, const double *xyz, const double *m, uint32_t p, uint32_t thr,
                   double G, int center_mode, int64_t p2p_cut, double *acc);
+/* the same on the cubic box of orc_tree_build_box(..., cubic = 1) when cubic != 0 */
+int orc_fmm_exact_box(int64_t n, const double *xyz, const double *m, uint32_t p, uint32_t thr, double G,
+                      int center_mode, int64_t p2p_cut, int cubic, double *acc);
 
 /* ---- Direct sums (naive_update.cpp:9-35) ------------------------------- */
 /* NaiveUpdate formula: grad -= m_j d / ((r^2 + eps2) * r), acc = G * grad.

@@ -34,6 +34,8 @@ def orc():
         L.orc_morton_decode3.argtypes = [ctypes.c_uint64, _vp, _vp, _vp]
         L.orc_tree_build.restype = _vp
         L.orc_tree_build.argtypes = [_i64, _vp, _u32]
+        L.orc_tree_build_box.restype = _vp
+        L.orc_tree_build_box.argtypes = [_i64, _vp, _u32, ctypes.c_int]
         L.orc_tree_free.argtypes = [_vp]
         L.orc_tree_depth.restype = ctypes.c_int
         L.orc_tree_depth.argtypes = [_vp]
@@ -60,6 +62,7 @@ def orc():
             getattr(L, f).argtypes = [_u32, _vp, _vp, _vp, _vp]
         L.orc_fmm_refcompat.argtypes = [_i64, _vp, _vp, _u32, _u32, _dbl, _vp]
         L.orc_fmm_exact.argtypes = [_i64, _vp, _vp, _u32, _u32, _dbl, ctypes.c_int, _i64, _vp]
+        L.orc_fmm_exact_box.argtypes = [_i64, _vp, _vp, _u32, _u32, _dbl, ctypes.c_int, _i64, ctypes.c_int, _vp]
         L.orc_direct_naive.argtypes = [_i64, _vp, _vp, _dbl, _dbl, _vp]
         L.orc_direct_targets.argtypes = [_i64, _vp, _vp, _dbl, _dbl, _i64, _vp, _vp]
         L.orc_step_se.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, _dbl]
@@ -91,12 +94,13 @@ def morton_decode(k):
     return tuple(v.value for v in c)
 
 
-def tree(xyz, thr):
-    """The reference tree rule as numpy arrays (cells level 0 first)."""
+def tree(xyz, thr, cubic=False):
+    """The reference tree rule as numpy arrays (cells level 0 first); with
+    cubic=True on the bounding cube centred on the reference's box."""
     xyz = _f64(xyz)
     n = len(xyz)
     L = orc()
-    h = L.orc_tree_build(n, _p(xyz), thr)
+    h = L.orc_tree_build_box(n, _p(xyz), thr, int(cubic))
     if not h:
         raise ValueError("oracle tree build rejected the input")
     try:
@@ -138,10 +142,10 @@ def fmm_refcompat(xyz, m, L=2, thr=20, G=1.0):
     return acc
 
 
-def fmm_exact(xyz, m, p, thr, G=1.0, center_mode=0, p2p_cut=0):
+def fmm_exact(xyz, m, p, thr, G=1.0, center_mode=0, p2p_cut=0, cubic=False):
     xyz, m = _f64(xyz), _f64(m)
     acc = np.zeros_like(xyz)
-    rc = orc().orc_fmm_exact(len(m), _p(xyz), _p(m), p, thr, G, center_mode, p2p_cut, _p(acc))
+    rc = orc().orc_fmm_exact_box(len(m), _p(xyz), _p(m), p, thr, G, center_mode, p2p_cut, int(cubic), _p(acc))
     if rc:
         raise ValueError("oracle exact FMM failed")
     return acc

@@ -226,8 +226,13 @@ static void p2p_cells(const orc_tree *t, const double *xyz, const double *mass, 
 
 int orc_fmm_exact(int64_t n, const double *xyz, const double *mass, uint32_t pu, uint32_t thr,
                   double G, int center_mode, int64_t p2p_cut, double *acc) {
+  return orc_fmm_exact_box(n, xyz, mass, pu, thr, G, center_mode, p2p_cut, 0, acc);
+}
+
+int orc_fmm_exact_box(int64_t n, const double *xyz, const double *mass, uint32_t pu, uint32_t thr,
+                      double G, int center_mode, int64_t p2p_cut, int cubic, double *acc) {
   if (n < 1 || pu < 1) return -1;
-  orc_tree *t = orc_tree_build(n, xyz, thr);
+  orc_tree *t = orc_tree_build_box(n, xyz, thr, cubic);
   if (!t) return -1;
   int p = (int)pu, D = t->depth;
   int64_t nc = t->ncells;
@@ -272,9 +277,10 @@ int orc_fmm_exact(int64_t n, const double *xyz, const double *mass, uint32_t pu,
       for (size_t q = 0; q < cs; ++q) Mc[cs * par + q] += tmp[q];
     }
   /* interaction lists: M2L, or direct P2P between the two cells when
-   * n_t * n_s < p2p_cut or when the offset is not admissible for the cell
-   * shape: (R_s + R_t) / |d| > 0.9 with R the half diagonal of a cell (the
-   * library's rule for boxes of the bounding box's aspect ratio) */
+   * n_t * n_s < p2p_cut, or when n_t * n_s < 2^20 and the offset is not
+   * admissible for the cell shape: (R_s + R_t) / |d| > 0.9 with R the half
+   * diagonal of a cell (the library's rule for boxes of the bounding box's
+   * aspect ratio) */
   const double ex = t->hi[0] - t->lo[0], ey = t->hi[1] - t->lo[1], ez = t->hi[2] - t->lo[2];
   const double diag2 = ex * ex + ey * ey + ez * ez;
   for (int64_t c = 0; c < nc; ++c)
@@ -287,7 +293,8 @@ int orc_fmm_exact(int64_t n, const double *xyz, const double *mass, uint32_t pu,
                    oz = (double)qs[2] - (double)qc[2];
       const double d2 = ox * ox * ex * ex + oy * oy * ey * ey + oz * oz * ez * ez;
       const int bad = diag2 > 0.9 * 0.9 * d2;
-      if (bad || (t->pend[c] - t->pbeg[c]) * (t->pend[s] - t->pbeg[s]) < p2p_cut) {
+      const int64_t np = (t->pend[c] - t->pbeg[c]) * (t->pend[s] - t->pbeg[s]);
+      if ((bad && np < ((int64_t)1 << 20)) || np < p2p_cut) {
         p2p_cells(t, xyz, mass, c, s, grad);
         continue;
       }

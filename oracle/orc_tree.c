@@ -177,7 +177,11 @@ static void cell_lists(const orc_tree *t, int lev, int64_t c, int64_t *nbr, int6
   }
 }
 
-orc_tree *orc_tree_build(int64_t n, const double *xyz, uint32_t thr) {
+orc_tree *orc_tree_build(int64_t n, const double *xyz, uint32_t thr) { return orc_tree_build_box(n, xyz, thr, 0); }
+
+/* cubic != 0: the bounding box is widened to the bounding cube centred on it
+ * (the library's cubic_box option); 0: the reference's box (tree.cpp:20-30) */
+orc_tree *orc_tree_build_box(int64_t n, const double *xyz, uint32_t thr, int cubic) {
   if (n < 1 || !xyz || thr < 1) return NULL;
   for (int64_t i = 0; i < 3 * n; ++i)
     if (!isfinite(xyz[i])) return NULL;
@@ -195,6 +199,19 @@ orc_tree *orc_tree_build(int64_t n, const double *xyz, uint32_t thr) {
       else if (v > b[d])
         b[d] = v;
     }
+  if (cubic) {
+    double ext = 0;
+    for (int d = 0; d < 3; ++d)
+      if (b[d] - a[d] > ext) ext = b[d] - a[d];
+    for (int d = 0; d < 3; ++d) {
+      const double c = 0.5 * (a[d] + b[d]);
+      double lo = c - 0.5 * ext, hi = c + 0.5 * ext;
+      if (lo > a[d]) lo = a[d];
+      if (hi < b[d]) hi = b[d];
+      a[d] = lo;
+      b[d] = hi;
+    }
+  }
   memcpy(t->lo, a, sizeof(a));
   memcpy(t->hi, b, sizeof(b));
 

@@ -108,6 +108,70 @@ def test_elongated_box_converges(ctx):
     assert err[12] < 0.1 * err[8] < 0.01 * err[4]
 
 
+def test_fmm_parity_elongated_large_cells(ctx):
+    # cells of about 1100 particles at level 2: pairs of the inadmissible
+    # classes with n_t n_s >= 2^20 stay M2L, smaller ones go to P2P (same cap
+    # in the oracle)
+    rng = np.random.default_rng(21)
+    xyz = rng.random((70000, 3)) * np.array([2.5, 1.25, 1.0])
+    m = rng.uniform(0.5, 1.5, 70000)
+    ctx.set_params(order=6, leaf_capacity=1300, p2p_cut=0, G=1.0, eps2=0.0)
+    a = ctx.eval_accel(xyz, m)
+    b = orc.fmm_exact(xyz, m, 6, 1300, G=1.0, center_mode=0, p2p_cut=0)
+    ctx.set_params(leaf_capacity=16)
+    assert _rel(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["elongated", "plummer"])
+def test_tree_bitexact_cubic(ctx, case):
+    if case == "elongated":
+        rng = np.random.default_rng(22)
+        xyz = rng.random((9000, 3)) * np.array([4.0, 1.0, 0.5]) + np.array([-3.0, 7.0, 0.25])
+        m = np.ones(9000)
+    else:
+        xyz, m = workloads.plummer(30000, seed=23)
+    ctx.set_params(leaf_capacity=16, order=4, p2p_cut=0, cubic_box=1)
+    try:
+        _, gt = ctx.eval_tree(xyz, m, with_il=True)
+    finally:
+        ctx.set_params(cubic_box=0)
+    ot = orc.tree(xyz, 16, cubic=True)
+    _check_tree(gt, ot, len(m))
+
+
+@pytest.mark.parametrize("p,cut", [(8, 0), (10, 512)])
+def test_fmm_parity_cubic(ctx, p, cut):
+    rng = np.random.default_rng(24)
+    xyz = rng.random((6000, 3)) * np.array([4.0, 1.0, 1.0])
+    m = rng.uniform(0.5, 1.5, 6000)
+    ctx.set_params(order=p, leaf_capacity=16, p2p_cut=cut, G=1.0, eps2=0.0, cubic_box=1)
+    try:
+        a = ctx.eval_accel(xyz, m)
+    finally:
+        ctx.set_params(cubic_box=0)
+    b = orc.fmm_exact(xyz, m, p, 16, G=1.0, center_mode=0, p2p_cut=cut, cubic=True)
+    assert _rel(a, b) < 1e-12
+
+
+def test_cubic_box_converges_on_long_box(ctx):
+    # a 4 : 1 : 1 box on its bounding cube: every class is admissible and the
+    # error falls geometrically with p
+    rng = np.random.default_rng(25)
+    xyz = rng.random((8000, 3)) * np.array([4.0, 1.0, 1.0])
+    m = rng.uniform(0.5, 1.5, 8000)
+    ctx.set_params(G=1.0, eps2=0.0)
+    d = ctx.direct_accel(xyz, m)
+    err = {}
+    try:
+        for p in (4, 8, 12):
+            ctx.set_params(order=p, leaf_capacity=16, p2p_cut=0, cubic_box=1)
+            err[p] = _rel(ctx.eval_accel(xyz, m), d)
+    finally:
+        ctx.set_params(cubic_box=0)
+    print(f"[cubic] errors {err}")
+    assert err[12] < 5e-6 and err[12] < 0.05 * err[8] < 0.0025 * err[4]
+
+
 def test_fmm_parity_plummer(ctx):
     xyz, m = workloads.plummer(40000, seed=1)
     ctx.set_params(order=8, leaf_capacity=64, p2p_cut=512, G=1.0, eps2=0.0)

@@ -236,6 +236,41 @@ def test_exact_fmm_converges_on_elongated_box():
     assert errs[2] < 0.1 * errs[1] < 0.01 * errs[0]
 
 
+def test_cubic_tree_equals_reference_tree_on_a_cube():
+    # when the bounding box already is a cube the cubic option changes nothing
+    rng = np.random.default_rng(26)
+    xyz = rng.random((3000, 3)) * 2.0 - 1.0
+    xyz[:8] = np.array([[x, y, z] for x in (-1.0, 1.0) for y in (-1.0, 1.0) for z in (-1.0, 1.0)])
+    a, b = orc.tree(xyz, 16), orc.tree(xyz, 16, cubic=True)
+    for k in ("level_start", "key", "pbeg", "pend", "parent", "perm", "nbr_off", "nbr", "il_off", "il", "lo", "hi"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_cubic_tree_geometry():
+    # the box is the bounding cube centred on the bounding box and contains it
+    rng = np.random.default_rng(27)
+    xyz = rng.random((4000, 3)) * np.array([5.0, 1.0, 0.25]) + np.array([1.0, -2.0, 3.0])
+    t = orc.tree(xyz, 16, cubic=True)
+    lo, hi = xyz.min(0), xyz.max(0)
+    ext = (hi - lo).max()
+    assert np.allclose(t["hi"] - t["lo"], ext, rtol=1e-15, atol=0)
+    assert np.allclose(0.5 * (t["lo"] + t["hi"]), 0.5 * (lo + hi), rtol=0, atol=1e-15)
+    assert np.all(t["lo"] <= lo) and np.all(t["hi"] >= hi)
+    # every particle lies in the leaf its key names
+    leaf = np.array(t["leaf_size"])
+    assert np.allclose(leaf, ext / (1 << t["depth"]), rtol=1e-15, atol=0)
+
+
+def test_exact_fmm_converges_on_long_box_cubic():
+    # a 4 : 1 : 1 box: on its bounding cube every class converges like a cube's
+    rng = np.random.default_rng(28)
+    xyz = rng.random((2000, 3)) * np.array([4.0, 1.0, 1.0])
+    m = rng.uniform(0.5, 1.5, 2000)
+    d = orc.direct_targets(xyz, m, np.arange(2000), G=1.0)
+    errs = [_rel(orc.fmm_exact(xyz, m, p, 16, G=1.0, cubic=True), d) for p in (4, 8, 12)]
+    assert errs[2] < 3e-6 and errs[2] < 0.05 * errs[1] < 0.0025 * errs[0]
+
+
 def test_direct_naive_two_body():
     xyz = np.array([[-10.0, 0, 0], [10.0, 0, 0]])
     m = np.array([1e9, 1e9])
