@@ -9,12 +9,12 @@
 // one cell against a range of the concatenated particles of a list of
 // source cells (a 27-neighbour row, or a cell's directly evaluated
 // interaction-list cells).  One wavefront runs a task: it prefix-sums the
-// list's cell sizes into LDS, then stages the sources 64 at a time (each
-// lane finds its source's cell by binary search), so lanes stay busy however
-// small the cells.  Every lane holds four targets and a source slot; the
-// slots are combined with xor-shuffles at the end.  Tasks of
-// one cell and of cells at different levels add into the same targets, so
-// the partial sums are added with FP64 atomics into the near-field gradient.
+// list's cell sizes into LDS and stages the task's targets there; each lane
+// then walks its own sources of the concatenation (its cell found by binary
+// search once, then moved forward), keeping one sum per target in
+// registers.  Tasks of one cell and of cells at different levels add into
+// the same targets, so the partial sums are added with FP64 atomics into the
+// near-field gradient.
 // P2P runs on its own stream, concurrently with the expansion passes (vector
 // ALU work beside MFMA work); k_finish writes acc = G * (far + near) in input
 // order (particle.cpp:16-21).
@@ -27,6 +27,33 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kW = 64; // one wavefront per task
+// Targets per pass over the sources: each takes 3 (4 with potentials) sums in
+// registers, and the register count sets how many wavefronts share a SIMD.
+#ifndef NBFMM_P2P_GROUP_F32
+#define NBFMM_P2P_GROUP_F32 16
+#endif
+#ifndef NBFMM_P2P_GROUP_F64
+#define NBFMM_P2P_GROUP_F64 8
+#endif
+static_assert(NBFMM_P2P_GROUP_F32 % 4 == 0 && NBFMM_P2P_GROUP_F64 % 4 == 0, "targets go in blocks of four");
+
+// list cell of concatenated source index g: the last with lp <= g (lp[0] = 0)
+__device__ __forceinline__ int first_cell(const int32_t *lp, int ne, int32_t g) {
+  int lo = 0, hi = ne;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (lp[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+// particle of source g, moving cell c forward (g never decreases for one c)
+__device__ __forceinline__ int32_t source_at(const int32_t *lb, const int32_t *lp, int ne, int &c, int32_t g) {
+  while (c + 1 < ne && lp[c + 1] <= g) ++c;
+  return lb[c] + (g - lp[c]);
+}
 static_assert(kTaskTargets == 4 * kW, "a P2P task holds at most four targets per lane");
 
 // A particle meets itself (and coincident particles meet each other) with
@@ -45,7 +72,9 @@ struct PairF64 {
   typedef double4 S;
   typedef double A;
   static constexpr bool kAnchor = false;
+  static constexpr int kGroup = NBFMM_P2P_GROUP_F64; // targets per pass (3 or 4 FP64 sums each in registers)
   static __device__ __forceinline__ S make(const double4 &p, const double4 &) { return p; }
+  static __device__ __forceinline__ void massless(S &v) { v.w = 0.0; }
   template <bool POT>
   static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
     const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -75,12 +104,14 @@ struct PairF32 {
   typedef f2pos S;
   typedef float A;
   static constexpr bool kAnchor = true;
+  static constexpr int kGroup = NBFMM_P2P_GROUP_F32; // targets per pass
   static __device__ __forceinline__ S make(const double4 &p, const double4 &a) {
     const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
     const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
     return {make_float4(hx, hy, hz, (float)p.w),
             make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.0f)};
   }
+  static __device__ __forceinline__ void massless(S &v) { v.hi.w = 0.0f; }
   template <bool POT>
   static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
     const float dx = (xi.hi.x - xj.hi.x) + (xi.lo.x - xj.lo.x);
@@ -103,11 +134,16 @@ struct PairF32 {
   }
 };
 
-// Lanes: tl target lanes (power of two, 4 targets each: t = ti + tl q) times
-// 64 / tl source slots; a staged chunk of 64 sources is read once per 4 pairs
-// and the slots are combined with xor-shuffles at the end.
-// One wavefront per task (tasks blockIdx.x, blockIdx.x + gridDim.x, ... when
-// the grid is smaller than the task list).  POT also sums m_j / r_ij into pot.
+// Lanes take sources, targets are unrolled: the task's targets are staged
+// in LDS and, kGroup at a time, every lane keeps one sum per target in
+// registers while it walks its sources (concatenated index lane, lane + 64,
+// ...); each target read is an LDS broadcast.  So every lane works whatever
+// the number of targets (blocks hold 1 to a few dozen particles against
+// thousands of sources), and a target's partial sums meet once, in an FP64
+// reduction across the wavefront.  A lane's sum runs over at most
+// kTaskPairs / 64 sources.  One wavefront per task (tasks blockIdx.x,
+// blockIdx.x + gridDim.x, ... when the grid is smaller than the task list).
+// POT also sums m_j / r_ij into pot.
 template <class Tr, bool POT>
 __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
@@ -117,117 +153,106 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *
                                                   double *__restrict__ pot) {
   typedef typename Tr::S S;
   typedef typename Tr::A A;
-  __shared__ S sh[kW];
+  __shared__ S st[kTaskTargets + Tr::kGroup]; // the last pass reads up to kGroup - 1 slots past nc
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   const int lane = threadIdx.x;
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
-  __syncthreads(); // the previous task's reads of lb, lp, sh are done
-  const P2PTask tk = tasks[task];
-  const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
-  double4 anchor = make_double4(0, 0, 0, 0);
-  if (Tr::kAnchor) {
-    for (int t = lane; t < nc; t += kW) {
-      const double4 q = pos[b + t];
-      anchor.x += q.x;
-      anchor.y += q.y;
-      anchor.z += q.z;
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-      anchor.x += __shfl_xor(anchor.x, off);
-      anchor.y += __shfl_xor(anchor.y, off);
-      anchor.z += __shfl_xor(anchor.z, off);
-    }
-    anchor.x /= nc;
-    anchor.y /= nc;
-    anchor.z /= nc;
-  }
-  const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
-  const int ne = tk.nlist;
-  // first concatenated index of every list cell
-  int32_t carry = 0;
-  for (int base = 0; base < ne; base += kW) {
-    const int e = base + lane;
-    const int32_t q = e < ne ? list[e] : -1;
-    const int32_t sb = q >= 0 ? cbeg[q] : 0;
-    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
-    int32_t x = cnt;
-    for (int off = 1; off < kW; off <<= 1) {
-      const int32_t y = __shfl_up(x, off);
-      if (lane >= off) x += y;
-    }
-    if (e < ne) {
-      lb[e] = sb;
-      lp[e] = carry + x - cnt;
-    }
-    carry += __shfl(x, kW - 1);
-  }
-  const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
-  int tl = 1;
-  while (tl * 4 < nc) tl <<= 1;
-  const int sub = kW / tl, ti = lane & (tl - 1), si = lane / tl;
-  S xi[4];
-  int32_t ii[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int32_t t = ti + tl * q;
-    ii[q] = t < nc ? b + t : -1;
-    xi[q] = Tr::make(pos[t < nc ? b + t : b], anchor);
-  }
-  A sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0}, sp[4] = {0, 0, 0, 0};
-  A cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
-  for (int32_t fb = f0; fb < f1; fb += kW) {
-    // stage sources [fb, fb + 64) of the concatenation
-    const int32_t g = fb + lane;
-    __syncthreads();
-    if (g < f1) {
-      int lo = 0, hi = ne; // last list cell with lp <= g (lp[0] = 0 <= g)
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (lp[mid] <= g)
-          lo = mid;
-        else
-          hi = mid;
+    __syncthreads(); // the previous task's reads of lb, lp, st are done
+    const P2PTask tk = tasks[task];
+    const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+    double4 anchor = make_double4(0, 0, 0, 0);
+    if (Tr::kAnchor) {
+      for (int t = lane; t < nc; t += kW) {
+        const double4 q = pos[b + t];
+        anchor.x += q.x;
+        anchor.y += q.y;
+        anchor.z += q.z;
       }
-      sh[lane] = Tr::make(pos[lb[lo] + (g - lp[lo])], anchor);
+      for (int off = 32; off >= 1; off >>= 1) {
+        anchor.x += __shfl_xor(anchor.x, off);
+        anchor.y += __shfl_xor(anchor.y, off);
+        anchor.z += __shfl_xor(anchor.z, off);
+      }
+      anchor.x /= nc;
+      anchor.y /= nc;
+      anchor.z /= nc;
     }
+    for (int t = lane; t < nc; t += kW) st[t] = Tr::make(pos[b + t], anchor);
+    const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
+    const int ne = tk.nlist;
+    // first concatenated index of every list cell
+    int32_t carry = 0;
+    for (int base = 0; base < ne; base += kW) {
+      const int e = base + lane;
+      const int32_t q = e < ne ? list[e] : -1;
+      const int32_t sb = q >= 0 ? cbeg[q] : 0;
+      const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
+      int32_t x = cnt;
+      for (int off = 1; off < kW; off <<= 1) {
+        const int32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+      }
+      if (e < ne) {
+        lb[e] = sb;
+        lp[e] = carry + x - cnt;
+      }
+      carry += __shfl(x, kW - 1);
+    }
+    const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
     __syncthreads();
-    const int cnt = min(kW, f1 - fb);
-    A ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0}, ap[4] = {0, 0, 0, 0};
-    for (int k = si; k < cnt; k += sub) {
-      const S xj = sh[k];
+    constexpr int kGroup = Tr::kGroup;
+    for (int t0 = 0; t0 < nc; t0 += kGroup) {
+      const int ng = min(kGroup, nc - t0);
+      A ax[kGroup], ay[kGroup], az[kGroup], ap[kGroup];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Tr::template pair<POT>(xi[q], xj, ax[q], ay[q], az[q], ap[q]);
-    }
+      for (int t = 0; t < kGroup; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0;
+      // a lane takes sources g and g + 64 (g = f0 + lane, + 128, ...): every
+      // target read serves two pairs; c1, c2 follow their list cells
+      int c1 = first_cell(lp, ne, f0 + lane), c2 = first_cell(lp, ne, f0 + lane + kW);
+      int32_t g = f0 + lane;
+      double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
+      if (g < f1) p1 = pos[source_at(lb, lp, ne, c1, g)];
+      if (g + kW < f1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
+      for (; g < f1; g += 2 * kW) {
+        const double4 q1 = p1, q2 = p2;
+        const bool has2 = g + kW < f1;
+        // the next sources' loads are in flight while these are evaluated
+        if (g + 2 * kW < f1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
+        if (g + 3 * kW < f1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
+        const S x1 = Tr::make(q1, anchor);
+        S x2 = Tr::make(has2 ? q2 : q1, anchor);
+        if (!has2) Tr::massless(x2); // contributes nothing
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      Tr::add(sx[q], cx[q], ax[q]);
-      Tr::add(sy[q], cy[q], ay[q]);
-      Tr::add(sz[q], cz[q], az[q]);
-      if (POT) Tr::add(sp[q], cp[q], ap[q]);
-    }
-  }
+        for (int t4 = 0; t4 < kGroup; t4 += 4)
+          if (t4 < ng) // four targets a branch (sums of slots past ng are never used)
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    double gx = (double)sx[q] + (double)cx[q], gy = (double)sy[q] + (double)cy[q],
-           gz = (double)sz[q] + (double)cz[q];
-    for (int o = tl; o < kW; o <<= 1) {
-      gx += __shfl_xor(gx, o);
-      gy += __shfl_xor(gy, o);
-      gz += __shfl_xor(gz, o);
+            for (int t = t4; t < t4 + 4; ++t) {
+              const S xt = st[t0 + t];
+              Tr::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
+              Tr::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
+            }
+      }
+#pragma unroll
+      for (int t = 0; t < kGroup; ++t) {
+        if (t >= ng) continue;
+        double gx = ax[t], gy = ay[t], gz = az[t], gp = POT ? (double)ap[t] : 0.0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          gx += __shfl_xor(gx, o);
+          gy += __shfl_xor(gy, o);
+          gz += __shfl_xor(gz, o);
+          if (POT) gp += __shfl_xor(gp, o);
+        }
+        if (lane == 0) {
+          const int64_t i = b + t0 + t;
+          atomicAdd(&grad[3 * i], gx);
+          atomicAdd(&grad[3 * i + 1], gy);
+          atomicAdd(&grad[3 * i + 2], gz);
+          if (POT) atomicAdd(&pot[i], gp);
+        }
+      }
     }
-    double gp = 0;
-    if (POT) {
-      gp = (double)sp[q] + (double)cp[q];
-      for (int o = tl; o < kW; o <<= 1) gp += __shfl_xor(gp, o);
-    }
-    if (si == 0 && ii[q] >= 0) {
-      atomicAdd(&grad[3 * (int64_t)ii[q]], gx);
-      atomicAdd(&grad[3 * (int64_t)ii[q] + 1], gy);
-      atomicAdd(&grad[3 * (int64_t)ii[q] + 2], gz);
-      if (POT) atomicAdd(&pot[ii[q]], gp);
-    }
-  }
   }
 }
 
