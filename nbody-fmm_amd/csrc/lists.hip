@@ -86,14 +86,34 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, in
     const int32_t ch = lane < 8 ? child8[(int64_t)P * 8 + lane] : -1;
     const bool need = ch >= 0 && !(skip_from >= 0 && ch >= skip_from && (int64_t)(cend[ch] - cbeg[ch]) * maxn < cut);
     todo = (uint32_t)__ballot(need);
-    if (todo)
-      for (int w = lane; w < kWin; w += kW) {
+    if (todo) {
+      // the four window entries of a lane: each load level issued for all
+      // four before the next (three dependent gathers, not twelve)
+      constexpr int kR = (kWin + kW - 1) / kW;
+      int32_t pn[kR], q[kR], nq[kR];
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        const int w = lane + r * kW;
         const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
-        const int32_t Pn = nbr27[(int64_t)P * 27 + (wz >> 1) * 9 + (wy >> 1) * 3 + (wx >> 1)];
-        const int32_t q = Pn >= 0 ? child8[(int64_t)Pn * 8 + ((wx & 1) | ((wy & 1) << 1) | ((wz & 1) << 2))] : -1;
-        cand[w] = q;
-        csz[w] = q >= 0 ? cend[q] - cbeg[q] : 0;
+        pn[r] = w < kWin ? nbr27[(int64_t)P * 27 + (wz >> 1) * 9 + (wy >> 1) * 3 + (wx >> 1)] : -1;
       }
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        const int w = lane + r * kW;
+        const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
+        q[r] = pn[r] >= 0 ? child8[(int64_t)pn[r] * 8 + ((wx & 1) | ((wy & 1) << 1) | ((wz & 1) << 2))] : -1;
+      }
+#pragma unroll
+      for (int r = 0; r < kR; ++r) nq[r] = q[r] >= 0 ? cend[q[r]] - cbeg[q[r]] : 0;
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        const int w = lane + r * kW;
+        if (w < kWin) {
+          cand[w] = q[r];
+          csz[w] = nq[r];
+        }
+      }
+    }
   }
   __syncthreads();
   if (!todo) return;
