@@ -40,9 +40,12 @@ import workloads  # noqa: E402
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X TFLOP/s (vector and dense MFMA alike)
 CONFIGS = {
     # N = 4,194,304 Plummer (a = 1, M = 1, equal masses), leaf capacity 64: the accuracy-gated metric's workload
-    "plummer4m": dict(n=1 << 22, leaf_capacity=64, leaves=(32, 64), orders=(10, 11, 12), cuts=(256, 512, 1024, 2048)),
+    # (tols: class_order_tol values tried; 0 = every M2L pair at p)
+    "plummer4m": dict(n=1 << 22, leaf_capacity=64, leaves=(32, 64), orders=(10, 11, 12), cuts=(256, 512, 1024, 2048),
+                      tols=(0.0, 0.05)),
     # N = 1,048,576 Plummer, p = 10, leaf capacity 64 (the plain FP64 configuration)
-    "plummer1m": dict(n=1 << 20, leaf_capacity=64, leaves=(32, 64), orders=(10, 11, 12), cuts=(256, 512, 1024, 2048)),
+    "plummer1m": dict(n=1 << 20, leaf_capacity=64, leaves=(32, 64), orders=(10, 11, 12), cuts=(256, 512, 1024, 2048),
+                      tols=(0.0, 0.05)),
     # 256 independent Plummer spheres, sizes log-uniform in [1e3, 6.4e4] (~3.9M in all), a uniform in [0.5, 2]
     "batched256": dict(n=None, leaf_capacity=32, leaves=(32,), orders=(12, 13, 14), cuts=(2048, 4096, 8192)),
     # N = 8,388,608: 32 Plummer spheres (centres uniform in [0,100)^3, a log-uniform in [0.1, 2], sphere mass
@@ -105,6 +108,8 @@ def main():
     ap.add_argument("--order", type=int, default=None, help="fix p instead of tuning")
     ap.add_argument("--p2p-cut", type=int, default=None, help="fix the P2P cut instead of tuning")
     ap.add_argument("--leaf", type=int, default=None, help="fix the leaf capacity instead of tuning")
+    ap.add_argument("--class-order-tol", type=float, default=None,
+                    help="fix the distance-dependent M2L order tolerance instead of tuning (0 = off)")
     ap.add_argument("--precision", default="all", choices=["all", "fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned order selection")
@@ -126,7 +131,8 @@ def main():
     orders = (args.order,) if args.order else cfg["orders"]
     cuts = (args.p2p_cut,) if args.p2p_cut is not None else cfg["cuts"]
     leaves = (args.leaf,) if args.leaf else cfg["leaves"]
-    log(f"[bench] config={args.config} N={n} leaves={leaves} orders={orders} cuts={cuts}")
+    tols = (args.class_order_tol,) if args.class_order_tol is not None else cfg.get("tols", (0.0,))
+    log(f"[bench] config={args.config} N={n} leaves={leaves} orders={orders} cuts={cuts} tols={tols}")
     xyz, m = workloads.plummer(n, seed=42)
     ctx = nbfmm.Context(order=orders[0], leaf_capacity=cfg["leaf_capacity"], G=1.0, eps2=0.0, p2p_cut=cuts[0])
     dx, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(m)
@@ -144,22 +150,24 @@ def main():
             log(f"[bench] precision {PREC_NAMES[prec]} unavailable: {e}")
             continue
         best, table = tune.tune(ctx, n, dx, dm, da, tgt, ref, orders, cuts, leaves=leaves, gate=args.gate,
-                                precision=prec, log=log)
+                                precision=prec, log=log, tols=tols)
         if best is None:
             best = min(table, key=lambda r: r["err"]["l2"])
             log(f"[bench] no setting meets the gate for {PREC_NAMES[prec]}; timing the most accurate one")
         ctx.set_params(order=best["order"], p2p_cut=best["p2p_cut"], leaf_capacity=best["leaf_capacity"],
-                       precision=prec)
+                       precision=prec, class_order_tol=best["class_order_tol"])
         wall, m2l_ms, p2p_ms, st = timed(ctx, n, dx, dm, da, args.steps, args.warmup)
         err = tune.rel_errors(da.download(ctx.stream)[tgt], ref)
         res = {"value": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps, "order": best["order"],
-               "p2p_cut": best["p2p_cut"], "leaf_capacity": best["leaf_capacity"], "err": err, "meets_gate": err["l2"] <= args.gate,
+               "p2p_cut": best["p2p_cut"], "leaf_capacity": best["leaf_capacity"],
+               "class_order_tol": best["class_order_tol"], "err": err, "meets_gate": err["l2"] <= args.gate,
                "m2l_kernel_ms": m2l_ms, "p2p_kernel_ms": p2p_ms, "stats": st.as_dict(),
                "tuning": [{"leaf_capacity": r["leaf_capacity"], "order": r["order"], "p2p_cut": r["p2p_cut"],
-                           "ms": round(r["ms"], 3), "l2": r["err"]["l2"]} for r in table]}
+                           "class_order_tol": r["class_order_tol"], "ms": round(r["ms"], 3), "l2": r["err"]["l2"]}
+                          for r in table]}
         results[PREC_NAMES[prec]] = res
         log(f"[bench] {PREC_NAMES[prec]}: leaf={best['leaf_capacity']} p={best['order']} cut={best['p2p_cut']} "
-            f"{res['ms_per_step']:.2f} ms/step, L2 err {err['l2']:.3e}")
+            f"tol={best['class_order_tol']} {res['ms_per_step']:.2f} ms/step, L2 err {err['l2']:.3e}")
     if "mixed" in results and not args.no_learned:
         lr = learned_mode(ctx, n, dx, dm, da, tgt, ref, results["mixed"], args)
         if lr is not None:
@@ -196,8 +204,10 @@ def main():
         "dtype": "f64" if head_name == "fp64" else "f32/f64", "data": "synthetic",
         "config": {
             "workload": f"Plummer sphere a=1 M=1 equal masses, N={n}, {head_name}, leaf capacity "
-                        f"{head['leaf_capacity']}, p={head['order']}, P2P cut {head['p2p_cut']}",
+                        f"{head['leaf_capacity']}, p={head['order']}, P2P cut {head['p2p_cut']}, M2L class-order tol "
+                        f"{head.get('class_order_tol', 0.0)}",
             "N": n, "precision": head_name, "order": head["order"], "p2p_cut": head["p2p_cut"],
+            "class_order_tol": head.get("class_order_tol", 0.0),
             "leaf_capacity": head["leaf_capacity"],
             "error_gate_l2": args.gate, "error_targets": len(tgt),
             "rel_force_err_vs_direct": head["err"], "meets_gate": head["meets_gate"],

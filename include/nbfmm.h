@@ -83,6 +83,14 @@ typedef struct nbfmm_params {
    * and by M2L above that, so such inputs reach high accuracy only with
    * cubic_box = 1. */
   int32_t cubic_box;
+  /* Distance-dependent M2L orders (no counterpart in the reference, which uses
+   * one L everywhere).  0: every M2L pair at `order`.  tol > 0: a pair of
+   * offset class d is truncated at the lowest q >= 4 (at most `order`) with
+   * rate_d^(q+1) <= tol * rate_0^(order+1), where rate_d = (cell diagonal) /
+   * |d| and rate_0 is the largest rate of the M2L classes (sqrt(3)/2 for
+   * cubic cells): far classes converge faster and need fewer terms.  Needs
+   * 4 <= order <= 14. */
+  double class_order_tol;
 } nbfmm_params;
 
 /* Reference-compatible defaults: p = 2, leaf capacity 20, G = 6.6743e-11,

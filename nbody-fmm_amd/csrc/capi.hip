@@ -38,6 +38,10 @@ int check_params(const nbfmm_params *p) {
   if (p->precision != NBFMM_PLAIN_FP64 && p->precision != NBFMM_MIXED)
     return fail(NBFMM_ERR_ARG, "unknown precision");
   if (p->cubic_box != 0 && p->cubic_box != 1) return fail(NBFMM_ERR_ARG, "cubic_box must be 0 or 1");
+  if (!(p->class_order_tol >= 0.0 && p->class_order_tol <= 1.0))
+    return fail(NBFMM_ERR_ARG, "class_order_tol must be in [0, 1]");
+  if (p->class_order_tol > 0.0 && (p->order < (uint32_t)kOrderLo || p->order > (uint32_t)kOrderHi))
+    return fail(NBFMM_ERR_ARG, "class orders need 4 <= order <= 14");
   if (p->order_select != NBFMM_ORDER_FIXED && p->order_select != NBFMM_ORDER_LEARNED)
     return fail(NBFMM_ERR_ARG, "unknown order_select");
   if (p->order_select == NBFMM_ORDER_LEARNED) {

@@ -764,7 +764,7 @@ void run_expansions(nbfmm_ctx *c) {
   c->p2p_started = false;
   dbuf_ensure(c->lexp, (size_t)nL * KP * esz + 16);
   NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));
-  run_m2l(c, KP, f32, ord);
+  run_m2l(c, KP, f32, ord, p);
   if (!c->p2p_started) launch_p2p(c, 1);
   // ---- L2L, coarsest level first ----
   for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);

@@ -42,6 +42,51 @@ ClsMask class_mask(const Geometry &geo) {
   return m;
 }
 
+// Distance-dependent orders.  The M2L of class d truncated after degree q
+// has the worst-case bound rate_d^(q+1) / |d|^2 (relative to the field of
+// the pair's source at unit distance), rate_d = (cell diagonal) / |d|; for
+// particles spread over their cells the error falls per degree by about
+// kOrderAlpha * rate_d instead (measured for cubes: 0.50 for the face class
+// (2,0,0) of rate 0.87, 0.15 for (3,3,3) of rate 0.33).  A class gets the
+// lowest q in [kOrderLo, p] with
+//   (a rate_d)^(q+1) / |d|^2 <= tol (a rate_0)^(p+1) / |d_0|^2,
+// class 0 being the slowest M2L class (rates and |d| the same for every
+// level: cells keep the box's aspect ratio); classes evaluated directly or
+// not admissible, and class 0, keep p.  The oracle computes the same table
+// with the same operations (orc_exact.c).
+ClsOrd class_orders(const Geometry &geo, int p, double tol) {
+  ClsOrd o{};
+  const double hx = geo.h(0, 0), hy = geo.h(0, 1), hz = geo.h(0, 2);
+  const double diag2 = hx * hx + hy * hy + hz * hz;
+  double r0 = 0, d0 = 0; // largest squared rate of an admissible class, its |d|^2
+  double r2[kNumClasses], dd[kNumClasses];
+  for (int k = 0; k < kNumClasses; ++k) {
+    const int dx = k % 7 - 3, dy = (k / 7) % 7 - 3, dz = k / 49 - 3;
+    r2[k] = -1;
+    if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) < 2) continue;
+    const double d2 = dx * dx * hx * hx + dy * dy * hy * hy + dz * dz * hz * hz;
+    if (diag2 > kMacRatio * kMacRatio * d2) continue;
+    r2[k] = diag2 / d2;
+    dd[k] = d2;
+    if (r2[k] > r0) {
+      r0 = r2[k];
+      d0 = d2;
+    }
+  }
+  const double la = std::log(kOrderAlpha);
+  for (int k = 0; k < kNumClasses; ++k) {
+    int q = p;
+    if (tol > 0 && r2[k] > 0 && r2[k] < r0) {
+      const double need = (std::log(tol) + (p + 1) * (la + 0.5 * std::log(r0)) + std::log(dd[k] / d0)) /
+                          (la + 0.5 * std::log(r2[k]));
+      q = (int)std::ceil(need) - 1;
+      q = std::min(p, std::max(kOrderLo, q));
+    }
+    o.q[k] = (uint8_t)q;
+  }
+  return o;
+}
+
 namespace {
 
 constexpr int kT = 256;

@@ -19,12 +19,13 @@
 // v_mfma_f32_16x16x32_bf16 (k_m2l_bf3); the per-target sum of the partials is
 // in FP64 in both.
 //
-// With an order per target cell (learned order selection), the pairs are
-// sorted by (order, class): a pair whose target has order q uses the leading
+// With an order per pair -- the order of its target cell (learned order
+// selection), capped by the order of its offset class (class_order_tol) --
+// the pairs are sorted by (order, class): a pair of order q uses the leading
 // (q+1)^2 x (q+1)^2 block of the class operator (coefficients of degree <= q
 // come first in the storage order), the tile width is the padded block
 // KPT = pad16((q+1)^2), source coefficients past (q+1)^2 are masked and the
-// sum takes the first (q+1)^2 rows.
+// sum takes the pair's first (q+1)^2 rows.
 #include "nbfmm_internal.h"
 
 #include <algorithm>
@@ -245,16 +246,19 @@ __global__ void __launch_bounds__(1024) k_m2l_bf3(const int4 *__restrict__ items
   }
 }
 
-// bin of every M2L pair with per-target orders: class + kNumClasses * (order - kOrderLo)
+// bin of every M2L pair with per-pair orders: class + kNumClasses * (order -
+// kOrderLo), order = min(target order (ord, or p when null), class order)
 __global__ void k_order_bins(int64_t ntgt, const int32_t *__restrict__ tgt_cells, const int32_t *__restrict__ off_m2l,
-                             const int32_t *__restrict__ cnt_m2l, const int8_t *__restrict__ ord,
+                             const int32_t *__restrict__ cnt_m2l, const int8_t *__restrict__ ord, int p, ClsOrd qc,
                              uint16_t *__restrict__ cls) {
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   if (t >= ntgt) return;
   const int32_t c = tgt_cells[t];
-  const int add = kNumClasses * ((int)ord[c] - kOrderLo);
-  for (int32_t q = off_m2l[c] + (threadIdx.x & 63), e = off_m2l[c] + cnt_m2l[c]; q < e; q += 64)
-    cls[q] = (uint16_t)(cls[q] + add);
+  const int ot = ord ? (int)ord[c] : p;
+  for (int32_t q = off_m2l[c] + (threadIdx.x & 63), e = off_m2l[c] + cnt_m2l[c]; q < e; q += 64) {
+    const int k = cls[q];
+    cls[q] = (uint16_t)(k + kNumClasses * (min(ot, (int)qc.q[k]) - kOrderLo));
+  }
 }
 
 // KP: tile width (padded block of the pair's order); KPS: storage width of
@@ -336,37 +340,50 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
 
 // L of every M2L target += sum of the partials of its pairs in the pass
 // [cb, ce) of class-sorted positions (pos[q] of list entry q), in list order,
-// in FP64.  With per-cell orders (ord != null) only the (q+1)^2 coefficients
-// of the target's order q are added.
+// in FP64.  With per-pair orders (bins != null: the pair's bin, order
+// kOrderLo + bin / kNumClasses) only the (q+1)^2 coefficients of a pair's
+// order q are added.
+__device__ __forceinline__ int pair_rows(const uint16_t *bins, int32_t q, int KP) {
+  if (!bins) return KP;
+  const int o = kOrderLo + (int)bins[q] / kNumClasses;
+  return (o + 1) * (o + 1);
+}
+
 template <class T>
 __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
                                                 const int32_t *__restrict__ lslot, const int32_t *__restrict__ pos,
                                                 int32_t cb, int32_t ce, const T *__restrict__ Y,
-                                                T *__restrict__ Lexp, const int8_t *__restrict__ ord) {
+                                                T *__restrict__ Lexp, const uint16_t *__restrict__ bins) {
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (t >= ntgt) return;
   const int32_t c = tgt_cells[t];
   const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
-  const int nv = ord ? ((int)ord[c] + 1) * ((int)ord[c] + 1) : KP; // rows added
   T *L = Lexp + (size_t)lslot[c] * KP;
   // rows lane, lane + 64, ... (KP <= 256), pairs four at a time
   double s[4] = {0, 0, 0, 0};
+  int nmax = 0; // rows any pair added
   int32_t q = b;
   for (; q + 4 <= e; q += 4) {
-    int32_t pr[4];
+    int32_t pr[4], nv[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) pr[u] = pos[q + u];
+    for (int u = 0; u < 4; ++u) {
+      pr[u] = pos[q + u];
+      nv[u] = pair_rows(bins, q + u, KP);
+      nmax = max(nmax, nv[u]);
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (pr[u] >= cb && pr[u] < ce)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (lane + 64 * r < nv) s[r] += (double)Y[(size_t)(pr[u] - cb) * KP + lane + 64 * r];
+          if (lane + 64 * r < nv[u]) s[r] += (double)Y[(size_t)(pr[u] - cb) * KP + lane + 64 * r];
   }
   for (; q < e; ++q) {
     const int32_t pr = pos[q];
+    const int nv = pair_rows(bins, q, KP);
+    nmax = max(nmax, nv);
     if (pr >= cb && pr < ce)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
@@ -374,7 +391,7 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    if (lane + 64 * r < nv) L[lane + 64 * r] = (T)((double)L[lane + 64 * r] + s[r]);
+    if (lane + 64 * r < nmax) L[lane + 64 * r] = (T)((double)L[lane + 64 * r] + s[r]);
 }
 
 template <class T>
@@ -600,16 +617,19 @@ inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT 
 // Expects the FP64 operators in c->tmat and the multipoles in c->mexp
 // (FP32 when `f32`), both of width KP.  `ord` (device, per cell) gives the
 // order of every target, or is null when all have the order of KP.
-void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
+void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
   Plan &P = c->plan;
   hipStream_t s = c->stream;
   const int64_t nt = P.n_tgt, npair = P.n_m2l;
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
-  const int nbins = ord ? kMaxBins : kNumClasses;
-  if (ord) {
+  // per-pair orders: the targets' (learned) and/or the classes'
+  const bool binned = ord || c->prm.class_order_tol > 0;
+  const int nbins = binned ? kMaxBins : kNumClasses;
+  if (binned) {
+    const ClsOrd qc = class_orders(P.geo, p, c->prm.class_order_tol);
     hipLaunchKernelGGL(k_order_bins, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, nt,
                        (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                       ord, (uint16_t *)c->m2l_cls.p);
+                       ord, p, qc, (uint16_t *)c->m2l_cls.p);
     NB_LAUNCH();
   }
   int32_t *ccnt = dget<int32_t>(c->cls_cnt, 2 * kMaxBins + 8);
@@ -627,15 +647,14 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
   // algorithmic flops: 2 n^2 per pair, n = (q+1)^2 coefficients of the pair's order q
   {
     double fl = 0;
-    const int p = (int)c->prm.order;
     for (int k = 0; k < nbins; ++k) {
-      const int q = ord ? kOrderLo + k / kNumClasses : p;
+      const int q = binned ? kOrderLo + k / kNumClasses : p;
       const double nq = (double)(q + 1) * (q + 1);
       fl += 2.0 * nq * nq * hcnt[k];
     }
     P.flop_m2l = fl;
     double qs = 0;
-    for (int k = 0; k < nbins; ++k) qs += (double)(ord ? kOrderLo + k / kNumClasses : p) * hcnt[k];
+    for (int k = 0; k < nbins; ++k) qs += (double)(binned ? kOrderLo + k / kNumClasses : p) * hcnt[k];
     P.mean_order = npair > 0 ? qs / (double)npair : (double)p;
   }
   // passes: consecutive ranges of the sorted pairs whose partials fit the
@@ -692,8 +711,8 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
   for (int ps = 0; ps < npass; ++ps) {
     const int32_t pb = (int32_t)((int64_t)ps * pass), pe = (int32_t)std::min<int64_t>(npair, (int64_t)pb + pass);
     for (; r < runs.size() && runs[r].x == ps; ++r) {
-      const int q = ord ? kOrderLo + runs[r].w : (int)c->prm.order;
-      const int KPT = ord ? padded_k(q) : KP, nvalid = ord ? (q + 1) * (q + 1) : KP;
+      const int q = binned ? kOrderLo + runs[r].w : p;
+      const int KPT = binned ? padded_k(q) : KP, nvalid = binned ? (q + 1) * (q + 1) : KP;
       const int4 *it = ditems + runs[r].y;
       if (f32)
         launch_bf3(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const uint4 *)frag, KP, nvalid);
@@ -709,12 +728,12 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord) {
       hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                          (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
                          (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
-                         (const float *)c->ypair.p, (float *)Lexp, ord);
+                         (const float *)c->ypair.p, (float *)Lexp, binned ? (const uint16_t *)c->m2l_cls.p : nullptr);
     else
       hipLaunchKernelGGL(k_m2l_sum<double>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                          (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
                          (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
-                         (const double *)c->ypair.p, (double *)Lexp, ord);
+                         (const double *)c->ypair.p, (double *)Lexp, binned ? (const uint16_t *)c->m2l_cls.p : nullptr);
     NB_LAUNCH();
   }
 }

@@ -111,6 +111,13 @@ struct ClsMask {
 constexpr double kMacRatio = 0.9;
 constexpr int64_t kBadDirect = (int64_t)1 << 20; // largest n_t n_s evaluated directly for such classes
 ClsMask class_mask(const Geometry &geo);
+// M2L order of every offset class (class_order_tol > 0), kernel argument;
+// kOrderAlpha: observed decay per degree over the worst-case rate
+constexpr double kOrderAlpha = 0.55;
+struct ClsOrd {
+  uint8_t q[kNumClasses + 1];
+};
+ClsOrd class_orders(const Geometry &geo, int p, double tol);
 
 // ---- the device-side state of one evaluation -------------------------------------
 struct Plan {
@@ -216,7 +223,7 @@ double sampled_error(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double 
 void dataset_get(nbfmm_ctx *c, float *feat_h, int8_t *lab_h, int64_t cap);
 void dataset_put(nbfmm_ctx *c, int64_t n, const float *feat_h, const int8_t *lab_h);
 // m2l.hip
-void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord);
+void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p);
 // M2M (l2l false) or L2L (true) for `ncell` parents listed in `cells` (device)
 void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells);
 void trans_to_f32(nbfmm_ctx *c, int KP);

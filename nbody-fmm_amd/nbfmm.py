@@ -38,6 +38,7 @@ class Params(ctypes.Structure):
         ("order_min", ctypes.c_int32),
         ("order_conf", ctypes.c_double),
         ("cubic_box", ctypes.c_int32),
+        ("class_order_tol", ctypes.c_double),
     ]
 
 

@@ -1,6 +1,7 @@
-"""Accuracy-gated choice of the expansion order, the P2P cut and the leaf capacity.
+"""Accuracy-gated choice of the expansion order, the P2P cut, the leaf capacity and
+the distance-dependent M2L order tolerance (class_order_tol).
 
-For a particle set resident on the device, every (order, p2p_cut, leaf capacity) candidate
+For a particle set resident on the device, every (order, p2p_cut, leaf capacity, tol) candidate
 is evaluated (one warm-up evaluation, then one timed with the library's
 events), its relative force error is measured against the FP64 direct sum
 (compensated, eps2 = 0) on sampled targets, and the fastest candidate whose
@@ -33,27 +34,30 @@ def rel_errors(acc_t, ref):
             "p99": float(np.percentile(e_i, 99))}
 
 
-def tune(ctx, n, d_xyz, d_m, d_acc, tgt, ref, orders, cuts, leaves=(None,), gate=1e-6, precision=0, log=None):
-    """Returns (best, table): best = dict(order, p2p_cut, leaf_capacity, ms, err) or None."""
+def tune(ctx, n, d_xyz, d_m, d_acc, tgt, ref, orders, cuts, leaves=(None,), gate=1e-6, precision=0, log=None,
+         tols=(0.0,)):
+    """Returns (best, table): best = dict(order, p2p_cut, leaf_capacity, class_order_tol, ms, err) or None."""
     table = []
     best = None
     for leaf in leaves:
         if leaf is not None:
             ctx.set_params(leaf_capacity=leaf)
         for p in orders:
-            for cut in cuts:
-                ctx.set_params(order=p, p2p_cut=cut, precision=precision)
-                ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
-                ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
-                ms = ctx.stats().ms_total
-                acc = d_acc.download(ctx.stream)
-                err = rel_errors(acc[tgt], ref)
-                row = {"order": p, "p2p_cut": cut, "leaf_capacity": int(ctx.params.leaf_capacity), "ms": ms,
-                       "err": err}
-                table.append(row)
-                if log:
-                    log(f"[tune] precision={precision} leaf={row['leaf_capacity']} p={p} cut={cut}: {ms:.2f} ms, "
-                        f"L2 err {err['l2']:.3e}")
-                if err["l2"] <= gate and (best is None or ms < best["ms"]):
-                    best = row
+            for tol in tols:
+                for cut in cuts:
+                    ctx.set_params(order=p, p2p_cut=cut, precision=precision, class_order_tol=tol)
+                    ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
+                    ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
+                    ms = ctx.stats().ms_total
+                    acc = d_acc.download(ctx.stream)
+                    err = rel_errors(acc[tgt], ref)
+                    row = {"order": p, "p2p_cut": cut, "leaf_capacity": int(ctx.params.leaf_capacity),
+                           "class_order_tol": tol, "ms": ms, "err": err}
+                    table.append(row)
+                    if log:
+                        log(f"[tune] precision={precision} leaf={row['leaf_capacity']} p={p} tol={tol} cut={cut}: "
+                            f"{ms:.2f} ms, L2 err {err['l2']:.3e}")
+                    if err["l2"] <= gate and (best is None or ms < best["ms"]):
+                        best = row
+    ctx.set_params(class_order_tol=0.0)
     return best, table

@@ -122,6 +122,11 @@ int orc_fmm_exact(int64_t n, const double *xyz, const double *m, uint32_t p, uin
 /* the same on the cubic box of orc_tree_build_box(..., cubic = 1) when cubic != 0 */
 int orc_fmm_exact_box(int64_t n, const double *xyz, const double *m, uint32_t p, uint32_t thr, double G,
                       int center_mode, int64_t p2p_cut, int cubic, double *acc);
+/* with distance-dependent M2L orders (class_order_tol > 0; 0 = p everywhere) */
+int orc_fmm_exact_opt(int64_t n, const double *xyz, const double *m, uint32_t p, uint32_t thr, double G,
+                      int center_mode, int64_t p2p_cut, int cubic, double class_order_tol, double *acc);
+/* the order of each of the 343 offset classes (index (dz+3)*49 + (dy+3)*7 + (dx+3)) */
+void orc_class_orders(const double ext[3], int p, double tol, int *q);
 
 /* ---- Direct sums (naive_update.cpp:9-35) ------------------------------- */
 /* NaiveUpdate formula: grad -= m_j d / ((r^2 + eps2) * r), acc = G * grad.

@@ -63,6 +63,10 @@ def orc():
         L.orc_fmm_refcompat.argtypes = [_i64, _vp, _vp, _u32, _u32, _dbl, _vp]
         L.orc_fmm_exact.argtypes = [_i64, _vp, _vp, _u32, _u32, _dbl, ctypes.c_int, _i64, _vp]
         L.orc_fmm_exact_box.argtypes = [_i64, _vp, _vp, _u32, _u32, _dbl, ctypes.c_int, _i64, ctypes.c_int, _vp]
+        L.orc_fmm_exact_opt.argtypes = [_i64, _vp, _vp, _u32, _u32, _dbl, ctypes.c_int, _i64, ctypes.c_int, _dbl,
+                                        _vp]
+        L.orc_class_orders.argtypes = [_vp, ctypes.c_int, _dbl, _vp]
+        L.orc_class_orders.restype = None
         L.orc_direct_naive.argtypes = [_i64, _vp, _vp, _dbl, _dbl, _vp]
         L.orc_direct_targets.argtypes = [_i64, _vp, _vp, _dbl, _dbl, _i64, _vp, _vp]
         L.orc_step_se.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, _dbl]
@@ -92,6 +96,15 @@ def morton_decode(k):
     c = [ctypes.c_uint32() for _ in range(3)]
     orc().orc_morton_decode3(k, *(ctypes.byref(v) for v in c))
     return tuple(v.value for v in c)
+
+
+def class_orders(ext, p, tol):
+    """M2L order of each of the 343 offset classes (index (dz+3)*49 + (dy+3)*7 + (dx+3))
+    for a box of extents `ext` at order p and class_order_tol `tol`."""
+    q = np.zeros(343, np.int32)
+    e = _f64(np.asarray(ext, dtype=np.float64))
+    orc().orc_class_orders(_p(e), int(p), float(tol), _p(q))
+    return q
 
 
 def tree(xyz, thr, cubic=False):
@@ -142,10 +155,11 @@ def fmm_refcompat(xyz, m, L=2, thr=20, G=1.0):
     return acc
 
 
-def fmm_exact(xyz, m, p, thr, G=1.0, center_mode=0, p2p_cut=0, cubic=False):
+def fmm_exact(xyz, m, p, thr, G=1.0, center_mode=0, p2p_cut=0, cubic=False, class_order_tol=0.0):
     xyz, m = _f64(xyz), _f64(m)
     acc = np.zeros_like(xyz)
-    rc = orc().orc_fmm_exact_box(len(m), _p(xyz), _p(m), p, thr, G, center_mode, p2p_cut, int(cubic), _p(acc))
+    rc = orc().orc_fmm_exact_opt(len(m), _p(xyz), _p(m), p, thr, G, center_mode, p2p_cut, int(cubic),
+                                 float(class_order_tol), _p(acc))
     if rc:
         raise ValueError("oracle exact FMM failed")
     return acc

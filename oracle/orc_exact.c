@@ -231,6 +231,49 @@ int orc_fmm_exact(int64_t n, const double *xyz, const double *mass, uint32_t pu,
 
 int orc_fmm_exact_box(int64_t n, const double *xyz, const double *mass, uint32_t pu, uint32_t thr,
                       double G, int center_mode, int64_t p2p_cut, int cubic, double *acc) {
+  return orc_fmm_exact_opt(n, xyz, mass, pu, thr, G, center_mode, p2p_cut, cubic, 0.0, acc);
+}
+
+/* Distance-dependent M2L orders (the library's class_order_tol; not in the
+ * reference).  For the offset classes d (integers in [-3,3]^3, Chebyshev
+ * distance >= 2) that are admissible (diag^2 <= 0.81 |d|^2), rate_d^2 =
+ * diag^2 / |d|^2 in units of the box extents; class 0 has the largest rate.
+ * A class with rate below it gets the lowest q in [4, p] with
+ * (0.55 rate_d)^(q+1) / |d|^2 <= tol (0.55 rate_0)^(p+1) / |d_0|^2; every
+ * other class keeps p.  The same operations as the library's table, so both
+ * give the same integers. */
+void orc_class_orders(const double ext[3], int p, double tol, int *q) {
+  const double ex = ext[0], ey = ext[1], ez = ext[2];
+  const double diag2 = ex * ex + ey * ey + ez * ez;
+  double r2[343], dd[343], r0 = 0, d0 = 0;
+  for (int k = 0; k < 343; ++k) {
+    const double ox = k % 7 - 3, oy = (k / 7) % 7 - 3, oz = k / 49 - 3;
+    r2[k] = -1;
+    if (fabs(ox) < 2 && fabs(oy) < 2 && fabs(oz) < 2) continue;
+    const double d2 = ox * ox * ex * ex + oy * oy * ey * ey + oz * oz * ez * ez;
+    if (diag2 > 0.9 * 0.9 * d2) continue;
+    r2[k] = diag2 / d2;
+    dd[k] = d2;
+    if (r2[k] > r0) {
+      r0 = r2[k];
+      d0 = d2;
+    }
+  }
+  const double la = log(0.55);
+  for (int k = 0; k < 343; ++k) {
+    int qk = p;
+    if (tol > 0 && r2[k] > 0 && r2[k] < r0) {
+      const double need = (log(tol) + (p + 1) * (la + 0.5 * log(r0)) + log(dd[k] / d0)) / (la + 0.5 * log(r2[k]));
+      qk = (int)ceil(need) - 1;
+      if (qk < 4) qk = 4;
+      if (qk > p) qk = p;
+    }
+    q[k] = qk;
+  }
+}
+
+int orc_fmm_exact_opt(int64_t n, const double *xyz, const double *mass, uint32_t pu, uint32_t thr, double G,
+                      int center_mode, int64_t p2p_cut, int cubic, double class_order_tol, double *acc) {
   if (n < 1 || pu < 1) return -1;
   orc_tree *t = orc_tree_build_box(n, xyz, thr, cubic);
   if (!t) return -1;
@@ -283,6 +326,11 @@ int orc_fmm_exact_box(int64_t n, const double *xyz, const double *mass, uint32_t
    * aspect ratio) */
   const double ex = t->hi[0] - t->lo[0], ey = t->hi[1] - t->lo[1], ez = t->hi[2] - t->lo[2];
   const double diag2 = ex * ex + ey * ey + ez * ez;
+  int qcls[343];
+  {
+    const double ext[3] = {ex, ey, ez};
+    orc_class_orders(ext, p, class_order_tol, qcls);
+  }
   for (int64_t c = 0; c < nc; ++c)
     for (int64_t q = t->il_off[c]; q < t->il_off[c + 1]; ++q) {
       int64_t s = t->il[q];
@@ -299,7 +347,11 @@ int orc_fmm_exact_box(int64_t n, const double *xyz, const double *mass, uint32_t
         continue;
       }
       double Dv[3] = {ctr[3 * c] - ctr[3 * s], ctr[3 * c + 1] - ctr[3 * s + 1], ctr[3 * c + 2] - ctr[3 * s + 2]};
-      orc_exact_m2l(pu, Mc + cs * s, Dv, tmp);
+      /* the pair's order: its class's (p unless class_order_tol > 0);
+       * degrees above it stay zero */
+      const int cls = ((int)oz + 3) * 49 + ((int)oy + 3) * 7 + ((int)ox + 3);
+      for (size_t k = 0; k < cs; ++k) tmp[k] = 0;
+      orc_exact_m2l((uint32_t)qcls[cls], Mc + cs * s, Dv, tmp);
       for (size_t k = 0; k < cs; ++k) Lc[cs * c + k] += tmp[k];
     }
   /* L2L downward */

@@ -172,6 +172,42 @@ def test_cubic_box_converges_on_long_box(ctx):
     assert err[12] < 5e-6 and err[12] < 0.05 * err[8] < 0.0025 * err[4]
 
 
+@pytest.mark.parametrize("p,cut,tol", [(8, 0, 0.1), (12, 256, 0.05), (6, 0, 1.0)])
+def test_fmm_parity_class_orders(ctx, p, cut, tol):
+    # distance-dependent M2L orders: the same class table and per-pair
+    # truncation as the oracle
+    xyz, m = workloads.uniform_cube(3000, seed=30 + p)
+    ctx.set_params(order=p, leaf_capacity=16, p2p_cut=cut, G=1.0, eps2=0.0, class_order_tol=tol)
+    try:
+        a = ctx.eval_accel(xyz, m)
+        st = ctx.stats()
+    finally:
+        ctx.set_params(class_order_tol=0.0)
+    b = orc.fmm_exact(xyz, m, p, 16, G=1.0, center_mode=0, p2p_cut=cut, class_order_tol=tol)
+    assert _rel(a, b) < 1e-12
+    assert st.mean_order < p  # some pairs run below p
+
+
+@pytest.mark.parametrize("tol", [0.05, 0.3])
+def test_mixed_parity_class_orders(ctx, tol):
+    xyz, m = workloads.plummer(20000, seed=31)
+    ctx.set_params(order=11, leaf_capacity=32, p2p_cut=1024, G=1.0, eps2=0.0, precision=1, class_order_tol=tol)
+    try:
+        a = ctx.eval_accel(xyz, m)
+    finally:
+        ctx.set_params(class_order_tol=0.0, precision=0)
+    b = orc.fmm_exact(xyz, m, 11, 32, G=1.0, center_mode=0, p2p_cut=1024, class_order_tol=tol)
+    assert _rel(a, b) < 2e-6
+
+
+def test_class_orders_bad_params(ctx):
+    with pytest.raises(nbfmm.NbfmmError):
+        ctx.set_params(order=3, class_order_tol=0.1)
+    with pytest.raises(nbfmm.NbfmmError):
+        ctx.set_params(class_order_tol=-0.5)
+    ctx.set_params(order=4, class_order_tol=0.0)
+
+
 def test_fmm_parity_plummer(ctx):
     xyz, m = workloads.plummer(40000, seed=1)
     ctx.set_params(order=8, leaf_capacity=64, p2p_cut=512, G=1.0, eps2=0.0)

@@ -271,6 +271,80 @@ def test_exact_fmm_converges_on_long_box_cubic():
     assert errs[2] < 3e-6 and errs[2] < 0.05 * errs[1] < 0.0025 * errs[0]
 
 
+def _class_of(dx, dy, dz):
+    return (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3)
+
+
+def test_class_orders_table_shape():
+    # cubes: the face classes keep p, orders stay in [4, p], fall with |d|
+    # (never rise), classes inside the near field keep p; tol 0 keeps p
+    for p in (6, 11, 14):
+        assert np.all(orc.class_orders([1, 1, 1], p, 0.0) == p)
+        for tol in (0.01, 0.1, 1.0):
+            q = orc.class_orders([1, 1, 1], p, tol)
+            assert q[_class_of(2, 0, 0)] == p and q[_class_of(0, 0, -2)] == p
+            assert q[_class_of(1, 1, 1)] == p and q[_class_of(0, 0, 0)] == p
+            assert q.min() >= min(4, p) and q.max() <= p
+            far = [(dx, dy, dz) for dx in range(-3, 4) for dy in range(-3, 4) for dz in range(-3, 4)
+                   if max(abs(dx), abs(dy), abs(dz)) >= 2]
+            by_d = sorted(far, key=lambda o: o[0] ** 2 + o[1] ** 2 + o[2] ** 2)
+            qs = [q[_class_of(*o)] for o in by_d]
+            assert all(a >= b for a, b in zip(qs, qs[1:]))
+            # symmetric under sign changes and axis permutations
+            assert q[_class_of(3, 1, 2)] == q[_class_of(-2, 3, -1)]
+        # a larger tolerance never raises an order
+        assert np.all(orc.class_orders([1, 1, 1], p, 1.0) <= orc.class_orders([1, 1, 1], p, 0.1))
+        assert orc.class_orders([1, 1, 1], p, 1.0)[_class_of(3, 3, 3)] < p
+
+
+def test_class_orders_truncation_matches_model():
+    # per-class M2L error of uniform cells at order q against the direct field
+    # falls per degree by at most 0.55 * rate (the rule's model), rate = sqrt(3) / |d|
+    rng = np.random.default_rng(40)
+    P = 12
+
+    def tsz(p):
+        return (p + 1) * (p + 2) // 2
+    L = orc.orc()
+    for cl in ((2, 2, 2), (3, 0, 0), (3, 3, 3)):
+        e = np.zeros(P + 1)
+        for _ in range(6):
+            src = rng.random((16, 3)) - 0.5 + np.array(cl, float)
+            tgt = rng.random((16, 3)) - 0.5
+            d = tgt[:, None, :] - src[None, :, :]
+            f = -(d / ((d ** 2).sum(-1) ** 1.5)[..., None]).sum(1)
+            M = np.zeros(2 * tsz(P))
+            for j in range(16):
+                r = np.ascontiguousarray(src[j] - np.array(cl, float))
+                L.orc_exact_p2m_add(P, orc._p(r), 1.0, orc._p(M))
+            D = np.ascontiguousarray(-np.array(cl, float))
+            for q in (6, 9):
+                Lc = np.zeros(2 * tsz(P))
+                L.orc_exact_m2l(q, orc._p(M), orc._p(D), orc._p(Lc))
+                for i in range(16):
+                    g = np.zeros(3)
+                    L.orc_exact_l2p(q, orc._p(Lc), orc._p(np.ascontiguousarray(tgt[i])), None, orc._p(g))
+                    e[q] += ((g - f[i]) ** 2).sum()
+        rate = np.sqrt(3.0) / np.linalg.norm(cl)
+        decay = (np.sqrt(e[9]) / np.sqrt(e[6])) ** (1.0 / 3.0)
+        assert decay < 0.55 * rate * 1.15, (cl, decay, rate)
+
+
+def test_exact_fmm_class_orders():
+    # tol 0 is the fixed-order FMM bit for bit; small tol stays within the
+    # model's quadrature sum of the ~150 classes each within tol of the face
+    # class's error; lower tol never gives a larger error here
+    xyz, m = workloads.uniform_cube(3000, seed=41)
+    d = orc.direct_targets(xyz, m, np.arange(3000), G=1.0)
+    a0 = orc.fmm_exact(xyz, m, 8, 16, G=1.0)
+    assert np.array_equal(orc.fmm_exact(xyz, m, 8, 16, G=1.0, class_order_tol=0.0), a0)
+    e0 = _rel(a0, d)
+    errs = [_rel(orc.fmm_exact(xyz, m, 8, 16, G=1.0, class_order_tol=t), d) for t in (0.02, 0.1, 0.5)]
+    assert errs[0] < 1.3 * e0
+    assert errs[0] <= errs[1] <= errs[2]
+    assert errs[1] < np.sqrt(1 + 150 * 0.1 ** 2) * 1.3 * e0
+
+
 def test_direct_naive_two_body():
     xyz = np.array([[-10.0, 0, 0], [10.0, 0, 0]])
     m = np.array([1e9, 1e9])
