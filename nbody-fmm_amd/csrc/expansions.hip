@@ -294,18 +294,23 @@ __global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, double hx, do
 }
 
 // ---- P2M ---------------------------------------------------------------------------
-// Item = (source slot, particle range).  Lane = particle; the orders m are
-// dealt round-robin to the 4 waves of the block, so every wave runs one
-// column m at a time with no divergence: q R_m^m by the sectoral recurrence,
-// then R_n^m, n = m+1..p, by the three-term recurrence in n, accumulated per
-// lane; one xor-shuffle reduction per coefficient at the end of the column.
-// T is the coefficient type (double, or float in mixed precision); the
-// offset from the cell centre is formed in FP64 and scaled before rounding.
+// Item = (source slot, particle range).  Lanes are 16 columns m (the orders;
+// m and m + 16 for p = 16) times 16 particle slots (4 per wave): a lane runs
+// its column for every 16th particle of the item -- q R_m^m by the sectoral
+// recurrence, then R_n^m, n = m+1..p, by the three-term recurrence in n --
+// accumulating per lane, so every column costs about p steps per particle
+// and no lane waits on another.  The 4 slots of a wave meet in two
+// xor-shuffles, the 4 waves in LDS, and each coefficient is added to the
+// multipole once per item.  T is the coefficient type (double, or float in
+// mixed precision); the offset from the cell centre is formed in FP64 and
+// scaled before rounding.
 template <class T>
 __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restrict__ items, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ cells, const int32_t *__restrict__ mslot,
                                             const uint64_t *__restrict__ ckey, LevelGeo g, const Consts *__restrict__ C,
                                             T *__restrict__ Mexp) {
+  constexpr int kCoef = (kMaxOrder + 1) * (kMaxOrder + 1);
+  __shared__ T part[kT / 64][kCoef]; // each wave's sums, in the multipole's storage order
   const int4 it = items[blockIdx.x];
   const int beg = it.y, end = it.z;
   const int32_t cell = cells[it.x];
@@ -315,72 +320,90 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
   cell_centre(ckey[cell], lev, g, ctr);
   const double ir = g.inv_rho[lev];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, ps = wave * 4 + (lane >> 4); // column, particle slot (16)
   T *M = Mexp + (size_t)slot * KP;
-  for (int m = wave; m <= p; m += kT / 64) {
+  for (int m0 = 0; m0 <= p; m0 += 16) {
+    const int m = m0 + col;
+    const int nj = p - m; // j = n - m in [0, nj]; no work when negative
     T ar[kMaxOrder + 1], ai[kMaxOrder + 1];
+    // the recurrence coefficients of the lane's column, in registers for all its particles
+    T fm[kMaxOrder + 1], fa[kMaxOrder + 1], fb[kMaxOrder + 1];
 #pragma unroll
     for (int j = 0; j <= kMaxOrder; ++j) {
       ar[j] = 0;
       ai[j] = 0;
+      fm[j] = j >= 1 && j <= m ? (T)C->mmf[j] : (T)0;
+      fa[j] = j >= 2 && j <= nj ? (T)C->ra[m + j][m] : (T)0;
+      fb[j] = j >= 2 && j <= nj ? (T)C->rb[m + j][m] : (T)0;
     }
-    const int nj = p - m; // j = n - m in [0, nj]
-    for (int i = beg + lane; i < end; i += 64) {
-      const double4 q = pos[i];
-      const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
-      const T r2 = x * x + y * y + z * z;
-      T vr = (T)q.w, vi = 0; // q * R_m^m
-      for (int k = 1; k <= m; ++k) {
-        const T f = (T)C->mmf[k];
-        const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
-        vr = nr;
-        vi = ni;
-      }
-      ar[0] += vr;
-      ai[0] -= vi;
-      T pr = vr, pi = vi;
-      T cr = 0, ci = 0;
-      if (nj >= 1) {
-        const T f = z * (T)C->s2m1[m];
-        cr = f * pr;
-        ci = f * pi;
-        ar[1] += cr;
-        ai[1] -= ci;
-      }
+    const T fs = nj >= 1 ? (T)C->s2m1[m] : (T)0;
+    if (nj >= 0)
+      for (int i = beg + ps; i < end; i += 16) {
+        const double4 q = pos[i];
+        const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
+        const T r2 = x * x + y * y + z * z;
+        T vr = (T)q.w, vi = 0; // q * R_m^m
 #pragma unroll
-      for (int j = 2; j <= kMaxOrder; ++j) {
-        if (j <= nj) {
-          const int n = m + j;
-          const T a = (T)C->ra[n][m] * z, b = (T)C->rb[n][m] * r2;
-          const T nr = a * cr - b * pr, ni = a * ci - b * pi;
-          pr = cr;
-          pi = ci;
-          cr = nr;
-          ci = ni;
-          ar[j] += cr;
-          ai[j] -= ci;
+        for (int k = 1; k <= kMaxOrder; ++k) {
+          if (k <= m) {
+            const T f = fm[k];
+            const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
+            vr = nr;
+            vi = ni;
+          }
+        }
+        ar[0] += vr;
+        ai[0] -= vi;
+        T pr = vr, pi = vi;
+        T cr = 0, ci = 0;
+        if (nj >= 1) {
+          const T f = z * fs;
+          cr = f * pr;
+          ci = f * pi;
+          ar[1] += cr;
+          ai[1] -= ci;
+        }
+#pragma unroll
+        for (int j = 2; j <= kMaxOrder; ++j) {
+          if (j <= nj) {
+            const T a = fa[j] * z, b = fb[j] * r2;
+            const T nr = a * cr - b * pr, ni = a * ci - b * pi;
+            pr = cr;
+            pi = ci;
+            cr = nr;
+            ci = ni;
+            ar[j] += cr;
+            ai[j] -= ci;
+          }
         }
       }
-    }
+    // the wave's 4 slots of a column: lanes col, col + 16, col + 32, col + 48
 #pragma unroll
     for (int j = 0; j <= kMaxOrder; ++j) {
-      if (j <= nj) {
+      if (j <= p - m0) { // uniform: every lane shuffles
         T vr = ar[j], vi = ai[j];
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-          vr += __shfl_xor(vr, o);
-          vi += __shfl_xor(vi, o);
-        }
-        if (lane == 0) {
+        vr += __shfl_xor(vr, 16);
+        vi += __shfl_xor(vi, 16);
+        vr += __shfl_xor(vr, 32);
+        vi += __shfl_xor(vi, 32);
+        if (lane < 16 && j <= nj) {
           const int n = m + j;
           if (m == 0) {
-            atomicAdd(&M[n * n], vr);
+            part[wave][n * n] = vr;
           } else {
-            atomicAdd(&M[n * n + 2 * m - 1], vr);
-            atomicAdd(&M[n * n + 2 * m], vi);
+            part[wave][n * n + 2 * m - 1] = vr;
+            part[wave][n * n + 2 * m] = vi;
           }
         }
       }
     }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (p + 1) * (p + 1); e += kT) {
+    T v = part[0][e];
+#pragma unroll
+    for (int w = 1; w < kT / 64; ++w) v += part[w][e];
+    atomicAdd(&M[e], v);
   }
 }
 
