@@ -215,8 +215,9 @@ def main():
                                               "ms_total")},
             "counts": {k: st[k] for k in ("depth", "n_cells", "n_leaves", "n_il", "n_m2l", "n_m2l_targets",
                                            "n_m2l_sources", "n_p2p_pairs")},
-            "modes": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "order", "p2p_cut", "leaf_capacity", "err",
-                                               "meets_gate", "tuning", "learned") if kk in v}
+            "modes": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "order", "p2p_cut", "leaf_capacity",
+                                               "class_order_tol", "err", "meets_gate", "tuning", "learned")
+                          if kk in v}
                       for k, v in results.items()},
             "mixed_over_plain": (results["mixed"]["value"] / results["fp64"]["value"]
                                  if "mixed" in results and "fp64" in results else None),
@@ -411,7 +412,7 @@ def learned_mode(ctx, n, dx, dm, da, tgt, ref, mixed, args):
     that meets the gate (or the most accurate)."""
     t0 = time.time()
     ctx.set_params(order=14, order_select=0, p2p_cut=mixed["p2p_cut"], leaf_capacity=mixed["leaf_capacity"],
-                   precision=1)
+                   precision=1, class_order_tol=mixed.get("class_order_tol", 0.0))
     ctx.order_model_init(seed=7)
     ctx.order_dataset_put()
     xt, mt = workloads.plummer(n, seed=11)
@@ -428,7 +429,8 @@ def learned_mode(ctx, n, dx, dm, da, tgt, ref, mixed, args):
         wall, m2l_ms, p2p_ms, st = timed(ctx, n, dx, dm, da, args.steps, args.warmup)
         err = tune.rel_errors(da.download(ctx.stream)[tgt], ref)
         runs.append({"value": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps, "order": pmax,
-                     "p2p_cut": mixed["p2p_cut"], "leaf_capacity": mixed["leaf_capacity"], "err": err,
+                     "p2p_cut": mixed["p2p_cut"], "leaf_capacity": mixed["leaf_capacity"],
+                     "class_order_tol": mixed.get("class_order_tol", 0.0), "err": err,
                      "meets_gate": err["l2"] <= args.gate, "m2l_kernel_ms": m2l_ms, "p2p_kernel_ms": p2p_ms,
                      "stats": st.as_dict(), "tuning": [],
                      "learned": {"max_order": pmax, "order_conf": 0.9, "mean_order": st.mean_order,
