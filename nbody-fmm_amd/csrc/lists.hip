@@ -335,10 +335,14 @@ __device__ __forceinline__ int32_t seg_len(int64_t ng, int64_t np) {
 }
 
 // Also the particle pairs each cell's tasks evaluate (a 27-neighbour row
-// contains the target cell itself, whose particles skip themselves).
+// contains the target cell itself, whose particles skip themselves).  The
+// counts of cells of at most narrow_max particles go to cnt[c], the others
+// to cnt[nc + 1 + c], so one scan over both halves numbers the narrow tasks
+// first and the wide ones after them.
 __global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
                              const int32_t *closed, const int32_t *cbeg, const int32_t *cend, const int32_t *cnt_sub,
-                             const int32_t *off_sub, const int64_t *sub_pairs, int32_t *cnt, int64_t *pairs) {
+                             const int32_t *off_sub, const int64_t *sub_pairs, int narrow_max, int32_t *cnt,
+                             int64_t *pairs) {
   const int32_t c = 1 + (int32_t)(blockIdx.x * kT + threadIdx.x);
   if (c >= nc) return;
   const SrcGroups g = src_groups(c, leaf_begin, cparent, nbr27, closed, cbeg, cend, cnt_sub, off_sub, sub_pairs);
@@ -352,19 +356,22 @@ __global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cpar
     k += (int32_t)((g.np[q] + sl - 1) / sl);
     np += (int64_t)nt * (g.np[q] - (g.list[q] < 0 ? 1 : 0));
   }
-  cnt[c] = k * ntg;
+  const bool wide = nt > narrow_max;
+  cnt[c] = wide ? 0 : k * ntg;
+  cnt[nc + 1 + c] = wide ? k * ntg : 0;
   pairs[c] = np;
 }
 
 __global__ void k_task_fill(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
                             const int32_t *closed, const int32_t *cbeg, const int32_t *cend, const int32_t *cnt_sub,
-                            const int32_t *off_sub, const int64_t *sub_pairs, const int32_t *toff, P2PTask *tasks) {
+                            const int32_t *off_sub, const int64_t *sub_pairs, int narrow_max, const int32_t *toff,
+                            P2PTask *tasks) {
   const int32_t c = 1 + (int32_t)(blockIdx.x * kT + threadIdx.x);
   if (c >= nc) return;
   const SrcGroups g = src_groups(c, leaf_begin, cparent, nbr27, closed, cbeg, cend, cnt_sub, off_sub, sub_pairs);
   const int32_t nt = cend[c] - cbeg[c];
   const int32_t ntg = (nt + kTaskTargets - 1) / kTaskTargets;
-  int32_t w = toff[c];
+  int32_t w = nt > narrow_max ? toff[nc + 1 + c] : toff[c];
   for (int32_t tg = 0; tg < ntg; ++tg)
     for (int q = 0; q < g.n; ++q) {
       if (g.np[q] <= 0) continue;
@@ -494,36 +501,40 @@ void build_lists(nbfmm_ctx *c) {
     NB_LAUNCH();
   }
   const int32_t *cpar = (const int32_t *)c->cparent.p;
-  int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)nc + 1);
-  int32_t *toff = dget<int32_t>(c->task_off, (size_t)nc + 1);
-  NB_HIP(hipMemsetAsync(tcnt, 0, sizeof(int32_t), s));
-  NB_HIP(hipMemsetAsync(tcnt + nc, 0, sizeof(int32_t), s));
+  // task counts of narrow cells, then of wide cells (2 (nc + 1) entries)
+  const int narrow_max = p2p_narrow_max(c->prm.precision == NBFMM_MIXED);
+  int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)2 * (nc + 1));
+  int32_t *toff = dget<int32_t>(c->task_off, (size_t)2 * (nc + 1));
+  NB_HIP(hipMemsetAsync(tcnt, 0, sizeof(int32_t) * 2 * (size_t)(nc + 1), s));
   int64_t *cell_pairs = dget<int64_t>(c->near_pairs, (size_t)nc + 1);
   NB_HIP(hipMemsetAsync(cell_pairs, 0, sizeof(int64_t), s));
   unsigned long long *pairs = (unsigned long long *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
   NB_HIP(hipMemsetAsync(pairs, 0, sizeof(unsigned long long), s));
   hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
                      (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
-                     (const int64_t *)sub_pairs, tcnt, cell_pairs);
+                     (const int64_t *)sub_pairs, narrow_max, tcnt, cell_pairs);
   NB_LAUNCH();
-  scan_excl_i32(tcnt, toff, nc + 1, c->scan_tmp, s);
+  scan_excl_i32(tcnt, toff, 2 * (nc + 1), c->scan_tmp, s);
   hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nc))), dim3(kT), 0, s,
                      (const int64_t *)cell_pairs, nc, pairs);
   NB_LAUNCH();
   NB_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)tcnt, nc, tot);
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)tcnt, 2 * (nc + 1), tot);
   NB_LAUNCH();
   unsigned long long ntask, npairs;
+  int32_t nnarrow;
   NB_HIP(hipMemcpyAsync(&ntask, tot, sizeof(ntask), hipMemcpyDeviceToHost, s));
   NB_HIP(hipMemcpyAsync(&npairs, pairs, sizeof(npairs), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&nnarrow, toff + nc + 1, sizeof(nnarrow), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   if (ntask > lim) throw Error(NBFMM_ERR_UNSUPPORTED, "P2P work list exceeds 2^31 tasks");
   P.n_tasks = (int64_t)ntask;
+  P.n_tasks_narrow = nnarrow;
   P.n_p2p_pairs = (int64_t)npairs;
   P2PTask *tasks = dget<P2PTask>(c->tasks, (size_t)ntask + 1);
   hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
                      (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
-                     (const int64_t *)sub_pairs, toff, tasks);
+                     (const int64_t *)sub_pairs, narrow_max, toff, tasks);
   NB_LAUNCH();
 }
 

@@ -45,6 +45,16 @@ constexpr int kNumClasses = 343; // offsets in [-3,3]^3, class = (dz+3)*49+(dy+3
 constexpr int kMaxOrder = 16;
 constexpr int kTaskTargets = 256; // targets of one P2P sub-task (one wavefront, 4 per lane)
 constexpr int kMaxList = 192;     // longest source-cell list of a P2P task (interaction list <= 189)
+// P2P blocks of at most this many targets go to the sources-across-lanes
+// kernel (two passes of its target group at most), larger ones to the
+// four-targets-per-lane kernel
+#ifndef NBFMM_P2P_NARROW_F32
+#define NBFMM_P2P_NARROW_F32 32
+#endif
+#ifndef NBFMM_P2P_NARROW_F64
+#define NBFMM_P2P_NARROW_F64 16
+#endif
+inline int p2p_narrow_max(bool f32) { return f32 ? NBFMM_P2P_NARROW_F32 : NBFMM_P2P_NARROW_F64; }
 // orders a learned selection may give a cell
 constexpr int kOrderLo = 4, kOrderHi = 14, kNumOrders = kOrderHi - kOrderLo + 1;
 // the order-selection MLP: kMlpIn -> kMlpHidden -> kMlpHidden -> kNumOrders,
@@ -129,6 +139,7 @@ struct Plan {
   int64_t n_il = 0, n_m2l = 0, n_sub = 0;
   int64_t n_src = 0, n_tgt = 0, n_p2p_pairs = 0, n_p2m_terms = 0, n_l2p_terms = 0;
   int64_t n_p2m_items = 0, n_m2l_tiles = 0, n_tasks = 0;
+  int64_t n_tasks_narrow = 0; // tasks [0, n_tasks_narrow) have blocks of at most p2p_narrow_max targets
   int kp = 0; // padded coefficient count (multiple of 16)
   double flop_m2l = 0; // algorithmic M2L flops of the evaluation
   double mean_order = 0; // mean order over the M2L pairs (the fixed order without selection)

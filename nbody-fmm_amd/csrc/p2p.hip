@@ -12,9 +12,11 @@
 // list's cell sizes into LDS and stages the task's targets there; each lane
 // then walks its own sources of the concatenation (its cell found by binary
 // search once, then moved forward), keeping one sum per target in
-// registers.  Tasks of one cell and of cells at different levels add into
-// the same targets, so the partial sums are added with FP64 atomics into the
-// near-field gradient.
+// registers.  Tasks of blocks with more than kNarrowMax targets (which this
+// kernel would walk in several passes) go to k_p2p_wide instead: four
+// targets per lane, one pass.  Tasks of one cell and of cells at different
+// levels add into the same targets, so the partial sums are added with FP64
+// atomics into the near-field gradient.
 // P2P runs on its own stream, concurrently with the expansion passes (vector
 // ALU work beside MFMA work); k_finish writes acc = G * (far + near) in input
 // order (particle.cpp:16-21).
@@ -256,6 +258,136 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *
   }
 }
 
+// The kernel for blocks of many targets (more than kNarrowMax: one pass over
+// the sources instead of several): lanes are tl target lanes (power of two,
+// 4 targets each: t = ti + tl q) times 64 / tl source slots; the sources are
+// staged 64 at a time in LDS (each lane finds its source's cell by binary
+// search) and read once per 4 pairs; a chunk's contributions are summed per
+// lane and added with the precision's add (Kahan in FP32), and the slots are
+// combined with xor-shuffles at the end.  One wavefront per task.
+template <class Tr, bool POT>
+__global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *__restrict__ tasks,
+                                                  const int32_t *__restrict__ nbr27,
+                                                  const int32_t *__restrict__ sub_src,
+                                                  const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                                                  const double4 *__restrict__ pos, double *__restrict__ grad,
+                                                  double *__restrict__ pot) {
+  typedef typename Tr::S S;
+  typedef typename Tr::A A;
+  __shared__ S sh[kW];
+  __shared__ int32_t lb[kMaxList];
+  __shared__ int32_t lp[kMaxList];
+  const int lane = threadIdx.x;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+  __syncthreads(); // the previous task's reads of lb, lp, sh are done
+  const P2PTask tk = tasks[task];
+  const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+  double4 anchor = make_double4(0, 0, 0, 0);
+  if (Tr::kAnchor) {
+    for (int t = lane; t < nc; t += kW) {
+      const double4 q = pos[b + t];
+      anchor.x += q.x;
+      anchor.y += q.y;
+      anchor.z += q.z;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      anchor.x += __shfl_xor(anchor.x, off);
+      anchor.y += __shfl_xor(anchor.y, off);
+      anchor.z += __shfl_xor(anchor.z, off);
+    }
+    anchor.x /= nc;
+    anchor.y /= nc;
+    anchor.z /= nc;
+  }
+  const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
+  const int ne = tk.nlist;
+  // first concatenated index of every list cell
+  int32_t carry = 0;
+  for (int base = 0; base < ne; base += kW) {
+    const int e = base + lane;
+    const int32_t q = e < ne ? list[e] : -1;
+    const int32_t sb = q >= 0 ? cbeg[q] : 0;
+    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
+    int32_t x = cnt;
+    for (int off = 1; off < kW; off <<= 1) {
+      const int32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (e < ne) {
+      lb[e] = sb;
+      lp[e] = carry + x - cnt;
+    }
+    carry += __shfl(x, kW - 1);
+  }
+  const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
+  int tl = 1;
+  while (tl * 4 < nc) tl <<= 1;
+  const int sub = kW / tl, ti = lane & (tl - 1), si = lane / tl;
+  S xi[4];
+  int32_t ii[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int32_t t = ti + tl * q;
+    ii[q] = t < nc ? b + t : -1;
+    xi[q] = Tr::make(pos[t < nc ? b + t : b], anchor);
+  }
+  A sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0}, sp[4] = {0, 0, 0, 0};
+  A cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
+  for (int32_t fb = f0; fb < f1; fb += kW) {
+    // stage sources [fb, fb + 64) of the concatenation
+    const int32_t g = fb + lane;
+    __syncthreads();
+    if (g < f1) {
+      int lo = 0, hi = ne; // last list cell with lp <= g (lp[0] = 0 <= g)
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lp[mid] <= g)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      sh[lane] = Tr::make(pos[lb[lo] + (g - lp[lo])], anchor);
+    }
+    __syncthreads();
+    const int cnt = min(kW, f1 - fb);
+    A ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0}, ap[4] = {0, 0, 0, 0};
+    for (int k = si; k < cnt; k += sub) {
+      const S xj = sh[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Tr::template pair<POT>(xi[q], xj, ax[q], ay[q], az[q], ap[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      Tr::add(sx[q], cx[q], ax[q]);
+      Tr::add(sy[q], cy[q], ay[q]);
+      Tr::add(sz[q], cz[q], az[q]);
+      if (POT) Tr::add(sp[q], cp[q], ap[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double gx = (double)sx[q] + (double)cx[q], gy = (double)sy[q] + (double)cy[q],
+           gz = (double)sz[q] + (double)cz[q];
+    for (int o = tl; o < kW; o <<= 1) {
+      gx += __shfl_xor(gx, o);
+      gy += __shfl_xor(gy, o);
+      gz += __shfl_xor(gz, o);
+    }
+    double gp = 0;
+    if (POT) {
+      gp = (double)sp[q] + (double)cp[q];
+      for (int o = tl; o < kW; o <<= 1) gp += __shfl_xor(gp, o);
+    }
+    if (si == 0 && ii[q] >= 0) {
+      atomicAdd(&grad[3 * (int64_t)ii[q]], gx);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 1], gy);
+      atomicAdd(&grad[3 * (int64_t)ii[q] + 2], gz);
+      if (POT) atomicAdd(&pot[ii[q]], gp);
+    }
+  }
+  }
+}
+
 // acc = G * grad; with potentials, phi = -G * sum_j m_j / r_ij
 __global__ void k_finish(int64_t n, const double *__restrict__ far, const double *__restrict__ near,
                          const double *__restrict__ pfar, const double *__restrict__ pnear,
@@ -388,8 +520,10 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   const bool pot = c->phi_out != nullptr;
   double *gn = dget<double>(c->grad_near, (size_t)P.n * (pot ? 4 : 3));
   double *pn = pot ? gn + 3 * (size_t)P.n : nullptr;
-  const int64_t split = (int64_t)((double)P.n_tasks * NBFMM_P2P_EARLY);
-  const int64_t t0 = part == 0 ? 0 : split, t1 = part == 0 ? split : P.n_tasks;
+  // the same share of the narrow tasks [0, nn) and of the wide ones [nn, n_tasks)
+  const int64_t nn = P.n_tasks_narrow, nw = P.n_tasks - nn;
+  const int64_t sn = (int64_t)((double)nn * NBFMM_P2P_EARLY), sw = nn + (int64_t)((double)nw * NBFMM_P2P_EARLY);
+  const int64_t r[2][2] = {{part == 0 ? 0 : sn, part == 0 ? sn : nn}, {part == 0 ? nn : sw, part == 0 ? sw : P.n_tasks}};
   if (part == 1) {
     // after the point of the main stream where part 1 is enqueued (the first M2L contraction)
     NB_HIP(hipEventRecord(c->ev_fork2, c->stream));
@@ -401,11 +535,15 @@ void launch_p2p(nbfmm_ctx *c, int part) {
     NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
   }
   mark2(c, part == 0 ? 9 : 11);
-  if (t1 > t0) {
+  const bool f32 = c->prm.precision == NBFMM_MIXED;
+  for (int w = 0; w < 2; ++w) {
+    const int64_t t0 = r[w][0], t1 = r[w][1];
+    if (t1 <= t0) continue;
     const int64_t grid = std::min<int64_t>(t1 - t0, ((int64_t)1 << 31) - 1);
-    const bool f32 = c->prm.precision == NBFMM_MIXED;
-    auto kern = f32 ? (pot ? k_p2p_tasks<PairF32, true> : k_p2p_tasks<PairF32, false>)
-                    : (pot ? k_p2p_tasks<PairF64, true> : k_p2p_tasks<PairF64, false>);
+    auto kern = w == 0 ? (f32 ? (pot ? k_p2p_tasks<PairF32, true> : k_p2p_tasks<PairF32, false>)
+                              : (pot ? k_p2p_tasks<PairF64, true> : k_p2p_tasks<PairF64, false>))
+                       : (f32 ? (pot ? k_p2p_wide<PairF32, true> : k_p2p_wide<PairF32, false>)
+                              : (pot ? k_p2p_wide<PairF64, true> : k_p2p_wide<PairF64, false>));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, t1 - t0, (const P2PTask *)c->tasks.p + t0,
                        (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
                        (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn, pn);
