@@ -187,11 +187,11 @@ def main():
     unit_prec = "fp64" if head_name == "fp64" else "fp32"
     if head["m2l_kernel_ms"] >= head["p2p_kernel_ms"]:
         ach = st["flop_m2l"] / (head["m2l_kernel_ms"] * 1e-3) / 1e12
-        roof = {"kernel": "k_m2l_cls (MFMA M2L, class-major)", "bound": "mfma", "avg_ms": head["m2l_kernel_ms"],
+        roof = {"kernel": "k_m2l_cls (MFMA M2L, class-major; k_m2l_bf3 in mixed precision)", "bound": "mfma", "avg_ms": head["m2l_kernel_ms"],
                 "flop_per_launch": st["flop_m2l"]}
     else:
         ach = st["flop_p2p"] / (head["p2p_kernel_ms"] * 1e-3) / 1e12
-        roof = {"kernel": "k_p2p_tasks (P2P on the vector ALUs)", "bound": "mfma", "avg_ms": head["p2p_kernel_ms"],
+        roof = {"kernel": "k_p2p_tasks + k_p2p_wide (P2P on the vector ALUs)", "bound": "mfma", "avg_ms": head["p2p_kernel_ms"],
                 "flop_per_launch": st["flop_p2p"]}
     peak = PEAK[unit_prec]
     roof.update({"achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
