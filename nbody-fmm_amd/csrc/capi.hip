@@ -213,7 +213,7 @@ void mark2(nbfmm_ctx *c, int slot) {
 nbfmm_ctx::~nbfmm_ctx() {
   if (stream2) (void)hipStreamSynchronize(stream2);
   if (stream) (void)hipStreamSynchronize(stream);
-  DBuf *all[] = {&xyz_in, &m_in, &acc_out, &key[0], &key[1], &idx[0], &idx[1], &sort_hist, &scan_tmp, &pos,
+  DBuf *all[] = {&xyz_in, &m_in, &acc_out, &key[0], &key[1], &idx[0], &idx[1], &sort_hist, &scan_tmp, &act_par, &pos,
                  &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &p2m_items, &tasks, &task_off, &task_cnt, &dpart,

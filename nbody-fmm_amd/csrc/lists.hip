@@ -112,17 +112,20 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, in
                                               int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
                                               const int32_t *off_il, const int32_t *off_m2l, const int32_t *off_sub,
                                               int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls, int32_t *sub_src,
-                                              int32_t *is_src) {
+                                              int32_t *is_src, int32_t *active, int32_t nactive) {
   __shared__ int32_t cand_s[kT / kW][kWin];
   __shared__ int32_t csz_s[kT / kW][kWin];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int32_t P = pbeg + (int32_t)blockIdx.x * (kT / kW) + wave;
-  const bool active = P < nparents;
+  // the count pass takes every parent from pbeg and lists those with a
+  // child to classify in active[]; the fill pass takes that list
+  const int32_t pw = (int32_t)blockIdx.x * (kT / kW) + wave;
+  const int32_t P = FILL ? (pw < nactive ? active[pw] : nparents) : pbeg + pw;
+  const bool live = P < nparents;
   int32_t *cand = cand_s[wave], *csz = csz_s[wave];
   int64_t maxn = 0;
   // children to classify (bit o): all when skipping is off
   uint32_t todo = 0;
-  if (active) {
+  if (live) {
     if (skip_from >= 0) {
       const int32_t q = lane < 27 ? nbr27[(int64_t)P * 27 + lane] : -1;
       maxn = q >= 0 ? cend[q] - cbeg[q] : 0;
@@ -161,6 +164,17 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, in
     }
   }
   __syncthreads();
+  if (!FILL) { // one counter update per block for its waves with a child to classify
+    __shared__ int32_t abase;
+    __shared__ uint32_t amask;
+    if (threadIdx.x == 0) amask = 0;
+    __syncthreads();
+    if (lane == 0 && todo) atomicOr(&amask, 1u << wave);
+    __syncthreads();
+    if (threadIdx.x == 0 && amask) abase = atomicAdd(&active[nparents + 1], __popc(amask));
+    __syncthreads();
+    if (lane == 0 && todo) active[abase + __popc(amask & ((1u << wave) - 1u))] = P; // counter after the list
+  }
   if (!todo) return;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int o = 0; o < 8; ++o) {
@@ -426,9 +440,12 @@ void build_lists(nbfmm_ctx *c) {
   const int32_t pbeg = P.lvl_start[std::min(lc - 1, D)]; // parents of the cells of level lc on
   const ClsMask bad = class_mask(P.geo);
   const unsigned nlb = (unsigned)std::max<int32_t>(1, (nparents - pbeg + kT / kW - 1) / (kT / kW));
+  // parents with a child to classify, then their count at [nparents + 1]
+  int32_t *active = dget<int32_t>(c->act_par, (size_t)nparents + 2);
+  NB_HIP(hipMemsetAsync(active + nparents + 1, 0, sizeof(int32_t), s));
   hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
                      cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, nullptr);
+                     nullptr, nullptr, active, 0);
   NB_LAUNCH();
   scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
@@ -444,7 +461,9 @@ void build_lists(nbfmm_ctx *c) {
   hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_sub, nc, tot + 2);
   NB_LAUNCH();
   unsigned long long htot[3];
+  int32_t nactive = 0;
   NB_HIP(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(&nactive, active + nparents + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   const unsigned long long lim = 0x7fffffffull - 1;
   if (htot[1] > lim || htot[2] > lim || (c->keep_lists && htot[0] > lim))
@@ -458,10 +477,13 @@ void build_lists(nbfmm_ctx *c) {
   int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
   int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
   NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  hipLaunchKernelGGL(k_lists<true>, dim3(nlb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
-                     nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il, m2l_src,
-                     m2l_cls, sub_src, is_src);
-  NB_LAUNCH();
+  if (nactive > 0) {
+    const unsigned nfb = (unsigned)((nactive + kT / kW - 1) / (kT / kW));
+    hipLaunchKernelGGL(k_lists<true>, dim3(nfb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend,
+                       cut, bad, nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il,
+                       m2l_src, m2l_cls, sub_src, is_src, active, nactive);
+    NB_LAUNCH();
+  }
 
   // ---- M2L source / target compaction ----
   const size_t fl = (size_t)std::max<int64_t>(nc, P.n) + 1;

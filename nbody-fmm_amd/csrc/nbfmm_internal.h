@@ -163,7 +163,7 @@ struct nbfmm_ctx {
   nbfmm_stats stats{};
   // particle arrays
   nbfmm::DBuf xyz_in, m_in, acc_out;           // staging for host-pointer calls
-  nbfmm::DBuf key[2], idx[2], sort_hist, scan_tmp;
+  nbfmm::DBuf key[2], idx[2], sort_hist, scan_tmp, act_par;
   nbfmm::DBuf pos;                             // double4 sorted (x,y,z,m)
   nbfmm::DBuf pcell;                           // leaf cell of each sorted particle
   nbfmm::DBuf grad;                            // 3 doubles per sorted particle (far field)
