@@ -594,6 +594,8 @@ int nbfmm_tree_sizes(nbfmm_ctx *c, int32_t *depth, int64_t *n_cells, int64_t *n_
   if (depth) *depth = P.geo.depth;
   if (n_cells) *n_cells = P.n_cells;
   if (n_nbr) {
+    nbr_leaf_rows(c, nullptr); // every leaf's row
+    NB_HIP(hipStreamSynchronize(c->stream));
     std::vector<int32_t> nb((size_t)P.n_cells * 27);
     NB_HIP(hipMemcpy(nb.data(), c->nbr27.p, nb.size() * 4, hipMemcpyDeviceToHost));
     int64_t k = 0;
@@ -621,6 +623,8 @@ int nbfmm_tree_get(nbfmm_ctx *c, int64_t *level_start, uint64_t *key, int32_t *p
   if (parent) NB_HIP(hipMemcpy(parent, c->cparent.p, nc * 4, hipMemcpyDeviceToHost));
   if (perm) NB_HIP(hipMemcpy(perm, c->idx[0].p, (size_t)P.n * 4, hipMemcpyDeviceToHost));
   if (nbr_off || nbr) {
+    nbr_leaf_rows(c, nullptr); // every leaf's row
+    NB_HIP(hipStreamSynchronize(c->stream));
     std::vector<int32_t> nb(nc * 27);
     NB_HIP(hipMemcpy(nb.data(), c->nbr27.p, nb.size() * 4, hipMemcpyDeviceToHost));
     int32_t k = 0;

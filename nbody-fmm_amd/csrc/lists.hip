@@ -522,6 +522,7 @@ void build_lists(nbfmm_ctx *c) {
                        child8, closed);
     NB_LAUNCH();
   }
+  nbr_leaf_rows(c, closed); // the P2P work list reads the rows of the leaves that are not closed
   const int32_t *cpar = (const int32_t *)c->cparent.p;
   // task counts of narrow cells, then of wide cells (2 (nc + 1) entries)
   const int narrow_max = p2p_narrow_max(c->prm.precision == NBFMM_MIXED);

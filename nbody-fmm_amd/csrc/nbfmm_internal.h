@@ -158,6 +158,7 @@ struct nbfmm_ctx {
   int n_cu = 256;      // compute units of this device
   bool timing = true;
   bool keep_lists = false;
+  int nbr_leaf = 0; // rows of the leaf level in nbr27: 0 none, 1 leaves that are not closed, 2 all
   double *phi_out = nullptr; // potential output of the current evaluation (device, input order) or null
   nbfmm::Plan plan;
   nbfmm_stats stats{};
@@ -213,6 +214,7 @@ void scan_excl_f64(const double *in, double *out, int64_t n, DBuf &tmp, hipStrea
 void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m);
 // lists.hip
 void build_lists(nbfmm_ctx *c);
+void nbr_leaf_rows(nbfmm_ctx *c, const int32_t *closed); // leaf-level rows of nbr27 (all when closed is null)
 // expansions.hip
 void run_expansions(nbfmm_ctx *c);
 int padded_k(int p); // pad16((p+1)^2)

@@ -642,6 +642,7 @@ void need_model(nbfmm_ctx *c) {
 
 // features of every cell with a local (by local slot) into c->feat
 void order_features(nbfmm_ctx *c, const LevelGeo &g) {
+  nbr_leaf_rows(c, nullptr); // features read the neighbour rows of every evaluation cell
   Plan &P = c->plan;
   hipStream_t s = c->stream;
   const int64_t n = P.n, nc = P.n_cells;

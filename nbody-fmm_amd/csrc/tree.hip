@@ -286,12 +286,14 @@ __global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__res
 }
 
 // 27-neighbourhood (offsets x fastest, self at 13) from the parent's table;
-// one thread per table entry.
+// one thread per table entry; with `skip`, rows of cells with skip[c] != 0
+// are left unwritten.
 __global__ void k_nbr27(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent,
-                        const int32_t *__restrict__ child8, int32_t cb, int32_t ce, int lev, int32_t *nbr27) {
+                        const int32_t *__restrict__ child8, int32_t cb, int32_t ce, int lev,
+                        const int32_t *__restrict__ skip, int32_t *nbr27) {
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
   const int32_t c = cb + (int32_t)(e / 27);
-  if (c >= ce) return;
+  if (c >= ce || (skip && skip[c])) return;
   const int o = (int)(e % 27);
   const uint64_t k = ckey[c];
   const int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
@@ -476,12 +478,29 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   }
   NB_HIP(hipMemsetAsync(nbr27, 0xff, sizeof(int32_t) * 27, s));
   NB_HIP(hipMemsetAsync(nbr27 + 13, 0, sizeof(int32_t), s));
-  for (int d = 1; d <= D; ++d) {
+  // the leaf level's rows follow in build_lists, for the leaves that are not
+  // closed (the only ones the evaluation reads); nbr_leaf_rows completes them
+  for (int d = 1; d < D; ++d) {
     int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
     hipLaunchKernelGGL(k_nbr27, dim3(nblk((int64_t)(ce - cb) * 27)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d,
-                       nbr27);
+                       nullptr, nbr27);
     NB_LAUNCH();
   }
+  c->nbr_leaf = D == 0 ? 2 : 0;
+}
+
+// rows of the leaf level: of the leaves with closed[c] == 0 (closed != null),
+// or of every leaf
+void nbr_leaf_rows(nbfmm_ctx *c, const int32_t *closed) {
+  Plan &P = c->plan;
+  const int D = P.geo.depth;
+  if (c->nbr_leaf == 2 || D == 0 || (closed && c->nbr_leaf == 1)) return;
+  const int32_t cb = P.lvl_start[D], ce = P.lvl_start[D + 1];
+  hipLaunchKernelGGL(k_nbr27, dim3(nblk((int64_t)(ce - cb) * 27)), dim3(kT), 0, c->stream, (const uint64_t *)c->ckey.p,
+                     (const int32_t *)c->cparent.p, (const int32_t *)c->child8.p, cb, ce, D, closed,
+                     (int32_t *)c->nbr27.p);
+  NB_LAUNCH();
+  c->nbr_leaf = closed ? 1 : 2;
 }
 
 } // namespace nbfmm
