@@ -103,6 +103,42 @@ constexpr int64_t kTaskPairs = 1 << 17;
 // does not keep the lists and c may be merged into a closed-cell task (cell
 // index >= skip_from), its lists are never read: they are skipped and its
 // counts stay 0.  skip_from < 0 classifies every cell.
+// One thread per parent: does it have a child to classify (the skip rule of
+// k_lists)?  Those parents are appended to active[] (order free; one counter
+// update per wavefront, the count at active[nparents + 1]).
+__global__ void __launch_bounds__(kT) k_list_filter(int32_t pbeg, int32_t nparents, int32_t skip_from,
+                                                    const int32_t *__restrict__ nbr27,
+                                                    const int32_t *__restrict__ child8,
+                                                    const int32_t *__restrict__ cbeg,
+                                                    const int32_t *__restrict__ cend, int64_t cut, int32_t *active) {
+  const int32_t P = pbeg + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  bool todo = false;
+  if (P < nparents) {
+    int64_t maxn = 0;
+    if (skip_from >= 0) {
+      int32_t q[27];
+#pragma unroll
+      for (int k = 0; k < 27; ++k) q[k] = nbr27[(int64_t)P * 27 + k];
+#pragma unroll
+      for (int k = 0; k < 27; ++k)
+        if (q[k] >= 0) maxn = max(maxn, (int64_t)(cend[q[k]] - cbeg[q[k]]));
+    }
+    int32_t ch[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) ch[o] = child8[(int64_t)P * 8 + o];
+#pragma unroll
+    for (int o = 0; o < 8; ++o)
+      todo |= ch[o] >= 0 &&
+              !(skip_from >= 0 && ch[o] >= skip_from && (int64_t)(cend[ch[o]] - cbeg[ch[o]]) * maxn < cut);
+  }
+  const uint64_t m = __ballot(todo);
+  const int lane = threadIdx.x & 63;
+  int32_t base = 0;
+  if (lane == 0 && m) base = atomicAdd(&active[nparents + 1], (int32_t)__popcll(m));
+  base = __shfl(base, 0);
+  if (todo) active[base + __popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = P;
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, int32_t skip_from,
                                               const int32_t *__restrict__ nbr27,
@@ -112,14 +148,13 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, in
                                               int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
                                               const int32_t *off_il, const int32_t *off_m2l, const int32_t *off_sub,
                                               int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls, int32_t *sub_src,
-                                              int32_t *is_src, int32_t *active, int32_t nactive) {
+                                              int32_t *is_src, const int32_t *active) {
   __shared__ int32_t cand_s[kT / kW][kWin];
   __shared__ int32_t csz_s[kT / kW][kWin];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // the count pass takes every parent from pbeg and lists those with a
-  // child to classify in active[]; the fill pass takes that list
+  // both passes take the parents k_list_filter listed (their count at active[nparents + 1])
   const int32_t pw = (int32_t)blockIdx.x * (kT / kW) + wave;
-  const int32_t P = FILL ? (pw < nactive ? active[pw] : nparents) : pbeg + pw;
+  const int32_t P = pw < active[nparents + 1] ? active[pw] : nparents;
   const bool live = P < nparents;
   int32_t *cand = cand_s[wave], *csz = csz_s[wave];
   int64_t maxn = 0;
@@ -164,17 +199,6 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, in
     }
   }
   __syncthreads();
-  if (!FILL) { // one counter update per block for its waves with a child to classify
-    __shared__ int32_t abase;
-    __shared__ uint32_t amask;
-    if (threadIdx.x == 0) amask = 0;
-    __syncthreads();
-    if (lane == 0 && todo) atomicOr(&amask, 1u << wave);
-    __syncthreads();
-    if (threadIdx.x == 0 && amask) abase = atomicAdd(&active[nparents + 1], __popc(amask));
-    __syncthreads();
-    if (lane == 0 && todo) active[abase + __popc(amask & ((1u << wave) - 1u))] = P; // counter after the list
-  }
   if (!todo) return;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int o = 0; o < 8; ++o) {
@@ -443,9 +467,13 @@ void build_lists(nbfmm_ctx *c) {
   // parents with a child to classify, then their count at [nparents + 1]
   int32_t *active = dget<int32_t>(c->act_par, (size_t)nparents + 2);
   NB_HIP(hipMemsetAsync(active + nparents + 1, 0, sizeof(int32_t), s));
-  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend, cut, bad,
-                     cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, nullptr, active, 0);
+  hipLaunchKernelGGL(k_list_filter, dim3(nblk(std::max<int32_t>(0, nparents - pbeg))), dim3(kT), 0, s, pbeg, nparents,
+                     skip_from, nbr27, child8, cbeg, cend, cut, active);
+  NB_LAUNCH();
+  // grid for every parent; wavefronts past the listed count exit
+  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend,
+                     cut, bad, cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, (const int32_t *)active);
   NB_LAUNCH();
   scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
   scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
@@ -481,7 +509,7 @@ void build_lists(nbfmm_ctx *c) {
     const unsigned nfb = (unsigned)((nactive + kT / kW - 1) / (kT / kW));
     hipLaunchKernelGGL(k_lists<true>, dim3(nfb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend,
                        cut, bad, nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il,
-                       m2l_src, m2l_cls, sub_src, is_src, active, nactive);
+                       m2l_src, m2l_cls, sub_src, is_src, (const int32_t *)active);
     NB_LAUNCH();
   }
 
