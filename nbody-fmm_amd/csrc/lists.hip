@@ -318,11 +318,13 @@ __global__ void k_closed(int32_t cb, int32_t ce, int32_t leaf_begin, const int32
 }
 
 __device__ __forceinline__ int64_t row_particles(const int32_t *row, const int32_t *cbeg, const int32_t *cend) {
+  int32_t q[27]; // the row's loads first, then the counts' (two rounds of gathers)
+#pragma unroll
+  for (int o = 0; o < 27; ++o) q[o] = row[o];
   int64_t t = 0;
-  for (int o = 0; o < 27; ++o) {
-    const int32_t q = row[o];
-    if (q >= 0) t += cend[q] - cbeg[q];
-  }
+#pragma unroll
+  for (int o = 0; o < 27; ++o)
+    if (q[o] >= 0) t += cend[q[o]] - cbeg[q[o]];
   return t;
 }
 

@@ -227,15 +227,19 @@ __global__ void k_level_counts(const uint64_t *__restrict__ key, int64_t n, int 
   if (threadIdx.x <= (unsigned)D) c[threadIdx.x] = 0;
   __syncthreads();
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  int d0 = D + 1; // the coarsest level at which particle i starts a new cell
   if (i < n) {
-    int d0;
     if (i == 0) {
       d0 = 0;
     } else {
       uint64_t x = key[i] ^ key[i - 1];
       d0 = x == 0 ? D + 1 : D - (63 - __clzll((long long)x)) / 3;
     }
-    for (int d = d0; d <= D; ++d) atomicAdd(&c[d], 1u);
+  }
+  // one count per wavefront and level
+  for (int d = 0; d <= D; ++d) {
+    const uint64_t m = __ballot(d0 <= d);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c[d], (uint32_t)__popcll(m));
   }
   __syncthreads();
   if (threadIdx.x <= (unsigned)D && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
