@@ -545,35 +545,46 @@ __global__ void k_terms(const int32_t *cells, int64_t ncell, const int32_t *cbeg
 // Modes: 0 none, 1 P2M / L2P, 2 M2M / L2L.
 constexpr int32_t kTransCut = 256;
 
-__global__ void k_updown_mark(int32_t cb, int32_t ce, int32_t leaf_begin, const int32_t *__restrict__ cparent,
-                              const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
-                              const int32_t *__restrict__ is_src, const int32_t *__restrict__ cnt_m2l,
-                              int32_t *mode_m, int32_t *mode_l) {
-  const int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
-  const bool big = c < leaf_begin && cend[c] - cbeg[c] > kTransCut;
-  const int32_t P = cparent[c];
-  const bool mneed = is_src[c] || (P >= 0 && mode_m[P] == 2);
-  const bool lhas = cnt_m2l[c] > 0 || (P >= 0 && mode_l[P] == 2);
-  mode_m[c] = mneed ? (big ? 2 : 1) : 0;
-  mode_l[c] = lhas ? (big ? 2 : 1) : 0;
-}
-
-// five flag arrays of length n + 1 (flag n = 0): M slot, P2M, M2M, L slot, L2L
-__global__ void k_updown_flags(const int32_t *__restrict__ mode_m, const int32_t *__restrict__ mode_l, int64_t n,
-                               int32_t *flags) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i > n) return;
-  const int mm = i < n ? mode_m[i] : 0, ml = i < n ? mode_l[i] : 0;
+// Modes of every cell in one pass.  A cell needs its multipole when it is an
+// M2L source or its parent forms its multipole by M2M (mode 2), and mode 2
+// needs a cell that needs its multipole and is big.  Parents are never leaves
+// and hold their children's particles, so the ancestors of a big parent are
+// all big: a cell needs its multipole iff it is a source or, with a big
+// parent, some ancestor is.  Locals alike, with "has an M2L list".  Also the
+// five flag arrays of length n + 1 (flag n = 0): M slot, P2M, M2M, L slot, L2L.
+__global__ void __launch_bounds__(kT) k_updown_modes(int64_t n, int32_t leaf_begin, const int32_t *__restrict__ cparent,
+                                                     const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                                                     const int32_t *__restrict__ is_src,
+                                                     const int32_t *__restrict__ cnt_m2l, int32_t *mode_m,
+                                                     int32_t *mode_l, int32_t *flags) {
   const int64_t s = n + 1;
-  flags[i] = mm > 0;
-  flags[s + i] = mm == 1;
-  flags[2 * s + i] = mm == 2;
-  flags[3 * s + i] = ml > 0;
-  flags[4 * s + i] = ml == 2;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i <= n; i += (int64_t)gridDim.x * kT) {
+    int mm = 0, ml = 0;
+    if (i < n) {
+      const bool big = i < leaf_begin && cend[i] - cbeg[i] > kTransCut;
+      bool m = is_src[i] != 0, l = cnt_m2l[i] > 0;
+      int32_t a = cparent[i];
+      if (a >= 0 && cend[a] - cbeg[a] > kTransCut) {
+        for (; a >= 0 && !(m && l); a = cparent[a]) {
+          m = m || is_src[a] != 0;
+          l = l || cnt_m2l[a] > 0;
+        }
+      }
+      mm = m ? (big ? 2 : 1) : 0;
+      ml = l ? (big ? 2 : 1) : 0;
+      mode_m[i] = mm;
+      mode_l[i] = ml;
+    }
+    flags[i] = mm > 0;
+    flags[s + i] = mm == 1;
+    flags[2 * s + i] = mm == 2;
+    flags[3 * s + i] = ml > 0;
+    flags[4 * s + i] = ml == 2;
+  }
 }
 
-// slots (M, L, evaluated L) and the P2M, M2M and L2L cell lists
+// slots (M, L, evaluated L) and the P2M, M2M and L2L cell lists; `scan` is one
+// exclusive scan over the five flag arrays, so segment k starts at scan[k s]
 __global__ void k_updown_lists(const int32_t *__restrict__ mode_l, const int32_t *__restrict__ flags,
                                const int32_t *__restrict__ scan, int64_t n, int32_t *mslot, int32_t *lslot,
                                int32_t *eslot, int32_t *p2m_cells, int32_t *m2m_cells, int32_t *l2l_cells) {
@@ -581,12 +592,22 @@ __global__ void k_updown_lists(const int32_t *__restrict__ mode_l, const int32_t
   if (i >= n) return;
   const int64_t s = n + 1;
   mslot[i] = flags[i] ? scan[i] : -1;
-  const int32_t ls = flags[3 * s + i] ? scan[3 * s + i] : -1;
+  const int32_t ls = flags[3 * s + i] ? scan[3 * s + i] - scan[3 * s] : -1;
   lslot[i] = ls;
   eslot[i] = mode_l[i] == 1 ? ls : -1;
-  if (flags[s + i]) p2m_cells[scan[s + i]] = (int32_t)i;
-  if (flags[2 * s + i]) m2m_cells[scan[2 * s + i]] = (int32_t)i;
-  if (flags[4 * s + i]) l2l_cells[scan[4 * s + i]] = (int32_t)i;
+  if (flags[s + i]) p2m_cells[scan[s + i] - scan[s]] = (int32_t)i;
+  if (flags[2 * s + i]) m2m_cells[scan[2 * s + i] - scan[2 * s]] = (int32_t)i;
+  if (flags[4 * s + i]) l2l_cells[scan[4 * s + i] - scan[4 * s]] = (int32_t)i;
+}
+
+// The planning's host reads in one transfer: dst[k] = src[off[k]] - src[base[k]], k < cnt.
+struct ReadList {
+  int64_t off[64];
+  int64_t base[64];
+};
+__global__ void k_gather_reads(const int32_t *__restrict__ src, ReadList r, int cnt, int32_t *__restrict__ dst) {
+  const int k = threadIdx.x;
+  if (k < cnt) dst[k] = src[r.off[k]] - src[r.base[k]];
 }
 
 // particles of the cells with mode == v
@@ -676,20 +697,15 @@ void run_expansions(nbfmm_ctx *c) {
   // ---- plan: modes per level (top-down), slots, lists ----
   int32_t *mode_m = dget<int32_t>(c->mode_m, (size_t)nc + 1);
   int32_t *mode_l = dget<int32_t>(c->mode_l, (size_t)nc + 1);
-  for (int d = 0; d <= D; ++d) {
-    const int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_updown_mark, dim3(nblk(ce - cb)), dim3(kT), 0, s, cb, ce, leaf_begin,
-                       (const int32_t *)c->cparent.p, cbeg, cend, (const int32_t *)c->is_src.p,
-                       (const int32_t *)c->cnt_m2l.p, mode_m, mode_l);
-    NB_LAUNCH();
-  }
   const int64_t s5 = nc + 1;
   int32_t *flags = dget<int32_t>(c->flags, (size_t)5 * s5);
   int32_t *fscan = dget<int32_t>(c->fscan, (size_t)5 * s5);
-  hipLaunchKernelGGL(k_updown_flags, dim3(nblk(nc + 1)), dim3(kT), 0, s, (const int32_t *)mode_m,
-                     (const int32_t *)mode_l, nc, flags);
+  // few workgroups: this runs beside the first share of P2P
+  hipLaunchKernelGGL(k_updown_modes, dim3(std::min<unsigned>(512, nblk(nc + 1))), dim3(kT), 0, s, nc, leaf_begin,
+                     (const int32_t *)c->cparent.p, cbeg, cend, (const int32_t *)c->is_src.p,
+                     (const int32_t *)c->cnt_m2l.p, mode_m, mode_l, flags);
   NB_LAUNCH();
-  for (int k = 0; k < 5; ++k) scan_excl_i32(flags + k * s5, fscan + k * s5, s5, c->scan_tmp, s);
+  scan_excl_i32(flags, fscan, 5 * s5, c->scan_tmp, s);
   int32_t *mslot = dget<int32_t>(c->mslot, (size_t)nc + 1);
   int32_t *lslot = dget<int32_t>(c->lslot, (size_t)nc + 1);
   int32_t *eslot = dget<int32_t>(c->eslot, (size_t)nc + 1);
@@ -698,13 +714,25 @@ void run_expansions(nbfmm_ctx *c) {
   hipLaunchKernelGGL(k_updown_lists, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)mode_l, (const int32_t *)flags,
                      (const int32_t *)fscan, nc, mslot, lslot, eslot, p2m_cells, m2m_cells, l2l_cells);
   NB_LAUNCH();
-  // totals and the per-level starts of the M2M and L2L lists (cells are level ordered)
+  // totals and the per-level starts of the M2M and L2L lists (cells are level
+  // ordered), gathered on the device and read through the pinned mirror
   int32_t *h = (int32_t *)c->hred.p;
-  for (int k = 0; k < 5; ++k)
-    NB_HIP(hipMemcpyAsync(h + k, fscan + k * s5 + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  for (int d = 0; d <= D + 1; ++d) {
-    NB_HIP(hipMemcpyAsync(h + 8 + d, fscan + 2 * s5 + P.lvl_start[d], sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NB_HIP(hipMemcpyAsync(h + 32 + d, fscan + 4 * s5 + P.lvl_start[d], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  {
+    ReadList rl{};
+    for (int k = 0; k < 5; ++k) {
+      rl.off[k] = k * s5 + nc;
+      rl.base[k] = k * s5;
+    }
+    for (int d = 0; d <= D + 1; ++d) {
+      rl.off[8 + d] = 2 * s5 + P.lvl_start[d];
+      rl.base[8 + d] = 2 * s5;
+      rl.off[32 + d] = 4 * s5 + P.lvl_start[d];
+      rl.base[32 + d] = 4 * s5;
+    }
+    int32_t *dreads = (int32_t *)((double *)c->red.p + 7 * 1024);
+    hipLaunchKernelGGL(k_gather_reads, dim3(1), dim3(64), 0, s, (const int32_t *)fscan, rl, 34 + D, dreads);
+    NB_LAUNCH();
+    NB_HIP(hipMemcpyAsync(h, dreads, sizeof(int32_t) * 64, hipMemcpyDeviceToHost, s));
   }
   unsigned long long *terms = (unsigned long long *)((double *)c->red.p + 6 * 1024 + 48);
   NB_HIP(hipMemsetAsync(terms, 0, 2 * sizeof(unsigned long long), s));
@@ -744,8 +772,8 @@ void run_expansions(nbfmm_ctx *c) {
     bm[d] = h[8 + d];
     bl[d] = h[32 + d];
   }
-  unsigned long long th[2];
-  NB_HIP(hipMemcpyAsync(th, terms, sizeof(th), hipMemcpyDeviceToHost, s));
+  unsigned long long *th = (unsigned long long *)(h + 64); // pinned
+  NB_HIP(hipMemcpyAsync(th, terms, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   // ---- P2M (cells that form their multipole from particles) ----
   int32_t *pcnt = dget<int32_t>(c->p2m_off, (size_t)nP2M * 2 + 2 + 8);
   int32_t *poff = pcnt + nP2M + 1;
@@ -754,9 +782,9 @@ void run_expansions(nbfmm_ctx *c) {
   NB_LAUNCH();
   NB_HIP(hipMemsetAsync(pcnt + nP2M, 0, sizeof(int32_t), s));
   scan_excl_i32(pcnt, poff, nP2M + 1, c->scan_tmp, s);
-  int32_t nitems;
-  NB_HIP(hipMemcpyAsync(&nitems, poff + nP2M, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(h + 72, poff + nP2M, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
+  const int32_t nitems = h[72];
   P.n_p2m_items = nitems;
   P.n_p2m_terms = (int64_t)th[0];
   P.n_l2p_terms = (int64_t)th[1];

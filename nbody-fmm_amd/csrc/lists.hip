@@ -254,18 +254,37 @@ __global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, in
   }
 }
 
-__global__ void k_flag_pos(const int32_t *v, int64_t n, int32_t *flags) {
+// Two compactions in one scan: cells with a[i] > 0 (flags [0, n)) and cells
+// with b[i] > 0 (flags [n, 2n)).
+__global__ void k_flag_pos2(const int32_t *a, const int32_t *b, int64_t n, int32_t *flags) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < n) flags[i] = v[i] > 0 ? 1 : 0;
+  if (i < n) {
+    flags[i] = a[i] > 0 ? 1 : 0;
+    flags[n + i] = b[i] > 0 ? 1 : 0;
+  }
 }
-__global__ void k_compact(const int32_t *flags, const int32_t *scan, int64_t n, int32_t *slot, int32_t *list) {
+// slot (or -1) and list of each half; tot[0], tot[1]: the two list lengths
+__global__ void k_compact2(const int32_t *flags, const int32_t *scan, int64_t n, int32_t *slot_a, int32_t *list_a,
+                           int32_t *slot_b, int32_t *list_b, int32_t *tot) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
+  const int32_t na = scan[n];
   if (flags[i]) {
-    slot[i] = scan[i];
-    list[scan[i]] = (int32_t)i;
+    slot_a[i] = scan[i];
+    list_a[scan[i]] = (int32_t)i;
   } else {
-    slot[i] = -1;
+    slot_a[i] = -1;
+  }
+  const int32_t k = scan[n + i] - na;
+  if (flags[n + i]) {
+    slot_b[i] = k;
+    list_b[k] = (int32_t)i;
+  } else {
+    slot_b[i] = -1;
+  }
+  if (i == n - 1) {
+    tot[0] = na;
+    tot[1] = scan[2 * n - 1] + flags[2 * n - 1] - na;
   }
 }
 __global__ void k_sum_i32(const int32_t *v, int64_t n, unsigned long long *out) {
@@ -490,11 +509,13 @@ void build_lists(nbfmm_ctx *c) {
   NB_LAUNCH();
   hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_sub, nc, tot + 2);
   NB_LAUNCH();
-  unsigned long long htot[3];
-  int32_t nactive = 0;
-  NB_HIP(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&nactive, active + nparents + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  // host reads through the pinned mirror
+  unsigned long long *htot = (unsigned long long *)c->hred.p;
+  int32_t *hint = (int32_t *)(htot + 8);
+  NB_HIP(hipMemcpyAsync(htot, tot, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(hint, active + nparents + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
+  const int32_t nactive = hint[0];
   const unsigned long long lim = 0x7fffffffull - 1;
   if (htot[1] > lim || htot[2] > lim || (c->keep_lists && htot[0] > lim))
     throw Error(NBFMM_ERR_UNSUPPORTED, "interaction lists exceed 2^31 entries (raise leaf_capacity or p2p_cut)");
@@ -516,32 +537,23 @@ void build_lists(nbfmm_ctx *c) {
   }
 
   // ---- M2L source / target compaction ----
-  const size_t fl = (size_t)std::max<int64_t>(nc, P.n) + 1;
+  const size_t fl = (size_t)std::max<int64_t>(2 * nc, P.n) + 1;
   int32_t *flags = dget<int32_t>(c->flags, fl);
   int32_t *fscan = dget<int32_t>(c->fscan, fl);
   int32_t *src_slot = dget<int32_t>(c->src_slot, (size_t)nc);
   int32_t *tgt_slot = dget<int32_t>(c->tgt_slot, (size_t)nc);
   int32_t *src_cells = dget<int32_t>(c->src_cells, (size_t)nc);
   int32_t *tgt_cells = dget<int32_t>(c->tgt_cells, (size_t)nc);
-  int32_t nst[4];
-  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, is_src, nc, flags);
+  // sources: cells in some M2L list; targets: cells with an M2L list.  Their
+  // counts reach the host with the task counts below.
+  int32_t *nst_d = (int32_t *)(tot + 3);
+  hipLaunchKernelGGL(k_flag_pos2, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)is_src, (const int32_t *)cnt_m2l,
+                     nc, flags);
   NB_LAUNCH();
-  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
-  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, src_slot, src_cells);
+  scan_excl_i32(flags, fscan, 2 * nc, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_compact2, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, src_slot, src_cells, tgt_slot,
+                     tgt_cells, nst_d);
   NB_LAUNCH();
-  NB_HIP(hipMemcpyAsync(&nst[0], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&nst[1], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.n_src = (int64_t)nst[0] + nst[1];
-  hipLaunchKernelGGL(k_flag_pos, dim3(nblk(nc)), dim3(kT), 0, s, cnt_m2l, nc, flags);
-  NB_LAUNCH();
-  scan_excl_i32(flags, fscan, nc, c->scan_tmp, s);
-  hipLaunchKernelGGL(k_compact, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, tgt_slot, tgt_cells);
-  NB_LAUNCH();
-  NB_HIP(hipMemcpyAsync(&nst[2], fscan + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&nst[3], flags + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.n_tgt = (int64_t)nst[2] + nst[3];
 
   // ---- P2P sub-tasks ----
   int32_t *closed = dget<int32_t>(c->closed, (size_t)nc + 1);
@@ -574,12 +586,15 @@ void build_lists(nbfmm_ctx *c) {
   NB_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long), s));
   hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)tcnt, 2 * (nc + 1), tot);
   NB_LAUNCH();
-  unsigned long long ntask, npairs;
-  int32_t nnarrow;
-  NB_HIP(hipMemcpyAsync(&ntask, tot, sizeof(ntask), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&npairs, pairs, sizeof(npairs), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(&nnarrow, toff + nc + 1, sizeof(nnarrow), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(htot, tot, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(htot + 1, pairs, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(hint, toff + nc + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipMemcpyAsync(hint + 1, nst_d, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
+  const unsigned long long ntask = htot[0], npairs = htot[1];
+  const int32_t nnarrow = hint[0];
+  P.n_src = hint[1];
+  P.n_tgt = hint[2];
   if (ntask > lim) throw Error(NBFMM_ERR_UNSUPPORTED, "P2P work list exceeds 2^31 tasks");
   P.n_tasks = (int64_t)ntask;
   P.n_tasks_narrow = nnarrow;

@@ -498,24 +498,29 @@ void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t yb
 }
 
 // ---- M2M and L2L on MFMA ---------------------------------------------------------------
-// Parent-stationary: a wave takes 16 parents of one level (the MFMA columns)
-// and runs the 8 octant operators (8 KP x KP matrices, small enough to stay in
-// the caches, read in A-fragment order).  M2M sums the children's multipoles
-// into the parents' (columns of a missing child are zero); L2L adds the
-// translated parent local to every child's local.
+// Parent-stationary: a workgroup takes 16 parents of one level (the MFMA
+// columns) and its 4 waves split the 8 octant operators (8 KP x KP matrices,
+// small enough to stay in the caches, read in A-fragment order): wave w runs
+// octants w and w + 4.  Splitting the octants keeps each wave's chain of
+// dependent MFMAs and operator loads short, which sets the time of the small
+// levels near the root.  M2M sums the children's multipoles into the parents'
+// (columns of a missing child are zero; the waves' partial sums meet in LDS,
+// added in a fixed order); L2L adds the translated parent local to every
+// child's local.
 template <int KP, class T>
 __global__ void __launch_bounds__(256) k_m2m(int32_t ncell, const int32_t *__restrict__ cells,
                                              const int32_t *__restrict__ child8, const int32_t *__restrict__ mslot,
                                              const T *__restrict__ frag8, T *__restrict__ Mexp) {
   constexpr int RB = KP / 16, KS = KP / 4;
+  __shared__ T part[2][RB * 4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kg = lane >> 4;
-  const int pi = ((int)blockIdx.x * 4 + wave) * 16 + col;
+  const int pi = (int)blockIdx.x * 16 + col;
   const int32_t cell = pi < ncell ? cells[pi] : -1;
   typename Mfma<T>::V acc[RB];
 #pragma unroll
   for (int r = 0; r < RB; ++r) acc[r] = typename Mfma<T>::V{0, 0, 0, 0};
-  for (int o = 0; o < 8; ++o) {
+  for (int o = wave; o < 8; o += 4) {
     const int32_t ch = cell >= 0 ? child8[(int64_t)cell * 8 + o] : -1;
     if (!__any(ch >= 0)) continue;
     const T *Mb = Mexp + (size_t)(ch >= 0 ? mslot[ch] : 0) * KP + kg;
@@ -531,12 +536,34 @@ __global__ void __launch_bounds__(256) k_m2m(int32_t ncell, const int32_t *__res
         for (int r = 0; r < RB; ++r) acc[r] = Mfma<T>::op(A[(size_t)(r * KS + k0 + u) * 64], b[u], acc[r]);
     }
   }
-  if (cell < 0) return;
+  // (w0 + w2) + (w1 + w3)
+  if (wave >= 2) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) part[wave - 2][r * 4 + q][lane] = acc[r][q];
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[r][q] += part[wave][r * 4 + q][lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) part[0][r * 4 + q][lane] = acc[r][q];
+  }
+  __syncthreads();
+  if (wave > 0 || cell < 0) return;
   T *M = Mexp + (size_t)mslot[cell] * KP;
 #pragma unroll
   for (int r = 0; r < RB; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) M[r * 16 + Mfma<T>::row(kg, q)] = acc[r][q];
+    for (int q = 0; q < 4; ++q) M[r * 16 + Mfma<T>::row(kg, q)] = acc[r][q] + part[0][r * 4 + q][lane];
 }
 
 template <int KP, class T>
@@ -546,10 +573,10 @@ __global__ void __launch_bounds__(256) k_l2l(int32_t ncell, const int32_t *__res
   constexpr int RB = KP / 16, KS = KP / 4;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kg = lane >> 4;
-  const int pi = ((int)blockIdx.x * 4 + wave) * 16 + col;
+  const int pi = (int)blockIdx.x * 16 + col;
   const int32_t cell = pi < ncell ? cells[pi] : -1;
   const T *Lb = Lexp + (size_t)(cell >= 0 ? lslot[cell] : 0) * KP + kg;
-  for (int o = 0; o < 8; ++o) {
+  for (int o = wave; o < 8; o += 4) {
     const int32_t ch = cell >= 0 ? child8[(int64_t)cell * 8 + o] : -1;
     if (!__any(ch >= 0)) continue;
     typename Mfma<T>::V acc[RB];
@@ -580,7 +607,7 @@ template <class T>
 void launch_trans(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *cells, const int32_t *slot,
                   const T *frag8, T *X) {
   if (ncell <= 0) return;
-  const unsigned nb = (unsigned)((ncell + 63) / 64);
+  const unsigned nb = (unsigned)((ncell + 15) / 16);
   const int32_t *child8 = (const int32_t *)c->child8.p;
   switch (KP) {
 #define NB_TR_CASE(k)                                                                                      \

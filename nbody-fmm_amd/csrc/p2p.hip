@@ -509,9 +509,24 @@ __global__ void k_direct_combine(int64_t ntgt, int nchunk, const double *__restr
 
 void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
 
-// Share of the P2P tasks run in part 0.
+// Share of the P2P tasks run in part 0 (mixed precision, FP64).
 #ifndef NBFMM_P2P_EARLY
-#define NBFMM_P2P_EARLY 0.2
+#define NBFMM_P2P_EARLY 0.3
+#endif
+#ifndef NBFMM_P2P_EARLY_F64
+#define NBFMM_P2P_EARLY_F64 0.2
+#endif
+// Part 0's narrow tasks go out in this many launches (mixed precision, FP64).
+// While one P2P launch still has workgroups to place, the kernels of the
+// main stream get almost no CU slots (measured: the up-pass planning stalled
+// until part 0 was placed); between launches they are placed.  Each launch
+// waits for the previous one to drain, which costs more with the longer FP64
+// tasks: measured best at 16 launches for mixed precision, 1 for FP64.
+#ifndef NBFMM_P2P_CHUNKS0
+#define NBFMM_P2P_CHUNKS0 16
+#endif
+#ifndef NBFMM_P2P_CHUNKS0_F64
+#define NBFMM_P2P_CHUNKS0_F64 1
 #endif
 
 void launch_p2p(nbfmm_ctx *c, int part) {
@@ -522,7 +537,9 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   double *pn = pot ? gn + 3 * (size_t)P.n : nullptr;
   // the same share of the narrow tasks [0, nn) and of the wide ones [nn, n_tasks)
   const int64_t nn = P.n_tasks_narrow, nw = P.n_tasks - nn;
-  const int64_t sn = (int64_t)((double)nn * NBFMM_P2P_EARLY), sw = nn + (int64_t)((double)nw * NBFMM_P2P_EARLY);
+  const bool f32 = c->prm.precision == NBFMM_MIXED;
+  const double early = f32 ? NBFMM_P2P_EARLY : NBFMM_P2P_EARLY_F64;
+  const int64_t sn = (int64_t)((double)nn * early), sw = nn + (int64_t)((double)nw * early);
   const int64_t r[2][2] = {{part == 0 ? 0 : sn, part == 0 ? sn : nn}, {part == 0 ? nn : sw, part == 0 ? sw : P.n_tasks}};
   if (part == 1) {
     // after the point of the main stream where part 1 is enqueued (the first M2L contraction)
@@ -535,19 +552,22 @@ void launch_p2p(nbfmm_ctx *c, int part) {
     NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
   }
   mark2(c, part == 0 ? 9 : 11);
-  const bool f32 = c->prm.precision == NBFMM_MIXED;
   for (int w = 0; w < 2; ++w) {
-    const int64_t t0 = r[w][0], t1 = r[w][1];
-    if (t1 <= t0) continue;
-    const int64_t grid = std::min<int64_t>(t1 - t0, ((int64_t)1 << 31) - 1);
-    auto kern = w == 0 ? (f32 ? (pot ? k_p2p_tasks<PairF32, true> : k_p2p_tasks<PairF32, false>)
-                              : (pot ? k_p2p_tasks<PairF64, true> : k_p2p_tasks<PairF64, false>))
-                       : (f32 ? (pot ? k_p2p_wide<PairF32, true> : k_p2p_wide<PairF32, false>)
-                              : (pot ? k_p2p_wide<PairF64, true> : k_p2p_wide<PairF64, false>));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, t1 - t0, (const P2PTask *)c->tasks.p + t0,
-                       (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
-                       (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn, pn);
-    NB_LAUNCH();
+    const int nch = (w == 0 && part == 0) ? (f32 ? NBFMM_P2P_CHUNKS0 : NBFMM_P2P_CHUNKS0_F64) : 1;
+    for (int ch = 0; ch < nch; ++ch) {
+      const int64_t t0 = r[w][0] + (r[w][1] - r[w][0]) * ch / nch;
+      const int64_t t1 = r[w][0] + (r[w][1] - r[w][0]) * (ch + 1) / nch;
+      if (t1 <= t0) continue;
+      const int64_t grid = std::min<int64_t>(t1 - t0, ((int64_t)1 << 31) - 1);
+      auto kern = w == 0 ? (f32 ? (pot ? k_p2p_tasks<PairF32, true> : k_p2p_tasks<PairF32, false>)
+                                : (pot ? k_p2p_tasks<PairF64, true> : k_p2p_tasks<PairF64, false>))
+                         : (f32 ? (pot ? k_p2p_wide<PairF32, true> : k_p2p_wide<PairF32, false>)
+                                : (pot ? k_p2p_wide<PairF64, true> : k_p2p_wide<PairF64, false>));
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, t1 - t0, (const P2PTask *)c->tasks.p + t0,
+                         (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
+                         (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn, pn);
+      NB_LAUNCH();
+    }
   }
   mark2(c, part == 0 ? 10 : 12);
   if (part == 1) NB_HIP(hipEventRecord(c->ev_join, s2));

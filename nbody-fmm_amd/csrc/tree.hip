@@ -194,8 +194,12 @@ __global__ void k_run_starts(const int32_t *flags, const int32_t *scan, int64_t 
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i < n && flags[i]) starts[scan[i]] = (int32_t)i;
 }
-__global__ void k_run_max(const int32_t *starts, int64_t nruns, int64_t n, uint32_t *out) {
+// out[0]: the longest run (out[0] zero on entry); out[1]: the number of runs
+__global__ void k_run_max(const int32_t *starts, const int32_t *flags, const int32_t *scan, int64_t n,
+                          uint32_t *out) {
   __shared__ uint32_t s[kT];
+  const int64_t nruns = (int64_t)scan[n - 1] + flags[n - 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[1] = (uint32_t)nruns;
   uint32_t best = 0;
   for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < nruns; r += (int64_t)gridDim.x * kT) {
     int64_t e = r + 1 < nruns ? starts[r + 1] : n;
@@ -222,64 +226,117 @@ __global__ void k_gather_pos(const double *__restrict__ xyz, const double *__res
 // ---- levels ------------------------------------------------------------------------
 // For consecutive sorted keys, the coarsest level at which they differ is
 // D - floor(msb/3); every level from there to D starts a new cell at i.
-__global__ void k_level_counts(const uint64_t *__restrict__ key, int64_t n, int D, uint32_t *counts) {
-  __shared__ uint32_t c[kMaxDepth + 1];
-  if (threadIdx.x <= (unsigned)D) c[threadIdx.x] = 0;
-  __syncthreads();
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  int d0 = D + 1; // the coarsest level at which particle i starts a new cell
-  if (i < n) {
-    if (i == 0) {
-      d0 = 0;
-    } else {
-      uint64_t x = key[i] ^ key[i - 1];
-      d0 = x == 0 ? D + 1 : D - (63 - __clzll((long long)x)) / 3;
-    }
-  }
-  // one count per wavefront and level
+__device__ inline int start_level(const uint64_t *__restrict__ key, int64_t i, int64_t n, int D) {
+  if (i >= n) return D + 1;
+  if (i == 0) return 0;
+  const uint64_t x = key[i] ^ key[i - 1];
+  return x == 0 ? D + 1 : D - (63 - __clzll((long long)x)) / 3;
+}
+
+// Cells that start in each block of kT particles, per level: bcnt[d * nb + b].
+__global__ void __launch_bounds__(kT) k_level_blocks(const uint64_t *__restrict__ key, int64_t n, int D,
+                                                     int32_t *__restrict__ bcnt) {
+  __shared__ int32_t w[kT / 64][kMaxDepth + 1];
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int d0 = start_level(key, i, n, D);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int d = 0; d <= D; ++d) {
     const uint64_t m = __ballot(d0 <= d);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c[d], (uint32_t)__popcll(m));
+    if (lane == 0) w[wv][d] = (int32_t)__popcll(m);
   }
   __syncthreads();
-  if (threadIdx.x <= (unsigned)D && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
-}
-
-__global__ void k_leaf_fill(const uint64_t *__restrict__ key, int64_t n, const int32_t *__restrict__ flags,
-                            const int32_t *__restrict__ scan, int32_t base, uint64_t *ckey, int32_t *cbeg,
-                            int32_t *cend, int32_t *pcell) {
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  int32_t leaf = base + scan[i] + flags[i] - 1;
-  pcell[i] = leaf;
-  if (flags[i]) {
-    ckey[leaf] = key[i];
-    cbeg[leaf] = (int32_t)i;
-    if (i > 0) cend[leaf - 1] = (int32_t)i;
+  if ((int)threadIdx.x <= D) {
+    int32_t t = 0;
+    for (int q = 0; q < kT / 64; ++q) t += w[q][threadIdx.x];
+    bcnt[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
   }
-  if (i == n - 1) cend[leaf] = (int32_t)n;
 }
 
-__global__ void k_parent_flags(const uint64_t *__restrict__ ckey, int32_t cb, int32_t ce, int32_t *flags) {
-  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
-  flags[c - cb] = (c == cb || (ckey[c] >> 3) != (ckey[c - 1] >> 3)) ? 1 : 0;
-}
-
-__global__ void k_parent_fill(uint64_t *ckey, int32_t *cbeg, int32_t *cend, int32_t *cparent, int32_t cb,
-                              int32_t ce, const int32_t *__restrict__ flags, const int32_t *__restrict__ scan,
-                              int32_t pbase) {
-  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
-  int32_t k = c - cb;
-  int32_t p = pbase + scan[k] + flags[k] - 1;
-  cparent[c] = p;
-  if (flags[k]) {
-    ckey[p] = ckey[c] >> 3;
-    cbeg[p] = cbeg[c];
-    if (c > cb) cend[p - 1] = cbeg[c];
+// One block per level: exclusive scan of that level's block counts in place
+// and the level's cell count.
+constexpr int kOffThreads = 1024, kOffItems = 8;
+__global__ void __launch_bounds__(kOffThreads) k_level_offsets(int32_t *bcnt, int nb, uint32_t *total) {
+  __shared__ int32_t ws[kOffThreads / 64];
+  int32_t *row = bcnt + (int64_t)blockIdx.x * nb;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int32_t carry = 0;
+  for (int base = 0; base < nb; base += kOffThreads * kOffItems) {
+    const int j0 = base + (int)threadIdx.x * kOffItems;
+    int32_t v[kOffItems], loc = 0;
+#pragma unroll
+    for (int k = 0; k < kOffItems; ++k) {
+      v[k] = j0 + k < nb ? row[j0 + k] : 0;
+      loc += v[k];
+    }
+    int32_t x = loc; // inclusive scan over the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wv] = x;
+    __syncthreads();
+    int32_t pre = 0, all = 0;
+    for (int q = 0; q < kOffThreads / 64; ++q) {
+      const int32_t t = ws[q];
+      if (q < wv) pre += t;
+      all += t;
+    }
+    int32_t run = carry + pre + x - loc;
+#pragma unroll
+    for (int k = 0; k < kOffItems; ++k) {
+      if (j0 + k < nb) row[j0 + k] = run;
+      run += v[k];
+    }
+    carry += all;
+    __syncthreads();
   }
-  if (c == ce - 1) cend[p] = cend[c];
+  if (threadIdx.x == 0) total[blockIdx.x] = (uint32_t)carry;
+}
+
+struct LevelStarts {
+  int32_t s[kMaxDepth + 2];
+};
+
+// Every level's cells at once: particle i finds the index of its cell at each
+// level from the block offsets and the ranks within its block; where it starts
+// a cell it writes the cell's key, first particle and parent, and ends the
+// level's previous cell.  Levels are concatenated from the root; keys of level
+// d are the leaf keys >> 3(D - d).
+__global__ void __launch_bounds__(kT) k_cells_fill(const uint64_t *__restrict__ key, int64_t n, int D,
+                                                   const int32_t *__restrict__ boff, LevelStarts L,
+                                                   uint64_t *__restrict__ ckey, int32_t *__restrict__ cbeg,
+                                                   int32_t *__restrict__ cend, int32_t *__restrict__ cpar,
+                                                   int32_t *__restrict__ pcell) {
+  __shared__ int32_t w[kT / 64][kMaxDepth + 1];
+  __shared__ int32_t bo[kMaxDepth + 1];
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int d0 = start_level(key, i, n, D);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t upto = ~0ull >> (63 - lane); // lanes 0..lane
+  if ((int)threadIdx.x <= D) bo[threadIdx.x] = boff[(int64_t)threadIdx.x * gridDim.x + blockIdx.x];
+  for (int d = 0; d <= D; ++d) {
+    const uint64_t m = __ballot(d0 <= d);
+    if (lane == 0) w[wv][d] = (int32_t)__popcll(m);
+  }
+  __syncthreads();
+  const uint64_t k = i < n ? key[i] : 0;
+  int32_t up = -1; // the cell one level up (-1 above the root)
+  for (int d = 0; d <= D; ++d) {
+    const uint64_t m = __ballot(d0 <= d);
+    int32_t pre = 0;
+    for (int q = 0; q < wv; ++q) pre += w[q][d];
+    const int32_t cell = L.s[d] + bo[d] + pre + (int32_t)__popcll(m & upto) - 1;
+    if (d0 <= d) {
+      ckey[cell] = k >> (3 * (D - d));
+      cbeg[cell] = (int32_t)i;
+      cpar[cell] = up;
+      if (cell > L.s[d]) cend[cell - 1] = (int32_t)i;
+    }
+    if (i == n - 1) cend[cell] = (int32_t)n;
+    up = cell;
+  }
+  if (i < n) pcell[i] = up;
 }
 
 __global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent, int32_t cb,
@@ -411,19 +468,16 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
     int32_t *starts = (int32_t *)c->idx[cur ^ 1].p; // free while not sorting
     hipLaunchKernelGGL(k_run_starts, dim3(nblk(n)), dim3(kT), 0, s, flags, fscan, n, starts);
     NB_LAUNCH();
-    int32_t tail[2];
-    NB_HIP(hipMemcpyAsync(&tail[0], fscan + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NB_HIP(hipMemcpyAsync(&tail[1], flags + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NB_HIP(hipStreamSynchronize(s));
-    int64_t nruns = (int64_t)tail[0] + tail[1];
     NB_HIP(hipMemsetAsync(dmax, 0, sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_run_max, dim3((unsigned)std::min<int64_t>(1024, nblk(nruns))), dim3(kT), 0, s, starts,
-                       nruns, n, dmax);
+    hipLaunchKernelGGL(k_run_max, dim3((unsigned)std::min<int64_t>(1024, nblk(n))), dim3(kT), 0, s, starts, flags,
+                       fscan, n, dmax);
     NB_LAUNCH();
-    NB_HIP(hipMemcpyAsync(&maxcnt, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    uint32_t *hm = (uint32_t *)(h + 16); // pinned
+    NB_HIP(hipMemcpyAsync(hm, dmax, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NB_HIP(hipStreamSynchronize(s));
+    maxcnt = hm[0];
     P.iterations++;
-    P.n_leaves = nruns;
+    P.n_leaves = hm[1];
   } while (depth < kMaxDepth && maxcnt > thr);
   P.geo.depth = depth;
   if (cur != 0) { // keep the sorted keys/indices in buffer 0
@@ -439,10 +493,13 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   // ---- cells of every level ----
   const int D = depth;
   uint32_t *dcounts = (uint32_t *)(out6 + 16);
-  NB_HIP(hipMemsetAsync(dcounts, 0, sizeof(uint32_t) * (kMaxDepth + 1), s));
-  hipLaunchKernelGGL(k_level_counts, dim3(nblk(n)), dim3(kT), 0, s, key, n, D, dcounts);
+  const int64_t nbl = nblk(n);
+  int32_t *bcnt = dget<int32_t>(c->flags, std::max<size_t>((size_t)n + 1, (size_t)(D + 1) * (size_t)nbl));
+  hipLaunchKernelGGL(k_level_blocks, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt);
   NB_LAUNCH();
-  uint32_t counts[kMaxDepth + 1];
+  hipLaunchKernelGGL(k_level_offsets, dim3(D + 1), dim3(kOffThreads), 0, s, bcnt, (int)nbl, dcounts);
+  NB_LAUNCH();
+  uint32_t *counts = (uint32_t *)(h + 16); // pinned
   NB_HIP(hipMemcpyAsync(counts, dcounts, sizeof(uint32_t) * (D + 1), hipMemcpyDeviceToHost, s));
   NB_HIP(hipStreamSynchronize(s));
   P.lvl_start.assign(D + 2, 0);
@@ -454,24 +511,11 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   int32_t *cend = dget<int32_t>(c->cend, (size_t)nc);
   int32_t *cpar = dget<int32_t>(c->cparent, (size_t)nc);
   int32_t *pcell = dget<int32_t>(c->pcell, (size_t)n);
-  // leaves
-  hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(kT), 0, s, key, n, flags);
+  LevelStarts L{};
+  for (int d = 0; d <= D + 1; ++d) L.s[d] = P.lvl_start[d];
+  hipLaunchKernelGGL(k_cells_fill, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, (const int32_t *)bcnt, L, ckey,
+                     cbeg, cend, cpar, pcell);
   NB_LAUNCH();
-  scan_excl_i32(flags, fscan, n, c->scan_tmp, s);
-  hipLaunchKernelGGL(k_leaf_fill, dim3(nblk(n)), dim3(kT), 0, s, key, n, flags, fscan, P.lvl_start[D], ckey, cbeg,
-                     cend, pcell);
-  NB_LAUNCH();
-  // ancestors
-  for (int d = D - 1; d >= 0; --d) {
-    int32_t cb = P.lvl_start[d + 1], ce = P.lvl_start[d + 2];
-    hipLaunchKernelGGL(k_parent_flags, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cb, ce, flags);
-    NB_LAUNCH();
-    scan_excl_i32(flags, fscan, ce - cb, c->scan_tmp, s);
-    hipLaunchKernelGGL(k_parent_fill, dim3(nblk(ce - cb)), dim3(kT), 0, s, ckey, cbeg, cend, cpar, cb, ce, flags,
-                       fscan, P.lvl_start[d]);
-    NB_LAUNCH();
-  }
-  NB_HIP(hipMemsetAsync(cpar, 0xff, sizeof(int32_t), s)); // root parent = -1
   // neighbour tables
   int32_t *child8 = dget<int32_t>(c->child8, (size_t)nc * 8);
   int32_t *nbr27 = dget<int32_t>(c->nbr27, (size_t)nc * 27);
