@@ -294,33 +294,38 @@ __global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, double hx, do
 }
 
 // ---- P2M ---------------------------------------------------------------------------
-// Item = (source slot, particle range).  Lanes are 16 columns m (the orders;
-// m and m + 16 for p = 16) times 16 particle slots (4 per wave): a lane runs
-// its column for every 16th particle of the item -- q R_m^m by the sectoral
-// recurrence, then R_n^m, n = m+1..p, by the three-term recurrence in n --
-// accumulating per lane, so every column costs about p steps per particle
-// and no lane waits on another.  The 4 slots of a wave meet in two
-// xor-shuffles, the 4 waves in LDS, and each coefficient is added to the
-// multipole once per item.  T is the coefficient type (double, or float in
-// mixed precision); the offset from the cell centre is formed in FP64 and
-// scaled before rounding.
+// Item = (source slot, particle range); one wavefront per item, four items per
+// workgroup.  Lanes are 16 columns m (the orders; m and m + 16 for p > 15)
+// times 4 particle slots: a lane runs its column for every 4th particle of the
+// item -- q R_m^m by the sectoral recurrence, then R_n^m, n = m+1..p, by the
+// three-term recurrence in n -- accumulating per lane, so every column costs
+// about p steps per particle and no lane waits on another.  The 4 slots meet
+// in two xor-shuffles and the column's lane writes the coefficients: a store
+// when the item is the whole cell (P2M cells hold at most kTransCut particles
+// unless they are leaves), else an atomic add.  P2M cells mostly hold tens of
+// particles, so a workgroup per item spent its time on set-up and on the
+// combine across waves.  T is the coefficient type (double, or float in mixed
+// precision); the offset from the cell centre is formed in FP64 and scaled
+// before rounding.
 template <class T>
-__global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restrict__ items, const double4 *__restrict__ pos,
-                                            const int32_t *__restrict__ cells, const int32_t *__restrict__ mslot,
-                                            const uint64_t *__restrict__ ckey, LevelGeo g, const Consts *__restrict__ C,
-                                            T *__restrict__ Mexp) {
-  constexpr int kCoef = (kMaxOrder + 1) * (kMaxOrder + 1);
-  __shared__ T part[kT / 64][kCoef]; // each wave's sums, in the multipole's storage order
-  const int4 it = items[blockIdx.x];
+__global__ void __launch_bounds__(kT) k_p2m(int p, int KP, int nitems, const int4 *__restrict__ items,
+                                            const double4 *__restrict__ pos, const int32_t *__restrict__ cells,
+                                            const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                                            const int32_t *__restrict__ mslot, const uint64_t *__restrict__ ckey,
+                                            LevelGeo g, const Consts *__restrict__ C, T *__restrict__ Mexp) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int item = (int)blockIdx.x * (kT / 64) + wave;
+  if (item >= nitems) return; // a whole wavefront; nothing below synchronises the workgroup
+  const int4 it = items[item];
   const int beg = it.y, end = it.z;
   const int32_t cell = cells[it.x];
+  const bool whole = beg == cbeg[cell] && end == cend[cell];
   const int slot = mslot[cell];
   const int lev = level_of(cell, g);
   double ctr[3];
   cell_centre(ckey[cell], lev, g, ctr);
   const double ir = g.inv_rho[lev];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int col = lane & 15, ps = wave * 4 + (lane >> 4); // column, particle slot (16)
+  const int col = lane & 15, ps = lane >> 4; // column, particle slot (4)
   T *M = Mexp + (size_t)slot * KP;
   for (int m0 = 0; m0 <= p; m0 += 16) {
     const int m = m0 + col;
@@ -338,7 +343,7 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
     }
     const T fs = nj >= 1 ? (T)C->s2m1[m] : (T)0;
     if (nj >= 0)
-      for (int i = beg + ps; i < end; i += 16) {
+      for (int i = beg + ps; i < end; i += 4) {
         const double4 q = pos[i];
         const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
         const T r2 = x * x + y * y + z * z;
@@ -377,7 +382,7 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
           }
         }
       }
-    // the wave's 4 slots of a column: lanes col, col + 16, col + 32, col + 48
+    // the 4 slots of a column: lanes col, col + 16, col + 32, col + 48
 #pragma unroll
     for (int j = 0; j <= kMaxOrder; ++j) {
       if (j <= p - m0) { // uniform: every lane shuffles
@@ -388,22 +393,17 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const int4 *__restric
         vi += __shfl_xor(vi, 32);
         if (lane < 16 && j <= nj) {
           const int n = m + j;
-          if (m == 0) {
-            part[wave][n * n] = vr;
+          const int er = m == 0 ? n * n : n * n + 2 * m - 1;
+          if (whole) {
+            M[er] = vr;
+            if (m > 0) M[er + 1] = vi;
           } else {
-            part[wave][n * n + 2 * m - 1] = vr;
-            part[wave][n * n + 2 * m] = vi;
+            atomicAdd(&M[er], vr);
+            if (m > 0) atomicAdd(&M[er + 1], vi);
           }
         }
       }
     }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < (p + 1) * (p + 1); e += kT) {
-    T v = part[0][e];
-#pragma unroll
-    for (int w = 1; w < kT / 64; ++w) v += part[w][e];
-    atomicAdd(&M[e], v);
   }
 }
 
@@ -795,12 +795,13 @@ void run_expansions(nbfmm_ctx *c) {
   void *Mexp = dbuf_ensure(c->mexp, (size_t)nM * KP * esz + 16);
   NB_HIP(hipMemsetAsync(Mexp, 0, esz * (size_t)nM * KP, s));
   if (nitems > 0) {
+    const unsigned nb = (unsigned)((nitems + kT / 64 - 1) / (kT / 64));
     if (f32)
-      hipLaunchKernelGGL(k_p2m<float>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
-                         (const int32_t *)p2m_cells, (const int32_t *)mslot, ckey, g, dC, (float *)Mexp);
+      hipLaunchKernelGGL(k_p2m<float>, dim3(nb), dim3(kT), 0, s, p, KP, nitems, items, (const double4 *)c->pos.p,
+                         (const int32_t *)p2m_cells, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (float *)Mexp);
     else
-      hipLaunchKernelGGL(k_p2m<double>, dim3(nitems), dim3(kT), 0, s, p, KP, items, (const double4 *)c->pos.p,
-                         (const int32_t *)p2m_cells, (const int32_t *)mslot, ckey, g, dC, (double *)Mexp);
+      hipLaunchKernelGGL(k_p2m<double>, dim3(nb), dim3(kT), 0, s, p, KP, nitems, items, (const double4 *)c->pos.p,
+                         (const int32_t *)p2m_cells, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (double *)Mexp);
     NB_LAUNCH();
   }
   // ---- M2M, finest level first ----
