@@ -61,12 +61,12 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
   mark(c, 0);
   build_tree(c, d_xyz, d_m);
   mark(c, 1);
-  build_lists(c);
-  mark(c, 2);
   try {
+    build_lists(c);
+    mark(c, 2);
     run_expansions(c); // marks 3, 4, 5 (7, 8 inside); starts P2P on the second stream (marks 9, 10)
   } catch (...) {
-    (void)hipStreamSynchronize(c->stream2); // P2P still reads the context's buffers
+    (void)hipStreamSynchronize(c->stream2); // the particle gather or P2P still reads buffers
     c->phi_out = nullptr;
     throw;
   }
@@ -228,6 +228,7 @@ nbfmm_ctx::~nbfmm_ctx() {
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_fork2) (void)hipEventDestroy(ev_fork2);
   if (ev_join) (void)hipEventDestroy(ev_join);
+  if (ev_pos) (void)hipEventDestroy(ev_pos);
   if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -269,6 +270,7 @@ int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
   NB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   NB_HIP(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming));
   NB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  NB_HIP(hipEventCreateWithFlags(&c->ev_pos, hipEventDisableTiming));
   for (auto &e : c->ev) NB_HIP(hipEventCreate(&e));
   NB_HIP(hipHostMalloc(&c->hred.p, 65536, hipHostMallocDefault));
   c->hred.cap = 65536;

@@ -604,6 +604,7 @@ void build_lists(nbfmm_ctx *c) {
                      (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
                      (const int64_t *)sub_pairs, narrow_max, toff, tasks);
   NB_LAUNCH();
+  NB_HIP(hipStreamWaitEvent(s, c->ev_pos, 0)); // the sorted particles (build_tree)
 }
 
 } // namespace nbfmm

@@ -153,6 +153,7 @@ struct nbfmm_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;            // P2P, concurrent with the expansion passes
   hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_join = nullptr;
+  hipEvent_t ev_pos = nullptr; // the sorted particles (gathered on stream2) are complete
   int device = 0;
   int lds_max = 65536; // LDS bytes one workgroup may allocate on this device
   int n_cu = 256;      // compute units of this device

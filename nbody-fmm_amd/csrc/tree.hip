@@ -487,8 +487,14 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   const uint64_t *key = (const uint64_t *)c->key[0].p;
   const int32_t *idx = (const int32_t *)c->idx[0].p;
   double4 *pos = dget<double4>(c->pos, (size_t)n);
-  hipLaunchKernelGGL(k_gather_pos, dim3(nblk(n)), dim3(kT), 0, s, d_xyz, d_m, idx, n, pos);
+  // the particle gather runs on the second stream, beside the cells, the
+  // neighbour tables and the lists; P2P follows it on that stream, and the
+  // main stream waits for it at the end of build_lists
+  NB_HIP(hipEventRecord(c->ev_pos, s));
+  NB_HIP(hipStreamWaitEvent(c->stream2, c->ev_pos, 0));
+  hipLaunchKernelGGL(k_gather_pos, dim3(nblk(n)), dim3(kT), 0, c->stream2, d_xyz, d_m, idx, n, pos);
   NB_LAUNCH();
+  NB_HIP(hipEventRecord(c->ev_pos, c->stream2));
 
   // ---- cells of every level ----
   const int D = depth;
