@@ -511,7 +511,7 @@ void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
 
 // Share of the P2P tasks run in part 0 (mixed precision, FP64).
 #ifndef NBFMM_P2P_EARLY
-#define NBFMM_P2P_EARLY 0.3
+#define NBFMM_P2P_EARLY 0.5
 #endif
 #ifndef NBFMM_P2P_EARLY_F64
 #define NBFMM_P2P_EARLY_F64 0.2
