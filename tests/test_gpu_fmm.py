@@ -40,9 +40,15 @@ def _check_tree(gt, ot, n):
         assert np.array_equal(gt["il"], ot["il"])
 
 
-@pytest.mark.parametrize("case", ["uniform2k", "plummer50k", "tiny", "clustered"])
+@pytest.mark.parametrize("case", ["uniform2k", "plummer50k", "tiny", "clustered", "duplicates"])
 def test_tree_bitexact(ctx, case):
-    if case == "uniform2k":
+    if case == "duplicates":
+        # 40 particles at one point: the depth loop runs to the deepest level
+        # (16), so every level's cells come from one set of launches
+        xyz, m = workloads.uniform_cube(3000, seed=4)
+        xyz[:40] = xyz[0]
+        thr = 16
+    elif case == "uniform2k":
         xyz, m = workloads.uniform_cube(2000, seed=3)
         thr = 16
     elif case == "plummer50k":
