@@ -426,8 +426,60 @@ __device__ __forceinline__ cpx<T> lget(const T *L, int n, int m, int p) {
   return {-L[n * n + 1], L[n * n + 2]};
 }
 
+// One evaluated local at a particle (offset x, y, z, r2 = |.|^2 in cell
+// units): the gradient sums hx, hy, hz and, with POT, the potential sum hv.
+template <class T, bool POT>
+__device__ __forceinline__ void l2p_local(const T *L, int pc, T x, T y, T z, T r2, const Consts *__restrict__ C,
+                                          T &hx, T &hy, T &hz, T &hv) {
+  T vr = 1, vi = 0; // R_m^m
+  for (int m = 0; m < (POT ? pc + 1 : pc); ++m) {
+    if (m > 0) {
+      const T f = (T)C->mmf[m];
+      const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
+      vr = nr;
+      vi = ni;
+    }
+    const T w = m == 0 ? (T)1 : (T)2;
+    T pr = 0, pi = 0, cr = vr, ci = vi;
+    for (int np = m; np < (POT ? pc + 1 : pc); ++np) {
+      if (np == m + 1) {
+        pr = cr;
+        pi = ci;
+        const T f = z * (T)C->s2m1[m];
+        cr = f * pr;
+        ci = f * pi;
+      } else if (np > m + 1) {
+        const T a = (T)C->ra[np][m] * z, b = (T)C->rb[np][m] * r2;
+        const T nr = a * cr - b * pr, ni = a * ci - b * pi;
+        pr = cr;
+        pi = ci;
+        cr = nr;
+        ci = ni;
+      }
+      if (POT) {
+        const cpx<T> Lv = lget(L, np, m, pc);
+        hv += w * (cr * Lv.x - ci * Lv.y);
+        if (np == pc) continue;
+      }
+      const int nn = np + 1;
+      const cpx<T> Lm = lget(L, nn, m - 1, pc), L0 = lget(L, nn, m, pc), Lp = lget(L, nn, m + 1, pc);
+      const T cz = (T)C->cz[nn][m], cu = (T)C->cu[nn][m], cdd = (T)C->cd[nn][m];
+      // Re(R * G)
+      hz += w * cz * (cr * L0.x - ci * L0.y);
+      const T ux = cu * Lm.x - cdd * Lp.x, uy = cu * Lm.y - cdd * Lp.y;
+      hx += (T)0.5 * w * (cr * ux - ci * uy);
+      const T vx = cu * Lm.x + cdd * Lp.x, vy = cu * Lm.y + cdd * Lp.y;
+      hy += (T)0.5 * w * (cr * vy + ci * vx);
+    }
+  }
+}
+
 // T as in k_p2m; the gradient is accumulated over levels in FP64.  POT also
 // evaluates the potential sum (1/rho) sum_{n<=p,m>=0} w(m) Re[L_n^m R_n^m] into pot.
+// When every active lane of a wavefront evaluates the same cell (consecutive
+// particles in Morton order mostly share their evaluated cells), the local is
+// addressed through a wave-uniform pointer, so its coefficients are read by
+// scalar loads instead of one vector load per lane.
 template <class T, bool POT>
 __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ pcell, const int32_t *__restrict__ cparent,
@@ -449,48 +501,13 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
       const double ir = g.inv_rho[lev];
       const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
       const T r2 = x * x + y * y + z * z;
-      const T *L = Lexp + (size_t)ts * KP;
       T hx = 0, hy = 0, hz = 0, hv = 0;
-      T vr = 1, vi = 0; // R_m^m
-      for (int m = 0; m < (POT ? pc + 1 : pc); ++m) {
-        if (m > 0) {
-          const T f = (T)C->mmf[m];
-          const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
-          vr = nr;
-          vi = ni;
-        }
-        const T w = m == 0 ? (T)1 : (T)2;
-        T pr = 0, pi = 0, cr = vr, ci = vi;
-        for (int np = m; np < (POT ? pc + 1 : pc); ++np) {
-          if (np == m + 1) {
-            pr = cr;
-            pi = ci;
-            const T f = z * (T)C->s2m1[m];
-            cr = f * pr;
-            ci = f * pi;
-          } else if (np > m + 1) {
-            const T a = (T)C->ra[np][m] * z, b = (T)C->rb[np][m] * r2;
-            const T nr = a * cr - b * pr, ni = a * ci - b * pi;
-            pr = cr;
-            pi = ci;
-            cr = nr;
-            ci = ni;
-          }
-          if (POT) {
-            const cpx<T> Lv = lget(L, np, m, pc);
-            hv += w * (cr * Lv.x - ci * Lv.y);
-            if (np == pc) continue;
-          }
-          const int nn = np + 1;
-          const cpx<T> Lm = lget(L, nn, m - 1, pc), L0 = lget(L, nn, m, pc), Lp = lget(L, nn, m + 1, pc);
-          const T cz = (T)C->cz[nn][m], cu = (T)C->cu[nn][m], cdd = (T)C->cd[nn][m];
-          // Re(R * G)
-          hz += w * cz * (cr * L0.x - ci * L0.y);
-          const T ux = cu * Lm.x - cdd * Lp.x, uy = cu * Lm.y - cdd * Lp.y;
-          hx += (T)0.5 * w * (cr * ux - ci * uy);
-          const T vx = cu * Lm.x + cdd * Lp.x, vy = cu * Lm.y + cdd * Lp.y;
-          hy += (T)0.5 * w * (cr * vy + ci * vx);
-        }
+      const int32_t ts0 = __builtin_amdgcn_readfirstlane(ts);
+      if (__all(ts == ts0)) {
+        const int pc0 = __builtin_amdgcn_readfirstlane(pc);
+        l2p_local<T, POT>(Lexp + (size_t)ts0 * KP, pc0, x, y, z, r2, C, hx, hy, hz, hv);
+      } else {
+        l2p_local<T, POT>(Lexp + (size_t)ts * KP, pc, x, y, z, r2, C, hx, hy, hz, hv);
       }
       const double s = ir * ir;
       gx += s * (double)hx;
