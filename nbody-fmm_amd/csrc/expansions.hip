@@ -539,18 +539,6 @@ __global__ void k_p2m_items(const int32_t *src_cells, int64_t nsrc, const int32_
   for (int32_t b = cbeg[c]; b < cend[c]; b += P2M_CHUNK)
     items[w++] = make_int4((int)s, b, min(b + P2M_CHUNK, cend[c]), 0);
 }
-__global__ void k_terms(const int32_t *cells, int64_t ncell, const int32_t *cbeg, const int32_t *cend,
-                        unsigned long long *out) {
-  __shared__ unsigned long long s[kT];
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  s[threadIdx.x] = i < ncell ? (unsigned long long)(cend[cells[i]] - cbeg[cells[i]]) : 0ull;
-  __syncthreads();
-  for (int o = kT / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(out, s[0]);
-}
 
 // ---- upward / downward plan ------------------------------------------------------------
 // A cell needs a multipole when it is an M2L source or its parent forms its
@@ -568,13 +556,17 @@ constexpr int32_t kTransCut = 256;
 // and hold their children's particles, so the ancestors of a big parent are
 // all big: a cell needs its multipole iff it is a source or, with a big
 // parent, some ancestor is.  Locals alike, with "has an M2L list".  Also the
-// five flag arrays of length n + 1 (flag n = 0): M slot, P2M, M2M, L slot, L2L.
+// five flag arrays of length n + 1 (flag n = 0): M slot, P2M, M2M, L slot, L2L,
+// and the particles of the P2M cells (terms[0]) and of the cells that
+// evaluate their local (terms[1]); terms are zero on entry.
 __global__ void __launch_bounds__(kT) k_updown_modes(int64_t n, int32_t leaf_begin, const int32_t *__restrict__ cparent,
                                                      const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                      const int32_t *__restrict__ is_src,
                                                      const int32_t *__restrict__ cnt_m2l, int32_t *mode_m,
-                                                     int32_t *mode_l, int32_t *flags) {
+                                                     int32_t *mode_l, int32_t *flags, unsigned long long *terms) {
+  __shared__ unsigned long long red[2][kT / 64];
   const int64_t s = n + 1;
+  unsigned long long tm = 0, tl = 0;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i <= n; i += (int64_t)gridDim.x * kT) {
     int mm = 0, ml = 0;
     if (i < n) {
@@ -591,12 +583,29 @@ __global__ void __launch_bounds__(kT) k_updown_modes(int64_t n, int32_t leaf_beg
       ml = l ? (big ? 2 : 1) : 0;
       mode_m[i] = mm;
       mode_l[i] = ml;
+      const unsigned long long np = (unsigned long long)(cend[i] - cbeg[i]);
+      if (mm == 1) tm += np;
+      if (ml == 1) tl += np;
     }
     flags[i] = mm > 0;
     flags[s + i] = mm == 1;
     flags[2 * s + i] = mm == 2;
     flags[3 * s + i] = ml > 0;
     flags[4 * s + i] = ml == 2;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    tm += __shfl_xor(tm, o);
+    tl += __shfl_xor(tl, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = tm;
+    red[1][threadIdx.x >> 6] = tl;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kT / 64; ++w) t += red[threadIdx.x][w];
+    if (t) atomicAdd(&terms[threadIdx.x], t);
   }
 }
 
@@ -625,22 +634,6 @@ struct ReadList {
 __global__ void k_gather_reads(const int32_t *__restrict__ src, ReadList r, int cnt, int32_t *__restrict__ dst) {
   const int k = threadIdx.x;
   if (k < cnt) dst[k] = src[r.off[k]] - src[r.base[k]];
-}
-
-// particles of the cells with mode == v
-__global__ void k_terms_mode(const int32_t *mode, int32_t v, int64_t ncell, const int32_t *cbeg, const int32_t *cend,
-                             unsigned long long *out) {
-  __shared__ unsigned long long s[kT];
-  unsigned long long a = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < ncell; i += (int64_t)gridDim.x * kT)
-    if (mode[i] == v) a += (unsigned long long)(cend[i] - cbeg[i]);
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int o = kT / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(out, s[0]);
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT > 0 ? (n + kT - 1) / kT : 1); }
@@ -717,10 +710,12 @@ void run_expansions(nbfmm_ctx *c) {
   const int64_t s5 = nc + 1;
   int32_t *flags = dget<int32_t>(c->flags, (size_t)5 * s5);
   int32_t *fscan = dget<int32_t>(c->fscan, (size_t)5 * s5);
+  unsigned long long *terms = (unsigned long long *)((double *)c->red.p + 6 * 1024 + 48);
+  NB_HIP(hipMemsetAsync(terms, 0, 2 * sizeof(unsigned long long), s));
   // few workgroups: this runs beside the first share of P2P
   hipLaunchKernelGGL(k_updown_modes, dim3(std::min<unsigned>(512, nblk(nc + 1))), dim3(kT), 0, s, nc, leaf_begin,
                      (const int32_t *)c->cparent.p, cbeg, cend, (const int32_t *)c->is_src.p,
-                     (const int32_t *)c->cnt_m2l.p, mode_m, mode_l, flags);
+                     (const int32_t *)c->cnt_m2l.p, mode_m, mode_l, flags, terms);
   NB_LAUNCH();
   scan_excl_i32(flags, fscan, 5 * s5, c->scan_tmp, s);
   int32_t *mslot = dget<int32_t>(c->mslot, (size_t)nc + 1);
@@ -751,14 +746,6 @@ void run_expansions(nbfmm_ctx *c) {
     NB_LAUNCH();
     NB_HIP(hipMemcpyAsync(h, dreads, sizeof(int32_t) * 64, hipMemcpyDeviceToHost, s));
   }
-  unsigned long long *terms = (unsigned long long *)((double *)c->red.p + 6 * 1024 + 48);
-  NB_HIP(hipMemsetAsync(terms, 0, 2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_terms_mode, dim3(std::min<unsigned>(1024, nblk(nc))), dim3(kT), 0, s, (const int32_t *)mode_m, 1, nc,
-                     cbeg, cend, terms);
-  NB_LAUNCH();
-  hipLaunchKernelGGL(k_terms_mode, dim3(std::min<unsigned>(1024, nblk(nc))), dim3(kT), 0, s, (const int32_t *)mode_l, 1, nc,
-                     cbeg, cend, terms + 1);
-  NB_LAUNCH();
   NB_HIP(hipStreamSynchronize(s));
   const int32_t nM = h[0], nP2M = h[1], nL = h[3];
   c->n_local = nL;
