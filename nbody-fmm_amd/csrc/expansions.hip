@@ -32,7 +32,6 @@ namespace nbfmm {
 namespace {
 
 constexpr int kT = 256;
-constexpr int P2M_CHUNK = 1024;
 
 // Coefficient tables (host-computed once per order, uploaded to c->consts).
 struct Consts {
@@ -294,32 +293,29 @@ __global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, double hx, do
 }
 
 // ---- P2M ---------------------------------------------------------------------------
-// Item = (source slot, particle range); one wavefront per item, four items per
-// workgroup.  Lanes are 16 columns m (the orders; m and m + 16 for p > 15)
-// times 4 particle slots: a lane runs its column for every 4th particle of the
-// item -- q R_m^m by the sectoral recurrence, then R_n^m, n = m+1..p, by the
-// three-term recurrence in n -- accumulating per lane, so every column costs
-// about p steps per particle and no lane waits on another.  The 4 slots meet
-// in two xor-shuffles and the column's lane writes the coefficients: a store
-// when the item is the whole cell (P2M cells hold at most kTransCut particles
-// unless they are leaves), else an atomic add.  P2M cells mostly hold tens of
-// particles, so a workgroup per item spent its time on set-up and on the
-// combine across waves.  T is the coefficient type (double, or float in mixed
-// precision); the offset from the cell centre is formed in FP64 and scaled
-// before rounding.
+// One wavefront per P2M cell, four cells per workgroup.  Lanes are 16 columns
+// m (the orders; m and m + 16 for p > 15) times 4 particle slots: a lane runs
+// its column for every 4th particle of the cell -- q R_m^m by the sectoral
+// recurrence, then R_n^m, n = m+1..p, by the three-term recurrence in n --
+// accumulating per lane, so every column costs about p steps per particle and
+// no lane waits on another.  The 4 slots meet in two xor-shuffles and the
+// column's lane stores the coefficients.  P2M cells hold at most kTransCut
+// particles unless they are leaves (more only where particles coincide at the
+// deepest level) and mostly tens, so a workgroup per cell spent its time on
+// set-up and on the combine across waves.  T is the coefficient type (double,
+// or float in mixed precision); the offset from the cell centre is formed in
+// FP64 and scaled before rounding.
 template <class T>
-__global__ void __launch_bounds__(kT) k_p2m(int p, int KP, int nitems, const int4 *__restrict__ items,
-                                            const double4 *__restrict__ pos, const int32_t *__restrict__ cells,
-                                            const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
-                                            const int32_t *__restrict__ mslot, const uint64_t *__restrict__ ckey,
-                                            LevelGeo g, const Consts *__restrict__ C, T *__restrict__ Mexp) {
+__global__ void __launch_bounds__(kT) k_p2m(int p, int KP, int ncell, const int32_t *__restrict__ cells,
+                                            const double4 *__restrict__ pos, const int32_t *__restrict__ cbeg,
+                                            const int32_t *__restrict__ cend, const int32_t *__restrict__ mslot,
+                                            const uint64_t *__restrict__ ckey, LevelGeo g,
+                                            const Consts *__restrict__ C, T *__restrict__ Mexp) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int item = (int)blockIdx.x * (kT / 64) + wave;
-  if (item >= nitems) return; // a whole wavefront; nothing below synchronises the workgroup
-  const int4 it = items[item];
-  const int beg = it.y, end = it.z;
-  const int32_t cell = cells[it.x];
-  const bool whole = beg == cbeg[cell] && end == cend[cell];
+  const int k = (int)blockIdx.x * (kT / 64) + wave;
+  if (k >= ncell) return; // a whole wavefront; nothing below synchronises the workgroup
+  const int32_t cell = cells[k];
+  const int beg = cbeg[cell], end = cend[cell];
   const int slot = mslot[cell];
   const int lev = level_of(cell, g);
   double ctr[3];
@@ -394,13 +390,8 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, int nitems, const int
         if (lane < 16 && j <= nj) {
           const int n = m + j;
           const int er = m == 0 ? n * n : n * n + 2 * m - 1;
-          if (whole) {
-            M[er] = vr;
-            if (m > 0) M[er + 1] = vi;
-          } else {
-            atomicAdd(&M[er], vr);
-            if (m > 0) atomicAdd(&M[er + 1], vi);
-          }
+          M[er] = vr;
+          if (m > 0) M[er + 1] = vi;
         }
       }
     }
@@ -521,23 +512,6 @@ __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const doub
   grad[3 * i + 1] = gy;
   grad[3 * i + 2] = gz;
   if (POT) pot[i] = gp;
-}
-
-__global__ void k_p2m_count(const int32_t *src_cells, int64_t nsrc, const int32_t *cbeg, const int32_t *cend,
-                            int32_t *cnt) {
-  int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (s >= nsrc) return;
-  int32_t c = src_cells[s];
-  cnt[s] = (cend[c] - cbeg[c] + P2M_CHUNK - 1) / P2M_CHUNK;
-}
-__global__ void k_p2m_items(const int32_t *src_cells, int64_t nsrc, const int32_t *cbeg, const int32_t *cend,
-                            const int32_t *off, int4 *items) {
-  int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (s >= nsrc) return;
-  int32_t c = src_cells[s];
-  int32_t w = off[s];
-  for (int32_t b = cbeg[c]; b < cend[c]; b += P2M_CHUNK)
-    items[w++] = make_int4((int)s, b, min(b + P2M_CHUNK, cend[c]), 0);
 }
 
 // ---- upward / downward plan ------------------------------------------------------------
@@ -776,36 +750,22 @@ void run_expansions(nbfmm_ctx *c) {
     bm[d] = h[8 + d];
     bl[d] = h[32 + d];
   }
-  unsigned long long *th = (unsigned long long *)(h + 64); // pinned
+  // the term counts reach the host with the M2L class histogram (run_m2l),
+  // in a part of the pinned mirror that it leaves alone
+  unsigned long long *th = (unsigned long long *)((char *)c->hred.p + 32768);
   NB_HIP(hipMemcpyAsync(th, terms, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   // ---- P2M (cells that form their multipole from particles) ----
-  int32_t *pcnt = dget<int32_t>(c->p2m_off, (size_t)nP2M * 2 + 2 + 8);
-  int32_t *poff = pcnt + nP2M + 1;
-  hipLaunchKernelGGL(k_p2m_count, dim3(nblk(nP2M)), dim3(kT), 0, s, (const int32_t *)p2m_cells, (int64_t)nP2M, cbeg,
-                     cend, pcnt);
-  NB_LAUNCH();
-  NB_HIP(hipMemsetAsync(pcnt + nP2M, 0, sizeof(int32_t), s));
-  scan_excl_i32(pcnt, poff, nP2M + 1, c->scan_tmp, s);
-  NB_HIP(hipMemcpyAsync(h + 72, poff + nP2M, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  const int32_t nitems = h[72];
-  P.n_p2m_items = nitems;
-  P.n_p2m_terms = (int64_t)th[0];
-  P.n_l2p_terms = (int64_t)th[1];
-  int4 *items = dget<int4>(c->p2m_items, (size_t)nitems + 1);
-  hipLaunchKernelGGL(k_p2m_items, dim3(nblk(nP2M)), dim3(kT), 0, s, (const int32_t *)p2m_cells, (int64_t)nP2M, cbeg,
-                     cend, poff, items);
-  NB_LAUNCH();
+  P.n_p2m_items = nP2M;
   void *Mexp = dbuf_ensure(c->mexp, (size_t)nM * KP * esz + 16);
   NB_HIP(hipMemsetAsync(Mexp, 0, esz * (size_t)nM * KP, s));
-  if (nitems > 0) {
-    const unsigned nb = (unsigned)((nitems + kT / 64 - 1) / (kT / 64));
+  if (nP2M > 0) {
+    const unsigned nb = (unsigned)((nP2M + kT / 64 - 1) / (kT / 64));
     if (f32)
-      hipLaunchKernelGGL(k_p2m<float>, dim3(nb), dim3(kT), 0, s, p, KP, nitems, items, (const double4 *)c->pos.p,
-                         (const int32_t *)p2m_cells, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (float *)Mexp);
+      hipLaunchKernelGGL(k_p2m<float>, dim3(nb), dim3(kT), 0, s, p, KP, nP2M, (const int32_t *)p2m_cells,
+                         (const double4 *)c->pos.p, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (float *)Mexp);
     else
-      hipLaunchKernelGGL(k_p2m<double>, dim3(nb), dim3(kT), 0, s, p, KP, nitems, items, (const double4 *)c->pos.p,
-                         (const int32_t *)p2m_cells, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (double *)Mexp);
+      hipLaunchKernelGGL(k_p2m<double>, dim3(nb), dim3(kT), 0, s, p, KP, nP2M, (const int32_t *)p2m_cells,
+                         (const double4 *)c->pos.p, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (double *)Mexp);
     NB_LAUNCH();
   }
   // ---- M2M, finest level first ----
@@ -822,6 +782,8 @@ void run_expansions(nbfmm_ctx *c) {
   NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));
   run_m2l(c, KP, f32, ord, p);
   if (!c->p2p_started) launch_p2p(c, 1);
+  P.n_p2m_terms = (int64_t)th[0]; // read back at run_m2l's synchronisation
+  P.n_l2p_terms = (int64_t)th[1];
   // ---- L2L, coarsest level first ----
   for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
   mark(c, 4);

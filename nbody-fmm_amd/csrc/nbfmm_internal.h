@@ -177,7 +177,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf cnt_il, cnt_m2l, cnt_sub, off_il, off_m2l, off_sub, near_pairs;
   nbfmm::DBuf il, m2l_src, m2l_cls, sub_src;
   nbfmm::DBuf src_slot, tgt_slot, src_cells, tgt_cells, is_src;
-  nbfmm::DBuf sub_pairs, p2m_items, p2m_off, tiles;
+  nbfmm::DBuf sub_pairs, tiles;
   nbfmm::DBuf tasks, task_off, task_cnt, dpart;
   // expansions
   nbfmm::DBuf stab, tmat, tmat32, trans, trans32, mode_m, mode_l, mslot, lslot, eslot, up_cells, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
