@@ -394,9 +394,15 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
     if (lane + 64 * r < nmax) L[lane + 64 * r] = (T)((double)L[lane + 64 * r] + s[r]);
 }
 
+// Row blocks per workgroup: the fewest slices of the KP / 16 row blocks that
+// fit the LDS, split evenly (at KP = 144 in FP64: 5 + 4, not 8 + 1, so that
+// no slice reloads the multipoles for a single row block).
 template <class T>
 constexpr int rbw_for(int KP, int lds) {
-  return lds / (16 * KP * (int)sizeof(T)) < KP / 16 ? lds / (16 * KP * (int)sizeof(T)) : KP / 16;
+  const int fit = lds / (16 * KP * (int)sizeof(T)), rb = KP / 16;
+  if (fit >= rb) return rb;
+  const int slices = (rb + fit - 1) / fit;
+  return (rb + slices - 1) / slices;
 }
 
 template <int KP, int RBW, class T>
