@@ -432,8 +432,12 @@ void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, 
   launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
 }
 
+// as rbw_for, for the three-term bf16 operator blocks
 constexpr int rbw_bf3(int KP) {
-  return kLds / (pad32(KP) / 32 * 3072) < KP / 16 ? kLds / (pad32(KP) / 32 * 3072) : KP / 16;
+  const int fit = kLds / (pad32(KP) / 32 * 3072), rb = KP / 16;
+  if (fit >= rb) return rb;
+  const int slices = (rb + fit - 1) / fit;
+  return (rb + slices - 1) / slices;
 }
 
 template <int KP>
