@@ -230,8 +230,9 @@ int nbfmm_eval_accel_checked_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz,
  * array of nsys + 1, offsets[0] = 0); no system acts on another.  All systems
  * are evaluated together, every phase of the FMM in one launch for all: each
  * is scaled by a power of two into its own cell of a grid of isolated cells
- * and the interaction lists stop at those cells.  Every system needs more
- * particles than leaf_capacity.  Device pointers, stream-ordered. */
+ * and the interaction lists stop at those cells.  Systems of any size (one
+ * particle or more) are accepted: the tree is refined at least one level
+ * below the system cells.  Device pointers, stream-ordered. */
 int nbfmm_batch_eval_accel_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets, const double *d_xyz,
                                const double *d_m, double *d_acc);
 /* nbfmm_step_se_dev / nbfmm_step_rk4_dev over a batch (each system advanced

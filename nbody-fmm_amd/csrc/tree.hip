@@ -478,7 +478,10 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
     maxcnt = hm[0];
     P.iterations++;
     P.n_leaves = hm[1];
-  } while (depth < kMaxDepth && maxcnt > thr);
+    // a batch's lists start at level list_level (below its system cells), so
+    // its tree reaches that level even when every system fits in one leaf;
+    // for one system list_level is 1 and the loop is the reference's
+  } while (depth < kMaxDepth && (maxcnt > thr || depth < c->list_level));
   P.geo.depth = depth;
   if (cur != 0) { // keep the sorted keys/indices in buffer 0
     std::swap(c->key[0], c->key[1]);

@@ -75,11 +75,14 @@ def plummer_mix(n, n_spheres=32, seed=7, box=100.0):
     return np.concatenate(xs), np.concatenate(ms)
 
 
-def batched_systems(n_systems=256, seed=11, lo=1e3, hi=6.4e4):
+def batched_systems(n_systems=256, seed=11, lo=1e3, hi=6.4e4, with_velocity=False):
+    """List of (xyz, m) per system, or (xyz, m, vel) with velocities (the
+    positions are the same either way)."""
     rng = np.random.default_rng(seed)
     sizes = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), n_systems))).astype(np.int64)
     a = rng.uniform(0.5, 2.0, n_systems)
-    systems = [plummer(int(sizes[k]), seed=seed * 10000 + k, a=a[k]) for k in range(n_systems)]
+    systems = [plummer(int(sizes[k]), seed=seed * 10000 + k, a=a[k], with_velocity=with_velocity)
+               for k in range(n_systems)]
     return systems
 
 

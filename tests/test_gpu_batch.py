@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import nbfmm
+import orc
 import workloads
 
 pytestmark = pytest.mark.gpu
@@ -114,7 +115,37 @@ def test_batch_errors(ctx):
     for bad in ([1, 500, 1000], [0, 500, 500, 1000], [0]):
         with pytest.raises(nbfmm.NbfmmError):
             ctx.batch_eval_accel_dev(np.array(bad), dx.ptr, dm.ptr, da.ptr)
-    # systems smaller than a leaf: the tree does not reach below the system cells
-    with pytest.raises(nbfmm.NbfmmError) as e:
-        ctx.batch_eval_accel_dev(np.array([0, 5, 10, 15, 20]), dx.ptr, dm.ptr, da.ptr)
-    assert e.value.status == 5
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_small_systems(ctx, precision):
+    # every system fits in one leaf (1 to 15 particles, leaf capacity 16): the
+    # tree is still refined one level below the system cells, and each system
+    # gets its own direct sum (single particles: zero acceleration)
+    rng = np.random.default_rng(21)
+    sizes = [5, 1, 15, 9, 2, 12, 7, 15, 3, 11]
+    x, m, off = _systems([20] * len(sizes), seed=13)
+    xs, ms = [], []
+    for k, nk in enumerate(sizes):
+        s = slice(off[k], off[k] + nk)
+        xs.append(x[s] * rng.uniform(0.5, 3.0))
+        ms.append(m[s])
+    x, m = np.concatenate(xs), np.concatenate(ms)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = len(m)
+    dx, dm, da = nbfmm.DeviceArray(x), nbfmm.DeviceArray(m), nbfmm.DeviceArray(np.zeros((n, 3)))
+    ctx.set_params(order=8, leaf_capacity=16, p2p_cut=256, precision=precision)
+    try:
+        print(f"[batch-small] precision={precision} systems={len(sizes)} n={n}", flush=True)
+        ctx.batch_eval_accel_dev(off, dx.ptr, dm.ptr, da.ptr)
+        a = da.download(ctx.stream)
+        tol = 1e-12 if precision == 0 else 1e-6
+        for k in range(len(sizes)):
+            s = slice(off[k], off[k + 1])
+            if sizes[k] == 1:
+                assert np.all(a[s] == 0.0), k
+                continue
+            ref = orc.direct_targets(x[s], m[s], np.arange(sizes[k]), G=1.0, eps2=0.0)
+            assert _rel(a[s], ref) < tol, (k, _rel(a[s], ref))
+    finally:
+        ctx.set_params(precision=0)

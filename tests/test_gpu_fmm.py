@@ -214,6 +214,36 @@ def test_class_orders_bad_params(ctx):
     ctx.set_params(order=4, class_order_tol=0.0)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("case", ["duplicates", "leaf500"])
+def test_fmm_parity_big_leaves(ctx, case, precision):
+    # P2M of one wavefront per cell over leaves of many particles, and P2P
+    # pairs at zero distance: 40 coincident particles (the depth loop runs to
+    # level 16, the deepest leaf holds all 40), and leaves of 415 to 511
+    # particles (30k uniform, leaf capacity 5000: depth 2)
+    if case == "duplicates":
+        xyz, m = workloads.uniform_cube(3000, seed=4)
+        xyz[:40] = xyz[0]
+        p, thr, cut = 8, 16, 0
+    else:
+        xyz, m = workloads.uniform_cube(30000, seed=23)
+        p, thr, cut = 8, 5000, 0
+    ctx.set_params(order=p, leaf_capacity=thr, p2p_cut=cut, G=1.0, eps2=0.0, precision=precision)
+    print(f"[big-leaves] case={case} precision={precision}", flush=True)
+    try:
+        a = ctx.eval_accel(xyz, m)
+    finally:
+        ctx.set_params(precision=0)
+    b = orc.fmm_exact(xyz, m, p, thr, G=1.0, center_mode=0, p2p_cut=cut)
+    err = _rel(a, b)
+    print(f"[big-leaves] rel L2 vs oracle {err:.3e}")
+    assert err < (1e-12 if precision == 0 else 2e-6)
+    if case == "duplicates":
+        # coincident particles exert nothing on each other: the 40 share one
+        # acceleration, that of the other 2960 particles
+        assert np.allclose(a[:40], a[0], rtol=1e-12 if precision == 0 else 1e-6, atol=0)
+
+
 def test_fmm_parity_plummer(ctx):
     xyz, m = workloads.plummer(40000, seed=1)
     ctx.set_params(order=8, leaf_capacity=64, p2p_cut=512, G=1.0, eps2=0.0)
