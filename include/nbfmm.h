@@ -127,6 +127,25 @@ int nbfmm_direct_accel(nbfmm_ctx *ctx, int64_t n, const double *xyz, const doubl
 int nbfmm_direct_accel_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_m,
                            int64_t ntgt, const int64_t *d_tgt, int mode, double *d_acc);
 
+/* Fixed device memory of nbfmm_direct_accel_noalloc_dev, per GPU, and the
+ * most targets one mode-1 call of it may ask for (their partial sums fill
+ * the array at most). */
+#define NBFMM_NOALLOC_SCRATCH_BYTES ((size_t)48 << 20)
+#define NBFMM_NOALLOC_MAX_TARGETS ((int64_t)1 << 20)
+/* nbfmm_direct_accel_dev without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results (bit for bit) as nbfmm_direct_accel_dev.
+ * Mode 1 keeps its partial sums (nchunk x ntgt x 6 doubles) in a fixed array
+ * of NBFMM_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends,
+ * set up by nbfmm_create; a mode-1 call with more than
+ * NBFMM_NOALLOC_MAX_TARGETS targets is rejected with NBFMM_ERR_ARG (mode 0
+ * uses no scratch and takes any ntgt).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (from any context or stream) race.
+ * Device pointers, stream-ordered on the context's stream. */
+int nbfmm_direct_accel_noalloc_dev(nbfmm_ctx *ctx, int64_t n, const double *d_xyz, const double *d_m,
+                                   int64_t ntgt, const int64_t *d_tgt, int mode, double *d_acc);
+
 /* SyplecticEuler::integrate (symplectic_euler.cpp:8-15), nsteps times:
  * acc = FMM(x); v += dt*acc; x += dt*v.  d_acc receives the last acceleration.
  * Device pointers, stream-ordered. */

@@ -272,6 +272,7 @@ int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
   NB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   NB_HIP(hipEventCreateWithFlags(&c->ev_pos, hipEventDisableTiming));
   for (auto &e : c->ev) NB_HIP(hipEventCreate(&e));
+  scratch_init(c->device); // this device's copy of the fixed scratch (once per device)
   NB_HIP(hipHostMalloc(&c->hred.p, 65536, hipHostMallocDefault));
   c->hred.cap = 65536;
   *out = c;
@@ -344,6 +345,19 @@ int nbfmm_direct_accel_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const d
   check_n(n);
   if (ntgt < 1) throw Error(NBFMM_ERR_ARG, "ntgt must be >= 1");
   run_direct(c, n, d_xyz, d_m, ntgt, d_tgt, mode, d_acc);
+  API_END
+}
+
+int nbfmm_direct_accel_noalloc_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,
+                                   const int64_t *d_tgt, int mode, double *d_acc) {
+  if (!c || !d_xyz || !d_m || !d_acc) return fail(NBFMM_ERR_ARG, "null argument");
+  if (mode != 0 && mode != 1) return fail(NBFMM_ERR_ARG, "mode must be 0 or 1");
+  API_BEGIN
+  check_n(n);
+  if (ntgt < 1) throw Error(NBFMM_ERR_ARG, "ntgt must be >= 1");
+  // the same kernels as nbfmm_direct_accel_dev; the partial sums go to this
+  // device's fixed scratch, so nothing is allocated
+  run_direct(c, n, d_xyz, d_m, ntgt, d_tgt, mode, d_acc, scratch_ptr(c->device));
   API_END
 }
 

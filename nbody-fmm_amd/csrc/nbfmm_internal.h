@@ -251,8 +251,15 @@ void fork_p2p(nbfmm_ctx *c);
 void launch_p2p(nbfmm_ctx *c, int part);
 // joins the P2P stream and writes acc = G * (far + near) in input order
 void run_finish(nbfmm_ctx *c, double *d_acc_out);
+// part_fixed: the partial sums go there (the fixed scratch) instead of a
+// context buffer; sizes beyond it are rejected
 void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt,
-                const int64_t *d_tgt, int mode, double *d_acc);
+                const int64_t *d_tgt, int mode, double *d_acc, double *part_fixed = nullptr);
+// the fixed scratch of the allocation-free direct sum: scratch_init(device)
+// looks up its address on `device` (current), once per device; scratch_ptr
+// returns it
+void scratch_init(int device);
+double *scratch_ptr(int device);
 // batch.hip: the systems [off[k], off[k+1]) of one concatenated set evaluated
 // as one forest; eval(n, x, m, a) runs the evaluation of the placed copy
 void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *h_off, const double *d_xyz, const double *d_m,

@@ -23,6 +23,8 @@
 #include "nbfmm_internal.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace nbfmm {
 namespace {
@@ -589,8 +591,49 @@ void run_finish(nbfmm_ctx *c, double *d_acc_out) {
   NB_LAUNCH();
 }
 
+namespace {
+// The partial sums of the compensated direct sum for callers that cannot
+// allocate (nbfmm_direct_accel_noalloc_dev): a fixed device array, one copy
+// per GPU, kept until the process ends.  Host code never uses the array's
+// name as an address: scratch_ptr(device) is its device address, looked up
+// once per device by scratch_init.
+constexpr size_t kScratchDoubles = (size_t)NBFMM_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// source chunks of the compensated direct sum for ntgt targets over n sources
+int64_t direct_chunks(int64_t n, int64_t ntgt) {
+  const int64_t gx = (ntgt + kT - 1) / kT;
+  // enough chunks to give every CU several blocks
+  int64_t nchunk = (2048 + gx - 1) / gx;
+  if (nchunk > 256) nchunk = 256;
+  if (nchunk > (n + kT - 1) / kT) nchunk = (n + kT - 1) / kT;
+  if (nchunk < 1) nchunk = 1;
+  return nchunk;
+}
+} // namespace
+
+void scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(NBFMM_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return;
+  // the caller has made `device` current; the symbol's address is that GPU's copy
+  void *ptr = nullptr;
+  NB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch[device] = (double *)ptr;
+}
+
+double *scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  double *p = device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+  if (!p) throw Error(NBFMM_ERR_HIP, "the fixed scratch of this device was not set up");
+  return p;
+}
+
 void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, int64_t ntgt, const int64_t *d_tgt,
-                int mode, double *d_acc) {
+                int mode, double *d_acc, double *part_fixed) {
   const unsigned gx = (unsigned)((ntgt + kT - 1) / kT);
   if (mode == 0) {
     hipLaunchKernelGGL(k_direct_naive, dim3(gx), dim3(kT), 0, c->stream, n, d_xyz, d_m, ntgt, d_tgt, c->prm.G,
@@ -598,13 +641,25 @@ void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m,
     NB_LAUNCH();
     return;
   }
-  // enough chunks to give every CU several blocks
-  int64_t nchunk = (2048 + gx - 1) / gx;
-  if (nchunk > 256) nchunk = 256;
-  if (nchunk > (n + kT - 1) / kT) nchunk = (n + kT - 1) / kT;
-  if (nchunk < 1) nchunk = 1;
+  const int64_t nchunk = direct_chunks(n, ntgt);
   const int64_t chunk = (n + nchunk - 1) / nchunk;
-  double *part = dget<double>(c->dpart, (size_t)nchunk * (size_t)ntgt * 6);
+  const size_t need = (size_t)nchunk * (size_t)ntgt * 6;
+  double *part;
+  if (part_fixed) {
+    // every entry the combine reads is written by k_direct_part first: nothing to clear
+    if (ntgt > NBFMM_NOALLOC_MAX_TARGETS)
+      throw Error(NBFMM_ERR_ARG, "the allocation-free direct sum takes at most " +
+                                     std::to_string(NBFMM_NOALLOC_MAX_TARGETS) + " targets in mode 1, got " +
+                                     std::to_string(ntgt));
+    // (up to NBFMM_NOALLOC_MAX_TARGETS targets the partial sums fit: nchunk <= 2^19 / ntgt + 1 chunks)
+    if (need > kScratchDoubles)
+      throw Error(NBFMM_ERR_ARG, "the direct sum of " + std::to_string(ntgt) + " targets needs " +
+                                     std::to_string(need * sizeof(double)) + " bytes of partial sums, more than the " +
+                                     std::to_string((size_t)NBFMM_NOALLOC_SCRATCH_BYTES) + " of the fixed scratch");
+    part = part_fixed;
+  } else {
+    part = dget<double>(c->dpart, need);
+  }
   hipLaunchKernelGGL(k_direct_part, dim3(gx, (unsigned)nchunk), dim3(kT), 0, c->stream, n, d_xyz, d_m, ntgt, d_tgt,
                      c->prm.eps2, chunk, part);
   NB_LAUNCH();

@@ -85,7 +85,10 @@ EXPORTS = [
     "nbfmm_order_dataset_get", "nbfmm_order_dataset_put", "nbfmm_order_train", "nbfmm_order_grad",
     "nbfmm_order_predict", "nbfmm_eval_accel_checked_dev",
     "nbfmm_batch_eval_accel_dev", "nbfmm_batch_step_se_dev", "nbfmm_batch_step_rk4_dev",
+    "nbfmm_direct_accel_noalloc_dev",
 ]
+NOALLOC_SCRATCH_BYTES = 48 << 20
+NOALLOC_MAX_TARGETS = 1 << 20
 
 DIAG_NAMES = ["kinetic", "potential", "total", "px", "py", "pz", "lx", "ly", "lz", "mass"]
 
@@ -106,6 +109,7 @@ def lib():
         for f in ("nbfmm_eval_accel", "nbfmm_eval_accel_dev", "nbfmm_direct_accel"):
             getattr(L, f).argtypes = [vp, i64, vp, vp, vp]
         L.nbfmm_direct_accel_dev.argtypes = [vp, i64, vp, vp, i64, vp, ctypes.c_int, vp]
+        L.nbfmm_direct_accel_noalloc_dev.argtypes = [vp, i64, vp, vp, i64, vp, ctypes.c_int, vp]
         L.nbfmm_step_se_dev.argtypes = [vp, i64, vp, vp, vp, vp, dbl, ctypes.c_int]
         L.nbfmm_step_rk4_dev.argtypes = [vp, i64, vp, vp, vp, vp, dbl, ctypes.c_int, ctypes.c_int]
         L.nbfmm_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
@@ -215,6 +219,10 @@ class Context:
 
     def direct_accel_dev(self, n, d_xyz, d_m, ntgt, d_tgt, mode, d_acc):
         _check(lib().nbfmm_direct_accel_dev(self.h, n, d_xyz, d_m, ntgt, d_tgt, mode, d_acc))
+
+    def direct_accel_noalloc_dev(self, n, d_xyz, d_m, ntgt, d_tgt, mode, d_acc):
+        """As direct_accel_dev, allocating no device memory (fixed per-GPU scratch)."""
+        _check(lib().nbfmm_direct_accel_noalloc_dev(self.h, n, d_xyz, d_m, ntgt, d_tgt, mode, d_acc))
 
     def step_se_dev(self, n, dx, dv, da, dm, dt, nsteps):
         _check(lib().nbfmm_step_se_dev(self.h, n, dx, dv, da, dm, dt, nsteps))
