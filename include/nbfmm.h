@@ -262,7 +262,12 @@ int nbfmm_batch_step_rk4_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offset
                              double *d_acc, const double *d_m, double dt, int nsteps, int mode);
 
 /* Statistics of the last evaluation: sizes for the roofline and per-phase
- * GPU times measured with HIP events on the context's stream. */
+ * GPU times measured with HIP events on the context's stream.  The phases
+ * follow each other on that stream; P2P runs on a second stream beside them
+ * (beside the up-pass and after the first M2L contraction), so with the
+ * default timing a phase's time includes waiting for CUs that P2P holds.
+ * With nbfmm_set_timing(ctx, 2) the phases, P2P included, run one after
+ * another, and each time is its kernels' own. */
 typedef struct nbfmm_stats {
   int32_t depth;            /* leaf depth of the reference tree rule */
   int32_t depth_iterations; /* iterations of the depth loop (tree.cpp:43-69) */
@@ -277,21 +282,29 @@ typedef struct nbfmm_stats {
   int64_t n_l2p_terms;      /* particle-cell pairs of the L2P passes */
   double ms_tree;           /* bbox, depth loop, sort, levels, neighbour tables */
   double ms_lists;          /* interaction lists, classification, P2P lists */
-  double ms_p2m;
-  double ms_m2l;            /* including the per-class operator build */
+  double ms_p2m;            /* up-pass planning, operator build, P2M */
+  double ms_m2l;            /* M2L: class schedule, contraction, per-target sums */
   double ms_l2p;
-  double ms_p2p;
+  double ms_p2p;            /* waiting for P2P, then acc = G (far + near) */
   double ms_total;          /* first to last event of the evaluation */
-  double ms_m2l_kernel;     /* the MFMA M2L contraction kernel alone */
-  double ms_p2p_kernel;     /* the P2P kernel alone */
+  double ms_m2l_kernel;     /* the first pass's MFMA M2L contraction launches */
+  double ms_p2p_kernel;     /* the P2P launches (on the second stream) */
   double flop_m2l;          /* algorithmic M2L flops: 2 (q+1)^4 per M2L pair of order q */
   double flop_p2p;          /* algorithmic P2P flops: 20 per particle pair */
   double mean_order;        /* mean order over the M2L pairs */
   int32_t fallback;         /* nbfmm_eval_accel_checked_dev fell back to the fixed order */
-  int32_t reserved0;
+  int32_t isolated;         /* the phases ran one after another (timing mode 2) */
+  int64_t n_p2m_cells;      /* cells forming their multipole from particles */
+  int64_t n_m2m_cells;      /* cells forming their multipole from their children */
+  int64_t n_l2l_cells;      /* cells pushing their local to their children */
+  int64_t n_l2p_cells;      /* cells evaluating their local at their particles */
+  double ms_m2m;            /* M2M, every level */
+  double ms_l2l;            /* L2L, every level */
 } nbfmm_stats;
 int nbfmm_get_stats(nbfmm_ctx *ctx, nbfmm_stats *stats);
-/* Enables (1) or disables (0) the per-phase events (on by default). */
+/* Per-phase events: 0 off, 1 on (the default), 2 on with the phases run one
+ * after another (P2P alone after L2P) so that each phase is timed alone;
+ * mode 2 is for measurement and is slower than mode 1. */
 int nbfmm_set_timing(nbfmm_ctx *ctx, int enabled);
 /* Keeps (1) the full interaction lists of later evaluations for
  * nbfmm_tree_get (off by default: only the lists the evaluation needs are built). */

@@ -87,16 +87,28 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
   S.n_p2p_pairs = P.n_p2p_pairs;
   S.n_p2m_terms = P.n_p2m_terms;
   S.n_l2p_terms = P.n_l2p_terms;
+  S.n_p2m_cells = P.n_p2m_cells;
+  S.n_m2m_cells = P.n_m2m_cells;
+  S.n_l2l_cells = P.n_l2l_cells;
+  S.n_l2p_cells = P.n_l2p_cells;
+  S.isolated = c->isolated ? 1 : 0;
   if (c->timing) {
     NB_HIP(hipEventSynchronize(c->ev[6]));
-    float t[7] = {0};
-    for (int k = 1; k <= 6; ++k) NB_HIP(hipEventElapsedTime(&t[k], c->ev[k - 1], c->ev[k]));
-    S.ms_tree = t[1];
-    S.ms_lists = t[2];
-    S.ms_p2m = t[3];
-    S.ms_m2l = t[4];
-    S.ms_l2p = t[5];
-    S.ms_p2p = t[6];
+    auto span = [&](int a, int b) {
+      float t = 0;
+      NB_HIP(hipEventElapsedTime(&t, c->ev[a], c->ev[b]));
+      return (double)t;
+    };
+    // event order on the main stream: 0 tree 1 lists 2 up-pass 14 M2M 3 M2L
+    // 13 L2L 4 L2P 5 finish 6
+    S.ms_tree = span(0, 1);
+    S.ms_lists = span(1, 2);
+    S.ms_p2m = span(2, 14);
+    S.ms_m2m = span(14, 3);
+    S.ms_m2l = span(3, 13);
+    S.ms_l2l = span(13, 4);
+    S.ms_l2p = span(4, 5);
+    S.ms_p2p = span(5, 6);
     float tot = 0;
     NB_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[6]));
     S.ms_total = tot;
@@ -305,7 +317,9 @@ void *nbfmm_stream(nbfmm_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 int nbfmm_set_timing(nbfmm_ctx *c, int enabled) {
   if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  if (enabled < 0 || enabled > 2) return fail(NBFMM_ERR_ARG, "timing mode must be 0, 1 or 2");
   c->timing = enabled != 0;
+  c->isolated = enabled == 2;
   return NBFMM_OK;
 }
 

@@ -659,12 +659,15 @@ void run_expansions(nbfmm_ctx *c) {
   double *pfar = pot ? grad + 3 * (size_t)P.n : nullptr;
   if (P.n_tgt == 0) {
     c->n_local = 0;
+    P.n_p2m_cells = P.n_m2m_cells = P.n_l2l_cells = P.n_l2p_cells = 0;
     NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s));
     P.n_p2m_terms = P.n_l2p_terms = 0;
     fork_p2p(c);
     launch_p2p(c, 0);
     launch_p2p(c, 1);
+    mark(c, 14);
     mark(c, 3);
+    mark(c, 13);
     mark(c, 4);
     mark(c, 5);
     return;
@@ -769,6 +772,7 @@ void run_expansions(nbfmm_ctx *c) {
     NB_LAUNCH();
   }
   // ---- M2M, finest level first ----
+  mark(c, 14);
   for (int d = D - 1; d >= 0; --d)
     run_trans(c, false, KP, f32, bm[d + 1] - bm[d], m2m_cells + bm[d]);
   mark(c, 3);
@@ -784,6 +788,11 @@ void run_expansions(nbfmm_ctx *c) {
   if (!c->p2p_started) launch_p2p(c, 1);
   P.n_p2m_terms = (int64_t)th[0]; // read back at run_m2l's synchronisation
   P.n_l2p_terms = (int64_t)th[1];
+  P.n_p2m_cells = nP2M;
+  P.n_m2m_cells = h[2];        // cells forming their multipole by M2M
+  P.n_l2l_cells = h[4];        // cells pushing their local by L2L
+  P.n_l2p_cells = h[3] - h[4]; // cells evaluating their local at their particles
+  mark(c, 13);
   // ---- L2L, coarsest level first ----
   for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
   mark(c, 4);

@@ -140,6 +140,7 @@ struct Plan {
   int64_t n_src = 0, n_tgt = 0, n_p2p_pairs = 0, n_p2m_terms = 0, n_l2p_terms = 0;
   int64_t n_p2m_items = 0, n_m2l_tiles = 0, n_tasks = 0;
   int64_t n_tasks_narrow = 0; // tasks [0, n_tasks_narrow) have blocks of at most p2p_narrow_max targets
+  int64_t n_p2m_cells = 0, n_m2m_cells = 0, n_l2l_cells = 0, n_l2p_cells = 0; // cells of each pass
   int kp = 0; // padded coefficient count (multiple of 16)
   double flop_m2l = 0; // algorithmic M2L flops of the evaluation
   double mean_order = 0; // mean order over the M2L pairs (the fixed order without selection)
@@ -158,6 +159,9 @@ struct nbfmm_ctx {
   int lds_max = 65536; // LDS bytes one workgroup may allocate on this device
   int n_cu = 256;      // compute units of this device
   bool timing = true;
+  // phases one after another (P2P alone at the end, after L2P) so that each
+  // phase's events time its kernels alone; the results are the same
+  bool isolated = false;
   bool keep_lists = false;
   int nbr_leaf = 0; // rows of the leaf level in nbr27: 0 none, 1 leaves that are not closed, 2 all
   double *phi_out = nullptr; // potential output of the current evaluation (device, input order) or null

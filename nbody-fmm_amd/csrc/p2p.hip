@@ -60,8 +60,6 @@ __device__ __forceinline__ int32_t source_at(const int32_t *lb, const int32_t *l
 }
 static_assert(kTaskTargets == 4 * kW, "a P2P task holds at most four targets per lane");
 
-// A particle meets itself (and coincident particles meet each other) with
-// r2 = 0; those pairs contribute nothing (a select, no branch).
 // 1/sqrt(r2) for positive finite r2: v_rsq_f64 estimate y refined by the
 // third-order step y + y e (1/2 + 3e/8), e = 1 - r2 y^2 (the Taylor series of
 // (1 - e)^(-1/2)), whose error is O(e^3).
@@ -71,13 +69,16 @@ __device__ __forceinline__ double rsqrt_pos(double r2) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
-// FP64: positions as stored, FP64 sums.
+// FP64: positions as stored, FP64 sums.  A particle meets itself (and
+// coincident particles meet each other) with r2 = 0: those pairs are
+// dropped by a select.
 struct PairF64 {
   typedef double4 S;
   typedef double A;
   static constexpr bool kAnchor = false;
   static constexpr int kGroup = NBFMM_P2P_GROUP_F64; // targets per pass (3 or 4 FP64 sums each in registers)
   static __device__ __forceinline__ S make(const double4 &p, const double4 &) { return p; }
+  static __device__ __forceinline__ S tgt(const double4 &v) { return v; }
   static __device__ __forceinline__ void massless(S &v) { v.w = 0.0; }
   template <bool POT>
   static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
@@ -94,10 +95,28 @@ struct PairF64 {
   static __device__ __forceinline__ void add(A &s, A &, A v) { s += v; }
 };
 
+// FP64 for sources in cells that do not touch the target cell: r > 0 for
+// every pair, so no select.
+struct PairF64Far : PairF64 {
+  template <bool POT>
+  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
+    const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double inv = rsqrt_pos(r2);
+    const double mi = xj.w * inv;
+    const double w = mi * inv * inv;
+    ax -= dx * w;
+    ay -= dy * w;
+    az -= dz * w;
+    if (POT) ap += mi;
+  }
+};
+
 // FP32 (mixed precision): offsets from an anchor (the mean of the task's
 // targets) formed in FP64 and kept as FP32 pairs hi + lo; the separation
 // (xi_hi - xj_hi) + (xi_lo - xj_lo) is accurate to a few FP32 ulps of the
-// separation itself however large the offsets are.  v_rsq_f32.  A chunk's
+// separation itself however large the offsets are, which the close pairs of
+// the target's own and adjacent cells need.  v_rsq_f32.  A chunk's
 // contributions (at most 64 sources) are summed in FP32 and the chunk sums
 // added with Kahan compensation into an FP32 (sum, correction) pair.
 struct f2pos {
@@ -115,6 +134,7 @@ struct PairF32 {
     return {make_float4(hx, hy, hz, (float)p.w),
             make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.0f)};
   }
+  static __device__ __forceinline__ S tgt(const f2pos &v) { return v; }
   static __device__ __forceinline__ void massless(S &v) { v.hi.w = 0.0f; }
   template <bool POT>
   static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
@@ -138,6 +158,108 @@ struct PairF32 {
   }
 };
 
+// FP32 for sources in cells that do not touch the target cell (separated
+// from every target by at least the target cell's edge h, while every target
+// lies within sqrt(3) h of the anchor): single FP32 offsets, whose rounding
+// moves a separation r >= h by at most 2^-24 (|x_s - a| + sqrt(3) h) <=
+// 2^-24 (r + 2 sqrt(3) h), a relative 2.7e-7 at worst; and r > 0, so no
+// select.  9 vector operations per pair fewer than PairF32.
+struct PairF32Far {
+  typedef float4 S;
+  typedef float A;
+  static __device__ __forceinline__ S make(const double4 &p, const double4 &a) {
+    return make_float4((float)(p.x - a.x), (float)(p.y - a.y), (float)(p.z - a.z), (float)p.w);
+  }
+  static __device__ __forceinline__ S tgt(const f2pos &v) { return v.hi; }
+  static __device__ __forceinline__ void massless(S &v) { v.w = 0.0f; }
+  template <bool POT>
+  static __device__ __forceinline__ void pair(const S &xi, const S &xj, A &ax, A &ay, A &az, A &ap) {
+    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float mi = xj.w * inv;
+    const float w = mi * inv * inv;
+    ax -= dx * w;
+    ay -= dy * w;
+    az -= dz * w;
+    if (POT) ap += mi;
+  }
+};
+
+// The task's list cells in evaluation order: lb[k] first particle, lp[k]
+// first concatenated index of cell k; returns the number of concatenated
+// sources, and in near_end the first concatenated index of the sources in
+// cells that do not touch the target cell (those run with the far pair).
+//  - a substituted interaction list (list >= 0): no cell touches the target;
+//  - a leaf's own 27-neighbour row: every cell touches it;
+//  - the 27-neighbour row of the parent P of a closed cell A: the eight
+//    cells touching A (per axis, offset 0 or towards A's octant) go first,
+//    the nineteen others (at least one edge of A away) after them.
+// tmp: 2 x 27 LDS words.  One wavefront; ends with a barrier.
+__device__ __forceinline__ int32_t stage_list(const P2PTask &tk, const int32_t *__restrict__ nbr27,
+                                              const int32_t *__restrict__ sub_src, const int32_t *__restrict__ cbeg,
+                                              const int32_t *__restrict__ cend, const uint64_t *__restrict__ ckey,
+                                              int32_t *lb, int32_t *lp, int32_t *tmp, int lane, int32_t &near_end) {
+  const int ne = tk.nlist;
+  if (tk.list < 0 && -tk.list - 1 != tk.tcell) {
+    // parent row of a closed cell: near cells first
+    const int32_t *row = nbr27 + (int64_t)(-tk.list - 1) * 27;
+    const uint32_t oct = (uint32_t)(ckey[tk.tcell] & 7u); // A's octant: bit d set = upper half along axis d
+    const int32_t q = lane < 27 ? row[lane] : -1;
+    const int32_t sb = q >= 0 ? cbeg[q] : 0;
+    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
+    bool near = true;
+    for (int d = 0; d < 3; ++d) {
+      const int od = (d == 0 ? lane % 3 : d == 1 ? (lane / 3) % 3 : lane / 9) - 1;
+      near = near && (od == 0 || od == (int)((oct >> d) & 1u) * 2 - 1);
+    }
+    const uint64_t nm = __ballot(lane < 27 && near), fm = __ballot(lane < 27 && !near);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int nn = __popcll(nm);
+    const int slot = near ? __popcll(nm & below) : nn + __popcll(fm & below);
+    if (lane < 27) {
+      tmp[slot] = sb;
+      tmp[27 + slot] = cnt;
+    }
+    __syncthreads();
+    const int32_t c2 = lane < 27 ? tmp[27 + lane] : 0;
+    int32_t x = c2;
+    for (int off = 1; off < kW; off <<= 1) {
+      const int32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane < 27) {
+      lb[lane] = tmp[lane];
+      lp[lane] = x - c2;
+    }
+    near_end = __shfl(x, nn - 1);
+    const int32_t total = __shfl(x, kW - 1);
+    __syncthreads();
+    return total;
+  }
+  const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
+  int32_t carry = 0;
+  for (int base = 0; base < ne; base += kW) {
+    const int e = base + lane;
+    const int32_t q = e < ne ? list[e] : -1;
+    const int32_t sb = q >= 0 ? cbeg[q] : 0;
+    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
+    int32_t x = cnt;
+    for (int off = 1; off < kW; off <<= 1) {
+      const int32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (e < ne) {
+      lb[e] = sb;
+      lp[e] = carry + x - cnt;
+    }
+    carry += __shfl(x, kW - 1);
+  }
+  near_end = tk.list >= 0 ? 0 : carry;
+  __syncthreads();
+  return carry;
+}
+
 // Lanes take sources, targets are unrolled: the task's targets are staged
 // in LDS and, kGroup at a time, every lane keeps one sum per target in
 // registers while it walks its sources (concatenated index lane, lane + 64,
@@ -145,21 +267,26 @@ struct PairF32 {
 // the number of targets (blocks hold 1 to a few dozen particles against
 // thousands of sources), and a target's partial sums meet once, in an FP64
 // reduction across the wavefront.  A lane's sum runs over at most
-// kTaskPairs / 64 sources.  One wavefront per task (tasks blockIdx.x,
-// blockIdx.x + gridDim.x, ... when the grid is smaller than the task list).
-// POT also sums m_j / r_ij into pot.
-template <class Tr, bool POT>
-__global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
+// kTaskPairs / 64 sources.  The sources in cells touching the target cell
+// run with the pair Tr, the others with TrFar, in two loops one after the
+// other; both keep the kernel at 128 registers (four wavefronts per SIMD).  One wavefront per task
+// (tasks blockIdx.x, blockIdx.x + gridDim.x, ... when the grid is smaller
+// than the task list).  POT also sums m_j / r_ij into pot.
+template <class Tr, class TrFar, bool POT>
+__global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3 : 4))) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                                                  const uint64_t *__restrict__ ckey,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad,
                                                   double *__restrict__ pot) {
   typedef typename Tr::S S;
+  typedef typename TrFar::S FS;
   typedef typename Tr::A A;
   __shared__ S st[kTaskTargets + Tr::kGroup]; // the last pass reads up to kGroup - 1 slots past nc
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
+  __shared__ int32_t tmp[54];
   const int lane = threadIdx.x;
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
     __syncthreads(); // the previous task's reads of lb, lp, st are done
@@ -183,60 +310,78 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *
       anchor.z /= nc;
     }
     for (int t = lane; t < nc; t += kW) st[t] = Tr::make(pos[b + t], anchor);
-    const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
+    int32_t near_end;
+    const int32_t total = stage_list(tk, nbr27, sub_src, cbeg, cend, ckey, lb, lp, tmp, lane, near_end);
     const int ne = tk.nlist;
-    // first concatenated index of every list cell
-    int32_t carry = 0;
-    for (int base = 0; base < ne; base += kW) {
-      const int e = base + lane;
-      const int32_t q = e < ne ? list[e] : -1;
-      const int32_t sb = q >= 0 ? cbeg[q] : 0;
-      const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
-      int32_t x = cnt;
-      for (int off = 1; off < kW; off <<= 1) {
-        const int32_t y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-      }
-      if (e < ne) {
-        lb[e] = sb;
-        lp[e] = carry + x - cnt;
-      }
-      carry += __shfl(x, kW - 1);
-    }
-    const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
-    __syncthreads();
+    const int32_t f0 = tk.f0, f1 = min(tk.f1, total);
+    const int32_t fm = max(f0, min(f1, near_end)); // [f0, fm) near cells, [fm, f1) the others
     constexpr int kGroup = Tr::kGroup;
     for (int t0 = 0; t0 < nc; t0 += kGroup) {
       const int ng = min(kGroup, nc - t0);
       A ax[kGroup], ay[kGroup], az[kGroup], ap[kGroup];
 #pragma unroll
       for (int t = 0; t < kGroup; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0;
-      // a lane takes sources g and g + 64 (g = f0 + lane, + 128, ...): every
-      // target read serves two pairs; c1, c2 follow their list cells
-      int c1 = first_cell(lp, ne, f0 + lane), c2 = first_cell(lp, ne, f0 + lane + kW);
-      int32_t g = f0 + lane;
-      double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
-      if (g < f1) p1 = pos[source_at(lb, lp, ne, c1, g)];
-      if (g + kW < f1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
-      for (; g < f1; g += 2 * kW) {
-        const double4 q1 = p1, q2 = p2;
-        const bool has2 = g + kW < f1;
-        // the next sources' loads are in flight while these are evaluated
-        if (g + 2 * kW < f1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
-        if (g + 3 * kW < f1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
-        const S x1 = Tr::make(q1, anchor);
-        S x2 = Tr::make(has2 ? q2 : q1, anchor);
-        if (!has2) Tr::massless(x2); // contributes nothing
+      {
+        // a lane takes sources g and g + 64 (g = f0 + lane, + 128, ...): every
+        // target read serves two pairs; c1, c2 follow their list cells
+        const int32_t g0 = f0, g1 = fm;
+        int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
+        int32_t g = g0 + lane;
+        double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
+        if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
+        if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
+        for (; g < g1; g += 2 * kW) {
+          const double4 q1 = p1, q2 = p2;
+          const bool has2 = g + kW < g1;
+          // the next sources' loads are in flight while these are evaluated
+          if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
+          if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
+          const S x1 = Tr::make(q1, anchor);
+          S x2 = Tr::make(has2 ? q2 : q1, anchor);
+          if (!has2) Tr::massless(x2); // contributes nothing
 #pragma unroll
-        for (int t4 = 0; t4 < kGroup; t4 += 4)
-          if (t4 < ng) // four targets a branch (sums of slots past ng are never used)
+          for (int t4 = 0; t4 < kGroup; t4 += 4)
+            if (t4 < ng) // four targets a branch (sums of slots past ng are never used)
 #pragma unroll
-            for (int t = t4; t < t4 + 4; ++t) {
-              const S xt = st[t0 + t];
-              Tr::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
-              Tr::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
-            }
+              for (int t = t4; t < t4 + 4; ++t) {
+                const S xt = Tr::tgt(st[t0 + t]);
+                Tr::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
+                Tr::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
+              }
+        }
       }
+      // the far loop's set-up (cell search, first loads) stays after the near
+      // loop: moved ahead of it, its registers would be held through it
+      asm volatile("" ::: "memory");
+      {
+        // the same over the sources of the cells that do not touch the target cell
+        const int32_t g0 = fm, g1 = f1;
+        int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
+        int32_t g = g0 + lane;
+        double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
+        if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
+        if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
+        for (; g < g1; g += 2 * kW) {
+          const double4 q1 = p1, q2 = p2;
+          const bool has2 = g + kW < g1;
+          // the next sources' loads are in flight while these are evaluated
+          if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
+          if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
+          const FS x1 = TrFar::make(q1, anchor);
+          FS x2 = TrFar::make(has2 ? q2 : q1, anchor);
+          if (!has2) TrFar::massless(x2); // contributes nothing
+#pragma unroll
+          for (int t4 = 0; t4 < kGroup; t4 += 4)
+            if (t4 < ng) // four targets a branch (sums of slots past ng are never used)
+#pragma unroll
+              for (int t = t4; t < t4 + 4; ++t) {
+                const FS xt = TrFar::tgt(st[t0 + t]);
+                TrFar::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
+                TrFar::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
+              }
+        }
+      }
+
 #pragma unroll
       for (int t = 0; t < kGroup; ++t) {
         if (t >= ng) continue;
@@ -260,25 +405,73 @@ __global__ void __launch_bounds__(kW) k_p2p_tasks(int64_t ntask, const P2PTask *
   }
 }
 
+// One stretch of k_p2p_wide over the sources [g0, g1): staged 64 at a time in
+// LDS (each lane finds its source's cell by binary search) and read once per
+// 4 pairs; a chunk's contributions are summed per lane and added with the
+// precision's add (Kahan in FP32).
+template <class Pr, class Ad, bool POT, class TS>
+__device__ __forceinline__ void wide_sweep(int32_t g0, int32_t g1, const int32_t *lb, const int32_t *lp, int ne,
+                                           const double4 *__restrict__ pos, const double4 &anchor,
+                                           typename Pr::S *sh, const TS (&xi)[4], int si, int sub, int lane,
+                                           typename Pr::A (&sx)[4], typename Pr::A (&sy)[4], typename Pr::A (&sz)[4],
+                                           typename Pr::A (&sp)[4], typename Pr::A (&cx)[4],
+                                           typename Pr::A (&cy)[4], typename Pr::A (&cz)[4],
+                                           typename Pr::A (&cp)[4]) {
+  typedef typename Pr::S S;
+  typedef typename Pr::A A;
+  for (int32_t fb = g0; fb < g1; fb += kW) {
+    const int32_t g = fb + lane;
+    __syncthreads();
+    if (g < g1) {
+      int lo = 0, hi = ne; // last list cell with lp <= g (lp[0] = 0 <= g)
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lp[mid] <= g)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      sh[lane] = Pr::make(pos[lb[lo] + (g - lp[lo])], anchor);
+    }
+    __syncthreads();
+    const int cnt = min(kW, g1 - fb);
+    A ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0}, ap[4] = {0, 0, 0, 0};
+    for (int k = si; k < cnt; k += sub) {
+      const S xj = sh[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Pr::template pair<POT>(Pr::tgt(xi[q]), xj, ax[q], ay[q], az[q], ap[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      Ad::add(sx[q], cx[q], ax[q]);
+      Ad::add(sy[q], cy[q], ay[q]);
+      Ad::add(sz[q], cz[q], az[q]);
+      if (POT) Ad::add(sp[q], cp[q], ap[q]);
+    }
+  }
+}
+
 // The kernel for blocks of many targets (more than kNarrowMax: one pass over
 // the sources instead of several): lanes are tl target lanes (power of two,
 // 4 targets each: t = ti + tl q) times 64 / tl source slots; the sources are
-// staged 64 at a time in LDS (each lane finds its source's cell by binary
-// search) and read once per 4 pairs; a chunk's contributions are summed per
-// lane and added with the precision's add (Kahan in FP32), and the slots are
-// combined with xor-shuffles at the end.  One wavefront per task.
-template <class Tr, bool POT>
+// walked by wide_sweep, the cells touching the target cell with the pair Tr
+// and the others with TrFar, and the slots are combined with xor-shuffles at
+// the end.  One wavefront per task.
+template <class Tr, class TrFar, bool POT>
 __global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                                                  const uint64_t *__restrict__ ckey,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad,
                                                   double *__restrict__ pot) {
   typedef typename Tr::S S;
   typedef typename Tr::A A;
   __shared__ S sh[kW];
+  __shared__ typename TrFar::S shf[kW];
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
+  __shared__ int32_t tmp[54];
   const int lane = threadIdx.x;
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
   __syncthreads(); // the previous task's reads of lb, lp, sh are done
@@ -301,27 +494,11 @@ __global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *_
     anchor.y /= nc;
     anchor.z /= nc;
   }
-  const int32_t *list = tk.list >= 0 ? sub_src + tk.list : nbr27 + (int64_t)(-tk.list - 1) * 27;
+  int32_t near_end;
+  const int32_t total = stage_list(tk, nbr27, sub_src, cbeg, cend, ckey, lb, lp, tmp, lane, near_end);
   const int ne = tk.nlist;
-  // first concatenated index of every list cell
-  int32_t carry = 0;
-  for (int base = 0; base < ne; base += kW) {
-    const int e = base + lane;
-    const int32_t q = e < ne ? list[e] : -1;
-    const int32_t sb = q >= 0 ? cbeg[q] : 0;
-    const int32_t cnt = q >= 0 ? cend[q] - sb : 0;
-    int32_t x = cnt;
-    for (int off = 1; off < kW; off <<= 1) {
-      const int32_t y = __shfl_up(x, off);
-      if (lane >= off) x += y;
-    }
-    if (e < ne) {
-      lb[e] = sb;
-      lp[e] = carry + x - cnt;
-    }
-    carry += __shfl(x, kW - 1);
-  }
-  const int32_t f0 = tk.f0, f1 = min(tk.f1, carry);
+  const int32_t f0 = tk.f0, f1 = min(tk.f1, total);
+  const int32_t fm = max(f0, min(f1, near_end)); // [f0, fm) near cells, [fm, f1) the others
   int tl = 1;
   while (tl * 4 < nc) tl <<= 1;
   const int sub = kW / tl, ti = lane & (tl - 1), si = lane / tl;
@@ -335,37 +512,9 @@ __global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *_
   }
   A sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0}, sp[4] = {0, 0, 0, 0};
   A cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0}, cz[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
-  for (int32_t fb = f0; fb < f1; fb += kW) {
-    // stage sources [fb, fb + 64) of the concatenation
-    const int32_t g = fb + lane;
-    __syncthreads();
-    if (g < f1) {
-      int lo = 0, hi = ne; // last list cell with lp <= g (lp[0] = 0 <= g)
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (lp[mid] <= g)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      sh[lane] = Tr::make(pos[lb[lo] + (g - lp[lo])], anchor);
-    }
-    __syncthreads();
-    const int cnt = min(kW, f1 - fb);
-    A ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0}, ap[4] = {0, 0, 0, 0};
-    for (int k = si; k < cnt; k += sub) {
-      const S xj = sh[k];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) Tr::template pair<POT>(xi[q], xj, ax[q], ay[q], az[q], ap[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      Tr::add(sx[q], cx[q], ax[q]);
-      Tr::add(sy[q], cy[q], ay[q]);
-      Tr::add(sz[q], cz[q], az[q]);
-      if (POT) Tr::add(sp[q], cp[q], ap[q]);
-    }
-  }
+  wide_sweep<Tr, Tr, POT>(f0, fm, lb, lp, ne, pos, anchor, sh, xi, si, sub, lane, sx, sy, sz, sp, cx, cy, cz, cp);
+  wide_sweep<TrFar, Tr, POT>(fm, f1, lb, lp, ne, pos, anchor, shf, xi, si, sub, lane, sx, sy, sz, sp, cx, cy, cz,
+                             cp);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     double gx = (double)sx[q] + (double)cx[q], gy = (double)sy[q] + (double)cy[q],
@@ -532,6 +681,9 @@ void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
 #endif
 
 void launch_p2p(nbfmm_ctx *c, int part) {
+  // isolated timing: parts 0 and 1 are left out and run_finish runs every
+  // task (part 2) once the main stream is idle
+  if (c->isolated && part != 2) return;
   Plan &P = c->plan;
   hipStream_t s2 = c->stream2;
   const bool pot = c->phi_out != nullptr;
@@ -542,7 +694,18 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   const bool f32 = c->prm.precision == NBFMM_MIXED;
   const double early = f32 ? NBFMM_P2P_EARLY : NBFMM_P2P_EARLY_F64;
   const int64_t sn = (int64_t)((double)nn * early), sw = nn + (int64_t)((double)nw * early);
-  const int64_t r[2][2] = {{part == 0 ? 0 : sn, part == 0 ? sn : nn}, {part == 0 ? nn : sw, part == 0 ? sw : P.n_tasks}};
+  int64_t r[2][2] = {{part == 0 ? 0 : sn, part == 0 ? sn : nn}, {part == 0 ? nn : sw, part == 0 ? sw : P.n_tasks}};
+  if (part == 2) { // every task, one launch per kernel
+    r[0][0] = 0;
+    r[0][1] = nn;
+    r[1][0] = nn;
+    r[1][1] = P.n_tasks;
+    NB_HIP(hipEventRecord(c->ev_fork2, c->stream));
+    NB_HIP(hipStreamWaitEvent(s2, c->ev_fork2, 0));
+    NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
+    mark2(c, 9);
+    mark2(c, 10);
+  }
   if (part == 1) {
     // after the point of the main stream where part 1 is enqueued (the first M2L contraction)
     NB_HIP(hipEventRecord(c->ev_fork2, c->stream));
@@ -553,7 +716,8 @@ void launch_p2p(nbfmm_ctx *c, int part) {
     // near-field gradient (3 per particle) and, with potentials, near-field sums of m / r
     NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
   }
-  mark2(c, part == 0 ? 9 : 11);
+  if (part != 2) mark2(c, part == 0 ? 9 : 11);
+  if (part == 2) mark2(c, 11);
   // part 1 in mixed precision starts with the wide tasks, so that its last
   // workgroups are short narrow tasks (measured 0.2 ms sooner at 4M; FP64,
   // whose narrow tasks are longer, measured slower that way)
@@ -565,22 +729,23 @@ void launch_p2p(nbfmm_ctx *c, int part) {
       const int64_t t1 = r[w][0] + (r[w][1] - r[w][0]) * (ch + 1) / nch;
       if (t1 <= t0) continue;
       const int64_t grid = std::min<int64_t>(t1 - t0, ((int64_t)1 << 31) - 1);
-      auto kern = w == 0 ? (f32 ? (pot ? k_p2p_tasks<PairF32, true> : k_p2p_tasks<PairF32, false>)
-                                : (pot ? k_p2p_tasks<PairF64, true> : k_p2p_tasks<PairF64, false>))
-                         : (f32 ? (pot ? k_p2p_wide<PairF32, true> : k_p2p_wide<PairF32, false>)
-                                : (pot ? k_p2p_wide<PairF64, true> : k_p2p_wide<PairF64, false>));
+      auto kern = w == 0 ? (f32 ? (pot ? k_p2p_tasks<PairF32, PairF32Far, true> : k_p2p_tasks<PairF32, PairF32Far, false>)
+                                : (pot ? k_p2p_tasks<PairF64, PairF64Far, true> : k_p2p_tasks<PairF64, PairF64Far, false>))
+                         : (f32 ? (pot ? k_p2p_wide<PairF32, PairF32Far, true> : k_p2p_wide<PairF32, PairF32Far, false>)
+                                : (pot ? k_p2p_wide<PairF64, PairF64Far, true> : k_p2p_wide<PairF64, PairF64Far, false>));
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, t1 - t0, (const P2PTask *)c->tasks.p + t0,
                          (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
-                         (const int32_t *)c->cend.p, (const double4 *)c->pos.p, gn, pn);
+                         (const int32_t *)c->cend.p, (const uint64_t *)c->ckey.p, (const double4 *)c->pos.p, gn, pn);
       NB_LAUNCH();
     }
   }
   mark2(c, part == 0 ? 10 : 12);
-  if (part == 1) NB_HIP(hipEventRecord(c->ev_join, s2));
+  if (part >= 1) NB_HIP(hipEventRecord(c->ev_join, s2));
 }
 
 void run_finish(nbfmm_ctx *c, double *d_acc_out) {
   Plan &P = c->plan;
+  if (c->isolated) launch_p2p(c, 2);
   NB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   const bool pot = c->phi_out != nullptr;
   const double *gn = (const double *)c->grad_near.p;

@@ -68,7 +68,13 @@ class Stats(ctypes.Structure):
         ("flop_p2p", ctypes.c_double),
         ("mean_order", ctypes.c_double),
         ("fallback", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("isolated", ctypes.c_int32),
+        ("n_p2m_cells", ctypes.c_int64),
+        ("n_m2m_cells", ctypes.c_int64),
+        ("n_l2l_cells", ctypes.c_int64),
+        ("n_l2p_cells", ctypes.c_int64),
+        ("ms_m2m", ctypes.c_double),
+        ("ms_l2l", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -331,8 +337,9 @@ class Context:
         _check(lib().nbfmm_get_stats(self.h, ctypes.byref(s)))
         return s
 
-    def set_timing(self, on):
-        _check(lib().nbfmm_set_timing(self.h, int(on)))
+    def set_timing(self, mode):
+        """0 off, 1 on, 2 on with every phase (P2P included) run alone."""
+        _check(lib().nbfmm_set_timing(self.h, int(mode)))
 
     def keep_lists(self, on):
         _check(lib().nbfmm_keep_lists(self.h, int(on)))

@@ -89,3 +89,11 @@ def batched_systems(n_systems=256, seed=11, lo=1e3, hi=6.4e4, with_velocity=Fals
 def sample_targets(n, k, seed=4096):
     rng = np.random.default_rng(seed)
     return np.sort(rng.choice(n, size=min(k, n), replace=False)).astype(np.int64)
+
+
+def sample_targets_disjoint(n, k, exclude, seed=8191):
+    """k targets drawn without replacement from [0, n) minus `exclude` (a
+    held-out sample beside the one a tuner chose its setting on)."""
+    rng = np.random.default_rng(seed)
+    pool = np.setdiff1d(np.arange(n, dtype=np.int64), np.asarray(exclude, dtype=np.int64), assume_unique=False)
+    return np.sort(rng.choice(pool, size=min(k, len(pool)), replace=False)).astype(np.int64)

@@ -244,6 +244,31 @@ def test_fmm_parity_big_leaves(ctx, case, precision):
         assert np.allclose(a[:40], a[0], rtol=1e-12 if precision == 0 else 1e-6, atol=0)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_isolated_timing_same_result(ctx, precision):
+    # timing mode 2 runs the phases one after another (P2P alone at the end):
+    # the same result to rounding (the near-field atomics may add in another
+    # order), and every phase timed
+    xyz, m = workloads.plummer(30000, seed=19)
+    ctx.set_params(order=8, leaf_capacity=32, p2p_cut=512, G=1.0, eps2=0.0, precision=precision)
+    try:
+        a = ctx.eval_accel(xyz, m)
+        ctx.set_timing(2)
+        b = ctx.eval_accel(xyz, m)
+        st = ctx.stats()
+    finally:
+        ctx.set_timing(1)
+        ctx.set_params(precision=0)
+    assert _rel(b, a) < 1e-13
+    assert st.isolated == 1
+    for k in ("ms_tree", "ms_lists", "ms_p2m", "ms_m2l", "ms_l2p", "ms_p2p", "ms_p2p_kernel"):
+        assert getattr(st, k) > 0.0, k
+    assert st.ms_m2m >= 0.0 and st.ms_l2l >= 0.0
+    assert st.n_p2m_cells > 0 and st.n_l2p_cells > 0
+    print(f"[isolated] precision={precision} " + " ".join(f"{k}={getattr(st, k):.3f}" for k in
+                                                            ("ms_tree", "ms_m2l", "ms_p2p_kernel", "ms_total")))
+
+
 def test_fmm_parity_plummer(ctx):
     xyz, m = workloads.plummer(40000, seed=1)
     ctx.set_params(order=8, leaf_capacity=64, p2p_cut=512, G=1.0, eps2=0.0)
