@@ -349,6 +349,14 @@ __device__ __forceinline__ int pair_rows(const uint16_t *bins, int32_t q, int KP
   return (o + 1) * (o + 1);
 }
 
+// Pairs whose partial loads k_m2l_sum keeps in flight (measured at 4M, the
+// sum alone: FP32 2 and 4 best, 8 and 16 slower; FP64 4 and 8 best)
+#ifndef NBFMM_M2L_SUM_UNROLL_F32
+#define NBFMM_M2L_SUM_UNROLL_F32 2
+#endif
+#ifndef NBFMM_M2L_SUM_UNROLL_F64
+#define NBFMM_M2L_SUM_UNROLL_F64 4
+#endif
 template <class T>
 __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
@@ -361,33 +369,52 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
   const int32_t c = tgt_cells[t];
   const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
   T *L = Lexp + (size_t)lslot[c] * KP;
-  // rows lane, lane + 64, ... (KP <= 256), pairs four at a time
+  // rows lane, lane + 64, ... (KP <= 256)
   double s[4] = {0, 0, 0, 0};
   int nmax = 0; // rows any pair added
-  int32_t q = b;
-  for (; q + 4 <= e; q += 4) {
-    int32_t pr[4], nv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      pr[u] = pos[q + u];
-      nv[u] = pair_rows(bins, q + u, KP);
-      nmax = max(nmax, nv[u]);
+  for (int32_t q0 = b; q0 < e; q0 += 64) {
+    // the next 64 entries' partial positions and row counts, one per lane,
+    // so that the partial loads below depend on no other load: each pair's
+    // position is read out of its lane (v_readlane, wavefront-uniform)
+    const int32_t qi = q0 + lane;
+    int32_t prl = cb, nvl = 0, nva = 0;
+    if (qi < e) {
+      const int32_t pr = pos[qi];
+      nva = pair_rows(bins, qi, KP);
+      if (pr >= cb && pr < ce) {
+        prl = pr;
+        nvl = nva;
+      }
     }
+    for (int o = 32; o >= 1; o >>= 1) nva = max(nva, __shfl_xor(nva, o));
+    nmax = max(nmax, nva);
+    const int m = min(64, e - q0);
+    int j = 0;
+    // kU pairs' loads in flight, then their sums in list order
+    constexpr int kU = sizeof(T) == 4 ? NBFMM_M2L_SUM_UNROLL_F32 : NBFMM_M2L_SUM_UNROLL_F64;
+    for (; j + kU <= m; j += kU) {
+      T v[kU][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (pr[u] >= cb && pr[u] < ce)
+      for (int u = 0; u < kU; ++u) {
+        const int32_t pr = __builtin_amdgcn_readlane(prl, j + u);
+        const int nv = __builtin_amdgcn_readlane(nvl, j + u);
+        const T *y = Y + (size_t)(pr - cb) * KP;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (lane + 64 * r < nv[u]) s[r] += (double)Y[(size_t)(pr[u] - cb) * KP + lane + 64 * r];
-  }
-  for (; q < e; ++q) {
-    const int32_t pr = pos[q];
-    const int nv = pair_rows(bins, q, KP);
-    nmax = max(nmax, nv);
-    if (pr >= cb && pr < ce)
+        for (int r = 0; r < 4; ++r) v[u][r] = lane + 64 * r < nv ? y[lane + 64 * r] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s[r] += (double)v[u][r];
+    }
+    for (; j < m; ++j) {
+      const int32_t pr = __builtin_amdgcn_readlane(prl, j);
+      const int nv = __builtin_amdgcn_readlane(nvl, j);
+      const T *y = Y + (size_t)(pr - cb) * KP;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (lane + 64 * r < nv) s[r] += (double)Y[(size_t)(pr - cb) * KP + lane + 64 * r];
+        if (lane + 64 * r < nv) s[r] += (double)y[lane + 64 * r];
+    }
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)

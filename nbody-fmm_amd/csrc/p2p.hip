@@ -39,6 +39,16 @@ constexpr int kW = 64; // one wavefront per task
 #ifndef NBFMM_P2P_GROUP_F64
 #define NBFMM_P2P_GROUP_F64 8
 #endif
+// Targets a wavefront-uniform branch covers in the pair loops of
+// k_p2p_tasks (a block's last group computes up to this many minus one
+// unused slots): measured at 4M, 1, 2 and 4 are within 0.2 ms of each other;
+// 2 for FP32 (9.49 ms against 9.65 with 4), 1 for FP64 (14.06 against 14.51)
+#ifndef NBFMM_P2P_TGT_STEP_F32
+#define NBFMM_P2P_TGT_STEP_F32 2
+#endif
+#ifndef NBFMM_P2P_TGT_STEP_F64
+#define NBFMM_P2P_TGT_STEP_F64 1
+#endif
 static_assert(NBFMM_P2P_GROUP_F32 % 4 == 0 && NBFMM_P2P_GROUP_F64 % 4 == 0, "targets go in blocks of four");
 
 // list cell of concatenated source index g: the last with lp <= g (lp[0] = 0)
@@ -77,6 +87,7 @@ struct PairF64 {
   typedef double A;
   static constexpr bool kAnchor = false;
   static constexpr int kGroup = NBFMM_P2P_GROUP_F64; // targets per pass (3 or 4 FP64 sums each in registers)
+  static constexpr int kStep = NBFMM_P2P_TGT_STEP_F64; // targets per branch of the pair loops
   static __device__ __forceinline__ S make(const double4 &p, const double4 &) { return p; }
   static __device__ __forceinline__ S tgt(const double4 &v) { return v; }
   static __device__ __forceinline__ void massless(S &v) { v.w = 0.0; }
@@ -128,6 +139,7 @@ struct PairF32 {
   typedef float A;
   static constexpr bool kAnchor = true;
   static constexpr int kGroup = NBFMM_P2P_GROUP_F32; // targets per pass
+  static constexpr int kStep = NBFMM_P2P_TGT_STEP_F32; // targets per branch of the pair loops
   static __device__ __forceinline__ S make(const double4 &p, const double4 &a) {
     const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
     const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
@@ -194,7 +206,10 @@ struct PairF32Far {
 //  - a leaf's own 27-neighbour row: every cell touches it;
 //  - the 27-neighbour row of the parent P of a closed cell A: the eight
 //    cells touching A (per axis, offset 0 or towards A's octant) go first,
-//    the nineteen others (at least one edge of A away) after them.
+//    the nineteen others (at least one edge of A away) after them.  (The
+//    eight touching cells replaced by their children, of which only A's
+//    27-neighbourhood touches A, gave more far pairs but lists of up to 83
+//    small cells, whose walk cost more: P2P 9.4 -> 10.5 ms at 4M.)
 // tmp: 2 x 27 LDS words.  One wavefront; ends with a barrier.
 __device__ __forceinline__ int32_t stage_list(const P2PTask &tk, const int32_t *__restrict__ nbr27,
                                               const int32_t *__restrict__ sub_src, const int32_t *__restrict__ cbeg,
@@ -258,6 +273,31 @@ __device__ __forceinline__ int32_t stage_list(const P2PTask &tk, const int32_t *
   near_end = tk.list >= 0 ? 0 : carry;
   __syncthreads();
   return carry;
+}
+
+// Sum over the 64 lanes of each of G values (G a power of two <= 64), in
+// FP64 and a fixed order, transposed: at every halving step a lane keeps
+// one half of its values and adds the other half of its partner's (offset
+// 32, 16, ...), then the remaining lanes of each value are added by plain
+// xor-shuffles.  G - 1 + log2(64 / G) shuffles instead of 6 G; lane l ends
+// with the total of value l >> (6 - log2 G).
+template <int G>
+__device__ __forceinline__ double tr_reduce(double (&v)[G], int lane) {
+  static_assert(G >= 1 && G <= kW && (G & (G - 1)) == 0, "G is a power of two up to the wavefront");
+#pragma unroll
+  for (int S = G, o = kW / 2; S > 1; S >>= 1, o >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int k = 0; k < S / 2; ++k) {
+      const double a = v[k], c = v[k + S / 2];
+      const double send = up ? a : c, keep = up ? c : a;
+      v[k] = keep + __shfl_xor(send, o);
+    }
+  }
+  double x = v[0];
+#pragma unroll
+  for (int o = kW / G / 2; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+  return x;
 }
 
 // Lanes take sources, targets are unrolled: the task's targets are staged
@@ -340,10 +380,10 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3
           S x2 = Tr::make(has2 ? q2 : q1, anchor);
           if (!has2) Tr::massless(x2); // contributes nothing
 #pragma unroll
-          for (int t4 = 0; t4 < kGroup; t4 += 4)
-            if (t4 < ng) // four targets a branch (sums of slots past ng are never used)
+          for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
+            if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
 #pragma unroll
-              for (int t = t4; t < t4 + 4; ++t) {
+              for (int t = t4; t < t4 + Tr::kStep; ++t) {
                 const S xt = Tr::tgt(st[t0 + t]);
                 Tr::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
                 Tr::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
@@ -371,10 +411,10 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3
           FS x2 = TrFar::make(has2 ? q2 : q1, anchor);
           if (!has2) TrFar::massless(x2); // contributes nothing
 #pragma unroll
-          for (int t4 = 0; t4 < kGroup; t4 += 4)
-            if (t4 < ng) // four targets a branch (sums of slots past ng are never used)
+          for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
+            if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
 #pragma unroll
-              for (int t = t4; t < t4 + 4; ++t) {
+              for (int t = t4; t < t4 + Tr::kStep; ++t) {
                 const FS xt = TrFar::tgt(st[t0 + t]);
                 TrFar::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
                 TrFar::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
@@ -382,24 +422,33 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3
         }
       }
 
+      // each component's kGroup sums meet in one transposed FP64 reduction:
+      // afterwards lane l holds the total of target l >> (6 - log2 kGroup)
+      constexpr int kSpan = kW / kGroup; // lanes sharing one target's total
+      const int tl = lane / kSpan;
+      const bool writer = (lane % kSpan) == 0 && tl < ng;
+      const int64_t i = b + t0 + tl;
+      double v[kGroup];
 #pragma unroll
-      for (int t = 0; t < kGroup; ++t) {
-        if (t >= ng) continue;
-        double gx = ax[t], gy = ay[t], gz = az[t], gp = POT ? (double)ap[t] : 0.0;
+      for (int t = 0; t < kGroup; ++t) v[t] = ax[t];
+      const double gx = tr_reduce<kGroup>(v, lane);
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-          gx += __shfl_xor(gx, o);
-          gy += __shfl_xor(gy, o);
-          gz += __shfl_xor(gz, o);
-          if (POT) gp += __shfl_xor(gp, o);
-        }
-        if (lane == 0) {
-          const int64_t i = b + t0 + t;
-          atomicAdd(&grad[3 * i], gx);
-          atomicAdd(&grad[3 * i + 1], gy);
-          atomicAdd(&grad[3 * i + 2], gz);
-          if (POT) atomicAdd(&pot[i], gp);
-        }
+      for (int t = 0; t < kGroup; ++t) v[t] = ay[t];
+      const double gy = tr_reduce<kGroup>(v, lane);
+#pragma unroll
+      for (int t = 0; t < kGroup; ++t) v[t] = az[t];
+      const double gz = tr_reduce<kGroup>(v, lane);
+      double gp = 0.0;
+      if (POT) {
+#pragma unroll
+        for (int t = 0; t < kGroup; ++t) v[t] = ap[t];
+        gp = tr_reduce<kGroup>(v, lane);
+      }
+      if (writer) {
+        atomicAdd(&grad[3 * i], gx);
+        atomicAdd(&grad[3 * i + 1], gy);
+        atomicAdd(&grad[3 * i + 2], gz);
+        if (POT) atomicAdd(&pot[i], gp);
       }
     }
   }
