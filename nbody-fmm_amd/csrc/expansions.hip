@@ -726,6 +726,10 @@ void run_expansions(nbfmm_ctx *c) {
   NB_HIP(hipStreamSynchronize(s));
   const int32_t nM = h[0], nP2M = h[1], nL = h[3];
   c->n_local = nL;
+  P.n_p2m_cells = nP2M;
+  P.n_m2m_cells = h[2];        // cells forming their multipole by M2M
+  P.n_l2l_cells = h[4];        // cells pushing their local by L2L
+  P.n_l2p_cells = h[3] - h[4]; // cells evaluating their local at their particles
   // orders per cell (learned selection): the expansions are formed to the
   // highest order in use
   const int8_t *ord = nullptr;
@@ -788,10 +792,6 @@ void run_expansions(nbfmm_ctx *c) {
   if (!c->p2p_started) launch_p2p(c, 1);
   P.n_p2m_terms = (int64_t)th[0]; // read back at run_m2l's synchronisation
   P.n_l2p_terms = (int64_t)th[1];
-  P.n_p2m_cells = nP2M;
-  P.n_m2m_cells = h[2];        // cells forming their multipole by M2M
-  P.n_l2l_cells = h[4];        // cells pushing their local by L2L
-  P.n_l2p_cells = h[3] - h[4]; // cells evaluating their local at their particles
   mark(c, 13);
   // ---- L2L, coarsest level first ----
   for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
