@@ -199,6 +199,9 @@ def bench_single(args, cfg, name, precs, learned=True, cpu=True):
             log(f"[bench] no setting meets the gate for {PREC_NAMES[prec]}; timing the most accurate one")
         ctx.set_params(order=best["order"], p2p_cut=best["p2p_cut"], leaf_capacity=best["leaf_capacity"],
                        precision=prec, class_order_tol=best["class_order_tol"])
+        tiles, tile_table = (ctx.tiles(), [])
+        if not args.no_tile_tuning:
+            tiles, tile_table = tune.tune_tiles(ctx, n, dx, dm, da, prec, log=log)
         wall, st = timed(ctx, n, dx, dm, da, args.steps, args.warmup)
         acc = da.download(ctx.stream)
         err = tune.rel_errors(acc[hold], ref_hold)
@@ -210,7 +213,7 @@ def bench_single(args, cfg, name, precs, learned=True, cpu=True):
                "p2p_cut": best["p2p_cut"], "leaf_capacity": best["leaf_capacity"],
                "class_order_tol": best["class_order_tol"], "err": err, "err_tuning_sample": err_tune,
                "meets_gate": err["l2"] <= args.gate, "stats": st.as_dict(), "isolated_ms": ms_iso,
-               "isolated_stats": st_iso,
+               "isolated_stats": st_iso, "tiles": tiles, "tile_tuning": tile_table,
                "tuning": [{"leaf_capacity": r["leaf_capacity"], "order": r["order"], "p2p_cut": r["p2p_cut"],
                            "class_order_tol": r["class_order_tol"], "ms": round(r["ms"], 3), "l2": r["err"]["l2"]}
                           for r in table]}
@@ -274,6 +277,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned order selection")
     ap.add_argument("--no-subrecords", action="store_true", help="only the headline workload")
+    ap.add_argument("--no-tile-tuning", action="store_true", help="keep the built-in tile shapes")
     ap.add_argument("--targets", type=int, default=4096)
     ap.add_argument("--gate", type=float, default=1e-6)
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -319,7 +323,7 @@ def main():
         "config": {
             "workload": f"Plummer sphere a=1 M=1 equal masses, N={cfg['n']}, {head_name}, leaf capacity "
                         f"{head['leaf_capacity']}, p={head['order']}, P2P cut {head['p2p_cut']}, M2L class-order tol "
-                        f"{head.get('class_order_tol', 0.0)}",
+                        f"{head.get('class_order_tol', 0.0)}, autotuned tiles",
             "N": cfg["n"], "precision": head_name, "order": head["order"], "p2p_cut": head["p2p_cut"],
             "class_order_tol": head.get("class_order_tol", 0.0),
             "leaf_capacity": head["leaf_capacity"],
@@ -332,9 +336,10 @@ def main():
             "counts": {k: st[k] for k in ("depth", "n_cells", "n_leaves", "n_il", "n_m2l", "n_m2l_targets",
                                            "n_m2l_sources", "n_p2p_pairs", "n_p2m_cells", "n_m2m_cells",
                                            "n_l2l_cells", "n_l2p_cells")},
+            "tiles": head.get("tiles"),
             "modes": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "order", "p2p_cut", "leaf_capacity",
-                                               "class_order_tol", "err", "meets_gate", "isolated_ms", "tuning",
-                                               "learned")
+                                               "class_order_tol", "err", "meets_gate", "isolated_ms", "tiles",
+                                               "tuning", "tile_tuning", "learned")
                           if kk in v}
                       for k, v in results.items()},
             "mixed_over_plain": (results["mixed"]["value"] / results["fp64"]["value"]
@@ -437,6 +442,10 @@ def bench_batched(args, cfg, precs, quick=False):
         if best is None:
             best = min(table, key=lambda r: r["max_l2"])
         ctx.set_params(order=best["order"], p2p_cut=best["p2p_cut"], precision=prec)
+        tiles, tile_table = ctx.tiles(), []
+        if not args.no_tile_tuning:
+            tiles, tile_table = tune.tune_tiles(ctx, n, dx, dm, da, prec, log=log, reps=3,
+                                                run=lambda: ctx.batch_eval_accel_dev(off, dx.ptr, dm.ptr, da.ptr))
         for _ in range(args.warmup):
             ctx.batch_eval_accel_dev(off, dx.ptr, dm.ptr, da.ptr)
         sync(ctx)
@@ -460,7 +469,8 @@ def bench_batched(args, cfg, precs, quick=False):
                                      "order": best["order"], "p2p_cut": best["p2p_cut"],
                                      "worst_system_l2": max(e), "mean_system_l2": float(np.mean(e)),
                                      "meets_gate": max(e) <= args.gate, "rk4_ms_per_step": rk4_ms,
-                                     "rk4_steps": 20, "rk4_finite": ok, "tuning": table}
+                                     "rk4_steps": 20, "rk4_finite": ok, "tuning": table, "tiles": tiles,
+                                     "tile_tuning": tile_table}
         log(f"[bench] batched {PREC_NAMES[prec]}: {results[PREC_NAMES[prec]]['ms_per_step']:.2f} ms/eval, "
             f"worst system {max(e):.3e}, RK4 {rk4_ms:.1f} ms/step")
     ctx.close()
@@ -570,6 +580,7 @@ def learned_mode(ctx, n, dx, dm, da, tgt, ref, mixed, args):
         wall, st = timed(ctx, n, dx, dm, da, args.steps, args.warmup)
         err = tune.rel_errors(da.download(ctx.stream)[tgt], ref)
         runs.append({"value": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps, "order": pmax,
+                     "tiles": ctx.tiles(),
                      "p2p_cut": mixed["p2p_cut"], "leaf_capacity": mixed["leaf_capacity"],
                      "class_order_tol": mixed.get("class_order_tol", 0.0), "err": err,
                      "meets_gate": err["l2"] <= args.gate, "stats": st.as_dict(), "tuning": [],

@@ -261,6 +261,32 @@ int nbfmm_batch_step_se_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets
 int nbfmm_batch_step_rk4_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets, double *d_xyz, double *d_vel,
                              double *d_acc, const double *d_m, double dt, int nsteps, int mode);
 
+/* Tile shapes and launch split of the P2P and M2L kernels (no counterpart
+ * in the reference, whose serial code has none; the tuner in
+ * nbody-fmm_amd/tune.py searches them per workload, as it searches the
+ * order, the P2P cut and the leaf capacity that replace the reference's
+ * hardcoded L and threshold at tree.cpp:32,94).  A zero (or, for the share,
+ * a negative) field keeps the built-in default.  They change speed only: the
+ * results agree to rounding (the near-field sums meet in FP64 atomics). */
+typedef struct nbfmm_tiles {
+  int32_t p2p_group_f32;      /* targets per source pass of the narrow FP32 P2P kernel: 8, 12, 16 (default) or 24 */
+  int32_t p2p_group_f64;      /* the same for FP64: 4, 8 (default) or 12 */
+  int32_t p2p_narrow_max_f32; /* blocks of at most this many targets go to the narrow kernel, larger ones to the
+                                 four-targets-per-lane kernel: 1..256 (default 32) */
+  int32_t p2p_narrow_max_f64; /* the same for FP64 (default 16) */
+  double p2p_early_share_f32; /* share of the P2P tasks started beside the up-pass, in [0, 1] (default 0.5) */
+  double p2p_early_share_f64; /* the same for FP64 (default 0.2) */
+  int32_t p2p_early_launches_f32; /* launches of that share's narrow tasks, 1..64 (default 16) */
+  int32_t p2p_early_launches_f64; /* the same for FP64 (default 1) */
+  int32_t m2l_slices;         /* row-block slices of a class operator per M2L contraction workgroup: 1 = the
+                                 fewest that fit the LDS (default), 2 = one more (half the LDS, more workgroups) */
+  int32_t reserved0;
+} nbfmm_tiles;
+/* Tiles in force for later evaluations; fields are validated (NBFMM_ERR_ARG). */
+int nbfmm_set_tiles(nbfmm_ctx *ctx, const nbfmm_tiles *tiles);
+/* The tiles in force, every default resolved to its value. */
+int nbfmm_get_tiles(nbfmm_ctx *ctx, nbfmm_tiles *tiles);
+
 /* Statistics of the last evaluation: sizes for the roofline and per-phase
  * GPU times measured with HIP events on the context's stream.  The phases
  * follow each other on that stream; P2P runs on a second stream beside them

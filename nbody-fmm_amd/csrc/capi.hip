@@ -315,6 +315,32 @@ int nbfmm_set_params(nbfmm_ctx *c, const nbfmm_params *params) {
 
 void *nbfmm_stream(nbfmm_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int nbfmm_set_tiles(nbfmm_ctx *c, const nbfmm_tiles *t) {
+  if (!c || !t) return fail(NBFMM_ERR_ARG, "null argument");
+  const nbfmm_tiles d = default_tiles();
+  nbfmm_tiles r = *t;
+  if (r.p2p_group_f32 == 0) r.p2p_group_f32 = d.p2p_group_f32;
+  if (r.p2p_group_f64 == 0) r.p2p_group_f64 = d.p2p_group_f64;
+  if (r.p2p_narrow_max_f32 == 0) r.p2p_narrow_max_f32 = d.p2p_narrow_max_f32;
+  if (r.p2p_narrow_max_f64 == 0) r.p2p_narrow_max_f64 = d.p2p_narrow_max_f64;
+  if (r.p2p_early_share_f32 < 0.0) r.p2p_early_share_f32 = d.p2p_early_share_f32;
+  if (r.p2p_early_share_f64 < 0.0) r.p2p_early_share_f64 = d.p2p_early_share_f64;
+  if (r.p2p_early_launches_f32 == 0) r.p2p_early_launches_f32 = d.p2p_early_launches_f32;
+  if (r.p2p_early_launches_f64 == 0) r.p2p_early_launches_f64 = d.p2p_early_launches_f64;
+  if (r.m2l_slices == 0) r.m2l_slices = d.m2l_slices;
+  r.reserved0 = 0;
+  std::string why;
+  if (!tiles_ok(r, why)) return fail(NBFMM_ERR_ARG, why);
+  c->shapes = r;
+  return NBFMM_OK;
+}
+
+int nbfmm_get_tiles(nbfmm_ctx *c, nbfmm_tiles *t) {
+  if (!c || !t) return fail(NBFMM_ERR_ARG, "null argument");
+  *t = c->shapes;
+  return NBFMM_OK;
+}
+
 int nbfmm_set_timing(nbfmm_ctx *c, int enabled) {
   if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
   if (enabled < 0 || enabled > 2) return fail(NBFMM_ERR_ARG, "timing mode must be 0, 1 or 2");

@@ -567,7 +567,7 @@ void build_lists(nbfmm_ctx *c) {
   nbr_leaf_rows(c, closed); // the P2P work list reads the rows of the leaves that are not closed
   const int32_t *cpar = (const int32_t *)c->cparent.p;
   // task counts of narrow cells, then of wide cells (2 (nc + 1) entries)
-  const int narrow_max = p2p_narrow_max(c->prm.precision == NBFMM_MIXED);
+  const int narrow_max = c->prm.precision == NBFMM_MIXED ? c->shapes.p2p_narrow_max_f32 : c->shapes.p2p_narrow_max_f64;
   int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)2 * (nc + 1));
   int32_t *toff = dget<int32_t>(c->task_off, (size_t)2 * (nc + 1));
   NB_HIP(hipMemsetAsync(tcnt, 0, sizeof(int32_t) * 2 * (size_t)(nc + 1), s));

@@ -425,10 +425,9 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
 // fit the LDS, split evenly (at KP = 144 in FP64: 5 + 4, not 8 + 1, so that
 // no slice reloads the multipoles for a single row block).
 template <class T>
-constexpr int rbw_for(int KP, int lds) {
+constexpr int rbw_for(int KP, int lds, int more = 0) {
   const int fit = lds / (16 * KP * (int)sizeof(T)), rb = KP / 16;
-  if (fit >= rb) return rb;
-  const int slices = (rb + fit - 1) / fit;
+  const int slices = (fit >= rb ? 1 : (rb + fit - 1) / fit) + more; // more: slices beyond the fewest
   return (rb + slices - 1) / slices;
 }
 
@@ -456,21 +455,23 @@ template <int KP, class T>
 void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
                const T *frag, int KPS, int nvalid) {
   if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
-  launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
+  if (c->shapes.m2l_slices == 2)
+    launch_rbw<KP, rbw_for<T>(KP, kLds, 1), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
+  else
+    launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
 }
 
 // as rbw_for, for the three-term bf16 operator blocks
-constexpr int rbw_bf3(int KP) {
+constexpr int rbw_bf3(int KP, int more = 0) {
   const int fit = kLds / (pad32(KP) / 32 * 3072), rb = KP / 16;
-  if (fit >= rb) return rb;
-  const int slices = (rb + fit - 1) / fit;
+  const int slices = (fit >= rb ? 1 : (rb + fit - 1) / fit) + more; // more: slices beyond the fewest
   return (rb + slices - 1) / slices;
 }
 
-template <int KP>
-void launch_bf3_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
-                   const uint4 *fragb, int KPS, int nvalid) {
-  constexpr int RB = KP / 16, RBW = rbw_bf3(KP);
+template <int KP, int RBW>
+void launch_bf3_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+                    const uint4 *fragb, int KPS, int nvalid) {
+  constexpr int RB = KP / 16;
   if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
   const size_t lds = (size_t)RBW * (pad32(KP) / 32) * 3072;
   static bool attr_set[64] = {};
@@ -483,6 +484,15 @@ void launch_bf3_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t yba
                      ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, fragb,
                      (const float *)c->mexp.p, (float *)c->ypair.p, KPS, nvalid);
   NB_LAUNCH();
+}
+
+template <int KP>
+void launch_bf3_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+                   const uint4 *fragb, int KPS, int nvalid) {
+  if (c->shapes.m2l_slices == 2)
+    launch_bf3_rbw<KP, rbw_bf3(KP, 1)>(c, nitems, items, ybase, cls_pairs, fragb, KPS, nvalid);
+  else
+    launch_bf3_rbw<KP, rbw_bf3(KP)>(c, nitems, items, ybase, cls_pairs, fragb, KPS, nvalid);
 }
 
 void launch_bf3(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,

@@ -54,7 +54,10 @@ constexpr int kMaxList = 192;     // longest source-cell list of a P2P task (int
 #ifndef NBFMM_P2P_NARROW_F64
 #define NBFMM_P2P_NARROW_F64 16
 #endif
-inline int p2p_narrow_max(bool f32) { return f32 ? NBFMM_P2P_NARROW_F32 : NBFMM_P2P_NARROW_F64; }
+// the built-in tiles (nbfmm_tiles, every field resolved); tiles_ok checks a
+// resolved set (why: the reason when it fails)
+nbfmm_tiles default_tiles();
+bool tiles_ok(const nbfmm_tiles &t, std::string &why);
 // orders a learned selection may give a cell
 constexpr int kOrderLo = 4, kOrderHi = 14, kNumOrders = kOrderHi - kOrderLo + 1;
 // the order-selection MLP: kMlpIn -> kMlpHidden -> kMlpHidden -> kNumOrders,
@@ -139,7 +142,7 @@ struct Plan {
   int64_t n_il = 0, n_m2l = 0, n_sub = 0;
   int64_t n_src = 0, n_tgt = 0, n_p2p_pairs = 0, n_p2m_terms = 0, n_l2p_terms = 0;
   int64_t n_p2m_items = 0, n_m2l_tiles = 0, n_tasks = 0;
-  int64_t n_tasks_narrow = 0; // tasks [0, n_tasks_narrow) have blocks of at most p2p_narrow_max targets
+  int64_t n_tasks_narrow = 0; // tasks [0, n_tasks_narrow) have blocks of at most the tiles' narrow_max targets
   int64_t n_p2m_cells = 0, n_m2m_cells = 0, n_l2l_cells = 0, n_l2p_cells = 0; // cells of each pass
   int kp = 0; // padded coefficient count (multiple of 16)
   double flop_m2l = 0; // algorithmic M2L flops of the evaluation
@@ -151,6 +154,7 @@ struct Plan {
 // The context (definition shared by the host translation units).
 struct nbfmm_ctx {
   nbfmm_params prm{};
+  nbfmm_tiles shapes = nbfmm::default_tiles(); // tile shapes, every field resolved
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;            // P2P, concurrent with the expansion passes
   hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_join = nullptr;

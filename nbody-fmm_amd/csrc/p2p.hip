@@ -23,6 +23,7 @@
 #include "nbfmm_internal.h"
 
 #include <algorithm>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 
@@ -312,8 +313,17 @@ __device__ __forceinline__ double tr_reduce(double (&v)[G], int lane) {
 // other; both keep the kernel at 128 registers (four wavefronts per SIMD).  One wavefront per task
 // (tasks blockIdx.x, blockIdx.x + gridDim.x, ... when the grid is smaller
 // than the task list).  POT also sums m_j / r_ij into pot.
-template <class Tr, class TrFar, bool POT>
-__global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3 : 4))) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
+// Wavefronts per SIMD the narrow kernel is held to: four (128 registers) for
+// the group sizes measured to fit without spills in their loops, otherwise
+// left to the compiler
+template <class Tr, int KG, bool POT>
+constexpr int p2p_waves() {
+  return (!POT && KG <= (Tr::kAnchor ? 16 : 8)) ? 4 : 1;
+}
+constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
+
+template <class Tr, class TrFar, int KG, bool POT>
+__global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_waves<Tr, KG, POT>()))) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
@@ -323,7 +333,7 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3
   typedef typename Tr::S S;
   typedef typename TrFar::S FS;
   typedef typename Tr::A A;
-  __shared__ S st[kTaskTargets + Tr::kGroup]; // the last pass reads up to kGroup - 1 slots past nc
+  __shared__ S st[kTaskTargets + KG]; // the last pass reads up to KG - 1 slots past nc
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   __shared__ int32_t tmp[54];
@@ -355,7 +365,7 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3
     const int ne = tk.nlist;
     const int32_t f0 = tk.f0, f1 = min(tk.f1, total);
     const int32_t fm = max(f0, min(f1, near_end)); // [f0, fm) near cells, [fm, f1) the others
-    constexpr int kGroup = Tr::kGroup;
+    constexpr int kGroup = KG;
     for (int t0 = 0; t0 < nc; t0 += kGroup) {
       const int ng = min(kGroup, nc - t0);
       A ax[kGroup], ay[kGroup], az[kGroup], ap[kGroup];
@@ -422,27 +432,29 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(POT ? 3
         }
       }
 
-      // each component's kGroup sums meet in one transposed FP64 reduction:
-      // afterwards lane l holds the total of target l >> (6 - log2 kGroup)
-      constexpr int kSpan = kW / kGroup; // lanes sharing one target's total
+      // each component's kGroup sums (zero-padded to a power of two RG) meet
+      // in one transposed FP64 reduction: afterwards lane l holds the total
+      // of target l >> (6 - log2 RG)
+      constexpr int RG = pow2_at_least(kGroup);
+      constexpr int kSpan = kW / RG; // lanes sharing one target's total
       const int tl = lane / kSpan;
       const bool writer = (lane % kSpan) == 0 && tl < ng;
       const int64_t i = b + t0 + tl;
-      double v[kGroup];
+      double v[RG];
 #pragma unroll
-      for (int t = 0; t < kGroup; ++t) v[t] = ax[t];
-      const double gx = tr_reduce<kGroup>(v, lane);
+      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ax[t] : 0.0;
+      const double gx = tr_reduce<RG>(v, lane);
 #pragma unroll
-      for (int t = 0; t < kGroup; ++t) v[t] = ay[t];
-      const double gy = tr_reduce<kGroup>(v, lane);
+      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ay[t] : 0.0;
+      const double gy = tr_reduce<RG>(v, lane);
 #pragma unroll
-      for (int t = 0; t < kGroup; ++t) v[t] = az[t];
-      const double gz = tr_reduce<kGroup>(v, lane);
+      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)az[t] : 0.0;
+      const double gz = tr_reduce<RG>(v, lane);
       double gp = 0.0;
       if (POT) {
 #pragma unroll
-        for (int t = 0; t < kGroup; ++t) v[t] = ap[t];
-        gp = tr_reduce<kGroup>(v, lane);
+        for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ap[t] : 0.0;
+        gp = tr_reduce<RG>(v, lane);
       }
       if (writer) {
         atomicAdd(&grad[3 * i], gx);
@@ -705,9 +717,33 @@ __global__ void k_direct_combine(int64_t ntgt, int nchunk, const double *__restr
   }
 }
 
-} // namespace
+typedef void (*P2PKernel)(int64_t, const P2PTask *, const int32_t *, const int32_t *, const int32_t *,
+                          const int32_t *, const uint64_t *, const double4 *, double *, double *);
 
-void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
+// the narrow kernel's instance for a target group size (the sizes the tiles allow)
+P2PKernel narrow_kernel(bool f32, bool pot, int group) {
+#define NB_NARROW(TR, FAR, G) \
+  case G:                     \
+    return pot ? k_p2p_tasks<TR, FAR, G, true> : k_p2p_tasks<TR, FAR, G, false>;
+  if (f32) {
+    switch (group) {
+      NB_NARROW(PairF32, PairF32Far, 8)
+      NB_NARROW(PairF32, PairF32Far, 12)
+      NB_NARROW(PairF32, PairF32Far, 16)
+      NB_NARROW(PairF32, PairF32Far, 24)
+    }
+  } else {
+    switch (group) {
+      NB_NARROW(PairF64, PairF64Far, 4)
+      NB_NARROW(PairF64, PairF64Far, 8)
+      NB_NARROW(PairF64, PairF64Far, 12)
+    }
+  }
+#undef NB_NARROW
+  throw Error(NBFMM_ERR_ARG, "no P2P kernel for target group " + std::to_string(group));
+}
+
+} // namespace
 
 // Share of the P2P tasks run in part 0 (mixed precision, FP64).
 #ifndef NBFMM_P2P_EARLY
@@ -729,6 +765,42 @@ void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
 #define NBFMM_P2P_CHUNKS0_F64 1
 #endif
 
+nbfmm_tiles default_tiles() {
+  nbfmm_tiles t{};
+  t.p2p_group_f32 = PairF32::kGroup;
+  t.p2p_group_f64 = PairF64::kGroup;
+  t.p2p_narrow_max_f32 = NBFMM_P2P_NARROW_F32;
+  t.p2p_narrow_max_f64 = NBFMM_P2P_NARROW_F64;
+  t.p2p_early_share_f32 = NBFMM_P2P_EARLY;
+  t.p2p_early_share_f64 = NBFMM_P2P_EARLY_F64;
+  t.p2p_early_launches_f32 = NBFMM_P2P_CHUNKS0;
+  t.p2p_early_launches_f64 = NBFMM_P2P_CHUNKS0_F64;
+  t.m2l_slices = 1;
+  return t;
+}
+
+bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
+  auto in = [](int v, std::initializer_list<int> set) {
+    for (int x : set)
+      if (v == x) return true;
+    return false;
+  };
+  if (!in(t.p2p_group_f32, {8, 12, 16, 24})) why = "p2p_group_f32 must be 8, 12, 16 or 24";
+  else if (!in(t.p2p_group_f64, {4, 8, 12})) why = "p2p_group_f64 must be 4, 8 or 12";
+  else if (t.p2p_narrow_max_f32 < 1 || t.p2p_narrow_max_f32 > kTaskTargets) why = "p2p_narrow_max_f32 must be in [1, 256]";
+  else if (t.p2p_narrow_max_f64 < 1 || t.p2p_narrow_max_f64 > kTaskTargets) why = "p2p_narrow_max_f64 must be in [1, 256]";
+  else if (!(t.p2p_early_share_f32 >= 0.0 && t.p2p_early_share_f32 <= 1.0)) why = "p2p_early_share_f32 must be in [0, 1]";
+  else if (!(t.p2p_early_share_f64 >= 0.0 && t.p2p_early_share_f64 <= 1.0)) why = "p2p_early_share_f64 must be in [0, 1]";
+  else if (t.p2p_early_launches_f32 < 1 || t.p2p_early_launches_f32 > 64) why = "p2p_early_launches_f32 must be in [1, 64]";
+  else if (t.p2p_early_launches_f64 < 1 || t.p2p_early_launches_f64 > 64) why = "p2p_early_launches_f64 must be in [1, 64]";
+  else if (t.m2l_slices != 1 && t.m2l_slices != 2) why = "m2l_slices must be 1 or 2";
+  else return true;
+  return false;
+}
+
+void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
+
+
 void launch_p2p(nbfmm_ctx *c, int part) {
   // isolated timing: parts 0 and 1 are left out and run_finish runs every
   // task (part 2) once the main stream is idle
@@ -741,7 +813,8 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   // the same share of the narrow tasks [0, nn) and of the wide ones [nn, n_tasks)
   const int64_t nn = P.n_tasks_narrow, nw = P.n_tasks - nn;
   const bool f32 = c->prm.precision == NBFMM_MIXED;
-  const double early = f32 ? NBFMM_P2P_EARLY : NBFMM_P2P_EARLY_F64;
+  const nbfmm_tiles &tl = c->shapes;
+  const double early = f32 ? tl.p2p_early_share_f32 : tl.p2p_early_share_f64;
   const int64_t sn = (int64_t)((double)nn * early), sw = nn + (int64_t)((double)nw * early);
   int64_t r[2][2] = {{part == 0 ? 0 : sn, part == 0 ? sn : nn}, {part == 0 ? nn : sw, part == 0 ? sw : P.n_tasks}};
   if (part == 2) { // every task, one launch per kernel
@@ -772,14 +845,13 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   // whose narrow tasks are longer, measured slower that way)
   for (int wi = 0; wi < 2; ++wi) {
     const int w = (part == 1 && f32) ? 1 - wi : wi;
-    const int nch = (w == 0 && part == 0) ? (f32 ? NBFMM_P2P_CHUNKS0 : NBFMM_P2P_CHUNKS0_F64) : 1;
+    const int nch = (w == 0 && part == 0) ? (f32 ? tl.p2p_early_launches_f32 : tl.p2p_early_launches_f64) : 1;
     for (int ch = 0; ch < nch; ++ch) {
       const int64_t t0 = r[w][0] + (r[w][1] - r[w][0]) * ch / nch;
       const int64_t t1 = r[w][0] + (r[w][1] - r[w][0]) * (ch + 1) / nch;
       if (t1 <= t0) continue;
       const int64_t grid = std::min<int64_t>(t1 - t0, ((int64_t)1 << 31) - 1);
-      auto kern = w == 0 ? (f32 ? (pot ? k_p2p_tasks<PairF32, PairF32Far, true> : k_p2p_tasks<PairF32, PairF32Far, false>)
-                                : (pot ? k_p2p_tasks<PairF64, PairF64Far, true> : k_p2p_tasks<PairF64, PairF64Far, false>))
+      auto kern = w == 0 ? narrow_kernel(f32, pot, f32 ? tl.p2p_group_f32 : tl.p2p_group_f64)
                          : (f32 ? (pot ? k_p2p_wide<PairF32, PairF32Far, true> : k_p2p_wide<PairF32, PairF32Far, false>)
                                 : (pot ? k_p2p_wide<PairF64, PairF64Far, true> : k_p2p_wide<PairF64, PairF64Far, false>));
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, t1 - t0, (const P2PTask *)c->tasks.p + t0,

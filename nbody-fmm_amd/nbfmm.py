@@ -42,6 +42,24 @@ class Params(ctypes.Structure):
     ]
 
 
+class Tiles(ctypes.Structure):
+    _fields_ = [
+        ("p2p_group_f32", ctypes.c_int32),
+        ("p2p_group_f64", ctypes.c_int32),
+        ("p2p_narrow_max_f32", ctypes.c_int32),
+        ("p2p_narrow_max_f64", ctypes.c_int32),
+        ("p2p_early_share_f32", ctypes.c_double),
+        ("p2p_early_share_f64", ctypes.c_double),
+        ("p2p_early_launches_f32", ctypes.c_int32),
+        ("p2p_early_launches_f64", ctypes.c_int32),
+        ("m2l_slices", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved0"}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("depth", ctypes.c_int32),
@@ -91,7 +109,7 @@ EXPORTS = [
     "nbfmm_order_dataset_get", "nbfmm_order_dataset_put", "nbfmm_order_train", "nbfmm_order_grad",
     "nbfmm_order_predict", "nbfmm_eval_accel_checked_dev",
     "nbfmm_batch_eval_accel_dev", "nbfmm_batch_step_se_dev", "nbfmm_batch_step_rk4_dev",
-    "nbfmm_direct_accel_noalloc_dev",
+    "nbfmm_direct_accel_noalloc_dev", "nbfmm_set_tiles", "nbfmm_get_tiles",
 ]
 NOALLOC_SCRATCH_BYTES = 48 << 20
 NOALLOC_MAX_TARGETS = 1 << 20
@@ -122,6 +140,8 @@ def lib():
         L.nbfmm_destroy.argtypes = [vp]
         L.nbfmm_set_params.argtypes = [vp, ctypes.POINTER(Params)]
         L.nbfmm_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.nbfmm_set_tiles.argtypes = [vp, ctypes.POINTER(Tiles)]
+        L.nbfmm_get_tiles.argtypes = [vp, ctypes.POINTER(Tiles)]
         L.nbfmm_set_timing.argtypes = [vp, ctypes.c_int]
         L.nbfmm_keep_lists.argtypes = [vp, ctypes.c_int]
         L.nbfmm_tree_sizes.argtypes = [vp, vp, vp, vp, vp]
@@ -336,6 +356,20 @@ class Context:
         s = Stats()
         _check(lib().nbfmm_get_stats(self.h, ctypes.byref(s)))
         return s
+
+    def tiles(self):
+        """The tile shapes in force (dict of nbfmm_tiles fields, defaults resolved)."""
+        t = Tiles()
+        _check(lib().nbfmm_get_tiles(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tiles(self, **kw):
+        """Changes the named tile fields (the others keep their current values)."""
+        t = Tiles()
+        _check(lib().nbfmm_get_tiles(self.h, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().nbfmm_set_tiles(self.h, ctypes.byref(t)))
 
     def set_timing(self, mode):
         """0 off, 1 on, 2 on with every phase (P2P included) run alone."""

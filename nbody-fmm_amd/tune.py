@@ -1,5 +1,6 @@
 """Accuracy-gated choice of the expansion order, the P2P cut, the leaf capacity and
-the distance-dependent M2L order tolerance (class_order_tol).
+the distance-dependent M2L order tolerance (class_order_tol), then of the tile
+shapes and launch split of the P2P and M2L kernels (tune_tiles).
 
 For a particle set resident on the device, every (order, p2p_cut, leaf capacity, tol) candidate
 is evaluated (one warm-up evaluation, then one timed with the library's
@@ -60,4 +61,55 @@ def tune(ctx, n, d_xyz, d_m, d_acc, tgt, ref, orders, cuts, leaves=(None,), gate
                     if err["l2"] <= gate and (best is None or ms < best["ms"]):
                         best = row
     ctx.set_params(class_order_tol=0.0)
+    return best, table
+
+
+# Tile fields searched per precision, in this order, with the values tried
+# (the built-in default is among them)
+TILE_GRID = {
+    1: [("p2p_group_f32", (8, 12, 16, 24)), ("p2p_narrow_max_f32", (16, 32, 64)),
+        ("p2p_early_share_f32", (0.3, 0.5, 0.7)), ("p2p_early_launches_f32", (8, 16, 24)), ("m2l_slices", (1, 2))],
+    0: [("p2p_group_f64", (4, 8, 12)), ("p2p_narrow_max_f64", (8, 16, 32)),
+        ("p2p_early_share_f64", (0.1, 0.2, 0.4)), ("p2p_early_launches_f64", (1, 4, 16)), ("m2l_slices", (1, 2))],
+}
+
+
+def _median_ms(ctx, run, reps):
+    run()
+    ms = []
+    for _ in range(reps):
+        run()
+        ms.append(ctx.stats().ms_total)
+    return float(np.median(ms))
+
+
+def tune_tiles(ctx, n, d_xyz, d_m, d_acc, precision, reps=5, log=None, grid=None, margin=0.01, run=None):
+    """Coordinate search over the tile fields of TILE_GRID[precision] at the
+    context's parameters (`run` performs one evaluation; by default
+    nbfmm_eval_accel_dev of the given particles): each field in turn takes the value of lowest median
+    evaluation time (over `reps` evaluations) with the others fixed; a value
+    replaces the current one only when faster by more than `margin`.  Tile
+    shapes change speed, not results (beyond the rounding of the near-field
+    atomics).  Returns (tiles dict, table of every trial)."""
+    grid = grid if grid is not None else TILE_GRID[precision]
+    if run is None:  # one evaluation of the particle set (the batch evaluator for batches)
+        def run():
+            ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
+    best = ctx.tiles()
+    best_ms = _median_ms(ctx, run, reps)
+    table = [{"tiles": dict(best), "ms": round(best_ms, 3)}]
+    for field, values in grid:
+        cur = best[field]
+        for v in values:
+            if v == cur:
+                continue
+            ctx.set_tiles(**{field: v})
+            ms = _median_ms(ctx, run, reps)
+            table.append({"tiles": ctx.tiles(), "ms": round(ms, 3)})
+            if log:
+                log(f"[tiles] precision={precision} {field}={v}: {ms:.2f} ms (best {best_ms:.2f})")
+            if ms < best_ms * (1.0 - margin):
+                best, best_ms = ctx.tiles(), ms
+            ctx.set_tiles(**best)
+    ctx.set_tiles(**best)
     return best, table
