@@ -42,6 +42,7 @@ int check_params(const nbfmm_params *p) {
     return fail(NBFMM_ERR_ARG, "class_order_tol must be in [0, 1]");
   if (p->class_order_tol > 0.0 && (p->order < (uint32_t)kOrderLo || p->order > (uint32_t)kOrderHi))
     return fail(NBFMM_ERR_ARG, "class orders need 4 <= order <= 14");
+  if (p->mixed_fp64_levels < 0) return fail(NBFMM_ERR_ARG, "mixed_fp64_levels must be >= 0");
   if (p->order_select != NBFMM_ORDER_FIXED && p->order_select != NBFMM_ORDER_LEARNED)
     return fail(NBFMM_ERR_ARG, "unknown order_select");
   if (p->order_select == NBFMM_ORDER_LEARNED) {
@@ -229,7 +230,7 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &tasks, &task_off, &task_cnt, &dpart,
-                 &tiles, &stab, &tmat, &tmat32, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
+                 &tiles, &stab, &tmat, &tmat32, &mexp64, &lexp64, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
                  &ord, &msum, &feat, &rmax, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
                  &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc, &bat_off, &bat_bb, &bat_st, &bat_x, &bat_m,
                  &bat_a};
@@ -259,6 +260,7 @@ void nbfmm_default_params(nbfmm_params *p) {
   p->order_select = NBFMM_ORDER_FIXED;
   p->order_min = 4;
   p->order_conf = 0.9;
+  p->mixed_fp64_levels = 0;
 }
 
 const char *nbfmm_last_error(void) { return g_err.c_str(); }

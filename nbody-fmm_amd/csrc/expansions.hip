@@ -776,9 +776,18 @@ void run_expansions(nbfmm_ctx *c) {
     NB_LAUNCH();
   }
   // ---- M2M, finest level first ----
+  // in mixed precision the parents of levels < lc translate in FP64 (on
+  // FP64 copies of the multipoles, rounded back to FP32 for M2L)
+  const int lc = f32 ? std::min(D, c->prm.mixed_fp64_levels) : 0;
   mark(c, 14);
-  for (int d = D - 1; d >= 0; --d)
+  for (int d = D - 1; d >= lc; --d)
     run_trans(c, false, KP, f32, bm[d + 1] - bm[d], m2m_cells + bm[d]);
+  if (lc > 0) {
+    double *M64 = dget<double>(c->mexp64, (size_t)nM * KP);
+    widen(c, (const float *)Mexp, (int64_t)nM * KP, M64);
+    for (int d = lc - 1; d >= 0; --d) run_trans64(c, false, KP, bm[d + 1] - bm[d], m2m_cells + bm[d], M64);
+    narrow(c, M64, (int64_t)nM * KP, (float *)Mexp);
+  }
   mark(c, 3);
   // ---- M2L (writes the locals of its targets) ----
   // The rest of P2P starts after the first pass's M2L contraction
@@ -793,8 +802,14 @@ void run_expansions(nbfmm_ctx *c) {
   P.n_p2m_terms = (int64_t)th[0]; // read back at run_m2l's synchronisation
   P.n_l2p_terms = (int64_t)th[1];
   mark(c, 13);
-  // ---- L2L, coarsest level first ----
-  for (int d = 0; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
+  // ---- L2L, coarsest level first (parents of levels < lc in FP64 in mixed precision) ----
+  if (lc > 0) {
+    double *L64 = dget<double>(c->lexp64, (size_t)nL * KP);
+    widen(c, (const float *)c->lexp.p, (int64_t)nL * KP, L64);
+    for (int d = 0; d < lc; ++d) run_trans64(c, true, KP, bl[d + 1] - bl[d], l2l_cells + bl[d], L64);
+    narrow(c, L64, (int64_t)nL * KP, (float *)c->lexp.p);
+  }
+  for (int d = lc; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
   mark(c, 4);
   // ---- L2P ----
   if (f32)

@@ -112,6 +112,10 @@ __global__ void k_to_f32(const double *__restrict__ in, int64_t n, float *__rest
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i < n) out[i] = (float)in[i];
 }
+__global__ void k_to_f64(const float *__restrict__ in, int64_t n, double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) out[i] = (double)in[i];
+}
 
 // ---- mixed-precision M2L as three-term bfloat16 products -------------------------------
 // An FP32 value x is split x = x1 + x2 + x3 with x1 = bf16(x), x2 = bf16(x - x1),
@@ -820,6 +824,24 @@ void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const in
     launch_trans<float>(c, l2l, KP, ncell, cells, slot, (const float *)c->trans32.p + off, (float *)X);
   else
     launch_trans<double>(c, l2l, KP, ncell, cells, slot, (const double *)c->trans.p + off, (double *)X);
+}
+
+void run_trans64(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *cells, double *X) {
+  const int32_t *slot = (const int32_t *)(l2l ? c->lslot.p : c->mslot.p);
+  const size_t off = l2l ? (size_t)8 * KP * KP : 0;
+  launch_trans<double>(c, l2l, KP, ncell, cells, slot, (const double *)c->trans.p + off, X);
+}
+
+void widen(nbfmm_ctx *c, const float *in, int64_t count, double *out) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_to_f64, dim3(nblk(count)), dim3(kT), 0, c->stream, in, count, out);
+  NB_LAUNCH();
+}
+
+void narrow(nbfmm_ctx *c, const double *in, int64_t count, float *out) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_to_f32, dim3(nblk(count)), dim3(kT), 0, c->stream, in, count, out);
+  NB_LAUNCH();
 }
 
 void trans_to_f32(nbfmm_ctx *c, int KP) {

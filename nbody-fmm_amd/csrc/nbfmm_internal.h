@@ -189,6 +189,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf tasks, task_off, task_cnt, dpart;
   // expansions
   nbfmm::DBuf stab, tmat, tmat32, trans, trans32, mode_m, mode_l, mslot, lslot, eslot, up_cells, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
+  nbfmm::DBuf mexp64, lexp64; // FP64 copies for the coarse M2M / L2L levels of mixed precision
   std::vector<int32_t> h_cls_off; // host schedule of the class-sorted M2L
   std::vector<int4> h_m2l_items;
   // integrator scratch
@@ -248,6 +249,11 @@ void dataset_put(nbfmm_ctx *c, int64_t n, const float *feat_h, const int8_t *lab
 void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p);
 // M2M (l2l false) or L2L (true) for `ncell` parents listed in `cells` (device)
 void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells);
+// the same in FP64 on the expansions X (FP64, slot-indexed like the context's)
+void run_trans64(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *cells, double *X);
+// count values between FP32 and FP64 arrays (stream-ordered)
+void widen(nbfmm_ctx *c, const float *in, int64_t count, double *out);
+void narrow(nbfmm_ctx *c, const double *in, int64_t count, float *out);
 void trans_to_f32(nbfmm_ctx *c, int KP);
 // p2p.hip
 // P2P on the second stream: fork_p2p marks the point of the main stream it

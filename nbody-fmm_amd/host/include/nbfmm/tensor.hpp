@@ -2,7 +2,8 @@
 // Same interface as the reference's Tensor (src/math_utils/tensor.hpp:18-218):
 // runtime dimension, element access, +, -, unary -, scalar * and /, compound
 // assignment, ==, squared_norm, norm, stream output.  Storage is inline (no
-// heap) for dimensions up to 3.  Errors throw instead of ending the process:
+// heap) for dimensions up to 3 (the reference's std::vector takes any; a
+// larger dimension throws std::invalid_argument here).  Errors throw instead of ending the process:
 // out-of-range access -> std::out_of_range (reference: std::exit(1),
 // tensor.cpp:47-63); dimension mismatch -> std::invalid_argument (reference:
 // std::abort, tensor.cpp:65-69).
@@ -16,7 +17,7 @@ class Tensor {
  public:
   static constexpr size_t kMaxDim = 3;
   Tensor() = default;
-  explicit Tensor(size_t dim);
+  Tensor(size_t dim); // implicit, as the reference's (tensor.hpp:35)
   Tensor(std::initializer_list<double> init);
 
   const double &operator[](size_t i) const;

@@ -39,6 +39,7 @@ class Params(ctypes.Structure):
         ("order_conf", ctypes.c_double),
         ("cubic_box", ctypes.c_int32),
         ("class_order_tol", ctypes.c_double),
+        ("mixed_fp64_levels", ctypes.c_int32),
     ]
 
 

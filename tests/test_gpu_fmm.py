@@ -366,6 +366,23 @@ def test_mixed_parity(ctx, p, cut):
     assert err < 2e-6
 
 
+def test_mixed_fp64_levels(ctx):
+    # mixed precision with the coarse M2M/L2L levels (or all of them) in FP64:
+    # the same FP64 oracle bar; the all-FP64 translations come closer to it
+    xyz, m = workloads.plummer(40000, seed=37)
+    b = orc.fmm_exact(xyz, m, 10, 32, G=1.0, center_mode=0, p2p_cut=256)
+    err = {}
+    try:
+        for k in (0, 3, 99):
+            ctx.set_params(order=10, leaf_capacity=32, p2p_cut=256, G=1.0, eps2=0.0, precision=1, mixed_fp64_levels=k)
+            err[k] = _rel(ctx.eval_accel(xyz, m), b)
+    finally:
+        ctx.set_params(precision=0, mixed_fp64_levels=0)
+    print(f"[mixed-fp64-levels] rel L2 vs FP64 oracle {err}")
+    assert all(e < 2e-6 for e in err.values())
+    assert err[99] <= err[0] * 1.05
+
+
 def test_mixed_all_p2p_vs_direct(ctx):
     # everything direct: the FP32 compensated P2P against the FP64 direct sum,
     # within the metric's 1e-6 (FP32 offsets from the task anchor bound the

@@ -202,8 +202,24 @@ def test_learned_selection_end_to_end():
         print(f"[learned] learned (max order {p_fix}): err {err:.2e}, fell back {fb}, mean order {st.mean_order:.2f}, "
               f"M2L {st.flop_m2l:.3e} flop ({st.flop_m2l / flop_fix:.2f} of fixed order {p_fix})")
         assert err <= gate and st.fallback == int(fb)
-        if not fb:
-            assert st.flop_m2l < flop_fix
+        # the gain: the learned orders meet the gate by themselves (no
+        # fallback) with less M2L work than the lowest passing fixed order
+        assert not fb
+        assert st.flop_m2l < flop_fix
+        # and the wall time of the two, measured (the selection's features,
+        # inference and per-order launches are paid against the M2L saved)
+        def wall(select):
+            c.set_params(order=p_fix, order_select=select)
+            c.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+            t = nbfmm.Timer(c.stream)
+            t.start()
+            for _ in range(5):
+                c.eval_accel_dev(n, dx.ptr, dm.ptr, da.ptr)
+            return t.stop() / 5
+        ms_fix, ms_learned = wall(0), wall(1)
+        print(f"[learned] wall time per evaluation: fixed order {p_fix} {ms_fix:.2f} ms, learned {ms_learned:.2f} ms "
+              f"({ms_learned / ms_fix:.2f}x)")
+        c.set_params(order_select=1)
         # a model that picks order 4 everywhere fails the check: the fallback runs and meets the gate
         w = c.order_model_get()
         c.order_model_set(_const_model(0))
