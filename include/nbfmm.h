@@ -289,6 +289,12 @@ typedef struct nbfmm_tiles {
   int32_t p2p_early_launches_f64; /* the same for FP64 (default 1) */
   int32_t m2l_slices;         /* row-block slices of a class operator per M2L contraction workgroup: 1 = the
                                  fewest that fit the LDS (default), 2 = one more (half the LDS, more workgroups) */
+  int32_t p2p_queue;          /* P2P scheduling: 0 (default) one workgroup per task; 1 a persistent launch over an
+                                 atomic work queue (wide tasks first) for the share after the up-pass; 2 the same
+                                 for the first share too (one launch per share, every system of a batch) */
+  int32_t p2p_queue_waves0;   /* wavefronts per CU of the first share's persistent launch (p2p_queue 2), 1..16
+                                 (default 8; the later share takes all the device holds, 16) */
+  int32_t p2p_queue_chunk;    /* tasks a wavefront takes per atomic grab of the queue, 1..64 (default 4) */
   int32_t reserved0;
 } nbfmm_tiles;
 /* Tiles in force for later evaluations; fields are validated (NBFMM_ERR_ARG). */

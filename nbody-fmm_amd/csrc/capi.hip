@@ -229,7 +229,7 @@ nbfmm_ctx::~nbfmm_ctx() {
   DBuf *all[] = {&xyz_in, &m_in, &acc_out, &key[0], &key[1], &idx[0], &idx[1], &sort_hist, &scan_tmp, &act_par, &pos,
                  &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
-                 &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &tasks, &task_off, &task_cnt, &dpart,
+                 &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &tasks, &task_off, &task_cnt, &dpart, &p2p_ctr,
                  &tiles, &stab, &tmat, &tmat32, &mexp64, &lexp64, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
                  &ord, &msum, &feat, &rmax, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
                  &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc, &bat_off, &bat_bb, &bat_st, &bat_x, &bat_m,
@@ -330,6 +330,8 @@ int nbfmm_set_tiles(nbfmm_ctx *c, const nbfmm_tiles *t) {
   if (r.p2p_early_launches_f32 == 0) r.p2p_early_launches_f32 = d.p2p_early_launches_f32;
   if (r.p2p_early_launches_f64 == 0) r.p2p_early_launches_f64 = d.p2p_early_launches_f64;
   if (r.m2l_slices == 0) r.m2l_slices = d.m2l_slices;
+  if (r.p2p_queue_waves0 == 0) r.p2p_queue_waves0 = d.p2p_queue_waves0;
+  if (r.p2p_queue_chunk == 0) r.p2p_queue_chunk = d.p2p_queue_chunk;
   r.reserved0 = 0;
   std::string why;
   if (!tiles_ok(r, why)) return fail(NBFMM_ERR_ARG, why);

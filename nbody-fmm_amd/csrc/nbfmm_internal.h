@@ -186,7 +186,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf il, m2l_src, m2l_cls, sub_src;
   nbfmm::DBuf src_slot, tgt_slot, src_cells, tgt_cells, is_src;
   nbfmm::DBuf sub_pairs, tiles;
-  nbfmm::DBuf tasks, task_off, task_cnt, dpart;
+  nbfmm::DBuf tasks, task_off, task_cnt, dpart, p2p_ctr;
   // expansions
   nbfmm::DBuf stab, tmat, tmat32, trans, trans32, mode_m, mode_l, mslot, lslot, eslot, up_cells, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
   nbfmm::DBuf mexp64, lexp64; // FP64 copies for the coarse M2M / L2L levels of mixed precision

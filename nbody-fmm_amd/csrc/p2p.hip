@@ -322,6 +322,142 @@ constexpr int p2p_waves() {
 }
 constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
 
+// One narrow task (the body of k_p2p_tasks and of the queue kernel's narrow
+// branch); st, lb, lp, tmp are the wavefront's LDS.
+template <class Tr, class TrFar, int KG, bool POT>
+__device__ __forceinline__ void narrow_task(const P2PTask &tk, const int32_t *__restrict__ nbr27,
+                                            const int32_t *__restrict__ sub_src, const int32_t *__restrict__ cbeg,
+                                            const int32_t *__restrict__ cend, const uint64_t *__restrict__ ckey,
+                                            const double4 *__restrict__ pos, double *__restrict__ grad,
+                                            double *__restrict__ pot, typename Tr::S *st, int32_t *lb, int32_t *lp,
+                                            int32_t *tmp, int lane) {
+  typedef typename Tr::S S;
+  typedef typename TrFar::S FS;
+  typedef typename Tr::A A;
+  const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+  double4 anchor = make_double4(0, 0, 0, 0);
+  if (Tr::kAnchor) {
+    for (int t = lane; t < nc; t += kW) {
+      const double4 q = pos[b + t];
+      anchor.x += q.x;
+      anchor.y += q.y;
+      anchor.z += q.z;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      anchor.x += __shfl_xor(anchor.x, off);
+      anchor.y += __shfl_xor(anchor.y, off);
+      anchor.z += __shfl_xor(anchor.z, off);
+    }
+    anchor.x /= nc;
+    anchor.y /= nc;
+    anchor.z /= nc;
+  }
+  for (int t = lane; t < nc; t += kW) st[t] = Tr::make(pos[b + t], anchor);
+  int32_t near_end;
+  const int32_t total = stage_list(tk, nbr27, sub_src, cbeg, cend, ckey, lb, lp, tmp, lane, near_end);
+  const int ne = tk.nlist;
+  const int32_t f0 = tk.f0, f1 = min(tk.f1, total);
+  const int32_t fm = max(f0, min(f1, near_end)); // [f0, fm) near cells, [fm, f1) the others
+  constexpr int kGroup = KG;
+  for (int t0 = 0; t0 < nc; t0 += kGroup) {
+    const int ng = min(kGroup, nc - t0);
+    A ax[kGroup], ay[kGroup], az[kGroup], ap[kGroup];
+#pragma unroll
+    for (int t = 0; t < kGroup; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0;
+    {
+      // a lane takes sources g and g + 64 (g = f0 + lane, + 128, ...): every
+      // target read serves two pairs; c1, c2 follow their list cells
+      const int32_t g0 = f0, g1 = fm;
+      int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
+      int32_t g = g0 + lane;
+      double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
+      if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
+      if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
+      for (; g < g1; g += 2 * kW) {
+        const double4 q1 = p1, q2 = p2;
+        const bool has2 = g + kW < g1;
+        // the next sources' loads are in flight while these are evaluated
+        if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
+        if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
+        const S x1 = Tr::make(q1, anchor);
+        S x2 = Tr::make(has2 ? q2 : q1, anchor);
+        if (!has2) Tr::massless(x2); // contributes nothing
+#pragma unroll
+        for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
+          if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
+#pragma unroll
+            for (int t = t4; t < t4 + Tr::kStep; ++t) {
+              const S xt = Tr::tgt(st[t0 + t]);
+              Tr::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
+              Tr::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
+            }
+      }
+    }
+    // the far loop's set-up (cell search, first loads) stays after the near
+    // loop: moved ahead of it, its registers would be held through it
+    asm volatile("" ::: "memory");
+    {
+      // the same over the sources of the cells that do not touch the target cell
+      const int32_t g0 = fm, g1 = f1;
+      int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
+      int32_t g = g0 + lane;
+      double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
+      if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
+      if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
+      for (; g < g1; g += 2 * kW) {
+        const double4 q1 = p1, q2 = p2;
+        const bool has2 = g + kW < g1;
+        // the next sources' loads are in flight while these are evaluated
+        if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
+        if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
+        const FS x1 = TrFar::make(q1, anchor);
+        FS x2 = TrFar::make(has2 ? q2 : q1, anchor);
+        if (!has2) TrFar::massless(x2); // contributes nothing
+#pragma unroll
+        for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
+          if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
+#pragma unroll
+            for (int t = t4; t < t4 + Tr::kStep; ++t) {
+              const FS xt = TrFar::tgt(st[t0 + t]);
+              TrFar::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
+              TrFar::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
+            }
+      }
+    }
+
+    // each component's kGroup sums (zero-padded to a power of two RG) meet
+    // in one transposed FP64 reduction: afterwards lane l holds the total
+    // of target l >> (6 - log2 RG)
+    constexpr int RG = pow2_at_least(kGroup);
+    constexpr int kSpan = kW / RG; // lanes sharing one target's total
+    const int tl = lane / kSpan;
+    const bool writer = (lane % kSpan) == 0 && tl < ng;
+    const int64_t i = b + t0 + tl;
+    double v[RG];
+#pragma unroll
+    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ax[t] : 0.0;
+    const double gx = tr_reduce<RG>(v, lane);
+#pragma unroll
+    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ay[t] : 0.0;
+    const double gy = tr_reduce<RG>(v, lane);
+#pragma unroll
+    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)az[t] : 0.0;
+    const double gz = tr_reduce<RG>(v, lane);
+    double gp = 0.0;
+    if (POT) {
+#pragma unroll
+      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ap[t] : 0.0;
+      gp = tr_reduce<RG>(v, lane);
+    }
+    if (writer) {
+      atomicAdd(&grad[3 * i], gx);
+      atomicAdd(&grad[3 * i + 1], gy);
+      atomicAdd(&grad[3 * i + 2], gz);
+      if (POT) atomicAdd(&pot[i], gp);
+    }
+  }
+}
+
 template <class Tr, class TrFar, int KG, bool POT>
 __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_waves<Tr, KG, POT>()))) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
@@ -330,139 +466,15 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
                                                   const uint64_t *__restrict__ ckey,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad,
                                                   double *__restrict__ pot) {
-  typedef typename Tr::S S;
-  typedef typename TrFar::S FS;
-  typedef typename Tr::A A;
-  __shared__ S st[kTaskTargets + KG]; // the last pass reads up to KG - 1 slots past nc
+  __shared__ typename Tr::S st[kTaskTargets + KG]; // the last pass reads up to KG - 1 slots past nc
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   __shared__ int32_t tmp[54];
   const int lane = threadIdx.x;
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
     __syncthreads(); // the previous task's reads of lb, lp, st are done
-    const P2PTask tk = tasks[task];
-    const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
-    double4 anchor = make_double4(0, 0, 0, 0);
-    if (Tr::kAnchor) {
-      for (int t = lane; t < nc; t += kW) {
-        const double4 q = pos[b + t];
-        anchor.x += q.x;
-        anchor.y += q.y;
-        anchor.z += q.z;
-      }
-      for (int off = 32; off >= 1; off >>= 1) {
-        anchor.x += __shfl_xor(anchor.x, off);
-        anchor.y += __shfl_xor(anchor.y, off);
-        anchor.z += __shfl_xor(anchor.z, off);
-      }
-      anchor.x /= nc;
-      anchor.y /= nc;
-      anchor.z /= nc;
-    }
-    for (int t = lane; t < nc; t += kW) st[t] = Tr::make(pos[b + t], anchor);
-    int32_t near_end;
-    const int32_t total = stage_list(tk, nbr27, sub_src, cbeg, cend, ckey, lb, lp, tmp, lane, near_end);
-    const int ne = tk.nlist;
-    const int32_t f0 = tk.f0, f1 = min(tk.f1, total);
-    const int32_t fm = max(f0, min(f1, near_end)); // [f0, fm) near cells, [fm, f1) the others
-    constexpr int kGroup = KG;
-    for (int t0 = 0; t0 < nc; t0 += kGroup) {
-      const int ng = min(kGroup, nc - t0);
-      A ax[kGroup], ay[kGroup], az[kGroup], ap[kGroup];
-#pragma unroll
-      for (int t = 0; t < kGroup; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0;
-      {
-        // a lane takes sources g and g + 64 (g = f0 + lane, + 128, ...): every
-        // target read serves two pairs; c1, c2 follow their list cells
-        const int32_t g0 = f0, g1 = fm;
-        int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
-        int32_t g = g0 + lane;
-        double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
-        if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
-        if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
-        for (; g < g1; g += 2 * kW) {
-          const double4 q1 = p1, q2 = p2;
-          const bool has2 = g + kW < g1;
-          // the next sources' loads are in flight while these are evaluated
-          if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
-          if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
-          const S x1 = Tr::make(q1, anchor);
-          S x2 = Tr::make(has2 ? q2 : q1, anchor);
-          if (!has2) Tr::massless(x2); // contributes nothing
-#pragma unroll
-          for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
-            if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
-#pragma unroll
-              for (int t = t4; t < t4 + Tr::kStep; ++t) {
-                const S xt = Tr::tgt(st[t0 + t]);
-                Tr::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
-                Tr::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
-              }
-        }
-      }
-      // the far loop's set-up (cell search, first loads) stays after the near
-      // loop: moved ahead of it, its registers would be held through it
-      asm volatile("" ::: "memory");
-      {
-        // the same over the sources of the cells that do not touch the target cell
-        const int32_t g0 = fm, g1 = f1;
-        int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
-        int32_t g = g0 + lane;
-        double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
-        if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
-        if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
-        for (; g < g1; g += 2 * kW) {
-          const double4 q1 = p1, q2 = p2;
-          const bool has2 = g + kW < g1;
-          // the next sources' loads are in flight while these are evaluated
-          if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
-          if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
-          const FS x1 = TrFar::make(q1, anchor);
-          FS x2 = TrFar::make(has2 ? q2 : q1, anchor);
-          if (!has2) TrFar::massless(x2); // contributes nothing
-#pragma unroll
-          for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
-            if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
-#pragma unroll
-              for (int t = t4; t < t4 + Tr::kStep; ++t) {
-                const FS xt = TrFar::tgt(st[t0 + t]);
-                TrFar::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
-                TrFar::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
-              }
-        }
-      }
-
-      // each component's kGroup sums (zero-padded to a power of two RG) meet
-      // in one transposed FP64 reduction: afterwards lane l holds the total
-      // of target l >> (6 - log2 RG)
-      constexpr int RG = pow2_at_least(kGroup);
-      constexpr int kSpan = kW / RG; // lanes sharing one target's total
-      const int tl = lane / kSpan;
-      const bool writer = (lane % kSpan) == 0 && tl < ng;
-      const int64_t i = b + t0 + tl;
-      double v[RG];
-#pragma unroll
-      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ax[t] : 0.0;
-      const double gx = tr_reduce<RG>(v, lane);
-#pragma unroll
-      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ay[t] : 0.0;
-      const double gy = tr_reduce<RG>(v, lane);
-#pragma unroll
-      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)az[t] : 0.0;
-      const double gz = tr_reduce<RG>(v, lane);
-      double gp = 0.0;
-      if (POT) {
-#pragma unroll
-        for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ap[t] : 0.0;
-        gp = tr_reduce<RG>(v, lane);
-      }
-      if (writer) {
-        atomicAdd(&grad[3 * i], gx);
-        atomicAdd(&grad[3 * i + 1], gy);
-        atomicAdd(&grad[3 * i + 2], gz);
-        if (POT) atomicAdd(&pot[i], gp);
-      }
-    }
+    narrow_task<Tr, TrFar, KG, POT>(tasks[task], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot, st, lb, lp, tmp,
+                                    lane);
   }
 }
 
@@ -518,25 +530,17 @@ __device__ __forceinline__ void wide_sweep(int32_t g0, int32_t g1, const int32_t
 // walked by wide_sweep, the cells touching the target cell with the pair Tr
 // and the others with TrFar, and the slots are combined with xor-shuffles at
 // the end.  One wavefront per task.
+// One wide task (the body of k_p2p_wide and of the queue kernel's wide
+// branch); sh, shf, lb, lp, tmp are the wavefront's LDS.
 template <class Tr, class TrFar, bool POT>
-__global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *__restrict__ tasks,
-                                                  const int32_t *__restrict__ nbr27,
-                                                  const int32_t *__restrict__ sub_src,
-                                                  const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
-                                                  const uint64_t *__restrict__ ckey,
-                                                  const double4 *__restrict__ pos, double *__restrict__ grad,
-                                                  double *__restrict__ pot) {
+__device__ __forceinline__ void wide_task(const P2PTask &tk, const int32_t *__restrict__ nbr27,
+                                          const int32_t *__restrict__ sub_src, const int32_t *__restrict__ cbeg,
+                                          const int32_t *__restrict__ cend, const uint64_t *__restrict__ ckey,
+                                          const double4 *__restrict__ pos, double *__restrict__ grad,
+                                          double *__restrict__ pot, typename Tr::S *sh, typename TrFar::S *shf,
+                                          int32_t *lb, int32_t *lp, int32_t *tmp, int lane) {
   typedef typename Tr::S S;
   typedef typename Tr::A A;
-  __shared__ S sh[kW];
-  __shared__ typename TrFar::S shf[kW];
-  __shared__ int32_t lb[kMaxList];
-  __shared__ int32_t lp[kMaxList];
-  __shared__ int32_t tmp[54];
-  const int lane = threadIdx.x;
-  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
-  __syncthreads(); // the previous task's reads of lb, lp, sh are done
-  const P2PTask tk = tasks[task];
   const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
   double4 anchor = make_double4(0, 0, 0, 0);
   if (Tr::kAnchor) {
@@ -597,7 +601,76 @@ __global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *_
       if (POT) atomicAdd(&pot[ii[q]], gp);
     }
   }
+}
+
+template <class Tr, class TrFar, bool POT>
+__global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *__restrict__ tasks,
+                                                  const int32_t *__restrict__ nbr27,
+                                                  const int32_t *__restrict__ sub_src,
+                                                  const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
+                                                  const uint64_t *__restrict__ ckey,
+                                                  const double4 *__restrict__ pos, double *__restrict__ grad,
+                                                  double *__restrict__ pot) {
+  __shared__ typename Tr::S sh[kW];
+  __shared__ typename TrFar::S shf[kW];
+  __shared__ int32_t lb[kMaxList];
+  __shared__ int32_t lp[kMaxList];
+  __shared__ int32_t tmp[54];
+  const int lane = threadIdx.x;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    __syncthreads(); // the previous task's reads of lb, lp, sh are done
+    wide_task<Tr, TrFar, POT>(tasks[task], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot, sh, shf, lb, lp, tmp, lane);
   }
+}
+
+// Persistent P2P over a work queue: a fixed grid of one-wavefront workgroups
+// takes tasks in queue order from an atomic counter until the queue is
+// drained, so one launch covers every task of the share (every system's, in
+// a batch) however uneven their sizes.  Queue index q < nwide is the wide
+// task wide0 + q (the larger ones first), the others the narrow tasks
+// narrow0 + (q - nwide).  A wavefront takes `chunk` consecutive tasks per
+// atomic: one task per grab serialises some 10^5 atomics on one address.
+// *ctr is zero at the launch.
+template <class Tr, class TrFar, int KG, bool POT>
+__global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_waves<Tr, KG, POT>()))) k_p2p_queue(
+    int64_t wide0, int64_t nwide, int64_t narrow0, int64_t nnarrow, int chunk, int32_t *__restrict__ ctr,
+    const P2PTask *__restrict__ tasks, const int32_t *__restrict__ nbr27, const int32_t *__restrict__ sub_src,
+    const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend, const uint64_t *__restrict__ ckey,
+    const double4 *__restrict__ pos, double *__restrict__ grad, double *__restrict__ pot) {
+  typedef typename Tr::S S;
+  typedef typename TrFar::S FS;
+  // the narrow branch's targets and the wide branch's source chunks share one buffer
+  constexpr size_t kNarrowBytes = sizeof(S) * (kTaskTargets + KG);
+  constexpr size_t kWideBytes = sizeof(S) * kW + sizeof(FS) * kW;
+  __shared__ __align__(16) char buf[kNarrowBytes > kWideBytes ? kNarrowBytes : kWideBytes];
+  __shared__ int32_t lb[kMaxList];
+  __shared__ int32_t lp[kMaxList];
+  __shared__ int32_t tmp[54];
+  __shared__ int32_t next;
+  const int lane = threadIdx.x;
+  const int64_t total = nwide + nnarrow;
+  for (;;) {
+    __syncthreads(); // the previous task's LDS reads are done
+    if (lane == 0) next = atomicAdd(ctr, chunk);
+    __syncthreads();
+    const int64_t q0 = next;
+    if (q0 >= total) break; // uniform: every wavefront leaves once the queue is drained
+    const int64_t q1 = min(q0 + chunk, total);
+    for (int64_t q = q0; q < q1; ++q) {
+      if (q > q0) __syncthreads();
+      if (q < nwide)
+        wide_task<Tr, TrFar, POT>(tasks[wide0 + q], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot,
+                                  reinterpret_cast<S *>(buf), reinterpret_cast<FS *>(buf + sizeof(S) * kW), lb, lp,
+                                  tmp, lane);
+      else
+        narrow_task<Tr, TrFar, KG, POT>(tasks[narrow0 + (q - nwide)], nbr27, sub_src, cbeg, cend, ckey, pos, grad,
+                                        pot, reinterpret_cast<S *>(buf), lb, lp, tmp, lane);
+    }
+  }
+}
+
+__global__ void k_zero_i32(int32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 // acc = G * grad; with potentials, phi = -G * sum_j m_j / r_ij
@@ -743,6 +816,33 @@ P2PKernel narrow_kernel(bool f32, bool pot, int group) {
   throw Error(NBFMM_ERR_ARG, "no P2P kernel for target group " + std::to_string(group));
 }
 
+typedef void (*P2PQueue)(int64_t, int64_t, int64_t, int64_t, int, int32_t *, const P2PTask *, const int32_t *,
+                         const int32_t *, const int32_t *, const int32_t *, const uint64_t *, const double4 *,
+                         double *, double *);
+
+// the queue kernel's instance for a target group size
+P2PQueue queue_kernel(bool f32, bool pot, int group) {
+#define NB_QUEUE(TR, FAR, G) \
+  case G:                    \
+    return pot ? k_p2p_queue<TR, FAR, G, true> : k_p2p_queue<TR, FAR, G, false>;
+  if (f32) {
+    switch (group) {
+      NB_QUEUE(PairF32, PairF32Far, 8)
+      NB_QUEUE(PairF32, PairF32Far, 12)
+      NB_QUEUE(PairF32, PairF32Far, 16)
+      NB_QUEUE(PairF32, PairF32Far, 24)
+    }
+  } else {
+    switch (group) {
+      NB_QUEUE(PairF64, PairF64Far, 4)
+      NB_QUEUE(PairF64, PairF64Far, 8)
+      NB_QUEUE(PairF64, PairF64Far, 12)
+    }
+  }
+#undef NB_QUEUE
+  throw Error(NBFMM_ERR_ARG, "no P2P kernel for target group " + std::to_string(group));
+}
+
 } // namespace
 
 // Share of the P2P tasks run in part 0 (mixed precision, FP64).
@@ -776,6 +876,9 @@ nbfmm_tiles default_tiles() {
   t.p2p_early_launches_f32 = NBFMM_P2P_CHUNKS0;
   t.p2p_early_launches_f64 = NBFMM_P2P_CHUNKS0_F64;
   t.m2l_slices = 1;
+  t.p2p_queue = 0;
+  t.p2p_queue_waves0 = 8;
+  t.p2p_queue_chunk = 4;
   return t;
 }
 
@@ -794,6 +897,9 @@ bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
   else if (t.p2p_early_launches_f32 < 1 || t.p2p_early_launches_f32 > 64) why = "p2p_early_launches_f32 must be in [1, 64]";
   else if (t.p2p_early_launches_f64 < 1 || t.p2p_early_launches_f64 > 64) why = "p2p_early_launches_f64 must be in [1, 64]";
   else if (t.m2l_slices != 1 && t.m2l_slices != 2) why = "m2l_slices must be 1 or 2";
+  else if (t.p2p_queue < 0 || t.p2p_queue > 2) why = "p2p_queue must be 0, 1 or 2";
+  else if (t.p2p_queue_waves0 < 1 || t.p2p_queue_waves0 > 16) why = "p2p_queue_waves0 must be in [1, 16]";
+  else if (t.p2p_queue_chunk < 1 || t.p2p_queue_chunk > 64) why = "p2p_queue_chunk must be in [1, 64]";
   else return true;
   return false;
 }
@@ -840,6 +946,29 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   }
   if (part != 2) mark2(c, part == 0 ? 9 : 11);
   if (part == 2) mark2(c, 11);
+  if (tl.p2p_queue == 2 || (tl.p2p_queue == 1 && part >= 1)) {
+    // one persistent launch over this share's wide then narrow tasks
+    const int64_t nwide = r[1][1] - r[1][0], nnar = r[0][1] - r[0][0];
+    if (nwide + nnar > 0) {
+      int32_t *ctr = dget<int32_t>(c->p2p_ctr, 4) + part;
+      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s2, ctr, 1);
+      NB_LAUNCH();
+      // a CU holds 16 of these wavefronts (10 KB of LDS and 128 VGPRs each): 16 per CU fill the device
+      const int per_cu = part == 0 ? tl.p2p_queue_waves0 : 16;
+      const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(nwide + nnar, (int64_t)c->n_cu * per_cu));
+      const int chunk = tl.p2p_queue_chunk;
+      if (nwide + nnar + grid * chunk >= INT32_MAX) throw Error(NBFMM_ERR_ARG, "too many P2P tasks for the work queue");
+      hipLaunchKernelGGL(queue_kernel(f32, pot, f32 ? tl.p2p_group_f32 : tl.p2p_group_f64), dim3((unsigned)grid),
+                         dim3(kW), 0, s2, r[1][0], nwide, r[0][0], nnar, chunk, ctr, (const P2PTask *)c->tasks.p,
+                         (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
+                         (const int32_t *)c->cend.p, (const uint64_t *)c->ckey.p, (const double4 *)c->pos.p, gn,
+                         pn);
+      NB_LAUNCH();
+    }
+    mark2(c, part == 0 ? 10 : 12);
+    if (part >= 1) NB_HIP(hipEventRecord(c->ev_join, s2));
+    return;
+  }
   // part 1 in mixed precision starts with the wide tasks, so that its last
   // workgroups are short narrow tasks (measured 0.2 ms sooner at 4M; FP64,
   // whose narrow tasks are longer, measured slower that way)

@@ -54,6 +54,9 @@ class Tiles(ctypes.Structure):
         ("p2p_early_launches_f32", ctypes.c_int32),
         ("p2p_early_launches_f64", ctypes.c_int32),
         ("m2l_slices", ctypes.c_int32),
+        ("p2p_queue", ctypes.c_int32),
+        ("p2p_queue_waves0", ctypes.c_int32),
+        ("p2p_queue_chunk", ctypes.c_int32),
         ("reserved0", ctypes.c_int32),
     ]
 
