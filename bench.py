@@ -140,7 +140,7 @@ def phase_table(st, ms, prec, order, t_eval_ms):
     w = 8 if prec == "fp64" else 4
     K = (order + 1) ** 2
     N = st["_n"]
-    k = st["depth_iterations"]
+    k = st["depth_sorts"]
     peak = PEAK["fp64" if prec == "fp64" else "fp32"] * 1e12
     p2p_alone = ms["ms_p2p_kernel"]
     rows = [

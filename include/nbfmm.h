@@ -341,6 +341,9 @@ typedef struct nbfmm_stats {
   int64_t n_l2p_cells;      /* cells evaluating their local at their particles */
   double ms_m2m;            /* M2M, every level */
   double ms_l2l;            /* L2L, every level */
+  int32_t depth_sorts;      /* key sorts the depth loop ran: one when the previous evaluation's depth was
+                               confirmed from every level's fullest cell, else one per iteration (or one more) */
+  int32_t reserved1;
 } nbfmm_stats;
 int nbfmm_get_stats(nbfmm_ctx *ctx, nbfmm_stats *stats);
 /* Per-phase events: 0 off, 1 on (the default), 2 on with the phases run one

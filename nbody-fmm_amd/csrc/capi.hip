@@ -79,6 +79,7 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
   std::memset(&S, 0, sizeof(S));
   S.depth = P.geo.depth;
   S.depth_iterations = P.iterations;
+  S.depth_sorts = P.depth_sorts;
   S.n_cells = P.n_cells;
   S.n_leaves = P.lvl_start[P.geo.depth + 1] - P.lvl_start[P.geo.depth];
   S.n_il = P.n_il;

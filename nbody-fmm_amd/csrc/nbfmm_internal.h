@@ -136,7 +136,8 @@ ClsOrd class_orders(const Geometry &geo, int p, double tol);
 struct Plan {
   int64_t n = 0;
   Geometry geo;
-  int iterations = 0;
+  int iterations = 0;  // iterations of the reference's depth loop
+  int depth_sorts = 0; // key sorts run to find the depth
   std::vector<int32_t> lvl_start; // depth+2, cells concatenated from level 0
   int64_t n_cells = 0, n_leaves = 0;
   int64_t n_il = 0, n_m2l = 0, n_sub = 0;
@@ -167,6 +168,7 @@ struct nbfmm_ctx {
   // phase's events time its kernels alone; the results are the same
   bool isolated = false;
   bool keep_lists = false;
+  int depth_guess = 0; // the last tree's depth: the first depth sorted at by the next
   int nbr_leaf = 0; // rows of the leaf level in nbr27: 0 none, 1 leaves that are not closed, 2 all
   double *phi_out = nullptr; // potential output of the current evaluation (device, input order) or null
   nbfmm::Plan plan;

@@ -339,6 +339,31 @@ __global__ void __launch_bounds__(kT) k_cells_fill(const uint64_t *__restrict__ 
   if (i < n) pcell[i] = up;
 }
 
+// Longest cell (most particles) of every level 0..D: cells are stored level
+// by level from L.s[d]; a thread walks its cells in increasing order, so its
+// level only moves forward.  out[0..D] is zero at the launch.
+__global__ void k_level_max(const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend, LevelStarts L,
+                            int D, uint32_t *out) {
+  __shared__ uint32_t s[kMaxDepth + 1];
+  if ((int)threadIdx.x <= D) s[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t nc = L.s[D + 1];
+  int d = 0;
+  uint32_t best = 0;
+  for (int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kT) {
+    if (c >= L.s[d + 1]) {
+      if (best) atomicMax(&s[d], best);
+      best = 0;
+      while (c >= L.s[d + 1]) ++d;
+    }
+    const uint32_t len = (uint32_t)(cend[c] - cbeg[c]);
+    best = len > best ? len : best;
+  }
+  if (best) atomicMax(&s[d], best);
+  __syncthreads();
+  if ((int)threadIdx.x <= D && s[threadIdx.x]) atomicMax(&out[threadIdx.x], s[threadIdx.x]);
+}
+
 __global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent, int32_t cb,
                          int32_t ce, int32_t *child8) {
   int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
@@ -400,6 +425,78 @@ int bit_width_u64(uint64_t v) {
 
 } // namespace
 
+// Every level's cells, the particles in key order and the neighbour tables,
+// from the keys sorted at `depth` (buffer `cur`).
+static void build_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, int depth, int cur, double *out6,
+                  double *h) {
+  Plan &P = c->plan;
+  const int64_t n = P.n;
+  hipStream_t s = c->stream;
+  P.geo.depth = depth;
+  if (cur != 0) { // keep the sorted keys/indices in buffer 0
+    std::swap(c->key[0], c->key[1]);
+    std::swap(c->idx[0], c->idx[1]);
+  }
+  const uint64_t *key = (const uint64_t *)c->key[0].p;
+  const int32_t *idx = (const int32_t *)c->idx[0].p;
+  double4 *pos = dget<double4>(c->pos, (size_t)n);
+  // the particle gather runs on the second stream, beside the cells, the
+  // neighbour tables and the lists; P2P follows it on that stream, and the
+  // main stream waits for it at the end of build_lists
+  NB_HIP(hipEventRecord(c->ev_pos, s));
+  NB_HIP(hipStreamWaitEvent(c->stream2, c->ev_pos, 0));
+  hipLaunchKernelGGL(k_gather_pos, dim3(nblk(n)), dim3(kT), 0, c->stream2, d_xyz, d_m, idx, n, pos);
+  NB_LAUNCH();
+  NB_HIP(hipEventRecord(c->ev_pos, c->stream2));
+
+  // ---- cells of every level ----
+  const int D = depth;
+  uint32_t *dcounts = (uint32_t *)(out6 + 16);
+  const int64_t nbl = nblk(n);
+  int32_t *bcnt = dget<int32_t>(c->flags, std::max<size_t>((size_t)n + 1, (size_t)(D + 1) * (size_t)nbl));
+  hipLaunchKernelGGL(k_level_blocks, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_level_offsets, dim3(D + 1), dim3(kOffThreads), 0, s, bcnt, (int)nbl, dcounts);
+  NB_LAUNCH();
+  uint32_t *counts = (uint32_t *)(h + 16); // pinned
+  NB_HIP(hipMemcpyAsync(counts, dcounts, sizeof(uint32_t) * (D + 1), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.lvl_start.assign(D + 2, 0);
+  for (int d = 0; d <= D; ++d) P.lvl_start[d + 1] = P.lvl_start[d] + (int32_t)counts[d];
+  P.n_leaves = counts[D];
+  const int64_t nc = P.lvl_start[D + 1];
+  P.n_cells = nc;
+  uint64_t *ckey = dget<uint64_t>(c->ckey, (size_t)nc);
+  int32_t *cbeg = dget<int32_t>(c->cbeg, (size_t)nc);
+  int32_t *cend = dget<int32_t>(c->cend, (size_t)nc);
+  int32_t *cpar = dget<int32_t>(c->cparent, (size_t)nc);
+  int32_t *pcell = dget<int32_t>(c->pcell, (size_t)n);
+  LevelStarts L{};
+  for (int d = 0; d <= D + 1; ++d) L.s[d] = P.lvl_start[d];
+  hipLaunchKernelGGL(k_cells_fill, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, (const int32_t *)bcnt, L, ckey,
+                     cbeg, cend, cpar, pcell);
+  NB_LAUNCH();
+  // neighbour tables
+  int32_t *child8 = dget<int32_t>(c->child8, (size_t)nc * 8);
+  int32_t *nbr27 = dget<int32_t>(c->nbr27, (size_t)nc * 27);
+  NB_HIP(hipMemsetAsync(child8, 0xff, sizeof(int32_t) * (size_t)nc * 8, s));
+  if (nc > 1) {
+    hipLaunchKernelGGL(k_child8, dim3(nblk(nc - 1)), dim3(kT), 0, s, ckey, cpar, 1, (int32_t)nc, child8);
+    NB_LAUNCH();
+  }
+  NB_HIP(hipMemsetAsync(nbr27, 0xff, sizeof(int32_t) * 27, s));
+  NB_HIP(hipMemsetAsync(nbr27 + 13, 0, sizeof(int32_t), s));
+  // the leaf level's rows follow in build_lists, for the leaves that are not
+  // closed (the only ones the evaluation reads); nbr_leaf_rows completes them
+  for (int d = 1; d < D; ++d) {
+    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
+    hipLaunchKernelGGL(k_nbr27, dim3(nblk((int64_t)(ce - cb) * 27)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d,
+                       nullptr, nbr27);
+    NB_LAUNCH();
+  }
+  c->nbr_leaf = D == 0 ? 2 : 0;
+}
+
 // ---------------------------------------------------------------------------------
 void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   Plan &P = c->plan;
@@ -438,112 +535,106 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   }
   // ---- depth loop (tree.cpp:32-69) ----
   const uint32_t thr = c->prm.leaf_capacity;
-  uint32_t maxcnt = (uint32_t)n;
-  int depth = 0;
   dget<uint64_t>(c->key[0], (size_t)n);
   dget<uint64_t>(c->key[1], (size_t)n);
   dget<int32_t>(c->idx[0], (size_t)n);
   dget<int32_t>(c->idx[1], (size_t)n);
-  int32_t *flags = dget<int32_t>(c->flags, (size_t)n + 1);
-  int32_t *fscan = dget<int32_t>(c->fscan, (size_t)n + 1);
   uint32_t *dmax = (uint32_t *)(out6 + 12);
+  uint32_t *lmax = (uint32_t *)(out6 + 26); // kMaxDepth + 1 values, below the other users of the area
+  uint32_t *hm = (uint32_t *)(h + 16);      // pinned
   int cur = 0;
   P.iterations = 0;
-  double ls[3];
-  do {
-    uint64_t nl = (uint64_t)((maxcnt - 1u) / thr + 1u);
-    depth += bit_width_u64(nl) / 3 + 1;
-    if (depth > kMaxDepth) depth = kMaxDepth;
-    uint64_t nld = 1ull << depth;
+  P.depth_sorts = 0;
+  // Morton keys at `depth`, sorted; buffer `cur` holds them
+  auto sort_at = [&](int depth) {
+    const uint64_t nld = 1ull << depth;
+    double ls[3];
     for (int d = 0; d < 3; ++d) ls[d] = (P.geo.hi[d] - P.geo.lo[d]) / (double)nld;
     hipLaunchKernelGGL(k_keys, dim3(nblk(n)), dim3(kT), 0, s, d_xyz, n, P.geo.lo[0], P.geo.lo[1], P.geo.lo[2],
                        ls[0], ls[1], ls[2], (uint32_t)nld, (uint64_t *)c->key[0].p, (int32_t *)c->idx[0].p);
     NB_LAUNCH();
     cur = radix_sort(c, n, 3 * depth, 0);
-    const uint64_t *k = (const uint64_t *)c->key[cur].p;
-    hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(kT), 0, s, k, n, flags);
+    P.depth_sorts++;
+  };
+  // the reference's step: the depth after `depth` when its fullest cell holds maxcnt
+  auto next_depth = [&](int depth, uint32_t maxcnt) {
+    const uint64_t nl = (uint64_t)((maxcnt - 1u) / thr + 1u);
+    return std::min(kMaxDepth, depth + bit_width_u64(nl) / 3 + 1);
+  };
+  // the loop's condition; a batch's lists start at level list_level (below
+  // its system cells), so its tree reaches that level even when every system
+  // fits in one leaf; for one system list_level is 1 and the loop is the reference's
+  auto more = [&](int depth, uint32_t maxcnt) {
+    return depth < kMaxDepth && (maxcnt > thr || depth < c->list_level);
+  };
+  int depth = 0;
+  uint32_t maxcnt = (uint32_t)n;
+  bool built = false, sorted = false;
+  const int guess = c->depth_guess;
+  if (guess >= 1 && guess <= kMaxDepth) {
+    // Sort once at the depth the previous evaluation ended at and build every
+    // level's cells: a level-d cell is a run of the level-d key, which is the
+    // sorted key's top 3d bits, so the fullest cell of every level <= guess is
+    // known, and the loop above is replayed on them.  It ends at `guess` (the
+    // usual case: particles move little between evaluations), before it
+    // (sorted again at that depth) or beyond it (the loop goes on from its
+    // last depth <= guess).  The result is the loop's in every case.
+    sort_at(guess);
+    build_levels(c, d_xyz, d_m, guess, cur, out6, h);
+    NB_HIP(hipMemsetAsync(lmax, 0, sizeof(uint32_t) * (kMaxDepth + 1), s));
+    LevelStarts L{};
+    for (int d = 0; d <= guess + 1; ++d) L.s[d] = P.lvl_start[d];
+    hipLaunchKernelGGL(k_level_max, dim3(256), dim3(kT), 0, s, (const int32_t *)c->cbeg.p,
+                       (const int32_t *)c->cend.p, L, guess, lmax);
     NB_LAUNCH();
-    scan_excl_i32(flags, fscan, n, c->scan_tmp, s);
-    // number of runs = fscan[n-1] + flags[n-1]
-    int32_t *starts = (int32_t *)c->idx[cur ^ 1].p; // free while not sorting
-    hipLaunchKernelGGL(k_run_starts, dim3(nblk(n)), dim3(kT), 0, s, flags, fscan, n, starts);
-    NB_LAUNCH();
-    NB_HIP(hipMemsetAsync(dmax, 0, sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_run_max, dim3((unsigned)std::min<int64_t>(1024, nblk(n))), dim3(kT), 0, s, starts, flags,
-                       fscan, n, dmax);
-    NB_LAUNCH();
-    uint32_t *hm = (uint32_t *)(h + 16); // pinned
-    NB_HIP(hipMemcpyAsync(hm, dmax, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NB_HIP(hipMemcpyAsync(hm, lmax, sizeof(uint32_t) * (guess + 1), hipMemcpyDeviceToHost, s));
     NB_HIP(hipStreamSynchronize(s));
-    maxcnt = hm[0];
-    P.iterations++;
-    P.n_leaves = hm[1];
-    // a batch's lists start at level list_level (below its system cells), so
-    // its tree reaches that level even when every system fits in one leaf;
-    // for one system list_level is 1 and the loop is the reference's
-  } while (depth < kMaxDepth && (maxcnt > thr || depth < c->list_level));
-  P.geo.depth = depth;
-  if (cur != 0) { // keep the sorted keys/indices in buffer 0
-    std::swap(c->key[0], c->key[1]);
-    std::swap(c->idx[0], c->idx[1]);
+    for (;;) {
+      const int nd = next_depth(depth, maxcnt);
+      if (nd > guess) break;
+      depth = nd;
+      maxcnt = hm[depth];
+      P.iterations++;
+      if (!more(depth, maxcnt)) break;
+    }
+    if (!more(depth, maxcnt) && depth == guess) built = true;
+    else {
+      // the particle gather on the second stream reads the sorted indices
+      // that the next sort overwrites
+      NB_HIP(hipStreamWaitEvent(s, c->ev_pos, 0));
+      if (!more(depth, maxcnt)) {
+        sort_at(depth);
+        sorted = true;
+      }
+    }
   }
-  const uint64_t *key = (const uint64_t *)c->key[0].p;
-  const int32_t *idx = (const int32_t *)c->idx[0].p;
-  double4 *pos = dget<double4>(c->pos, (size_t)n);
-  // the particle gather runs on the second stream, beside the cells, the
-  // neighbour tables and the lists; P2P follows it on that stream, and the
-  // main stream waits for it at the end of build_lists
-  NB_HIP(hipEventRecord(c->ev_pos, s));
-  NB_HIP(hipStreamWaitEvent(c->stream2, c->ev_pos, 0));
-  hipLaunchKernelGGL(k_gather_pos, dim3(nblk(n)), dim3(kT), 0, c->stream2, d_xyz, d_m, idx, n, pos);
-  NB_LAUNCH();
-  NB_HIP(hipEventRecord(c->ev_pos, c->stream2));
-
-  // ---- cells of every level ----
-  const int D = depth;
-  uint32_t *dcounts = (uint32_t *)(out6 + 16);
-  const int64_t nbl = nblk(n);
-  int32_t *bcnt = dget<int32_t>(c->flags, std::max<size_t>((size_t)n + 1, (size_t)(D + 1) * (size_t)nbl));
-  hipLaunchKernelGGL(k_level_blocks, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt);
-  NB_LAUNCH();
-  hipLaunchKernelGGL(k_level_offsets, dim3(D + 1), dim3(kOffThreads), 0, s, bcnt, (int)nbl, dcounts);
-  NB_LAUNCH();
-  uint32_t *counts = (uint32_t *)(h + 16); // pinned
-  NB_HIP(hipMemcpyAsync(counts, dcounts, sizeof(uint32_t) * (D + 1), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.lvl_start.assign(D + 2, 0);
-  for (int d = 0; d <= D; ++d) P.lvl_start[d + 1] = P.lvl_start[d] + (int32_t)counts[d];
-  const int64_t nc = P.lvl_start[D + 1];
-  P.n_cells = nc;
-  uint64_t *ckey = dget<uint64_t>(c->ckey, (size_t)nc);
-  int32_t *cbeg = dget<int32_t>(c->cbeg, (size_t)nc);
-  int32_t *cend = dget<int32_t>(c->cend, (size_t)nc);
-  int32_t *cpar = dget<int32_t>(c->cparent, (size_t)nc);
-  int32_t *pcell = dget<int32_t>(c->pcell, (size_t)n);
-  LevelStarts L{};
-  for (int d = 0; d <= D + 1; ++d) L.s[d] = P.lvl_start[d];
-  hipLaunchKernelGGL(k_cells_fill, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, (const int32_t *)bcnt, L, ckey,
-                     cbeg, cend, cpar, pcell);
-  NB_LAUNCH();
-  // neighbour tables
-  int32_t *child8 = dget<int32_t>(c->child8, (size_t)nc * 8);
-  int32_t *nbr27 = dget<int32_t>(c->nbr27, (size_t)nc * 27);
-  NB_HIP(hipMemsetAsync(child8, 0xff, sizeof(int32_t) * (size_t)nc * 8, s));
-  if (nc > 1) {
-    hipLaunchKernelGGL(k_child8, dim3(nblk(nc - 1)), dim3(kT), 0, s, ckey, cpar, 1, (int32_t)nc, child8);
-    NB_LAUNCH();
+  if (!built && !sorted) {
+    do {
+      depth = next_depth(depth, maxcnt);
+      sort_at(depth);
+      // (fetched here: build_levels may have grown the flags buffer)
+      int32_t *flags = dget<int32_t>(c->flags, (size_t)n + 1);
+      int32_t *fscan = dget<int32_t>(c->fscan, (size_t)n + 1);
+      const uint64_t *k = (const uint64_t *)c->key[cur].p;
+      hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(kT), 0, s, k, n, flags);
+      NB_LAUNCH();
+      scan_excl_i32(flags, fscan, n, c->scan_tmp, s);
+      // number of runs = fscan[n-1] + flags[n-1]
+      int32_t *starts = (int32_t *)c->idx[cur ^ 1].p; // free while not sorting
+      hipLaunchKernelGGL(k_run_starts, dim3(nblk(n)), dim3(kT), 0, s, flags, fscan, n, starts);
+      NB_LAUNCH();
+      NB_HIP(hipMemsetAsync(dmax, 0, sizeof(uint32_t), s));
+      hipLaunchKernelGGL(k_run_max, dim3((unsigned)std::min<int64_t>(1024, nblk(n))), dim3(kT), 0, s, starts,
+                         flags, fscan, n, dmax);
+      NB_LAUNCH();
+      NB_HIP(hipMemcpyAsync(hm, dmax, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      NB_HIP(hipStreamSynchronize(s));
+      maxcnt = hm[0];
+      P.iterations++;
+    } while (more(depth, maxcnt));
   }
-  NB_HIP(hipMemsetAsync(nbr27, 0xff, sizeof(int32_t) * 27, s));
-  NB_HIP(hipMemsetAsync(nbr27 + 13, 0, sizeof(int32_t), s));
-  // the leaf level's rows follow in build_lists, for the leaves that are not
-  // closed (the only ones the evaluation reads); nbr_leaf_rows completes them
-  for (int d = 1; d < D; ++d) {
-    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_nbr27, dim3(nblk((int64_t)(ce - cb) * 27)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d,
-                       nullptr, nbr27);
-    NB_LAUNCH();
-  }
-  c->nbr_leaf = D == 0 ? 2 : 0;
+  if (!built) build_levels(c, d_xyz, d_m, depth, cur, out6, h);
+  c->depth_guess = depth;
 }
 
 // rows of the leaf level: of the leaves with closed[c] == 0 (closed != null),

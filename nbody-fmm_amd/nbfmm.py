@@ -97,6 +97,8 @@ class Stats(ctypes.Structure):
         ("n_l2p_cells", ctypes.c_int64),
         ("ms_m2m", ctypes.c_double),
         ("ms_l2l", ctypes.c_double),
+        ("depth_sorts", ctypes.c_int32),
+        ("reserved1", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -67,6 +67,43 @@ def test_tree_bitexact(ctx, case):
     _check_tree(gt, ot, len(m))
 
 
+def test_tree_depth_guess():
+    """Each evaluation first sorts at the previous tree's depth and replays the
+    reference's depth loop on every level's fullest cell.  A sequence of inputs
+    whose depths go up and down takes every branch (confirmed, shallower:
+    sorted again, deeper: the loop goes on); every tree is the oracle's, with
+    the reference loop's iteration count, and a repeated input sorts once."""
+    c = nbfmm.Context(order=4, leaf_capacity=16, G=1.0, eps2=0.0, p2p_cut=0)
+    dup, dm = workloads.uniform_cube(3000, seed=4)
+    dup[:40] = dup[0]
+    seq = [("plummer50k", *workloads.plummer(50000, seed=5), 64),
+           ("plummer50k again", *workloads.plummer(50000, seed=5), 64),
+           ("uniform2k", *workloads.uniform_cube(2000, seed=3), 16),
+           ("duplicates", dup, dm, 16),
+           ("duplicates again", dup, dm, 16),
+           ("clustered", *workloads.plummer_mix(20000, n_spheres=6, seed=2), 20),
+           ("tiny", *workloads.uniform_cube(7, seed=9), 20),
+           ("plummer50k thr 8", *workloads.plummer(50000, seed=5), 8)]
+    try:
+        prev = None
+        for name, xyz, m, thr in seq:
+            c.set_params(leaf_capacity=thr)
+            _, gt = c.eval_tree(xyz, m)
+            st = c.stats()
+            ot = orc.tree(xyz, thr)
+            print(f"[guess] {name}: depth {gt['depth']} (previous {prev}), iterations {st.depth_iterations}, "
+                  f"sorts {st.depth_sorts}", flush=True)
+            _check_tree(gt, ot, len(m))
+            assert st.depth_iterations == len(ot["trace"])
+            if prev == gt["depth"]:
+                assert st.depth_sorts == 1
+            else:
+                assert st.depth_sorts >= 2 or prev is None
+            prev = gt["depth"]
+    finally:
+        c.close()
+
+
 def test_direct_naive_bitwise(ctx):
     xyz, m = workloads.uniform_cube(1500, seed=11)
     ctx.set_params(G=6.6743e-11, eps2=1e-4)
