@@ -185,36 +185,6 @@ __global__ void __launch_bounds__(kT) k_rs_scatter(const uint64_t *__restrict__ 
   }
 }
 
-// ---- runs of equal keys ----------------------------------------------------------
-__global__ void k_run_flags(const uint64_t *__restrict__ key, int64_t n, int32_t *flags) {
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < n) flags[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
-}
-__global__ void k_run_starts(const int32_t *flags, const int32_t *scan, int64_t n, int32_t *starts) {
-  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < n && flags[i]) starts[scan[i]] = (int32_t)i;
-}
-// out[0]: the longest run (out[0] zero on entry); out[1]: the number of runs
-__global__ void k_run_max(const int32_t *starts, const int32_t *flags, const int32_t *scan, int64_t n,
-                          uint32_t *out) {
-  __shared__ uint32_t s[kT];
-  const int64_t nruns = (int64_t)scan[n - 1] + flags[n - 1];
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[1] = (uint32_t)nruns;
-  uint32_t best = 0;
-  for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < nruns; r += (int64_t)gridDim.x * kT) {
-    int64_t e = r + 1 < nruns ? starts[r + 1] : n;
-    uint32_t len = (uint32_t)(e - starts[r]);
-    best = len > best ? len : best;
-  }
-  s[threadIdx.x] = best;
-  __syncthreads();
-  for (int o = kT / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + o]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicMax(out, s[0]);
-}
-
 __global__ void k_gather_pos(const double *__restrict__ xyz, const double *__restrict__ m,
                              const int32_t *__restrict__ idx, int64_t n, double4 *__restrict__ pos) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -233,65 +203,136 @@ __device__ inline int start_level(const uint64_t *__restrict__ key, int64_t i, i
   return x == 0 ? D + 1 : D - (63 - __clzll((long long)x)) / 3;
 }
 
-// Cells that start in each block of kT particles, per level: bcnt[d * nb + b].
+// Cells that start in each block of kT particles, per level: bcnt[d * nb + b];
+// with them the block's first and last cell start (-1: none) and the longest
+// cell that starts and ends inside the block (bfirst, blast, bgap, same layout).
 __global__ void __launch_bounds__(kT) k_level_blocks(const uint64_t *__restrict__ key, int64_t n, int D,
-                                                     int32_t *__restrict__ bcnt) {
+                                                     int32_t *__restrict__ bcnt, int32_t *__restrict__ bfirst,
+                                                     int32_t *__restrict__ blast, int32_t *__restrict__ bgap) {
   __shared__ int32_t w[kT / 64][kMaxDepth + 1];
+  __shared__ int32_t wf[kT / 64][kMaxDepth + 1], wl[kT / 64][kMaxDepth + 1], wg[kT / 64][kMaxDepth + 1];
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   const int d0 = start_level(key, i, n, D);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int d = 0; d <= D; ++d) {
     const uint64_t m = __ballot(d0 <= d);
-    if (lane == 0) w[wv][d] = (int32_t)__popcll(m);
+    // a start's distance to the next start of its wavefront (0: none)
+    const uint64_t above = lane == 63 ? 0ull : m >> (lane + 1);
+    int g = (d0 <= d && above) ? __builtin_ctzll(above) + 1 : 0;
+    if (m & (m - 1)) { // two starts or more: the longest such distance
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) g = max(g, __shfl_xor(g, o, 64));
+    }
+    if (lane == 0) {
+      w[wv][d] = (int32_t)__popcll(m);
+      wf[wv][d] = m ? __builtin_ctzll(m) : -1;
+      wl[wv][d] = m ? 63 - __builtin_clzll(m) : -1;
+      wg[wv][d] = g;
+    }
   }
   __syncthreads();
   if ((int)threadIdx.x <= D) {
-    int32_t t = 0;
-    for (int q = 0; q < kT / 64; ++q) t += w[q][threadIdx.x];
-    bcnt[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    const int d = threadIdx.x;
+    const int32_t base = (int32_t)blockIdx.x * kT;
+    int32_t t = 0, F = -1, S = -1, G = 0;
+    for (int q = 0; q < kT / 64; ++q) {
+      t += w[q][d];
+      if (wf[q][d] >= 0) {
+        const int32_t pf = base + 64 * q + wf[q][d];
+        if (S >= 0) G = max(G, pf - S);
+        if (F < 0) F = pf;
+        S = base + 64 * q + wl[q][d];
+        G = max(G, wg[q][d]);
+      }
+    }
+    const int64_t o = (int64_t)d * gridDim.x + blockIdx.x;
+    bcnt[o] = t;
+    bfirst[o] = F;
+    blast[o] = S;
+    bgap[o] = G;
   }
 }
 
-// One block per level: exclusive scan of that level's block counts in place
-// and the level's cell count.
+// One block per level: exclusive scan of that level's block counts in place,
+// the level's cell count (total[d]) and its fullest cell (total[kMaxDepth + 1 + d]):
+// the longest of the cells inside one block and of the cells from a block's
+// first start back to the last start before it (a max-scan of blast), the
+// last cell ending at n.
 constexpr int kOffThreads = 1024, kOffItems = 8;
-__global__ void __launch_bounds__(kOffThreads) k_level_offsets(int32_t *bcnt, int nb, uint32_t *total) {
-  __shared__ int32_t ws[kOffThreads / 64];
-  int32_t *row = bcnt + (int64_t)blockIdx.x * nb;
+__global__ void __launch_bounds__(kOffThreads) k_level_offsets(int32_t *bcnt, const int32_t *__restrict__ bfirst,
+                                                               const int32_t *__restrict__ blast,
+                                                               const int32_t *__restrict__ bgap, int nb, int64_t n,
+                                                               uint32_t *total) {
+  __shared__ int32_t ws[kOffThreads / 64], wm[kOffThreads / 64], wg[kOffThreads / 64];
+  const int64_t ro = (int64_t)blockIdx.x * nb;
+  int32_t *row = bcnt + ro;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int32_t carry = 0;
+  int32_t carry = 0, carryS = -1, gmax = 0;
   for (int base = 0; base < nb; base += kOffThreads * kOffItems) {
     const int j0 = base + (int)threadIdx.x * kOffItems;
-    int32_t v[kOffItems], loc = 0;
+    int32_t v[kOffItems], f[kOffItems], l[kOffItems], loc = 0, sl = -1;
 #pragma unroll
     for (int k = 0; k < kOffItems; ++k) {
-      v[k] = j0 + k < nb ? row[j0 + k] : 0;
+      const bool in = j0 + k < nb;
+      v[k] = in ? row[j0 + k] : 0;
+      f[k] = in ? bfirst[ro + j0 + k] : -1;
+      l[k] = in ? blast[ro + j0 + k] : -1;
+      if (in) gmax = max(gmax, bgap[ro + j0 + k]);
       loc += v[k];
+      sl = max(sl, l[k]);
     }
-    int32_t x = loc; // inclusive scan over the wavefront
+    int32_t x = loc, y = sl; // inclusive sum and max scans over the wavefront
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-      const int32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
+      const int32_t xs = __shfl_up(x, o, 64), ys = __shfl_up(y, o, 64);
+      if (lane >= o) {
+        x += xs;
+        y = max(y, ys);
+      }
     }
-    if (lane == 63) ws[wv] = x;
+    int32_t yprev = __shfl_up(y, 1, 64);
+    if (lane == 0) yprev = -1;
+    if (lane == 63) {
+      ws[wv] = x;
+      wm[wv] = y;
+    }
     __syncthreads();
-    int32_t pre = 0, all = 0;
+    int32_t pre = 0, all = 0, preS = carryS, allS = carryS;
     for (int q = 0; q < kOffThreads / 64; ++q) {
-      const int32_t t = ws[q];
-      if (q < wv) pre += t;
+      const int32_t t = ws[q], u = wm[q];
+      if (q < wv) {
+        pre += t;
+        preS = max(preS, u);
+      }
       all += t;
+      allS = max(allS, u);
     }
     int32_t run = carry + pre + x - loc;
+    int32_t last = max(preS, yprev); // the last cell start before this thread's blocks
 #pragma unroll
     for (int k = 0; k < kOffItems; ++k) {
       if (j0 + k < nb) row[j0 + k] = run;
       run += v[k];
+      if (f[k] >= 0) {
+        if (last >= 0) gmax = max(gmax, f[k] - last);
+        last = l[k];
+      }
     }
     carry += all;
+    carryS = allS;
     __syncthreads();
   }
-  if (threadIdx.x == 0) total[blockIdx.x] = (uint32_t)carry;
+  // the fullest cell over the block's threads
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) gmax = max(gmax, __shfl_xor(gmax, o, 64));
+  if (lane == 0) wg[wv] = gmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t g = carryS >= 0 ? (int32_t)(n - carryS) : 0;
+    for (int q = 0; q < kOffThreads / 64; ++q) g = max(g, wg[q]);
+    total[blockIdx.x] = (uint32_t)carry;
+    total[kMaxDepth + 1 + blockIdx.x] = (uint32_t)g;
+  }
 }
 
 struct LevelStarts {
@@ -337,31 +378,6 @@ __global__ void __launch_bounds__(kT) k_cells_fill(const uint64_t *__restrict__ 
     up = cell;
   }
   if (i < n) pcell[i] = up;
-}
-
-// Longest cell (most particles) of every level 0..D: cells are stored level
-// by level from L.s[d]; a thread walks its cells in increasing order, so its
-// level only moves forward.  out[0..D] is zero at the launch.
-__global__ void k_level_max(const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend, LevelStarts L,
-                            int D, uint32_t *out) {
-  __shared__ uint32_t s[kMaxDepth + 1];
-  if ((int)threadIdx.x <= D) s[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t nc = L.s[D + 1];
-  int d = 0;
-  uint32_t best = 0;
-  for (int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kT) {
-    if (c >= L.s[d + 1]) {
-      if (best) atomicMax(&s[d], best);
-      best = 0;
-      while (c >= L.s[d + 1]) ++d;
-    }
-    const uint32_t len = (uint32_t)(cend[c] - cbeg[c]);
-    best = len > best ? len : best;
-  }
-  if (best) atomicMax(&s[d], best);
-  __syncthreads();
-  if ((int)threadIdx.x <= D && s[threadIdx.x]) atomicMax(&out[threadIdx.x], s[threadIdx.x]);
 }
 
 __global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent, int32_t cb,
@@ -425,10 +441,38 @@ int bit_width_u64(uint64_t v) {
 
 } // namespace
 
+// Cells per level of the keys sorted at depth D (buffer cur): P.lvl_start
+// and, in lm[0..D], every level's fullest cell (one transfer, one sync).
+// The per-block offsets stay in c->flags for fill_levels.
+static void count_levels(nbfmm_ctx *c, int D, int cur, double *out6, double *h, uint32_t *lm) {
+  Plan &P = c->plan;
+  const int64_t n = P.n;
+  hipStream_t s = c->stream;
+  const uint64_t *key = (const uint64_t *)c->key[cur].p;
+  uint32_t *dcounts = (uint32_t *)(out6 + 16); // counts, then maxima at kMaxDepth + 1
+  const int64_t nbl = nblk(n);
+  const size_t per = (size_t)(D + 1) * (size_t)nbl;
+  int32_t *bcnt = dget<int32_t>(c->flags, std::max<size_t>((size_t)n + 1, per));
+  int32_t *aux = dget<int32_t>(c->fscan, std::max<size_t>((size_t)n + 1, 3 * per));
+  hipLaunchKernelGGL(k_level_blocks, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt, aux, aux + per,
+                     aux + 2 * per);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_level_offsets, dim3(D + 1), dim3(kOffThreads), 0, s, bcnt, (const int32_t *)aux,
+                     (const int32_t *)(aux + per), (const int32_t *)(aux + 2 * per), (int)nbl, n, dcounts);
+  NB_LAUNCH();
+  uint32_t *counts = (uint32_t *)(h + 16); // pinned
+  NB_HIP(hipMemcpyAsync(counts, dcounts, sizeof(uint32_t) * 2 * (kMaxDepth + 1), hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+  P.lvl_start.assign(D + 2, 0);
+  for (int d = 0; d <= D; ++d) {
+    P.lvl_start[d + 1] = P.lvl_start[d] + (int32_t)counts[d];
+    lm[d] = counts[kMaxDepth + 1 + d];
+  }
+}
+
 // Every level's cells, the particles in key order and the neighbour tables,
-// from the keys sorted at `depth` (buffer `cur`).
-static void build_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, int depth, int cur, double *out6,
-                  double *h) {
+// from the keys sorted at `depth` (buffer `cur`) and count_levels' offsets.
+static void fill_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, int depth, int cur) {
   Plan &P = c->plan;
   const int64_t n = P.n;
   hipStream_t s = c->stream;
@@ -451,19 +495,9 @@ static void build_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, i
 
   // ---- cells of every level ----
   const int D = depth;
-  uint32_t *dcounts = (uint32_t *)(out6 + 16);
   const int64_t nbl = nblk(n);
-  int32_t *bcnt = dget<int32_t>(c->flags, std::max<size_t>((size_t)n + 1, (size_t)(D + 1) * (size_t)nbl));
-  hipLaunchKernelGGL(k_level_blocks, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt);
-  NB_LAUNCH();
-  hipLaunchKernelGGL(k_level_offsets, dim3(D + 1), dim3(kOffThreads), 0, s, bcnt, (int)nbl, dcounts);
-  NB_LAUNCH();
-  uint32_t *counts = (uint32_t *)(h + 16); // pinned
-  NB_HIP(hipMemcpyAsync(counts, dcounts, sizeof(uint32_t) * (D + 1), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  P.lvl_start.assign(D + 2, 0);
-  for (int d = 0; d <= D; ++d) P.lvl_start[d + 1] = P.lvl_start[d] + (int32_t)counts[d];
-  P.n_leaves = counts[D];
+  const int32_t *bcnt = (const int32_t *)c->flags.p;
+  P.n_leaves = P.lvl_start[D + 1] - P.lvl_start[D];
   const int64_t nc = P.lvl_start[D + 1];
   P.n_cells = nc;
   uint64_t *ckey = dget<uint64_t>(c->ckey, (size_t)nc);
@@ -473,8 +507,8 @@ static void build_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, i
   int32_t *pcell = dget<int32_t>(c->pcell, (size_t)n);
   LevelStarts L{};
   for (int d = 0; d <= D + 1; ++d) L.s[d] = P.lvl_start[d];
-  hipLaunchKernelGGL(k_cells_fill, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, (const int32_t *)bcnt, L, ckey,
-                     cbeg, cend, cpar, pcell);
+  hipLaunchKernelGGL(k_cells_fill, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt, L, ckey, cbeg, cend, cpar,
+                     pcell);
   NB_LAUNCH();
   // neighbour tables
   int32_t *child8 = dget<int32_t>(c->child8, (size_t)nc * 8);
@@ -539,13 +573,11 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   dget<uint64_t>(c->key[1], (size_t)n);
   dget<int32_t>(c->idx[0], (size_t)n);
   dget<int32_t>(c->idx[1], (size_t)n);
-  uint32_t *dmax = (uint32_t *)(out6 + 12);
-  uint32_t *lmax = (uint32_t *)(out6 + 26); // kMaxDepth + 1 values, below the other users of the area
-  uint32_t *hm = (uint32_t *)(h + 16);      // pinned
+  uint32_t lm[kMaxDepth + 1]; // fullest cell of every level of the last count
   int cur = 0;
   P.iterations = 0;
   P.depth_sorts = 0;
-  // Morton keys at `depth`, sorted; buffer `cur` holds them
+  // Morton keys at `depth`, sorted (buffer `cur`), and their cells per level
   auto sort_at = [&](int depth) {
     const uint64_t nld = 1ull << depth;
     double ls[3];
@@ -555,6 +587,7 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
     NB_LAUNCH();
     cur = radix_sort(c, n, 3 * depth, 0);
     P.depth_sorts++;
+    count_levels(c, depth, cur, out6, h, lm);
   };
   // the reference's step: the depth after `depth` when its fullest cell holds maxcnt
   auto next_depth = [&](int depth, uint32_t maxcnt) {
@@ -569,71 +602,38 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   };
   int depth = 0;
   uint32_t maxcnt = (uint32_t)n;
-  bool built = false, sorted = false;
+  bool done = false;
   const int guess = c->depth_guess;
   if (guess >= 1 && guess <= kMaxDepth) {
-    // Sort once at the depth the previous evaluation ended at and build every
-    // level's cells: a level-d cell is a run of the level-d key, which is the
-    // sorted key's top 3d bits, so the fullest cell of every level <= guess is
-    // known, and the loop above is replayed on them.  It ends at `guess` (the
-    // usual case: particles move little between evaluations), before it
-    // (sorted again at that depth) or beyond it (the loop goes on from its
-    // last depth <= guess).  The result is the loop's in every case.
+    // Sort once at the depth the previous evaluation ended at.  A level-d
+    // cell is a run of the level-d key, the sorted key's top 3d bits, so the
+    // cell counts give the fullest cell of every level <= guess, and the loop
+    // is replayed on them.  It ends at `guess` (the usual case: particles move
+    // little between evaluations), before it (sorted again at that depth) or
+    // beyond it (the loop goes on from its last depth <= guess); its result
+    // is the loop's in every case.
     sort_at(guess);
-    build_levels(c, d_xyz, d_m, guess, cur, out6, h);
-    NB_HIP(hipMemsetAsync(lmax, 0, sizeof(uint32_t) * (kMaxDepth + 1), s));
-    LevelStarts L{};
-    for (int d = 0; d <= guess + 1; ++d) L.s[d] = P.lvl_start[d];
-    hipLaunchKernelGGL(k_level_max, dim3(256), dim3(kT), 0, s, (const int32_t *)c->cbeg.p,
-                       (const int32_t *)c->cend.p, L, guess, lmax);
-    NB_LAUNCH();
-    NB_HIP(hipMemcpyAsync(hm, lmax, sizeof(uint32_t) * (guess + 1), hipMemcpyDeviceToHost, s));
-    NB_HIP(hipStreamSynchronize(s));
     for (;;) {
       const int nd = next_depth(depth, maxcnt);
       if (nd > guess) break;
       depth = nd;
-      maxcnt = hm[depth];
+      maxcnt = lm[depth];
       P.iterations++;
       if (!more(depth, maxcnt)) break;
     }
-    if (!more(depth, maxcnt) && depth == guess) built = true;
-    else {
-      // the particle gather on the second stream reads the sorted indices
-      // that the next sort overwrites
-      NB_HIP(hipStreamWaitEvent(s, c->ev_pos, 0));
-      if (!more(depth, maxcnt)) {
-        sort_at(depth);
-        sorted = true;
-      }
+    if (!more(depth, maxcnt)) {
+      done = true;
+      if (depth != guess) sort_at(depth);
     }
   }
-  if (!built && !sorted) {
-    do {
-      depth = next_depth(depth, maxcnt);
-      sort_at(depth);
-      // (fetched here: build_levels may have grown the flags buffer)
-      int32_t *flags = dget<int32_t>(c->flags, (size_t)n + 1);
-      int32_t *fscan = dget<int32_t>(c->fscan, (size_t)n + 1);
-      const uint64_t *k = (const uint64_t *)c->key[cur].p;
-      hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(kT), 0, s, k, n, flags);
-      NB_LAUNCH();
-      scan_excl_i32(flags, fscan, n, c->scan_tmp, s);
-      // number of runs = fscan[n-1] + flags[n-1]
-      int32_t *starts = (int32_t *)c->idx[cur ^ 1].p; // free while not sorting
-      hipLaunchKernelGGL(k_run_starts, dim3(nblk(n)), dim3(kT), 0, s, flags, fscan, n, starts);
-      NB_LAUNCH();
-      NB_HIP(hipMemsetAsync(dmax, 0, sizeof(uint32_t), s));
-      hipLaunchKernelGGL(k_run_max, dim3((unsigned)std::min<int64_t>(1024, nblk(n))), dim3(kT), 0, s, starts,
-                         flags, fscan, n, dmax);
-      NB_LAUNCH();
-      NB_HIP(hipMemcpyAsync(hm, dmax, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-      NB_HIP(hipStreamSynchronize(s));
-      maxcnt = hm[0];
-      P.iterations++;
-    } while (more(depth, maxcnt));
+  while (!done) {
+    depth = next_depth(depth, maxcnt);
+    sort_at(depth);
+    maxcnt = lm[depth];
+    P.iterations++;
+    done = !more(depth, maxcnt);
   }
-  if (!built) build_levels(c, d_xyz, d_m, depth, cur, out6, h);
+  fill_levels(c, d_xyz, d_m, depth, cur);
   c->depth_guess = depth;
 }
 
