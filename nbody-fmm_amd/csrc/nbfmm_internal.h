@@ -44,6 +44,7 @@ constexpr int kMaxDepth = 16;   // ((64/3)>>3)<<3, tree.cpp:38
 constexpr int kNumClasses = 343; // offsets in [-3,3]^3, class = (dz+3)*49+(dy+3)*7+(dx+3)
 constexpr int kMaxOrder = 16;
 constexpr int kTaskTargets = 256; // targets of one P2P sub-task (one wavefront, 4 per lane)
+constexpr int kNarrowCap = 64;    // the largest narrow threshold (sets the narrow kernel's LDS)
 constexpr int kMaxList = 192;     // longest source-cell list of a P2P task (interaction list <= 189)
 // P2P blocks of at most this many targets go to the sources-across-lanes
 // kernel (two passes of its target group at most), larger ones to the

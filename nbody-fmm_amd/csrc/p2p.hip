@@ -466,7 +466,8 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
                                                   const uint64_t *__restrict__ ckey,
                                                   const double4 *__restrict__ pos, double *__restrict__ grad,
                                                   double *__restrict__ pot) {
-  __shared__ typename Tr::S st[kTaskTargets + KG]; // the last pass reads up to KG - 1 slots past nc
+  // narrow tasks hold at most kNarrowCap targets; the last pass reads up to KG - 1 slots past nc
+  __shared__ typename Tr::S st[kNarrowCap + KG];
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   __shared__ int32_t tmp[54];
@@ -640,7 +641,7 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
   typedef typename Tr::S S;
   typedef typename TrFar::S FS;
   // the narrow branch's targets and the wide branch's source chunks share one buffer
-  constexpr size_t kNarrowBytes = sizeof(S) * (kTaskTargets + KG);
+  constexpr size_t kNarrowBytes = sizeof(S) * (kNarrowCap + KG);
   constexpr size_t kWideBytes = sizeof(S) * kW + sizeof(FS) * kW;
   __shared__ __align__(16) char buf[kNarrowBytes > kWideBytes ? kNarrowBytes : kWideBytes];
   __shared__ int32_t lb[kMaxList];
@@ -890,8 +891,8 @@ bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
   };
   if (!in(t.p2p_group_f32, {8, 12, 16, 24})) why = "p2p_group_f32 must be 8, 12, 16 or 24";
   else if (!in(t.p2p_group_f64, {4, 8, 12})) why = "p2p_group_f64 must be 4, 8 or 12";
-  else if (t.p2p_narrow_max_f32 < 1 || t.p2p_narrow_max_f32 > kTaskTargets) why = "p2p_narrow_max_f32 must be in [1, 256]";
-  else if (t.p2p_narrow_max_f64 < 1 || t.p2p_narrow_max_f64 > kTaskTargets) why = "p2p_narrow_max_f64 must be in [1, 256]";
+  else if (t.p2p_narrow_max_f32 < 1 || t.p2p_narrow_max_f32 > kNarrowCap) why = "p2p_narrow_max_f32 must be in [1, 64]";
+  else if (t.p2p_narrow_max_f64 < 1 || t.p2p_narrow_max_f64 > kNarrowCap) why = "p2p_narrow_max_f64 must be in [1, 64]";
   else if (!(t.p2p_early_share_f32 >= 0.0 && t.p2p_early_share_f32 <= 1.0)) why = "p2p_early_share_f32 must be in [0, 1]";
   else if (!(t.p2p_early_share_f64 >= 0.0 && t.p2p_early_share_f64 <= 1.0)) why = "p2p_early_share_f64 must be in [0, 1]";
   else if (t.p2p_early_launches_f32 < 1 || t.p2p_early_launches_f32 > 64) why = "p2p_early_launches_f32 must be in [1, 64]";
@@ -953,7 +954,7 @@ void launch_p2p(nbfmm_ctx *c, int part) {
       int32_t *ctr = dget<int32_t>(c->p2p_ctr, 4) + part;
       hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s2, ctr, 1);
       NB_LAUNCH();
-      // a CU holds 16 of these wavefronts (10 KB of LDS and 128 VGPRs each): 16 per CU fill the device
+      // a CU holds 16 of these wavefronts (128 VGPRs each): 16 per CU fill the device
       const int per_cu = part == 0 ? tl.p2p_queue_waves0 : 16;
       const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(nwide + nnar, (int64_t)c->n_cu * per_cu));
       const int chunk = tl.p2p_queue_chunk;
