@@ -42,12 +42,15 @@ struct Consts {
   double cz[kMaxOrder + 2][kMaxOrder + 2];    // sqrt((n+m)(n-m))
   double cu[kMaxOrder + 2][kMaxOrder + 2];    // sqrt((n-m+1)(n-m))
   double cd[kMaxOrder + 2][kMaxOrder + 2];    // sqrt((n+m+1)(n+m))
+  double smm[kMaxOrder + 2];                  // mmf[1] ... mmf[m] (1 for m = 0)
 };
 
 void fill_consts(Consts &C) {
   std::memset(&C, 0, sizeof(C));
   for (int k = 1; k <= kMaxOrder + 1; ++k) C.mmf[k] = -std::sqrt((2.0 * k - 1.0) / (2.0 * k));
   for (int m = 0; m <= kMaxOrder + 1; ++m) C.s2m1[m] = std::sqrt(2.0 * m + 1.0);
+  C.smm[0] = 1.0;
+  for (int m = 1; m <= kMaxOrder + 1; ++m) C.smm[m] = C.smm[m - 1] * C.mmf[m];
   for (int n = 0; n <= kMaxOrder + 1; ++n)
     for (int m = 0; m <= n && m <= kMaxOrder + 1; ++m) {
       if (n >= 2 && m <= n - 2) {
@@ -293,24 +296,30 @@ __global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, double hx, do
 }
 
 // ---- P2M ---------------------------------------------------------------------------
-// One wavefront per P2M cell, four cells per workgroup.  Lanes are 16 columns
-// m (the orders; m and m + 16 for p > 15) times 4 particle slots: a lane runs
-// its column for every 4th particle of the cell -- q R_m^m by the sectoral
-// recurrence, then R_n^m, n = m+1..p, by the three-term recurrence in n --
-// accumulating per lane, so every column costs about p steps per particle and
-// no lane waits on another.  The 4 slots meet in two xor-shuffles and the
-// column's lane stores the coefficients.  P2M cells hold at most kTransCut
-// particles unless they are leaves (more only where particles coincide at the
-// deepest level) and mostly tens, so a workgroup per cell spent its time on
-// set-up and on the combine across waves.  T is the coefficient type (double,
-// or float in mixed precision); the offset from the cell centre is formed in
+// One wavefront per P2M cell, four cells per workgroup.  Lanes are LP column
+// lanes times 64 / LP particle slots.  Column lane c holds two columns, the
+// orders m1 = c and m2 = p - c (one when they meet), so every lane has p + 2
+// terms: q R_m^m = q S_m w^m (w = x + i y, S_m the product of the sectoral
+// recurrence factors, w^m by binary powers), then R_n^m, n = m+1..p, by the
+// three-term recurrence in n.  A slot runs every (64/LP)-th particle of the
+// cell, accumulating per lane; the slots meet in xor-shuffles and the column
+// lanes store the coefficients.  (The previous layout, one column per lane,
+// ran p sectoral and p recurrence steps for each particle of 4 slots.)
+// PM bounds the order (register arrays); P2M cells hold at most kTransCut
+// particles unless they are leaves (more only where particles coincide at
+// the deepest level) and mostly tens.  T is the coefficient type (double, or
+// float in mixed precision); the offset from the cell centre is formed in
 // FP64 and scaled before rounding.
-template <class T>
+template <class T, int LP, int PM>
 __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, int ncell, const int32_t *__restrict__ cells,
                                             const double4 *__restrict__ pos, const int32_t *__restrict__ cbeg,
                                             const int32_t *__restrict__ cend, const int32_t *__restrict__ mslot,
                                             const uint64_t *__restrict__ ckey, LevelGeo g,
                                             const Consts *__restrict__ C, T *__restrict__ Mexp) {
+  static_assert(LP >= PM / 2 + 1 && 64 % LP == 0, "a column lane per pair of orders");
+  constexpr int NS = 64 / LP;      // particle slots
+  constexpr int N1 = PM + 1;       // terms of a lane's first column, at most
+  constexpr int N2 = (PM + 1) / 2; // of its second
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int k = (int)blockIdx.x * (kT / 64) + wave;
   if (k >= ncell) return; // a whole wavefront; nothing below synchronises the workgroup
@@ -321,78 +330,122 @@ __global__ void __launch_bounds__(kT) k_p2m(int p, int KP, int ncell, const int3
   double ctr[3];
   cell_centre(ckey[cell], lev, g, ctr);
   const double ir = g.inv_rho[lev];
-  const int col = lane & 15, ps = lane >> 4; // column, particle slot (4)
-  T *M = Mexp + (size_t)slot * KP;
-  for (int m0 = 0; m0 <= p; m0 += 16) {
-    const int m = m0 + col;
-    const int nj = p - m; // j = n - m in [0, nj]; no work when negative
-    T ar[kMaxOrder + 1], ai[kMaxOrder + 1];
-    // the recurrence coefficients of the lane's column, in registers for all its particles
-    T fm[kMaxOrder + 1], fa[kMaxOrder + 1], fb[kMaxOrder + 1];
+  const int cl = lane & (LP - 1), ps = lane / LP;
+  const int m1 = cl, m2 = p - cl;
+  const int n1 = 2 * cl <= p ? p - m1 + 1 : 0; // terms j = 0..p-m1 of column m1
+  const int n2 = 2 * cl < p ? p - m2 + 1 : 0;  // terms of column m2 (none when m2 <= m1)
+  // term j >= 1 of a column m: c_j = (a_j z) c_{j-1} - (b_j r^2) c_{j-2}
+  T a1[N1], b1[N1], a2[N2], b2[N2];
 #pragma unroll
-    for (int j = 0; j <= kMaxOrder; ++j) {
-      ar[j] = 0;
-      ai[j] = 0;
-      fm[j] = j >= 1 && j <= m ? (T)C->mmf[j] : (T)0;
-      fa[j] = j >= 2 && j <= nj ? (T)C->ra[m + j][m] : (T)0;
-      fb[j] = j >= 2 && j <= nj ? (T)C->rb[m + j][m] : (T)0;
+  for (int j = 0; j < N1; ++j) {
+    a1[j] = j == 1 && j < n1 ? (T)C->s2m1[m1] : j >= 2 && j < n1 ? (T)C->ra[m1 + j][m1] : (T)0;
+    b1[j] = j >= 2 && j < n1 ? (T)C->rb[m1 + j][m1] : (T)0;
+  }
+#pragma unroll
+  for (int j = 0; j < N2; ++j) {
+    a2[j] = j == 1 && j < n2 ? (T)C->s2m1[m2] : j >= 2 && j < n2 ? (T)C->ra[m2 + j][m2] : (T)0;
+    b2[j] = j >= 2 && j < n2 ? (T)C->rb[m2 + j][m2] : (T)0;
+  }
+  const T s1 = n1 > 0 ? (T)C->smm[m1] : (T)0, s2 = n2 > 0 ? (T)C->smm[m2] : (T)0;
+  T r1[N1], i1[N1], r2[N2], i2[N2];
+#pragma unroll
+  for (int j = 0; j < N1; ++j) r1[j] = i1[j] = 0;
+#pragma unroll
+  for (int j = 0; j < N2; ++j) r2[j] = i2[j] = 0;
+  for (int i = beg + ps; i < end; i += NS) {
+    const double4 q = pos[i];
+    const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
+    const T rr = x * x + y * y + z * z;
+    // w^m1 and w^m2 from w, w^2, w^4, ...
+    T wr = x, wi = y, e1r = 1, e1i = 0, e2r = 1, e2i = 0;
+#pragma unroll
+    for (int bit = 0; (1 << bit) <= PM; ++bit) {
+      if ((1 << bit) > p) break; // uniform
+      if ((m1 >> bit) & 1) {
+        const T tr = e1r * wr - e1i * wi;
+        e1i = e1r * wi + e1i * wr;
+        e1r = tr;
+      }
+      if ((m2 >> bit) & 1) {
+        const T tr = e2r * wr - e2i * wi;
+        e2i = e2r * wi + e2i * wr;
+        e2r = tr;
+      }
+      const T sr = wr * wr - wi * wi;
+      wi = 2 * wr * wi;
+      wr = sr;
     }
-    const T fs = nj >= 1 ? (T)C->s2m1[m] : (T)0;
-    if (nj >= 0)
-      for (int i = beg + ps; i < end; i += 4) {
-        const double4 q = pos[i];
-        const T x = (T)((q.x - ctr[0]) * ir), y = (T)((q.y - ctr[1]) * ir), z = (T)((q.z - ctr[2]) * ir);
-        const T r2 = x * x + y * y + z * z;
-        T vr = (T)q.w, vi = 0; // q * R_m^m
+    const T qm = (T)q.w;
+    {
+      T cr = qm * s1 * e1r, ci = qm * s1 * e1i, pr = 0, pi = 0; // q R_m1^m1
+      r1[0] += cr;
+      i1[0] -= ci;
 #pragma unroll
-        for (int k = 1; k <= kMaxOrder; ++k) {
-          if (k <= m) {
-            const T f = fm[k];
-            const T nr = f * (x * vr - y * vi), ni = f * (x * vi + y * vr);
-            vr = nr;
-            vi = ni;
-          }
-        }
-        ar[0] += vr;
-        ai[0] -= vi;
-        T pr = vr, pi = vi;
-        T cr = 0, ci = 0;
-        if (nj >= 1) {
-          const T f = z * fs;
-          cr = f * pr;
-          ci = f * pi;
-          ar[1] += cr;
-          ai[1] -= ci;
-        }
-#pragma unroll
-        for (int j = 2; j <= kMaxOrder; ++j) {
-          if (j <= nj) {
-            const T a = fa[j] * z, b = fb[j] * r2;
-            const T nr = a * cr - b * pr, ni = a * ci - b * pi;
-            pr = cr;
-            pi = ci;
-            cr = nr;
-            ci = ni;
-            ar[j] += cr;
-            ai[j] -= ci;
-          }
+      for (int j = 1; j < N1; ++j) {
+        if (j <= p) { // uniform
+          const T a = a1[j] * z, b = b1[j] * rr;
+          const T nr = a * cr - b * pr, ni = a * ci - b * pi;
+          pr = cr;
+          pi = ci;
+          cr = nr;
+          ci = ni;
+          r1[j] += cr;
+          i1[j] -= ci;
         }
       }
-    // the 4 slots of a column: lanes col, col + 16, col + 32, col + 48
+    }
+    {
+      T cr = qm * s2 * e2r, ci = qm * s2 * e2i, pr = 0, pi = 0; // q R_m2^m2
+      r2[0] += cr;
+      i2[0] -= ci;
 #pragma unroll
-    for (int j = 0; j <= kMaxOrder; ++j) {
-      if (j <= p - m0) { // uniform: every lane shuffles
-        T vr = ar[j], vi = ai[j];
-        vr += __shfl_xor(vr, 16);
-        vi += __shfl_xor(vi, 16);
-        vr += __shfl_xor(vr, 32);
-        vi += __shfl_xor(vi, 32);
-        if (lane < 16 && j <= nj) {
-          const int n = m + j;
-          const int er = m == 0 ? n * n : n * n + 2 * m - 1;
-          M[er] = vr;
-          if (m > 0) M[er + 1] = vi;
+      for (int j = 1; j < N2; ++j) {
+        if (2 * j <= p) { // uniform: n2 <= (p + 1) / 2
+          const T a = a2[j] * z, b = b2[j] * rr;
+          const T nr = a * cr - b * pr, ni = a * ci - b * pi;
+          pr = cr;
+          pi = ci;
+          cr = nr;
+          ci = ni;
+          r2[j] += cr;
+          i2[j] -= ci;
         }
+      }
+    }
+  }
+  // the slots of a column lane: lanes cl, cl + LP, cl + 2 LP, ...
+  T *M = Mexp + (size_t)slot * KP;
+#pragma unroll
+  for (int j = 0; j < N1; ++j) {
+    if (j <= p) { // uniform: every lane shuffles
+      T vr = r1[j], vi = i1[j];
+#pragma unroll
+      for (int o = LP; o < 64; o <<= 1) {
+        vr += __shfl_xor(vr, o);
+        vi += __shfl_xor(vi, o);
+      }
+      if (lane < LP && j < n1) {
+        const int n = m1 + j;
+        const int er = m1 == 0 ? n * n : n * n + 2 * m1 - 1;
+        M[er] = vr;
+        if (m1 > 0) M[er + 1] = vi;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < N2; ++j) {
+    if (2 * j <= p) { // uniform
+      T vr = r2[j], vi = i2[j];
+#pragma unroll
+      for (int o = LP; o < 64; o <<= 1) {
+        vr += __shfl_xor(vr, o);
+        vi += __shfl_xor(vi, o);
+      }
+      if (lane < LP && j < n2) {
+        const int n = m2 + j;
+        const int er = n * n + 2 * m2 - 1; // m2 > m1 >= 0
+        M[er] = vr;
+        M[er + 1] = vi;
       }
     }
   }
@@ -767,12 +820,25 @@ void run_expansions(nbfmm_ctx *c) {
   NB_HIP(hipMemsetAsync(Mexp, 0, esz * (size_t)nM * KP, s));
   if (nP2M > 0) {
     const unsigned nb = (unsigned)((nP2M + kT / 64 - 1) / (kT / 64));
-    if (f32)
-      hipLaunchKernelGGL(k_p2m<float>, dim3(nb), dim3(kT), 0, s, p, KP, nP2M, (const int32_t *)p2m_cells,
-                         (const double4 *)c->pos.p, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (float *)Mexp);
-    else
-      hipLaunchKernelGGL(k_p2m<double>, dim3(nb), dim3(kT), 0, s, p, KP, nP2M, (const int32_t *)p2m_cells,
-                         (const double4 *)c->pos.p, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (double *)Mexp);
+#define NB_P2M(T, LP, PM)                                                                                       \
+  hipLaunchKernelGGL((k_p2m<T, LP, PM>), dim3(nb), dim3(kT), 0, s, p, KP, nP2M, (const int32_t *)p2m_cells,      \
+                     (const double4 *)c->pos.p, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (T *)Mexp)
+    if (f32) {
+      if (p <= 8)
+        NB_P2M(float, 8, 8);
+      else if (p <= 12)
+        NB_P2M(float, 8, 12);
+      else
+        NB_P2M(float, 16, 16);
+    } else {
+      if (p <= 8)
+        NB_P2M(double, 8, 8);
+      else if (p <= 12)
+        NB_P2M(double, 8, 12);
+      else
+        NB_P2M(double, 16, 16);
+    }
+#undef NB_P2M
     NB_LAUNCH();
   }
   // ---- M2M, finest level first ----
