@@ -425,6 +425,84 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
     if (lane + 64 * r < nmax) L[lane + 64 * r] = (T)((double)L[lane + 64 * r] + s[r]);
 }
 
+// FP32 partials with one 16-byte load per pair: lane l (< KP / 4) sums rows
+// 4l .. 4l + 3 in FP64 (k_m2l_sum<float> needs KP / 64 load instructions per
+// pair).  Pairs in flight per wavefront, measured at 4M with P2P alongside:
+// 1 best (M2L alone 5.23 -> 4.72 ms, the overlapped evaluation 17.95 ->
+// 17.82 ms); 2, 4 and 8 were faster alone but slower overlapped (18.0-18.8 ms:
+// more loads in flight take memory bandwidth from the P2P beside them).
+#ifndef NBFMM_M2L_SUM4_UNROLL
+#define NBFMM_M2L_SUM4_UNROLL 1
+#endif
+__global__ void __launch_bounds__(kT) k_m2l_sum4(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
+                                                 const int32_t *__restrict__ off_m2l,
+                                                 const int32_t *__restrict__ cnt_m2l, const int32_t *__restrict__ lslot,
+                                                 const int32_t *__restrict__ pos, int32_t cb, int32_t ce,
+                                                 const float *__restrict__ Y, float *__restrict__ Lexp,
+                                                 const uint16_t *__restrict__ bins) {
+  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= ntgt) return;
+  const int32_t c = tgt_cells[t];
+  const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
+  float *L = Lexp + (size_t)lslot[c] * KP;
+  const int r0 = 4 * lane;
+  const bool ld = r0 < KP;
+  double s[4] = {0, 0, 0, 0};
+  int nmax = 0; // rows any pair added
+  for (int32_t q0 = b; q0 < e; q0 += 64) {
+    // the next 64 entries' partial positions and row counts, one per lane (as in k_m2l_sum)
+    const int32_t qi = q0 + lane;
+    int32_t prl = cb, nvl = 0, nva = 0;
+    if (qi < e) {
+      const int32_t pr = pos[qi];
+      nva = pair_rows(bins, qi, KP);
+      if (pr >= cb && pr < ce) {
+        prl = pr;
+        nvl = nva;
+      }
+    }
+    for (int o = 32; o >= 1; o >>= 1) nva = max(nva, __shfl_xor(nva, o));
+    nmax = max(nmax, nva);
+    const int m = min(64, e - q0);
+    int j = 0;
+    constexpr int kU = NBFMM_M2L_SUM4_UNROLL;
+    for (; j + kU <= m; j += kU) {
+      float4 v[kU];
+      int nv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int32_t pr = __builtin_amdgcn_readlane(prl, j + u);
+        nv[u] = __builtin_amdgcn_readlane(nvl, j + u);
+        v[u] = ld && r0 < nv[u] ? *reinterpret_cast<const float4 *>(Y + (size_t)(pr - cb) * KP + r0)
+                                : make_float4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        s[0] += r0 < nv[u] ? (double)v[u].x : 0.0;
+        s[1] += r0 + 1 < nv[u] ? (double)v[u].y : 0.0;
+        s[2] += r0 + 2 < nv[u] ? (double)v[u].z : 0.0;
+        s[3] += r0 + 3 < nv[u] ? (double)v[u].w : 0.0;
+      }
+    }
+    for (; j < m; ++j) {
+      const int32_t pr = __builtin_amdgcn_readlane(prl, j);
+      const int nv = __builtin_amdgcn_readlane(nvl, j);
+      if (ld && r0 < nv) {
+        const float4 v = *reinterpret_cast<const float4 *>(Y + (size_t)(pr - cb) * KP + r0);
+        s[0] += (double)v.x;
+        s[1] += r0 + 1 < nv ? (double)v.y : 0.0;
+        s[2] += r0 + 2 < nv ? (double)v.z : 0.0;
+        s[3] += r0 + 3 < nv ? (double)v.w : 0.0;
+      }
+    }
+  }
+  if (ld)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (r0 + i < nmax) L[r0 + i] = (float)((double)L[r0 + i] + s[i]);
+}
+
 // Row blocks per workgroup: the fewest slices of the KP / 16 row blocks that
 // fit the LDS, split evenly (at KP = 144 in FP64: 5 + 4, not 8 + 1, so that
 // no slice reloads the multipoles for a single row block).
@@ -802,7 +880,12 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
       launch_p2p(c, 1); // the rest of P2P beside the rest of the chain
       c->p2p_started = true;
     }
-    if (f32)
+    if (f32 && KP % 4 == 0 && KP <= 4 * 64)
+      hipLaunchKernelGGL(k_m2l_sum4, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
+                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
+                         (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
+                         (const float *)c->ypair.p, (float *)Lexp, binned ? (const uint16_t *)c->m2l_cls.p : nullptr);
+    else if (f32)
       hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
                          (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
                          (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
