@@ -270,6 +270,29 @@ int nbfmm_batch_step_se_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets
 int nbfmm_batch_step_rk4_dev(nbfmm_ctx *ctx, int32_t nsys, const int64_t *offsets, double *d_xyz, double *d_vel,
                              double *d_acc, const double *d_m, double dt, int nsteps, int mode);
 
+/* Integration steps replayed from a HIP graph (no counterpart in the
+ * reference, which calls RungeKutta4::integrate step by step,
+ * runge_kutta_4.cpp:32,50,66,81 evaluating four times per step).  With
+ * enabled = 1, the step functions above (and nbfmm_run_dev) capture one whole
+ * step -- every evaluation of it (tree, lists, expansions, P2P on both
+ * streams) and the integrator updates -- as a graph, and replay it for the
+ * following steps with the same arrays, n, dt, parameters and tiles.  A
+ * replayed evaluation reads no size back to the host: its launches and
+ * buffers are sized by capacities a quarter above the sizes of the last
+ * direct evaluation, and the kernels read the actual sizes on the device.
+ * When a size exceeds its capacity (or the tree depth changes) the replayed
+ * step leaves the state untouched, and the step runs again directly and is
+ * captured anew.  Results equal the direct steps' to rounding (the near-field
+ * sums meet in FP64 atomics).  The first step of a run, and every step after
+ * a change, runs directly.  Requires order_select = NBFMM_ORDER_FIXED and
+ * m2l_pass_pairs = 0 (NBFMM_ERR_UNSUPPORTED otherwise); graph steps take no
+ * timing events, and the statistics (nbfmm_get_stats, nbfmm_tree_get) keep
+ * describing the last direct evaluation. */
+int nbfmm_set_graph(nbfmm_ctx *ctx, int enabled);
+/* out[0] graphs captured, out[1] steps replayed, out[2] replayed steps whose
+ * sizes overflowed and that ran again directly, out[3] steps run directly. */
+int nbfmm_graph_stats(nbfmm_ctx *ctx, int64_t *out4);
+
 /* Tile shapes and launch split of the P2P and M2L kernels (no counterpart
  * in the reference, whose serial code has none; the tuner in
  * nbody-fmm_amd/tune.py searches them per workload, as it searches the

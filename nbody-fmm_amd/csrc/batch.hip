@@ -93,20 +93,38 @@ __global__ void k_unscale(int64_t n, int nsys, const int64_t *__restrict__ off, 
   acc[3 * i + 2] = ab[3 * i + 2] * s2;
 }
 
+// Placement of every system: s = 2^e with s * extent <= 0.98 (e = floor(log2(0.98 /
+// extent)), from the exponent of 0.98 / extent: exact), so the system stays
+// inside its cell; t moves the box centre to the cell centre (2a+1.5, 2b+1.5,
+// 2c+1.5), a, b, c the system's grid coordinates.
+__global__ void k_place_plan(int nsys, int g, const double *__restrict__ bb, double4 *__restrict__ st) {
+#pragma clang fp contract(off)
+  const int k = (int)(blockIdx.x * kT + threadIdx.x);
+  if (k >= nsys) return;
+  const double *b = bb + 6 * k;
+  const double ext = fmax(b[3] - b[0], fmax(b[4] - b[1], b[5] - b[2]));
+  int e = 0;
+  if (ext > 0) {
+    int x2 = 0;
+    (void)frexp(0.98 / ext, &x2); // 0.98 / ext = f 2^x2, f in [0.5, 1)
+    e = x2 - 1;
+  }
+  const double sc = ldexp(1.0, e);
+  const int a = k % g, bq = (k / g) % g, cq = k / (g * g);
+  const double org[3] = {2.0 * a + 1.5, 2.0 * bq + 1.5, 2.0 * cq + 1.5}; // cell centres
+  double t[3];
+  for (int d = 0; d < 3; ++d) t[d] = org[d] - sc * 0.5 * (b[d] + b[3 + d]);
+  st[k] = make_double4(t[0], t[1], t[2], sc);
+}
+
 } // namespace
 
-void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *h_off, const double *d_xyz, const double *d_m,
+void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *d_off, int64_t n, const double *d_xyz, const double *d_m,
                 double *d_acc, const std::function<void(int64_t, const double *, const double *, double *)> &eval) {
   hipStream_t s = c->stream;
-  const int64_t n = h_off[nsys];
-  int64_t *doff = dget<int64_t>(c->bat_off, (size_t)nsys + 1);
-  NB_HIP(hipMemcpyAsync(doff, h_off, sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice, s));
   double *bb = dget<double>(c->bat_bb, (size_t)6 * nsys);
-  hipLaunchKernelGGL(k_sys_bbox, dim3(nsys), dim3(kT), 0, s, (const int64_t *)doff, d_xyz, bb);
+  hipLaunchKernelGGL(k_sys_bbox, dim3(nsys), dim3(kT), 0, s, d_off, d_xyz, bb);
   NB_LAUNCH();
-  std::vector<double> hb((size_t)6 * nsys);
-  NB_HIP(hipMemcpyAsync(hb.data(), bb, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
   // grid: g systems per axis at odd cell coordinates, W = 2^L0 >= 2g + 2 cells
   int g = 1;
   while ((int64_t)g * g * g < nsys) ++g;
@@ -114,27 +132,14 @@ void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *h_off, const double *d_xy
   while ((1 << L0) < 2 * g + 2) ++L0;
   if (L0 + 1 > kMaxDepth) throw Error(NBFMM_ERR_UNSUPPORTED, "too many systems for one batch");
   const double W = (double)(1 << L0);
-  std::vector<double> st((size_t)4 * nsys);
-  for (int k = 0; k < nsys; ++k) {
-    const double *b = &hb[6 * k];
-    for (int d = 0; d < 6; ++d)
-      if (!std::isfinite(b[d])) throw Error(NBFMM_ERR_DEGENERATE, "non-finite coordinate in a system");
-    const double ext = std::max(b[3] - b[0], std::max(b[4] - b[1], b[5] - b[2]));
-    // s = 2^e with s * extent <= 0.98: the system stays inside its cell
-    const int e = ext > 0 ? (int)std::floor(std::log2(0.98 / ext)) : 0;
-    const double sc = std::ldexp(1.0, e);
-    const int a = k % g, bq = (k / g) % g, cq = k / (g * g);
-    const double org[3] = {2.0 * a + 1.5, 2.0 * bq + 1.5, 2.0 * cq + 1.5}; // cell centres
-    for (int d = 0; d < 3; ++d) st[4 * k + d] = org[d] - sc * 0.5 * (b[d] + b[3 + d]);
-    st[4 * k + 3] = sc;
-  }
   double4 *dst = dget<double4>(c->bat_st, (size_t)nsys);
-  NB_HIP(hipMemcpyAsync(dst, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_place_plan, dim3(nblk(nsys)), dim3(kT), 0, s, nsys, g, (const double *)bb, dst);
+  NB_LAUNCH();
   double *xb = dget<double>(c->bat_x, (size_t)3 * (n + 2));
   double *mb = dget<double>(c->bat_m, (size_t)(n + 2));
   double *ab = dget<double>(c->bat_a, (size_t)3 * (n + 2));
-  hipLaunchKernelGGL(k_place, dim3(nblk(n + 2)), dim3(kT), 0, s, n, nsys, (const int64_t *)doff,
-                     (const double4 *)dst, d_xyz, d_m, W, xb, mb);
+  hipLaunchKernelGGL(k_place, dim3(nblk(n + 2)), dim3(kT), 0, s, n, nsys, d_off, (const double4 *)dst, d_xyz, d_m, W,
+                     xb, mb);
   NB_LAUNCH();
   struct Level {
     nbfmm_ctx *c;
@@ -143,9 +148,15 @@ void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *h_off, const double *d_xy
   } restore{c, c->list_level};
   c->list_level = L0 + 1;
   eval(n + 2, xb, mb, ab);
-  hipLaunchKernelGGL(k_unscale, dim3(nblk(n)), dim3(kT), 0, s, n, nsys, (const int64_t *)doff, (const double4 *)dst,
+  hipLaunchKernelGGL(k_unscale, dim3(nblk(n)), dim3(kT), 0, s, n, nsys, d_off, (const double4 *)dst,
                      (const double *)ab, d_acc);
   NB_LAUNCH();
+}
+
+int64_t *upload_offsets(nbfmm_ctx *c, int nsys, const int64_t *h_off) {
+  int64_t *doff = dget<int64_t>(c->bat_off, (size_t)nsys + 1);
+  NB_HIP(hipMemcpy(doff, h_off, sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice));
+  return doff;
 }
 
 } // namespace nbfmm

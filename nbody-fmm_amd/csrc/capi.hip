@@ -1,6 +1,8 @@
 // capi.hip -- the extern "C" boundary of libnbfmm.so (declared in include/nbfmm.h).
 #include "nbfmm_internal.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -59,6 +61,12 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
   c->plan.n = n;
   c->phi_out = d_phi;
   c->nev = 0;
+  const bool planned = c->cap.on;
+  if (!planned) {
+    // a direct evaluation: its sizes set the capacities of later planned ones
+    c->last = PlanSizes{};
+    fill_i32(c->stream, &dplan(c)->ovf, 1, 0);
+  }
   mark(c, 0);
   build_tree(c, d_xyz, d_m);
   mark(c, 1);
@@ -74,7 +82,21 @@ void eval_dev(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m, d
   run_finish(c, d_acc);
   c->phi_out = nullptr;
   mark(c, 6);
+  if (planned) return; // the sizes stay on the device
   Plan &P = c->plan;
+  {
+    PlanSizes &L = c->last;
+    L.depth = P.geo.depth;
+    L.n = n;
+    L.nc = P.n_cells;
+    L.parents = P.lvl_start[P.geo.depth];
+    for (int d = 0; d <= P.geo.depth; ++d) L.lev[d] = P.lvl_start[d + 1] - P.lvl_start[d];
+    L.n_m2l = P.n_m2l;
+    L.n_sub = P.n_sub;
+    L.n_tgt = P.n_tgt;
+    L.ntask = P.n_tasks;
+    L.nnarrow = P.n_tasks_narrow;
+  }
   nbfmm_stats &S = c->stats;
   std::memset(&S, 0, sizeof(S));
   S.depth = P.geo.depth;
@@ -177,12 +199,159 @@ void step_rk4(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, const do
   step_rk4(c, n, x, v, a, [&](const double *xx, double *aa) { eval_dev(c, n, xx, m, aa); }, dt, mode);
 }
 
-// the batched evaluation of nsys systems (offsets on the host)
-Eval batch_evaluator(nbfmm_ctx *c, int nsys, const int64_t *off, const double *m) {
+// the batched evaluation of nsys systems (offsets on the device)
+Eval batch_evaluator(nbfmm_ctx *c, int nsys, const int64_t *d_off, int64_t n, const double *m) {
   return [=](const double *x, double *a) {
-    batch_eval(c, nsys, off, x, m, a,
+    batch_eval(c, nsys, d_off, n, x, m, a,
                [c](int64_t nb, const double *xb, const double *mb, double *ab) { eval_dev(c, nb, xb, mb, ab); });
   };
+}
+
+// ---- integration steps, direct or replayed from a captured graph ----------------------
+// kind: 0 SyplecticEuler, 1 RungeKutta4 with the reference's semantics, 2 classic RK4
+struct StepSpec {
+  int kind;
+  int64_t n;
+  double *x, *v, *a;
+  const double *m;
+  double dt;
+  int nsys;             // 0: one system
+  const int64_t *d_off; // a batch's offsets (device)
+};
+
+// one step; the plan's overflow flag is cleared first (a planned step's
+// kernels leave at once once it is set, and the integrator does not update)
+void step_one(nbfmm_ctx *c, const StepSpec &sp, const Eval &ev) {
+  fill_i32(c->stream, &dplan(c)->ovf, 1, 0);
+  if (sp.kind == 0)
+    step_se(c, sp.n, sp.x, sp.v, sp.a, ev, sp.dt);
+  else
+    step_rk4(c, sp.n, sp.x, sp.v, sp.a, ev, sp.dt, sp.kind == 1 ? 0 : 1);
+}
+
+bool same_key(const nbfmm_ctx::GraphKey &k, const StepSpec &sp, const nbfmm_ctx *c) {
+  return k.kind == sp.kind && k.n == sp.n && k.x == sp.x && k.v == sp.v && k.a == sp.a && k.m == sp.m &&
+         k.dt == sp.dt && k.nsys == sp.nsys && k.off == sp.d_off && std::memcmp(&k.prm, &c->prm, sizeof(k.prm)) == 0 &&
+         std::memcmp(&k.shapes, &c->shapes, sizeof(k.shapes)) == 0 && k.gen == dbuf_generation();
+}
+
+void drop_graph(nbfmm_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->gkey = nbfmm_ctx::GraphKey{};
+}
+
+// Captures one step as a graph: the capacities come from the last direct
+// evaluation (a quarter more); a planned evaluation outside the capture
+// allocates every buffer at them, so the capture allocates nothing.
+void capture_step(nbfmm_ctx *c, const StepSpec &sp, const Eval &ev) {
+  drop_graph(c);
+  if (c->isolated) throw Error(NBFMM_ERR_UNSUPPORTED, "graph steps do not run with isolated timing");
+  if (c->prm.order_select != NBFMM_ORDER_FIXED) throw Error(NBFMM_ERR_UNSUPPORTED, "graph steps take a fixed order");
+  if (c->keep_lists) throw Error(NBFMM_ERR_UNSUPPORTED, "graph steps keep no lists");
+  if (c->last.depth < 0) throw Error(NBFMM_ERR_HIP, "no direct evaluation to plan the graph step from");
+  const bool timing = c->timing;
+  struct Restore {
+    nbfmm_ctx *c;
+    bool timing;
+    ~Restore() {
+      c->cap.on = false;
+      c->timing = timing;
+    }
+  } restore{c, timing};
+  c->cap = caps_from(c->last);
+  c->cap.on = true;
+  c->timing = false;
+  // the allocation pass: the step's own scratch and one planned evaluation
+  const size_t m3 = (size_t)sp.n * 3;
+  double *scratch = dget<double>(c->ak, m3);
+  if (sp.kind != 0) {
+    dget<double>(c->xk, m3);
+    dget<double>(c->vk, m3);
+    dget<double>(c->asum, m3);
+    dget<double>(c->vsum, m3);
+  }
+  fill_i32(c->stream, &dplan(c)->ovf, 1, 0);
+  ev(sp.x, scratch);
+  NB_HIP(hipStreamSynchronize(c->stream));
+  const uint64_t gen = dbuf_generation();
+  hipGraph_t g = nullptr;
+  set_capturing(true);
+  NB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    step_one(c, sp, ev);
+  } catch (...) {
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    set_capturing(false);
+    throw;
+  }
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  set_capturing(false);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    throw Error(NBFMM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    c->gexec = nullptr;
+    throw Error(NBFMM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  if (dbuf_generation() != gen) { // nothing may move between the allocation pass and the replays
+    drop_graph(c);
+    throw Error(NBFMM_ERR_HIP, "a buffer grew during the graph capture");
+  }
+  nbfmm_ctx::GraphKey &k = c->gkey;
+  k.kind = sp.kind;
+  k.n = sp.n;
+  k.x = sp.x;
+  k.v = sp.v;
+  k.a = sp.a;
+  k.m = sp.m;
+  k.dt = sp.dt;
+  k.nsys = sp.nsys;
+  k.off = sp.d_off;
+  k.prm = c->prm;
+  k.shapes = c->shapes;
+  k.gen = gen;
+  c->g_captures++;
+}
+
+// nsteps steps; with graphs on, a step replays the captured graph of the
+// same step (same arrays, sizes, dt and parameters), or runs directly and
+// captures one for the next.  A replay whose planned sizes overflowed left
+// the state unchanged: the step runs again directly (and is captured anew).
+void run_steps(nbfmm_ctx *c, const StepSpec &sp, int nsteps, const Eval &ev,
+               const std::function<void(int)> &after = nullptr) {
+  for (int st = 1; st <= nsteps; ++st) {
+    if (!c->graph_on) {
+      step_one(c, sp, ev);
+    } else if (c->gexec && same_key(c->gkey, sp, c)) {
+      NB_HIP(hipGraphLaunch(c->gexec, c->stream));
+      c->g_launches++;
+      read_plan_field(c, &DevPlan::ovf, c->stream);
+      if (c->hplan->ovf) {
+        c->g_redone++;
+        drop_graph(c);
+        step_one(c, sp, ev);
+        c->g_direct++;
+      }
+    } else {
+      step_one(c, sp, ev);
+      c->g_direct++;
+      if (st < nsteps) capture_step(c, sp, ev);
+    }
+    if (after) after(st);
+  }
+}
+
+// the one-system step of `kind` (0 SE, 1 RK4 reference semantics, 2 classic RK4)
+StepSpec one_system(int kind, int64_t n, double *x, double *v, double *a, const double *m, double dt) {
+  return StepSpec{kind, n, x, v, a, m, dt, 0, nullptr};
+}
+Eval one_system_eval(nbfmm_ctx *c, int64_t n, const double *m) {
+  return [c, n, m](const double *xx, double *aa) { eval_dev(c, n, xx, m, aa); };
 }
 
 void check_batch(int nsys, const int64_t *off) {
@@ -201,8 +370,35 @@ void check_n(int64_t n) {
 } // namespace
 
 namespace nbfmm {
+namespace {
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (((uintptr_t)p & 15) == 0) { // four at a time where aligned
+    int4 *q = reinterpret_cast<int4 *>(p);
+    const int64_t n4 = n / 4;
+    for (int64_t j = i; j < n4; j += stride) q[j] = make_int4(v, v, v, v);
+    for (int64_t j = 4 * n4 + i; j < n; j += stride) p[j] = v;
+  } else {
+    for (; i < n; i += stride) p[i] = v;
+  }
+}
+std::atomic<uint64_t> g_dbuf_gen{0}; // (re)allocations of any context's buffers
+thread_local bool g_capturing = false;
+} // namespace
+uint64_t dbuf_generation() { return g_dbuf_gen.load(); }
+void fill_i32(hipStream_t s, void *p, int64_t count, int32_t v) {
+  if (count <= 0) return;
+  const int64_t nb = std::min<int64_t>(4096, (count / 4 + 255) / 256 + 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)nb), dim3(256), 0, s, (int32_t *)p, count, v);
+  NB_LAUNCH();
+}
+void set_capturing(bool on) { g_capturing = on; }
 void *dbuf_ensure(DBuf &b, size_t bytes) {
   if (b.cap >= bytes && b.p) return b.p;
+  // a captured graph holds the addresses: nothing may move while capturing
+  if (g_capturing) throw Error(NBFMM_ERR_HIP, "buffer growth during a graph capture");
+  g_dbuf_gen++;
   if (b.p) NB_HIP(hipFree(b.p));
   b.p = nullptr;
   b.cap = 0;
@@ -215,6 +411,35 @@ void dbuf_free(DBuf &b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.cap = 0;
+}
+void read_plan(nbfmm_ctx *c, const void *field, size_t bytes, hipStream_t s) {
+  const size_t off = (size_t)((const char *)field - (const char *)c->hplan);
+  NB_HIP(hipMemcpyAsync((char *)c->hplan + off, (const char *)c->dplan.p + off, bytes, hipMemcpyDeviceToHost, s));
+  NB_HIP(hipStreamSynchronize(s));
+}
+Caps caps_from(const PlanSizes &s) {
+  auto up = [](int64_t v) { return v + v / 4 + 64; };
+  Caps K;
+  K.depth = s.depth;
+  K.nc = up(s.nc);
+  K.parents = 0;
+  for (int d = 0; d <= kMaxDepth; ++d) {
+    K.lev[d] = d <= s.depth ? (int32_t)up(s.lev[d]) : 0;
+    if (d < s.depth) K.parents += K.lev[d];
+    K.m2m_lev[d] = (int32_t)up(s.m2m_lev[d]);
+    K.l2l_lev[d] = (int32_t)up(s.l2l_lev[d]);
+  }
+  K.n_m2l = up(s.n_m2l);
+  K.n_sub = up(s.n_sub);
+  K.n_tgt = up(s.n_tgt);
+  K.ntask = up(s.ntask);
+  K.nnarrow = up(s.nnarrow);
+  K.nM = up(s.nM);
+  K.nL = up(s.nL);
+  K.nP2M = up(s.nP2M);
+  K.items = up(s.items) + kMaxBins;
+  for (int oi = 0; oi < kNumOrders; ++oi) K.items_ord[oi] = (int32_t)up(s.items_ord[oi]);
+  return K;
 }
 void mark(nbfmm_ctx *c, int slot) {
   if (c->timing) NB_HIP(hipEventRecord(c->ev[slot], c->stream));
@@ -234,9 +459,11 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &tiles, &stab, &tmat, &tmat32, &mexp64, &lexp64, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
                  &ord, &msum, &feat, &rmax, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
                  &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc, &bat_off, &bat_bb, &bat_st, &bat_x, &bat_m,
-                 &bat_a};
+                 &bat_a, &dplan};
   for (DBuf *b : all) dbuf_free(*b);
   if (hred.p) (void)hipHostFree(hred.p);
+  if (hplan) (void)hipHostFree(hplan);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -290,6 +517,11 @@ int nbfmm_create(const nbfmm_params *params, nbfmm_ctx **out) {
   scratch_init(c->device); // this device's copy of the fixed scratch (once per device)
   NB_HIP(hipHostMalloc(&c->hred.p, 65536, hipHostMallocDefault));
   c->hred.cap = 65536;
+  dbuf_ensure(c->dplan, sizeof(DevPlan));
+  NB_HIP(hipMemset(c->dplan.p, 0, sizeof(DevPlan)));
+  NB_HIP(hipHostMalloc((void **)&c->hplan, sizeof(DevPlan), hipHostMallocDefault));
+  std::memset((void *)c->hplan, 0, sizeof(DevPlan));
+  upload_consts(c);
   *out = c;
   c = nullptr;
   }
@@ -426,7 +658,7 @@ int nbfmm_step_se_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, 
   if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
   API_BEGIN
   check_n(n);
-  for (int s = 0; s < nsteps; ++s) step_se(c, n, x, v, a, m, dt);
+  run_steps(c, one_system(0, n, x, v, a, m, dt), nsteps, one_system_eval(c, n, m));
   API_END
 }
 
@@ -437,7 +669,7 @@ int nbfmm_step_rk4_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a,
   if (mode != 0 && mode != 1) return fail(NBFMM_ERR_ARG, "mode must be 0 or 1");
   API_BEGIN
   check_n(n);
-  for (int st = 0; st < nsteps; ++st) step_rk4(c, n, x, v, a, m, dt, mode);
+  run_steps(c, one_system(mode == 0 ? 1 : 2, n, x, v, a, m, dt), nsteps, one_system_eval(c, n, m));
   API_END
 }
 
@@ -477,13 +709,9 @@ int nbfmm_run_dev(nbfmm_ctx *c, int64_t n, double *x, double *v, double *a, cons
     diagnostics(c, n, x, v, m, dp, h_diag + 10 * rec++);
   };
   if (h_diag) monitor();
-  for (int s = 1; s <= nsteps; ++s) {
-    if (integrator == 0)
-      step_se(c, n, x, v, a, m, dt);
-    else
-      step_rk4(c, n, x, v, a, m, dt, integrator == 1 ? 0 : 1);
+  run_steps(c, one_system(integrator, n, x, v, a, m, dt), nsteps, one_system_eval(c, n, m), [&](int s) {
     if (h_diag && s % every == 0) monitor();
-  }
+  });
   NB_HIP(hipStreamSynchronize(c->stream));
   API_END
 }
@@ -614,7 +842,8 @@ int nbfmm_batch_eval_accel_dev(nbfmm_ctx *c, int32_t nsys, const int64_t *offset
   if (!c || !d_xyz || !d_m || !d_acc) return fail(NBFMM_ERR_ARG, "null argument");
   API_BEGIN
   check_batch(nsys, offsets);
-  batch_evaluator(c, nsys, offsets, d_m)(d_xyz, d_acc);
+  const int64_t *d_off = upload_offsets(c, nsys, offsets);
+  batch_evaluator(c, nsys, d_off, offsets[nsys], d_m)(d_xyz, d_acc);
   API_END
 }
 
@@ -624,8 +853,9 @@ int nbfmm_batch_step_se_dev(nbfmm_ctx *c, int32_t nsys, const int64_t *offsets, 
   if (nsteps < 0) return fail(NBFMM_ERR_ARG, "nsteps must be >= 0");
   API_BEGIN
   check_batch(nsys, offsets);
-  const Eval ev = batch_evaluator(c, nsys, offsets, m);
-  for (int st = 0; st < nsteps; ++st) step_se(c, offsets[nsys], x, v, a, ev, dt);
+  const int64_t *d_off = upload_offsets(c, nsys, offsets);
+  const int64_t n = offsets[nsys];
+  run_steps(c, StepSpec{0, n, x, v, a, m, dt, nsys, d_off}, nsteps, batch_evaluator(c, nsys, d_off, n, m));
   API_END
 }
 
@@ -636,9 +866,34 @@ int nbfmm_batch_step_rk4_dev(nbfmm_ctx *c, int32_t nsys, const int64_t *offsets,
   if (mode != 0 && mode != 1) return fail(NBFMM_ERR_ARG, "mode must be 0 or 1");
   API_BEGIN
   check_batch(nsys, offsets);
-  const Eval ev = batch_evaluator(c, nsys, offsets, m);
-  for (int st = 0; st < nsteps; ++st) step_rk4(c, offsets[nsys], x, v, a, ev, dt, mode);
+  const int64_t *d_off = upload_offsets(c, nsys, offsets);
+  const int64_t n = offsets[nsys];
+  run_steps(c, StepSpec{mode == 0 ? 1 : 2, n, x, v, a, m, dt, nsys, d_off}, nsteps,
+            batch_evaluator(c, nsys, d_off, n, m));
   API_END
+}
+
+int nbfmm_set_graph(nbfmm_ctx *c, int enabled) {
+  if (!c) return fail(NBFMM_ERR_ARG, "ctx is null");
+  if (enabled != 0 && enabled != 1) return fail(NBFMM_ERR_ARG, "enabled must be 0 or 1");
+  API_BEGIN
+  if (enabled && c->prm.order_select != NBFMM_ORDER_FIXED)
+    throw Error(NBFMM_ERR_UNSUPPORTED, "graph steps take a fixed order (order_select = NBFMM_ORDER_FIXED)");
+  if (enabled && c->prm.m2l_pass_pairs > 0)
+    throw Error(NBFMM_ERR_UNSUPPORTED, "graph steps run the M2L in one pass (m2l_pass_pairs = 0)");
+  (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  c->graph_on = enabled != 0;
+  API_END
+}
+
+int nbfmm_graph_stats(nbfmm_ctx *c, int64_t *out4) {
+  if (!c || !out4) return fail(NBFMM_ERR_ARG, "null argument");
+  out4[0] = c->g_captures;
+  out4[1] = c->g_launches;
+  out4[2] = c->g_redone;
+  out4[3] = c->g_direct;
+  return NBFMM_OK;
 }
 
 int nbfmm_get_stats(nbfmm_ctx *c, nbfmm_stats *s) {

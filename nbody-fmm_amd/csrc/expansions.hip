@@ -24,6 +24,8 @@
 // (m2l.hip); P2M and L2P run on the vector ALUs.
 #include "nbfmm_internal.h"
 
+#include <cstddef>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -70,8 +72,10 @@ void fill_consts(Consts &C) {
 // offset, then the real KP x KP matrix in MFMA A-fragment order:
 //   frag[(rb*KS + ks)*64 + lane] = T[rb*16 + lane%16][ks*4 + lane/16].
 // L_j^k = (-1)^{j+k}/N_jk sum_{n,m} (M_n^m/N_nm) S_{n+j}^{m-k}(D),  N_nm = sqrt((n+m)!(n-m)!)
-__global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, double rx, double ry, double rz,
+// (rx, ry, rz): the leaf cell's edges over rho, from the plan
+__global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, const DevPlan *__restrict__ dp,
                                                   double *__restrict__ frag) {
+  const double rx = dp->hs[0], ry = dp->hs[1], rz = dp->hs[2];
   const int cls = blockIdx.x;
   const int dx = cls % 7 - 3, dy = (cls / 7) % 7 - 3, dz = cls / 49 - 3;
   const int P2 = 2 * p;
@@ -201,8 +205,9 @@ __device__ __forceinline__ void yget(const double *Rre, const double *Rim, int n
   }
 }
 
-__global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, double hx, double hy, double hz,
+__global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, const DevPlan *__restrict__ dp,
                                                     double *__restrict__ m2m, double *__restrict__ l2l) {
+  const double hx = dp->hs[0], hy = dp->hs[1], hz = dp->hs[2];
   const int o = blockIdx.x, op = blockIdx.y;
   __shared__ double Rre[(kMaxOrder + 1) * (kMaxOrder + 2) / 2];
   __shared__ double Rim[(kMaxOrder + 1) * (kMaxOrder + 2) / 2];
@@ -311,11 +316,15 @@ __global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, double hx, do
 // float in mixed precision); the offset from the cell centre is formed in
 // FP64 and scaled before rounding.
 template <class T, int LP, int PM>
-__global__ void __launch_bounds__(kT) k_p2m(int p, int KP, int ncell, const int32_t *__restrict__ cells,
+__global__ void __launch_bounds__(kT) k_p2m(int p, int KP, const DevPlan *__restrict__ dp,
+                                            const int32_t *__restrict__ cells,
                                             const double4 *__restrict__ pos, const int32_t *__restrict__ cbeg,
                                             const int32_t *__restrict__ cend, const int32_t *__restrict__ mslot,
-                                            const uint64_t *__restrict__ ckey, LevelGeo g,
+                                            const uint64_t *__restrict__ ckey,
                                             const Consts *__restrict__ C, T *__restrict__ Mexp) {
+  if (plan_dead(dp)) return;
+  const LevelGeo &g = dp->g;
+  const int ncell = dp->reads[1]; // cells forming their multipole by P2M
   static_assert(LP >= PM / 2 + 1 && 64 % LP == 0, "a column lane per pair of orders");
   constexpr int NS = 64 / LP;      // particle slots
   constexpr int N1 = PM + 1;       // terms of a lane's first column, at most
@@ -528,9 +537,11 @@ template <class T, bool POT>
 __global__ void __launch_bounds__(kT) k_l2p(int p, int KP, int64_t n, const double4 *__restrict__ pos,
                                             const int32_t *__restrict__ pcell, const int32_t *__restrict__ cparent,
                                             const uint64_t *__restrict__ ckey, const int32_t *__restrict__ eslot,
-                                            LevelGeo g, const Consts *__restrict__ C, const T *__restrict__ Lexp,
-                                            double *__restrict__ grad, double *__restrict__ pot,
-                                            const int8_t *__restrict__ ord) {
+                                            const DevPlan *__restrict__ dp, const Consts *__restrict__ C,
+                                            const T *__restrict__ Lexp, double *__restrict__ grad,
+                                            double *__restrict__ pot, const int8_t *__restrict__ ord) {
+  if (plan_dead(dp)) return;
+  const LevelGeo &g = dp->g;
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double4 q = pos[i];
@@ -583,18 +594,21 @@ constexpr int32_t kTransCut = 256;
 // and hold their children's particles, so the ancestors of a big parent are
 // all big: a cell needs its multipole iff it is a source or, with a big
 // parent, some ancestor is.  Locals alike, with "has an M2L list".  Also the
-// five flag arrays of length n + 1 (flag n = 0): M slot, P2M, M2M, L slot, L2L,
+// five flag arrays of stride s >= n + 1 (flags past the cells 0): M slot, P2M, M2M, L slot, L2L,
 // and the particles of the P2M cells (terms[0]) and of the cells that
 // evaluate their local (terms[1]); terms are zero on entry.
-__global__ void __launch_bounds__(kT) k_updown_modes(int64_t n, int32_t leaf_begin, const int32_t *__restrict__ cparent,
+__global__ void __launch_bounds__(kT) k_updown_modes(const DevPlan *__restrict__ dp, int D, int64_t s,
+                                                     const int32_t *__restrict__ cparent,
                                                      const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
                                                      const int32_t *__restrict__ is_src,
                                                      const int32_t *__restrict__ cnt_m2l, int32_t *mode_m,
                                                      int32_t *mode_l, int32_t *flags, unsigned long long *terms) {
+  if (plan_dead(dp)) return;
   __shared__ unsigned long long red[2][kT / 64];
-  const int64_t s = n + 1;
+  const int64_t n = dp->nc; // cells; the flags of [n, s) are zero
+  const int32_t leaf_begin = dp->lvl_start[D];
   unsigned long long tm = 0, tl = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i <= n; i += (int64_t)gridDim.x * kT) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < s; i += (int64_t)gridDim.x * kT) {
     int mm = 0, ml = 0;
     if (i < n) {
       const bool big = i < leaf_begin && cend[i] - cbeg[i] > kTransCut;
@@ -638,12 +652,13 @@ __global__ void __launch_bounds__(kT) k_updown_modes(int64_t n, int32_t leaf_beg
 
 // slots (M, L, evaluated L) and the P2M, M2M and L2L cell lists; `scan` is one
 // exclusive scan over the five flag arrays, so segment k starts at scan[k s]
-__global__ void k_updown_lists(const int32_t *__restrict__ mode_l, const int32_t *__restrict__ flags,
-                               const int32_t *__restrict__ scan, int64_t n, int32_t *mslot, int32_t *lslot,
-                               int32_t *eslot, int32_t *p2m_cells, int32_t *m2m_cells, int32_t *l2l_cells) {
+__global__ void k_updown_lists(const DevPlan *__restrict__ dp, int64_t s, const int32_t *__restrict__ mode_l,
+                               const int32_t *__restrict__ flags, const int32_t *__restrict__ scan, int32_t *mslot,
+                               int32_t *lslot, int32_t *eslot, int32_t *p2m_cells, int32_t *m2m_cells,
+                               int32_t *l2l_cells) {
+  if (plan_dead(dp)) return;
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  const int64_t s = n + 1;
+  if (i >= dp->nc) return;
   mslot[i] = flags[i] ? scan[i] : -1;
   const int32_t ls = flags[3 * s + i] ? scan[3 * s + i] - scan[3 * s] : -1;
   lslot[i] = ls;
@@ -653,14 +668,27 @@ __global__ void k_updown_lists(const int32_t *__restrict__ mode_l, const int32_t
   if (flags[4 * s + i]) l2l_cells[scan[4 * s + i] - scan[4 * s]] = (int32_t)i;
 }
 
-// The planning's host reads in one transfer: dst[k] = src[off[k]] - src[base[k]], k < cnt.
-struct ReadList {
-  int64_t off[64];
-  int64_t base[64];
-};
-__global__ void k_gather_reads(const int32_t *__restrict__ src, ReadList r, int cnt, int32_t *__restrict__ dst) {
+// The planning's sizes into the plan: the five segment totals (M cells, P2M
+// cells, M2M cells, L cells, L2L cells) and the M2M / L2L cells before each
+// level (the cell lists are level ordered).  With check, the capacities.
+__global__ void k_up_reads(DevPlan *dp, const int32_t *__restrict__ scan, int64_t s, int D, int check, int64_t cap_m,
+                           int64_t cap_p2m, int64_t cap_l, CapLevels cap_m2m, CapLevels cap_l2l) {
   const int k = threadIdx.x;
-  if (k < cnt) dst[k] = src[r.off[k]] - src[r.base[k]];
+  const int32_t nc = dp->nc;
+  if (k < 5) dp->reads[k] = scan[k * s + nc] - scan[k * s];
+  if (k <= D + 1) {
+    const int32_t ls = dp->lvl_start[k];
+    dp->reads[8 + k] = scan[2 * s + ls] - scan[2 * s];
+    dp->reads[32 + k] = scan[4 * s + ls] - scan[4 * s];
+  }
+  __syncthreads();
+  if (!check || k != 0) return;
+  int ovf = 0;
+  if (dp->reads[0] > cap_m || dp->reads[1] > cap_p2m || dp->reads[3] > cap_l) ovf = 2;
+  for (int d = 0; d < D; ++d)
+    if (dp->reads[9 + d] - dp->reads[8 + d] > cap_m2m.v[d] || dp->reads[33 + d] - dp->reads[32 + d] > cap_l2l.v[d])
+      ovf = 2;
+  if (ovf) dp->ovf |= ovf;
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT > 0 ? (n + kT - 1) / kT : 1); }
@@ -683,37 +711,40 @@ LevelGeo level_geo(const Plan &P) {
   return g;
 }
 
+void upload_consts(nbfmm_ctx *c) {
+  static Consts hC;
+  static bool init = false;
+  if (!init) {
+    fill_consts(hC);
+    init = true;
+  }
+  Consts *dC = (Consts *)dbuf_ensure(c->consts, sizeof(Consts));
+  NB_HIP(hipMemcpy(dC, &hC, sizeof(Consts), hipMemcpyHostToDevice));
+}
+
 void run_expansions(nbfmm_ctx *c) {
   Plan &P = c->plan;
   hipStream_t s = c->stream;
+  const Caps &K = c->cap;
+  DevPlan *dp = dplan(c);
   int p = (int)c->prm.order; // with learned orders: the highest order in use, set below
   int KP = padded_k(p);
   P.kp = KP;
   const int D = P.geo.depth;
   const bool f32 = c->prm.precision == NBFMM_MIXED;
   const bool learned = c->prm.order_select == NBFMM_ORDER_LEARNED;
-  const LevelGeo g = level_geo(P);
+  if (K.on && learned) throw Error(NBFMM_ERR_UNSUPPORTED, "planned evaluations take a fixed order");
   P.flop_m2l = 0;
   P.mean_order = p;
-  // constants
-  Consts *dC = (Consts *)dbuf_ensure(c->consts, sizeof(Consts));
-  {
-    static Consts hC;
-    static bool init = false;
-    if (!init) {
-      fill_consts(hC);
-      init = true;
-    }
-    NB_HIP(hipMemcpyAsync(dC, &hC, sizeof(Consts), hipMemcpyHostToDevice, s));
-  }
+  const Consts *dC = (const Consts *)c->consts.p; // uploaded at nbfmm_create
   // far-field gradient (3 per particle) and, with potentials, far-field potential sums
   const bool pot = c->phi_out != nullptr;
   double *grad = dget<double>(c->grad, (size_t)P.n * (pot ? 4 : 3));
   double *pfar = pot ? grad + 3 * (size_t)P.n : nullptr;
-  if (P.n_tgt == 0) {
+  if (!K.on && P.n_tgt == 0) {
     c->n_local = 0;
     P.n_p2m_cells = P.n_m2m_cells = P.n_l2l_cells = P.n_l2p_cells = 0;
-    NB_HIP(hipMemsetAsync(grad, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s));
+    fill_zero(s, grad, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n);
     P.n_p2m_terms = P.n_l2p_terms = 0;
     fork_p2p(c);
     launch_p2p(c, 0);
@@ -731,8 +762,7 @@ void run_expansions(nbfmm_ctx *c) {
   launch_p2p(c, 0);
   const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
   const uint64_t *ckey = (const uint64_t *)c->ckey.p;
-  const int64_t nc = P.n_cells;
-  const int32_t leaf_begin = P.lvl_start[D];
+  const int64_t nc = K.on ? K.nc : P.n_cells; // cells, or their capacity
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
   // ---- plan: modes per level (top-down), slots, lists ----
   int32_t *mode_m = dget<int32_t>(c->mode_m, (size_t)nc + 1);
@@ -740,11 +770,11 @@ void run_expansions(nbfmm_ctx *c) {
   const int64_t s5 = nc + 1;
   int32_t *flags = dget<int32_t>(c->flags, (size_t)5 * s5);
   int32_t *fscan = dget<int32_t>(c->fscan, (size_t)5 * s5);
-  unsigned long long *terms = (unsigned long long *)((double *)c->red.p + 6 * 1024 + 48);
-  NB_HIP(hipMemsetAsync(terms, 0, 2 * sizeof(unsigned long long), s));
+  unsigned long long *terms = dp->terms;
+  fill_zero(s, terms, 2 * sizeof(unsigned long long));
   // few workgroups: this runs beside the first share of P2P
-  hipLaunchKernelGGL(k_updown_modes, dim3(std::min<unsigned>(512, nblk(nc + 1))), dim3(kT), 0, s, nc, leaf_begin,
-                     (const int32_t *)c->cparent.p, cbeg, cend, (const int32_t *)c->is_src.p,
+  hipLaunchKernelGGL(k_updown_modes, dim3(std::min<unsigned>(512, nblk(s5))), dim3(kT), 0, s, (const DevPlan *)dp, D,
+                     s5, (const int32_t *)c->cparent.p, cbeg, cend, (const int32_t *)c->is_src.p,
                      (const int32_t *)c->cnt_m2l.p, mode_m, mode_l, flags, terms);
   NB_LAUNCH();
   scan_excl_i32(flags, fscan, 5 * s5, c->scan_tmp, s);
@@ -753,76 +783,80 @@ void run_expansions(nbfmm_ctx *c) {
   int32_t *eslot = dget<int32_t>(c->eslot, (size_t)nc + 1);
   int32_t *lists = dget<int32_t>(c->up_cells, (size_t)3 * s5);
   int32_t *p2m_cells = lists, *m2m_cells = lists + s5, *l2l_cells = lists + 2 * s5;
-  hipLaunchKernelGGL(k_updown_lists, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)mode_l, (const int32_t *)flags,
-                     (const int32_t *)fscan, nc, mslot, lslot, eslot, p2m_cells, m2m_cells, l2l_cells);
+  hipLaunchKernelGGL(k_updown_lists, dim3(nblk(nc)), dim3(kT), 0, s, (const DevPlan *)dp, s5, (const int32_t *)mode_l,
+                     (const int32_t *)flags, (const int32_t *)fscan, mslot, lslot, eslot, p2m_cells, m2m_cells,
+                     l2l_cells);
   NB_LAUNCH();
   // totals and the per-level starts of the M2M and L2L lists (cells are level
-  // ordered), gathered on the device and read through the pinned mirror
-  int32_t *h = (int32_t *)c->hred.p;
+  // ordered) into the plan; a direct evaluation reads them with the term counts
   {
-    ReadList rl{};
-    for (int k = 0; k < 5; ++k) {
-      rl.off[k] = k * s5 + nc;
-      rl.base[k] = k * s5;
+    CapLevels cm{}, cl{};
+    for (int d = 0; d <= kMaxDepth; ++d) {
+      cm.v[d] = K.m2m_lev[d];
+      cl.v[d] = K.l2l_lev[d];
     }
-    for (int d = 0; d <= D + 1; ++d) {
-      rl.off[8 + d] = 2 * s5 + P.lvl_start[d];
-      rl.base[8 + d] = 2 * s5;
-      rl.off[32 + d] = 4 * s5 + P.lvl_start[d];
-      rl.base[32 + d] = 4 * s5;
-    }
-    int32_t *dreads = (int32_t *)((double *)c->red.p + 7 * 1024);
-    hipLaunchKernelGGL(k_gather_reads, dim3(1), dim3(64), 0, s, (const int32_t *)fscan, rl, 34 + D, dreads);
+    hipLaunchKernelGGL(k_up_reads, dim3(1), dim3(64), 0, s, dp, (const int32_t *)fscan, s5, D, K.on ? 1 : 0, K.nM,
+                       K.nP2M, K.nL, cm, cl);
     NB_LAUNCH();
-    NB_HIP(hipMemcpyAsync(h, dreads, sizeof(int32_t) * 64, hipMemcpyDeviceToHost, s));
   }
-  NB_HIP(hipStreamSynchronize(s));
-  const int32_t nM = h[0], nP2M = h[1], nL = h[3];
-  c->n_local = nL;
-  P.n_p2m_cells = nP2M;
-  P.n_m2m_cells = h[2];        // cells forming their multipole by M2M
-  P.n_l2l_cells = h[4];        // cells pushing their local by L2L
-  P.n_l2p_cells = h[3] - h[4]; // cells evaluating their local at their particles
+  const int32_t *h = c->hplan->reads;
+  if (!K.on) {
+    read_plan(c, &c->hplan->reads[0], offsetof(DevPlan, terms) + 2 * sizeof(unsigned long long) -
+                                          offsetof(DevPlan, reads), s);
+    c->n_local = h[3];
+    P.n_p2m_cells = h[1];
+    P.n_m2m_cells = h[2];        // cells forming their multipole by M2M
+    P.n_l2l_cells = h[4];        // cells pushing their local by L2L
+    P.n_l2p_cells = h[3] - h[4]; // cells evaluating their local at their particles
+    P.n_p2m_terms = (int64_t)c->hplan->terms[0];
+    P.n_l2p_terms = (int64_t)c->hplan->terms[1];
+  }
+  const int64_t nM = K.on ? K.nM : h[0], nP2M = K.on ? K.nP2M : h[1], nL = K.on ? K.nL : h[3];
+  // M2M / L2L parents of each level: exact or the capacity
+  int64_t m2m_lev[kMaxDepth + 1], l2l_lev[kMaxDepth + 1];
+  for (int d = 0; d < D; ++d) {
+    m2m_lev[d] = K.on ? K.m2m_lev[d] : h[9 + d] - h[8 + d];
+    l2l_lev[d] = K.on ? K.l2l_lev[d] : h[33 + d] - h[32 + d];
+  }
+  if (!K.on) {
+    c->last.nM = nM;
+    c->last.nP2M = nP2M;
+    c->last.nL = nL;
+    for (int d = 0; d < D; ++d) {
+      c->last.m2m_lev[d] = (int32_t)m2m_lev[d];
+      c->last.l2l_lev[d] = (int32_t)l2l_lev[d];
+    }
+  }
   // orders per cell (learned selection): the expansions are formed to the
   // highest order in use
   const int8_t *ord = nullptr;
   if (learned) {
-    ord = order_select(c, g);
+    ord = order_select(c, level_geo(P));
     p = c->ord_max;
     KP = padded_k(p);
     P.kp = KP;
     P.mean_order = p;
   }
   // ---- operators: M2L classes and the M2M / L2L octants (level independent) ----
-  const double rho = P.geo.rho(D);
-  const double hx = P.geo.h(D, 0) / rho, hy = P.geo.h(D, 1) / rho, hz = P.geo.h(D, 2) / rho;
   double *frag = dget<double>(c->tmat, (size_t)kNumClasses * KP * KP);
-  hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, hx, hy, hz, frag);
+  hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, (const DevPlan *)dp, frag);
   NB_LAUNCH();
   double *trans = dget<double>(c->trans, (size_t)16 * KP * KP);
-  hipLaunchKernelGGL(k_build_trans, dim3(8, 2), dim3(kT), 0, s, p, KP, hx, hy, hz, trans,
+  hipLaunchKernelGGL(k_build_trans, dim3(8, 2), dim3(kT), 0, s, p, KP, (const DevPlan *)dp, trans,
                      trans + (size_t)8 * KP * KP);
   NB_LAUNCH();
   if (f32) trans_to_f32(c, KP);
 
-  int32_t bm[kMaxDepth + 2], bl[kMaxDepth + 2];
-  for (int d = 0; d <= D + 1; ++d) {
-    bm[d] = h[8 + d];
-    bl[d] = h[32 + d];
-  }
-  // the term counts reach the host with the M2L class histogram (run_m2l),
-  // in a part of the pinned mirror that it leaves alone
-  unsigned long long *th = (unsigned long long *)((char *)c->hred.p + 32768);
-  NB_HIP(hipMemcpyAsync(th, terms, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   // ---- P2M (cells that form their multipole from particles) ----
   P.n_p2m_items = nP2M;
   void *Mexp = dbuf_ensure(c->mexp, (size_t)nM * KP * esz + 16);
-  NB_HIP(hipMemsetAsync(Mexp, 0, esz * (size_t)nM * KP, s));
+  fill_zero(s, Mexp, esz * (size_t)nM * KP);
   if (nP2M > 0) {
     const unsigned nb = (unsigned)((nP2M + kT / 64 - 1) / (kT / 64));
 #define NB_P2M(T, LP, PM)                                                                                       \
-  hipLaunchKernelGGL((k_p2m<T, LP, PM>), dim3(nb), dim3(kT), 0, s, p, KP, nP2M, (const int32_t *)p2m_cells,      \
-                     (const double4 *)c->pos.p, cbeg, cend, (const int32_t *)mslot, ckey, g, dC, (T *)Mexp)
+  hipLaunchKernelGGL((k_p2m<T, LP, PM>), dim3(nb), dim3(kT), 0, s, p, KP, (const DevPlan *)dp,                   \
+                     (const int32_t *)p2m_cells, (const double4 *)c->pos.p, cbeg, cend, (const int32_t *)mslot,   \
+                     ckey, dC, (T *)Mexp)
     if (f32) {
       if (p <= 8)
         NB_P2M(float, 8, 8);
@@ -846,12 +880,11 @@ void run_expansions(nbfmm_ctx *c) {
   // FP64 copies of the multipoles, rounded back to FP32 for M2L)
   const int lc = f32 ? std::min(D, c->prm.mixed_fp64_levels) : 0;
   mark(c, 14);
-  for (int d = D - 1; d >= lc; --d)
-    run_trans(c, false, KP, f32, bm[d + 1] - bm[d], m2m_cells + bm[d]);
+  for (int d = D - 1; d >= lc; --d) run_trans(c, false, KP, f32, m2m_cells, &dp->reads[8 + d], m2m_lev[d]);
   if (lc > 0) {
     double *M64 = dget<double>(c->mexp64, (size_t)nM * KP);
     widen(c, (const float *)Mexp, (int64_t)nM * KP, M64);
-    for (int d = lc - 1; d >= 0; --d) run_trans64(c, false, KP, bm[d + 1] - bm[d], m2m_cells + bm[d], M64);
+    for (int d = lc - 1; d >= 0; --d) run_trans64(c, false, KP, m2m_cells, &dp->reads[8 + d], m2m_lev[d], M64);
     narrow(c, M64, (int64_t)nM * KP, (float *)Mexp);
   }
   mark(c, 3);
@@ -862,30 +895,29 @@ void run_expansions(nbfmm_ctx *c) {
   // run beside P2P instead of after it
   c->p2p_started = false;
   dbuf_ensure(c->lexp, (size_t)nL * KP * esz + 16);
-  NB_HIP(hipMemsetAsync(c->lexp.p, 0, esz * (size_t)nL * KP, s));
+  fill_zero(s, c->lexp.p, esz * (size_t)nL * KP);
   run_m2l(c, KP, f32, ord, p);
   if (!c->p2p_started) launch_p2p(c, 1);
-  P.n_p2m_terms = (int64_t)th[0]; // read back at run_m2l's synchronisation
-  P.n_l2p_terms = (int64_t)th[1];
   mark(c, 13);
   // ---- L2L, coarsest level first (parents of levels < lc in FP64 in mixed precision) ----
   if (lc > 0) {
     double *L64 = dget<double>(c->lexp64, (size_t)nL * KP);
     widen(c, (const float *)c->lexp.p, (int64_t)nL * KP, L64);
-    for (int d = 0; d < lc; ++d) run_trans64(c, true, KP, bl[d + 1] - bl[d], l2l_cells + bl[d], L64);
+    for (int d = 0; d < lc; ++d) run_trans64(c, true, KP, l2l_cells, &dp->reads[32 + d], l2l_lev[d], L64);
     narrow(c, L64, (int64_t)nL * KP, (float *)c->lexp.p);
   }
-  for (int d = lc; d < D; ++d) run_trans(c, true, KP, f32, bl[d + 1] - bl[d], l2l_cells + bl[d]);
+  for (int d = lc; d < D; ++d) run_trans(c, true, KP, f32, l2l_cells, &dp->reads[32 + d], l2l_lev[d]);
   mark(c, 4);
   // ---- L2P ----
   if (f32)
     hipLaunchKernelGGL((pot ? k_l2p<float, true> : k_l2p<float, false>), dim3(nblk(P.n)), dim3(kT), 0, s, p, KP, P.n,
                        (const double4 *)c->pos.p, (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p, ckey,
-                       (const int32_t *)eslot, g, dC, (const float *)c->lexp.p, grad, pfar, ord);
+                       (const int32_t *)eslot, (const DevPlan *)dp, dC, (const float *)c->lexp.p, grad, pfar, ord);
   else
     hipLaunchKernelGGL((pot ? k_l2p<double, true> : k_l2p<double, false>), dim3(nblk(P.n)), dim3(kT), 0, s, p, KP,
                        P.n, (const double4 *)c->pos.p, (const int32_t *)c->pcell.p, (const int32_t *)c->cparent.p,
-                       ckey, (const int32_t *)eslot, g, dC, (const double *)c->lexp.p, grad, pfar, ord);
+                       ckey, (const int32_t *)eslot, (const DevPlan *)dp, dC, (const double *)c->lexp.p, grad, pfar,
+                       ord);
   NB_LAUNCH();
   mark(c, 5);
 }

@@ -10,6 +10,8 @@
 //   each component by the scalar: tensor.cpp:226-232.)
 #include "nbfmm_internal.h"
 
+#include <algorithm>
+
 // HIP contracts a*b+c into FMA by default (and __dmul_rn is a plain multiply);
 // the reference's x86-64 build rounds every operation, so contraction is off.
 #pragma clang fp contract(off)
@@ -18,7 +20,10 @@ namespace nbfmm {
 namespace {
 constexpr int kT = 256;
 
-__global__ void k_se(int64_t m, double *x, double *v, const double *a, double dt) {
+// The updates of the state (x, v, a) leave it unchanged when the planned
+// evaluations of the step overflowed (the step is then run again directly).
+__global__ void k_se(int64_t m, double *x, double *v, const double *a, double dt, const DevPlan *dp) {
+  if (plan_dead(dp)) return;
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= m) return;
   double vi = __dadd_rn(v[i], __dmul_rn(a[i], dt));
@@ -52,14 +57,22 @@ __global__ void k_rk4_accum(int64_t m, double *asum, double *vsum, const double 
   }
 }
 
-__global__ void k_rk4_final(int64_t m, double *x, double *v, const double *asum, const double *vsum, double h6) {
+__global__ void k_rk4_final(int64_t m, double *x, double *v, const double *asum, const double *vsum, double h6,
+                            const DevPlan *dp) {
+  if (plan_dead(dp)) return;
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= m) return;
   v[i] = __dadd_rn(v[i], __dmul_rn(asum[i], h6));
   x[i] = __dadd_rn(x[i], __dmul_rn(vsum[i], h6));
 }
 
-inline unsigned nb(int64_t m) { return (unsigned)((m + kT - 1) / kT); }
+__global__ void k_copy(int64_t m, double *dst, const double *src, const DevPlan *dp) {
+  if (plan_dead(dp)) return;
+  int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < m) dst[i] = src[i];
+}
+
+inline unsigned nb(int64_t m) { return (unsigned)std::max<int64_t>(1, (m + kT - 1) / kT); }
 // ---- diagnostics --------------------------------------------------------------------------
 // Sums of a particle state in FP64, in a fixed order (strided per thread, tree
 // per block, blocks in order): 0 kinetic energy 1/2 m v^2, 1 potential energy
@@ -105,7 +118,7 @@ __global__ void k_diag_final(int nb, const double *__restrict__ part, double *__
 } // namespace
 
 void se_update(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *a, double dt) {
-  hipLaunchKernelGGL(k_se, dim3(nb(3 * n)), dim3(kT), 0, c->stream, 3 * n, x, v, a, dt);
+  hipLaunchKernelGGL(k_se, dim3(nb(3 * n)), dim3(kT), 0, c->stream, 3 * n, x, v, a, dt, (const DevPlan *)dplan(c));
   NB_LAUNCH();
 }
 void rk4_stage(nbfmm_ctx *c, int64_t n, const double *x, const double *v, const double *vk_prev,
@@ -119,11 +132,13 @@ void rk4_accum(nbfmm_ctx *c, int64_t n, double *asum, double *vsum, const double
   NB_LAUNCH();
 }
 void rk4_final(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *asum, const double *vsum, double h6) {
-  hipLaunchKernelGGL(k_rk4_final, dim3(nb(3 * n)), dim3(kT), 0, c->stream, 3 * n, x, v, asum, vsum, h6);
+  hipLaunchKernelGGL(k_rk4_final, dim3(nb(3 * n)), dim3(kT), 0, c->stream, 3 * n, x, v, asum, vsum, h6,
+                     (const DevPlan *)dplan(c));
   NB_LAUNCH();
 }
 void copy_d(nbfmm_ctx *c, double *dst, const double *src, int64_t count) {
-  NB_HIP(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)count, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_copy, dim3(nb(count)), dim3(kT), 0, c->stream, count, dst, src, (const DevPlan *)dplan(c));
+  NB_LAUNCH();
 }
 
 void diagnostics(nbfmm_ctx *c, int64_t n, const double *x, const double *v, const double *m, const double *phi,

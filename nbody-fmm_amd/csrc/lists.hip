@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 
 namespace nbfmm {
 
@@ -29,63 +30,8 @@ namespace nbfmm {
 // p2p_cut (exact; for bounding boxes within 1.15:1 no class is affected),
 // as long as n_t * n_s < kBadDirect: larger pairs stay M2L, since evaluating
 // them directly grows as N^2 (the cubic_box option avoids such classes).
-ClsMask class_mask(const Geometry &geo) {
-  ClsMask m{};
-  const double hx = geo.h(0, 0), hy = geo.h(0, 1), hz = geo.h(0, 2);
-  const double diag2 = hx * hx + hy * hy + hz * hz;
-  for (int k = 0; k < kNumClasses; ++k) {
-    const int dx = k % 7 - 3, dy = (k / 7) % 7 - 3, dz = k / 49 - 3;
-    if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) < 2) continue;
-    const double d2 = dx * dx * hx * hx + dy * dy * hy * hy + dz * dz * hz * hz;
-    if (diag2 > kMacRatio * kMacRatio * d2) m.w[k >> 5] |= 1u << (k & 31);
-  }
-  return m;
-}
-
-// Distance-dependent orders.  The M2L of class d truncated after degree q
-// has the worst-case bound rate_d^(q+1) / |d|^2 (relative to the field of
-// the pair's source at unit distance), rate_d = (cell diagonal) / |d|; for
-// particles spread over their cells the error falls per degree by about
-// kOrderAlpha * rate_d instead (measured for cubes: 0.50 for the face class
-// (2,0,0) of rate 0.87, 0.15 for (3,3,3) of rate 0.33).  A class gets the
-// lowest q in [kOrderLo, p] with
-//   (a rate_d)^(q+1) / |d|^2 <= tol (a rate_0)^(p+1) / |d_0|^2,
-// class 0 being the slowest M2L class (rates and |d| the same for every
-// level: cells keep the box's aspect ratio); classes evaluated directly or
-// not admissible, and class 0, keep p.  The oracle computes the same table
-// with the same operations (orc_exact.c).
-ClsOrd class_orders(const Geometry &geo, int p, double tol) {
-  ClsOrd o{};
-  const double hx = geo.h(0, 0), hy = geo.h(0, 1), hz = geo.h(0, 2);
-  const double diag2 = hx * hx + hy * hy + hz * hz;
-  double r0 = 0, d0 = 0; // largest squared rate of an admissible class, its |d|^2
-  double r2[kNumClasses], dd[kNumClasses];
-  for (int k = 0; k < kNumClasses; ++k) {
-    const int dx = k % 7 - 3, dy = (k / 7) % 7 - 3, dz = k / 49 - 3;
-    r2[k] = -1;
-    if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) < 2) continue;
-    const double d2 = dx * dx * hx * hx + dy * dy * hy * hy + dz * dz * hz * hz;
-    if (diag2 > kMacRatio * kMacRatio * d2) continue;
-    r2[k] = diag2 / d2;
-    dd[k] = d2;
-    if (r2[k] > r0) {
-      r0 = r2[k];
-      d0 = d2;
-    }
-  }
-  const double la = std::log(kOrderAlpha);
-  for (int k = 0; k < kNumClasses; ++k) {
-    int q = p;
-    if (tol > 0 && r2[k] > 0 && r2[k] < r0) {
-      const double need = (std::log(tol) + (p + 1) * (la + 0.5 * std::log(r0)) + std::log(dd[k] / d0)) /
-                          (la + 0.5 * std::log(r2[k]));
-      q = (int)std::ceil(need) - 1;
-      q = std::min(p, std::max(kOrderLo, q));
-    }
-    o.q[k] = (uint8_t)q;
-  }
-  return o;
-}
+// The mask of these classes is built on the device with the tree
+// (k_tree_plan, tree.hip) and read from the plan here.
 
 namespace {
 
@@ -105,13 +51,31 @@ constexpr int64_t kTaskPairs = 1 << 17;
 // counts stay 0.  skip_from < 0 classifies every cell.
 // One thread per parent: does it have a child to classify (the skip rule of
 // k_lists)?  Those parents are appended to active[] (order free; one counter
-// update per wavefront, the count at active[nparents + 1]).
-__global__ void __launch_bounds__(kT) k_list_filter(int32_t pbeg, int32_t nparents, int32_t skip_from,
+// update per wavefront, the count at *nactive).
+// The range of parents and the skip level come from the plan: parents of
+// levels [lc - 1, D) (cells [pbeg, nparents)), cells of level >= lc may be
+// skipped (skip_from = their first cell, or -1 when every cell is classified).
+struct ParentRange {
+  int32_t pbeg, nparents, skip_from;
+};
+__device__ __forceinline__ ParentRange parent_range(const DevPlan *dp, int lc, int D, int keep) {
+  ParentRange r;
+  r.pbeg = dp->lvl_start[min(lc - 1, D)];
+  r.nparents = dp->lvl_start[D];
+  r.skip_from = keep ? -1 : dp->lvl_start[lc];
+  return r;
+}
+
+__global__ void __launch_bounds__(kT) k_list_filter(const DevPlan *__restrict__ dp, int lc, int D, int keep,
                                                     const int32_t *__restrict__ nbr27,
                                                     const int32_t *__restrict__ child8,
                                                     const int32_t *__restrict__ cbeg,
-                                                    const int32_t *__restrict__ cend, int64_t cut, int32_t *active) {
-  const int32_t P = pbeg + (int32_t)(blockIdx.x * kT + threadIdx.x);
+                                                    const int32_t *__restrict__ cend, int64_t cut, int32_t *active,
+                                                    int32_t *nactive) {
+  if (plan_dead(dp)) return;
+  const ParentRange pr = parent_range(dp, lc, D, keep);
+  const int32_t nparents = pr.nparents, skip_from = pr.skip_from;
+  const int32_t P = pr.pbeg + (int32_t)(blockIdx.x * kT + threadIdx.x);
   bool todo = false;
   if (P < nparents) {
     int64_t maxn = 0;
@@ -134,27 +98,32 @@ __global__ void __launch_bounds__(kT) k_list_filter(int32_t pbeg, int32_t nparen
   const uint64_t m = __ballot(todo);
   const int lane = threadIdx.x & 63;
   int32_t base = 0;
-  if (lane == 0 && m) base = atomicAdd(&active[nparents + 1], (int32_t)__popcll(m));
+  if (lane == 0 && m) base = atomicAdd(nactive, (int32_t)__popcll(m));
   base = __shfl(base, 0);
   if (todo) active[base + __popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = P;
 }
 
 template <bool FILL>
-__global__ void __launch_bounds__(kT) k_lists(int32_t pbeg, int32_t nparents, int32_t skip_from,
+__global__ void __launch_bounds__(kT) k_lists(const DevPlan *__restrict__ dp, int lc, int D, int keep,
                                               const int32_t *__restrict__ nbr27,
                                               const int32_t *__restrict__ child8, const int32_t *__restrict__ cbeg,
-                                              const int32_t *__restrict__ cend, int64_t cut, ClsMask bad,
+                                              const int32_t *__restrict__ cend, int64_t cut,
                                               int32_t *cnt_il,
                                               int32_t *cnt_m2l, int32_t *cnt_sub, int64_t *sub_pairs,
                                               const int32_t *off_il, const int32_t *off_m2l, const int32_t *off_sub,
                                               int32_t *il, int32_t *m2l_src, uint16_t *m2l_cls, int32_t *sub_src,
                                               int32_t *is_src, const int32_t *active) {
+  if (plan_dead(dp)) return;
   __shared__ int32_t cand_s[kT / kW][kWin];
   __shared__ int32_t csz_s[kT / kW][kWin];
+  __shared__ ClsMask bad;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // both passes take the parents k_list_filter listed (their count at active[nparents + 1])
+  if (threadIdx.x < (kNumClasses + 31) / 32) bad.w[threadIdx.x] = dp->bad_cls.w[threadIdx.x];
+  const ParentRange pr = parent_range(dp, lc, D, keep);
+  const int32_t nparents = pr.nparents, skip_from = pr.skip_from;
+  // both passes take the parents k_list_filter listed (their count at dp->nactive)
   const int32_t pw = (int32_t)blockIdx.x * (kT / kW) + wave;
-  const int32_t P = pw < active[nparents + 1] ? active[pw] : nparents;
+  const int32_t P = pw < dp->nactive ? active[pw] : nparents;
   const bool live = P < nparents;
   int32_t *cand = cand_s[wave], *csz = csz_s[wave];
   int64_t maxn = 0;
@@ -323,10 +292,12 @@ __global__ void k_sum_i64(const int64_t *v, int64_t n, unsigned long long *out) 
 // substituted list.  Targets are cut into groups of kTaskTargets, the
 // concatenated sources into whole 64-particle chunks of about kTaskPairs
 // pair interactions per task.
-__global__ void k_closed(int32_t cb, int32_t ce, int32_t leaf_begin, const int32_t *__restrict__ cnt_m2l,
+__global__ void k_closed(const DevPlan *__restrict__ dp, int lev, int D, const int32_t *__restrict__ cnt_m2l,
                          const int32_t *__restrict__ child8, int32_t *closed) {
-  const int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
+  if (plan_dead(dp)) return;
+  const int32_t leaf_begin = dp->lvl_start[D];
+  const int32_t c = dp->lvl_start[lev] + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= dp->lvl_start[lev + 1]) return;
   bool ok = cnt_m2l[c] == 0;
   if (ok && c < leaf_begin)
     for (int o = 0; o < 8; ++o) {
@@ -396,12 +367,15 @@ __device__ __forceinline__ int32_t seg_len(int64_t ng, int64_t np) {
 // Also the particle pairs each cell's tasks evaluate (a 27-neighbour row
 // contains the target cell itself, whose particles skip themselves).  The
 // counts of cells of at most narrow_max particles go to cnt[c], the others
-// to cnt[nc + 1 + c], so one scan over both halves numbers the narrow tasks
-// first and the wide ones after them.
-__global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
-                             const int32_t *closed, const int32_t *cbeg, const int32_t *cend, const int32_t *cnt_sub,
-                             const int32_t *off_sub, const int64_t *sub_pairs, int narrow_max, int32_t *cnt,
-                             int64_t *pairs) {
+// to cnt[S + c] (S >= nc + 1; the entries past the cells stay zero), so one
+// scan over both halves numbers the narrow tasks first and the wide ones
+// after them.
+__global__ void k_task_count(const DevPlan *__restrict__ dp, int D, int64_t S, const int32_t *cparent,
+                             const int32_t *nbr27, const int32_t *closed, const int32_t *cbeg, const int32_t *cend,
+                             const int32_t *cnt_sub, const int32_t *off_sub, const int64_t *sub_pairs, int narrow_max,
+                             int32_t *cnt, int64_t *pairs) {
+  if (plan_dead(dp)) return;
+  const int32_t nc = dp->nc, leaf_begin = dp->lvl_start[D];
   const int32_t c = 1 + (int32_t)(blockIdx.x * kT + threadIdx.x);
   if (c >= nc) return;
   const SrcGroups g = src_groups(c, leaf_begin, cparent, nbr27, closed, cbeg, cend, cnt_sub, off_sub, sub_pairs);
@@ -417,20 +391,22 @@ __global__ void k_task_count(int32_t nc, int32_t leaf_begin, const int32_t *cpar
   }
   const bool wide = nt > narrow_max;
   cnt[c] = wide ? 0 : k * ntg;
-  cnt[nc + 1 + c] = wide ? k * ntg : 0;
+  cnt[S + c] = wide ? k * ntg : 0;
   pairs[c] = np;
 }
 
-__global__ void k_task_fill(int32_t nc, int32_t leaf_begin, const int32_t *cparent, const int32_t *nbr27,
-                            const int32_t *closed, const int32_t *cbeg, const int32_t *cend, const int32_t *cnt_sub,
-                            const int32_t *off_sub, const int64_t *sub_pairs, int narrow_max, const int32_t *toff,
-                            P2PTask *tasks) {
+__global__ void k_task_fill(const DevPlan *__restrict__ dp, int D, int64_t S, const int32_t *cparent,
+                            const int32_t *nbr27, const int32_t *closed, const int32_t *cbeg, const int32_t *cend,
+                            const int32_t *cnt_sub, const int32_t *off_sub, const int64_t *sub_pairs, int narrow_max,
+                            const int32_t *toff, P2PTask *tasks) {
+  if (plan_dead(dp)) return;
+  const int32_t nc = dp->nc, leaf_begin = dp->lvl_start[D];
   const int32_t c = 1 + (int32_t)(blockIdx.x * kT + threadIdx.x);
   if (c >= nc) return;
   const SrcGroups g = src_groups(c, leaf_begin, cparent, nbr27, closed, cbeg, cend, cnt_sub, off_sub, sub_pairs);
   const int32_t nt = cend[c] - cbeg[c];
   const int32_t ntg = (nt + kTaskTargets - 1) / kTaskTargets;
-  int32_t w = nt > narrow_max ? toff[nc + 1 + c] : toff[c];
+  int32_t w = nt > narrow_max ? toff[S + c] : toff[c];
   for (int32_t tg = 0; tg < ntg; ++tg)
     for (int q = 0; q < g.n; ++q) {
       if (g.np[q] <= 0) continue;
@@ -452,22 +428,41 @@ inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT 
 
 } // namespace
 
+// Planned evaluations: sizes beyond the capacities (k_list_check after the
+// list counts, k_task_totals after the task counts) set the plan's ovf.
+__global__ void k_list_check(DevPlan *dp, int64_t cap_m2l, int64_t cap_sub) {
+  if (threadIdx.x == 0 && (dp->tot[1] > (unsigned long long)cap_m2l || dp->tot[2] > (unsigned long long)cap_sub))
+    dp->ovf |= 2;
+}
+// the narrow task count (the scan at the start of the wide half) into the
+// plan; with check, the task and target capacities
+__global__ void k_task_totals(DevPlan *dp, const int32_t *toff, int64_t S, int check, int64_t cap_task,
+                              int64_t cap_tgt) {
+  if (threadIdx.x != 0) return;
+  dp->nnarrow = toff[S];
+  if (check && (dp->ntask > (unsigned long long)cap_task || dp->nst[1] > cap_tgt)) dp->ovf |= 2;
+}
+
 void build_lists(nbfmm_ctx *c) {
   Plan &P = c->plan;
   hipStream_t s = c->stream;
+  const Caps &K = c->cap;
+  DevPlan *dp = dplan(c);
   const int D = P.geo.depth;
-  const int64_t nc = P.n_cells;
+  // cell arrays hold nc cells (direct) or the capacity (planned); S >= nc + 1
+  // is the stride of the arrays of two or five cell-indexed halves
+  const int64_t nc = K.on ? K.nc : P.n_cells;
+  const int64_t S = nc + 1;
   const int32_t *cbeg = (const int32_t *)c->cbeg.p, *cend = (const int32_t *)c->cend.p;
   const int32_t *child8 = (const int32_t *)c->child8.p, *nbr27 = (const int32_t *)c->nbr27.p;
   const int64_t cut = c->prm.p2p_cut;
-  const int32_t leaf_begin = P.lvl_start[D];
-  int32_t *cnt_il = dget<int32_t>(c->cnt_il, (size_t)nc + 1);
-  int32_t *cnt_m2l = dget<int32_t>(c->cnt_m2l, (size_t)nc + 1);
-  int32_t *cnt_sub = dget<int32_t>(c->cnt_sub, (size_t)nc + 1);
-  int32_t *off_il = dget<int32_t>(c->off_il, (size_t)nc + 1);
-  int32_t *off_m2l = dget<int32_t>(c->off_m2l, (size_t)nc + 1);
-  int32_t *off_sub = dget<int32_t>(c->off_sub, (size_t)nc + 1);
-  int64_t *sub_pairs = dget<int64_t>(c->sub_pairs, (size_t)nc + 1);
+  int32_t *cnt_il = dget<int32_t>(c->cnt_il, (size_t)S);
+  int32_t *cnt_m2l = dget<int32_t>(c->cnt_m2l, (size_t)S);
+  int32_t *cnt_sub = dget<int32_t>(c->cnt_sub, (size_t)S);
+  int32_t *off_il = dget<int32_t>(c->off_il, (size_t)S);
+  int32_t *off_m2l = dget<int32_t>(c->off_m2l, (size_t)S);
+  int32_t *off_sub = dget<int32_t>(c->off_sub, (size_t)S);
+  int64_t *sub_pairs = dget<int64_t>(c->sub_pairs, (size_t)S);
   // Closed cells (below) merge the near field of their whole subtree into one
   // P2P task; cells of level >= lc may be merged.  Cells of levels below lc
   // have no lists (lc = 1 for one system; below the system cells for a batch).
@@ -475,134 +470,146 @@ void build_lists(nbfmm_ctx *c) {
   if (lc > D)
     throw Error(NBFMM_ERR_UNSUPPORTED, "batched systems need a tree deeper than the system cells (more particles per "
                                        "system than leaf_capacity)");
-  const int32_t skip_from = c->keep_lists ? -1 : P.lvl_start[lc];
+  if (K.on && c->keep_lists) throw Error(NBFMM_ERR_UNSUPPORTED, "planned evaluations keep no lists");
+  const int keep = c->keep_lists ? 1 : 0;
   // cells that are not classified (the root, skipped children) have empty lists
-  NB_HIP(hipMemsetAsync(cnt_il, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  NB_HIP(hipMemsetAsync(cnt_m2l, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  NB_HIP(hipMemsetAsync(cnt_sub, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  NB_HIP(hipMemsetAsync(sub_pairs, 0, sizeof(int64_t) * (size_t)(nc + 1), s));
-  const int32_t nparents = leaf_begin; // every non-leaf cell is a parent
-  const int32_t pbeg = P.lvl_start[std::min(lc - 1, D)]; // parents of the cells of level lc on
-  const ClsMask bad = class_mask(P.geo);
-  const unsigned nlb = (unsigned)std::max<int32_t>(1, (nparents - pbeg + kT / kW - 1) / (kT / kW));
-  // parents with a child to classify, then their count at [nparents + 1]
-  int32_t *active = dget<int32_t>(c->act_par, (size_t)nparents + 2);
-  NB_HIP(hipMemsetAsync(active + nparents + 1, 0, sizeof(int32_t), s));
-  hipLaunchKernelGGL(k_list_filter, dim3(nblk(std::max<int32_t>(0, nparents - pbeg))), dim3(kT), 0, s, pbeg, nparents,
-                     skip_from, nbr27, child8, cbeg, cend, cut, active);
+  fill_i32(s, cnt_il, S, 0);
+  fill_i32(s, cnt_m2l, S, 0);
+  fill_i32(s, cnt_sub, S, 0);
+  fill_zero(s, sub_pairs, sizeof(int64_t) * (size_t)S);
+  // every non-leaf cell is a parent: the grids cover all of them, the kernels
+  // take those of levels [lc - 1, D) from the plan
+  int64_t npar = 0;
+  if (K.on)
+    npar = K.parents;
+  else
+    npar = P.lvl_start[D] - P.lvl_start[std::min(lc - 1, D)];
+  const unsigned nlb = (unsigned)std::max<int64_t>(1, (npar + kT / kW - 1) / (kT / kW));
+  // parents with a child to classify, their count in the plan
+  int32_t *active = dget<int32_t>(c->act_par, (size_t)npar + 2);
+  fill_i32(s, &dp->nactive, 1, 0);
+  hipLaunchKernelGGL(k_list_filter, dim3(nblk(npar)), dim3(kT), 0, s, (const DevPlan *)dp, lc, D, keep, nbr27, child8,
+                     cbeg, cend, cut, active, &dp->nactive);
   NB_LAUNCH();
   // grid for every parent; wavefronts past the listed count exit
-  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend,
-                     cut, bad, cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr,
+  hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, (const DevPlan *)dp, lc, D, keep, nbr27, child8, cbeg,
+                     cend, cut, cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr,
                      nullptr, nullptr, nullptr, (const int32_t *)active);
   NB_LAUNCH();
-  scan_excl_i32(cnt_il, off_il, nc + 1, c->scan_tmp, s);
-  scan_excl_i32(cnt_m2l, off_m2l, nc + 1, c->scan_tmp, s);
-  scan_excl_i32(cnt_sub, off_sub, nc + 1, c->scan_tmp, s);
+  scan_excl_i32(cnt_il, off_il, S, c->scan_tmp, s);
+  scan_excl_i32(cnt_m2l, off_m2l, S, c->scan_tmp, s);
+  scan_excl_i32(cnt_sub, off_sub, S, c->scan_tmp, s);
   // list lengths in 64 bits: the offsets above are 32-bit
-  unsigned long long *tot = (unsigned long long *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 52;
-  NB_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 3, s));
-  const unsigned nsb = std::min<unsigned>(512, nblk(nc));
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_il, nc, tot);
+  unsigned long long *tot = dp->tot;
+  fill_zero(s, tot, sizeof(unsigned long long) * 3);
+  const unsigned nsb = std::min<unsigned>(512, nblk(S));
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_il, S, tot);
   NB_LAUNCH();
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_m2l, nc, tot + 1);
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_m2l, S, tot + 1);
   NB_LAUNCH();
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_sub, nc, tot + 2);
+  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_sub, S, tot + 2);
   NB_LAUNCH();
-  // host reads through the pinned mirror
-  unsigned long long *htot = (unsigned long long *)c->hred.p;
-  int32_t *hint = (int32_t *)(htot + 8);
-  NB_HIP(hipMemcpyAsync(htot, tot, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(hint, active + nparents + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  const int32_t nactive = hint[0];
   const unsigned long long lim = 0x7fffffffull - 1;
-  if (htot[1] > lim || htot[2] > lim || (c->keep_lists && htot[0] > lim))
-    throw Error(NBFMM_ERR_UNSUPPORTED, "interaction lists exceed 2^31 entries (raise leaf_capacity or p2p_cut)");
-  P.n_il = (int64_t)htot[0];
-  P.n_m2l = (int64_t)htot[1];
-  P.n_sub = (int64_t)htot[2];
+  // the host reads the counts (direct) or the plan checks them (planned)
+  const DevPlan *hp = c->hplan;
+  auto read_counts = [&]() {
+    read_plan(c, &c->hplan->nactive, offsetof(DevPlan, npairs) + sizeof(unsigned long long) - offsetof(DevPlan, nactive),
+              s);
+  };
+  int32_t nactive = 0;
+  if (K.on) {
+    hipLaunchKernelGGL(k_list_check, dim3(1), dim3(64), 0, s, dp, K.n_m2l, K.n_sub);
+    NB_LAUNCH();
+  } else {
+    read_counts();
+    nactive = hp->nactive;
+    if (hp->tot[1] > lim || hp->tot[2] > lim || (c->keep_lists && hp->tot[0] > lim))
+      throw Error(NBFMM_ERR_UNSUPPORTED, "interaction lists exceed 2^31 entries (raise leaf_capacity or p2p_cut)");
+    P.n_il = (int64_t)hp->tot[0];
+    P.n_m2l = (int64_t)hp->tot[1];
+    P.n_sub = (int64_t)hp->tot[2];
+  }
+  const int64_t n_m2l = K.on ? K.n_m2l : P.n_m2l, n_sub = K.on ? K.n_sub : P.n_sub;
   int32_t *il = c->keep_lists ? dget<int32_t>(c->il, (size_t)P.n_il + 1) : nullptr;
-  int32_t *m2l_src = dget<int32_t>(c->m2l_src, (size_t)P.n_m2l + 1);
-  uint16_t *m2l_cls = dget<uint16_t>(c->m2l_cls, (size_t)P.n_m2l + 1);
-  int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)P.n_sub + 1);
-  int32_t *is_src = dget<int32_t>(c->is_src, (size_t)nc + 1);
-  NB_HIP(hipMemsetAsync(is_src, 0, sizeof(int32_t) * (size_t)(nc + 1), s));
-  if (nactive > 0) {
-    const unsigned nfb = (unsigned)((nactive + kT / kW - 1) / (kT / kW));
-    hipLaunchKernelGGL(k_lists<true>, dim3(nfb), dim3(kT), 0, s, pbeg, nparents, skip_from, nbr27, child8, cbeg, cend,
-                       cut, bad, nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il,
+  int32_t *m2l_src = dget<int32_t>(c->m2l_src, (size_t)n_m2l + 1);
+  uint16_t *m2l_cls = dget<uint16_t>(c->m2l_cls, (size_t)n_m2l + 1);
+  int32_t *sub_src = dget<int32_t>(c->sub_src, (size_t)n_sub + 1);
+  int32_t *is_src = dget<int32_t>(c->is_src, (size_t)S);
+  fill_i32(s, is_src, S, 0);
+  const int64_t nfill = K.on ? npar : nactive;
+  if (nfill > 0) {
+    const unsigned nfb = (unsigned)((nfill + kT / kW - 1) / (kT / kW));
+    hipLaunchKernelGGL(k_lists<true>, dim3(nfb), dim3(kT), 0, s, (const DevPlan *)dp, lc, D, keep, nbr27, child8, cbeg,
+                       cend, cut, nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il,
                        m2l_src, m2l_cls, sub_src, is_src, (const int32_t *)active);
     NB_LAUNCH();
   }
 
   // ---- M2L source / target compaction ----
-  const size_t fl = (size_t)std::max<int64_t>(2 * nc, P.n) + 1;
+  const size_t fl = (size_t)std::max<int64_t>(2 * S, P.n) + 1;
   int32_t *flags = dget<int32_t>(c->flags, fl);
   int32_t *fscan = dget<int32_t>(c->fscan, fl);
-  int32_t *src_slot = dget<int32_t>(c->src_slot, (size_t)nc);
-  int32_t *tgt_slot = dget<int32_t>(c->tgt_slot, (size_t)nc);
-  int32_t *src_cells = dget<int32_t>(c->src_cells, (size_t)nc);
-  int32_t *tgt_cells = dget<int32_t>(c->tgt_cells, (size_t)nc);
-  // sources: cells in some M2L list; targets: cells with an M2L list.  Their
-  // counts reach the host with the task counts below.
-  int32_t *nst_d = (int32_t *)(tot + 3);
-  hipLaunchKernelGGL(k_flag_pos2, dim3(nblk(nc)), dim3(kT), 0, s, (const int32_t *)is_src, (const int32_t *)cnt_m2l,
-                     nc, flags);
+  int32_t *src_slot = dget<int32_t>(c->src_slot, (size_t)S);
+  int32_t *tgt_slot = dget<int32_t>(c->tgt_slot, (size_t)S);
+  int32_t *src_cells = dget<int32_t>(c->src_cells, (size_t)S);
+  int32_t *tgt_cells = dget<int32_t>(c->tgt_cells, (size_t)S);
+  // sources: cells in some M2L list; targets: cells with an M2L list (the
+  // entries past the cells are zero).  Their counts reach the plan.
+  hipLaunchKernelGGL(k_flag_pos2, dim3(nblk(S)), dim3(kT), 0, s, (const int32_t *)is_src, (const int32_t *)cnt_m2l, S,
+                     flags);
   NB_LAUNCH();
-  scan_excl_i32(flags, fscan, 2 * nc, c->scan_tmp, s);
-  hipLaunchKernelGGL(k_compact2, dim3(nblk(nc)), dim3(kT), 0, s, flags, fscan, nc, src_slot, src_cells, tgt_slot,
-                     tgt_cells, nst_d);
+  scan_excl_i32(flags, fscan, 2 * S, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_compact2, dim3(nblk(S)), dim3(kT), 0, s, flags, fscan, S, src_slot, src_cells, tgt_slot,
+                     tgt_cells, dp->nst);
   NB_LAUNCH();
 
   // ---- P2P sub-tasks ----
-  int32_t *closed = dget<int32_t>(c->closed, (size_t)nc + 1);
-  NB_HIP(hipMemsetAsync(closed, 0, sizeof(int32_t) * (size_t)P.lvl_start[lc], s)); // never task cells
+  int32_t *closed = dget<int32_t>(c->closed, (size_t)S);
+  fill_i32(s, closed, S, 0); // cells of levels < lc are never task cells
   for (int d = D; d >= lc; --d) {
-    const int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_closed, dim3(nblk(ce - cb)), dim3(kT), 0, s, cb, ce, leaf_begin, (const int32_t *)cnt_m2l,
-                       child8, closed);
+    hipLaunchKernelGGL(k_closed, dim3(nblk(level_cells(c, d))), dim3(kT), 0, s, (const DevPlan *)dp, d, D,
+                       (const int32_t *)cnt_m2l, child8, closed);
     NB_LAUNCH();
   }
   nbr_leaf_rows(c, closed); // the P2P work list reads the rows of the leaves that are not closed
   const int32_t *cpar = (const int32_t *)c->cparent.p;
-  // task counts of narrow cells, then of wide cells (2 (nc + 1) entries)
+  // task counts of narrow cells, then of wide cells (two halves of S entries)
   const int narrow_max = c->prm.precision == NBFMM_MIXED ? c->shapes.p2p_narrow_max_f32 : c->shapes.p2p_narrow_max_f64;
-  int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)2 * (nc + 1));
-  int32_t *toff = dget<int32_t>(c->task_off, (size_t)2 * (nc + 1));
-  NB_HIP(hipMemsetAsync(tcnt, 0, sizeof(int32_t) * 2 * (size_t)(nc + 1), s));
-  int64_t *cell_pairs = dget<int64_t>(c->near_pairs, (size_t)nc + 1);
-  NB_HIP(hipMemsetAsync(cell_pairs, 0, sizeof(int64_t), s));
-  unsigned long long *pairs = (unsigned long long *)dget<double>(c->red, 8 * 1024 + 64) + 6 * 1024 + 40;
-  NB_HIP(hipMemsetAsync(pairs, 0, sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
+  int32_t *tcnt = dget<int32_t>(c->task_cnt, (size_t)2 * S);
+  int32_t *toff = dget<int32_t>(c->task_off, (size_t)2 * S);
+  fill_i32(s, tcnt, 2 * S, 0);
+  int64_t *cell_pairs = dget<int64_t>(c->near_pairs, (size_t)S);
+  fill_zero(s, cell_pairs, sizeof(int64_t) * (size_t)S);
+  fill_zero(s, &dp->npairs, sizeof(unsigned long long));
+  hipLaunchKernelGGL(k_task_count, dim3(nblk(nc)), dim3(kT), 0, s, (const DevPlan *)dp, D, S, cpar, nbr27,
                      (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
                      (const int64_t *)sub_pairs, narrow_max, tcnt, cell_pairs);
   NB_LAUNCH();
-  scan_excl_i32(tcnt, toff, 2 * (nc + 1), c->scan_tmp, s);
-  hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(nc))), dim3(kT), 0, s,
-                     (const int64_t *)cell_pairs, nc, pairs);
+  scan_excl_i32(tcnt, toff, 2 * S, c->scan_tmp, s);
+  hipLaunchKernelGGL(k_sum_i64, dim3(std::min<unsigned>(512, nblk(S))), dim3(kT), 0, s, (const int64_t *)cell_pairs,
+                     S, &dp->npairs);
   NB_LAUNCH();
-  NB_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)tcnt, 2 * (nc + 1), tot);
+  fill_zero(s, &dp->ntask, sizeof(unsigned long long));
+  hipLaunchKernelGGL(k_sum_i32, dim3(std::min<unsigned>(512, nblk(2 * S))), dim3(kT), 0, s, (const int32_t *)tcnt,
+                     2 * S, &dp->ntask);
   NB_LAUNCH();
-  NB_HIP(hipMemcpyAsync(htot, tot, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(htot + 1, pairs, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(hint, toff + nc + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipMemcpyAsync(hint + 1, nst_d, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  const unsigned long long ntask = htot[0], npairs = htot[1];
-  const int32_t nnarrow = hint[0];
-  P.n_src = hint[1];
-  P.n_tgt = hint[2];
-  if (ntask > lim) throw Error(NBFMM_ERR_UNSUPPORTED, "P2P work list exceeds 2^31 tasks");
-  P.n_tasks = (int64_t)ntask;
-  P.n_tasks_narrow = nnarrow;
-  P.n_p2p_pairs = (int64_t)npairs;
+  hipLaunchKernelGGL(k_task_totals, dim3(1), dim3(64), 0, s, dp, (const int32_t *)toff, S, K.on ? 1 : 0, K.ntask,
+                     K.n_tgt);
+  NB_LAUNCH();
+  if (!K.on) {
+    read_counts();
+    if (hp->ntask > lim) throw Error(NBFMM_ERR_UNSUPPORTED, "P2P work list exceeds 2^31 tasks");
+    P.n_src = hp->nst[0];
+    P.n_tgt = hp->nst[1];
+    P.n_tasks = (int64_t)hp->ntask;
+    P.n_tasks_narrow = hp->nnarrow;
+    P.n_p2p_pairs = (int64_t)hp->npairs;
+  }
+  const int64_t ntask = K.on ? K.ntask : P.n_tasks;
   P2PTask *tasks = dget<P2PTask>(c->tasks, (size_t)ntask + 1);
-  hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (int32_t)nc, leaf_begin, cpar, nbr27,
+  hipLaunchKernelGGL(k_task_fill, dim3(nblk(nc)), dim3(kT), 0, s, (const DevPlan *)dp, D, S, cpar, nbr27,
                      (const int32_t *)closed, cbeg, cend, (const int32_t *)cnt_sub, (const int32_t *)off_sub,
-                     (const int64_t *)sub_pairs, narrow_max, toff, tasks);
+                     (const int64_t *)sub_pairs, narrow_max, (const int32_t *)toff, tasks);
   NB_LAUNCH();
   NB_HIP(hipStreamWaitEvent(s, c->ev_pos, 0)); // the sorted particles (build_tree)
 }

@@ -68,11 +68,12 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int kg, int q) { return 4 * kg + q; }
 };
 
-// bins: the class, plus kNumClasses * (order - kOrderLo) with per-target orders
-constexpr int kMaxBins = kNumClasses * kNumOrders;
-
-__global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cls, int64_t n, int nbins,
-                                                 int32_t *cnt) {
+// bins (kMaxBins at most): the class, plus kNumClasses * (order - kOrderLo)
+// with per-pair orders.  The pair count comes from the plan.
+__global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cls, const DevPlan *__restrict__ dp,
+                                                 int nbins, int32_t *cnt) {
+  if (plan_dead(dp)) return;
+  const int64_t n = (int64_t)dp->tot[1];
   __shared__ int32_t h[kMaxBins];
   for (int k = threadIdx.x; k < nbins; k += kT) h[k] = 0;
   __syncthreads();
@@ -85,9 +86,11 @@ __global__ void __launch_bounds__(kT) k_cls_hist(const uint16_t *__restrict__ cl
 
 // cursor[k] starts at the first position of bin k; the order of the pairs
 // inside a bin is free (every pair's result is computed independently).
-__global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, int64_t n, int nbins,
-                                                    int32_t *cursor, int32_t *__restrict__ out,
+__global__ void __launch_bounds__(kT) k_cls_scatter(const uint16_t *__restrict__ cls, const DevPlan *__restrict__ dp,
+                                                    int nbins, int32_t *cursor, int32_t *__restrict__ out,
                                                     int32_t *__restrict__ inv) {
+  if (plan_dead(dp)) return;
+  const int64_t n = (int64_t)dp->tot[1];
   __shared__ int32_t h[kMaxBins];
   __shared__ int32_t base[kMaxBins];
   for (int k = threadIdx.x; k < nbins; k += kT) h[k] = 0;
@@ -173,7 +176,8 @@ __device__ __forceinline__ void split3(const float x[8], bf16x8 &b1, bf16x8 &b2,
 // k_m2l_cls for FP32 coefficients with the three-term bf16 products; same
 // items, partials and tile/column mapping (D row 4 (l / 16) + q, column l % 16).
 template <int KP, int RBW>
-__global__ void __launch_bounds__(1024) k_m2l_bf3(const int4 *__restrict__ items, int32_t ybase,
+__global__ void __launch_bounds__(1024) k_m2l_bf3(const DevPlan *__restrict__ dp, const int32_t *__restrict__ irng,
+                                                  const int4 *__restrict__ items, int32_t ybase,
                                                   const int32_t *__restrict__ cls_pairs,
                                                   const int32_t *__restrict__ m2l_src,
                                                   const int32_t *__restrict__ src_slot, const uint4 *__restrict__ fragb,
@@ -182,7 +186,8 @@ __global__ void __launch_bounds__(1024) k_m2l_bf3(const int4 *__restrict__ items
   constexpr int RB = KP / 16, KSB = pad32(KP) / 32;
   constexpr int V = KSB * 3 * 64; // 16-byte vectors per staged row block
   extern __shared__ uint4 sA[];
-  const int4 it = items[blockIdx.x];
+  if (plan_dead(dp) || (int)blockIdx.x >= irng[1]) return; // items [irng[0], + irng[1])
+  const int4 it = items[irng[0] + (int)blockIdx.x];
   const int cls = it.x % kNumClasses;
   const int r0 = (int)blockIdx.y * RBW;
   const int nrb = min(RBW, RB - r0);
@@ -252,11 +257,15 @@ __global__ void __launch_bounds__(1024) k_m2l_bf3(const int4 *__restrict__ items
 
 // bin of every M2L pair with per-pair orders: class + kNumClasses * (order -
 // kOrderLo), order = min(target order (ord, or p when null), class order)
-__global__ void k_order_bins(int64_t ntgt, const int32_t *__restrict__ tgt_cells, const int32_t *__restrict__ off_m2l,
-                             const int32_t *__restrict__ cnt_m2l, const int8_t *__restrict__ ord, int p, ClsOrd qc,
-                             uint16_t *__restrict__ cls) {
+__global__ void k_order_bins(const DevPlan *__restrict__ dp, const int32_t *__restrict__ tgt_cells,
+                             const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
+                             const int8_t *__restrict__ ord, int p, uint16_t *__restrict__ cls) {
+  if (plan_dead(dp)) return;
+  __shared__ ClsOrd qc;
+  for (int k = threadIdx.x; k < kNumClasses; k += kT) qc.q[k] = dp->qc.q[k];
+  __syncthreads();
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  if (t >= ntgt) return;
+  if (t >= dp->nst[1]) return;
   const int32_t c = tgt_cells[t];
   const int ot = ord ? (int)ord[c] : p;
   for (int32_t q = off_m2l[c] + (threadIdx.x & 63), e = off_m2l[c] + cnt_m2l[c]; q < e; q += 64) {
@@ -269,7 +278,9 @@ __global__ void k_order_bins(int64_t ntgt, const int32_t *__restrict__ tgt_cells
 // the operators, multipoles and partials; nvalid: source coefficients used
 // ((q+1)^2 for order q; KPS when every pair has the storage order).
 template <int KP, int RBW, class T>
-__global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__restrict__ items, int32_t ybase,
+__global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const DevPlan *__restrict__ dp,
+                                                         const int32_t *__restrict__ irng,
+                                                         const int4 *__restrict__ items, int32_t ybase,
                                                          const int32_t *__restrict__ cls_pairs,
                                                          const int32_t *__restrict__ m2l_src,
                                                          const int32_t *__restrict__ src_slot, const T *__restrict__ frag,
@@ -278,7 +289,8 @@ __global__ void __launch_bounds__(m2l_threads<T>()) k_m2l_cls(const int4 *__rest
   constexpr int RB = KP / 16, KS = KP / 4;
   extern __shared__ double4 s_raw[];
   T *sA = reinterpret_cast<T *>(s_raw);
-  const int4 it = items[blockIdx.x];
+  if (plan_dead(dp) || (int)blockIdx.x >= irng[1]) return; // items [irng[0], + irng[1])
+  const int4 it = items[irng[0] + (int)blockIdx.x];
   const int cls = it.x % kNumClasses;
   const int r0 = (int)blockIdx.y * RBW;
   const int nrb = min(RBW, RB - r0);
@@ -362,14 +374,16 @@ __device__ __forceinline__ int pair_rows(const uint16_t *bins, int32_t q, int KP
 #define NBFMM_M2L_SUM_UNROLL_F64 4
 #endif
 template <class T>
-__global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
+__global__ void __launch_bounds__(kT) k_m2l_sum(int KP, const DevPlan *__restrict__ dp,
+                                                const int32_t *__restrict__ tgt_cells,
                                                 const int32_t *__restrict__ off_m2l, const int32_t *__restrict__ cnt_m2l,
                                                 const int32_t *__restrict__ lslot, const int32_t *__restrict__ pos,
                                                 int32_t cb, int32_t ce, const T *__restrict__ Y,
                                                 T *__restrict__ Lexp, const uint16_t *__restrict__ bins) {
+  if (plan_dead(dp)) return;
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (t >= ntgt) return;
+  if (t >= dp->nst[1]) return;
   const int32_t c = tgt_cells[t];
   const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
   T *L = Lexp + (size_t)lslot[c] * KP;
@@ -434,15 +448,17 @@ __global__ void __launch_bounds__(kT) k_m2l_sum(int KP, int64_t ntgt, const int3
 #ifndef NBFMM_M2L_SUM4_UNROLL
 #define NBFMM_M2L_SUM4_UNROLL 1
 #endif
-__global__ void __launch_bounds__(kT) k_m2l_sum4(int KP, int64_t ntgt, const int32_t *__restrict__ tgt_cells,
+__global__ void __launch_bounds__(kT) k_m2l_sum4(int KP, const DevPlan *__restrict__ dp,
+                                                 const int32_t *__restrict__ tgt_cells,
                                                  const int32_t *__restrict__ off_m2l,
                                                  const int32_t *__restrict__ cnt_m2l, const int32_t *__restrict__ lslot,
                                                  const int32_t *__restrict__ pos, int32_t cb, int32_t ce,
                                                  const float *__restrict__ Y, float *__restrict__ Lexp,
                                                  const uint16_t *__restrict__ bins) {
+  if (plan_dead(dp)) return;
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (t >= ntgt) return;
+  if (t >= dp->nst[1]) return;
   const int32_t c = tgt_cells[t];
   const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
   float *L = Lexp + (size_t)lslot[c] * KP;
@@ -514,7 +530,7 @@ constexpr int rbw_for(int KP, int lds, int more = 0) {
 }
 
 template <int KP, int RBW, class T>
-void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int32_t *irng, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
                 const T *frag, int KPS, int nvalid) {
   constexpr int RB = KP / 16;
   const size_t lds = (size_t)RBW * 16 * KP * sizeof(T);
@@ -525,7 +541,7 @@ void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase,
     attr_set[c->device & 63] = true;
   }
   hipLaunchKernelGGL((k_m2l_cls<KP, RBW, T>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(m2l_threads<T>()), lds, c->stream,
-                     items, ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, frag,
+                     (const DevPlan *)dplan(c), irng, items, ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, frag,
                      (const T *)c->mexp.p, (T *)c->ypair.p, KPS, nvalid);
   NB_LAUNCH();
 }
@@ -534,13 +550,13 @@ void launch_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase,
 constexpr int kLds = 163840;
 
 template <int KP, class T>
-void launch_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+void launch_kp(nbfmm_ctx *c, unsigned nitems, const int32_t *irng, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
                const T *frag, int KPS, int nvalid) {
   if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
   if (c->shapes.m2l_slices == 2)
-    launch_rbw<KP, rbw_for<T>(KP, kLds, 1), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
+    launch_rbw<KP, rbw_for<T>(KP, kLds, 1), T>(c, nitems, irng, items, ybase, cls_pairs, frag, KPS, nvalid);
   else
-    launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid);
+    launch_rbw<KP, rbw_for<T>(KP, kLds), T>(c, nitems, irng, items, ybase, cls_pairs, frag, KPS, nvalid);
 }
 
 // as rbw_for, for the three-term bf16 operator blocks
@@ -551,7 +567,7 @@ constexpr int rbw_bf3(int KP, int more = 0) {
 }
 
 template <int KP, int RBW>
-void launch_bf3_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+void launch_bf3_rbw(nbfmm_ctx *c, unsigned nitems, const int32_t *irng, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
                     const uint4 *fragb, int KPS, int nvalid) {
   constexpr int RB = KP / 16;
   if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
@@ -562,27 +578,27 @@ void launch_bf3_rbw(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t yb
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set[c->device & 63] = true;
   }
-  hipLaunchKernelGGL((k_m2l_bf3<KP, RBW>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(1024), lds, c->stream, items,
-                     ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, fragb,
+  hipLaunchKernelGGL((k_m2l_bf3<KP, RBW>), dim3(nitems, (RB + RBW - 1) / RBW), dim3(1024), lds, c->stream,
+                     (const DevPlan *)dplan(c), irng, items, ybase, cls_pairs, (const int32_t *)c->m2l_src.p, (const int32_t *)c->mslot.p, fragb,
                      (const float *)c->mexp.p, (float *)c->ypair.p, KPS, nvalid);
   NB_LAUNCH();
 }
 
 template <int KP>
-void launch_bf3_kp(nbfmm_ctx *c, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+void launch_bf3_kp(nbfmm_ctx *c, unsigned nitems, const int32_t *irng, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
                    const uint4 *fragb, int KPS, int nvalid) {
   if (c->shapes.m2l_slices == 2)
-    launch_bf3_rbw<KP, rbw_bf3(KP, 1)>(c, nitems, items, ybase, cls_pairs, fragb, KPS, nvalid);
+    launch_bf3_rbw<KP, rbw_bf3(KP, 1)>(c, nitems, irng, items, ybase, cls_pairs, fragb, KPS, nvalid);
   else
-    launch_bf3_rbw<KP, rbw_bf3(KP)>(c, nitems, items, ybase, cls_pairs, fragb, KPS, nvalid);
+    launch_bf3_rbw<KP, rbw_bf3(KP)>(c, nitems, irng, items, ybase, cls_pairs, fragb, KPS, nvalid);
 }
 
-void launch_bf3(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+void launch_bf3(nbfmm_ctx *c, int KP, unsigned nitems, const int32_t *irng, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
                 const uint4 *fragb, int KPS, int nvalid) {
   switch (KP) {
 #define NB_BF3_CASE(k)                                                                \
   case k:                                                                             \
-    launch_bf3_kp<k>(c, nitems, items, ybase, cls_pairs, fragb, KPS, nvalid);         \
+    launch_bf3_kp<k>(c, nitems, irng, items, ybase, cls_pairs, fragb, KPS, nvalid);         \
     break;
     NB_BF3_CASE(16)
     NB_BF3_CASE(32)
@@ -602,12 +618,12 @@ void launch_bf3(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_
 }
 
 template <class T>
-void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
+void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int32_t *irng, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
             const T *frag, int KPS, int nvalid) {
   switch (KP) {
 #define NB_M2L_CASE(k)                                                      \
   case k:                                                                   \
-    launch_kp<k, T>(c, nitems, items, ybase, cls_pairs, frag, KPS, nvalid); \
+    launch_kp<k, T>(c, nitems, irng, items, ybase, cls_pairs, frag, KPS, nvalid); \
     break;
     NB_M2L_CASE(16)
     NB_M2L_CASE(32)
@@ -636,16 +652,20 @@ void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int4 *items, int32_t yb
 // (columns of a missing child are zero; the waves' partial sums meet in LDS,
 // added in a fixed order); L2L adds the translated parent local to every
 // child's local.
+// The parents cells[rng[0], rng[1]) (a device range: one level's).
 template <int KP, class T>
-__global__ void __launch_bounds__(256) k_m2m(int32_t ncell, const int32_t *__restrict__ cells,
+__global__ void __launch_bounds__(256) k_m2m(const DevPlan *__restrict__ dp, const int32_t *__restrict__ rng,
+                                             const int32_t *__restrict__ cells,
                                              const int32_t *__restrict__ child8, const int32_t *__restrict__ mslot,
                                              const T *__restrict__ frag8, T *__restrict__ Mexp) {
+  if (plan_dead(dp)) return;
   constexpr int RB = KP / 16, KS = KP / 4;
   __shared__ T part[2][RB * 4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kg = lane >> 4;
   const int pi = (int)blockIdx.x * 16 + col;
-  const int32_t cell = pi < ncell ? cells[pi] : -1;
+  const int32_t ncell = rng[1] - rng[0];
+  const int32_t cell = pi < ncell ? cells[rng[0] + pi] : -1;
   typename Mfma<T>::V acc[RB];
 #pragma unroll
   for (int r = 0; r < RB; ++r) acc[r] = typename Mfma<T>::V{0, 0, 0, 0};
@@ -696,14 +716,17 @@ __global__ void __launch_bounds__(256) k_m2m(int32_t ncell, const int32_t *__res
 }
 
 template <int KP, class T>
-__global__ void __launch_bounds__(256) k_l2l(int32_t ncell, const int32_t *__restrict__ cells,
+__global__ void __launch_bounds__(256) k_l2l(const DevPlan *__restrict__ dp, const int32_t *__restrict__ rng,
+                                             const int32_t *__restrict__ cells,
                                              const int32_t *__restrict__ child8, const int32_t *__restrict__ lslot,
                                              const T *__restrict__ frag8, T *__restrict__ Lexp) {
+  if (plan_dead(dp)) return;
   constexpr int RB = KP / 16, KS = KP / 4;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kg = lane >> 4;
   const int pi = (int)blockIdx.x * 16 + col;
-  const int32_t cell = pi < ncell ? cells[pi] : -1;
+  const int32_t ncell = rng[1] - rng[0];
+  const int32_t cell = pi < ncell ? cells[rng[0] + pi] : -1;
   const T *Lb = Lexp + (size_t)(cell >= 0 ? lslot[cell] : 0) * KP + kg;
   for (int o = wave; o < 8; o += 4) {
     const int32_t ch = cell >= 0 ? child8[(int64_t)cell * 8 + o] : -1;
@@ -732,19 +755,21 @@ __global__ void __launch_bounds__(256) k_l2l(int32_t ncell, const int32_t *__res
   }
 }
 
+// the parents cells[rng[0], rng[1]) of one level, at most `grid_cells` of them
 template <class T>
-void launch_trans(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *cells, const int32_t *slot,
-                  const T *frag8, T *X) {
-  if (ncell <= 0) return;
-  const unsigned nb = (unsigned)((ncell + 15) / 16);
+void launch_trans(nbfmm_ctx *c, bool l2l, int KP, const int32_t *cells, const int32_t *rng, int64_t grid_cells,
+                  const int32_t *slot, const T *frag8, T *X) {
+  if (grid_cells <= 0) return;
+  const unsigned nb = (unsigned)((grid_cells + 15) / 16);
   const int32_t *child8 = (const int32_t *)c->child8.p;
+  const DevPlan *dp = dplan(c);
   switch (KP) {
-#define NB_TR_CASE(k)                                                                                      \
-  case k:                                                                                                  \
-    if (l2l)                                                                                               \
-      hipLaunchKernelGGL((k_l2l<k, T>), dim3(nb), dim3(256), 0, c->stream, ncell, cells, child8, slot, frag8, X); \
-    else                                                                                                   \
-      hipLaunchKernelGGL((k_m2m<k, T>), dim3(nb), dim3(256), 0, c->stream, ncell, cells, child8, slot, frag8, X); \
+#define NB_TR_CASE(k)                                                                                          \
+  case k:                                                                                                      \
+    if (l2l)                                                                                                   \
+      hipLaunchKernelGGL((k_l2l<k, T>), dim3(nb), dim3(256), 0, c->stream, dp, rng, cells, child8, slot, frag8, X); \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_m2m<k, T>), dim3(nb), dim3(256), 0, c->stream, dp, rng, cells, child8, slot, frag8, X); \
     break;
     NB_TR_CASE(16)
     NB_TR_CASE(32)
@@ -766,60 +791,306 @@ void launch_trans(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *
 
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
 
+// Distance-dependent orders.  The M2L of class d truncated after degree q
+// has the worst-case bound rate_d^(q+1) / |d|^2 (relative to the field of
+// the pair's source at unit distance), rate_d = (cell diagonal) / |d|; for
+// particles spread over their cells the error falls per degree by about
+// kOrderAlpha * rate_d instead (measured for cubes: 0.50 for the face class
+// (2,0,0) of rate 0.87, 0.15 for (3,3,3) of rate 0.33).  A class gets the
+// lowest q in [kOrderLo, p] with
+//   (a rate_d)^(q+1) / |d|^2 <= tol (a rate_0)^(p+1) / |d_0|^2,
+// class 0 being the slowest M2L class (rates and |d| the same for every
+// level: cells keep the box's aspect ratio); classes evaluated directly or
+// not admissible, and class 0, keep p (every class keeps p when tol = 0).
+// The oracle computes the same table with the same operations (orc_exact.c).
+// One workgroup of 512 threads, from the plan's box into dp->qc.
+__global__ void __launch_bounds__(512) k_class_orders(DevPlan *dp, int p, double tol) {
+#pragma clang fp contract(off)
+  __shared__ double r2s[kNumClasses], d2s[kNumClasses];
+  __shared__ double r0s, d0s;
+  const int t = threadIdx.x;
+  const double hx = dp->hi[0] - dp->lo[0], hy = dp->hi[1] - dp->lo[1], hz = dp->hi[2] - dp->lo[2];
+  const double diag2 = hx * hx + hy * hy + hz * hz;
+  if (t < kNumClasses) {
+    const int dx = t % 7 - 3, dy = (t / 7) % 7 - 3, dz = t / 49 - 3;
+    double r2 = -1, d2 = 0;
+    if (max(abs(dx), max(abs(dy), abs(dz))) >= 2) {
+      d2 = dx * dx * hx * hx + dy * dy * hy * hy + dz * dz * hz * hz;
+      if (!(diag2 > kMacRatio * kMacRatio * d2)) r2 = diag2 / d2;
+    }
+    r2s[t] = r2;
+    d2s[t] = d2;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double r0 = 0, d0 = 0; // the largest squared rate of an admissible class (the first), its |d|^2
+    for (int k = 0; k < kNumClasses; ++k)
+      if (r2s[k] > r0) {
+        r0 = r2s[k];
+        d0 = d2s[k];
+      }
+    r0s = r0;
+    d0s = d0;
+  }
+  __syncthreads();
+  if (t < kNumClasses) {
+    int q = p;
+    const double r2 = r2s[t], r0 = r0s, d0 = d0s;
+    if (tol > 0 && r2 > 0 && r2 < r0) {
+      const double la = log(kOrderAlpha);
+      const double need = (log(tol) + (p + 1) * (la + 0.5 * log(r0)) + log(d2s[t] / d0)) / (la + 0.5 * log(r2));
+      q = (int)ceil(need) - 1;
+      q = min(p, max(kOrderLo, q));
+    }
+    dp->qc.q[t] = (uint8_t)q;
+  }
+}
+
+// The M2L schedule of one pass on the device: every bin's first position
+// (the scatter's cursors), items of at most `per` consecutive pairs of one
+// bin in bin order, each order's item range (bins are order-major), and the
+// algorithmic flops (2 n^2 per pair, n = (q+1)^2) and order sum of the pairs.
+// Items are written only when they fit `items_cap` (a planned evaluation
+// flags the overflow instead).  One workgroup; bins in runs of kSchedPer per
+// thread.
+constexpr int kSchedThreads = 1024;
+constexpr int kSchedPer = (kMaxBins + kSchedThreads - 1) / kSchedThreads;
+struct OrdCaps {
+  int32_t v[kNumOrders];
+};
+__global__ void __launch_bounds__(kSchedThreads) k_m2l_sched(DevPlan *dp, int nbins, int per, int binned, int p,
+                                                            int check, int64_t items_cap, OrdCaps cap_ord,
+                                                            int4 *__restrict__ items) {
+  if (plan_dead(dp)) return;
+  __shared__ int32_t wsum[kSchedThreads / 64][2];
+  __shared__ double wfl[kSchedThreads / 64][2];
+  __shared__ int32_t ostart[kNumOrders + 1];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int k0 = t * kSchedPer;
+  int32_t cnt[kSchedPer], ni[kSchedPer];
+  int32_t sc = 0, si = 0;
+  double fl = 0, qs = 0;
+#pragma unroll
+  for (int j = 0; j < kSchedPer; ++j) {
+    const int k = k0 + j;
+    cnt[j] = k < nbins ? dp->bin_cnt[k] : 0;
+    ni[j] = (cnt[j] + per - 1) / per;
+    sc += cnt[j];
+    si += ni[j];
+    const int q = binned ? kOrderLo + k / kNumClasses : p;
+    const double nq = (double)(q + 1) * (q + 1);
+    fl += 2.0 * nq * nq * cnt[j];
+    qs += (double)q * cnt[j];
+  }
+  int32_t xc = sc, xi = si;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t yc = __shfl_up(xc, o, 64), yi = __shfl_up(xi, o, 64);
+    if (lane >= o) {
+      xc += yc;
+      xi += yi;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    fl += __shfl_xor(fl, o, 64);
+    qs += __shfl_xor(qs, o, 64);
+  }
+  if (lane == 63) {
+    wsum[wv][0] = xc;
+    wsum[wv][1] = xi;
+  }
+  if (lane == 0) {
+    wfl[wv][0] = fl;
+    wfl[wv][1] = qs;
+  }
+  __syncthreads();
+  int32_t pc = 0, pi = 0, ti = 0;
+  for (int q = 0; q < kSchedThreads / 64; ++q) {
+    if (q < wv) {
+      pc += wsum[q][0];
+      pi += wsum[q][1];
+    }
+    ti += wsum[q][1];
+  }
+  const bool fits = ti <= items_cap;
+  int32_t bc = pc + xc - sc, bi = pi + xi - si; // this thread's first bin: first pair, first item
+#pragma unroll
+  for (int j = 0; j < kSchedPer; ++j) {
+    const int k = k0 + j;
+    if (k >= nbins) break;
+    dp->bin_cur[k] = bc;
+    if (binned && k % kNumClasses == 0) ostart[k / kNumClasses] = bi;
+    if (fits)
+      for (int m = 0; m < ni[j]; ++m)
+        items[bi + m] = make_int4(k, bc + m * per, min(bc + (m + 1) * per, bc + cnt[j]), 0);
+    bc += cnt[j];
+    bi += ni[j];
+  }
+  __syncthreads();
+  if (t != 0) return;
+  const int nord = binned ? nbins / kNumClasses : 1;
+  if (!binned) ostart[0] = 0;
+  ostart[nord] = ti;
+  int ovf = check && !fits ? 2 : 0;
+  for (int oi = 0; oi < kNumOrders; ++oi) {
+    const int32_t f = oi < nord ? ostart[oi] : ti, e = oi < nord ? ostart[oi + 1] : ti;
+    dp->ord_items[oi][0] = f;
+    dp->ord_items[oi][1] = e - f;
+    if (check && e - f > cap_ord.v[oi]) ovf = 2;
+  }
+  dp->n_items = ti;
+  double a = 0, b = 0;
+  for (int q = 0; q < kSchedThreads / 64; ++q) {
+    a += wfl[q][0];
+    b += wfl[q][1];
+  }
+  dp->flop_m2l = a;
+  dp->order_sum = b;
+  if (ovf) dp->ovf |= ovf;
+}
+
 } // namespace
 
-// Sorts the pairs by bin (counts -> host schedule -> scatter), runs the
-// MFMA contraction (between events 7 and 8) and the per-target sum.
-// Expects the FP64 operators in c->tmat and the multipoles in c->mexp
-// (FP32 when `f32`), both of width KP.  `ord` (device, per cell) gives the
-// order of every target, or is null when all have the order of KP.
+// Sorts the pairs by bin (counts -> schedule -> scatter), runs the MFMA
+// contraction (between events 7 and 8) and the per-target sum.  Expects the
+// FP64 operators in c->tmat and the multipoles in c->mexp (FP32 when `f32`),
+// both of width KP.  `ord` (device, per cell) gives the order of every
+// target, or is null when all have the order of KP.  One pass (the partials
+// of every pair in one buffer) is scheduled on the device; several passes
+// (partials beyond kPassBytes, or m2l_pass_pairs) are scheduled on the host
+// (direct evaluations only).
 void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
   Plan &P = c->plan;
+  const Caps &K = c->cap;
+  DevPlan *dp = dplan(c);
   hipStream_t s = c->stream;
-  const int64_t nt = P.n_tgt, npair = P.n_m2l;
+  const int64_t nt = K.on ? K.n_tgt : P.n_tgt, npair = K.on ? K.n_m2l : P.n_m2l; // exact, or the capacities
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
   // per-pair orders: the targets' (learned) and/or the classes'
   const bool binned = ord || c->prm.class_order_tol > 0;
   const int nbins = binned ? kMaxBins : kNumClasses;
   if (binned) {
-    const ClsOrd qc = class_orders(P.geo, p, c->prm.class_order_tol);
-    hipLaunchKernelGGL(k_order_bins, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, nt,
-                       (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
-                       ord, p, qc, (uint16_t *)c->m2l_cls.p);
+    hipLaunchKernelGGL(k_class_orders, dim3(1), dim3(512), 0, s, dp, p, c->prm.class_order_tol);
+    NB_LAUNCH();
+    hipLaunchKernelGGL(k_order_bins, dim3((unsigned)std::max<int64_t>(1, (nt + 3) / 4)), dim3(kT), 0, s,
+                       (const DevPlan *)dp, (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
+                       (const int32_t *)c->cnt_m2l.p, ord, p, (uint16_t *)c->m2l_cls.p);
     NB_LAUNCH();
   }
-  int32_t *ccnt = dget<int32_t>(c->cls_cnt, 2 * kMaxBins + 8);
-  NB_HIP(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * nbins, s));
-  const unsigned nchunk = (unsigned)((npair + kClsChunk - 1) / kClsChunk);
-  hipLaunchKernelGGL(k_cls_hist, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, nbins, ccnt);
+  fill_i32(s, dp->bin_cnt, nbins, 0);
+  const unsigned nchunk = (unsigned)std::max<int64_t>(1, (npair + kClsChunk - 1) / kClsChunk);
+  hipLaunchKernelGGL(k_cls_hist, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, (const DevPlan *)dp,
+                     nbins, dp->bin_cnt);
   NB_LAUNCH();
-  int32_t *hcnt = (int32_t *)c->hred.p; // pinned, >= kMaxBins ints
-  NB_HIP(hipMemcpyAsync(hcnt, ccnt, sizeof(int32_t) * nbins, hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
+  // passes: consecutive ranges of the sorted pairs whose partials fit the
+  // buffer (at most kPassBytes of the 288 GB, or the caller's pairs-per-pass limit)
+  constexpr int64_t kPassBytes = (int64_t)16 << 30;
+  int64_t pass = c->prm.m2l_pass_pairs > 0 ? c->prm.m2l_pass_pairs : kPassBytes / (KP * (int64_t)esz);
+  pass = std::max<int64_t>(1, std::min<int64_t>(pass, std::max<int64_t>(1, npair)));
+  const bool one = npair <= pass;
+  if (K.on && !one) throw Error(NBFMM_ERR_UNSUPPORTED, "planned evaluations need the M2L partials in one pass");
+  int64_t per = npair / 2048;
+  per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
+  int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair * 2 + 2);
+  int32_t *pos = cls_pairs + npair + 1; // sorted position of every list entry
+  void *Lexp = c->lexp.p; // sized and cleared by the caller (one slot per cell that holds a local)
+  auto fragments = [&]() -> const void * {
+    if (!f32) return c->tmat.p;
+    // three-term bf16 fragments of the operators
+    const int64_t nl = (int64_t)kNumClasses * (KP / 16) * (pad32(KP) / 32) * 64;
+    uint4 *fb = dget<uint4>(c->tmat32, (size_t)nl * 3);
+    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, KP, (const double *)c->tmat.p, fb);
+    NB_LAUNCH();
+    return fb;
+  };
+  auto contract = [&](int oi, unsigned nitems, const int32_t *irng, const int4 *items, int32_t pb, const void *frag) {
+    const int q = binned ? kOrderLo + oi : p;
+    const int KPT = binned ? padded_k(q) : KP, nvalid = binned ? (q + 1) * (q + 1) : KP;
+    if (f32)
+      launch_bf3(c, KPT, nitems, irng, items, pb, cls_pairs, (const uint4 *)frag, KP, nvalid);
+    else
+      launch<double>(c, KPT, nitems, irng, items, pb, cls_pairs, (const double *)frag, KP, nvalid);
+  };
+  auto sum = [&](int32_t pb, int32_t pe) {
+    const unsigned nb = (unsigned)std::max<int64_t>(1, (nt + 3) / 4);
+    const uint16_t *bins = binned ? (const uint16_t *)c->m2l_cls.p : nullptr;
+    if (f32 && KP % 4 == 0 && KP <= 4 * 64)
+      hipLaunchKernelGGL(k_m2l_sum4, dim3(nb), dim3(kT), 0, s, KP, (const DevPlan *)dp, (const int32_t *)c->tgt_cells.p,
+                         (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p,
+                         (const int32_t *)pos, pb, pe, (const float *)c->ypair.p, (float *)Lexp, bins);
+    else if (f32)
+      hipLaunchKernelGGL(k_m2l_sum<float>, dim3(nb), dim3(kT), 0, s, KP, (const DevPlan *)dp,
+                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                         (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe, (const float *)c->ypair.p,
+                         (float *)Lexp, bins);
+    else
+      hipLaunchKernelGGL(k_m2l_sum<double>, dim3(nb), dim3(kT), 0, s, KP, (const DevPlan *)dp,
+                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                         (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe, (const double *)c->ypair.p,
+                         (double *)Lexp, bins);
+    NB_LAUNCH();
+  };
+  const int nord = binned ? (ord ? kNumOrders : p - kOrderLo + 1) : 1; // order indices that may hold pairs
+
+  if (one) {
+    const int64_t icap = K.on ? K.items : npair / per + nbins + 1;
+    int4 *ditems = dget<int4>(c->m2l_items, (size_t)icap + 1);
+    OrdCaps oc{};
+    for (int oi = 0; oi < kNumOrders; ++oi) oc.v[oi] = K.items_ord[oi];
+    hipLaunchKernelGGL(k_m2l_sched, dim3(1), dim3(kSchedThreads), 0, s, dp, nbins, (int)per, binned ? 1 : 0, p,
+                       K.on ? 1 : 0, icap, oc, ditems);
+    NB_LAUNCH();
+    int32_t cnt_ord[kNumOrders] = {};
+    if (!K.on) {
+      read_plan(c, &c->hplan->ord_items[0][0],
+                offsetof(DevPlan, order_sum) + sizeof(double) - offsetof(DevPlan, ord_items), s);
+      const DevPlan *hp = c->hplan;
+      P.flop_m2l = hp->flop_m2l;
+      P.mean_order = npair > 0 ? hp->order_sum / (double)npair : (double)p;
+      P.n_m2l_tiles = hp->n_items;
+      c->last.items = hp->n_items;
+      for (int oi = 0; oi < kNumOrders; ++oi) {
+        cnt_ord[oi] = hp->ord_items[oi][1];
+        c->last.items_ord[oi] = cnt_ord[oi];
+      }
+    } else {
+      for (int oi = 0; oi < kNumOrders; ++oi) cnt_ord[oi] = K.items_ord[oi];
+    }
+    hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p,
+                       (const DevPlan *)dp, nbins, dp->bin_cur, cls_pairs, pos);
+    NB_LAUNCH();
+    dbuf_ensure(c->ypair, (size_t)std::max<int64_t>(1, npair) * KP * esz + 16);
+    const void *frag = fragments();
+    mark(c, 7);
+    for (int oi = 0; oi < nord; ++oi)
+      if (cnt_ord[oi] > 0) contract(oi, (unsigned)cnt_ord[oi], &dp->ord_items[oi][0], ditems, 0, frag);
+    mark(c, 8);
+    launch_p2p(c, 1); // the rest of P2P beside the rest of the chain
+    c->p2p_started = true;
+    sum(0, INT32_MAX);
+    return;
+  }
+
+  // ---- several passes (direct evaluations): the schedule on the host ----
+  int32_t *hcnt = c->hplan->bin_cnt;
+  read_plan(c, hcnt, sizeof(int32_t) * nbins, s);
   std::vector<int32_t> &hoff = c->h_cls_off;
   hoff.assign(nbins + 1, 0);
   for (int k = 0; k < nbins; ++k) hoff[k + 1] = hoff[k] + hcnt[k];
   if (hoff[nbins] != npair) throw Error(NBFMM_ERR_HIP, "M2L class histogram does not match the pair count");
   // algorithmic flops: 2 n^2 per pair, n = (q+1)^2 coefficients of the pair's order q
   {
-    double fl = 0;
+    double fl = 0, qs = 0;
     for (int k = 0; k < nbins; ++k) {
       const int q = binned ? kOrderLo + k / kNumClasses : p;
       const double nq = (double)(q + 1) * (q + 1);
       fl += 2.0 * nq * nq * hcnt[k];
+      qs += (double)q * hcnt[k];
     }
     P.flop_m2l = fl;
-    double qs = 0;
-    for (int k = 0; k < nbins; ++k) qs += (double)(binned ? kOrderLo + k / kNumClasses : p) * hcnt[k];
     P.mean_order = npair > 0 ? qs / (double)npair : (double)p;
   }
-  // passes: consecutive ranges of the sorted pairs whose partials fit the
-  // buffer (at most kPassBytes of the 288 GB, or the caller's pairs-per-pass limit)
-  constexpr int64_t kPassBytes = (int64_t)16 << 30;
-  int64_t pass = c->prm.m2l_pass_pairs > 0 ? c->prm.m2l_pass_pairs : kPassBytes / (KP * (int64_t)esz);
-  pass = std::max<int64_t>(1, std::min<int64_t>(pass, npair));
-  int64_t per = npair / 2048;
-  per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
   std::vector<int4> &hitems = c->h_m2l_items;
   // launches: (pass, first item, item count, order index) -- items of one pass
   // and one order form one launch
@@ -840,79 +1111,58 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
       }
     }
   }
-  NB_HIP(hipMemcpyAsync(ccnt + kMaxBins, hoff.data(), sizeof(int32_t) * nbins, hipMemcpyHostToDevice, s));
-  int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair * 2 + 2);
-  int32_t *pos = cls_pairs + npair + 1; // sorted position of every list entry
-  hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, npair, nbins,
-                     ccnt + kMaxBins, cls_pairs, pos);
+  NB_HIP(hipMemcpyAsync(dp->bin_cur, hoff.data(), sizeof(int32_t) * nbins, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_cls_scatter, dim3(nchunk), dim3(kT), 0, s, (const uint16_t *)c->m2l_cls.p, (const DevPlan *)dp,
+                     nbins, dp->bin_cur, cls_pairs, pos);
   NB_LAUNCH();
   const unsigned nitems = (unsigned)hitems.size();
   int4 *ditems = dget<int4>(c->m2l_items, (size_t)nitems + 1);
   NB_HIP(hipMemcpyAsync(ditems, hitems.data(), sizeof(int4) * nitems, hipMemcpyHostToDevice, s));
+  // each launch's item range (first, count) for the kernels
+  std::vector<int32_t> hr(2 * runs.size());
+  for (size_t r = 0; r < runs.size(); ++r) {
+    hr[2 * r] = runs[r].y;
+    hr[2 * r + 1] = runs[r].z;
+  }
+  int32_t *drng = dget<int32_t>(c->cls_cnt, hr.size() + 2);
+  NB_HIP(hipMemcpyAsync(drng, hr.data(), sizeof(int32_t) * hr.size(), hipMemcpyHostToDevice, s));
   P.n_m2l_tiles = nitems;
   dbuf_ensure(c->ypair, (size_t)pass * KP * esz + 16);
-  void *Lexp = c->lexp.p; // sized and cleared by the caller (one slot per cell that holds a local)
-  const void *frag = c->tmat.p;
-  if (f32) {
-    // three-term bf16 fragments of the operators
-    const int64_t nl = (int64_t)kNumClasses * (KP / 16) * (pad32(KP) / 32) * 64;
-    uint4 *fb = dget<uint4>(c->tmat32, (size_t)nl * 3);
-    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, KP, (const double *)c->tmat.p, fb);
-    NB_LAUNCH();
-    frag = fb;
-  }
+  const void *frag = fragments();
+  // the host vectors above are read by the copies: wait before they change
+  NB_HIP(hipStreamSynchronize(s));
   mark(c, 7);
   const int npass = (int)((npair + pass - 1) / pass);
   size_t r = 0;
   for (int ps = 0; ps < npass; ++ps) {
     const int32_t pb = (int32_t)((int64_t)ps * pass), pe = (int32_t)std::min<int64_t>(npair, (int64_t)pb + pass);
-    for (; r < runs.size() && runs[r].x == ps; ++r) {
-      const int q = binned ? kOrderLo + runs[r].w : p;
-      const int KPT = binned ? padded_k(q) : KP, nvalid = binned ? (q + 1) * (q + 1) : KP;
-      const int4 *it = ditems + runs[r].y;
-      if (f32)
-        launch_bf3(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const uint4 *)frag, KP, nvalid);
-      else
-        launch<double>(c, KPT, (unsigned)runs[r].z, it, pb, cls_pairs, (const double *)frag, KP, nvalid);
-    }
+    for (; r < runs.size() && runs[r].x == ps; ++r)
+      contract(runs[r].w, (unsigned)runs[r].z, drng + 2 * r, ditems, pb, frag);
     if (ps == 0) {
       mark(c, 8);
       launch_p2p(c, 1); // the rest of P2P beside the rest of the chain
       c->p2p_started = true;
     }
-    if (f32 && KP % 4 == 0 && KP <= 4 * 64)
-      hipLaunchKernelGGL(k_m2l_sum4, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
-                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
-                         (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
-                         (const float *)c->ypair.p, (float *)Lexp, binned ? (const uint16_t *)c->m2l_cls.p : nullptr);
-    else if (f32)
-      hipLaunchKernelGGL(k_m2l_sum<float>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
-                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
-                         (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
-                         (const float *)c->ypair.p, (float *)Lexp, binned ? (const uint16_t *)c->m2l_cls.p : nullptr);
-    else
-      hipLaunchKernelGGL(k_m2l_sum<double>, dim3((unsigned)((nt + 3) / 4)), dim3(kT), 0, s, KP, nt,
-                         (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p,
-                         (const int32_t *)c->cnt_m2l.p, (const int32_t *)c->lslot.p, (const int32_t *)pos, pb, pe,
-                         (const double *)c->ypair.p, (double *)Lexp, binned ? (const uint16_t *)c->m2l_cls.p : nullptr);
-    NB_LAUNCH();
+    sum(pb, pe);
   }
 }
 
-void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells) {
+void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, const int32_t *cells, const int32_t *rng,
+               int64_t grid_cells) {
   const int32_t *slot = (const int32_t *)(l2l ? c->lslot.p : c->mslot.p);
   void *X = l2l ? c->lexp.p : c->mexp.p;
   const size_t off = l2l ? (size_t)8 * KP * KP : 0;
   if (f32)
-    launch_trans<float>(c, l2l, KP, ncell, cells, slot, (const float *)c->trans32.p + off, (float *)X);
+    launch_trans<float>(c, l2l, KP, cells, rng, grid_cells, slot, (const float *)c->trans32.p + off, (float *)X);
   else
-    launch_trans<double>(c, l2l, KP, ncell, cells, slot, (const double *)c->trans.p + off, (double *)X);
+    launch_trans<double>(c, l2l, KP, cells, rng, grid_cells, slot, (const double *)c->trans.p + off, (double *)X);
 }
 
-void run_trans64(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *cells, double *X) {
+void run_trans64(nbfmm_ctx *c, bool l2l, int KP, const int32_t *cells, const int32_t *rng, int64_t grid_cells,
+                 double *X) {
   const int32_t *slot = (const int32_t *)(l2l ? c->lslot.p : c->mslot.p);
   const size_t off = l2l ? (size_t)8 * KP * KP : 0;
-  launch_trans<double>(c, l2l, KP, ncell, cells, slot, (const double *)c->trans.p + off, X);
+  launch_trans<double>(c, l2l, KP, cells, rng, grid_cells, slot, (const double *)c->trans.p + off, X);
 }
 
 void widen(nbfmm_ctx *c, const float *in, int64_t count, double *out) {

@@ -34,6 +34,10 @@ struct DBuf {
 };
 void *dbuf_ensure(DBuf &b, size_t bytes);
 void dbuf_free(DBuf &b);
+// counts every (re)allocation: a captured graph is valid while it is unchanged
+uint64_t dbuf_generation();
+// while set, a buffer that would have to grow throws instead (graph capture)
+void set_capturing(bool on);
 template <class T>
 inline T *dget(DBuf &b, size_t count) {
   return static_cast<T *>(dbuf_ensure(b, count * sizeof(T) + 16));
@@ -118,20 +122,106 @@ __device__ __forceinline__ int level_of(int32_t cell, const LevelGeo &g) {
 }
 
 // Offset classes whose M2L does not converge fast enough for the cell shape
-// (bit k: class k); their pairs are evaluated directly (lists.hip).
+// (bit k: class k); their pairs are evaluated directly (lists.hip).  Built
+// on the device from the tree's box (k_tree_plan, tree.hip).
 struct ClsMask {
   uint32_t w[(kNumClasses + 31) / 32];
 };
 constexpr double kMacRatio = 0.9;
 constexpr int64_t kBadDirect = (int64_t)1 << 20; // largest n_t n_s evaluated directly for such classes
-ClsMask class_mask(const Geometry &geo);
-// M2L order of every offset class (class_order_tol > 0), kernel argument;
+// M2L order of every offset class (class_order_tol > 0), built beside the mask;
 // kOrderAlpha: observed decay per degree over the worst-case rate
 constexpr double kOrderAlpha = 0.55;
 struct ClsOrd {
   uint8_t q[kNumClasses + 1];
 };
-ClsOrd class_orders(const Geometry &geo, int p, double tol);
+
+// ---- the counts of one evaluation, in device memory ---------------------------------
+// Every size an evaluation derives from the particles (the box, the level
+// starts, list and task totals, the up-pass planning, the M2L schedule) is
+// written by kernels into one DevPlan, and the kernels that depend on a size
+// read it there.  A direct evaluation also reads the sizes on the host, to
+// launch and allocate exactly; a planned evaluation (Caps::on) launches and
+// allocates by capacities fixed beforehand and reads nothing back, so that a
+// whole integration step can be captured in a HIP graph.  A planned size
+// beyond its capacity, or a depth other than the planned one, sets `ovf`;
+// every later kernel of the step then leaves at once, the integrator does
+// not update the state, and the host runs the step again directly.
+constexpr int kMaxBins = kNumClasses * kNumOrders; // M2L bins: (order, class)
+constexpr int kMaxChunks0 = 64;                    // P2P first-share launches at most
+// P2P task ranges [begin, end) in DevPlan::p2p_rng: the first share's narrow
+// tasks in chunks (chunk ch at 2 ch) and whole, its wide tasks, the rest's
+// narrow and wide tasks, and all narrow and all wide tasks
+constexpr int kRngNarrow0 = 2 * kMaxChunks0, kRngWide0 = kRngNarrow0 + 2, kRngNarrow1 = kRngNarrow0 + 4,
+              kRngWide1 = kRngNarrow0 + 6, kRngNarrowAll = kRngNarrow0 + 8, kRngWideAll = kRngNarrow0 + 10,
+              kRngCount = kRngNarrow0 + 12;
+struct DevPlan {
+  double lo[3], hi[3]; // the tree's box (tree.cpp:20-30; the cube around it with cubic_box)
+  double hs[3];        // leaf edges over rho at the depth (operator geometry)
+  int32_t bad;         // a non-finite coordinate or mass
+  int32_t ovf;         // planned evaluation: a size beyond its capacity (bit 1), another depth (bit 0), bad input (bit 2)
+  int32_t depth, nc;
+  int32_t iterations;                    // iterations of the reference's depth loop
+  int32_t one_nc[2];                     // {1, nc}: the cells that have a parent
+  int32_t lvl_start[kMaxDepth + 2];      // cells concatenated from the root
+  uint32_t lcount[2 * (kMaxDepth + 1)];  // the last count: cells per level, then each level's fullest cell
+  LevelGeo g;
+  ClsMask bad_cls;
+  ClsOrd qc;
+  // lists and tasks
+  int32_t nactive;                       // parents with a child to classify
+  int32_t nst[2];                        // M2L sources, M2L targets
+  int32_t nnarrow;                       // tasks of narrow blocks (numbered first)
+  unsigned long long tot[3];             // interaction-list, M2L and substituted entries
+  unsigned long long ntask, npairs;
+  int32_t p2p_rng[kRngCount];
+  // up-pass planning: M cells, P2M cells, M2M cells, L cells, L2L cells; the
+  // M2M cells before level d at 8 + d, the L2L cells before level d at 32 + d
+  int32_t reads[64];
+  unsigned long long terms[2];           // particles of the P2M cells, of the L2P cells
+  // M2L schedule (single pass): items of order index oi at [ord_items[oi][0], + ord_items[oi][1])
+  int32_t ord_items[kNumOrders][2];
+  int32_t n_items;
+  double flop_m2l, order_sum;
+  int32_t bin_cnt[kMaxBins];
+  int32_t bin_cur[kMaxBins];
+};
+
+// Sizes of a direct evaluation, kept to set the capacities of planned ones.
+struct PlanSizes {
+  int depth = -1;
+  int64_t n = 0, nc = 0, parents = 0;
+  int32_t lev[kMaxDepth + 1] = {};
+  int64_t n_m2l = 0, n_sub = 0, n_tgt = 0, ntask = 0, nnarrow = 0;
+  int64_t nM = 0, nL = 0, nP2M = 0;
+  int32_t m2m_lev[kMaxDepth + 1] = {}, l2l_lev[kMaxDepth + 1] = {};
+  int64_t items = 0;
+  int32_t items_ord[kNumOrders] = {};
+};
+
+// Capacities of a planned evaluation (every field a bound the kernels check).
+struct Caps {
+  bool on = false;
+  int depth = 0;
+  int64_t nc = 0, parents = 0;
+  int32_t lev[kMaxDepth + 1] = {};
+  int64_t n_m2l = 0, n_sub = 0, n_tgt = 0, ntask = 0, nnarrow = 0;
+  int64_t nM = 0, nL = 0, nP2M = 0;
+  int32_t m2m_lev[kMaxDepth + 1] = {}, l2l_lev[kMaxDepth + 1] = {};
+  int64_t items = 0;
+  int32_t items_ord[kNumOrders] = {};
+};
+Caps caps_from(const PlanSizes &s);
+
+// capacities as kernel arguments of the checks (by value)
+struct CapLevels {
+  int32_t v[kMaxDepth + 1];
+};
+
+// the planned evaluation of a step has overflowed: kernels after a check leave
+__device__ __forceinline__ bool plan_dead(const DevPlan *dp) {
+  return __builtin_amdgcn_readfirstlane(*(const volatile int32_t *)&dp->ovf) != 0;
+}
 
 // ---- the device-side state of one evaluation -------------------------------------
 struct Plan {
@@ -212,6 +302,25 @@ struct nbfmm_ctx {
   nbfmm::DBuf bat_off, bat_bb, bat_st, bat_x, bat_m, bat_a;
   int list_level = 1;
   bool p2p_started = false; // P2P of the current evaluation is enqueued
+  // the sizes of the current evaluation on the device, their pinned mirror
+  nbfmm::DBuf dplan;
+  nbfmm::DevPlan *hplan = nullptr;
+  nbfmm::Caps cap;        // capacities of planned evaluations (cap.on: this evaluation is planned)
+  nbfmm::PlanSizes last;  // sizes of the last direct evaluation
+  // graph-captured integration steps: the instantiated graph, what it was captured for
+  bool graph_on = false;
+  hipGraphExec_t gexec = nullptr;
+  struct GraphKey {
+    int kind = -1; // 0 SE, 1 RK4 reference semantics, 2 RK4 classic; + 3 for a batch
+    int64_t n = 0;
+    const void *x = nullptr, *v = nullptr, *a = nullptr, *m = nullptr, *off = nullptr;
+    double dt = 0;
+    int nsys = 0;
+    nbfmm_params prm{};
+    nbfmm_tiles shapes{};
+    uint64_t gen = 0; // buffer generation at the capture
+  } gkey;
+  int64_t g_captures = 0, g_launches = 0, g_redone = 0, g_direct = 0;
   // events
   hipEvent_t ev[16] = {};
   int nev = 0;
@@ -230,8 +339,24 @@ void build_lists(nbfmm_ctx *c);
 void nbr_leaf_rows(nbfmm_ctx *c, const int32_t *closed); // leaf-level rows of nbr27 (all when closed is null)
 // expansions.hip
 void run_expansions(nbfmm_ctx *c);
+void upload_consts(nbfmm_ctx *c); // the recurrence constants, once per context
 int padded_k(int p); // pad16((p+1)^2)
-LevelGeo level_geo(const Plan &P);
+LevelGeo level_geo(const Plan &P); // the host's copy of the geometry (direct evaluations)
+// the device plan of the context
+inline DevPlan *dplan(nbfmm_ctx *c) { return static_cast<DevPlan *>(c->dplan.p); }
+// cells of level d: exact (direct evaluation) or the capacity (planned)
+inline int64_t level_cells(nbfmm_ctx *c, int d) {
+  return c->cap.on ? c->cap.lev[d] : (int64_t)(c->plan.lvl_start[d + 1] - c->plan.lvl_start[d]);
+}
+// reads [field, field + bytes) of the device plan into its pinned mirror (stream-ordered, then waits)
+void read_plan(nbfmm_ctx *c, const void *field, size_t bytes, hipStream_t s);
+template <class F>
+inline void read_plan_field(nbfmm_ctx *c, F DevPlan::*f, hipStream_t s) {
+  read_plan(c, &(c->hplan->*f), sizeof(F), s);
+}
+// p[0, count) = v (a kernel: graph captures take no memset nodes)
+void fill_i32(hipStream_t s, void *p, int64_t count, int32_t v);
+inline void fill_zero(hipStream_t s, void *p, size_t bytes) { fill_i32(s, p, (int64_t)(bytes / 4), 0); }
 // order.hip
 void order_features(nbfmm_ctx *c, const LevelGeo &g);
 const int8_t *order_select(nbfmm_ctx *c, const LevelGeo &g);
@@ -250,10 +375,13 @@ void dataset_get(nbfmm_ctx *c, float *feat_h, int8_t *lab_h, int64_t cap);
 void dataset_put(nbfmm_ctx *c, int64_t n, const float *feat_h, const int8_t *lab_h);
 // m2l.hip
 void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p);
-// M2M (l2l false) or L2L (true) for `ncell` parents listed in `cells` (device)
-void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, int32_t ncell, const int32_t *cells);
+// M2M (l2l false) or L2L (true) for the parents cells[rng[0], rng[1]) (a
+// device range, one level's), launched for at most grid_cells of them
+void run_trans(nbfmm_ctx *c, bool l2l, int KP, bool f32, const int32_t *cells, const int32_t *rng,
+               int64_t grid_cells);
 // the same in FP64 on the expansions X (FP64, slot-indexed like the context's)
-void run_trans64(nbfmm_ctx *c, bool l2l, int KP, int32_t ncell, const int32_t *cells, double *X);
+void run_trans64(nbfmm_ctx *c, bool l2l, int KP, const int32_t *cells, const int32_t *rng, int64_t grid_cells,
+                 double *X);
 // count values between FP32 and FP64 arrays (stream-ordered)
 void widen(nbfmm_ctx *c, const float *in, int64_t count, double *out);
 void narrow(nbfmm_ctx *c, const double *in, int64_t count, float *out);
@@ -277,10 +405,12 @@ void run_direct(nbfmm_ctx *c, int64_t n, const double *d_xyz, const double *d_m,
 // returns it
 void scratch_init(int device);
 double *scratch_ptr(int device);
-// batch.hip: the systems [off[k], off[k+1]) of one concatenated set evaluated
-// as one forest; eval(n, x, m, a) runs the evaluation of the placed copy
-void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *h_off, const double *d_xyz, const double *d_m,
+// batch.hip: the systems [off[k], off[k+1]) of one concatenated set of n
+// particles evaluated as one forest (d_off: the offsets on the device, from
+// upload_offsets); eval(n, x, m, a) runs the evaluation of the placed copy
+void batch_eval(nbfmm_ctx *c, int nsys, const int64_t *d_off, int64_t n, const double *d_xyz, const double *d_m,
                 double *d_acc, const std::function<void(int64_t, const double *, const double *, double *)> &eval);
+int64_t *upload_offsets(nbfmm_ctx *c, int nsys, const int64_t *h_off);
 // integrate.hip
 void se_update(nbfmm_ctx *c, int64_t n, double *x, double *v, const double *a, double dt);
 void rk4_stage(nbfmm_ctx *c, int64_t n, const double *x, const double *v, const double *vk_prev,

@@ -459,7 +459,7 @@ __device__ __forceinline__ void narrow_task(const P2PTask &tk, const int32_t *__
 }
 
 template <class Tr, class TrFar, int KG, bool POT>
-__global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_waves<Tr, KG, POT>()))) k_p2p_tasks(int64_t ntask, const P2PTask *__restrict__ tasks,
+__global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_waves<Tr, KG, POT>()))) k_p2p_tasks(const DevPlan *__restrict__ dp, const int32_t *__restrict__ rng, const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
@@ -472,9 +472,11 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
   __shared__ int32_t lp[kMaxList];
   __shared__ int32_t tmp[54];
   const int lane = threadIdx.x;
+  if (plan_dead(dp)) return;
+  const int64_t t0 = rng[0], ntask = rng[1] - t0; // the tasks [rng[0], rng[1])
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
     __syncthreads(); // the previous task's reads of lb, lp, st are done
-    narrow_task<Tr, TrFar, KG, POT>(tasks[task], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot, st, lb, lp, tmp,
+    narrow_task<Tr, TrFar, KG, POT>(tasks[t0 + task], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot, st, lb, lp, tmp,
                                     lane);
   }
 }
@@ -605,7 +607,8 @@ __device__ __forceinline__ void wide_task(const P2PTask &tk, const int32_t *__re
 }
 
 template <class Tr, class TrFar, bool POT>
-__global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *__restrict__ tasks,
+__global__ void __launch_bounds__(kW) k_p2p_wide(const DevPlan *__restrict__ dp, const int32_t *__restrict__ rng,
+                                                  const P2PTask *__restrict__ tasks,
                                                   const int32_t *__restrict__ nbr27,
                                                   const int32_t *__restrict__ sub_src,
                                                   const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend,
@@ -618,23 +621,27 @@ __global__ void __launch_bounds__(kW) k_p2p_wide(int64_t ntask, const P2PTask *_
   __shared__ int32_t lp[kMaxList];
   __shared__ int32_t tmp[54];
   const int lane = threadIdx.x;
+  if (plan_dead(dp)) return;
+  const int64_t t0 = rng[0], ntask = rng[1] - t0; // the tasks [rng[0], rng[1])
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
     __syncthreads(); // the previous task's reads of lb, lp, sh are done
-    wide_task<Tr, TrFar, POT>(tasks[task], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot, sh, shf, lb, lp, tmp, lane);
+    wide_task<Tr, TrFar, POT>(tasks[t0 + task], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot, sh, shf, lb, lp, tmp, lane);
   }
 }
 
 // Persistent P2P over a work queue: a fixed grid of one-wavefront workgroups
 // takes tasks in queue order from an atomic counter until the queue is
 // drained, so one launch covers every task of the share (every system's, in
-// a batch) however uneven their sizes.  Queue index q < nwide is the wide
-// task wide0 + q (the larger ones first), the others the narrow tasks
-// narrow0 + (q - nwide).  A wavefront takes `chunk` consecutive tasks per
+// a batch) however uneven their sizes.  The share's wide tasks are
+// [rw[0], rw[1]), its narrow ones [rn[0], rn[1]) (device ranges); queue index
+// q < nwide is the wide task wide0 + q (the larger ones first), the others
+// the narrow tasks narrow0 + (q - nwide).  A wavefront takes `chunk` consecutive tasks per
 // atomic: one task per grab serialises some 10^5 atomics on one address.
 // *ctr is zero at the launch.
 template <class Tr, class TrFar, int KG, bool POT>
 __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_waves<Tr, KG, POT>()))) k_p2p_queue(
-    int64_t wide0, int64_t nwide, int64_t narrow0, int64_t nnarrow, int chunk, int32_t *__restrict__ ctr,
+    const DevPlan *__restrict__ dp, const int32_t *__restrict__ rw, const int32_t *__restrict__ rn, int chunk,
+    int32_t *__restrict__ ctr,
     const P2PTask *__restrict__ tasks, const int32_t *__restrict__ nbr27, const int32_t *__restrict__ sub_src,
     const int32_t *__restrict__ cbeg, const int32_t *__restrict__ cend, const uint64_t *__restrict__ ckey,
     const double4 *__restrict__ pos, double *__restrict__ grad, double *__restrict__ pot) {
@@ -649,7 +656,9 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
   __shared__ int32_t tmp[54];
   __shared__ int32_t next;
   const int lane = threadIdx.x;
-  const int64_t total = nwide + nnarrow;
+  if (plan_dead(dp)) return;
+  const int64_t wide0 = rw[0], nwide = rw[1] - rw[0], narrow0 = rn[0];
+  const int64_t total = nwide + (rn[1] - rn[0]);
   for (;;) {
     __syncthreads(); // the previous task's LDS reads are done
     if (lane == 0) next = atomicAdd(ctr, chunk);
@@ -791,8 +800,8 @@ __global__ void k_direct_combine(int64_t ntgt, int nchunk, const double *__restr
   }
 }
 
-typedef void (*P2PKernel)(int64_t, const P2PTask *, const int32_t *, const int32_t *, const int32_t *,
-                          const int32_t *, const uint64_t *, const double4 *, double *, double *);
+typedef void (*P2PKernel)(const DevPlan *, const int32_t *, const P2PTask *, const int32_t *, const int32_t *,
+                          const int32_t *, const int32_t *, const uint64_t *, const double4 *, double *, double *);
 
 // the narrow kernel's instance for a target group size (the sizes the tiles allow)
 P2PKernel narrow_kernel(bool f32, bool pot, int group) {
@@ -817,9 +826,9 @@ P2PKernel narrow_kernel(bool f32, bool pot, int group) {
   throw Error(NBFMM_ERR_ARG, "no P2P kernel for target group " + std::to_string(group));
 }
 
-typedef void (*P2PQueue)(int64_t, int64_t, int64_t, int64_t, int, int32_t *, const P2PTask *, const int32_t *,
-                         const int32_t *, const int32_t *, const int32_t *, const uint64_t *, const double4 *,
-                         double *, double *);
+typedef void (*P2PQueue)(const DevPlan *, const int32_t *, const int32_t *, int, int32_t *, const P2PTask *,
+                         const int32_t *, const int32_t *, const int32_t *, const int32_t *, const uint64_t *,
+                         const double4 *, double *, double *);
 
 // the queue kernel's instance for a target group size
 P2PQueue queue_kernel(bool f32, bool pot, int group) {
@@ -905,8 +914,56 @@ bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
   return false;
 }
 
-void fork_p2p(nbfmm_ctx *c) { NB_HIP(hipEventRecord(c->ev_fork, c->stream)); }
+// The task ranges of the P2P launches (DevPlan::p2p_rng) from the task
+// counts: the same share `early` of the narrow tasks [0, nn) and of the wide
+// ones [nn, ntask) goes first, its narrow tasks in nch launches.
+__global__ void k_p2p_ranges(DevPlan *dp, double early, int nch) {
+  if (threadIdx.x != 0) return;
+  const int64_t nn = dp->nnarrow, nt = (int64_t)dp->ntask, nw = nt - nn;
+  const int64_t sn = (int64_t)((double)nn * early), sw = nn + (int64_t)((double)nw * early);
+  int32_t *r = dp->p2p_rng;
+  for (int ch = 0; ch < nch; ++ch) {
+    r[2 * ch] = (int32_t)(sn * ch / nch);
+    r[2 * ch + 1] = (int32_t)(sn * (ch + 1) / nch);
+  }
+  const int32_t v[12] = {0, (int32_t)sn, (int32_t)nn, (int32_t)sw, (int32_t)sn, (int32_t)nn,
+                         (int32_t)sw, (int32_t)nt, 0, (int32_t)nn, (int32_t)nn, (int32_t)nt};
+  for (int k = 0; k < 12; ++k) r[kRngNarrow0 + k] = v[k];
+}
 
+// The shares as k_p2p_ranges computes them, for the launch sizes: exact (the
+// task counts of a direct evaluation) or from the capacities (planned; the
+// kernels stride over their tasks, so any grid covers them)
+struct P2PSplit {
+  int nch;
+  double early;
+  int64_t rng[kRngCount];
+};
+static P2PSplit p2p_split(nbfmm_ctx *c) {
+  const Plan &P = c->plan;
+  const bool f32 = c->prm.precision == NBFMM_MIXED;
+  const nbfmm_tiles &tl = c->shapes;
+  P2PSplit sp{};
+  sp.early = f32 ? tl.p2p_early_share_f32 : tl.p2p_early_share_f64;
+  sp.nch = std::min(kMaxChunks0, f32 ? tl.p2p_early_launches_f32 : tl.p2p_early_launches_f64);
+  const int64_t nn = c->cap.on ? c->cap.nnarrow : P.n_tasks_narrow;
+  const int64_t nt = c->cap.on ? c->cap.ntask : P.n_tasks, nw = std::max<int64_t>(0, nt - nn);
+  const int64_t sn = (int64_t)((double)nn * sp.early), sw = nn + (int64_t)((double)nw * sp.early);
+  for (int ch = 0; ch < sp.nch; ++ch) {
+    sp.rng[2 * ch] = sn * ch / sp.nch;
+    sp.rng[2 * ch + 1] = sn * (ch + 1) / sp.nch;
+  }
+  const int64_t v[12] = {0, sn, nn, sw, sn, nn, sw, nt, 0, nn, nn, nt};
+  for (int k = 0; k < 12; ++k) sp.rng[kRngNarrow0 + k] = v[k];
+  return sp;
+}
+
+void fork_p2p(nbfmm_ctx *c) {
+  const P2PSplit sp = p2p_split(c);
+  hipLaunchKernelGGL(k_p2p_ranges, dim3(1), dim3(64), 0, c->stream, dplan(c), sp.early, sp.nch);
+  NB_LAUNCH();
+  NB_HIP(hipEventRecord(c->ev_fork, c->stream));
+}
 
 void launch_p2p(nbfmm_ctx *c, int part) {
   // isolated timing: parts 0 and 1 are left out and run_finish runs every
@@ -917,21 +974,15 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   const bool pot = c->phi_out != nullptr;
   double *gn = dget<double>(c->grad_near, (size_t)P.n * (pot ? 4 : 3));
   double *pn = pot ? gn + 3 * (size_t)P.n : nullptr;
-  // the same share of the narrow tasks [0, nn) and of the wide ones [nn, n_tasks)
-  const int64_t nn = P.n_tasks_narrow, nw = P.n_tasks - nn;
   const bool f32 = c->prm.precision == NBFMM_MIXED;
   const nbfmm_tiles &tl = c->shapes;
-  const double early = f32 ? tl.p2p_early_share_f32 : tl.p2p_early_share_f64;
-  const int64_t sn = (int64_t)((double)nn * early), sw = nn + (int64_t)((double)nw * early);
-  int64_t r[2][2] = {{part == 0 ? 0 : sn, part == 0 ? sn : nn}, {part == 0 ? nn : sw, part == 0 ? sw : P.n_tasks}};
+  const DevPlan *dp = dplan(c);
+  const int32_t *rg = dp->p2p_rng;
+  const P2PSplit sp = p2p_split(c);
   if (part == 2) { // every task, one launch per kernel
-    r[0][0] = 0;
-    r[0][1] = nn;
-    r[1][0] = nn;
-    r[1][1] = P.n_tasks;
     NB_HIP(hipEventRecord(c->ev_fork2, c->stream));
     NB_HIP(hipStreamWaitEvent(s2, c->ev_fork2, 0));
-    NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
+    fill_zero(s2, gn, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n);
     mark2(c, 9);
     mark2(c, 10);
   }
@@ -943,24 +994,31 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   if (part == 0) {
     NB_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
     // near-field gradient (3 per particle) and, with potentials, near-field sums of m / r
-    NB_HIP(hipMemsetAsync(gn, 0, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n, s2));
+    fill_zero(s2, gn, sizeof(double) * (pot ? 4 : 3) * (size_t)P.n);
   }
   if (part != 2) mark2(c, part == 0 ? 9 : 11);
   if (part == 2) mark2(c, 11);
+  // this part's narrow and wide ranges (indices into p2p_rng)
+  const int in = part == 0 ? kRngNarrow0 : part == 1 ? kRngNarrow1 : kRngNarrowAll;
+  const int iw = part == 0 ? kRngWide0 : part == 1 ? kRngWide1 : kRngWideAll;
+  auto len = [&](int k) { return sp.rng[k + 1] - sp.rng[k]; };
+  auto grid_of = [&](int64_t cnt) { // a planned evaluation's counts are capacities: a quarter more
+    return std::max<int64_t>(1, std::min<int64_t>(cnt, ((int64_t)1 << 31) - 1));
+  };
   if (tl.p2p_queue == 2 || (tl.p2p_queue == 1 && part >= 1)) {
     // one persistent launch over this share's wide then narrow tasks
-    const int64_t nwide = r[1][1] - r[1][0], nnar = r[0][1] - r[0][0];
-    if (nwide + nnar > 0) {
+    const int64_t tot = len(iw) + len(in);
+    if (tot > 0 || c->cap.on) {
       int32_t *ctr = dget<int32_t>(c->p2p_ctr, 4) + part;
       hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s2, ctr, 1);
       NB_LAUNCH();
       // a CU holds 16 of these wavefronts (128 VGPRs each): 16 per CU fill the device
       const int per_cu = part == 0 ? tl.p2p_queue_waves0 : 16;
-      const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(nwide + nnar, (int64_t)c->n_cu * per_cu));
+      const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(tot, 1), (int64_t)c->n_cu * per_cu));
       const int chunk = tl.p2p_queue_chunk;
-      if (nwide + nnar + grid * chunk >= INT32_MAX) throw Error(NBFMM_ERR_ARG, "too many P2P tasks for the work queue");
+      if (tot + grid * chunk >= INT32_MAX) throw Error(NBFMM_ERR_ARG, "too many P2P tasks for the work queue");
       hipLaunchKernelGGL(queue_kernel(f32, pot, f32 ? tl.p2p_group_f32 : tl.p2p_group_f64), dim3((unsigned)grid),
-                         dim3(kW), 0, s2, r[1][0], nwide, r[0][0], nnar, chunk, ctr, (const P2PTask *)c->tasks.p,
+                         dim3(kW), 0, s2, dp, rg + iw, rg + in, chunk, ctr, (const P2PTask *)c->tasks.p,
                          (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
                          (const int32_t *)c->cend.p, (const uint64_t *)c->ckey.p, (const double4 *)c->pos.p, gn,
                          pn);
@@ -975,16 +1033,16 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   // whose narrow tasks are longer, measured slower that way)
   for (int wi = 0; wi < 2; ++wi) {
     const int w = (part == 1 && f32) ? 1 - wi : wi;
-    const int nch = (w == 0 && part == 0) ? (f32 ? tl.p2p_early_launches_f32 : tl.p2p_early_launches_f64) : 1;
+    const int nch = (w == 0 && part == 0) ? sp.nch : 1;
     for (int ch = 0; ch < nch; ++ch) {
-      const int64_t t0 = r[w][0] + (r[w][1] - r[w][0]) * ch / nch;
-      const int64_t t1 = r[w][0] + (r[w][1] - r[w][0]) * (ch + 1) / nch;
-      if (t1 <= t0) continue;
-      const int64_t grid = std::min<int64_t>(t1 - t0, ((int64_t)1 << 31) - 1);
+      // the range's index: part 0's narrow chunks, or the part's whole range
+      const int k = (w == 0 && part == 0) ? 2 * ch : (w == 0 ? in : iw);
+      const int64_t cnt = len(k);
+      if (cnt <= 0 && !c->cap.on) continue;
       auto kern = w == 0 ? narrow_kernel(f32, pot, f32 ? tl.p2p_group_f32 : tl.p2p_group_f64)
                          : (f32 ? (pot ? k_p2p_wide<PairF32, PairF32Far, true> : k_p2p_wide<PairF32, PairF32Far, false>)
                                 : (pot ? k_p2p_wide<PairF64, PairF64Far, true> : k_p2p_wide<PairF64, PairF64Far, false>));
-      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kW), 0, s2, t1 - t0, (const P2PTask *)c->tasks.p + t0,
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid_of(cnt)), dim3(kW), 0, s2, dp, rg + k, (const P2PTask *)c->tasks.p,
                          (const int32_t *)c->nbr27.p, (const int32_t *)c->sub_src.p, (const int32_t *)c->cbeg.p,
                          (const int32_t *)c->cend.p, (const uint64_t *)c->ckey.p, (const double4 *)c->pos.p, gn, pn);
       NB_LAUNCH();

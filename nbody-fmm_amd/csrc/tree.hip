@@ -18,6 +18,7 @@
 #include "nbfmm_internal.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 
 namespace nbfmm {
@@ -82,7 +83,10 @@ __global__ void __launch_bounds__(kT) k_bbox(const double *__restrict__ xyz, con
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void __launch_bounds__(kT) k_bbox_final(const double *part, int nb, double *out) {
+// The box (tree.cpp:20-30) into the device plan; with `cubic` the bounding
+// cube centred on it, widened outward where rounding would cut the box.
+__global__ void __launch_bounds__(kT) k_bbox_final(const double *part, int nb, int cubic, DevPlan *dp) {
+#pragma clang fp contract(off)
   __shared__ double s[6][kT];
   double v[6];
   for (int d = 0; d < 6; ++d) v[d] = part[d];
@@ -96,7 +100,28 @@ __global__ void __launch_bounds__(kT) k_bbox_final(const double *part, int nb, d
         s[d][threadIdx.x] = d < 3 ? fmin(s[d][threadIdx.x], s[d][threadIdx.x + o]) : fmax(s[d][threadIdx.x], s[d][threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x != 0) return;
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = s[d][0];
+    hi[d] = s[3 + d][0];
+  }
+  if (cubic) {
+    double ext = 0;
+    for (int d = 0; d < 3; ++d) ext = fmax(ext, hi[d] - lo[d]);
+    for (int d = 0; d < 3; ++d) {
+      const double cc = 0.5 * (lo[d] + hi[d]);
+      double a = cc - 0.5 * ext, b = cc + 0.5 * ext;
+      if (a > lo[d]) a = lo[d];
+      if (b < hi[d]) b = hi[d];
+      lo[d] = a;
+      hi[d] = b;
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    dp->lo[d] = lo[d];
+    dp->hi[d] = hi[d];
+  }
 }
 
 // ---- quantisation + Morton key (tree.cpp:55-61) -------------------------------
@@ -114,11 +139,14 @@ __device__ __forceinline__ uint32_t quantize(double x, double a, double ls, uint
   return c >= nld ? nld - 1 : c;
 }
 
-__global__ void __launch_bounds__(kT) k_keys(const double *__restrict__ xyz, int64_t n, double a0, double a1,
-                                             double a2, double l0, double l1, double l2, uint32_t nld,
-                                             uint64_t *__restrict__ key, int32_t *__restrict__ idx) {
+// leaf size per dimension (hi - lo) / 2^depth (IEEE division) from the plan's box
+__global__ void __launch_bounds__(kT) k_keys(const double *__restrict__ xyz, int64_t n, const DevPlan *__restrict__ dp,
+                                             uint32_t nld, uint64_t *__restrict__ key, int32_t *__restrict__ idx) {
   int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
+  const double a0 = dp->lo[0], a1 = dp->lo[1], a2 = dp->lo[2];
+  const double l0 = __ddiv_rn(dp->hi[0] - a0, (double)nld), l1 = __ddiv_rn(dp->hi[1] - a1, (double)nld),
+               l2 = __ddiv_rn(dp->hi[2] - a2, (double)nld);
   uint32_t cx = quantize(xyz[3 * i], a0, l0, nld);
   uint32_t cy = quantize(xyz[3 * i + 1], a1, l1, nld);
   uint32_t cz = quantize(xyz[3 * i + 2], a2, l2, nld);
@@ -335,9 +363,97 @@ __global__ void __launch_bounds__(kOffThreads) k_level_offsets(int32_t *bcnt, co
   }
 }
 
-struct LevelStarts {
-  int32_t s[kMaxDepth + 2];
-};
+// The plan of the tree sorted at depth D, from the last level count: level
+// starts, cell count, per-level geometry, the operator geometry and the mask
+// of offset classes evaluated directly (the box's aspect ratio, lists.hip).
+// A planned evaluation (check != 0) also replays the reference's depth loop
+// (tree.cpp:32-69) on the levels' fullest cells and flags a depth other than
+// D, more cells than the capacities, and a non-finite input.  One workgroup
+// of 512 threads.
+__global__ void __launch_bounds__(512) k_tree_plan(DevPlan *dp, int D, int64_t n, uint32_t thr, int list_level,
+                                                   int check, int64_t cap_nc, CapLevels cap_lev) {
+#pragma clang fp contract(off)
+  __shared__ uint32_t maskw[(kNumClasses + 31) / 32];
+  __shared__ int32_t ls[kMaxDepth + 2];
+  const int t = threadIdx.x;
+  const uint32_t *cnt = dp->lcount;
+  const double lo[3] = {dp->lo[0], dp->lo[1], dp->lo[2]}, hi[3] = {dp->hi[0], dp->hi[1], dp->hi[2]};
+  if (t == 0) {
+    int32_t s = 0;
+    ls[0] = 0;
+    for (int d = 0; d <= D; ++d) {
+      s += (int32_t)cnt[d];
+      ls[d + 1] = s;
+    }
+    for (int d = D + 2; d < kMaxDepth + 2; ++d) ls[d] = s;
+    dp->nc = s;
+    dp->one_nc[0] = 1;
+    dp->one_nc[1] = s;
+    dp->depth = D;
+    if (check) {
+      int ovf = 0;
+      // the loop as the host runs it from the depth the keys were sorted at
+      auto more = [&](int depth, uint32_t maxcnt) {
+        return depth < kMaxDepth && (maxcnt > thr || depth < list_level);
+      };
+      int depth = 0, it = 0;
+      uint32_t maxcnt = (uint32_t)n;
+      for (;;) {
+        const uint64_t nl = (uint64_t)((maxcnt - 1u) / thr + 1u);
+        const int bw = nl ? 64 - __clzll((long long)nl) : 0;
+        const int nd = min(kMaxDepth, depth + bw / 3 + 1);
+        if (nd > D) break;
+        depth = nd;
+        maxcnt = cnt[kMaxDepth + 1 + depth];
+        ++it;
+        if (!more(depth, maxcnt)) break;
+      }
+      if (more(depth, maxcnt) || depth != D) ovf |= 1;
+      if ((int64_t)s > cap_nc) ovf |= 2;
+      for (int d = 0; d <= D; ++d)
+        if ((int32_t)cnt[d] > cap_lev.v[d]) ovf |= 2;
+      if (dp->bad) ovf |= 4;
+      dp->iterations = it;
+      if (ovf) dp->ovf |= ovf;
+    }
+  }
+  // per-level geometry: h = (hi - lo) / 2^l, rho the largest edge (1 for a point)
+  LevelGeo &g = dp->g;
+  if (t < 3) g.lo[t] = lo[t];
+  if (t <= D) {
+    double r = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double h = (hi[d] - lo[d]) / (double)(1ull << t);
+      g.h[t][d] = h;
+      if (d == 0 || h > r) r = h;
+    }
+    g.inv_rho[t] = 1.0 / (r > 0 ? r : 1.0);
+    if (t == D) {
+      const double rho = r > 0 ? r : 1.0;
+      for (int d = 0; d < 3; ++d) dp->hs[d] = (hi[d] - lo[d]) / (double)(1ull << t) / rho;
+    }
+  }
+  if (t == 0) g.depth = D;
+  __syncthreads(); // ls written
+  if (t < kMaxDepth + 2) {
+    dp->lvl_start[t] = ls[t];
+    g.lvl_start[t] = ls[t];
+  }
+  // offset classes (lists.hip: class 0 is the slowest admissible one)
+  if (t < (kNumClasses + 31) / 32) maskw[t] = 0;
+  const double hx = hi[0] - lo[0], hy = hi[1] - lo[1], hz = hi[2] - lo[2];
+  const double diag2 = hx * hx + hy * hy + hz * hz;
+  __syncthreads();
+  if (t < kNumClasses) {
+    const int dx = t % 7 - 3, dy = (t / 7) % 7 - 3, dz = t / 49 - 3;
+    if (max(abs(dx), max(abs(dy), abs(dz))) >= 2) {
+      const double d2 = dx * dx * hx * hx + dy * dy * hy * hy + dz * dz * hz * hz;
+      if (diag2 > kMacRatio * kMacRatio * d2) atomicOr(&maskw[t >> 5], 1u << (t & 31));
+    }
+  }
+  __syncthreads();
+  if (t < (kNumClasses + 31) / 32) dp->bad_cls.w[t] = maskw[t];
+}
 
 // Every level's cells at once: particle i finds the index of its cell at each
 // level from the block offsets and the ranks within its block; where it starts
@@ -345,10 +461,12 @@ struct LevelStarts {
 // level's previous cell.  Levels are concatenated from the root; keys of level
 // d are the leaf keys >> 3(D - d).
 __global__ void __launch_bounds__(kT) k_cells_fill(const uint64_t *__restrict__ key, int64_t n, int D,
-                                                   const int32_t *__restrict__ boff, LevelStarts L,
+                                                   const int32_t *__restrict__ boff, const DevPlan *__restrict__ dp,
                                                    uint64_t *__restrict__ ckey, int32_t *__restrict__ cbeg,
                                                    int32_t *__restrict__ cend, int32_t *__restrict__ cpar,
                                                    int32_t *__restrict__ pcell) {
+  if (plan_dead(dp)) return;
+  const int32_t *__restrict__ Ls = dp->lvl_start;
   __shared__ int32_t w[kT / 64][kMaxDepth + 1];
   __shared__ int32_t bo[kMaxDepth + 1];
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -367,12 +485,12 @@ __global__ void __launch_bounds__(kT) k_cells_fill(const uint64_t *__restrict__ 
     const uint64_t m = __ballot(d0 <= d);
     int32_t pre = 0;
     for (int q = 0; q < wv; ++q) pre += w[q][d];
-    const int32_t cell = L.s[d] + bo[d] + pre + (int32_t)__popcll(m & upto) - 1;
+    const int32_t cell = Ls[d] + bo[d] + pre + (int32_t)__popcll(m & upto) - 1;
     if (d0 <= d) {
       ckey[cell] = k >> (3 * (D - d));
       cbeg[cell] = (int32_t)i;
       cpar[cell] = up;
-      if (cell > L.s[d]) cend[cell - 1] = (int32_t)i;
+      if (cell > Ls[d]) cend[cell - 1] = (int32_t)i;
     }
     if (i == n - 1) cend[cell] = (int32_t)n;
     up = cell;
@@ -380,10 +498,12 @@ __global__ void __launch_bounds__(kT) k_cells_fill(const uint64_t *__restrict__ 
   if (i < n) pcell[i] = up;
 }
 
-__global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent, int32_t cb,
-                         int32_t ce, int32_t *child8) {
-  int32_t c = cb + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= ce) return;
+// cells [rng[0], rng[1]) (device range) and the plan's guard
+__global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent,
+                         const int32_t *__restrict__ rng, const DevPlan *__restrict__ dp, int32_t *child8) {
+  if (plan_dead(dp)) return;
+  int32_t c = rng[0] + (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= rng[1]) return;
   child8[(int64_t)cparent[c] * 8 + (int)(ckey[c] & 7u)] = c;
 }
 
@@ -391,11 +511,12 @@ __global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__res
 // one thread per table entry; with `skip`, rows of cells with skip[c] != 0
 // are left unwritten.
 __global__ void k_nbr27(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent,
-                        const int32_t *__restrict__ child8, int32_t cb, int32_t ce, int lev,
+                        const int32_t *__restrict__ child8, const DevPlan *__restrict__ dp, int lev,
                         const int32_t *__restrict__ skip, int32_t *nbr27) {
+  if (plan_dead(dp)) return;
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
-  const int32_t c = cb + (int32_t)(e / 27);
-  if (c >= ce || (skip && skip[c])) return;
+  const int32_t c = dp->lvl_start[lev] + (int32_t)(e / 27);
+  if (c >= dp->lvl_start[lev + 1] || (skip && skip[c])) return;
   const int o = (int)(e % 27);
   const uint64_t k = ckey[c];
   const int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
@@ -441,15 +562,16 @@ int bit_width_u64(uint64_t v) {
 
 } // namespace
 
-// Cells per level of the keys sorted at depth D (buffer cur): P.lvl_start
-// and, in lm[0..D], every level's fullest cell (one transfer, one sync).
-// The per-block offsets stay in c->flags for fill_levels.
-static void count_levels(nbfmm_ctx *c, int D, int cur, double *out6, double *h, uint32_t *lm) {
+// Cells per level of the keys sorted at depth D (buffer cur), and every
+// level's fullest cell, into the device plan.  A direct evaluation reads them
+// (P.lvl_start, lm: one transfer, one sync); a planned one does not.  The
+// per-block offsets stay in c->flags for fill_levels.
+static void count_levels(nbfmm_ctx *c, int D, int cur, uint32_t *lm) {
   Plan &P = c->plan;
   const int64_t n = P.n;
   hipStream_t s = c->stream;
+  DevPlan *dp = dplan(c);
   const uint64_t *key = (const uint64_t *)c->key[cur].p;
-  uint32_t *dcounts = (uint32_t *)(out6 + 16); // counts, then maxima at kMaxDepth + 1
   const int64_t nbl = nblk(n);
   const size_t per = (size_t)(D + 1) * (size_t)nbl;
   int32_t *bcnt = dget<int32_t>(c->flags, std::max<size_t>((size_t)n + 1, per));
@@ -458,11 +580,11 @@ static void count_levels(nbfmm_ctx *c, int D, int cur, double *out6, double *h, 
                      aux + 2 * per);
   NB_LAUNCH();
   hipLaunchKernelGGL(k_level_offsets, dim3(D + 1), dim3(kOffThreads), 0, s, bcnt, (const int32_t *)aux,
-                     (const int32_t *)(aux + per), (const int32_t *)(aux + 2 * per), (int)nbl, n, dcounts);
+                     (const int32_t *)(aux + per), (const int32_t *)(aux + 2 * per), (int)nbl, n, dp->lcount);
   NB_LAUNCH();
-  uint32_t *counts = (uint32_t *)(h + 16); // pinned
-  NB_HIP(hipMemcpyAsync(counts, dcounts, sizeof(uint32_t) * 2 * (kMaxDepth + 1), hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
+  if (c->cap.on) return;
+  read_plan_field(c, &DevPlan::lcount, s);
+  const uint32_t *counts = c->hplan->lcount;
   P.lvl_start.assign(D + 2, 0);
   for (int d = 0; d <= D; ++d) {
     P.lvl_start[d + 1] = P.lvl_start[d] + (int32_t)counts[d];
@@ -476,6 +598,7 @@ static void fill_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, in
   Plan &P = c->plan;
   const int64_t n = P.n;
   hipStream_t s = c->stream;
+  DevPlan *dp = dplan(c);
   P.geo.depth = depth;
   if (cur != 0) { // keep the sorted keys/indices in buffer 0
     std::swap(c->key[0], c->key[1]);
@@ -497,35 +620,33 @@ static void fill_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, in
   const int D = depth;
   const int64_t nbl = nblk(n);
   const int32_t *bcnt = (const int32_t *)c->flags.p;
-  P.n_leaves = P.lvl_start[D + 1] - P.lvl_start[D];
-  const int64_t nc = P.lvl_start[D + 1];
-  P.n_cells = nc;
+  if (!c->cap.on) {
+    P.n_leaves = P.lvl_start[D + 1] - P.lvl_start[D];
+    P.n_cells = P.lvl_start[D + 1];
+  }
+  const int64_t nc = c->cap.on ? c->cap.nc : P.n_cells; // cells, or their capacity
   uint64_t *ckey = dget<uint64_t>(c->ckey, (size_t)nc);
   int32_t *cbeg = dget<int32_t>(c->cbeg, (size_t)nc);
   int32_t *cend = dget<int32_t>(c->cend, (size_t)nc);
   int32_t *cpar = dget<int32_t>(c->cparent, (size_t)nc);
   int32_t *pcell = dget<int32_t>(c->pcell, (size_t)n);
-  LevelStarts L{};
-  for (int d = 0; d <= D + 1; ++d) L.s[d] = P.lvl_start[d];
-  hipLaunchKernelGGL(k_cells_fill, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt, L, ckey, cbeg, cend, cpar,
-                     pcell);
+  hipLaunchKernelGGL(k_cells_fill, dim3((unsigned)nbl), dim3(kT), 0, s, key, n, D, bcnt, (const DevPlan *)dp, ckey,
+                     cbeg, cend, cpar, pcell);
   NB_LAUNCH();
   // neighbour tables
   int32_t *child8 = dget<int32_t>(c->child8, (size_t)nc * 8);
   int32_t *nbr27 = dget<int32_t>(c->nbr27, (size_t)nc * 27);
-  NB_HIP(hipMemsetAsync(child8, 0xff, sizeof(int32_t) * (size_t)nc * 8, s));
-  if (nc > 1) {
-    hipLaunchKernelGGL(k_child8, dim3(nblk(nc - 1)), dim3(kT), 0, s, ckey, cpar, 1, (int32_t)nc, child8);
-    NB_LAUNCH();
-  }
-  NB_HIP(hipMemsetAsync(nbr27, 0xff, sizeof(int32_t) * 27, s));
-  NB_HIP(hipMemsetAsync(nbr27 + 13, 0, sizeof(int32_t), s));
+  fill_i32(s, child8, nc * 8, -1);
+  hipLaunchKernelGGL(k_child8, dim3(nblk(nc - 1)), dim3(kT), 0, s, ckey, cpar, (const int32_t *)dp->one_nc,
+                     (const DevPlan *)dp, child8);
+  NB_LAUNCH();
+  fill_i32(s, nbr27, 27, -1);
+  fill_i32(s, nbr27 + 13, 1, 0);
   // the leaf level's rows follow in build_lists, for the leaves that are not
   // closed (the only ones the evaluation reads); nbr_leaf_rows completes them
   for (int d = 1; d < D; ++d) {
-    int32_t cb = P.lvl_start[d], ce = P.lvl_start[d + 1];
-    hipLaunchKernelGGL(k_nbr27, dim3(nblk((int64_t)(ce - cb) * 27)), dim3(kT), 0, s, ckey, cpar, child8, cb, ce, d,
-                       nullptr, nbr27);
+    hipLaunchKernelGGL(k_nbr27, dim3(nblk(level_cells(c, d) * 27)), dim3(kT), 0, s, ckey, cpar, child8,
+                       (const DevPlan *)dp, d, nullptr, nbr27);
     NB_LAUNCH();
   }
   c->nbr_leaf = D == 0 ? 2 : 0;
@@ -536,35 +657,23 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   Plan &P = c->plan;
   const int64_t n = P.n;
   hipStream_t s = c->stream;
+  const Caps &K = c->cap;
+  DevPlan *dp = dplan(c);
   // ---- bbox (tree.cpp:20-30) ----
   int nbb = (int)std::min<int64_t>(1024, (n + kT - 1) / kT);
-  double *red = dget<double>(c->red, 8 * 1024 + 64);
-  double *out6 = red + 6 * 1024;
-  int *bad = (int *)(out6 + 8);
-  NB_HIP(hipMemsetAsync(bad, 0, sizeof(int) * 4, s));
-  hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(kT), 0, s, d_xyz, d_m, n, red, bad);
+  double *red = dget<double>(c->red, 6 * 1024 + 64);
+  fill_i32(s, &dp->bad, 1, 0);
+  hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(kT), 0, s, d_xyz, d_m, n, red, &dp->bad);
   NB_LAUNCH();
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kT), 0, s, red, nbb, out6);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kT), 0, s, (const double *)red, nbb, (int)c->prm.cubic_box, dp);
   NB_LAUNCH();
-  double *h = (double *)dbuf_ensure(c->hred, 0); // pinned host mirror
-  NB_HIP(hipMemcpyAsync(h, out6, sizeof(double) * 8 + sizeof(int) * 4, hipMemcpyDeviceToHost, s));
-  NB_HIP(hipStreamSynchronize(s));
-  if (((int *)(h + 8))[0]) throw Error(NBFMM_ERR_DEGENERATE, "non-finite coordinate or mass");
-  for (int d = 0; d < 3; ++d) {
-    P.geo.lo[d] = h[d];
-    P.geo.hi[d] = h[3 + d];
-  }
-  if (c->prm.cubic_box) {
-    // the bounding cube centred on the box (cubic cells), never smaller than the box
-    double ext = 0;
-    for (int d = 0; d < 3; ++d) ext = std::max(ext, P.geo.hi[d] - P.geo.lo[d]);
+  if (!K.on) {
+    // the box and the finiteness flag (adjacent in the plan)
+    read_plan(c, &c->hplan->lo[0], offsetof(DevPlan, bad) + sizeof(int32_t), s);
+    if (c->hplan->bad) throw Error(NBFMM_ERR_DEGENERATE, "non-finite coordinate or mass");
     for (int d = 0; d < 3; ++d) {
-      const double cc = 0.5 * (P.geo.lo[d] + P.geo.hi[d]);
-      double lo = cc - 0.5 * ext, hi = cc + 0.5 * ext;
-      if (lo > P.geo.lo[d]) lo = P.geo.lo[d];
-      if (hi < P.geo.hi[d]) hi = P.geo.hi[d];
-      P.geo.lo[d] = lo;
-      P.geo.hi[d] = hi;
+      P.geo.lo[d] = c->hplan->lo[d];
+      P.geo.hi[d] = c->hplan->hi[d];
     }
   }
   // ---- depth loop (tree.cpp:32-69) ----
@@ -579,15 +688,12 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
   P.depth_sorts = 0;
   // Morton keys at `depth`, sorted (buffer `cur`), and their cells per level
   auto sort_at = [&](int depth) {
-    const uint64_t nld = 1ull << depth;
-    double ls[3];
-    for (int d = 0; d < 3; ++d) ls[d] = (P.geo.hi[d] - P.geo.lo[d]) / (double)nld;
-    hipLaunchKernelGGL(k_keys, dim3(nblk(n)), dim3(kT), 0, s, d_xyz, n, P.geo.lo[0], P.geo.lo[1], P.geo.lo[2],
-                       ls[0], ls[1], ls[2], (uint32_t)nld, (uint64_t *)c->key[0].p, (int32_t *)c->idx[0].p);
+    hipLaunchKernelGGL(k_keys, dim3(nblk(n)), dim3(kT), 0, s, d_xyz, n, (const DevPlan *)dp, (uint32_t)(1u << depth),
+                       (uint64_t *)c->key[0].p, (int32_t *)c->idx[0].p);
     NB_LAUNCH();
     cur = radix_sort(c, n, 3 * depth, 0);
     P.depth_sorts++;
-    count_levels(c, depth, cur, out6, h, lm);
+    count_levels(c, depth, cur, lm);
   };
   // the reference's step: the depth after `depth` when its fullest cell holds maxcnt
   auto next_depth = [&](int depth, uint32_t maxcnt) {
@@ -601,38 +707,49 @@ void build_tree(nbfmm_ctx *c, const double *d_xyz, const double *d_m) {
     return depth < kMaxDepth && (maxcnt > thr || depth < c->list_level);
   };
   int depth = 0;
-  uint32_t maxcnt = (uint32_t)n;
-  bool done = false;
-  const int guess = c->depth_guess;
-  if (guess >= 1 && guess <= kMaxDepth) {
-    // Sort once at the depth the previous evaluation ended at.  A level-d
-    // cell is a run of the level-d key, the sorted key's top 3d bits, so the
-    // cell counts give the fullest cell of every level <= guess, and the loop
-    // is replayed on them.  It ends at `guess` (the usual case: particles move
-    // little between evaluations), before it (sorted again at that depth) or
-    // beyond it (the loop goes on from its last depth <= guess); its result
-    // is the loop's in every case.
-    sort_at(guess);
-    for (;;) {
-      const int nd = next_depth(depth, maxcnt);
-      if (nd > guess) break;
-      depth = nd;
+  if (K.on) {
+    // planned: the keys sorted once at the planned depth; k_tree_plan replays
+    // the loop on the levels' fullest cells and flags another depth
+    depth = K.depth;
+    sort_at(depth);
+  } else {
+    uint32_t maxcnt = (uint32_t)n;
+    bool done = false;
+    const int guess = c->depth_guess;
+    if (guess >= 1 && guess <= kMaxDepth) {
+      // Sort once at the depth the previous evaluation ended at.  A level-d
+      // cell is a run of the level-d key, the sorted key's top 3d bits, so the
+      // cell counts give the fullest cell of every level <= guess, and the loop
+      // is replayed on them.  It ends at `guess` (the usual case: particles move
+      // little between evaluations), before it (sorted again at that depth) or
+      // beyond it (the loop goes on from its last depth <= guess); its result
+      // is the loop's in every case.
+      sort_at(guess);
+      for (;;) {
+        const int nd = next_depth(depth, maxcnt);
+        if (nd > guess) break;
+        depth = nd;
+        maxcnt = lm[depth];
+        P.iterations++;
+        if (!more(depth, maxcnt)) break;
+      }
+      if (!more(depth, maxcnt)) {
+        done = true;
+        if (depth != guess) sort_at(depth);
+      }
+    }
+    while (!done) {
+      depth = next_depth(depth, maxcnt);
+      sort_at(depth);
       maxcnt = lm[depth];
       P.iterations++;
-      if (!more(depth, maxcnt)) break;
-    }
-    if (!more(depth, maxcnt)) {
-      done = true;
-      if (depth != guess) sort_at(depth);
+      done = !more(depth, maxcnt);
     }
   }
-  while (!done) {
-    depth = next_depth(depth, maxcnt);
-    sort_at(depth);
-    maxcnt = lm[depth];
-    P.iterations++;
-    done = !more(depth, maxcnt);
-  }
+  CapLevels cl{};
+  for (int d = 0; d <= kMaxDepth; ++d) cl.v[d] = K.lev[d];
+  hipLaunchKernelGGL(k_tree_plan, dim3(1), dim3(512), 0, s, dp, depth, n, thr, c->list_level, K.on ? 1 : 0, K.nc, cl);
+  NB_LAUNCH();
   fill_levels(c, d_xyz, d_m, depth, cur);
   c->depth_guess = depth;
 }
@@ -643,10 +760,9 @@ void nbr_leaf_rows(nbfmm_ctx *c, const int32_t *closed) {
   Plan &P = c->plan;
   const int D = P.geo.depth;
   if (c->nbr_leaf == 2 || D == 0 || (closed && c->nbr_leaf == 1)) return;
-  const int32_t cb = P.lvl_start[D], ce = P.lvl_start[D + 1];
-  hipLaunchKernelGGL(k_nbr27, dim3(nblk((int64_t)(ce - cb) * 27)), dim3(kT), 0, c->stream, (const uint64_t *)c->ckey.p,
-                     (const int32_t *)c->cparent.p, (const int32_t *)c->child8.p, cb, ce, D, closed,
-                     (int32_t *)c->nbr27.p);
+  hipLaunchKernelGGL(k_nbr27, dim3(nblk(level_cells(c, D) * 27)), dim3(kT), 0, c->stream, (const uint64_t *)c->ckey.p,
+                     (const int32_t *)c->cparent.p, (const int32_t *)c->child8.p, (const DevPlan *)dplan(c), D,
+                     closed, (int32_t *)c->nbr27.p);
   NB_LAUNCH();
   c->nbr_leaf = closed ? 1 : 2;
 }

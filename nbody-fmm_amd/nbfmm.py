@@ -116,6 +116,7 @@ EXPORTS = [
     "nbfmm_order_predict", "nbfmm_eval_accel_checked_dev",
     "nbfmm_batch_eval_accel_dev", "nbfmm_batch_step_se_dev", "nbfmm_batch_step_rk4_dev",
     "nbfmm_direct_accel_noalloc_dev", "nbfmm_set_tiles", "nbfmm_get_tiles",
+    "nbfmm_set_graph", "nbfmm_graph_stats",
 ]
 NOALLOC_SCRATCH_BYTES = 48 << 20
 NOALLOC_MAX_TARGETS = 1 << 20
@@ -169,6 +170,8 @@ def lib():
         L.nbfmm_batch_eval_accel_dev.argtypes = [vp, i32, vp, vp, vp, vp]
         L.nbfmm_batch_step_se_dev.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, ctypes.c_int]
         L.nbfmm_batch_step_rk4_dev.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, ctypes.c_int, ctypes.c_int]
+        L.nbfmm_set_graph.argtypes = [vp, ctypes.c_int]
+        L.nbfmm_graph_stats.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -357,6 +360,17 @@ class Context:
     def batch_step_rk4_dev(self, offsets, dx, dv, da, dm, dt, nsteps, mode):
         off = np.ascontiguousarray(offsets, dtype=np.int64)
         _check(lib().nbfmm_batch_step_rk4_dev(self.h, len(off) - 1, _ptr(off), dx, dv, da, dm, dt, nsteps, mode))
+
+    def set_graph(self, on):
+        """Integration steps captured once as a HIP graph and replayed (1), or run directly (0)."""
+        _check(lib().nbfmm_set_graph(self.h, int(on)))
+
+    def graph_stats(self):
+        """{captures, launches, redone, direct}: graphs captured, steps replayed, replays whose sizes
+        overflowed (run again directly), steps run directly."""
+        out = np.zeros(4, dtype=np.int64)
+        _check(lib().nbfmm_graph_stats(self.h, _ptr(out)))
+        return dict(zip(("captures", "launches", "redone", "direct"), (int(v) for v in out)))
 
     def stats(self):
         s = Stats()
