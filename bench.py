@@ -111,6 +111,49 @@ def timed(ctx, n, dx, dm, da, steps, warmup):
     return wall, ctx.stats()
 
 
+def rk4_direct_vs_graph(ctx, n, xyz, vel, dm, da, nsteps, dt=1e-4, off=None):
+    """nsteps classic RK4 steps from (xyz, vel), run directly and replayed from
+    a captured graph (nbfmm_set_graph): each run takes one untimed step first
+    (the graph run's first step runs directly and captures the graph), then
+    nsteps timed steps in one call.  Returns ms per step of both, the graph
+    statistics and the relative difference of the two final states."""
+    out = {}
+    states = {}
+    for graph in (0, 1):
+        cx, cv = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(vel)
+
+        def steps(k):
+            if off is None:
+                ctx.step_rk4_dev(n, cx.ptr, cv.ptr, da.ptr, dm.ptr, dt, k, 1)
+            else:
+                ctx.batch_step_rk4_dev(off, cx.ptr, cv.ptr, da.ptr, dm.ptr, dt, k, 1)
+        ctx.set_graph(graph)
+        try:
+            before = ctx.graph_stats()
+            steps(1)
+            sync(ctx)
+            t0 = time.perf_counter()
+            steps(nsteps)
+            sync(ctx)
+            ms = (time.perf_counter() - t0) * 1e3 / nsteps
+            after = ctx.graph_stats()
+        finally:
+            ctx.set_graph(0)
+        states[graph] = (cx.download(ctx.stream), cv.download(ctx.stream))
+        cx.free()
+        cv.free()
+        out["graph_ms_per_step" if graph else "direct_ms_per_step"] = ms
+        if graph:
+            out["graph_stats"] = {k: after[k] - before[k] for k in after}
+    (xd, vd), (xg, vg) = states[0], states[1]
+    out["graph_vs_direct_rel_diff"] = {
+        "displacement": float(np.linalg.norm(xg - xd) / np.linalg.norm(xd - xyz)),
+        "velocity_change": float(np.linalg.norm(vg - vd) / np.linalg.norm(vd - vel))}
+    out["graph_speedup"] = out["direct_ms_per_step"] / out["graph_ms_per_step"]
+    out["steps"] = nsteps
+    return out
+
+
 MS_FIELDS = ("ms_tree", "ms_lists", "ms_p2m", "ms_m2m", "ms_m2l", "ms_l2l", "ms_l2p", "ms_p2p", "ms_total",
              "ms_m2l_kernel", "ms_p2p_kernel")
 
@@ -381,6 +424,7 @@ def bench_1m(args):
         rec = ctx.run_dev(n, dx.ptr, dv.ptr, da.ptr, dm.ptr, 1e-4, 5, integrator=2, every=5)
         rk4_ms = (time.perf_counter() - t0) * 1e3 / 5
         dE = float(abs(rec[-1, 2] - rec[0, 2]) / abs(rec[0, 2]))
+        graph = rk4_direct_vs_graph(ctx, n, xyz, vel, dm, da, 5)
     finally:
         ctx.close()
     r = {"workload": "Plummer N=1,048,576, plain FP64, p=10, leaf capacity 64, P2P cut 1024",
@@ -388,9 +432,11 @@ def bench_1m(args):
          "err": err, "depth": st.depth, "rk4_classic_ms_per_step": rk4_ms,
          "rk4_energy_drift_5_steps": dE,
          "rk4_note": "5 classic RK4 steps dt=1e-4 (4 evaluations each) with the energy monitor before and after "
-                     "(two extra evaluations with potentials inside the timed span)"}
+                     "(two extra evaluations with potentials inside the timed span)",
+         "rk4_graph": graph}
     log(f"[bench] plummer1m fp64: {r['ms_per_step']:.2f} ms/eval, L2 err {err['l2']:.3e}, RK4 {rk4_ms:.1f} ms/step, "
-        f"|dE/E| {dE:.1e}")
+        f"|dE/E| {dE:.1e}; RK4 direct {graph['direct_ms_per_step']:.2f} ms/step, graph "
+        f"{graph['graph_ms_per_step']:.2f} ms/step {graph['graph_stats']}, rel diff {graph['graph_vs_direct_rel_diff']}")
     return r
 
 
@@ -465,11 +511,17 @@ def bench_batched(args, cfg, precs, quick=False):
         ok = bool(np.all(np.isfinite(cx.download(ctx.stream))))
         cx.free()
         vx.free()
+        # the same 20 steps replayed from a captured graph (one persistent step graph)
+        graph = rk4_direct_vs_graph(ctx, n, xyz, vel, dm, da, 20, off=off)
+        log(f"[bench] batched {PREC_NAMES[prec]} RK4: direct {graph['direct_ms_per_step']:.2f} ms/step, graph "
+            f"{graph['graph_ms_per_step']:.2f} ms/step {graph['graph_stats']}, rel diff "
+            f"{graph['graph_vs_direct_rel_diff']}")
         results[PREC_NAMES[prec]] = {"value": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
                                      "order": best["order"], "p2p_cut": best["p2p_cut"],
                                      "worst_system_l2": max(e), "mean_system_l2": float(np.mean(e)),
                                      "meets_gate": max(e) <= args.gate, "rk4_ms_per_step": rk4_ms,
-                                     "rk4_steps": 20, "rk4_finite": ok, "tuning": table, "tiles": tiles,
+                                     "rk4_steps": 20, "rk4_finite": ok, "rk4_graph": graph,
+                                     "tuning": table, "tiles": tiles,
                                      "tile_tuning": tile_table}
         log(f"[bench] batched {PREC_NAMES[prec]}: {results[PREC_NAMES[prec]]['ms_per_step']:.2f} ms/eval, "
             f"worst system {max(e):.3e}, RK4 {rk4_ms:.1f} ms/step")

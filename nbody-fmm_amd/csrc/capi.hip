@@ -320,7 +320,7 @@ void capture_step(nbfmm_ctx *c, const StepSpec &sp, const Eval &ev) {
 
 // nsteps steps; with graphs on, a step replays the captured graph of the
 // same step (same arrays, sizes, dt and parameters), or runs directly and
-// captures one for the next.  A replay whose planned sizes overflowed left
+// captures one for the next step (of this call or a later one).  A replay whose planned sizes overflowed left
 // the state unchanged: the step runs again directly (and is captured anew).
 void run_steps(nbfmm_ctx *c, const StepSpec &sp, int nsteps, const Eval &ev,
                const std::function<void(int)> &after = nullptr) {
@@ -336,11 +336,13 @@ void run_steps(nbfmm_ctx *c, const StepSpec &sp, int nsteps, const Eval &ev,
         drop_graph(c);
         step_one(c, sp, ev);
         c->g_direct++;
+        capture_step(c, sp, ev); // at this step's sizes
       }
     } else {
+      // the graph is kept across calls: one step per call replays it too
       step_one(c, sp, ev);
       c->g_direct++;
-      if (st < nsteps) capture_step(c, sp, ev);
+      capture_step(c, sp, ev);
     }
     if (after) after(st);
   }

@@ -153,7 +153,65 @@ def test_config2_eval_and_rk4_1m(plummer1m):
     assert dP < 1e-7
 
 
+def _graph_vs_direct(ctx, n, xyz, vel, m, nsteps, off=None):
+    """nsteps classic RK4 steps dt = 1e-4 run directly and with graphs on (the
+    first step direct and captured, the rest replayed); the two final states."""
+    out = []
+    for graph in (0, 1):
+        dx, dv, dm = nbfmm.DeviceArray(xyz), nbfmm.DeviceArray(vel), nbfmm.DeviceArray(m)
+        da = nbfmm.DeviceArray(nbytes=n * 24, shape=(n, 3))
+        ctx.set_graph(graph)
+        try:
+            before = ctx.graph_stats()
+            if off is None:
+                ctx.step_rk4_dev(n, dx.ptr, dv.ptr, da.ptr, dm.ptr, 1e-4, nsteps, 1)
+            else:
+                ctx.batch_step_rk4_dev(off, dx.ptr, dv.ptr, da.ptr, dm.ptr, 1e-4, nsteps, 1)
+            after = ctx.graph_stats()
+        finally:
+            ctx.set_graph(0)
+        out.append((dx.download(ctx.stream), dv.download(ctx.stream), {k: after[k] - before[k] for k in after}))
+    (xd, vd, _), (xg, vg, st) = out
+    return _rel(xg - xyz, xd - xyz), _rel(vg - vel, vd - vel), st
+
+
+def test_config2_graph_rk4_1m(plummer1m):
+    # the captured RK4 step equals the direct one on the configuration's input
+    xyz, m, vel = plummer1m
+    n = len(m)
+    ctx = nbfmm.Context(order=10, leaf_capacity=64, G=1.0, eps2=0.0, p2p_cut=1024)
+    try:
+        _say("[cfg2] 4 classic RK4 steps, direct and replayed from a graph")
+        ex, ev, st = _graph_vs_direct(ctx, n, xyz, vel, m, 4)
+    finally:
+        ctx.close()
+    _say(f"[cfg2] graph vs direct: displacement {ex:.2e}, velocity change {ev:.2e}, stats {st}")
+    assert st["captures"] == 1 and st["launches"] == 3 and st["redone"] == 0, st
+    assert ex < 1e-12 and ev < 1e-10
+
+
 # ---- config 4 -----------------------------------------------------------------------
+def test_config4_graph_rk4():
+    # the batch of 256 systems (mixed precision, p = 13, cut 4096): replayed
+    # RK4 steps equal the direct ones
+    systems = workloads.batched_systems(256, seed=11, with_velocity=True)
+    sizes = np.array([len(s[1]) for s in systems], np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    xyz = np.concatenate([s[0] for s in systems])
+    m = np.concatenate([s[1] for s in systems])
+    vel = np.concatenate([s[2] for s in systems])
+    n = int(off[-1])
+    ctx = nbfmm.Context(order=13, leaf_capacity=32, G=1.0, eps2=0.0, p2p_cut=4096, precision=1)
+    try:
+        _say(f"[cfg4] {len(sizes)} systems, N={n}: 6 classic RK4 steps, direct and replayed from a graph")
+        ex, ev, st = _graph_vs_direct(ctx, n, xyz, vel, m, 6, off=off)
+    finally:
+        ctx.close()
+    _say(f"[cfg4] graph vs direct: displacement {ex:.2e}, velocity change {ev:.2e}, stats {st}")
+    assert st["captures"] == 1 and st["launches"] == 5 and st["redone"] == 0, st
+    assert ex < 1e-12 and ev < 1e-10
+
+
 def test_config4_batched256():
     systems = workloads.batched_systems(256, seed=11, with_velocity=True)
     sizes = np.array([len(s[1]) for s in systems], np.int64)
