@@ -318,7 +318,9 @@ typedef struct nbfmm_tiles {
   int32_t p2p_queue_waves0;   /* wavefronts per CU of the first share's persistent launch (p2p_queue 2), 1..16
                                  (default 8; the later share takes all the device holds, 16) */
   int32_t p2p_queue_chunk;    /* tasks a wavefront takes per atomic grab of the queue, 1..64 (default 4) */
-  int32_t reserved0;
+  int32_t m2l_item_pairs;     /* most M2L pairs of one offset class per contraction workgroup (the workgroup
+                                 stages the class operator once for them): a multiple of 128 in [128, 16384]
+                                 (default 1024; items hold npair / 2048 pairs, at least 128, up to this) */
 } nbfmm_tiles;
 /* Tiles in force for later evaluations; fields are validated (NBFMM_ERR_ARG). */
 int nbfmm_set_tiles(nbfmm_ctx *ctx, const nbfmm_tiles *tiles);

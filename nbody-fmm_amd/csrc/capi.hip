@@ -567,7 +567,7 @@ int nbfmm_set_tiles(nbfmm_ctx *c, const nbfmm_tiles *t) {
   if (r.m2l_slices == 0) r.m2l_slices = d.m2l_slices;
   if (r.p2p_queue_waves0 == 0) r.p2p_queue_waves0 = d.p2p_queue_waves0;
   if (r.p2p_queue_chunk == 0) r.p2p_queue_chunk = d.p2p_queue_chunk;
-  r.reserved0 = 0;
+  if (r.m2l_item_pairs == 0) r.m2l_item_pairs = d.m2l_item_pairs;
   std::string why;
   if (!tiles_ok(r, why)) return fail(NBFMM_ERR_ARG, why);
   c->shapes = r;

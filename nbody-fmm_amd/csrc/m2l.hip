@@ -991,7 +991,7 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
   const bool one = npair <= pass;
   if (K.on && !one) throw Error(NBFMM_ERR_UNSUPPORTED, "planned evaluations need the M2L partials in one pass");
   int64_t per = npair / 2048;
-  per = std::max<int64_t>(128, std::min<int64_t>(2048, per)) / 128 * 128;
+  per = std::max<int64_t>(128, std::min<int64_t>(c->shapes.m2l_item_pairs, per)) / 128 * 128;
   int32_t *cls_pairs = dget<int32_t>(c->cls_pairs, (size_t)npair * 2 + 2);
   int32_t *pos = cls_pairs + npair + 1; // sorted position of every list entry
   void *Lexp = c->lexp.p; // sized and cleared by the caller (one slot per cell that holds a local)

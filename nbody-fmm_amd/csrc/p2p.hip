@@ -886,6 +886,7 @@ nbfmm_tiles default_tiles() {
   t.p2p_early_launches_f32 = NBFMM_P2P_CHUNKS0;
   t.p2p_early_launches_f64 = NBFMM_P2P_CHUNKS0_F64;
   t.m2l_slices = 1;
+  t.m2l_item_pairs = 1024;
   t.p2p_queue = 0;
   t.p2p_queue_waves0 = 8;
   t.p2p_queue_chunk = 4;
@@ -907,6 +908,8 @@ bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
   else if (t.p2p_early_launches_f32 < 1 || t.p2p_early_launches_f32 > 64) why = "p2p_early_launches_f32 must be in [1, 64]";
   else if (t.p2p_early_launches_f64 < 1 || t.p2p_early_launches_f64 > 64) why = "p2p_early_launches_f64 must be in [1, 64]";
   else if (t.m2l_slices != 1 && t.m2l_slices != 2) why = "m2l_slices must be 1 or 2";
+  else if (t.m2l_item_pairs < 128 || t.m2l_item_pairs > 16384 || t.m2l_item_pairs % 128)
+    why = "m2l_item_pairs must be a multiple of 128 in [128, 16384]";
   else if (t.p2p_queue < 0 || t.p2p_queue > 2) why = "p2p_queue must be 0, 1 or 2";
   else if (t.p2p_queue_waves0 < 1 || t.p2p_queue_waves0 > 16) why = "p2p_queue_waves0 must be in [1, 16]";
   else if (t.p2p_queue_chunk < 1 || t.p2p_queue_chunk > 64) why = "p2p_queue_chunk must be in [1, 64]";

@@ -57,11 +57,11 @@ class Tiles(ctypes.Structure):
         ("p2p_queue", ctypes.c_int32),
         ("p2p_queue_waves0", ctypes.c_int32),
         ("p2p_queue_chunk", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("m2l_item_pairs", ctypes.c_int32),
     ]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved0"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class Stats(ctypes.Structure):
