@@ -307,17 +307,18 @@ def roofline_record(head_name, head, config):
     if ms["ms_p2p_kernel"] >= ms["ms_m2l_kernel"]:
         kern, bound, avg, F = ("k_p2p_tasks + k_p2p_wide (P2P, vector ALUs)", "valu", ms["ms_p2p_kernel"],
                                st["flop_p2p"])
-        traffic = read_traffic("k_p2p_tasks", config)
+        traffic, tsrc = read_traffic("k_p2p_tasks", config, prec, head)
     else:
         kern, bound, avg, F = ("k_m2l_cls / k_m2l_bf3 (M2L contraction, MFMA)", "mfma", ms["ms_m2l_kernel"],
                                st["flop_m2l"])
-        traffic = read_traffic("k_m2l_cls", config)
+        traffic, tsrc = read_traffic("k_m2l_cls", config, prec, head)
     ach = F / (avg * 1e-3) / 1e12
     peak = PEAK[unit_prec]
     return {"kernel": kern, "bound": bound, "avg_ms": avg,
             "avg_ms_note": "kernel-alone time per evaluation: HIP events on the kernel's stream around its "
                            "launches, in an evaluation whose phases run one after another (timing mode 2)",
             "flop_per_launch": F, "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+            "traffic_source": tsrc,
             "traffic": traffic, "phases": phases, "overall": overall}
 
 
@@ -685,17 +686,20 @@ def learned_mode(ctx, n, dx, dm, da, tgt, ref, mixed, args):
     return best
 
 
-def read_traffic(kernel, config):
-    """HBM bytes per launch of `kernel` on `config` from the committed counter
-    summary profiles/<kernel>_<config>_traffic.md (line 'traffic_bytes: X'),
-    or None when no summary exists."""
+def read_traffic(kernel, config, prec, head):
+    """HBM bytes per evaluation of `kernel` (with its companion kernels, as the
+    summary states) from the committed counter summary
+    profiles/<kernel>_<config>_traffic.md, measured at the head's settings:
+    the line 'traffic_bytes[<prec> p=<order> cut=<cut> leaf=<leaf> tol=<tol>]: X'.
+    Returns (bytes or None when no summary was measured at these settings, source)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{config}_traffic.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes:"):
-            return float(line.split(":", 1)[1])
-    return None
+    key = (f"traffic_bytes[{prec} p={head['order']} cut={head['p2p_cut']} leaf={head['leaf_capacity']} "
+           f"tol={head['class_order_tol']:g}]:")
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith(key):
+                return float(line.split(":", 1)[1]), f"profiles/{kernel}_{config}_traffic.md, {key[:-1]}"
+    return None, f"no rocprofv3 FETCH_SIZE/WRITE_SIZE summary at these settings ({key[:-1]})"
 
 
 if __name__ == "__main__":

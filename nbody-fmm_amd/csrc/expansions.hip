@@ -862,6 +862,8 @@ void run_expansions(nbfmm_ctx *c) {
         NB_P2M(float, 8, 8);
       else if (p <= 12)
         NB_P2M(float, 8, 12);
+      else if (p <= 14)
+        NB_P2M(float, 8, 14);
       else
         NB_P2M(float, 16, 16);
     } else {
@@ -869,6 +871,8 @@ void run_expansions(nbfmm_ctx *c) {
         NB_P2M(double, 8, 8);
       else if (p <= 12)
         NB_P2M(double, 8, 12);
+      else if (p <= 14)
+        NB_P2M(double, 8, 14);
       else
         NB_P2M(double, 16, 16);
     }

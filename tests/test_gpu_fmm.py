@@ -113,7 +113,7 @@ def test_direct_naive_bitwise(ctx):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("p,cut", [(2, 0), (4, 0), (8, 0), (10, 256), (12, 1024), (6, 1 << 40)])
+@pytest.mark.parametrize("p,cut", [(2, 0), (4, 0), (8, 0), (10, 256), (12, 1024), (13, 512), (14, 0), (6, 1 << 40)])
 def test_fmm_parity_exact(ctx, p, cut):
     xyz, m = workloads.uniform_cube(3000, seed=p)
     ctx.set_params(order=p, leaf_capacity=16, p2p_cut=cut, G=1.0, eps2=0.0)
@@ -389,7 +389,7 @@ def test_bad_inputs(ctx):
 # Tolerance against the FP64 oracle on the same tree, lists, p and cut: 2e-6
 # relative L2 (FP32 rounding of ~1e-7 per term over the expansion and pair
 # sums); a wrong index, sign or layout gives errors of order 1.
-@pytest.mark.parametrize("p,cut", [(4, 0), (8, 256), (10, 0), (12, 1024)])
+@pytest.mark.parametrize("p,cut", [(4, 0), (8, 256), (10, 0), (12, 1024), (13, 256), (14, 0)])
 def test_mixed_parity(ctx, p, cut):
     xyz, m = workloads.uniform_cube(3000, seed=p + 7)
     ctx.set_params(order=p, leaf_capacity=16, p2p_cut=cut, G=1.0, eps2=0.0, precision=1)
