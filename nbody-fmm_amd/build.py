@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NBFMM_ARCH", "gfx950")
 
-# -packed-fp32-ops: no v_pk_*_f32 generation (build notes).
+# -packed-fp32-ops: the compiler emits no packed FP32 instructions (v_pk_*_f32),
+# some forms of which have given wrong results on gfx950 without an error.
 FLAGS = [
     "-O3", "-std=c++20", f"--offload-arch={ARCH}", "-fPIC", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

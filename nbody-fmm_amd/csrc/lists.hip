@@ -38,6 +38,8 @@ namespace {
 constexpr int kT = 256;
 constexpr int kW = 64;
 constexpr int kWin = 216;
+// workgroups of the list passes (four parents per round)
+constexpr int64_t kListBlocks = 4096;
 // direct-evaluation work per P2P sub-task (particle pairs)
 constexpr int64_t kTaskPairs = 1 << 17;
 
@@ -121,105 +123,111 @@ __global__ void __launch_bounds__(kT) k_lists(const DevPlan *__restrict__ dp, in
   if (threadIdx.x < (kNumClasses + 31) / 32) bad.w[threadIdx.x] = dp->bad_cls.w[threadIdx.x];
   const ParentRange pr = parent_range(dp, lc, D, keep);
   const int32_t nparents = pr.nparents, skip_from = pr.skip_from;
-  // both passes take the parents k_list_filter listed (their count at dp->nactive)
-  const int32_t pw = (int32_t)blockIdx.x * (kT / kW) + wave;
-  const int32_t P = pw < dp->nactive ? active[pw] : nparents;
-  const bool live = P < nparents;
+  // both passes take the parents k_list_filter listed (their count at
+  // dp->nactive), four per workgroup round: the grid strides over them
+  const int32_t nact = dp->nactive;
   int32_t *cand = cand_s[wave], *csz = csz_s[wave];
-  int64_t maxn = 0;
-  // children to classify (bit o): all when skipping is off
-  uint32_t todo = 0;
-  if (live) {
-    if (skip_from >= 0) {
-      const int32_t q = lane < 27 ? nbr27[(int64_t)P * 27 + lane] : -1;
-      maxn = q >= 0 ? cend[q] - cbeg[q] : 0;
-      for (int off = 32; off >= 1; off >>= 1) maxn = max(maxn, __shfl_xor(maxn, off));
-    }
-    const int32_t ch = lane < 8 ? child8[(int64_t)P * 8 + lane] : -1;
-    const bool need = ch >= 0 && !(skip_from >= 0 && ch >= skip_from && (int64_t)(cend[ch] - cbeg[ch]) * maxn < cut);
-    todo = (uint32_t)__ballot(need);
-    if (todo) {
-      // the four window entries of a lane: each load level issued for all
-      // four before the next (three dependent gathers, not twelve)
-      constexpr int kR = (kWin + kW - 1) / kW;
-      int32_t pn[kR], q[kR], nq[kR];
-#pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        const int w = lane + r * kW;
-        const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
-        pn[r] = w < kWin ? nbr27[(int64_t)P * 27 + (wz >> 1) * 9 + (wy >> 1) * 3 + (wx >> 1)] : -1;
-      }
-#pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        const int w = lane + r * kW;
-        const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
-        q[r] = pn[r] >= 0 ? child8[(int64_t)pn[r] * 8 + ((wx & 1) | ((wy & 1) << 1) | ((wz & 1) << 2))] : -1;
-      }
-#pragma unroll
-      for (int r = 0; r < kR; ++r) nq[r] = q[r] >= 0 ? cend[q[r]] - cbeg[q[r]] : 0;
-#pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        const int w = lane + r * kW;
-        if (w < kWin) {
-          cand[w] = q[r];
-          csz[w] = nq[r];
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if (!todo) return;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  for (int o = 0; o < 8; ++o) {
-    if (!((todo >> o) & 1u)) continue;
-    const int32_t s = child8[(int64_t)P * 8 + o];
-    const int bx = o & 1, by = (o >> 1) & 1, bz = o >> 2;
-    const int64_t nt = cend[s] - cbeg[s];
-    int32_t a_il = 0, a_m = 0, a_s = 0;
-    if (FILL) {
-      a_il = il ? off_il[s] : 0;
-      a_m = off_m2l[s];
-      a_s = off_sub[s];
-    }
-    int32_t c_il = 0, c_m = 0, c_s = 0;
-    int64_t sp = 0;
-    for (int base = 0; base < kWin; base += kW) {
-      const int w = base + lane;
-      const bool valid = w < kWin;
-      const int32_t q = valid ? cand[w] : -1;
-      const int64_t ns = valid ? csz[w] : 0;
-      const int dx = w % 6 - 2 - bx, dy = (w / 6) % 6 - 2 - by, dz = w / 36 - 2 - bz;
-      const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
-      const bool far = q >= 0 && !near;
-      const int cls = (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3);
-      const bool sub = far && (nt * ns < cut || (((bad.w[cls >> 5] >> (cls & 31)) & 1u) && nt * ns < kBadDirect));
-      const bool m2l = far && !sub;
-      const uint64_t bf = __ballot(far), bs = __ballot(sub), bm = __ballot(m2l);
-      if (FILL) {
-        if (il && far) il[a_il + c_il + __popcll(bf & lt)] = q;
-        if (sub) sub_src[a_s + c_s + __popcll(bs & lt)] = q;
-        if (m2l) {
-          const int32_t k = a_m + c_m + __popcll(bm & lt);
-          m2l_src[k] = q;
-          m2l_cls[k] = (uint16_t)cls;
-          is_src[q] = 1;
+  for (int32_t b = (int32_t)blockIdx.x; b < (nact + kT / kW - 1) / (kT / kW); b += (int32_t)gridDim.x) {
+    const int32_t pw = b * (kT / kW) + wave;
+    const int32_t P = pw < nact ? active[pw] : nparents;
+    const bool live = P < nparents;
+    int64_t maxn = 0;
+    // children to classify (bit o): all when skipping is off
+    uint32_t todo = 0;
+    if (live) {
+      if (skip_from >= 0) {
+        const int32_t q = lane < 27 ? nbr27[(int64_t)P * 27 + lane] : -1;
+        maxn = q >= 0 ? cend[q] - cbeg[q] : 0;
+        for (int off = 32; off >= 1; off >>= 1) maxn = max(maxn, __shfl_xor(maxn, off));
+      }
+      const int32_t ch = lane < 8 ? child8[(int64_t)P * 8 + lane] : -1;
+      const bool need =
+          ch >= 0 && !(skip_from >= 0 && ch >= skip_from && (int64_t)(cend[ch] - cbeg[ch]) * maxn < cut);
+      todo = (uint32_t)__ballot(need);
+      if (todo) {
+        // the four window entries of a lane: each load level issued for all
+        // four before the next (three dependent gathers, not twelve)
+        constexpr int kR = (kWin + kW - 1) / kW;
+        int32_t pn[kR], q[kR], nq[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const int w = lane + r * kW;
+          const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
+          pn[r] = w < kWin ? nbr27[(int64_t)P * 27 + (wz >> 1) * 9 + (wy >> 1) * 3 + (wx >> 1)] : -1;
         }
-      } else if (sub) {
-        sp += nt * ns;
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const int w = lane + r * kW;
+          const int wx = w % 6, wy = (w / 6) % 6, wz = w / 36;
+          q[r] = pn[r] >= 0 ? child8[(int64_t)pn[r] * 8 + ((wx & 1) | ((wy & 1) << 1) | ((wz & 1) << 2))] : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) nq[r] = q[r] >= 0 ? cend[q[r]] - cbeg[q[r]] : 0;
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const int w = lane + r * kW;
+          if (w < kWin) {
+            cand[w] = q[r];
+            csz[w] = nq[r];
+          }
+        }
       }
-      c_il += __popcll(bf);
-      c_s += __popcll(bs);
-      c_m += __popcll(bm);
     }
-    if (!FILL) {
-      for (int off = 32; off >= 1; off >>= 1) sp += __shfl_xor(sp, off);
-      if (lane == 0) {
-        cnt_il[s] = c_il;
-        cnt_m2l[s] = c_m;
-        cnt_sub[s] = c_s;
-        sub_pairs[s] = sp;
+    __syncthreads();
+    for (int o = 0; o < 8; ++o) {
+      if (!((todo >> o) & 1u)) continue;
+      const int32_t s = child8[(int64_t)P * 8 + o];
+      const int bx = o & 1, by = (o >> 1) & 1, bz = o >> 2;
+      const int64_t nt = cend[s] - cbeg[s];
+      int32_t a_il = 0, a_m = 0, a_s = 0;
+      if (FILL) {
+        a_il = il ? off_il[s] : 0;
+        a_m = off_m2l[s];
+        a_s = off_sub[s];
+      }
+      int32_t c_il = 0, c_m = 0, c_s = 0;
+      int64_t sp = 0;
+      for (int base = 0; base < kWin; base += kW) {
+        const int w = base + lane;
+        const bool valid = w < kWin;
+        const int32_t q = valid ? cand[w] : -1;
+        const int64_t ns = valid ? csz[w] : 0;
+        const int dx = w % 6 - 2 - bx, dy = (w / 6) % 6 - 2 - by, dz = w / 36 - 2 - bz;
+        const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+        const bool far = q >= 0 && !near;
+        const int cls = (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3);
+        const bool sub =
+            far && (nt * ns < cut || (((bad.w[cls >> 5] >> (cls & 31)) & 1u) && nt * ns < kBadDirect));
+        const bool m2l = far && !sub;
+        const uint64_t bf = __ballot(far), bs = __ballot(sub), bm = __ballot(m2l);
+        if (FILL) {
+          if (il && far) il[a_il + c_il + __popcll(bf & lt)] = q;
+          if (sub) sub_src[a_s + c_s + __popcll(bs & lt)] = q;
+          if (m2l) {
+            const int32_t k = a_m + c_m + __popcll(bm & lt);
+            m2l_src[k] = q;
+            m2l_cls[k] = (uint16_t)cls;
+            is_src[q] = 1;
+          }
+        } else if (sub) {
+          sp += nt * ns;
+        }
+        c_il += __popcll(bf);
+        c_s += __popcll(bs);
+        c_m += __popcll(bm);
+      }
+      if (!FILL) {
+        for (int off = 32; off >= 1; off >>= 1) sp += __shfl_xor(sp, off);
+        if (lane == 0) {
+          cnt_il[s] = c_il;
+          cnt_m2l[s] = c_m;
+          cnt_sub[s] = c_s;
+          sub_pairs[s] = sp;
+        }
       }
     }
+    __syncthreads(); // the window is restaged in the next round
   }
 }
 
@@ -267,6 +275,26 @@ __global__ void k_sum_i32(const int32_t *v, int64_t n, unsigned long long *out) 
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+// the sums of three arrays of n entries into out[0..2]
+__global__ void k_sum3_i32(const int32_t *a, const int32_t *b, const int32_t *c, int64_t n, unsigned long long *out) {
+  __shared__ unsigned long long s[3][kT];
+  unsigned long long x = 0, y = 0, z = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    x += (unsigned)a[i];
+    y += (unsigned)b[i];
+    z += (unsigned)c[i];
+  }
+  s[0][threadIdx.x] = x;
+  s[1][threadIdx.x] = y;
+  s[2][threadIdx.x] = z;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) atomicAdd(out + threadIdx.x, s[threadIdx.x][0]);
 }
 __global__ void k_sum_i64(const int64_t *v, int64_t n, unsigned long long *out) {
   __shared__ unsigned long long s[kT];
@@ -484,30 +512,27 @@ void build_lists(nbfmm_ctx *c) {
     npar = K.parents;
   else
     npar = P.lvl_start[D] - P.lvl_start[std::min(lc - 1, D)];
-  const unsigned nlb = (unsigned)std::max<int64_t>(1, (npar + kT / kW - 1) / (kT / kW));
+  // the list passes stride over the active parents (few of all at the usual
+  // cuts): a grid of at most kListBlocks workgroups
+  const unsigned nlb = (unsigned)std::clamp<int64_t>((npar + kT / kW - 1) / (kT / kW), 1, kListBlocks);
   // parents with a child to classify, their count in the plan
   int32_t *active = dget<int32_t>(c->act_par, (size_t)npar + 2);
   fill_i32(s, &dp->nactive, 1, 0);
   hipLaunchKernelGGL(k_list_filter, dim3(nblk(npar)), dim3(kT), 0, s, (const DevPlan *)dp, lc, D, keep, nbr27, child8,
                      cbeg, cend, cut, active, &dp->nactive);
   NB_LAUNCH();
-  // grid for every parent; wavefronts past the listed count exit
   hipLaunchKernelGGL(k_lists<false>, dim3(nlb), dim3(kT), 0, s, (const DevPlan *)dp, lc, D, keep, nbr27, child8, cbeg,
                      cend, cut, cnt_il, cnt_m2l, cnt_sub, sub_pairs, nullptr, nullptr, nullptr, nullptr, nullptr,
                      nullptr, nullptr, nullptr, (const int32_t *)active);
   NB_LAUNCH();
-  scan_excl_i32(cnt_il, off_il, S, c->scan_tmp, s);
+  if (keep) scan_excl_i32(cnt_il, off_il, S, c->scan_tmp, s); // the full lists' offsets only when they are kept
   scan_excl_i32(cnt_m2l, off_m2l, S, c->scan_tmp, s);
   scan_excl_i32(cnt_sub, off_sub, S, c->scan_tmp, s);
   // list lengths in 64 bits: the offsets above are 32-bit
   unsigned long long *tot = dp->tot;
   fill_zero(s, tot, sizeof(unsigned long long) * 3);
-  const unsigned nsb = std::min<unsigned>(512, nblk(S));
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_il, S, tot);
-  NB_LAUNCH();
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_m2l, S, tot + 1);
-  NB_LAUNCH();
-  hipLaunchKernelGGL(k_sum_i32, dim3(nsb), dim3(kT), 0, s, (const int32_t *)cnt_sub, S, tot + 2);
+  hipLaunchKernelGGL(k_sum3_i32, dim3(std::min<unsigned>(512, nblk(S))), dim3(kT), 0, s, (const int32_t *)cnt_il,
+                     (const int32_t *)cnt_m2l, (const int32_t *)cnt_sub, S, tot);
   NB_LAUNCH();
   const unsigned long long lim = 0x7fffffffull - 1;
   // the host reads the counts (direct) or the plan checks them (planned)
@@ -538,7 +563,7 @@ void build_lists(nbfmm_ctx *c) {
   fill_i32(s, is_src, S, 0);
   const int64_t nfill = K.on ? npar : nactive;
   if (nfill > 0) {
-    const unsigned nfb = (unsigned)((nfill + kT / kW - 1) / (kT / kW));
+    const unsigned nfb = (unsigned)std::min<int64_t>((nfill + kT / kW - 1) / (kT / kW), kListBlocks);
     hipLaunchKernelGGL(k_lists<true>, dim3(nfb), dim3(kT), 0, s, (const DevPlan *)dp, lc, D, keep, nbr27, child8, cbeg,
                        cend, cut, nullptr, nullptr, nullptr, nullptr, il ? off_il : nullptr, off_m2l, off_sub, il,
                        m2l_src, m2l_cls, sub_src, is_src, (const int32_t *)active);

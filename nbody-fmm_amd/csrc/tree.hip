@@ -508,27 +508,67 @@ __global__ void k_child8(const uint64_t *__restrict__ ckey, const int32_t *__res
 }
 
 // 27-neighbourhood (offsets x fastest, self at 13) from the parent's table;
-// one thread per table entry; with `skip`, rows of cells with skip[c] != 0
-// are left unwritten.
-__global__ void k_nbr27(const uint64_t *__restrict__ ckey, const int32_t *__restrict__ cparent,
-                        const int32_t *__restrict__ child8, const DevPlan *__restrict__ dp, int lev,
-                        const int32_t *__restrict__ skip, int32_t *nbr27) {
+// one thread per cell: the cell's coordinates are decoded once and, per axis,
+// the three offsets fall in two of the parent's neighbours (the parent itself
+// and the one on the side of the cell's octant), so 8 parent entries and 27
+// child entries are read.  Rows go through LDS (stride 27, odd: no bank
+// conflicts) and leave as contiguous stores.  With `skip`, rows of cells with
+// skip[c] != 0 are left unwritten.
+constexpr int kNbrCells = 256; // cells per workgroup
+__global__ void __launch_bounds__(kNbrCells) k_nbr27(const uint64_t *__restrict__ ckey,
+                                                     const int32_t *__restrict__ cparent,
+                                                     const int32_t *__restrict__ child8,
+                                                     const DevPlan *__restrict__ dp, int lev,
+                                                     const int32_t *__restrict__ skip, int32_t *nbr27) {
+  __shared__ int32_t rows[kNbrCells * 27];
   if (plan_dead(dp)) return;
-  const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
-  const int32_t c = dp->lvl_start[lev] + (int32_t)(e / 27);
-  if (c >= dp->lvl_start[lev + 1] || (skip && skip[c])) return;
-  const int o = (int)(e % 27);
-  const uint64_t k = ckey[c];
-  const int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
-  const int side = 1 << lev;
-  const int qx = x + o % 3 - 1, qy = y + (o / 3) % 3 - 1, qz = z + o / 9 - 1;
-  int32_t r = -1;
-  if (qx >= 0 && qy >= 0 && qz >= 0 && qx < side && qy < side && qz < side) {
-    const int px = (qx >> 1) - (x >> 1) + 1, py = (qy >> 1) - (y >> 1) + 1, pz = (qz >> 1) - (z >> 1) + 1;
-    const int32_t Pn = nbr27[(int64_t)cparent[c] * 27 + pz * 9 + py * 3 + px];
-    if (Pn >= 0) r = child8[(int64_t)Pn * 8 + ((qx & 1) | ((qy & 1) << 1) | ((qz & 1) << 2))];
+  const int32_t c0 = dp->lvl_start[lev] + (int32_t)blockIdx.x * kNbrCells;
+  const int32_t nb = min(kNbrCells, dp->lvl_start[lev + 1] - c0);
+  if (nb <= 0) return;
+  const int t = threadIdx.x;
+  const int32_t c = c0 + t;
+  if (t < nb && !(skip && skip[c])) {
+    const uint64_t k = ckey[c];
+    const int x = (int)compact3(k), y = (int)compact3(k >> 1), z = (int)compact3(k >> 2);
+    const int32_t *prow = nbr27 + (int64_t)cparent[c] * 27;
+    // Along an axis with coordinate u and bit b = u & 1, the offset slots
+    // o (offset o - 1) are taken as k = 0: the octant's side (o = 2b), in
+    // the parent's neighbour on that side, child bit b ^ 1; k = 1: o = 1, in
+    // the parent, bit b; k = 2: o = 2 - 2b, in the parent, bit b ^ 1.  A
+    // neighbour outside the domain lies across the parent's edge, where the
+    // parent's row holds -1 (the root's row is -1 but for itself).
+    const int bx = x & 1, by = y & 1, bz = z & 1;
+    int32_t pn[8]; // parent neighbours: bit d set = the octant's side along axis d
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int px = (j & 1) ? 2 * bx : 1, py = (j & 2) ? 2 * by : 1, pz = (j & 4) ? 2 * bz : 1;
+      pn[j] = prow[pz * 9 + py * 3 + px];
+    }
+    int32_t *row = rows + t * 27;
+#pragma unroll
+    for (int kz = 0; kz < 3; ++kz)
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+          const int ox = kx == 0 ? 2 * bx : kx == 1 ? 1 : 2 - 2 * bx;
+          const int oy = ky == 0 ? 2 * by : ky == 1 ? 1 : 2 - 2 * by;
+          const int oz = kz == 0 ? 2 * bz : kz == 1 ? 1 : 2 - 2 * bz;
+          const int32_t P = pn[(kx == 0 ? 1 : 0) | (ky == 0 ? 2 : 0) | (kz == 0 ? 4 : 0)];
+          const int cb = (kx == 1 ? bx : bx ^ 1) | ((ky == 1 ? by : by ^ 1) << 1) | ((kz == 1 ? bz : bz ^ 1) << 2);
+          row[oz * 9 + oy * 3 + ox] = P >= 0 ? child8[(int64_t)P * 8 + cb] : -1;
+        }
   }
-  nbr27[(int64_t)c * 27 + o] = r;
+  __syncthreads();
+  int32_t *out = nbr27 + (int64_t)c0 * 27;
+  for (int i = t; i < nb * 27; i += kNbrCells) {
+    if (skip && skip[c0 + i / 27]) continue;
+    out[i] = rows[i];
+  }
+}
+
+inline unsigned nbr_blocks(int64_t cells) {
+  return (unsigned)std::max<int64_t>(1, (cells + kNbrCells - 1) / kNbrCells);
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kT - 1) / kT); }
@@ -645,7 +685,7 @@ static void fill_levels(nbfmm_ctx *c, const double *d_xyz, const double *d_m, in
   // the leaf level's rows follow in build_lists, for the leaves that are not
   // closed (the only ones the evaluation reads); nbr_leaf_rows completes them
   for (int d = 1; d < D; ++d) {
-    hipLaunchKernelGGL(k_nbr27, dim3(nblk(level_cells(c, d) * 27)), dim3(kT), 0, s, ckey, cpar, child8,
+    hipLaunchKernelGGL(k_nbr27, dim3(nbr_blocks(level_cells(c, d))), dim3(kNbrCells), 0, s, ckey, cpar, child8,
                        (const DevPlan *)dp, d, nullptr, nbr27);
     NB_LAUNCH();
   }
@@ -760,7 +800,7 @@ void nbr_leaf_rows(nbfmm_ctx *c, const int32_t *closed) {
   Plan &P = c->plan;
   const int D = P.geo.depth;
   if (c->nbr_leaf == 2 || D == 0 || (closed && c->nbr_leaf == 1)) return;
-  hipLaunchKernelGGL(k_nbr27, dim3(nblk(level_cells(c, D) * 27)), dim3(kT), 0, c->stream, (const uint64_t *)c->ckey.p,
+  hipLaunchKernelGGL(k_nbr27, dim3(nbr_blocks(level_cells(c, D))), dim3(kNbrCells), 0, c->stream, (const uint64_t *)c->ckey.p,
                      (const int32_t *)c->cparent.p, (const int32_t *)c->child8.p, (const DevPlan *)dplan(c), D,
                      closed, (int32_t *)c->nbr27.p);
   NB_LAUNCH();
