@@ -304,8 +304,8 @@ typedef struct nbfmm_tiles {
   int32_t p2p_group_f32;      /* targets per source pass of the narrow FP32 P2P kernel: 8, 12, 16 (default) or 24 */
   int32_t p2p_group_f64;      /* the same for FP64: 4, 8 (default) or 12 */
   int32_t p2p_narrow_max_f32; /* blocks of at most this many targets go to the narrow kernel, larger ones to the
-                                 four-targets-per-lane kernel: 1..64 (default 32) */
-  int32_t p2p_narrow_max_f64; /* the same for FP64 (default 16) */
+                                 four-targets-per-lane kernel: 1..64 (default 64) */
+  int32_t p2p_narrow_max_f64; /* the same for FP64 (default 64) */
   double p2p_early_share_f32; /* share of the P2P tasks started beside the up-pass, in [0, 1] (default 0.5) */
   double p2p_early_share_f64; /* the same for FP64 (default 0.2) */
   int32_t p2p_early_launches_f32; /* launches of that share's narrow tasks, 1..64 (default 16) */

@@ -64,11 +64,6 @@ __device__ __forceinline__ int first_cell(const int32_t *lp, int ne, int32_t g) 
   }
   return lo;
 }
-// particle of source g, moving cell c forward (g never decreases for one c)
-__device__ __forceinline__ int32_t source_at(const int32_t *lb, const int32_t *lp, int ne, int &c, int32_t g) {
-  while (c + 1 < ne && lp[c + 1] <= g) ++c;
-  return lb[c] + (g - lp[c]);
-}
 static_assert(kTaskTargets == 4 * kW, "a P2P task holds at most four targets per lane");
 
 // 1/sqrt(r2) for positive finite r2: v_rsq_f64 estimate y refined by the
@@ -310,9 +305,13 @@ __device__ __forceinline__ double tr_reduce(double (&v)[G], int lane) {
 // reduction across the wavefront.  A lane's sum runs over at most
 // kTaskPairs / 64 sources.  The sources in cells touching the target cell
 // run with the pair Tr, the others with TrFar, in two loops one after the
-// other; both keep the kernel at 128 registers (four wavefronts per SIMD).  One wavefront per task
-// (tasks blockIdx.x, blockIdx.x + gridDim.x, ... when the grid is smaller
-// than the task list).  POT also sums m_j / r_ij into pot.
+// other; both keep the kernel at 128 registers (four wavefronts per SIMD).
+// A source's particle comes from an LDS table of the list cell of every
+// concatenated index in a window of kSrcWin indices (one byte each) and the
+// cell's offset particle - index: two LDS reads, where walking the cell
+// boundaries cost a loop of dependent reads per source.  One wavefront per
+// task (tasks blockIdx.x, blockIdx.x + gridDim.x, ... when the grid is
+// smaller than the task list).  POT also sums m_j / r_ij into pot.
 // Wavefronts per SIMD the narrow kernel is held to: four (128 registers) for
 // the group sizes measured to fit without spills in their loops, otherwise
 // left to the compiler
@@ -322,17 +321,90 @@ constexpr int p2p_waves() {
 }
 constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
 
+constexpr int kSrcWin = 4096; // concatenated source indices per table window (list cells < 256)
+static_assert(kMaxList <= 256, "list cells are stored in one byte");
+
+// tab[g - w0] = list cell of concatenated index g, for g in [w0, w1)
+// (lp: first index of each of the ne cells, total: their sum).  Lane e
+// writes the entries of cell e (e + 64, ... for longer lists) itself when
+// they are at most kFillRun, one byte per step; longer runs are written by
+// the whole wavefront afterwards.  Ends with a barrier.
+constexpr int kFillRun = 32;
+__device__ __forceinline__ void fill_cells(uint8_t *tab, const int32_t *lp, int ne, int32_t total, int32_t w0,
+                                           int32_t w1, int lane) {
+  for (int e0 = 0; e0 < ne; e0 += kW) {
+    const int e = e0 + lane;
+    int32_t a = w1, b = w1;
+    if (e < ne) {
+      a = max(lp[e], w0);
+      b = min(e + 1 < ne ? lp[e + 1] : total, w1);
+    }
+    const bool own = b - a <= kFillRun;
+    if (own)
+      for (int32_t g = a; g < b; ++g) tab[g - w0] = (uint8_t)e;
+    for (uint64_t m = __ballot(!own); m; m &= m - 1) {
+      const int k = __ffsll((unsigned long long)m) - 1;
+      const int32_t ka = __shfl(a, k), kb = __shfl(b, k);
+      for (int32_t g = ka + lane; g < kb; g += kW) tab[g - w0] = (uint8_t)(e0 + k);
+    }
+  }
+  __syncthreads();
+}
+
+// The pairs of targets st[t0, t0 + ng) with the sources [g0, g1) of the
+// table window at w0, two per lane and iteration (g and g + 64): the two
+// sources are converted, the next two loads are issued into the same
+// registers, and the pairs are evaluated while they are in flight.  The
+// loads are unconditional (past the end they repeat a source and are not
+// used) and a scheduling barrier keeps them after the conversions: hoisted
+// above them, both pairs of positions are live at once and the loop copies
+// them at every iteration (16 64-bit moves) or spills.
+template <class P, class SS, int KG, int STEP, bool POT>
+__device__ __forceinline__ void src_pairs(int32_t g0, int32_t g1, int32_t w0, const uint8_t *tab, const int32_t *dl,
+                                          const double4 *__restrict__ pos, const double4 &anchor, const SS *st,
+                                          int t0, int ng, typename P::A (&ax)[KG], typename P::A (&ay)[KG],
+                                          typename P::A (&az)[KG], typename P::A (&ap)[KG], int lane) {
+  typedef typename P::S S;
+  // particle of source g < g1; a missing second source repeats the first
+  // (its mass is zeroed), so every pair stays finite
+  auto load = [&](int32_t g, int32_t g_first) -> double4 {
+    const int32_t gg = g < g1 ? g : g_first;
+    return pos[dl[tab[gg - w0]] + gg];
+  };
+  int32_t g = g0 + lane;
+  if (g >= g1) return;
+  double4 q1 = load(g, g), q2 = load(g + kW, g);
+  for (; g < g1; g += 2 * kW) {
+    const S x1 = P::make(q1, anchor);
+    S x2 = P::make(q2, anchor);
+    if (!(g + kW < g1)) P::massless(x2); // contributes nothing
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      const int32_t n1 = g + 2 * kW < g1 ? g + 2 * kW : g;
+      q1 = load(n1, n1);
+      q2 = load(g + 3 * kW, n1);
+    }
+#pragma unroll
+    for (int t4 = 0; t4 < KG; t4 += STEP)
+      if (t4 < ng) // STEP targets a branch (sums of slots past ng are never used)
+#pragma unroll
+        for (int t = t4; t < t4 + STEP; ++t) {
+          const auto xt = P::tgt(st[t0 + t]);
+          P::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
+          P::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
+        }
+  }
+}
+
 // One narrow task (the body of k_p2p_tasks and of the queue kernel's narrow
-// branch); st, lb, lp, tmp are the wavefront's LDS.
+// branch); st, lb, lp, tmp, tab are the wavefront's LDS.
 template <class Tr, class TrFar, int KG, bool POT>
 __device__ __forceinline__ void narrow_task(const P2PTask &tk, const int32_t *__restrict__ nbr27,
                                             const int32_t *__restrict__ sub_src, const int32_t *__restrict__ cbeg,
                                             const int32_t *__restrict__ cend, const uint64_t *__restrict__ ckey,
                                             const double4 *__restrict__ pos, double *__restrict__ grad,
                                             double *__restrict__ pot, typename Tr::S *st, int32_t *lb, int32_t *lp,
-                                            int32_t *tmp, int lane) {
-  typedef typename Tr::S S;
-  typedef typename TrFar::S FS;
+                                            int32_t *tmp, uint8_t *tab, int lane) {
   typedef typename Tr::A A;
   const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
   double4 anchor = make_double4(0, 0, 0, 0);
@@ -358,71 +430,30 @@ __device__ __forceinline__ void narrow_task(const P2PTask &tk, const int32_t *__
   const int ne = tk.nlist;
   const int32_t f0 = tk.f0, f1 = min(tk.f1, total);
   const int32_t fm = max(f0, min(f1, near_end)); // [f0, fm) near cells, [fm, f1) the others
+  // lb becomes the offset particle - concatenated index of every list cell
+  for (int e = lane; e < ne; e += kW) lb[e] -= lp[e];
+  const bool one_win = f1 - f0 <= kSrcWin;
+  if (one_win) fill_cells(tab, lp, ne, total, f0, f1, lane); // ends with a barrier
+  else __syncthreads();
   constexpr int kGroup = KG;
   for (int t0 = 0; t0 < nc; t0 += kGroup) {
     const int ng = min(kGroup, nc - t0);
     A ax[kGroup], ay[kGroup], az[kGroup], ap[kGroup];
 #pragma unroll
     for (int t = 0; t < kGroup; ++t) ax[t] = ay[t] = az[t] = ap[t] = 0;
-    {
-      // a lane takes sources g and g + 64 (g = f0 + lane, + 128, ...): every
-      // target read serves two pairs; c1, c2 follow their list cells
-      const int32_t g0 = f0, g1 = fm;
-      int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
-      int32_t g = g0 + lane;
-      double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
-      if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
-      if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
-      for (; g < g1; g += 2 * kW) {
-        const double4 q1 = p1, q2 = p2;
-        const bool has2 = g + kW < g1;
-        // the next sources' loads are in flight while these are evaluated
-        if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
-        if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
-        const S x1 = Tr::make(q1, anchor);
-        S x2 = Tr::make(has2 ? q2 : q1, anchor);
-        if (!has2) Tr::massless(x2); // contributes nothing
-#pragma unroll
-        for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
-          if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
-#pragma unroll
-            for (int t = t4; t < t4 + Tr::kStep; ++t) {
-              const S xt = Tr::tgt(st[t0 + t]);
-              Tr::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
-              Tr::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
-            }
+    for (int32_t w0 = f0; w0 < f1; w0 += kSrcWin) {
+      const int32_t w1 = min(f1, w0 + kSrcWin);
+      if (!one_win) {
+        __syncthreads(); // the previous window's table reads are done
+        fill_cells(tab, lp, ne, total, w0, w1, lane);
       }
-    }
-    // the far loop's set-up (cell search, first loads) stays after the near
-    // loop: moved ahead of it, its registers would be held through it
-    asm volatile("" ::: "memory");
-    {
-      // the same over the sources of the cells that do not touch the target cell
-      const int32_t g0 = fm, g1 = f1;
-      int c1 = first_cell(lp, ne, g0 + lane), c2 = first_cell(lp, ne, g0 + lane + kW);
-      int32_t g = g0 + lane;
-      double4 p1 = make_double4(0, 0, 0, 0), p2 = make_double4(0, 0, 0, 0);
-      if (g < g1) p1 = pos[source_at(lb, lp, ne, c1, g)];
-      if (g + kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + kW)];
-      for (; g < g1; g += 2 * kW) {
-        const double4 q1 = p1, q2 = p2;
-        const bool has2 = g + kW < g1;
-        // the next sources' loads are in flight while these are evaluated
-        if (g + 2 * kW < g1) p1 = pos[source_at(lb, lp, ne, c1, g + 2 * kW)];
-        if (g + 3 * kW < g1) p2 = pos[source_at(lb, lp, ne, c2, g + 3 * kW)];
-        const FS x1 = TrFar::make(q1, anchor);
-        FS x2 = TrFar::make(has2 ? q2 : q1, anchor);
-        if (!has2) TrFar::massless(x2); // contributes nothing
-#pragma unroll
-        for (int t4 = 0; t4 < kGroup; t4 += Tr::kStep)
-          if (t4 < ng) // Tr::kStep targets a branch (sums of slots past ng are never used)
-#pragma unroll
-            for (int t = t4; t < t4 + Tr::kStep; ++t) {
-              const FS xt = TrFar::tgt(st[t0 + t]);
-              TrFar::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
-              TrFar::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
-            }
-      }
+      src_pairs<Tr, typename Tr::S, KG, Tr::kStep, POT>(max(f0, w0), min(fm, w1), w0, tab, lb, pos, anchor, st, t0, ng, ax, ay,
+                                             az, ap, lane);
+      // the far loop's set-up stays after the near loop: moved ahead of it,
+      // its registers would be held through it
+      asm volatile("" ::: "memory");
+      src_pairs<TrFar, typename Tr::S, KG, Tr::kStep, POT>(max(fm, w0), w1, w0, tab, lb, pos, anchor, st, t0, ng, ax, ay, az,
+                                                ap, lane);
     }
 
     // each component's kGroup sums (zero-padded to a power of two RG) meet
@@ -471,13 +502,14 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   __shared__ int32_t tmp[54];
+  __shared__ uint8_t tab[kSrcWin];
   const int lane = threadIdx.x;
   if (plan_dead(dp)) return;
   const int64_t t0 = rng[0], ntask = rng[1] - t0; // the tasks [rng[0], rng[1])
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
-    __syncthreads(); // the previous task's reads of lb, lp, st are done
+    __syncthreads(); // the previous task's reads of lb, lp, st, tab are done
     narrow_task<Tr, TrFar, KG, POT>(tasks[t0 + task], nbr27, sub_src, cbeg, cend, ckey, pos, grad, pot, st, lb, lp, tmp,
-                                    lane);
+                                    tab, lane);
   }
 }
 
@@ -654,6 +686,7 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
   __shared__ int32_t lb[kMaxList];
   __shared__ int32_t lp[kMaxList];
   __shared__ int32_t tmp[54];
+  __shared__ uint8_t tab[kSrcWin];
   __shared__ int32_t next;
   const int lane = threadIdx.x;
   if (plan_dead(dp)) return;
@@ -674,7 +707,7 @@ __global__ void __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(p2p_wav
                                   tmp, lane);
       else
         narrow_task<Tr, TrFar, KG, POT>(tasks[narrow0 + (q - nwide)], nbr27, sub_src, cbeg, cend, ckey, pos, grad,
-                                        pot, reinterpret_cast<S *>(buf), lb, lp, tmp, lane);
+                                        pot, reinterpret_cast<S *>(buf), lb, lp, tmp, tab, lane);
     }
   }
 }

@@ -70,7 +70,7 @@ TILE_GRID = {
     1: [("p2p_group_f32", (8, 12, 16, 24)), ("p2p_narrow_max_f32", (16, 32, 64)),
         ("p2p_early_share_f32", (0.3, 0.5, 0.7)), ("p2p_early_launches_f32", (8, 16, 24)), ("m2l_slices", (1, 2)),
         ("m2l_item_pairs", (512, 1024, 2048)), ("p2p_queue", (0, 1, 2))],
-    0: [("p2p_group_f64", (4, 8, 12)), ("p2p_narrow_max_f64", (8, 16, 32)),
+    0: [("p2p_group_f64", (4, 8, 12)), ("p2p_narrow_max_f64", (16, 32, 64)),
         ("p2p_early_share_f64", (0.1, 0.2, 0.4)), ("p2p_early_launches_f64", (1, 4, 16)), ("m2l_slices", (1, 2)),
         ("m2l_item_pairs", (512, 1024, 2048)), ("p2p_queue", (0, 1, 2))],
 }

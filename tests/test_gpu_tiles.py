@@ -24,16 +24,16 @@ def ctx():
 
 
 CASES = {
-    1: [dict(p2p_group_f32=g) for g in (8, 12, 24)] + [dict(p2p_narrow_max_f32=1), dict(p2p_narrow_max_f32=64)]
+    1: [dict(p2p_group_f32=g) for g in (8, 12, 24)] + [dict(p2p_narrow_max_f32=1), dict(p2p_narrow_max_f32=32)]
        + [dict(p2p_early_share_f32=0.0, p2p_early_launches_f32=3), dict(p2p_early_share_f32=1.0),
           dict(m2l_slices=2), dict(m2l_item_pairs=128), dict(m2l_item_pairs=16384)]
        + [dict(p2p_queue=1), dict(p2p_queue=2, p2p_queue_waves0=4, p2p_queue_chunk=1),
           dict(p2p_queue=2, p2p_queue_waves0=1, p2p_group_f32=24, p2p_narrow_max_f32=1),
           dict(p2p_queue=2, p2p_queue_waves0=16, p2p_queue_chunk=64, p2p_early_share_f32=0.0)],
-    0: [dict(p2p_group_f64=g) for g in (4, 12)] + [dict(p2p_narrow_max_f64=1), dict(p2p_narrow_max_f64=64)]
+    0: [dict(p2p_group_f64=g) for g in (4, 12)] + [dict(p2p_narrow_max_f64=1), dict(p2p_narrow_max_f64=16)]
        + [dict(p2p_early_share_f64=0.7, p2p_early_launches_f64=8), dict(m2l_slices=2), dict(m2l_item_pairs=256)]
        + [dict(p2p_queue=1, p2p_queue_chunk=3), dict(p2p_queue=2, p2p_queue_waves0=2),
-          dict(p2p_queue=2, p2p_early_share_f64=1.0, p2p_narrow_max_f64=64)],
+          dict(p2p_queue=2, p2p_early_share_f64=1.0, p2p_narrow_max_f64=32)],
 }
 
 
