@@ -119,8 +119,8 @@ struct PairF64Far : PairF64 {
   }
 };
 
-// FP32 (mixed precision): offsets from an anchor (the mean of the task's
-// targets) formed in FP64 and kept as FP32 pairs hi + lo; the separation
+// FP32 (mixed precision): offsets from an anchor (the task's first target,
+// so every target lies within its cell's diagonal of it) formed in FP64 and kept as FP32 pairs hi + lo; the separation
 // (xi_hi - xj_hi) + (xi_lo - xj_lo) is accurate to a few FP32 ulps of the
 // separation itself however large the offsets are, which the close pairs of
 // the target's own and adjacent cells need.  v_rsq_f32.  A chunk's
@@ -271,26 +271,45 @@ __device__ __forceinline__ int32_t stage_list(const P2PTask &tk, const int32_t *
   return carry;
 }
 
-// Sum over the 64 lanes of each of G values (G a power of two <= 64), in
-// FP64 and a fixed order, transposed: at every halving step a lane keeps
-// one half of its values and adds the other half of its partner's (offset
-// 32, 16, ...), then the remaining lanes of each value are added by plain
-// xor-shuffles.  G - 1 + log2(64 / G) shuffles instead of 6 G; lane l ends
-// with the total of value l >> (6 - log2 G).
-template <int G>
-__device__ __forceinline__ double tr_reduce(double (&v)[G], int lane) {
+// Sum over the 64 lanes of each of G values (G a power of two <= 64) in a
+// fixed order, transposed: at every halving step a lane keeps one half of
+// its values and adds the other half of its partner's (offset 32, 16, ...),
+// then the remaining lanes of each value are added by plain xor-shuffles.
+// G - 1 + log2(64 / G) shuffles instead of 6 G; lane l ends with the total
+// of value l >> (6 - log2 G).  The first KLO steps add in the values' own
+// type T (FP32 in the mixed kernels: each step adds one rounding to lane sums
+// of tens of FP32 terms, and those steps hold most of the shuffles), the
+// rest in FP64.
+template <int G, int KLO, class T>
+__device__ __forceinline__ double tr_reduce(T (&v)[G], int lane) {
   static_assert(G >= 1 && G <= kW && (G & (G - 1)) == 0, "G is a power of two up to the wavefront");
+  static_assert((G >> KLO) >= 1, "at most log2 G steps in T");
 #pragma unroll
-  for (int S = G, o = kW / 2; S > 1; S >>= 1, o >>= 1) {
+  for (int l = 0; l < KLO; ++l) {
+    const int S = G >> l, o = (kW / 2) >> l;
     const bool up = (lane & o) != 0;
 #pragma unroll
     for (int k = 0; k < S / 2; ++k) {
-      const double a = v[k], c = v[k + S / 2];
-      const double send = up ? a : c, keep = up ? c : a;
+      const T a = v[k], c = v[k + S / 2];
+      const T send = up ? a : c, keep = up ? c : a;
       v[k] = keep + __shfl_xor(send, o);
     }
   }
-  double x = v[0];
+  constexpr int G2 = G >> KLO;
+  double d[G2];
+#pragma unroll
+  for (int k = 0; k < G2; ++k) d[k] = (double)v[k];
+#pragma unroll
+  for (int S = G2, o = (kW / 2) >> KLO; S > 1; S >>= 1, o >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int k = 0; k < S / 2; ++k) {
+      const double a = d[k], c = d[k + S / 2];
+      const double send = up ? a : c, keep = up ? c : a;
+      d[k] = keep + __shfl_xor(send, o);
+    }
+  }
+  double x = d[0];
 #pragma unroll
   for (int o = kW / G / 2; o >= 1; o >>= 1) x += __shfl_xor(x, o);
   return x;
@@ -407,23 +426,10 @@ __device__ __forceinline__ void narrow_task(const P2PTask &tk, const int32_t *__
                                             int32_t *tmp, uint8_t *tab, int lane) {
   typedef typename Tr::A A;
   const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+  // the offsets' origin: the task's first target (every target lies in its
+  // cell, within the cell's diagonal of it)
   double4 anchor = make_double4(0, 0, 0, 0);
-  if (Tr::kAnchor) {
-    for (int t = lane; t < nc; t += kW) {
-      const double4 q = pos[b + t];
-      anchor.x += q.x;
-      anchor.y += q.y;
-      anchor.z += q.z;
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-      anchor.x += __shfl_xor(anchor.x, off);
-      anchor.y += __shfl_xor(anchor.y, off);
-      anchor.z += __shfl_xor(anchor.z, off);
-    }
-    anchor.x /= nc;
-    anchor.y /= nc;
-    anchor.z /= nc;
-  }
+  if (Tr::kAnchor) anchor = pos[b];
   for (int t = lane; t < nc; t += kW) st[t] = Tr::make(pos[b + t], anchor);
   int32_t near_end;
   const int32_t total = stage_list(tk, nbr27, sub_src, cbeg, cend, ckey, lb, lp, tmp, lane, near_end);
@@ -464,21 +470,22 @@ __device__ __forceinline__ void narrow_task(const P2PTask &tk, const int32_t *__
     const int tl = lane / kSpan;
     const bool writer = (lane % kSpan) == 0 && tl < ng;
     const int64_t i = b + t0 + tl;
-    double v[RG];
+    constexpr int kLo = sizeof(A) == 4 ? 2 : 0; // steps of the reduction in FP32
+    A v[RG];
 #pragma unroll
-    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ax[t] : 0.0;
-    const double gx = tr_reduce<RG>(v, lane);
+    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? ax[t] : A(0);
+    const double gx = tr_reduce<RG, kLo>(v, lane);
 #pragma unroll
-    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ay[t] : 0.0;
-    const double gy = tr_reduce<RG>(v, lane);
+    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? ay[t] : A(0);
+    const double gy = tr_reduce<RG, kLo>(v, lane);
 #pragma unroll
-    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)az[t] : 0.0;
-    const double gz = tr_reduce<RG>(v, lane);
+    for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? az[t] : A(0);
+    const double gz = tr_reduce<RG, kLo>(v, lane);
     double gp = 0.0;
     if (POT) {
 #pragma unroll
-      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? (double)ap[t] : 0.0;
-      gp = tr_reduce<RG>(v, lane);
+      for (int t = 0; t < RG; ++t) v[t] = t < kGroup ? ap[t] : A(0);
+      gp = tr_reduce<RG, kLo>(v, lane);
     }
     if (writer) {
       atomicAdd(&grad[3 * i], gx);
@@ -577,23 +584,10 @@ __device__ __forceinline__ void wide_task(const P2PTask &tk, const int32_t *__re
   typedef typename Tr::S S;
   typedef typename Tr::A A;
   const int32_t b = cbeg[tk.tcell] + tk.toff, nc = min(cend[tk.tcell] - b, kTaskTargets);
+  // the offsets' origin: the task's first target (every target lies in its
+  // cell, within the cell's diagonal of it)
   double4 anchor = make_double4(0, 0, 0, 0);
-  if (Tr::kAnchor) {
-    for (int t = lane; t < nc; t += kW) {
-      const double4 q = pos[b + t];
-      anchor.x += q.x;
-      anchor.y += q.y;
-      anchor.z += q.z;
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-      anchor.x += __shfl_xor(anchor.x, off);
-      anchor.y += __shfl_xor(anchor.y, off);
-      anchor.z += __shfl_xor(anchor.z, off);
-    }
-    anchor.x /= nc;
-    anchor.y /= nc;
-    anchor.z /= nc;
-  }
+  if (Tr::kAnchor) anchor = pos[b];
   int32_t near_end;
   const int32_t total = stage_list(tk, nbr27, sub_src, cbeg, cend, ckey, lb, lp, tmp, lane, near_end);
   const int ne = tk.nlist;
