@@ -291,8 +291,18 @@ __device__ __forceinline__ double tr_reduce(T (&v)[G], int lane) {
 #pragma unroll
     for (int k = 0; k < S / 2; ++k) {
       const T a = v[k], c = v[k + S / 2];
-      const T send = up ? a : c, keep = up ? c : a;
-      v[k] = keep + __shfl_xor(send, o);
+      if (sizeof(T) == 4 && (o == 32 || o == 16)) {
+        // gfx950 lane swaps: the half (o = 32) or the odd rows (o = 16) of a
+        // trade places with the other half / even rows of c, so r[0] + r[1]
+        // holds a's pair sums where lane & o is clear and c's where it is set
+        const unsigned ua = __builtin_bit_cast(unsigned, (float)a), uc = __builtin_bit_cast(unsigned, (float)c);
+        const auto r = o == 32 ? __builtin_amdgcn_permlane32_swap(ua, uc, false, false)
+                               : __builtin_amdgcn_permlane16_swap(ua, uc, false, false);
+        v[k] = (T)(__builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]));
+      } else {
+        const T send = up ? a : c, keep = up ? c : a;
+        v[k] = keep + __shfl_xor(send, o);
+      }
     }
   }
   constexpr int G2 = G >> KLO;
