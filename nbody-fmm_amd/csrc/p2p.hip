@@ -400,19 +400,7 @@ __device__ __forceinline__ void src_pairs(int32_t g0, int32_t g1, int32_t w0, co
     const int32_t gg = g < g1 ? g : g_first;
     return pos[dl[tab[gg - w0]] + gg];
   };
-  int32_t g = g0 + lane;
-  if (g >= g1) return;
-  double4 q1 = load(g, g), q2 = load(g + kW, g);
-  for (; g < g1; g += 2 * kW) {
-    const S x1 = P::make(q1, anchor);
-    S x2 = P::make(q2, anchor);
-    if (!(g + kW < g1)) P::massless(x2); // contributes nothing
-    __builtin_amdgcn_sched_barrier(0);
-    {
-      const int32_t n1 = g + 2 * kW < g1 ? g + 2 * kW : g;
-      q1 = load(n1, n1);
-      q2 = load(g + 3 * kW, n1);
-    }
+  auto pairs2 = [&](const S &x1, const S &x2) {
 #pragma unroll
     for (int t4 = 0; t4 < KG; t4 += STEP)
       if (t4 < ng) // STEP targets a branch (sums of slots past ng are never used)
@@ -422,6 +410,39 @@ __device__ __forceinline__ void src_pairs(int32_t g0, int32_t g1, int32_t w0, co
           P::template pair<POT>(xt, x1, ax[t], ay[t], az[t], ap[t]);
           P::template pair<POT>(xt, x2, ax[t], ay[t], az[t], ap[t]);
         }
+  };
+  if (g0 >= g1) return;
+  int32_t g = g0 + lane, base = g0; // the lane's source, the wavefront's first
+  // a lane without a source (fewer than 64) loads the first one and leaves
+  // before the tail
+  const int32_t gv = g < g1 ? g : g0;
+  double4 q1 = load(gv, gv), q2 = load(g + kW, gv);
+  // full iterations: every lane has both sources
+  for (; base + 2 * kW <= g1; base += 2 * kW, g += 2 * kW) {
+    const S x1 = P::make(q1, anchor), x2 = P::make(q2, anchor);
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      const int32_t n1 = g + 2 * kW < g1 ? g + 2 * kW : g;
+      q1 = load(n1, n1);
+      q2 = load(g + 3 * kW, n1);
+    }
+    pairs2(x1, x2);
+  }
+  if (base >= g1 || g >= g1) return; // no tail, or none for this lane
+  // the tail (at most 127 sources): two per lane while some lane has a second
+  // one (a missing second one is massless), else one
+  const S x1 = P::make(q1, anchor);
+  if (base + kW < g1) {
+    S x2 = P::make(q2, anchor);
+    if (!(g + kW < g1)) P::massless(x2);
+    pairs2(x1, x2);
+  } else {
+#pragma unroll
+    for (int t4 = 0; t4 < KG; t4 += STEP)
+      if (t4 < ng)
+#pragma unroll
+        for (int t = t4; t < t4 + STEP; ++t)
+          P::template pair<POT>(P::tgt(st[t0 + t]), x1, ax[t], ay[t], az[t], ap[t]);
   }
 }
 
