@@ -245,12 +245,17 @@ __global__ void __launch_bounds__(1024) k_m2l_bf3(const DevPlan *__restrict__ dp
         }
     }
     if (pid >= 0) {
+      // rows past the pair's (q+1)^2 are stored as zeros, so that the sum
+      // adds whole 4-row groups
       float *y = Y + (size_t)(pi - ybase) * KPS + r0 * 16;
 #pragma unroll
       for (int r = 0; r < RBW; ++r)
         if (r < nrb)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) y[r * 16 + 4 * kg + q] = acc[r][q];
+          for (int q = 0; q < 4; ++q) {
+            const int row = (r0 + r) * 16 + 4 * kg + q;
+            y[r * 16 + 4 * kg + q] = row < nvalid ? acc[r][q] : 0.0f;
+          }
     }
   }
 }
@@ -495,10 +500,11 @@ __global__ void __launch_bounds__(kT) k_m2l_sum4(int KP, const DevPlan *__restri
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        s[0] += r0 < nv[u] ? (double)v[u].x : 0.0;
-        s[1] += r0 + 1 < nv[u] ? (double)v[u].y : 0.0;
-        s[2] += r0 + 2 < nv[u] ? (double)v[u].z : 0.0;
-        s[3] += r0 + 3 < nv[u] ? (double)v[u].w : 0.0;
+        // a loaded group has r0 < nv; its rows from nv on are zeros (k_m2l_bf3)
+        s[0] += (double)v[u].x;
+        s[1] += (double)v[u].y;
+        s[2] += (double)v[u].z;
+        s[3] += (double)v[u].w;
       }
     }
     for (; j < m; ++j) {
