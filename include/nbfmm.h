@@ -321,6 +321,8 @@ typedef struct nbfmm_tiles {
   int32_t m2l_item_pairs;     /* most M2L pairs of one offset class per contraction workgroup (the workgroup
                                  stages the class operator once for them): a multiple of 128 in [128, 16384]
                                  (default 1024; items hold npair / 2048 pairs, at least 128, up to this) */
+  int32_t p2p_rest_launches;  /* launches of the narrow P2P tasks not started beside the up-pass, 1..8: between
+                                 two, the main stream's sums, L2L and L2P get compute units (default 1) */
 } nbfmm_tiles;
 /* Tiles in force for later evaluations; fields are validated (NBFMM_ERR_ARG). */
 int nbfmm_set_tiles(nbfmm_ctx *ctx, const nbfmm_tiles *tiles);

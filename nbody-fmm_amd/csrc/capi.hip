@@ -568,6 +568,7 @@ int nbfmm_set_tiles(nbfmm_ctx *c, const nbfmm_tiles *t) {
   if (r.p2p_queue_waves0 == 0) r.p2p_queue_waves0 = d.p2p_queue_waves0;
   if (r.p2p_queue_chunk == 0) r.p2p_queue_chunk = d.p2p_queue_chunk;
   if (r.m2l_item_pairs == 0) r.m2l_item_pairs = d.m2l_item_pairs;
+  if (r.p2p_rest_launches == 0) r.p2p_rest_launches = d.p2p_rest_launches;
   std::string why;
   if (!tiles_ok(r, why)) return fail(NBFMM_ERR_ARG, why);
   c->shapes = r;

@@ -154,9 +154,10 @@ constexpr int kMaxChunks0 = 64;                    // P2P first-share launches a
 // P2P task ranges [begin, end) in DevPlan::p2p_rng: the first share's narrow
 // tasks in chunks (chunk ch at 2 ch) and whole, its wide tasks, the rest's
 // narrow and wide tasks, and all narrow and all wide tasks
+constexpr int kMaxChunks1 = 8;                      // launches of the rest of the narrow P2P tasks at most
 constexpr int kRngNarrow0 = 2 * kMaxChunks0, kRngWide0 = kRngNarrow0 + 2, kRngNarrow1 = kRngNarrow0 + 4,
               kRngWide1 = kRngNarrow0 + 6, kRngNarrowAll = kRngNarrow0 + 8, kRngWideAll = kRngNarrow0 + 10,
-              kRngCount = kRngNarrow0 + 12;
+              kRngRest0 = kRngNarrow0 + 12, kRngCount = kRngRest0 + 2 * kMaxChunks1;
 struct DevPlan {
   double lo[3], hi[3]; // the tree's box (tree.cpp:20-30; the cube around it with cubic_box)
   double hs[3];        // leaf edges over rho at the depth (operator geometry)

@@ -945,6 +945,7 @@ nbfmm_tiles default_tiles() {
   t.p2p_early_launches_f64 = NBFMM_P2P_CHUNKS0_F64;
   t.m2l_slices = 1;
   t.m2l_item_pairs = 1024;
+  t.p2p_rest_launches = 1;
   t.p2p_queue = 0;
   t.p2p_queue_waves0 = 8;
   t.p2p_queue_chunk = 4;
@@ -968,6 +969,8 @@ bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
   else if (t.m2l_slices != 1 && t.m2l_slices != 2) why = "m2l_slices must be 1 or 2";
   else if (t.m2l_item_pairs < 128 || t.m2l_item_pairs > 16384 || t.m2l_item_pairs % 128)
     why = "m2l_item_pairs must be a multiple of 128 in [128, 16384]";
+  else if (t.p2p_rest_launches < 1 || t.p2p_rest_launches > kMaxChunks1)
+    why = "p2p_rest_launches must be in [1, 8]";
   else if (t.p2p_queue < 0 || t.p2p_queue > 2) why = "p2p_queue must be 0, 1 or 2";
   else if (t.p2p_queue_waves0 < 1 || t.p2p_queue_waves0 > 16) why = "p2p_queue_waves0 must be in [1, 16]";
   else if (t.p2p_queue_chunk < 1 || t.p2p_queue_chunk > 64) why = "p2p_queue_chunk must be in [1, 64]";
@@ -978,7 +981,7 @@ bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
 // The task ranges of the P2P launches (DevPlan::p2p_rng) from the task
 // counts: the same share `early` of the narrow tasks [0, nn) and of the wide
 // ones [nn, ntask) goes first, its narrow tasks in nch launches.
-__global__ void k_p2p_ranges(DevPlan *dp, double early, int nch) {
+__global__ void k_p2p_ranges(DevPlan *dp, double early, int nch, int nch1) {
   if (threadIdx.x != 0) return;
   const int64_t nn = dp->nnarrow, nt = (int64_t)dp->ntask, nw = nt - nn;
   const int64_t sn = (int64_t)((double)nn * early), sw = nn + (int64_t)((double)nw * early);
@@ -986,6 +989,10 @@ __global__ void k_p2p_ranges(DevPlan *dp, double early, int nch) {
   for (int ch = 0; ch < nch; ++ch) {
     r[2 * ch] = (int32_t)(sn * ch / nch);
     r[2 * ch + 1] = (int32_t)(sn * (ch + 1) / nch);
+  }
+  for (int ch = 0; ch < nch1; ++ch) {
+    r[kRngRest0 + 2 * ch] = (int32_t)(sn + (nn - sn) * ch / nch1);
+    r[kRngRest0 + 2 * ch + 1] = (int32_t)(sn + (nn - sn) * (ch + 1) / nch1);
   }
   const int32_t v[12] = {0, (int32_t)sn, (int32_t)nn, (int32_t)sw, (int32_t)sn, (int32_t)nn,
                          (int32_t)sw, (int32_t)nt, 0, (int32_t)nn, (int32_t)nn, (int32_t)nt};
@@ -996,7 +1003,7 @@ __global__ void k_p2p_ranges(DevPlan *dp, double early, int nch) {
 // task counts of a direct evaluation) or from the capacities (planned; the
 // kernels stride over their tasks, so any grid covers them)
 struct P2PSplit {
-  int nch;
+  int nch, nch1;
   double early;
   int64_t rng[kRngCount];
 };
@@ -1014,6 +1021,11 @@ static P2PSplit p2p_split(nbfmm_ctx *c) {
     sp.rng[2 * ch] = sn * ch / sp.nch;
     sp.rng[2 * ch + 1] = sn * (ch + 1) / sp.nch;
   }
+  sp.nch1 = std::min(kMaxChunks1, tl.p2p_rest_launches);
+  for (int ch = 0; ch < sp.nch1; ++ch) {
+    sp.rng[kRngRest0 + 2 * ch] = sn + (nn - sn) * ch / sp.nch1;
+    sp.rng[kRngRest0 + 2 * ch + 1] = sn + (nn - sn) * (ch + 1) / sp.nch1;
+  }
   const int64_t v[12] = {0, sn, nn, sw, sn, nn, sw, nt, 0, nn, nn, nt};
   for (int k = 0; k < 12; ++k) sp.rng[kRngNarrow0 + k] = v[k];
   return sp;
@@ -1021,7 +1033,7 @@ static P2PSplit p2p_split(nbfmm_ctx *c) {
 
 void fork_p2p(nbfmm_ctx *c) {
   const P2PSplit sp = p2p_split(c);
-  hipLaunchKernelGGL(k_p2p_ranges, dim3(1), dim3(64), 0, c->stream, dplan(c), sp.early, sp.nch);
+  hipLaunchKernelGGL(k_p2p_ranges, dim3(1), dim3(64), 0, c->stream, dplan(c), sp.early, sp.nch, sp.nch1);
   NB_LAUNCH();
   NB_HIP(hipEventRecord(c->ev_fork, c->stream));
 }
@@ -1094,10 +1106,10 @@ void launch_p2p(nbfmm_ctx *c, int part) {
   // whose narrow tasks are longer, measured slower that way)
   for (int wi = 0; wi < 2; ++wi) {
     const int w = (part == 1 && f32) ? 1 - wi : wi;
-    const int nch = (w == 0 && part == 0) ? sp.nch : 1;
+    const int nch = w == 0 && part == 0 ? sp.nch : w == 0 && part == 1 ? sp.nch1 : 1;
     for (int ch = 0; ch < nch; ++ch) {
-      // the range's index: part 0's narrow chunks, or the part's whole range
-      const int k = (w == 0 && part == 0) ? 2 * ch : (w == 0 ? in : iw);
+      // the range's index: part 0's or part 1's narrow chunks, or the part's whole range
+      const int k = w == 0 && part == 0 ? 2 * ch : w == 0 && part == 1 ? kRngRest0 + 2 * ch : (w == 0 ? in : iw);
       const int64_t cnt = len(k);
       if (cnt <= 0 && !c->cap.on) continue;
       auto kern = w == 0 ? narrow_kernel(f32, pot, f32 ? tl.p2p_group_f32 : tl.p2p_group_f64)

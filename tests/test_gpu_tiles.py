@@ -26,12 +26,14 @@ def ctx():
 CASES = {
     1: [dict(p2p_group_f32=g) for g in (8, 12, 24)] + [dict(p2p_narrow_max_f32=1), dict(p2p_narrow_max_f32=32)]
        + [dict(p2p_early_share_f32=0.0, p2p_early_launches_f32=3), dict(p2p_early_share_f32=1.0),
-          dict(m2l_slices=2), dict(m2l_item_pairs=128), dict(m2l_item_pairs=16384)]
+          dict(m2l_slices=2), dict(m2l_item_pairs=128), dict(m2l_item_pairs=16384), dict(p2p_rest_launches=3),
+          dict(p2p_rest_launches=8, p2p_early_share_f32=0.0)]
        + [dict(p2p_queue=1), dict(p2p_queue=2, p2p_queue_waves0=4, p2p_queue_chunk=1),
           dict(p2p_queue=2, p2p_queue_waves0=1, p2p_group_f32=24, p2p_narrow_max_f32=1),
           dict(p2p_queue=2, p2p_queue_waves0=16, p2p_queue_chunk=64, p2p_early_share_f32=0.0)],
     0: [dict(p2p_group_f64=g) for g in (4, 12)] + [dict(p2p_narrow_max_f64=1), dict(p2p_narrow_max_f64=16)]
-       + [dict(p2p_early_share_f64=0.7, p2p_early_launches_f64=8), dict(m2l_slices=2), dict(m2l_item_pairs=256)]
+       + [dict(p2p_early_share_f64=0.7, p2p_early_launches_f64=8), dict(m2l_slices=2), dict(m2l_item_pairs=256),
+          dict(p2p_rest_launches=5)]
        + [dict(p2p_queue=1, p2p_queue_chunk=3), dict(p2p_queue=2, p2p_queue_waves0=2),
           dict(p2p_queue=2, p2p_early_share_f64=1.0, p2p_narrow_max_f64=32)],
 }
@@ -100,12 +102,12 @@ def test_tiles_rejected(ctx):
     default = ctx.tiles()
     for kw in (dict(p2p_group_f32=10), dict(p2p_group_f64=16), dict(p2p_narrow_max_f32=65),
                dict(p2p_early_share_f64=1.5), dict(p2p_early_launches_f32=65), dict(m2l_slices=3), dict(m2l_item_pairs=100),
-               dict(p2p_queue=3), dict(p2p_queue_waves0=17), dict(p2p_queue_chunk=65), dict(p2p_queue_waves0=-1)):
+               dict(p2p_rest_launches=9), dict(p2p_queue=3), dict(p2p_queue_waves0=17), dict(p2p_queue_chunk=65), dict(p2p_queue_waves0=-1)):
         with pytest.raises(nbfmm.NbfmmError) as e:
             ctx.set_tiles(**kw)
         assert e.value.status == 1
         assert ctx.tiles() == default
     # zeros (and a negative share) select the built-in defaults
     ctx.set_tiles(p2p_group_f32=0, p2p_early_share_f32=-1.0, m2l_slices=0, p2p_queue_waves0=0, p2p_queue_chunk=0,
-                  m2l_item_pairs=0)
+                  m2l_item_pairs=0, p2p_rest_launches=0)
     assert ctx.tiles() == default
