@@ -320,19 +320,30 @@ __global__ void k_sum_i64(const int64_t *v, int64_t n, unsigned long long *out) 
 // substituted list.  Targets are cut into groups of kTaskTargets, the
 // concatenated sources into whole 64-particle chunks of about kTaskPairs
 // pair interactions per task.
-__global__ void k_closed(const DevPlan *__restrict__ dp, int lev, int D, const int32_t *__restrict__ cnt_m2l,
-                         const int32_t *__restrict__ child8, int32_t *closed) {
+// closed[c] = 1 for the cells of levels >= lc, 0 below (k_open then clears
+// the cells whose subtree holds an M2L list)
+__global__ void k_closed_init(const DevPlan *__restrict__ dp, int lc, int64_t S, int32_t *closed) {
   if (plan_dead(dp)) return;
-  const int32_t leaf_begin = dp->lvl_start[D];
-  const int32_t c = dp->lvl_start[lev] + (int32_t)(blockIdx.x * kT + threadIdx.x);
-  if (c >= dp->lvl_start[lev + 1]) return;
-  bool ok = cnt_m2l[c] == 0;
-  if (ok && c < leaf_begin)
-    for (int o = 0; o < 8; ++o) {
-      const int32_t ch = child8[(int64_t)c * 8 + o];
-      if (ch >= 0 && !closed[ch]) ok = false;
-    }
-  closed[c] = ok ? 1 : 0;
+  const int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (c >= S) return;
+  closed[c] = c >= dp->lvl_start[lc] && c < dp->nc ? 1 : 0;
+}
+// A cell is closed when no cell of its subtree (itself included) has an M2L
+// list: every cell with one clears itself and its ancestors, stopping at the
+// first one found cleared (whoever cleared it goes on upwards, so every
+// ancestor of a cleared cell ends cleared; two walks may clear the same
+// cells, with the same value).  The reference's recursion (closed = no list
+// and every child closed) gives the same flags.
+__global__ void k_open(const DevPlan *__restrict__ dp, const int32_t *__restrict__ cnt_m2l,
+                       const int32_t *__restrict__ cparent, int32_t *closed) {
+  if (plan_dead(dp)) return;
+  const int32_t c = (int32_t)(blockIdx.x * kT + threadIdx.x);
+  if (c >= dp->nc || cnt_m2l[c] == 0) return;
+  volatile int32_t *cl = closed;
+  for (int32_t x = c; x >= 0; x = cparent[x]) {
+    if (x != c && cl[x] == 0) break;
+    cl[x] = 0;
+  }
 }
 
 __device__ __forceinline__ int64_t row_particles(const int32_t *row, const int32_t *cbeg, const int32_t *cend) {
@@ -590,12 +601,12 @@ void build_lists(nbfmm_ctx *c) {
 
   // ---- P2P sub-tasks ----
   int32_t *closed = dget<int32_t>(c->closed, (size_t)S);
-  fill_i32(s, closed, S, 0); // cells of levels < lc are never task cells
-  for (int d = D; d >= lc; --d) {
-    hipLaunchKernelGGL(k_closed, dim3(nblk(level_cells(c, d))), dim3(kT), 0, s, (const DevPlan *)dp, d, D,
-                       (const int32_t *)cnt_m2l, child8, closed);
-    NB_LAUNCH();
-  }
+  // cells of levels < lc are never task cells
+  hipLaunchKernelGGL(k_closed_init, dim3(nblk(S)), dim3(kT), 0, s, (const DevPlan *)dp, lc, S, closed);
+  NB_LAUNCH();
+  hipLaunchKernelGGL(k_open, dim3(nblk(nc)), dim3(kT), 0, s, (const DevPlan *)dp, (const int32_t *)cnt_m2l,
+                     (const int32_t *)c->cparent.p, closed);
+  NB_LAUNCH();
   nbr_leaf_rows(c, closed); // the P2P work list reads the rows of the leaves that are not closed
   const int32_t *cpar = (const int32_t *)c->cparent.p;
   // task counts of narrow cells, then of wide cells (two halves of S entries)
