@@ -458,7 +458,7 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &tasks, &task_off, &task_cnt, &dpart, &p2p_ctr,
-                 &tiles, &stab, &tmat, &tmat32, &mexp64, &lexp64, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
+                 &tiles, &stab, &tmat, &tmat32, &mexp64, &lexp64, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &m2l_tab, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
                  &ord, &msum, &feat, &rmax, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
                  &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc, &bat_off, &bat_bb, &bat_st, &bat_x, &bat_m,
                  &bat_a, &dplan};
@@ -569,6 +569,7 @@ int nbfmm_set_tiles(nbfmm_ctx *c, const nbfmm_tiles *t) {
   if (r.p2p_queue_chunk == 0) r.p2p_queue_chunk = d.p2p_queue_chunk;
   if (r.m2l_item_pairs == 0) r.m2l_item_pairs = d.m2l_item_pairs;
   if (r.p2p_rest_launches == 0) r.p2p_rest_launches = d.p2p_rest_launches;
+  if (r.m2l_mode == 0) r.m2l_mode = d.m2l_mode;
   std::string why;
   if (!tiles_ok(r, why)) return fail(NBFMM_ERR_ARG, why);
   c->shapes = r;
@@ -901,8 +902,20 @@ int nbfmm_graph_stats(nbfmm_ctx *c, int64_t *out4) {
 
 int nbfmm_get_stats(nbfmm_ctx *c, nbfmm_stats *s) {
   if (!c || !s) return fail(NBFMM_ERR_ARG, "null argument");
+  API_BEGIN
+  Plan &P = c->plan;
+  if (P.m2l_stats_dev) {
+    // the target-major M2L of the last direct evaluation copies its flops and
+    // order sum to the pinned plan mirror (in stream order, no wait)
+    NB_HIP(hipStreamSynchronize(c->stream));
+    P.flop_m2l = c->hplan->flop_m2l;
+    P.mean_order = P.n_m2l > 0 ? c->hplan->order_sum / (double)P.n_m2l : P.mean_order;
+    P.m2l_stats_dev = false;
+    c->stats.flop_m2l = P.flop_m2l;
+    c->stats.mean_order = P.mean_order;
+  }
   *s = c->stats;
-  return NBFMM_OK;
+  API_END
 }
 
 int nbfmm_tree_sizes(nbfmm_ctx *c, int32_t *depth, int64_t *n_cells, int64_t *n_nbr, int64_t *n_il) {

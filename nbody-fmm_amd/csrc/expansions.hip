@@ -735,6 +735,7 @@ void run_expansions(nbfmm_ctx *c) {
   const bool learned = c->prm.order_select == NBFMM_ORDER_LEARNED;
   if (K.on && learned) throw Error(NBFMM_ERR_UNSUPPORTED, "planned evaluations take a fixed order");
   P.flop_m2l = 0;
+  P.m2l_stats_dev = false;
   P.mean_order = p;
   const Consts *dC = (const Consts *)c->consts.p; // uploaded at nbfmm_create
   // far-field gradient (3 per particle) and, with potentials, far-field potential sums

@@ -260,6 +260,327 @@ __global__ void __launch_bounds__(1024) k_m2l_bf3(const DevPlan *__restrict__ dp
   }
 }
 
+// ---- target-major mixed-precision M2L ------------------------------------------------
+// Reflections.  The operator of the offset class S d, S a sign change of
+// some axes, is E_S T(d) E_S, E_S diagonal +-1 over the coefficients: a
+// coefficient of degree n and order m changes sign under x -> -x by (-1)^m
+// (once more for an imaginary part), under y -> -y when it is an imaginary
+// part, under z -> -z by (-1)^(n+m) (the reflections of the regular and
+// singular harmonics).  So 56 operators, one per |d| in {0..3}^3 with
+// max >= 2, serve the 316 far classes.
+//
+// A workgroup owns 128 M2L targets, 16 per wave as the MFMA columns, and
+// keeps their locals in the accumulators across every class: it stages each
+// of the 56 operators (its row blocks [r0, r0 + RBW)) in LDS once, by direct
+// global-to-LDS loads, and each
+// wave applies it to the source multipoles of its 16 targets under every
+// sign change S, as E_S T (E_S M): the class is contracted into FP32
+// accumulators and added, rows negated by E_S, to the targets' FP64 sums (the
+// precision of the class-major per-target sum of pair partials).  A class that is in none of the wave's 16 lists is skipped; the
+// tile's targets are sorted by child octant first, since siblings of one
+// octant have the same far classes.  Only the source multipoles and the
+// locals (added to) move through memory: no per-pair partials.  The classes
+// are taken in a fixed order and every target is owned by one workgroup, so
+// the result does not depend on the schedule.
+constexpr int kTgtWaves = 8;
+constexpr int kTgtTile = 16 * kTgtWaves; // targets per workgroup
+
+// bit 0: the sign change of storage coefficient k under x -> -x, bit 1 under
+// y -> -y, bit 2 under z -> -z (storage: n^2 -> Re X_n^0, n^2+2m-1 ->
+// Re X_n^m, n^2+2m -> Im X_n^m)
+__device__ __forceinline__ uint32_t coef_sign_code(int k) {
+  int n = (int)sqrtf((float)k);
+  while (n * n > k) --n;
+  while ((n + 1) * (n + 1) <= k) ++n;
+  const int rr = k - n * n, m = (rr + 1) >> 1, im = (rr > 0 && (rr & 1) == 0) ? 1 : 0;
+  return (uint32_t)(((m + im) & 1) | (im << 1) | (((n + m) & 1) << 2));
+}
+
+__device__ __forceinline__ uint64_t sign_pick(int s, uint64_t px, uint64_t py, uint64_t pz) {
+  return ((s & 1) ? px : 0ull) ^ ((s & 2) ? py : 0ull) ^ ((s & 4) ? pz : 0ull);
+}
+
+__device__ __forceinline__ float neg_if(float x, uint64_t bits, int i) {
+  return __uint_as_float(__float_as_uint(x) ^ ((uint32_t)((bits >> i) & 1ull) << 31));
+}
+
+// class -> multipole slot of each M2L target's sources: tab[t * kNumClasses +
+// class] (cleared to -1 before), one wave per target, a fixed grid striding
+// over the targets; each workgroup's algorithmic flops (2 n^2 per pair,
+// n = (q+1)^2) and order sum go to part[2 * block] (k_m2l_tab_sum adds them).
+constexpr int kTabBlocks = 1024;
+__global__ void __launch_bounds__(kT) k_m2l_tab(const DevPlan *__restrict__ dp, int p, int binned,
+                                                const int32_t *__restrict__ tgt_cells,
+                                                const int32_t *__restrict__ off_m2l,
+                                                const int32_t *__restrict__ cnt_m2l,
+                                                const int32_t *__restrict__ m2l_src,
+                                                const uint16_t *__restrict__ m2l_cls,
+                                                const int32_t *__restrict__ mslot, int32_t *__restrict__ tab,
+                                                double *__restrict__ part) {
+  __shared__ double red[kT / 64][2];
+  if (plan_dead(dp)) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t nt = dp->nst[1];
+  double fl = 0, qs = 0;
+  for (int64_t t = (int64_t)blockIdx.x * (kT / 64) + wave; t < nt; t += (int64_t)gridDim.x * (kT / 64)) {
+    const int32_t c = tgt_cells[t];
+    const int32_t b = off_m2l[c], e = b + cnt_m2l[c];
+    for (int32_t i = b + lane; i < e; i += 64) {
+      const int cls = (int)m2l_cls[i] % kNumClasses;
+      tab[t * kNumClasses + cls] = mslot[m2l_src[i]];
+      const int q = binned ? (int)dp->qc.q[cls] : p;
+      const double nq = (double)((q + 1) * (q + 1));
+      fl += 2.0 * nq * nq;
+      qs += q;
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    fl += __shfl_xor(fl, o);
+    qs += __shfl_xor(qs, o);
+  }
+  if (lane == 0) {
+    red[wave][0] = fl;
+    red[wave][1] = qs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int w = 0; w < kT / 64; ++w) {
+      a += red[w][0];
+      b += red[w][1];
+    }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// the sums of k_m2l_tab's kTabBlocks partials into dp->flop_m2l, dp->order_sum
+__global__ void __launch_bounds__(kT) k_m2l_tab_sum(DevPlan *__restrict__ dp, const double *__restrict__ part) {
+  __shared__ double red[kT / 64][2];
+  if (plan_dead(dp)) return;
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < kTabBlocks; i += kT) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wave][0] = a;
+    red[wave][1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0, y = 0;
+    for (int w = 0; w < kT / 64; ++w) {
+      x += red[w][0];
+      y += red[w][1];
+    }
+    dp->flop_m2l = x;
+    dp->order_sum = y;
+  }
+}
+
+// RBW row blocks per workgroup (slice blockIdx.y), KSBM k steps of the widest
+// operator (the expansion's KP).  Class orders: binned, dp->qc (one order per
+// |d|: the rule depends on d through |d_i| only); else p for every class.
+template <int RBW, int KSBM>
+__global__ void __launch_bounds__(64 * kTgtWaves) k_m2l_tgt(const DevPlan *__restrict__ dp, int p, int binned,
+                                                            const int32_t *__restrict__ tgt_cells,
+                                                            const int32_t *__restrict__ lslot,
+                                                            const uint64_t *__restrict__ ckey,
+                                                            const int32_t *__restrict__ tab,
+                                                            const uint4 *__restrict__ fragb,
+                                                            const float *__restrict__ Mexp, float *__restrict__ Lexp,
+                                                            int KP) {
+  extern __shared__ uint4 sA[];
+  __shared__ int32_t perm[kTgtTile];
+  __shared__ int32_t boff[9];
+  if (plan_dead(dp)) return;
+  const int32_t nt = dp->nst[1];
+  const int32_t t0 = (int32_t)blockIdx.x * kTgtTile;
+  if (t0 >= nt) return;
+  const int r0 = (int)blockIdx.y * RBW;
+  const int RBS = KP / 16, KSBS = pad32(KP) / 32; // layout of the stored operators
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, kg = lane >> 4;
+  // the tile's targets sorted by child octant (a counting sort in LDS)
+  if (tid < 9) boff[tid] = 0;
+  if (tid < kTgtTile) perm[tid] = -1;
+  __syncthreads();
+  int oct = 8, rank = 0;
+  if (tid < kTgtTile && t0 + tid < nt) {
+    oct = (int)(ckey[tgt_cells[t0 + tid]] & 7u);
+    rank = atomicAdd(&boff[oct + 1], 1);
+  }
+  __syncthreads();
+  if (tid == 0)
+    for (int b = 1; b <= 8; ++b) boff[b] += boff[b - 1];
+  __syncthreads();
+  if (oct < 8) perm[boff[oct] + rank] = t0 + tid;
+  __syncthreads();
+  const int32_t ti = perm[wave * 16 + col];
+  // sign patterns of this lane's accumulator rows and source coefficients
+  uint64_t rx = 0, ry = 0, rz = 0, kx = 0, ky = 0, kz = 0;
+#pragma unroll
+  for (int r = 0; r < RBW; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint64_t cd = coef_sign_code((r0 + r) * 16 + 4 * kg + q);
+      const int i = r * 4 + q;
+      rx |= (cd & 1) << i;
+      ry |= ((cd >> 1) & 1) << i;
+      rz |= ((cd >> 2) & 1) << i;
+    }
+#pragma unroll
+  for (int ks = 0; ks < KSBM; ++ks)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint64_t cd = coef_sign_code(ks * 32 + 8 * kg + j);
+      const int i = ks * 8 + j;
+      kx |= (cd & 1) << i;
+      ky |= ((cd >> 1) & 1) << i;
+      kz |= ((cd >> 2) & 1) << i;
+    }
+  // the targets' locals in FP64 (as the class-major sum of FP32 pair
+  // partials): each class is contracted into FP32 accumulators from zero and
+  // added with the signs of its reflection
+  double tot[RBW][4];
+#pragma unroll
+  for (int r = 0; r < RBW; ++r)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) tot[r][qq] = 0.0;
+  bool any = false; // the wave added a class
+  for (int ub = 0; ub < 64; ++ub) {
+    const int ax = ub & 3, ay = (ub >> 2) & 3, az = ub >> 4;
+    if (max(ax, max(ay, az)) < 2) continue;
+    const int cb = (az + 3) * 49 + (ay + 3) * 7 + (ax + 3);
+    const int q = binned ? (int)dp->qc.q[cb] : p;
+    const int nvalid = (q + 1) * (q + 1);
+    const int KPT = (nvalid + 15) / 16 * 16, KSB = pad32(KPT) / 32;
+    const int nrb = min(RBW, KPT / 16 - r0);
+    if (nrb <= 0) continue;
+    // the multipole slots of the eight sign changes (in flight during the staging)
+    int32_t sl[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const bool ok = !(((s & 1) && !ax) || ((s & 2) && !ay) || ((s & 4) && !az));
+      const int dx = (s & 1) ? -ax : ax, dy = (s & 2) ? -ay : ay, dz = (s & 4) ? -az : az;
+      sl[s] = ok && ti >= 0 ? tab[(size_t)ti * kNumClasses + (dz + 3) * 49 + (dy + 3) * 7 + (dx + 3)] : -1;
+    }
+    __syncthreads(); // every wave is done with the previous operator
+    {
+      // the operator's row blocks straight into LDS: chunks of 64 vectors
+      // (one k step's term of one row block), one per lane
+      const uint4 *A = fragb + (size_t)cb * RBS * KSBS * 192;
+      const int nch = nrb * KSB * 3;
+      for (int ch = wave; ch < nch; ch += kTgtWaves) {
+        const int r = ch / (KSB * 3), rem = ch - r * KSB * 3;
+        const int ks = rem / 3, term = rem - ks * 3;
+        __builtin_amdgcn_global_load_lds(A + ((size_t)(r0 + r) * KSBS + ks) * 192 + term * 64 + lane,
+                                         sA + ((r * KSBM + ks) * 3 + term) * 64, 16, 0, 0);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+    __syncthreads();
+    uint32_t smask = 0; // the wave's classes among the eight (wave-uniform)
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      if (__ballot(sl[s] >= 0) != 0) smask |= 1u << s;
+    if (smask == 0) continue;
+    any = true;
+    auto pick = [&](int s) {
+      int32_t v = sl[0];
+#pragma unroll
+      for (int j = 1; j < 8; ++j) v = s == j ? sl[j] : v;
+      return v;
+    };
+    // k step ks of a class's source coefficients for this lane
+    auto load = [&](int32_t slot, int ks, float v[8]) {
+      const float *Mb = Mexp + (size_t)(slot >= 0 ? slot : 0) * KP + 8 * kg + ks * 32;
+      if (slot >= 0 && ks * 32 + 8 * kg < KP) {
+        const float4 u = *reinterpret_cast<const float4 *>(Mb);
+        const float4 w = *reinterpret_cast<const float4 *>(Mb + 4);
+        v[0] = u.x, v[1] = u.y, v[2] = u.z, v[3] = u.w, v[4] = w.x, v[5] = w.y, v[6] = w.z, v[7] = w.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = 0.0f;
+      }
+    };
+    // the steps (class s, k step ks) in one sequence, the next step's
+    // coefficients in flight while the current one is contracted
+    int s = __builtin_ctz(smask), ks = 0;
+    float xn[8];
+    load(pick(s), 0, xn);
+    flt4 acc[RBW];
+#pragma unroll
+    for (int r = 0; r < RBW; ++r) acc[r] = flt4{0, 0, 0, 0};
+    while (true) {
+      const uint64_t pk = sign_pick(s, kx, ky, kz);
+      float x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        x[j] = ks * 32 + 8 * kg + j < nvalid ? neg_if(xn[j], pk, ks * 8 + j) : 0.0f;
+      int s2 = s, ks2 = ks + 1;
+      if (ks2 == KSB) {
+        const uint32_t later = smask & ~((2u << s) - 1u);
+        s2 = later ? __builtin_ctz(later) : -1;
+        ks2 = 0;
+      }
+      if (s2 >= 0) load(pick(s2), ks2, xn);
+      bf16x8 b1, b2, b3;
+      split3(x, b1, b2, b3);
+#pragma unroll
+      for (int r = 0; r < RBW; ++r)
+        if (r < nrb) {
+          const uint4 *a = sA + ((r * KSBM + ks) * 3) * 64 + lane;
+          const uint4 u1 = a[0], u2 = a[64], u3 = a[128];
+          const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(&u1), a2 = *reinterpret_cast<const bf16x8 *>(&u2),
+                       a3 = *reinterpret_cast<const bf16x8 *>(&u3);
+          flt4 c = acc[r];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
+          acc[r] = c;
+        }
+      if (ks == KSB - 1) {
+        // the class is complete: its rows up to the order, with the signs of
+        // its reflection, into the FP64 sums
+        const uint64_t pm = sign_pick(s, rx, ry, rz);
+#pragma unroll
+        for (int r = 0; r < RBW; ++r)
+          if (r < nrb)
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) {
+              if ((r0 + r) * 16 + 4 * kg + qq < nvalid) tot[r][qq] += (double)neg_if(acc[r][qq], pm, r * 4 + qq);
+              acc[r][qq] = 0.0f;
+            }
+      }
+      if (s2 < 0) break;
+      s = s2;
+      ks = ks2;
+    }
+  }
+  if (!any || ti < 0) return;
+  float *L = Lexp + (size_t)lslot[tgt_cells[ti]] * KP + r0 * 16 + 4 * kg;
+#pragma unroll
+  for (int r = 0; r < RBW; ++r)
+    if (r0 + r < RBS) {
+      float4 v = *reinterpret_cast<float4 *>(L + r * 16);
+      v.x = (float)((double)v.x + tot[r][0]);
+      v.y = (float)((double)v.y + tot[r][1]);
+      v.z = (float)((double)v.z + tot[r][2]);
+      v.w = (float)((double)v.w + tot[r][3]);
+      *reinterpret_cast<float4 *>(L + r * 16) = v;
+    }
+}
+
 // bin of every M2L pair with per-pair orders: class + kNumClasses * (order -
 // kOrderLo), order = min(target order (ord, or p when null), class order)
 __global__ void k_order_bins(const DevPlan *__restrict__ dp, const int32_t *__restrict__ tgt_cells,
@@ -623,6 +944,63 @@ void launch_bf3(nbfmm_ctx *c, int KP, unsigned nitems, const int32_t *irng, cons
   }
 }
 
+// row blocks per workgroup of the target-major contraction: the fewest
+// slices whose operator rows fit the LDS beside the tile's target table
+constexpr int rbw_tgt(int KP, int more = 0) {
+  const int fit = (kLds - 1024) / (pad32(KP) / 32 * 3072), rb = KP / 16;
+  const int slices = (fit >= rb ? 1 : (rb + fit - 1) / fit) + more;
+  return (rb + slices - 1) / slices;
+}
+
+template <int RBW, int KSBM>
+void launch_tgt_rbw(nbfmm_ctx *c, int KP, unsigned ntile, int p, bool binned, const int32_t *tab, const uint4 *fragb) {
+  if (c->lds_max < kLds) throw Error(NBFMM_ERR_UNSUPPORTED, "M2L needs 160 KB of LDS per workgroup (gfx950)");
+  const size_t lds = (size_t)RBW * KSBM * 3072;
+  static bool attr_set[64] = {};
+  if (lds > 65536 && !attr_set[c->device & 63]) {
+    NB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_tgt<RBW, KSBM>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set[c->device & 63] = true;
+  }
+  const int RB = KP / 16;
+  hipLaunchKernelGGL((k_m2l_tgt<RBW, KSBM>), dim3(ntile, (RB + RBW - 1) / RBW), dim3(64 * kTgtWaves), lds, c->stream,
+                     (const DevPlan *)dplan(c), p, binned ? 1 : 0, (const int32_t *)c->tgt_cells.p,
+                     (const int32_t *)c->lslot.p, (const uint64_t *)c->ckey.p, tab, fragb, (const float *)c->mexp.p,
+                     (float *)c->lexp.p, KP);
+  NB_LAUNCH();
+}
+
+template <int KP>
+void launch_tgt_kp(nbfmm_ctx *c, unsigned ntile, int p, bool binned, const int32_t *tab, const uint4 *fragb) {
+  if (c->shapes.m2l_slices == 2)
+    launch_tgt_rbw<rbw_tgt(KP, 1), pad32(KP) / 32>(c, KP, ntile, p, binned, tab, fragb);
+  else
+    launch_tgt_rbw<rbw_tgt(KP), pad32(KP) / 32>(c, KP, ntile, p, binned, tab, fragb);
+}
+
+void launch_tgt(nbfmm_ctx *c, int KP, unsigned ntile, int p, bool binned, const int32_t *tab, const uint4 *fragb) {
+  switch (KP) {
+#define NB_TGT_CASE(k)                                      \
+  case k:                                                   \
+    launch_tgt_kp<k>(c, ntile, p, binned, tab, fragb);      \
+    break;
+    NB_TGT_CASE(16)
+    NB_TGT_CASE(32)
+    NB_TGT_CASE(48)
+    NB_TGT_CASE(64)
+    NB_TGT_CASE(96)
+    NB_TGT_CASE(112)
+    NB_TGT_CASE(128)
+    NB_TGT_CASE(144)
+    NB_TGT_CASE(176)
+    NB_TGT_CASE(208)
+    NB_TGT_CASE(240)
+#undef NB_TGT_CASE
+    default:
+      throw Error(NBFMM_ERR_UNSUPPORTED, "unsupported expansion order");
+  }
+}
+
 template <class T>
 void launch(nbfmm_ctx *c, int KP, unsigned nitems, const int32_t *irng, const int4 *items, int32_t ybase, const int32_t *cls_pairs,
             const T *frag, int KPS, int nvalid) {
@@ -976,6 +1354,40 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
   // per-pair orders: the targets' (learned) and/or the classes'
   const bool binned = ord || c->prm.class_order_tol > 0;
   const int nbins = binned ? kMaxBins : kNumClasses;
+  if (f32 && !ord && c->shapes.m2l_mode != 1) {
+    // ---- target-major contraction (fixed or class orders): no pair partials ----
+    if (binned) {
+      hipLaunchKernelGGL(k_class_orders, dim3(1), dim3(512), 0, s, dp, p, c->prm.class_order_tol);
+      NB_LAUNCH();
+    }
+    const int64_t ntc = (std::max<int64_t>(1, nt) + 1) / 2 * 2; // even: the partials after the table are 8-byte aligned
+    int32_t *tab = dget<int32_t>(c->m2l_tab, (size_t)ntc * kNumClasses + 4 * kTabBlocks);
+    double *part = reinterpret_cast<double *>(tab + (size_t)ntc * kNumClasses);
+    fill_i32(s, tab, ntc * kNumClasses, -1);
+    hipLaunchKernelGGL(k_m2l_tab, dim3(kTabBlocks), dim3(kT), 0, s, (const DevPlan *)dp, p, binned ? 1 : 0,
+                       (const int32_t *)c->tgt_cells.p, (const int32_t *)c->off_m2l.p, (const int32_t *)c->cnt_m2l.p,
+                       (const int32_t *)c->m2l_src.p, (const uint16_t *)c->m2l_cls.p, (const int32_t *)c->mslot.p,
+                       tab, part);
+    NB_LAUNCH();
+    hipLaunchKernelGGL(k_m2l_tab_sum, dim3(1), dim3(kT), 0, s, dp, (const double *)part);
+    NB_LAUNCH();
+    if (!K.on) // the counts for the statistics, read when they are asked for
+      NB_HIP(hipMemcpyAsync(&c->hplan->flop_m2l, &dp->flop_m2l, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    // the operators' three-term bf16 fragments
+    const int64_t nl = (int64_t)kNumClasses * (KP / 16) * (pad32(KP) / 32) * 64;
+    uint4 *fb = dget<uint4>(c->tmat32, (size_t)nl * 3);
+    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, KP, (const double *)c->tmat.p, fb);
+    NB_LAUNCH();
+    const unsigned ntile = (unsigned)((ntc + kTgtTile - 1) / kTgtTile);
+    P.n_m2l_tiles = ntile;
+    P.m2l_stats_dev = !K.on;
+    mark(c, 7);
+    launch_tgt(c, KP, ntile, p, binned, tab, fb);
+    mark(c, 8);
+    launch_p2p(c, 1); // the rest of P2P beside the rest of the chain
+    c->p2p_started = true;
+    return;
+  }
   if (binned) {
     hipLaunchKernelGGL(k_class_orders, dim3(1), dim3(512), 0, s, dp, p, c->prm.class_order_tol);
     NB_LAUNCH();

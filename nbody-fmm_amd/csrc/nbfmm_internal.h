@@ -242,6 +242,7 @@ struct Plan {
   int kp = 0; // padded coefficient count (multiple of 16)
   double flop_m2l = 0; // algorithmic M2L flops of the evaluation
   double mean_order = 0; // mean order over the M2L pairs (the fixed order without selection)
+  bool m2l_stats_dev = false; // flop_m2l and mean_order are still in the device plan (read with the stats)
 };
 
 } // namespace nbfmm
@@ -284,7 +285,7 @@ struct nbfmm_ctx {
   nbfmm::DBuf sub_pairs, tiles;
   nbfmm::DBuf tasks, task_off, task_cnt, dpart, p2p_ctr;
   // expansions
-  nbfmm::DBuf stab, tmat, tmat32, trans, trans32, mode_m, mode_l, mslot, lslot, eslot, up_cells, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items;
+  nbfmm::DBuf stab, tmat, tmat32, trans, trans32, mode_m, mode_l, mslot, lslot, eslot, up_cells, mexp, lexp, consts, ypair, cls_pairs, cls_cnt, m2l_items, m2l_tab;
   nbfmm::DBuf mexp64, lexp64; // FP64 copies for the coarse M2M / L2L levels of mixed precision
   std::vector<int32_t> h_cls_off; // host schedule of the class-sorted M2L
   std::vector<int4> h_m2l_items;

@@ -946,6 +946,7 @@ nbfmm_tiles default_tiles() {
   t.m2l_slices = 1;
   t.m2l_item_pairs = 1024;
   t.p2p_rest_launches = 1;
+  t.m2l_mode = 1;
   t.p2p_queue = 0;
   t.p2p_queue_waves0 = 8;
   t.p2p_queue_chunk = 4;
@@ -971,6 +972,7 @@ bool tiles_ok(const nbfmm_tiles &t, std::string &why) {
     why = "m2l_item_pairs must be a multiple of 128 in [128, 16384]";
   else if (t.p2p_rest_launches < 1 || t.p2p_rest_launches > kMaxChunks1)
     why = "p2p_rest_launches must be in [1, 8]";
+  else if (t.m2l_mode != 1 && t.m2l_mode != 2) why = "m2l_mode must be 1 or 2";
   else if (t.p2p_queue < 0 || t.p2p_queue > 2) why = "p2p_queue must be 0, 1 or 2";
   else if (t.p2p_queue_waves0 < 1 || t.p2p_queue_waves0 > 16) why = "p2p_queue_waves0 must be in [1, 16]";
   else if (t.p2p_queue_chunk < 1 || t.p2p_queue_chunk > 64) why = "p2p_queue_chunk must be in [1, 64]";

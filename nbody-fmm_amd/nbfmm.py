@@ -59,6 +59,7 @@ class Tiles(ctypes.Structure):
         ("p2p_queue_chunk", ctypes.c_int32),
         ("m2l_item_pairs", ctypes.c_int32),
         ("p2p_rest_launches", ctypes.c_int32),
+        ("m2l_mode", ctypes.c_int32),
     ]
 
     def as_dict(self):

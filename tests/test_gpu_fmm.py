@@ -461,14 +461,17 @@ def test_mixed_accuracy_vs_direct(ctx):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_m2l_in_passes(ctx, precision):
     # the translation done in many passes over the class-sorted pairs (a small
-    # partial buffer) gives the same result as one pass
+    # partial buffer) gives the same result as one pass (mixed precision: the
+    # class-major contraction, m2l_mode 1, which has the passes)
     xyz, m = workloads.uniform_cube(3000, seed=8)
     ctx.set_params(order=8, leaf_capacity=16, p2p_cut=0, G=1.0, eps2=0.0, precision=precision, m2l_pass_pairs=5000)
+    ctx.set_tiles(m2l_mode=1)
     try:
         a = ctx.eval_accel(xyz, m)
         npass = -(-ctx.stats().n_m2l // 5000)
     finally:
         ctx.set_params(precision=0, m2l_pass_pairs=0)
+        ctx.set_tiles(m2l_mode=0)
     b = orc.fmm_exact(xyz, m, 8, 16, G=1.0, center_mode=0, p2p_cut=0)
     print(f"[passes] precision={precision} passes={npass} rel L2 {_rel(a, b):.3e}")
     assert npass > 10

@@ -148,6 +148,54 @@ def test_exact_addition_theorem():
     assert abs(tot - 1.0 / np.linalg.norm(x - y)) < 1e-14
 
 
+def test_m2l_reflection_symmetry():
+    """The identity the target-major M2L kernel (csrc/m2l.hip, k_m2l_tgt)
+    rests on: for a sign change S of the axes, M2L(S d) = E_S M2L(d) E_S with
+    E_S diagonal +-1 per coefficient -- (-1)^m times -1 for an imaginary part
+    under x -> -x, -1 for an imaginary part under y -> -y, (-1)^(n+m) under
+    z -> -z.  Checked on the oracle's exact M2L for every sign change of a
+    generic offset; a wrong sign rule (e.g. (-1)^n for z) fails it."""
+    p = 11
+    O = orc.orc()
+    T = (p + 1) * (p + 2) // 2
+    rng = np.random.default_rng(5)
+
+    def signs(sx, sy, sz, zrule=lambda n, m: (n + m) & 1):
+        e = np.ones(2 * T)
+        for n in range(p + 1):
+            for m in range(n + 1):
+                for part in (0, 1):
+                    bit = 0
+                    if sx:
+                        bit ^= (m + part) & 1
+                    if sy:
+                        bit ^= part
+                    if sz:
+                        bit ^= zrule(n, m)
+                    e[2 * _tri(n, m) + part] = -1.0 if bit else 1.0
+        return e
+
+    M = rng.normal(size=2 * T)
+    for n in range(p + 1):
+        M[2 * _tri(n, 0) + 1] = 0.0
+    D = np.array([2.0, -1.0, 3.0])
+    L0 = np.zeros(2 * T)
+    O.orc_exact_m2l(p, P(M), P(D), P(L0))
+    for s in range(1, 8):
+        sx, sy, sz = s & 1, (s >> 1) & 1, (s >> 2) & 1
+        e = signs(sx, sy, sz)
+        DS = np.ascontiguousarray(D * np.array([-1.0 if sx else 1.0, -1.0 if sy else 1.0, -1.0 if sz else 1.0]))
+        Ms = np.ascontiguousarray(e * M)
+        L1 = np.zeros(2 * T)
+        O.orc_exact_m2l(p, P(Ms), P(DS), P(L1))
+        assert np.max(np.abs(e * L1 - L0)) <= 1e-13 * np.max(np.abs(L0)), s
+        if sz:
+            bad = signs(sx, sy, sz, zrule=lambda n, m: n & 1)
+            L2 = np.zeros(2 * T)
+            O.orc_exact_m2l(p, P(np.ascontiguousarray(bad * M)), P(DS), P(L2))
+            assert np.max(np.abs(bad * L2 - L0)) > 1e-3 * np.max(np.abs(L0)), s
+
+
 def test_exact_m2m_identity_and_convergence():
     O = orc.orc()
     rng = np.random.default_rng(4)
