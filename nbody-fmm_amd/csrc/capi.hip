@@ -458,7 +458,7 @@ nbfmm_ctx::~nbfmm_ctx() {
                  &pcell, &grad, &grad_near, &red, &ckey, &cbeg, &cend, &cparent, &child8, &nbr27, &flags, &fscan, &closed, &cnt_il,
                  &cnt_m2l, &cnt_sub, &off_il, &off_m2l, &off_sub, &near_pairs, &il, &m2l_src, &m2l_cls, &sub_src,
                  &src_slot, &tgt_slot, &src_cells, &tgt_cells, &is_src, &sub_pairs, &tasks, &task_off, &task_cnt, &dpart, &p2p_ctr,
-                 &tiles, &stab, &tmat, &tmat32, &mexp64, &lexp64, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &m2l_tab, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
+                 &tiles, &stab, &tmat, &tmat32, &mexp64, &lexp64, &trans, &trans32, &mode_m, &mode_l, &mslot, &lslot, &eslot, &up_cells, &mexp, &lexp, &consts, &ypair, &cls_pairs, &cls_cnt, &m2l_items, &m2l_tab, &ops_key, &xk, &vk, &ak, &asum, &vsum, &diag_acc, &diag_phi,
                  &ord, &msum, &feat, &rmax, &mlp_w, &mlp_wb, &mlp_m, &mlp_v, &mlp_act, &mlp_perm, &ds_feat, &ds_label, &lab_tgt,
                  &lab_dir, &lab_sum, &lab_flag, &lab_lab, &chk_acc, &bat_off, &bat_bb, &bat_st, &bat_x, &bat_m,
                  &bat_a, &dplan};

@@ -67,6 +67,23 @@ void fill_consts(Consts &C) {
 }
 
 
+// The operators (M2L classes, M2M/L2L octants, their FP32 and bf16 forms)
+// depend only on the leaf geometry dp->hs, the order and the precision: they
+// are rebuilt only when one of these changed since the last build (or the
+// host moved a buffer: force).  Sets dp->ops_valid, which the builders read.
+__global__ void k_ops_check(DevPlan *__restrict__ dp, OpsKey *__restrict__ key, int p, int KP, int f32, int force) {
+  if (threadIdx.x != 0 || plan_dead(dp)) return;
+  const bool same = !force && key->hs[0] == dp->hs[0] && key->hs[1] == dp->hs[1] && key->hs[2] == dp->hs[2] &&
+                    key->p == p && key->kp == KP && key->f32 == f32;
+  dp->ops_valid = same ? 1 : 0;
+  if (!same) {
+    for (int i = 0; i < 3; ++i) key->hs[i] = dp->hs[i];
+    key->p = p;
+    key->kp = KP;
+    key->f32 = f32;
+  }
+}
+
 // ---- M2L operators -----------------------------------------------------------------
 // Yoshida singular harmonics S_N^M (N <= 2p, 0 <= M <= N) of the scaled class
 // offset, then the real KP x KP matrix in MFMA A-fragment order:
@@ -75,6 +92,7 @@ void fill_consts(Consts &C) {
 // (rx, ry, rz): the leaf cell's edges over rho, from the plan
 __global__ void __launch_bounds__(kT) k_build_ops(int p, int KP, const DevPlan *__restrict__ dp,
                                                   double *__restrict__ frag) {
+  if (dp->ops_valid) return;
   const double rx = dp->hs[0], ry = dp->hs[1], rz = dp->hs[2];
   const int cls = blockIdx.x;
   const int dx = cls % 7 - 3, dy = (cls / 7) % 7 - 3, dz = cls / 49 - 3;
@@ -207,6 +225,7 @@ __device__ __forceinline__ void yget(const double *Rre, const double *Rim, int n
 
 __global__ void __launch_bounds__(kT) k_build_trans(int p, int KP, const DevPlan *__restrict__ dp,
                                                     double *__restrict__ m2m, double *__restrict__ l2l) {
+  if (dp->ops_valid) return;
   const double hx = dp->hs[0], hy = dp->hs[1], hz = dp->hs[2];
   const int o = blockIdx.x, op = blockIdx.y;
   __shared__ double Rre[(kMaxOrder + 1) * (kMaxOrder + 2) / 2];
@@ -840,9 +859,20 @@ void run_expansions(nbfmm_ctx *c) {
   }
   // ---- operators: M2L classes and the M2M / L2L octants (level independent) ----
   double *frag = dget<double>(c->tmat, (size_t)kNumClasses * KP * KP);
+  double *trans = dget<double>(c->trans, (size_t)16 * KP * KP);
+  {
+    const void *bufs[4] = {frag, trans, nullptr, nullptr};
+    if (f32) { // the FP32 and bf16 forms' buffers now, so that none moves after the check
+      bufs[2] = ops_f32_buffers(c, KP, &bufs[3]);
+    }
+    const bool moved = std::memcmp(bufs, c->ops_bufs, sizeof(bufs)) != 0;
+    OpsKey *key = dget<OpsKey>(c->ops_key, 1);
+    if (moved) std::memcpy(c->ops_bufs, bufs, sizeof(bufs));
+    hipLaunchKernelGGL(k_ops_check, dim3(1), dim3(64), 0, s, dp, key, p, KP, f32 ? 1 : 0, moved ? 1 : 0);
+    NB_LAUNCH();
+  }
   hipLaunchKernelGGL(k_build_ops, dim3(kNumClasses), dim3(kT), 0, s, p, KP, (const DevPlan *)dp, frag);
   NB_LAUNCH();
-  double *trans = dget<double>(c->trans, (size_t)16 * KP * KP);
   hipLaunchKernelGGL(k_build_trans, dim3(8, 2), dim3(kT), 0, s, p, KP, (const DevPlan *)dp, trans,
                      trans + (size_t)8 * KP * KP);
   NB_LAUNCH();

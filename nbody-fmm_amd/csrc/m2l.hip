@@ -115,6 +115,13 @@ __global__ void k_to_f32(const double *__restrict__ in, int64_t n, float *__rest
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i < n) out[i] = (float)in[i];
 }
+// the M2M/L2L operators' FP32 form, unless built by an earlier evaluation
+__global__ void k_ops_to_f32(const DevPlan *__restrict__ dp, const double *__restrict__ in, int64_t n,
+                             float *__restrict__ out) {
+  if (dp->ops_valid) return;
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
 __global__ void k_to_f64(const float *__restrict__ in, int64_t n, double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i < n) out[i] = (double)in[i];
@@ -136,7 +143,9 @@ constexpr int pad32(int k) { return (k + 31) / 32 * 32; }
 // [class][row block r < KP/16][k step < pad32(KP)/32][term < 3][lane] of 8
 // bf16 (16 bytes), from the FP64 operators in FP32 MFMA order
 // (frag[((class RB + r) KP/4 + k/4) 64 + i + 16 (k % 4)]); columns >= KP are 0.
-__global__ void k_to_bf3(int KP, const double *__restrict__ frag, uint4 *__restrict__ out) {
+__global__ void k_to_bf3(const DevPlan *__restrict__ dp, int KP, const double *__restrict__ frag,
+                         uint4 *__restrict__ out) {
+  if (dp->ops_valid) return; // built by an earlier evaluation (k_ops_check)
   const int RB = KP / 16, KS4 = KP / 4, KSB = pad32(KP) / 32;
   const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; // (class, r, ks, lane)
   const int64_t nlane = (int64_t)kNumClasses * RB * KSB * 64;
@@ -1376,7 +1385,7 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
     // the operators' three-term bf16 fragments
     const int64_t nl = (int64_t)kNumClasses * (KP / 16) * (pad32(KP) / 32) * 64;
     uint4 *fb = dget<uint4>(c->tmat32, (size_t)nl * 3);
-    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, KP, (const double *)c->tmat.p, fb);
+    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, (const DevPlan *)dp, KP, (const double *)c->tmat.p, fb);
     NB_LAUNCH();
     const unsigned ntile = (unsigned)((ntc + kTgtTile - 1) / kTgtTile);
     P.n_m2l_tiles = ntile;
@@ -1418,7 +1427,7 @@ void run_m2l(nbfmm_ctx *c, int KP, bool f32, const int8_t *ord, int p) {
     // three-term bf16 fragments of the operators
     const int64_t nl = (int64_t)kNumClasses * (KP / 16) * (pad32(KP) / 32) * 64;
     uint4 *fb = dget<uint4>(c->tmat32, (size_t)nl * 3);
-    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, KP, (const double *)c->tmat.p, fb);
+    hipLaunchKernelGGL(k_to_bf3, dim3(nblk(nl)), dim3(kT), 0, s, (const DevPlan *)dp, KP, (const double *)c->tmat.p, fb);
     NB_LAUNCH();
     return fb;
   };
@@ -1598,8 +1607,15 @@ void narrow(nbfmm_ctx *c, const double *in, int64_t count, float *out) {
 void trans_to_f32(nbfmm_ctx *c, int KP) {
   const int64_t n = (int64_t)16 * KP * KP;
   float *t32 = dget<float>(c->trans32, (size_t)n);
-  hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(kT), 0, c->stream, (const double *)c->trans.p, n, t32);
+  hipLaunchKernelGGL(k_ops_to_f32, dim3(nblk(n)), dim3(kT), 0, c->stream, (const DevPlan *)dplan(c),
+                     (const double *)c->trans.p, n, t32);
   NB_LAUNCH();
+}
+
+const void *ops_f32_buffers(nbfmm_ctx *c, int KP, const void **trans32) {
+  const int64_t nl = (int64_t)kNumClasses * (KP / 16) * (pad32(KP) / 32) * 64;
+  *trans32 = dget<float>(c->trans32, (size_t)16 * KP * KP);
+  return dget<uint4>(c->tmat32, (size_t)nl * 3);
 }
 
 } // namespace nbfmm

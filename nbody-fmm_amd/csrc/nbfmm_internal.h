@@ -188,6 +188,13 @@ struct DevPlan {
   double flop_m2l, order_sum;
   int32_t bin_cnt[kMaxBins];
   int32_t bin_cur[kMaxBins];
+  int32_t ops_valid; // the operators built by an earlier evaluation fit this one (k_ops_check)
+};
+
+// Geometry and order the operator buffers were last built for (device side).
+struct OpsKey {
+  double hs[3];
+  int32_t p, kp, f32, pad;
 };
 
 // Sizes of a direct evaluation, kept to set the capacities of planned ones.
@@ -308,6 +315,8 @@ struct nbfmm_ctx {
   bool p2p_started = false; // P2P of the current evaluation is enqueued
   // the sizes of the current evaluation on the device, their pinned mirror
   nbfmm::DBuf dplan;
+  nbfmm::DBuf ops_key;                 // OpsKey of the operators in tmat, trans, tmat32, trans32
+  const void *ops_bufs[4] = {};        // those buffers' addresses when the key was written
   nbfmm::DevPlan *hplan = nullptr;
   nbfmm::Caps cap;        // capacities of planned evaluations (cap.on: this evaluation is planned)
   nbfmm::PlanSizes last;  // sizes of the last direct evaluation
@@ -390,6 +399,8 @@ void run_trans64(nbfmm_ctx *c, bool l2l, int KP, const int32_t *cells, const int
 void widen(nbfmm_ctx *c, const float *in, int64_t count, double *out);
 void narrow(nbfmm_ctx *c, const double *in, int64_t count, float *out);
 void trans_to_f32(nbfmm_ctx *c, int KP);
+// the FP32 / bf16 operator buffers (allocated at KP): tmat32, and trans32 through *trans32
+const void *ops_f32_buffers(nbfmm_ctx *c, int KP, const void **trans32);
 // p2p.hip
 // P2P on the second stream: fork_p2p marks the point of the main stream it
 // waits for (its inputs are complete there).  launch_p2p(c, 0) enqueues the

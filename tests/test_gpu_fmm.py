@@ -476,3 +476,29 @@ def test_m2l_in_passes(ctx, precision):
     print(f"[passes] precision={precision} passes={npass} rel L2 {_rel(a, b):.3e}")
     assert npass > 10
     assert _rel(a, b) < (1e-12 if precision == 0 else 2e-6)
+
+
+def test_operator_cache_follows_geometry_order_and_precision():
+    # the operators are rebuilt only when the leaf geometry, the order or the
+    # precision changed since the last build: a context that goes through
+    # boxes of other shapes, orders and precisions gives what a fresh context
+    # gives for each evaluation
+    xa, ma = workloads.plummer(30000, seed=81)
+    xb = xa * np.array([3.0, 1.0, 0.4])
+    runs = [(xa, 1, 9), (xb, 1, 9), (xa, 1, 9), (xa, 0, 9), (xa, 1, 9), (xa, 1, 7), (xb, 0, 7), (xa, 1, 9)]
+    c = nbfmm.Context(order=9, leaf_capacity=32, G=1.0, eps2=0.0, p2p_cut=256, class_order_tol=0.05)
+    try:
+        for k, (x, prec, p) in enumerate(runs):
+            c.set_params(precision=prec, order=p)
+            a = c.eval_accel(x, ma)
+            f = nbfmm.Context(order=p, leaf_capacity=32, G=1.0, eps2=0.0, p2p_cut=256, class_order_tol=0.05,
+                              precision=prec)
+            try:
+                b = f.eval_accel(x, ma)
+            finally:
+                f.close()
+            e = _rel(a, b)
+            print(f"[ops cache] run {k} precision={prec} p={p}: rel diff to a fresh context {e:.2e}", flush=True)
+            assert e < (1e-13 if prec == 0 else 1e-7), (k, e)
+    finally:
+        c.close()
