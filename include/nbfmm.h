@@ -322,7 +322,7 @@ typedef struct nbfmm_tiles {
                                  stages the class operator once for them): a multiple of 128 in [128, 16384]
                                  (default 1024; items hold npair / 2048 pairs, at least 128, up to this) */
   int32_t p2p_rest_launches;  /* launches of the narrow P2P tasks not started beside the up-pass, 1..8: between
-                                 two, the main stream's sums, L2L and L2P get compute units (default 1) */
+                                 two, the main stream's sums, L2L and L2P get compute units (default 4) */
   int32_t m2l_mode;           /* mixed-precision M2L: 1 (default) class-major items with per-pair partials and a
                                  per-target sum (always used in FP64 and with learned orders); 2 target-major, each
                                  workgroup keeps 128 targets' locals on chip over the 56 class operators up to

@@ -945,7 +945,7 @@ nbfmm_tiles default_tiles() {
   t.p2p_early_launches_f64 = NBFMM_P2P_CHUNKS0_F64;
   t.m2l_slices = 1;
   t.m2l_item_pairs = 1024;
-  t.p2p_rest_launches = 1;
+  t.p2p_rest_launches = 4;
   t.m2l_mode = 1;
   t.p2p_queue = 0;
   t.p2p_queue_waves0 = 8;
