@@ -254,12 +254,14 @@ __global__ void __launch_bounds__(1024) k_m2l_bf3(const DevPlan *__restrict__ dp
         }
     }
     if (pid >= 0) {
-      // rows past the pair's (q+1)^2 are stored as zeros, so that the sum
-      // adds whole 4-row groups
+      // rows past the pair's (q+1)^2 up to the next multiple of 4 are stored
+      // as zeros, so that the sum adds whole 4-row groups; it reads no row
+      // past that, so none is stored
+      const int nvalid4 = (nvalid + 3) & ~3;
       float *y = Y + (size_t)(pi - ybase) * KPS + r0 * 16;
 #pragma unroll
       for (int r = 0; r < RBW; ++r)
-        if (r < nrb)
+        if (r < nrb && (r0 + r) * 16 + 4 * kg < nvalid4)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int row = (r0 + r) * 16 + 4 * kg + q;
