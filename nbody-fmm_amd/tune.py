@@ -69,7 +69,7 @@ def tune(ctx, n, d_xyz, d_m, d_acc, tgt, ref, orders, cuts, leaves=(None,), gate
 TILE_GRID = {
     1: [("p2p_group_f32", (8, 12, 16, 24)), ("p2p_narrow_max_f32", (16, 32, 64)),
         ("p2p_early_share_f32", (0.3, 0.5, 0.7)), ("p2p_early_launches_f32", (8, 16, 24)), ("m2l_slices", (1, 2)),
-        ("m2l_item_pairs", (512, 1024, 2048)), ("p2p_queue", (0, 1, 2)), ("p2p_rest_launches", (1, 4, 6)),
+        ("m2l_item_pairs", (512, 1024, 2048)), ("p2p_queue", (0, 1, 2)), ("p2p_rest_launches", (1, 4, 6, 8)),
         ("m2l_mode", (1, 2))],
     0: [("p2p_group_f64", (4, 8, 12)), ("p2p_narrow_max_f64", (16, 32, 64)),
         ("p2p_early_share_f64", (0.1, 0.2, 0.4)), ("p2p_early_launches_f64", (1, 4, 16)), ("m2l_slices", (1, 2)),
