@@ -47,9 +47,12 @@ def tune(ctx, n, d_xyz, d_m, d_acc, tgt, ref, orders, cuts, leaves=(None,), gate
             for tol in tols:
                 for cut in cuts:
                     ctx.set_params(order=p, p2p_cut=cut, precision=precision, class_order_tol=tol)
-                    ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
-                    ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
-                    ms = ctx.stats().ms_total
+                    ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)  # warm-up at this setting
+                    ts = []
+                    for _ in range(3):  # settings within ~0.2 ms of each other: a median, not one sample
+                        ctx.eval_accel_dev(n, d_xyz.ptr, d_m.ptr, d_acc.ptr)
+                        ts.append(ctx.stats().ms_total)
+                    ms = float(np.median(ts))
                     acc = d_acc.download(ctx.stream)
                     err = rel_errors(acc[tgt], ref)
                     row = {"order": p, "p2p_cut": cut, "leaf_capacity": int(ctx.params.leaf_capacity),
