@@ -281,18 +281,24 @@ __global__ void __launch_bounds__(1024) k_m2l_bf3(const DevPlan *__restrict__ dp
 // max >= 2, serve the 316 far classes.
 //
 // A workgroup owns 128 M2L targets, 16 per wave as the MFMA columns, and
-// keeps their locals in the accumulators across every class: it stages each
-// of the 56 operators (its row blocks [r0, r0 + RBW)) in LDS once, by direct
-// global-to-LDS loads, and each
-// wave applies it to the source multipoles of its 16 targets under every
-// sign change S, as E_S T (E_S M): the class is contracted into FP32
-// accumulators and added, rows negated by E_S, to the targets' FP64 sums (the
-// precision of the class-major per-target sum of pair partials).  A class that is in none of the wave's 16 lists is skipped; the
+// keeps their locals on chip across every class: it stages each of the 56
+// operators (its row blocks [r0, r0 + RBW)) in LDS once, by direct
+// global-to-LDS loads, and each wave applies it to the source multipoles of
+// its 16 targets under every sign change S, as E_S T (E_S M): the class is
+// contracted into FP32 accumulators and added, rows negated by E_S, to the
+// targets' FP64 sums (the precision of the class-major per-target sum of
+// pair partials).  A table (target, class) -> multipole slot gives the
+// sources; a class in none of the wave's 16 lists is skipped, and the
 // tile's targets are sorted by child octant first, since siblings of one
-// octant have the same far classes.  Only the source multipoles and the
-// locals (added to) move through memory: no per-pair partials.  The classes
-// are taken in a fixed order and every target is owned by one workgroup, so
-// the result does not depend on the schedule.
+// octant have the same far classes.  Only the table, the source multipoles
+// and the locals (added to) move through memory: no per-pair partials.  The
+// classes are taken in a fixed order and every target is owned by one
+// workgroup, so the result does not depend on the schedule.
+//
+// It is the tile option m2l_mode 2, not the default: an order-11 operator
+// fills 138 KB of the 160 KB LDS, so a CU holds one workgroup, two waves per
+// SIMD, too few to hide the dependent gathers (table, then multipoles)
+// behind a class's MFMAs; the class-major contraction is faster on gfx950.
 constexpr int kTgtWaves = 8;
 constexpr int kTgtTile = 16 * kTgtWaves; // targets per workgroup
 
